@@ -1,0 +1,201 @@
+// Driver around the reference engine's own headers -- TEST INFRASTRUCTURE ONLY.
+//
+// Built by oracle/Makefile from the reference sources where they lie
+// (include/velographx/..., src/*.cpp) into oracle/_ref/vgx_ref.  It runs
+// scripted scenarios through the reference's DynamicGraph / IncrementalBFS /
+// IncrementalPageRank and prints the results, so tests can cross-check the
+// C restatement in vgx_oracle.c; its bench modes time the reference engine
+// as the CPU baseline.
+//
+// Scenario language (one command per line, whitespace separated):
+//   graph <n> <directed>            new DynamicGraph(n, directed)
+//   bulk <m>  + m lines "u v"       bulk_load_edges
+//   batch <k> + k lines "u v add"   stage an UpdateBatch
+//   graph_apply                     graph.apply(batch)
+//   bfs_new <source> <fallback>     IncrementalBFS(graph, source, fallback)
+//   bfs_apply                       bfs.apply(batch)
+//   bfs_recompute                   bfs.recompute()
+//   pr_new <damping>                IncrementalPageRank(graph, damping)
+//   pr_apply <iters> <tol> <frac>   pr.apply(batch, iters, tol, frac)
+//   pr_recompute <iters> <tol>      pr.recompute(iters, tol)
+//   pr_validate <it> <tol> <l1> <linf>
+//   neighbors <u>
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "velographx/incremental/bfs.hpp"
+#include "velographx/incremental/pagerank.hpp"
+#include "velographx/storage/dynamic_graph.hpp"
+
+using velographx::DynamicGraph;
+using velographx::IncrementalBFS;
+using velographx::IncrementalPageRank;
+using velographx::UpdateBatch;
+using velographx::VertexId;
+
+namespace {
+
+void print_dist(const IncrementalBFS& bfs) {
+  const auto& d = bfs.distances();
+  std::printf("dist %zu", d.size());
+  for (auto x : d) std::printf(" %u", x);
+  std::printf("\n");
+  std::printf("bfsdiag %zu %zu %zu %d\n", bfs.reachable_count(), bfs.last_deletion_candidates(),
+              bfs.last_affected_vertices(), bfs.last_used_full_recompute() ? 1 : 0);
+}
+
+void print_pr(const IncrementalPageRank& pr) {
+  const auto& v = pr.values();
+  std::printf("values %zu", v.size());
+  for (auto x : v) std::printf(" %.17g", x);
+  std::printf("\n");
+  std::printf("prdiag %zu %zu %.17g %.17g %d\n", pr.last_repaired_vertices(), pr.last_repair_iterations(),
+              pr.last_residual_l1(), pr.last_residual_linf(), pr.last_full_recompute_converged() ? 1 : 0);
+}
+
+int run_scenario(std::istream& in) {
+  std::unique_ptr<DynamicGraph> g;
+  std::unique_ptr<IncrementalBFS> bfs;
+  std::unique_ptr<IncrementalPageRank> pr;
+  UpdateBatch batch;
+  std::string cmd;
+  while (in >> cmd) {
+    if (cmd == "graph") {
+      std::size_t n;
+      int directed;
+      in >> n >> directed;
+      bfs.reset();
+      pr.reset();
+      g = std::make_unique<DynamicGraph>(n, directed != 0);
+    } else if (cmd == "bulk") {
+      std::size_t m;
+      in >> m;
+      std::vector<std::pair<VertexId, VertexId>> e(m);
+      for (auto& p : e) in >> p.first >> p.second;
+      g->bulk_load_edges(e);
+    } else if (cmd == "batch") {
+      std::size_t k;
+      in >> k;
+      batch = UpdateBatch{};
+      for (std::size_t i = 0; i < k; ++i) {
+        VertexId u, v;
+        int a;
+        in >> u >> v >> a;
+        if (a) batch.add(u, v);
+        else batch.remove(u, v);
+      }
+    } else if (cmd == "graph_apply") {
+      g->apply(batch);
+    } else if (cmd == "bfs_new") {
+      VertexId s;
+      double f;
+      in >> s >> f;
+      bfs = std::make_unique<IncrementalBFS>(*g, s, f);
+      print_dist(*bfs);
+    } else if (cmd == "bfs_apply") {
+      bfs->apply(batch);
+      print_dist(*bfs);
+    } else if (cmd == "bfs_recompute") {
+      bfs->recompute();
+      print_dist(*bfs);
+    } else if (cmd == "pr_new") {
+      double d;
+      in >> d;
+      pr = std::make_unique<IncrementalPageRank>(*g, d);
+      print_pr(*pr);
+    } else if (cmd == "pr_apply") {
+      std::size_t it;
+      double tol, frac;
+      in >> it >> tol >> frac;
+      pr->apply(batch, it, tol, frac);
+      print_pr(*pr);
+    } else if (cmd == "pr_recompute") {
+      std::size_t it;
+      double tol;
+      in >> it >> tol;
+      pr->recompute(it, tol);
+      print_pr(*pr);
+    } else if (cmd == "pr_validate") {
+      std::size_t it;
+      double tol, l1, linf;
+      in >> it >> tol >> l1 >> linf;
+      const auto v = pr->validate_against_full(it, tol, l1, linf);
+      std::printf("prvalid %.17g %.17g %zu %d %d\n", v.l1_error, v.linf_error, v.reference_iterations,
+                  v.reference_converged ? 1 : 0, v.within_tolerance ? 1 : 0);
+    } else if (cmd == "neighbors") {
+      VertexId u;
+      in >> u;
+      const auto nb = g->neighbors(u);
+      std::printf("neighbors %zu", nb.size());
+      for (auto x : nb) std::printf(" %u", x);
+      std::printf("\n");
+    } else {
+      std::fprintf(stderr, "unknown command %s\n", cmd.c_str());
+      return 2;
+    }
+  }
+  return 0;
+}
+
+std::vector<std::pair<VertexId, VertexId>> read_edges_bin(const char* path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open edge file");
+  f.seekg(0, std::ios::end);
+  const auto bytes = static_cast<std::size_t>(f.tellg());
+  f.seekg(0);
+  std::vector<std::uint32_t> raw(bytes / 4);
+  f.read(reinterpret_cast<char*>(raw.data()), static_cast<std::streamsize>(raw.size() * 4));
+  std::vector<std::pair<VertexId, VertexId>> e(raw.size() / 2);
+  for (std::size_t i = 0; i < e.size(); ++i) e[i] = {raw[2 * i], raw[2 * i + 1]};
+  return e;
+}
+
+using Clock = std::chrono::steady_clock;
+
+// bench-bfs <edges.bin> <n> <directed> <time_budget_s> <root...>
+// Builds the graph, then runs IncrementalBFS::recompute from each root in
+// turn (one thread), stopping once the time budget is spent.  Prints one
+// line per root: "root <r> <seconds> <reached> <arcs_in_component>".
+int bench_bfs(int argc, char** argv) {
+  const auto edges = read_edges_bin(argv[2]);
+  const std::size_t n = std::stoull(argv[3]);
+  const bool directed = std::stoi(argv[4]) != 0;
+  const double budget = std::stod(argv[5]);
+  DynamicGraph g(n, directed);
+  g.bulk_load_edges(edges);
+  const auto t_all = Clock::now();
+  for (int i = 6; i < argc; ++i) {
+    const VertexId r = static_cast<VertexId>(std::stoul(argv[i]));
+    const auto t0 = Clock::now();
+    IncrementalBFS bfs(g, r, 2.0);
+    const auto t1 = Clock::now();
+    std::uint64_t arcs = 0;
+    const auto& d = bfs.distances();
+    for (std::size_t u = 0; u < d.size(); ++u)
+      if (d[u] != IncrementalBFS::unreachable) g.for_each_neighbor(static_cast<VertexId>(u), [&](VertexId) { ++arcs; });
+    std::printf("root %u %.9f %zu %llu\n", r, std::chrono::duration<double>(t1 - t0).count(),
+                bfs.reachable_count(), static_cast<unsigned long long>(arcs));
+    std::fflush(stdout);
+    if (std::chrono::duration<double>(Clock::now() - t_all).count() > budget) break;
+  }
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc >= 2 && std::strcmp(argv[1], "bench-bfs") == 0 && argc >= 7) return bench_bfs(argc, argv);
+  if (argc >= 3 && std::strcmp(argv[1], "scenario") == 0) {
+    std::ifstream f(argv[2]);
+    return run_scenario(f);
+  }
+  return run_scenario(std::cin);
+}
