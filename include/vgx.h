@@ -1,0 +1,233 @@
+/*
+ * vgx.h -- C ABI of the MI355X (gfx950) traversal engine.
+ *
+ * The engine keeps a VeloGraphX DynamicGraph resident in HBM and runs BFS,
+ * incremental BFS repair, PageRank, multi-source BFS and weighted
+ * personalized PageRank on it with hand-written HIP kernels.  This header is
+ * the drop-in boundary: plain pointers and sizes, no HIP or torch types,
+ * status codes instead of exceptions.  Each entry point names the reference
+ * interface it replaces (paths relative to the VeloGraphX source tree).
+ *
+ * Conventions
+ *  - Every call returns vgx_status; on failure vgx_last_error() holds a
+ *    message for the calling thread.
+ *  - Host pointers ("const uint32_t* src") are read during the call only.
+ *    Arguments named *_dev are device pointers (HBM), used on the stream
+ *    passed alongside (NULL = the graph's own stream).
+ *  - Results are answer-ready when a call returns (the call synchronises
+ *    its stream), unless the name ends in _async.
+ *  - Vertex ids are uint32 below 2^31 (bit 31 marks tombstones on device).
+ *  - Algorithm handles borrow their graph (non-owning, like the reference's
+ *    Graph& members, incremental/bfs.hpp:356); destroy them before the
+ *    graph.  Calls on one graph and its algorithms must not overlap.
+ */
+#ifndef VGX_H
+#define VGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  VGX_OK = 0,
+  VGX_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+  VGX_OUT_OF_RANGE = 2,     /* std::out_of_range */
+  VGX_DEVICE_ERROR = 3,     /* HIP runtime failure */
+  VGX_OUT_OF_MEMORY = 4,    /* std::bad_alloc */
+  VGX_UNSUPPORTED = 5
+} vgx_status;
+
+#define VGX_UNREACHABLE 0xFFFFFFFFu /* types.hpp:7 kUnreachable */
+
+typedef struct vgx_graph vgx_graph;
+typedef struct vgx_bfs vgx_bfs;
+typedef struct vgx_pagerank vgx_pagerank;
+typedef struct vgx_msbfs vgx_msbfs;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* vgx_last_error(void);
+/* Version string of the library. */
+const char* vgx_version(void);
+/* Number of visible gfx950 devices; 0 when none (no error). */
+int vgx_device_count(void);
+
+/* ------------------------------------------------------------------ graph */
+
+/* DynamicGraph(vertices, directed) -- storage/dynamic_graph.hpp:481-488. */
+vgx_status vgx_graph_create(uint64_t vertices, int directed, vgx_graph** out);
+vgx_status vgx_graph_destroy(vgx_graph* g);
+
+/* DynamicGraph::bulk_load_edges -- dynamic_graph.hpp:502-529: replaces the
+ * graph with the canonical (sorted, de-duplicated, loop-free) CSR of the m
+ * edges, mirrored when undirected; grows to max id + 1. */
+vgx_status vgx_graph_bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t* dst, uint64_t m);
+
+/* DynamicGraph::apply(UpdateBatch) -- dynamic_graph.hpp:543-549, 717-733.
+ * add[i] != 0 inserts (src[i], dst[i]), 0 removes it; sequential semantics
+ * (last op on an edge wins), self-loops ignored, vertex growth to max id+1,
+ * version += 1 when k > 0. */
+vgx_status vgx_graph_apply(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
+                           uint64_t k);
+
+/* vertex_count / version / directed / edge_count_directed / delta_edge_count
+ * -- dynamic_graph.hpp:486-491, 589-597. */
+vgx_status vgx_graph_info(const vgx_graph* g, uint64_t* vertex_count, uint64_t* version, int* directed,
+                          uint64_t* edge_count_directed);
+
+/* DynamicGraph::neighbors / in_neighbors -- dynamic_graph.hpp:551-557.
+ * Writes at most cap ids (ascending) to out and the full count to *count. */
+vgx_status vgx_graph_neighbors(vgx_graph* g, uint32_t u, int in_direction, uint32_t* out, uint64_t cap,
+                               uint64_t* count);
+
+/* DynamicGraph::has_edge -- dynamic_graph.hpp:583-587. */
+vgx_status vgx_graph_has_edge(vgx_graph* g, uint32_t u, uint32_t v, int* present);
+
+/* Live out-degree of every vertex (neighbor_count, graph_access.hpp:142-153)
+ * into out[vertex_count]. */
+vgx_status vgx_graph_degrees(vgx_graph* g, int in_direction, uint32_t* out);
+
+/* DynamicGraph::compact / consolidate_to_csr_snapshot --
+ * dynamic_graph.hpp:631-639, storage/consolidation.hpp:132-163: rewrites
+ * every row contiguously without tombstones (same logical graph). */
+vgx_status vgx_graph_compact(vgx_graph* g);
+
+/* Storage introspection (storage_bytes, delta counts; dynamic_graph.hpp:597-618). */
+typedef struct {
+  uint64_t arena_entries;    /* allocated id slots, both directions */
+  uint64_t arena_used;       /* slots handed out to rows */
+  uint64_t tombstones;       /* deleted ids still stored */
+  uint64_t storage_bytes;    /* device bytes held by the graph */
+  uint64_t relocated_rows;   /* rows moved since the last compaction */
+} vgx_graph_storage;
+vgx_status vgx_graph_storage_info(vgx_graph* g, vgx_graph_storage* out);
+
+/* Export the logical graph as CSR (ascending rows) into host buffers:
+ * off[vertex_count+1], adj[edge_count_directed]. */
+vgx_status vgx_graph_export_csr(vgx_graph* g, int in_direction, uint64_t* off, uint32_t* adj);
+
+/* -------------------------------------------------------------------- BFS */
+
+/* BasicIncrementalBFS(graph, source, deletion_fallback_fraction) --
+ * incremental/bfs.hpp:18-21; runs recompute() (direction-optimizing). */
+vgx_status vgx_bfs_create(vgx_graph* g, uint32_t source, double deletion_fallback_fraction, vgx_bfs** out);
+vgx_status vgx_bfs_destroy(vgx_bfs* b);
+
+/* BasicIncrementalBFS::recompute -- bfs.hpp:100-120. */
+vgx_status vgx_bfs_recompute(vgx_bfs* b);
+
+/* BasicIncrementalBFS::apply -- bfs.hpp:29-98: applies the batch to the
+ * graph and repairs the distances exactly (localized invalidation and
+ * re-relaxation, or a full recompute past the fallback fraction). */
+vgx_status vgx_bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
+                         uint64_t k);
+
+/* distances() -- bfs.hpp:23: copies *count (= vertex entries) uint32 values
+ * into out (cap entries max). */
+vgx_status vgx_bfs_distances(vgx_bfs* b, uint32_t* out, uint64_t cap, uint64_t* count);
+/* Device pointer to the distance array (valid until the next call). */
+vgx_status vgx_bfs_distances_device(vgx_bfs* b, const uint32_t** dist_dev, uint64_t* count);
+
+typedef struct {
+  uint64_t reachable_count;        /* bfs.hpp:24 */
+  uint64_t last_deletion_candidates;
+  uint64_t last_affected_vertices;
+  int last_used_full_recompute;
+  uint32_t levels;                 /* BFS levels of the last full recompute */
+  uint32_t bottom_up_levels;
+  uint64_t last_relaxed_vertices;  /* vertices whose distance changed in repair */
+} vgx_bfs_stats;
+vgx_status vgx_bfs_stats_get(vgx_bfs* b, vgx_bfs_stats* out);
+
+/* Direction-optimizing switch and launch shape (autotuned). */
+typedef struct {
+  double alpha;          /* top-down -> bottom-up when m_f > m_u / alpha */
+  double beta;           /* bottom-up -> top-down when n_f < n / beta */
+  uint32_t blocks;       /* level-kernel workgroups (0 = auto) */
+  uint32_t levels_per_sync; /* levels enqueued between host checks */
+} vgx_bfs_tuning;
+vgx_status vgx_bfs_set_tuning(vgx_bfs* b, const vgx_bfs_tuning* t);
+vgx_status vgx_bfs_get_tuning(vgx_bfs* b, vgx_bfs_tuning* t);
+
+/* --------------------------------------------------------------- PageRank */
+
+/* BasicIncrementalPageRank(graph, damping) -- incremental/pagerank.hpp:32-38.
+ * precision: 64 = FP64 ranks and sums (reference semantics), 32 = FP32 ranks
+ * with FP32 compensated accumulation. */
+vgx_status vgx_pagerank_create(vgx_graph* g, double damping, int precision, vgx_pagerank** out);
+vgx_status vgx_pagerank_destroy(vgx_pagerank* p);
+/* recompute(max_iterations, tol) -- pagerank.hpp:193-202 (L-inf stop). */
+vgx_status vgx_pagerank_recompute(vgx_pagerank* p, uint64_t max_iterations, double tol);
+/* Full solve with an L1 stop (the static push pagerank's criterion,
+ * src/algorithms.cpp:56-81), optionally warm-started from the current ranks. */
+vgx_status vgx_pagerank_solve_l1(vgx_pagerank* p, uint64_t max_iterations, double l1_tol, int warm_start);
+/* apply(batch, local_iterations, tol, full_fallback_fraction) -- pagerank.hpp:47-191. */
+vgx_status vgx_pagerank_apply(vgx_pagerank* p, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
+                              uint64_t k, uint64_t local_iterations, double tol, double full_fallback_fraction);
+/* values() as float64 -- pagerank.hpp:40. */
+vgx_status vgx_pagerank_values(vgx_pagerank* p, double* out, uint64_t cap, uint64_t* count);
+
+/* validate_against_full -- pagerank.hpp:204-232: FP64 reference solve on the
+ * device; out_errors = {l1_error, linf_error, reference_residual_l1,
+ * reference_residual_linf}. */
+vgx_status vgx_pagerank_validate(vgx_pagerank* p, uint64_t reference_max_iterations, double reference_tol,
+                                 double* out_errors, uint64_t* reference_iterations, int* reference_converged);
+/* apply_validated's fallback -- pagerank.hpp:246-256: install the FP64
+ * reference solve as the current values. */
+vgx_status vgx_pagerank_install_reference(vgx_pagerank* p, uint64_t reference_max_iterations, double reference_tol);
+/* Device pointer to the current ranks (float32 or float64 per precision). */
+vgx_status vgx_pagerank_values_device(vgx_pagerank* p, const void** dev, uint64_t* count, int* precision);
+
+typedef struct {
+  uint64_t last_repaired_vertices;  /* pagerank.hpp:41-45 */
+  uint64_t last_repair_iterations;
+  double last_residual_l1;
+  double last_residual_linf;
+  int last_full_recompute_converged;
+  uint64_t dangling_vertices;
+  int last_used_full_recompute;
+} vgx_pagerank_stats;
+vgx_status vgx_pagerank_stats_get(vgx_pagerank* p, vgx_pagerank_stats* out);
+
+/* ------------------------------------------------------ multi-source BFS */
+
+/* 1..64 roots traversed together (bit-parallel frontiers, uint8 levels). */
+vgx_status vgx_msbfs_create(vgx_graph* g, const uint32_t* roots, uint32_t k, vgx_msbfs** out);
+vgx_status vgx_msbfs_destroy(vgx_msbfs* m);
+vgx_status vgx_msbfs_run(vgx_msbfs* m);
+/* Distances of root r as uint32 (VGX_UNREACHABLE) into out[vertex_count]. */
+vgx_status vgx_msbfs_distances(vgx_msbfs* m, uint32_t r, uint32_t* out, uint64_t cap);
+/* Root-major uint8 levels (255 = unreachable) of all roots: out[k * vertex_count]. */
+vgx_status vgx_msbfs_levels_u8(vgx_msbfs* m, uint8_t* out, uint64_t cap);
+vgx_status vgx_msbfs_reached(vgx_msbfs* m, uint64_t* reached_per_root, uint32_t cap);
+
+/* ------------------------------------------ weighted personalized PageRank */
+
+/* Forward: X = (1-d) S + d P^T X with P_uv = w_uv / sum_v' w_uv' over the
+ * out-CSR snapshot (csr_off/csr_adj device arrays of the graph, obtained
+ * with vgx_graph_csr_device).  S, X are n x K row-major float32 device
+ * arrays.  Fixed iteration count `iterations`. */
+vgx_status vgx_graph_csr_device(vgx_graph* g, int in_direction, const uint32_t** off_dev,
+                                const uint32_t** adj_dev, uint64_t* vertices, uint64_t* arcs);
+vgx_status vgx_ppr_forward(vgx_graph* g, const float* w_dev, const float* s_dev, float* x_dev, uint32_t K,
+                           double damping, uint32_t iterations, void* stream);
+/* Adjoint: L = C + d P L; grad_s = (1-d) L; grad_w (raw weights, through the
+ * row normalisation) per arc of the out-CSR snapshot. */
+vgx_status vgx_ppr_backward(vgx_graph* g, const float* w_dev, const float* x_dev, const float* c_dev,
+                            float* lambda_dev, float* grad_s_dev, float* grad_w_dev, uint32_t K,
+                            double damping, uint32_t iterations, void* stream);
+
+/* ------------------------------------------------------- workload helpers */
+
+/* Graph500-style R-MAT tuple stream of scripts/prepare_publication_datasets.py
+ * :80-129 (A,B,C,D = .57,.19,.19,.05, no permutation), driven by the MT19937
+ * stream of Python's random.Random(seed).random().  Writes m = edgefactor <<
+ * scale tuples into src/dst (host). */
+vgx_status vgx_rmat_generate(uint32_t scale, uint32_t edgefactor, uint64_t seed, uint32_t* src, uint32_t* dst);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VGX_H */

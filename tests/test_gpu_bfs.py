@@ -1,0 +1,266 @@
+"""GPU parity: graph storage, full BFS and incremental BFS repair against the
+CPU oracle (bit-exact distances; diagnostics as the reference defines them)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+INF = 0xFFFFFFFF
+
+
+@pytest.fixture(scope="module")
+def vx():
+    import velographx_amd as vx
+
+    assert vx.device_count() > 0, "no gfx950 device visible"
+    return vx
+
+
+def ops_arrays(ops):
+    return (np.array([o[0] for o in ops], np.uint32), np.array([o[1] for o in ops], np.uint32),
+            np.array([o[2] for o in ops], np.uint8))
+
+
+def batch_of(vx, ops):
+    b = vx.UpdateBatch()
+    for u, v, a in ops:
+        (b.add if a else b.remove)(u, v)
+    return b
+
+
+def check_graph(g, model):
+    assert g.vertex_count == model.n
+    assert g.edge_count_directed == model.edge_count_directed()
+    off, adj = g.export_csr()
+    moff, madj = model.csr()
+    assert np.array_equal(off, moff) and np.array_equal(adj, madj)
+    if model.directed:
+        off, adj = g.export_csr(True)
+        moff, madj = model.csr(reverse=True)
+        assert np.array_equal(off, moff) and np.array_equal(adj, madj)
+
+
+# ------------------------------------------------------------ reference smoke
+
+
+def test_ci_smoke(vx):
+    # .github/workflows/ci.yml:188-227
+    g = vx.from_numpy_edges(np.array([[0, 1], [1, 2], [2, 0]], dtype=np.uint32))
+    assert g.vertex_count == 3
+    assert g.neighbors(0) == [1, 2]
+    import scipy.sparse as sp
+
+    csr = sp.csr_matrix((np.ones(3), ([0, 1, 2], [1, 2, 0])), shape=(3, 3))
+    g2 = vx.from_scipy_csr(csr, directed=True)
+    assert g2.vertex_count == 3 and g2.neighbors(1) == [2]
+    g4 = vx.Graph(4, False)
+    b = vx.UpdateBatch()
+    b.add(0, 1)
+    b.add(1, 2)
+    g4.apply(b)
+    bfs = vx.IncrementalBFS(g4, 0)
+    assert bfs.distances == [0, 1, 2, vx.IncrementalBFS.unreachable]
+    up = vx.UpdateBatch()
+    up.add(2, 3)
+    bfs.apply(up)
+    assert bfs.distances == [0, 1, 2, 3]
+
+
+def test_static_known_answer(vx):
+    # tests/test_static.cpp:12-18
+    g = vx.Graph(0, False)
+    g.bulk_load_edges(np.array([[0, 1], [1, 2], [2, 0], [2, 3]], dtype=np.uint32))
+    assert g.edge_count_directed == 8
+    assert vx.IncrementalBFS(g, 0).distances == [0, 1, 1, 2]
+
+
+def test_self_loop_on_new_vertex_ignored(vx):
+    # tests/test_incremental.cpp:205-217
+    g = vx.Graph(2, False)
+    g.add_edge(9, 9)
+    assert g.vertex_count == 2 and g.edge_count_directed == 0
+    assert not g.has_edge(9, 9)
+    g.compact()
+    assert not g.has_edge(9, 9)
+
+
+def test_graph_semantics_random(vx, orc):
+    rng = np.random.default_rng(3)
+    for directed in (False, True):
+        g = vx.Graph(30, directed)
+        m = orc.ModelGraph(30, directed)
+        e = rng.integers(0, 30, size=(80, 2)).astype(np.uint32)
+        g.bulk_load_edges(e)
+        m.bulk_load_edges(e.tolist())
+        check_graph(g, m)
+        assert g.version == m.version
+        for step in range(25):
+            k = int(rng.integers(1, 40))
+            s = rng.integers(0, 36 + step, size=k).astype(np.uint32)
+            d = rng.integers(0, 36 + step, size=k).astype(np.uint32)
+            a = (rng.random(k) < 0.6).astype(np.uint8)
+            g.apply_arrays(s, d, a)
+            m.apply(s, d, a)
+            check_graph(g, m)
+            assert g.version == m.version
+            for u in range(0, m.n, 7):
+                assert g.neighbors(u) == m.neighbors(u)
+            if step % 8 == 7:
+                g.compact()
+                check_graph(g, m)
+
+
+def test_row_relocation_and_big_rows(vx, orc):
+    # a hub that outgrows its slack several times, merged block-wide
+    g = vx.Graph(9000, False)
+    m = orc.ModelGraph(9000, False)
+    g.bulk_load_edges(np.array([[0, 1], [0, 2]], dtype=np.uint32))
+    m.bulk_load_edges([[0, 1], [0, 2]])
+    rng = np.random.default_rng(1)
+    for k in (10, 100, 1000, 5000):
+        v = rng.integers(1, 9000, size=k).astype(np.uint32)
+        s = np.zeros(k, np.uint32)
+        a = np.ones(k, np.uint8)
+        g.apply_arrays(s, v, a)
+        m.apply(s, v, a)
+        check_graph(g, m)
+    st = g.storage()
+    assert st["relocated_rows"] > 0
+
+
+# ------------------------------------------------------------ BFS
+
+
+HAND = [
+    ("alternate-parent", 5, True, [(0, 1), (0, 2), (1, 3), (2, 3), (3, 4)], 0, 0.75, [(1, 3, 0)], 0, False),
+    ("support-fixpoint", 5, True, [(0, 1), (0, 2), (1, 3), (2, 3), (3, 4)], 0, 0.90, [(0, 1, 0), (0, 2, 0)], 4,
+     False),
+    ("surviving-branch", 7, True, [(0, 1), (1, 2), (2, 3), (3, 4), (0, 5), (5, 6), (6, 3)], 0, 0.75, [(1, 2, 0)],
+     None, False),
+    ("mixed", 6, True, [(0, 1), (1, 2), (2, 3), (3, 4), (0, 5)], 0, 0.75, [(1, 2, 0), (5, 3, 1)], None, False),
+    ("undirected", 5, False, [(0, 1), (1, 2), (2, 3), (0, 4), (4, 3)], 0, 0.75, [(1, 2, 0)], None, False),
+    ("final-state", 5, True, [(0, 1), (1, 2), (2, 3), (3, 4)], 0, 0.75, [(0, 3, 1), (0, 3, 0)], None, False),
+    ("fallback", 20, True, [(i, i + 1) for i in range(19)], 0, 0.05, [(0, 1, 0)], None, True),
+]
+
+
+@pytest.mark.parametrize("case", HAND, ids=[c[0] for c in HAND])
+def test_incremental_hand_cases(vx, orc, case):
+    # tests/test_incremental_bfs_deletion.cpp:57-161
+    name, n, directed, edges, src, fb, ops, affected, full = case
+    g = vx.Graph(n, directed)
+    g.bulk_load_edges(np.array(edges, dtype=np.uint32))
+    bfs = vx.IncrementalBFS(g, src, fb)
+    mg = orc.ModelGraph(n, directed)
+    mg.bulk_load_edges(edges)
+    ref = orc.ModelIncrementalBFS(mg, src, fb)
+    assert bfs.distances == ref.dist.tolist()
+    bfs.apply(batch_of(vx, ops))
+    ref.apply(*ops_arrays(ops))
+    assert bfs.distances == ref.dist.tolist(), name
+    assert bfs.reachable_count == ref.reachable
+    assert bfs.last_used_full_recompute == ref.diag["used_full_recompute"] == full
+    assert bfs.last_deletion_candidates == ref.diag["deletion_candidates"]
+    if not full:
+        assert bfs.last_affected_vertices == ref.diag["affected_vertices"]
+    if affected is not None:
+        assert bfs.last_affected_vertices == affected
+
+
+def test_out_of_range_source(vx):
+    # test_incremental_bfs_deletion.cpp:163-169
+    g = vx.Graph(4, True)
+    g.bulk_load_edges(np.array([[0, 1], [1, 2]], dtype=np.uint32))
+    bfs = vx.IncrementalBFS(g, 7, 0.75)
+    assert bfs.reachable_count == 0
+    assert bfs.distances == [INF] * 4
+
+
+@pytest.mark.parametrize("directed", [False, True])
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_incremental_randomized_vs_oracle(vx, orc, directed, seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(30, 200))
+    e = rng.integers(0, n, size=(n * 3, 2)).astype(np.uint32)
+    fb = [0.35, 0.05, 2.0][seed % 3]
+    g = vx.Graph(n, directed)
+    g.bulk_load_edges(e)
+    mg = orc.ModelGraph(n, directed)
+    mg.bulk_load_edges(e.tolist())
+    bfs = vx.IncrementalBFS(g, 0, fb)
+    ref = orc.ModelIncrementalBFS(mg, 0, fb)
+    assert np.array_equal(bfs.distances_numpy(), ref.dist)
+    for step in range(40):
+        k = int(rng.integers(1, 25))
+        s = rng.integers(0, n + (step % 5), size=k).astype(np.uint32)
+        d = rng.integers(0, n + (step % 5), size=k).astype(np.uint32)
+        # deletions aimed at existing edges half of the time
+        a = (rng.random(k) < 0.5).astype(np.uint8)
+        off, adj = mg.csr()
+        for i in range(k):
+            if not a[i] and adj.size:
+                j = int(rng.integers(0, adj.size))
+                s[i] = np.searchsorted(off, j, side="right") - 1
+                d[i] = adj[j]
+        bfs.apply_arrays(s, d, a)
+        ref.apply(s, d, a)
+        got = bfs.distances_numpy()
+        assert np.array_equal(got, ref.dist), (step, np.nonzero(got != ref.dist)[0][:10])
+        assert bfs.reachable_count == ref.reachable
+        assert bfs.last_used_full_recompute == ref.diag["used_full_recompute"]
+        assert bfs.last_deletion_candidates == ref.diag["deletion_candidates"]
+        if not ref.diag["used_full_recompute"]:
+            assert bfs.last_affected_vertices == ref.diag["affected_vertices"], step
+        full, _ = orc.model_bfs(mg, 0)
+        assert np.array_equal(got, full)
+
+
+def test_rmat_s16_full_bfs_and_10_batches(vx, orc):
+    """R-MAT scale 16, ef 16, seed 1 (undirected): full BFS bit-exact, then 10
+    batches of 1,000 ops (80% inserts of random non-edges, 20% deletions of
+    existing edges) repaired and compared with a full recompute each time."""
+    from velographx_amd import workloads as wl
+
+    s, d = vx.rmat_edges(16, 16, 1)
+    g = vx.Graph(1 << 16, False)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(1 << 16, False)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    assert g.edge_count_directed == mg.edge_count_directed()
+    lcc = wl.largest_component_from_hub(vx, g)
+    root = int(wl.roots_in_component(lcc, 1, seed=1)[0])
+    bfs = vx.IncrementalBFS(g, root)
+    full, reached = orc.model_bfs(mg, root)
+    assert np.array_equal(bfs.distances_numpy(), full)
+    assert bfs.reachable_count == reached == lcc.size
+    eu, ev = wl.canonical_edges(s, d, False)
+    stream = wl.BatchStream(eu, ev, 1 << 16, False, seed=1)
+    for i in range(10):
+        bs, bd, ba = stream.next(1000, 0.8)
+        bfs.apply_arrays(bs, bd, ba)
+        mg.apply(bs, bd, ba)
+        full, reached = orc.model_bfs(mg, root)
+        got = bfs.distances_numpy()
+        assert np.array_equal(got, full), (i, int((got != full).sum()))
+        assert bfs.reachable_count == reached
+    assert g.edge_count_directed == mg.edge_count_directed()
+
+
+def test_direction_switch_levels(vx, orc):
+    """Force bottom-up from the first level and top-down only: both agree
+    with the oracle."""
+    s, d = vx.rmat_edges(12, 16, 1)
+    g = vx.Graph(1 << 12, False)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(1 << 12, False)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    for root in (0, 5, 100):
+        full, _ = orc.model_bfs(mg, root)
+        bfs = vx.IncrementalBFS(g, root)
+        for alpha, beta in ((1e9, 1e9), (1e-9, 1.0), (15.0, 18.0)):
+            bfs.set_tuning(alpha=alpha, beta=beta, blocks=0, levels_per_sync=3)
+            bfs.recompute()
+            assert np.array_equal(bfs.distances_numpy(), full)
+            st = bfs.stats()
+            if alpha > 1e8:
+                assert st["bottom_up_levels"] >= 1

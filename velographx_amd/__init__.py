@@ -1,0 +1,35 @@
+"""VeloGraphX traversal path on AMD Instinct MI355X (gfx950).
+
+Drop-in for the reference's ``velographx`` Python module on the traversal
+path: ``Graph``, ``UpdateBatch``, ``IncrementalBFS``, ``IncrementalPageRank``,
+``PageRankValidation`` and the ``from_*`` ingestion helpers, backed by
+hand-written HIP kernels (``libvgx.so``) through a C++ facade.
+
+``import torch`` comes first on purpose: torch ships its own HIP runtime,
+and loading it before ``libvgx.so`` makes both share one runtime (same
+SONAME), so device pointers from torch tensors are valid in the engine.
+The extension must be built (``python velographx_amd/build.py``); there is
+no CPU fallback.
+"""
+import torch  # noqa: F401  (see module docstring)
+
+from ._velographx import (  # noqa: F401
+    Graph,
+    IncrementalBFS,
+    IncrementalPageRank,
+    PageRankValidation,
+    UpdateBatch,
+    device_count,
+    from_arrow_columns,
+    from_arrow_table,
+    from_numpy_edges,
+    from_scipy_csr,
+    rmat_edges,
+    version,
+)
+
+__all__ = [
+    "Graph", "UpdateBatch", "IncrementalBFS", "IncrementalPageRank", "PageRankValidation",
+    "from_numpy_edges", "from_scipy_csr", "from_arrow_columns", "from_arrow_table",
+    "rmat_edges", "device_count", "version",
+]

@@ -1,0 +1,981 @@
+// Direction-optimizing BFS and exact incremental BFS repair for gfx950.
+//
+// Reference: include/velographx/incremental/bfs.hpp (BasicIncrementalBFS:
+// recompute :100-120, apply :29-98, compute_affected_prebatch :222-259,
+// repair_affected :271-319, propagate_decreases :341-354) and the static
+// bfs_distances in src/algorithms.cpp:11-27.  BFS distances are unique, so
+// every traversal order below yields the reference's vector bit for bit.
+//
+// Full BFS: one kernel launch per level on a fixed grid.  The direction of
+// each level is chosen on the device by the last workgroup of the previous
+// level (Beamer's rule: top-down -> bottom-up when m_f > m_u / alpha,
+// bottom-up -> top-down when n_f < n / beta), so the host only checks for
+// completion every few levels.
+//   top-down : the frontier is a packed queue (vertex, edge prefix); every
+//              wave takes an equal slice of the frontier's edges (load-
+//              balanced search), claims new vertices with one atomicOr on
+//              the visited bitmap and pushes them wave-aggregated.
+//   bottom-up: every wave owns 64 consecutive vertices (one visited word);
+//              unvisited lanes scan their in-row (parents) until one is in
+//              the frontier bitmap; one ballot gives the next frontier word.
+//
+// Incremental repair (apply):
+//   1. deletions are applied first (tombstones), then vertices that lost
+//      every live shortest parent are found level-asynchronously with
+//      counters: support[v] = live in-edges from d(v)-1, lost[v] = losses
+//      from affected parents; v is affected once lost[v] >= support[v].
+//      This is the reference's support-count fixpoint (bfs.hpp:222-259)
+//      with the deleted edges removed from both sides of the comparison.
+//   2. additions are applied; past the fallback fraction -> full BFS.
+//   3. affected vertices are reset, seeded from their unaffected
+//      in-neighbours, the final additions are relaxed, and decreases are
+//      propagated in Bellman-Ford rounds (each round one edge-balanced
+//      launch).  The fixpoint satisfies d(w) <= d(u)+1 on every arc with
+//      realisable upper bounds, hence equals the BFS distances.
+#include <algorithm>
+#include <vector>
+
+#include "vgx_internal.hpp"
+
+namespace vgx {
+vgx_status set_error(vgx_status s, const std::string& msg);
+extern thread_local std::string t_last_error;
+
+constexpr int kBfsBlock = 256;
+constexpr uint32_t kWavesPerBlock = kBfsBlock / kWave;
+
+struct BfsState {
+  uint32_t level;
+  uint32_t mode;  // 0 top-down, 1 bottom-up
+  uint32_t done;
+  uint32_t blocks_done;
+  unsigned long long q[2];      // packed frontier counters by level parity
+  unsigned long long found[2];  // discovered vertices (incl. zero-length rows)
+  unsigned long long reached;
+  unsigned long long visited_edges;
+  unsigned long long total_edges;
+  uint32_t n;
+  uint32_t levels;
+  uint32_t bu_levels;
+  uint32_t pad;
+  float alpha;
+  float beta;
+};
+
+// Counters of the incremental repair.
+struct IncState {
+  unsigned long long tq;        // touched list (plain count)
+  unsigned long long tfq;       // first-touched list, packed (in-rows)
+  unsigned long long aq;        // newly affected, packed (out-rows)
+  unsigned long long aall;      // all affected (plain count)
+  unsigned long long tall;      // all touched (plain count)
+  unsigned long long candidates;
+  unsigned long long fq[2];     // repair frontier, packed (out-rows)
+  unsigned long long sq;        // seed list, packed (in-rows)
+  unsigned long long became_reachable;
+  unsigned long long relaxed;
+};
+
+enum : uint32_t { kFAffected = 1u, kFSupport = 2u, kFCand = 4u };
+
+struct BfsWork {
+  DevBuf<uint32_t> dist;
+  DevBuf<unsigned long long> visited;
+  DevBuf<unsigned long long> bm;  // 3 bitmaps
+  DevBuf<uint32_t> queue[2];
+  DevBuf<unsigned long long> pre[2];
+  // incremental
+  DevBuf<uint32_t> lost, support, flags, stamp;
+  DevBuf<uint32_t> tlist, tflist, alist, aall, tall, slist;
+  DevBuf<unsigned long long> tfpre, apre, spre;
+  uint64_t cap = 0;
+};
+
+}  // namespace vgx
+
+struct vgx_bfs {
+  vgx_graph* g = nullptr;
+  uint32_t source = 0;
+  double fallback = 0.35;
+  uint64_t dist_len = 0;
+  uint64_t reachable = 0;
+  uint64_t last_candidates = 0, last_affected = 0, last_relaxed = 0;
+  bool last_full = false;
+  uint32_t levels = 0, bu_levels = 0;
+  vgx_bfs_tuning tune{15.0, 18.0, 0, 8};
+  vgx::BfsWork w;
+  vgx::BfsState* d_st = nullptr;
+  vgx::BfsState* h_st = nullptr;
+  vgx::IncState* d_inc = nullptr;
+  vgx::IncState* h_inc = nullptr;
+  uint32_t epoch = 0;
+};
+
+namespace vgx {
+
+// ---------------------------------------------------------------- full BFS
+
+struct LevelArgs {
+  const uint2* row;       // out-rows (top-down)
+  const uint32_t* arena;
+  const uint2* in_row;    // in-rows (bottom-up scans parents)
+  const uint32_t* in_arena;
+  uint32_t* dist;
+  unsigned long long* visited;
+  unsigned long long* bm;
+  uint32_t* queue0;
+  uint32_t* queue1;
+  unsigned long long* pre0;
+  unsigned long long* pre1;
+  BfsState* st;
+  uint32_t n;
+  uint32_t words;
+};
+
+__global__ void k_bfs_setup(LevelArgs a, uint32_t root, unsigned long long total_edges, float alpha, float beta) {
+  BfsState* st = a.st;
+  st->level = 0;
+  st->mode = 0;
+  st->blocks_done = 0;
+  st->q[0] = st->q[1] = 0;
+  st->found[0] = st->found[1] = 0;
+  st->total_edges = total_edges;
+  st->n = a.n;
+  st->levels = 0;
+  st->bu_levels = 0;
+  st->alpha = alpha;
+  st->beta = beta;
+  if (root >= a.n) {
+    st->done = 1;
+    st->reached = 0;
+    st->visited_edges = 0;
+    return;
+  }
+  st->done = 0;
+  a.dist[root] = 0;
+  a.visited[root >> 6] |= 1ull << (root & 63);
+  a.bm[root >> 6] |= 1ull << (root & 63);
+  uint32_t len = a.row[root].y;
+  st->found[0] = 1;
+  st->reached = 1;
+  st->visited_edges = len;
+  if (len) {
+    a.queue0[0] = root;
+    a.pre0[0] = 0;
+    st->q[0] = (1ull << kEdgeBits) | len;
+  }
+}
+
+__global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a) {
+  __shared__ int s_last;
+  BfsState* st = a.st;
+  if (st->done) return;
+  const uint32_t level = st->level;
+  const uint32_t mode = st->mode;
+  const int cur = level & 1, nxt = cur ^ 1;
+  const uint32_t words = a.words;
+  unsigned long long* bm_cur = a.bm + static_cast<uint64_t>(level % 3) * words;
+  unsigned long long* bm_next = a.bm + static_cast<uint64_t>((level + 1) % 3) * words;
+  unsigned long long* bm_clear = a.bm + static_cast<uint64_t>((level + 2) % 3) * words;
+  const uint32_t gthreads = gridDim.x * blockDim.x;
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t i = gtid; i < words; i += gthreads) bm_clear[i] = 0;
+  const uint32_t W = gridDim.x * kWavesPerBlock;
+  const uint32_t gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  uint32_t* qn = nxt ? a.queue1 : a.queue0;
+  unsigned long long* pn = nxt ? a.pre1 : a.pre0;
+  const uint32_t next_level = level + 1;
+
+  if (mode == 0) {
+    const unsigned long long qc = st->q[cur];
+    const uint32_t F = static_cast<uint32_t>(qc >> kEdgeBits);
+    const unsigned long long E = qc & kEdgeMask;
+    const uint32_t* qcur = cur ? a.queue1 : a.queue0;
+    const unsigned long long* pcur = cur ? a.pre1 : a.pre0;
+    lbs_edges(qcur, pcur, F, E, a.row, gw, W,
+              [&](uint32_t, uint32_t, uint2, uint32_t slot, bool valid) {
+                uint32_t w = valid ? a.arena[slot] : kTomb;
+                bool newly = false;
+                uint32_t wl = 0;
+                if (!(w & kTomb)) {
+                  unsigned long long bit = 1ull << (w & 63);
+                  unsigned long long* wp = a.visited + (w >> 6);
+                  if (!(*wp & bit)) {
+                    unsigned long long old = atomicOr(wp, bit);
+                    newly = !(old & bit);
+                  }
+                }
+                if (newly) {
+                  a.dist[w] = next_level;
+                  atomicOr(bm_next + (w >> 6), 1ull << (w & 63));
+                  wl = a.row[w].y;
+                }
+                unsigned long long nb = __ballot(newly);
+                if (nb && lane == 0) atomicAdd(&st->found[nxt], static_cast<unsigned long long>(__popcll(nb)));
+                wave_push(&st->q[nxt], qn, pn, newly && wl > 0, w, wl);
+              });
+  } else {
+    for (uint32_t wi = gw; wi < words; wi += W) {
+      const unsigned long long vw = a.visited[wi];
+      if (vw == ~0ull) continue;
+      const uint32_t v = wi * 64 + lane;
+      const bool unv = v < a.n && !((vw >> lane) & 1ull);
+      bool found = false;
+      uint32_t len = 0;
+      if (unv) {
+        const uint2 rb = a.in_row[v];
+        len = a.row[v].y;
+        for (uint32_t j = 0; j < rb.y; ++j) {
+          uint32_t x = a.in_arena[rb.x + j];
+          if (x & kTomb) continue;
+          if ((bm_cur[x >> 6] >> (x & 63)) & 1ull) {
+            found = true;
+            break;
+          }
+        }
+      }
+      const unsigned long long m = __ballot(found);
+      if (m) {
+        if (lane == 0) {
+          bm_next[wi] = m;
+          a.visited[wi] = vw | m;
+          atomicAdd(&st->found[nxt], static_cast<unsigned long long>(__popcll(m)));
+        }
+        if (found) a.dist[v] = next_level;
+        wave_push(&st->q[nxt], qn, pn, found && len > 0, v, len);
+      }
+    }
+  }
+
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    s_last = atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last && threadIdx.x == 0) {
+    __threadfence();
+    const unsigned long long qv = atomic_load_u64(&st->q[nxt]);
+    const unsigned long long fv = atomic_load_u64(&st->found[nxt]);
+    const unsigned long long mf = qv & kEdgeMask;
+    st->reached += fv;
+    st->visited_edges += mf;
+    st->levels += 1;
+    if (mode == 1) st->bu_levels += 1;
+    uint32_t next_mode = mode;
+    if (fv == 0) {
+      st->done = 1;
+    } else if (mode == 0) {
+      const unsigned long long mu = st->total_edges > st->visited_edges ? st->total_edges - st->visited_edges : 0;
+      if (static_cast<double>(mf) * st->alpha > static_cast<double>(mu)) next_mode = 1;
+    } else {
+      if (static_cast<double>(fv) * st->beta < static_cast<double>(st->n)) next_mode = 0;
+    }
+    st->mode = next_mode;
+    st->q[cur] = 0;
+    st->found[cur] = 0;
+    st->level = next_level;
+    st->blocks_done = 0;
+    __threadfence();
+  }
+}
+
+// -------------------------------------------------------- incremental kernels
+
+struct IncArgs {
+  const uint2* out_row;
+  const uint32_t* out_arena;
+  const uint2* in_row;
+  const uint32_t* in_arena;
+  uint32_t* dist;
+  uint32_t* lost;
+  uint32_t* support;
+  uint32_t* flags;
+  uint32_t* stamp;
+  IncState* st;
+  uint32_t epoch;
+  uint32_t source;
+};
+
+// touch(): record that v received a loss (or is a candidate) this round.
+__device__ __forceinline__ void touch(const IncArgs& a, uint32_t v, uint32_t* tlist, uint32_t* tall,
+                                      uint32_t* tflist, unsigned long long* tfpre, bool want) {
+  bool t_new = false, first = false;
+  uint32_t inlen = 0;
+  if (want) {
+    t_new = atomicExch(&a.stamp[v], a.epoch) != a.epoch;
+    uint32_t old = atomicOr(&a.flags[v], kFSupport);
+    first = !(old & kFSupport);
+    if (first) {
+      a.support[v] = 0;
+      inlen = a.in_row[v].y;
+    }
+  }
+  unsigned long long bt = __ballot(t_new);
+  if (bt) {
+    unsigned long long base = 0;
+    if (lane_id() == 0) base = atomicAdd(&a.st->tq, static_cast<unsigned long long>(__popcll(bt)));
+    base = __shfl(base, 0, kWave);
+    if (t_new) tlist[base + lanes_below(bt)] = v;
+  }
+  unsigned long long bf = __ballot(first);
+  if (bf) {
+    unsigned long long base = 0;
+    if (lane_id() == 0) base = atomicAdd(&a.st->tall, static_cast<unsigned long long>(__popcll(bf)));
+    base = __shfl(base, 0, kWave);
+    if (first) tall[base + lanes_below(bf)] = v;
+  }
+  wave_push(&a.st->tfq, tflist, tfpre, first && inlen > 0, v, inlen);
+}
+
+// Heads of deleted arcs that were shortest-parent arcs (bfs.hpp:52-63, 242-245).
+__global__ void k_inc_candidates(IncArgs a, const uint32_t* fin, uint64_t nfin, const unsigned long long* skeys,
+                                 const uint8_t* act, uint32_t* tlist, uint32_t* tall, uint32_t* tflist,
+                                 unsigned long long* tfpre) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (uint64_t j0 = base - lane_id(); j0 < nfin; j0 += stride) {
+    const uint64_t j = j0 + lane_id();
+    bool want = false;
+    uint32_t v = 0;
+    if (j < nfin) {
+      uint32_t p = fin[j];
+      if (act[p] == kActDel) {
+        uint32_t u = static_cast<uint32_t>(skeys[p] >> 32);
+        v = static_cast<uint32_t>(skeys[p]);
+        uint32_t du = a.dist[u], dv = a.dist[v];
+        if (du != kInf && dv != kInf && du + 1 == dv) {
+          uint32_t old = atomicOr(&a.flags[v], kFCand);
+          if (!(old & kFCand)) atomicAdd(&a.st->candidates, 1ull);
+          want = v != a.source;
+        }
+      }
+    }
+    touch(a, v, tlist, tall, tflist, tfpre, want);
+  }
+}
+
+// support[v] += live in-arcs p->v with d(p) + 1 == d(v).
+__global__ void k_inc_support(IncArgs a, const uint32_t* tflist, const unsigned long long* tfpre) {
+  const unsigned long long qc = a.st->tfq;
+  const uint32_t W = gridDim.x * kWavesPerBlock;
+  const uint32_t gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  lbs_edges(tflist, tfpre, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.in_row, gw, W,
+            [&](uint32_t, uint32_t v, uint2, uint32_t slot, bool valid) {
+              if (!valid) return;
+              uint32_t p = a.in_arena[slot];
+              if (p & kTomb) return;
+              uint32_t dp = a.dist[p];
+              if (dp != kInf && dp + 1 == a.dist[v]) atomicAdd(&a.support[v], 1u);
+            });
+}
+
+// v is affected once every live shortest parent is lost.
+__global__ void k_inc_evaluate(IncArgs a, const uint32_t* tlist, uint64_t nt, uint32_t* alist,
+                               unsigned long long* apre, uint32_t* aall) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (uint64_t j0 = base - lane_id(); j0 < nt; j0 += stride) {
+    const uint64_t j = j0 + lane_id();
+    bool aff = false;
+    uint32_t v = 0, len = 0;
+    if (j < nt) {
+      v = tlist[j];
+      uint32_t f = a.flags[v];
+      if (!(f & kFAffected) && a.lost[v] >= a.support[v]) {
+        a.flags[v] = f | kFAffected;
+        aff = true;
+        len = a.out_row[v].y;
+      }
+    }
+    unsigned long long b = __ballot(aff);
+    if (b) {
+      unsigned long long base2 = 0;
+      if (lane_id() == 0) base2 = atomicAdd(&a.st->aall, static_cast<unsigned long long>(__popcll(b)));
+      base2 = __shfl(base2, 0, kWave);
+      if (aff) aall[base2 + lanes_below(b)] = v;
+    }
+    wave_push(&a.st->aq, alist, apre, aff && len > 0, v, len);
+  }
+}
+
+// Affected u makes its live children at d(u)+1 lose a parent (bfs.hpp:247-257).
+__global__ void k_inc_emit(IncArgs a, const uint32_t* alist, const unsigned long long* apre, uint32_t* tlist,
+                           uint32_t* tall, uint32_t* tflist, unsigned long long* tfpre) {
+  const unsigned long long qc = a.st->aq;
+  const uint32_t W = gridDim.x * kWavesPerBlock;
+  const uint32_t gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  lbs_edges(alist, apre, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.out_row, gw, W,
+            [&](uint32_t, uint32_t u, uint2, uint32_t slot, bool valid) {
+              bool want = false;
+              uint32_t w = 0;
+              if (valid) {
+                w = a.out_arena[slot];
+                if (!(w & kTomb)) {
+                  uint32_t du = a.dist[u];
+                  if (du != kInf && a.dist[w] == du + 1) {
+                    atomicAdd(&a.lost[w], 1u);
+                    want = true;
+                  }
+                } else {
+                  w = 0;
+                }
+              }
+              touch(a, w, tlist, tall, tflist, tfpre, want);
+            });
+}
+
+__global__ void k_inc_reset_affected(IncArgs a, const uint32_t* aall, uint64_t na, uint32_t* slist,
+                                     unsigned long long* spre) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (uint64_t j0 = base - lane_id(); j0 < na; j0 += stride) {
+    const uint64_t j = j0 + lane_id();
+    uint32_t v = 0, len = 0;
+    bool want = false;
+    if (j < na) {
+      v = aall[j];
+      a.dist[v] = kInf;
+      len = a.in_row[v].y;
+      want = len > 0;
+    }
+    wave_push(&a.st->sq, slist, spre, want, v, len);
+  }
+}
+
+// best_boundary_distance (bfs.hpp:261-269) as an atomicMin over in-arcs.
+__global__ void k_inc_seed(IncArgs a, const uint32_t* slist, const unsigned long long* spre) {
+  const unsigned long long qc = a.st->sq;
+  const uint32_t W = gridDim.x * kWavesPerBlock;
+  const uint32_t gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  lbs_edges(slist, spre, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.in_row, gw, W,
+            [&](uint32_t, uint32_t v, uint2, uint32_t slot, bool valid) {
+              if (!valid) return;
+              uint32_t p = a.in_arena[slot];
+              if (p & kTomb) return;
+              if (a.flags[p] & kFAffected) return;
+              uint32_t dp = a.dist[p];
+              if (dp != kInf) atomicMin(&a.dist[v], dp + 1);
+            });
+}
+
+__device__ __forceinline__ void frontier_push(const IncArgs& a, int slot_par, uint32_t* fq, unsigned long long* fp,
+                                              uint32_t v, bool want) {
+  bool push = false;
+  uint32_t len = 0;
+  if (want) {
+    push = atomicExch(&a.stamp[v], a.epoch) != a.epoch;
+    if (push) len = a.out_row[v].y;
+  }
+  wave_push(&a.st->fq[slot_par], fq, fp, push && len > 0, v, len);
+}
+
+__global__ void k_inc_collect_seeds(IncArgs a, const uint32_t* aall, uint64_t na, uint32_t* fq,
+                                    unsigned long long* fp) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  unsigned long long reach = 0;
+  for (uint64_t j0 = base - lane_id(); j0 < na; j0 += stride) {
+    const uint64_t j = j0 + lane_id();
+    uint32_t v = 0;
+    bool want = false;
+    if (j < na) {
+      v = aall[j];
+      want = a.dist[v] != kInf;
+      reach += want;
+    }
+    frontier_push(a, 0, fq, fp, v, want);
+  }
+  reach = wave_sum(reach);
+  if (lane_id() == 0 && reach) {
+    atomicAdd(&a.st->became_reachable, reach);
+    atomicAdd(&a.st->relaxed, reach);
+  }
+}
+
+// relax_edge over the final additions (bfs.hpp:91-96, 332-339).
+__global__ void k_inc_relax_additions(IncArgs a, const uint32_t* fin, uint64_t nfin,
+                                      const unsigned long long* skeys, const uint8_t* act, uint32_t* fq,
+                                      unsigned long long* fp) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  unsigned long long reach = 0, relaxed = 0;
+  for (uint64_t j0 = base - lane_id(); j0 < nfin; j0 += stride) {
+    const uint64_t j = j0 + lane_id();
+    bool want = false;
+    uint32_t v = 0;
+    if (j < nfin) {
+      uint32_t p = fin[j];
+      if (act[p] >= kActNopAdd) {
+        uint32_t u = static_cast<uint32_t>(skeys[p] >> 32);
+        v = static_cast<uint32_t>(skeys[p]);
+        uint32_t du = a.dist[u];
+        if (du != kInf && du + 1 < a.dist[v]) {
+          uint32_t old = atomicMin(&a.dist[v], du + 1);
+          if (old > du + 1) {
+            want = true;
+            reach += old == kInf;
+            ++relaxed;
+          }
+        }
+      }
+    }
+    frontier_push(a, 0, fq, fp, v, want);
+  }
+  reach = wave_sum(reach);
+  relaxed = wave_sum(relaxed);
+  if (lane_id() == 0 && (reach || relaxed)) {
+    atomicAdd(&a.st->became_reachable, reach);
+    atomicAdd(&a.st->relaxed, relaxed);
+  }
+}
+
+// propagate_decreases (bfs.hpp:341-354) as one Bellman-Ford round.
+__global__ void k_inc_propagate(IncArgs a, int par, const uint32_t* fq_cur, const unsigned long long* fp_cur,
+                                uint32_t* fq_next, unsigned long long* fp_next) {
+  const unsigned long long qc = a.st->fq[par];
+  const uint32_t W = gridDim.x * kWavesPerBlock;
+  const uint32_t gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  unsigned long long reach = 0, relaxed = 0;
+  lbs_edges(fq_cur, fp_cur, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.out_row, gw, W,
+            [&](uint32_t, uint32_t u, uint2, uint32_t slot, bool valid) {
+              bool want = false;
+              uint32_t w = 0;
+              if (valid) {
+                w = a.out_arena[slot];
+                if (!(w & kTomb)) {
+                  uint32_t du = a.dist[u];
+                  if (du != kInf && du + 1 < a.dist[w]) {
+                    uint32_t old = atomicMin(&a.dist[w], du + 1);
+                    if (old > du + 1) {
+                      want = true;
+                      reach += old == kInf;
+                      ++relaxed;
+                    }
+                  }
+                } else {
+                  w = 0;
+                }
+              }
+              frontier_push(a, par ^ 1, fq_next, fp_next, w, want);
+            });
+  reach = wave_sum(reach);
+  relaxed = wave_sum(relaxed);
+  if (lane_id() == 0 && (reach || relaxed)) {
+    atomicAdd(&a.st->became_reachable, reach);
+    atomicAdd(&a.st->relaxed, relaxed);
+  }
+}
+
+__global__ void k_inc_cleanup(IncArgs a, const uint32_t* tall, uint64_t nt) {
+  for (uint64_t j = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; j < nt;
+       j += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t v = tall[j];
+    a.lost[v] = 0;
+    a.support[v] = 0;
+    a.flags[v] = 0;
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// ------------------------------------------------------------- host side
+
+static unsigned blocks_for(uint64_t items, unsigned per = kBfsBlock, unsigned cap = 4096) {
+  uint64_t b = (items + per - 1) / per;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return static_cast<unsigned>(b);
+}
+
+static unsigned lbs_blocks(unsigned long long edges) {
+  // about 256 edges per wave, at most 2048 workgroups
+  uint64_t waves = (edges + 255) / 256;
+  uint64_t b = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (b < 1) b = 1;
+  if (b > 2048) b = 2048;
+  return static_cast<unsigned>(b);
+}
+
+static void ensure_work(vgx_bfs* b) {
+  vgx_graph* g = b->g;
+  BfsWork& w = b->w;
+  if (w.cap >= g->ncap && w.cap != 0) return;
+  uint64_t c = g->ncap;
+  w.dist.grow_copy(c, b->dist_len, g->stream);
+  uint64_t words = (c + 63) / 64;
+  w.visited.ensure(words);
+  w.bm.ensure(3 * words);
+  for (int i = 0; i < 2; ++i) {
+    w.queue[i].ensure(c);
+    w.pre[i].ensure(c);
+  }
+  // incremental workspace: zero-initialised counters and flags
+  uint64_t old = w.lost.n ? w.cap : 0;
+  w.lost.grow_copy(c, old, g->stream);
+  w.support.grow_copy(c, old, g->stream);
+  w.flags.grow_copy(c, old, g->stream);
+  w.stamp.grow_copy(c, old, g->stream);
+  VGX_HIP(hipMemsetAsync(w.lost.p + old, 0, (c - old) * 4, g->stream));
+  VGX_HIP(hipMemsetAsync(w.support.p + old, 0, (c - old) * 4, g->stream));
+  VGX_HIP(hipMemsetAsync(w.flags.p + old, 0, (c - old) * 4, g->stream));
+  VGX_HIP(hipMemsetAsync(w.stamp.p + old, 0, (c - old) * 4, g->stream));
+  for (auto* l : {&w.tlist, &w.tflist, &w.alist, &w.aall, &w.tall, &w.slist}) l->ensure(c);
+  for (auto* l : {&w.tfpre, &w.apre, &w.spre}) l->ensure(c);
+  w.cap = c;
+}
+
+// Entries of dist beyond dist_len belong to vertices added after the last
+// recompute; the reference resizes with `unreachable` (bfs.hpp:73).
+static void extend_dist(vgx_bfs* b) {
+  vgx_graph* g = b->g;
+  ensure_work(b);
+  if (b->dist_len < g->n) {
+    k_fill_u32<<<blocks_for(g->n - b->dist_len), kBfsBlock, 0, g->stream>>>(b->w.dist.p + b->dist_len,
+                                                                          g->n - b->dist_len, kInf);
+    VGX_LAUNCH_CHECK();
+    b->dist_len = g->n;
+  }
+}
+
+static LevelArgs level_args(vgx_bfs* b) {
+  vgx_graph* g = b->g;
+  LevelArgs a;
+  a.row = g->adj[0].row.p;
+  a.arena = g->adj[0].arena.p;
+  a.in_row = g->in().row.p;
+  a.in_arena = g->in().arena.p;
+  a.dist = b->w.dist.p;
+  a.visited = b->w.visited.p;
+  a.bm = b->w.bm.p;
+  a.queue0 = b->w.queue[0].p;
+  a.queue1 = b->w.queue[1].p;
+  a.pre0 = b->w.pre[0].p;
+  a.pre1 = b->w.pre[1].p;
+  a.st = b->d_st;
+  a.n = static_cast<uint32_t>(g->n);
+  a.words = static_cast<uint32_t>((g->n + 63) / 64);
+  return a;
+}
+
+static void bfs_full(vgx_bfs* b) {
+  vgx_graph* g = b->g;
+  ensure_work(b);
+  hipStream_t s = g->stream;
+  const uint64_t n = g->n;
+  b->dist_len = n;
+  const uint64_t words = (n + 63) / 64;
+  if (n) {
+    VGX_HIP(hipMemsetAsync(b->w.dist.p, 0xFF, n * sizeof(uint32_t), s));
+    VGX_HIP(hipMemsetAsync(b->w.visited.p, 0, words * 8, s));
+    VGX_HIP(hipMemsetAsync(b->w.bm.p, 0, 3 * words * 8, s));
+  }
+  LevelArgs a = level_args(b);
+  k_bfs_setup<<<1, 1, 0, s>>>(a, b->source, g->adj[0].stored, static_cast<float>(b->tune.alpha),
+                              static_cast<float>(b->tune.beta));
+  VGX_LAUNCH_CHECK();
+  unsigned blocks = b->tune.blocks;
+  if (blocks == 0) {
+    uint64_t waves = std::max<uint64_t>(words, g->adj[0].stored / 512);
+    blocks = static_cast<unsigned>(std::min<uint64_t>(2048, std::max<uint64_t>(1, (waves + 3) / 4)));
+  }
+  uint32_t chunk = std::max<uint32_t>(1, b->tune.levels_per_sync);
+  uint32_t launched = 0;
+  for (;;) {
+    for (uint32_t i = 0; i < chunk; ++i) {
+      k_bfs_level<<<blocks, kBfsBlock, 0, s>>>(a);
+      VGX_LAUNCH_CHECK();
+    }
+    launched += chunk;
+    VGX_HIP(hipMemcpyAsync(b->h_st, b->d_st, sizeof(BfsState), hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    if (b->h_st->done) break;
+    if (launched > n + 2) throw Error(VGX_DEVICE_ERROR, "BFS did not terminate");
+    chunk = std::max<uint32_t>(2, chunk / 2);
+  }
+  b->reachable = b->h_st->reached;
+  b->levels = b->h_st->levels;
+  b->bu_levels = b->h_st->bu_levels;
+}
+
+static IncArgs inc_args(vgx_bfs* b) {
+  vgx_graph* g = b->g;
+  IncArgs a;
+  a.out_row = g->adj[0].row.p;
+  a.out_arena = g->adj[0].arena.p;
+  a.in_row = g->in().row.p;
+  a.in_arena = g->in().arena.p;
+  a.dist = b->w.dist.p;
+  a.lost = b->w.lost.p;
+  a.support = b->w.support.p;
+  a.flags = b->w.flags.p;
+  a.stamp = b->w.stamp.p;
+  a.st = b->d_inc;
+  a.epoch = b->epoch;
+  a.source = b->source;
+  return a;
+}
+
+static void next_epoch(vgx_bfs* b) {
+  b->epoch += 1;
+  if (b->epoch == 0) {
+    VGX_HIP(hipMemsetAsync(b->w.stamp.p, 0, b->w.cap * 4, b->g->stream));
+    b->epoch = 1;
+  }
+}
+
+static void inc_sync(vgx_bfs* b) {
+  VGX_HIP(hipMemcpyAsync(b->h_inc, b->d_inc, sizeof(IncState), hipMemcpyDeviceToHost, b->g->stream));
+  VGX_HIP(hipStreamSynchronize(b->g->stream));
+}
+
+static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k) {
+  b->last_candidates = 0;
+  b->last_affected = 0;
+  b->last_relaxed = 0;
+  b->last_full = false;
+  if (k == 0) return;
+  vgx_graph* g = b->g;
+  hipStream_t s = g->stream;
+  const uint64_t n_pre = g->n;
+  uint64_t limit = static_cast<uint64_t>(static_cast<double>(n_pre) * b->fallback);
+  if (limit < 1) limit = 1;
+
+  graph_stage(g, src, dst, add, k);  // may grow vertices
+  extend_dist(b);
+  BfsWork& w = b->w;
+  VGX_HIP(hipMemsetAsync(b->d_inc, 0, sizeof(IncState), s));
+  graph_apply_deletions(g);
+  BatchScratch& bs = g->bs[0];
+
+  // ---- invalidation cascade on the graph without the deleted arcs
+  next_epoch(b);
+  IncArgs a = inc_args(b);
+  bool fallback = false;
+  if (bs.nfinal) {
+    k_inc_candidates<<<blocks_for(bs.nfinal), kBfsBlock, 0, s>>>(a, bs.fin.p, bs.nfinal, bs.skeys.p, bs.act.p,
+                                                                 w.tlist.p, w.tall.p, w.tflist.p, w.tfpre.p);
+    VGX_LAUNCH_CHECK();
+  }
+  inc_sync(b);
+  b->last_candidates = b->h_inc->candidates;
+  uint64_t nt = b->h_inc->tq;
+  unsigned long long tfq = b->h_inc->tfq;
+  while (nt) {
+    if (tfq) {
+      k_inc_support<<<lbs_blocks(tfq & kEdgeMask), kBfsBlock, 0, s>>>(a, w.tflist.p, w.tfpre.p);
+      VGX_LAUNCH_CHECK();
+    }
+    k_inc_evaluate<<<blocks_for(nt), kBfsBlock, 0, s>>>(a, w.tlist.p, nt, w.alist.p, w.apre.p, w.aall.p);
+    VGX_LAUNCH_CHECK();
+    inc_sync(b);
+    if (b->h_inc->aall > limit) {
+      fallback = true;
+      break;
+    }
+    unsigned long long aq = b->h_inc->aq;
+    if (aq == 0) break;
+    // next round
+    next_epoch(b);
+    a = inc_args(b);
+    unsigned long long zero3[3] = {0, 0, 0};
+    VGX_HIP(hipMemcpyAsync(&b->d_inc->tq, zero3, 2 * sizeof(unsigned long long), hipMemcpyHostToDevice, s));
+    k_inc_emit<<<lbs_blocks(aq & kEdgeMask), kBfsBlock, 0, s>>>(a, w.alist.p, w.apre.p, w.tlist.p, w.tall.p,
+                                                                w.tflist.p, w.tfpre.p);
+    VGX_LAUNCH_CHECK();
+    VGX_HIP(hipMemsetAsync(&b->d_inc->aq, 0, sizeof(unsigned long long), s));
+    inc_sync(b);
+    nt = b->h_inc->tq;
+    tfq = b->h_inc->tfq;
+  }
+  const uint64_t n_aff = b->h_inc->aall;
+  b->last_affected = n_aff;
+
+  // ---- apply the additions; the graph is now the post-batch graph
+  graph_apply_additions(g);
+  graph_finish_batch(g);  // may compact: row pointers change, refresh args
+  extend_dist(b);
+  a = inc_args(b);
+  if (fallback) {
+    inc_sync(b);
+    if (b->h_inc->tall) {
+      k_inc_cleanup<<<blocks_for(b->h_inc->tall), kBfsBlock, 0, s>>>(a, w.tall.p, b->h_inc->tall);
+      VGX_LAUNCH_CHECK();
+    }
+    bfs_full(b);
+    b->last_full = true;
+    return;
+  }
+
+  // ---- re-relaxation
+  next_epoch(b);
+  a = inc_args(b);
+  if (n_aff) {
+    k_inc_reset_affected<<<blocks_for(n_aff), kBfsBlock, 0, s>>>(a, w.aall.p, n_aff, w.slist.p, w.spre.p);
+    VGX_LAUNCH_CHECK();
+    inc_sync(b);
+    unsigned long long sq = b->h_inc->sq;
+    if (sq) {
+      k_inc_seed<<<lbs_blocks(sq & kEdgeMask), kBfsBlock, 0, s>>>(a, w.slist.p, w.spre.p);
+      VGX_LAUNCH_CHECK();
+    }
+    k_inc_collect_seeds<<<blocks_for(n_aff), kBfsBlock, 0, s>>>(a, w.aall.p, n_aff, w.queue[0].p, w.pre[0].p);
+    VGX_LAUNCH_CHECK();
+  }
+  // The staged batch of the out-direction still lists every final arc.
+  if (bs.nfinal) {
+    k_inc_relax_additions<<<blocks_for(bs.nfinal), kBfsBlock, 0, s>>>(a, bs.fin.p, bs.nfinal, bs.skeys.p,
+                                                                      bs.act.p, w.queue[0].p, w.pre[0].p);
+    VGX_LAUNCH_CHECK();
+  }
+  inc_sync(b);
+  int par = 0;
+  unsigned long long fq = b->h_inc->fq[0];
+  while (fq) {
+    next_epoch(b);
+    a = inc_args(b);
+    VGX_HIP(hipMemsetAsync(&b->d_inc->fq[par ^ 1], 0, sizeof(unsigned long long), s));
+    k_inc_propagate<<<lbs_blocks(fq & kEdgeMask), kBfsBlock, 0, s>>>(a, par, w.queue[par].p, w.pre[par].p,
+                                                                     w.queue[par ^ 1].p, w.pre[par ^ 1].p);
+    VGX_LAUNCH_CHECK();
+    inc_sync(b);
+    par ^= 1;
+    fq = b->h_inc->fq[par];
+  }
+  if (b->h_inc->tall) {
+    k_inc_cleanup<<<blocks_for(b->h_inc->tall), kBfsBlock, 0, s>>>(a, w.tall.p, b->h_inc->tall);
+    VGX_LAUNCH_CHECK();
+  }
+  b->reachable = b->reachable - n_aff + b->h_inc->became_reachable;
+  b->last_relaxed = b->h_inc->relaxed;
+  VGX_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace vgx
+
+using namespace vgx;
+
+#define VGX_API_BEGIN try {
+#define VGX_API_END                                                   \
+  }                                                                   \
+  catch (const vgx::Error& e) {                                       \
+    return vgx::set_error(e.code, e.what());                          \
+  }                                                                   \
+  catch (const std::bad_alloc&) {                                     \
+    return vgx::set_error(VGX_OUT_OF_MEMORY, "host allocation failed"); \
+  }                                                                   \
+  catch (const std::exception& e) {                                   \
+    return vgx::set_error(VGX_INVALID_ARGUMENT, e.what());            \
+  }                                                                   \
+  vgx::t_last_error.clear();                                          \
+  return VGX_OK;
+
+extern "C" {
+
+vgx_status vgx_bfs_create(vgx_graph* g, uint32_t source, double deletion_fallback_fraction, vgx_bfs** out) {
+  VGX_API_BEGIN
+  if (!g || !out) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  VGX_HIP(hipSetDevice(g->device));
+  auto* b = new vgx_bfs();
+  try {
+    b->g = g;
+    b->source = source;
+    b->fallback = deletion_fallback_fraction;
+    b->d_st = static_cast<BfsState*>(dev_alloc(sizeof(BfsState)));
+    b->d_inc = static_cast<IncState*>(dev_alloc(sizeof(IncState)));
+    VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_st), sizeof(BfsState), hipHostMallocDefault));
+    VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_inc), sizeof(IncState), hipHostMallocDefault));
+    std::memset(b->h_inc, 0, sizeof(IncState));
+    bfs_full(b);
+  } catch (...) {
+    vgx_bfs_destroy(b);
+    throw;
+  }
+  *out = b;
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_destroy(vgx_bfs* b) {
+  VGX_API_BEGIN
+  if (!b) return VGX_OK;
+  if (b->g) (void)hipStreamSynchronize(b->g->stream);
+  if (b->d_st) dev_free(b->d_st);
+  if (b->d_inc) dev_free(b->d_inc);
+  if (b->h_st) (void)hipHostFree(b->h_st);
+  if (b->h_inc) (void)hipHostFree(b->h_inc);
+  delete b;
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_recompute(vgx_bfs* b) {
+  VGX_API_BEGIN
+  if (!b) throw Error(VGX_INVALID_ARGUMENT, "null bfs");
+  VGX_HIP(hipSetDevice(b->g->device));
+  bfs_full(b);
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k) {
+  VGX_API_BEGIN
+  if (!b || (k && (!src || !dst || !add))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  VGX_HIP(hipSetDevice(b->g->device));
+  bfs_apply(b, src, dst, add, k);
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_distances(vgx_bfs* b, uint32_t* out, uint64_t cap, uint64_t* count) {
+  VGX_API_BEGIN
+  if (!b || !count) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  VGX_HIP(hipSetDevice(b->g->device));
+  *count = b->dist_len;
+  uint64_t c = std::min<uint64_t>(cap, b->dist_len);
+  if (out && c) {
+    VGX_HIP(hipMemcpyAsync(out, b->w.dist.p, c * sizeof(uint32_t), hipMemcpyDeviceToHost, b->g->stream));
+    VGX_HIP(hipStreamSynchronize(b->g->stream));
+  }
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_distances_device(vgx_bfs* b, const uint32_t** dist_dev, uint64_t* count) {
+  VGX_API_BEGIN
+  if (!b || !dist_dev) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  *dist_dev = b->w.dist.p;
+  if (count) *count = b->dist_len;
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_stats_get(vgx_bfs* b, vgx_bfs_stats* out) {
+  VGX_API_BEGIN
+  if (!b || !out) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  out->reachable_count = b->reachable;
+  out->last_deletion_candidates = b->last_candidates;
+  out->last_affected_vertices = b->last_affected;
+  out->last_used_full_recompute = b->last_full ? 1 : 0;
+  out->levels = b->levels;
+  out->bottom_up_levels = b->bu_levels;
+  out->last_relaxed_vertices = b->last_relaxed;
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_set_tuning(vgx_bfs* b, const vgx_bfs_tuning* t) {
+  VGX_API_BEGIN
+  if (!b || !t) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  if (!(t->alpha > 0) || !(t->beta > 0)) throw Error(VGX_INVALID_ARGUMENT, "alpha and beta must be positive");
+  if (t->blocks > 65535) throw Error(VGX_INVALID_ARGUMENT, "too many workgroups");
+  b->tune = *t;
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_get_tuning(vgx_bfs* b, vgx_bfs_tuning* t) {
+  VGX_API_BEGIN
+  if (!b || !t) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  *t = b->tune;
+  VGX_API_END
+}
+
+}  // extern "C"

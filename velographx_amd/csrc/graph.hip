@@ -1,0 +1,996 @@
+// HBM-resident DynamicGraph for gfx950.
+//
+// Reference: include/velographx/storage/dynamic_graph.hpp (SegmentedCsr
+// :33-219, PackedDeltaStore :221-446, CompactRowPatches :448-473,
+// DynamicGraph :479-816) and storage/consolidation.hpp:132-163.
+//
+// Layout (per direction; the in-direction is shared when undirected):
+//   row[u] = {beg, len}  one 8-byte load gives a row's arena range
+//   cap[u]               slots reserved for the row (len <= cap)
+//   deg[u]               live degree (neighbor_count)
+//   arena[]              vertex ids, each row sorted ascending; a deleted
+//                        entry keeps its place with bit 31 set (tombstone)
+// The base CSR of the reference is the initial packed arena; its delta
+// overlay becomes in-place tombstones plus inserts merged into the row's
+// slack; a row that outgrows its slack is rewritten at the arena's bump
+// pointer (the role of a row patch).  Compaction rewrites every row packed
+// without tombstones, which is the consolidation snapshot.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "vgx_internal.hpp"
+
+namespace vgx {
+
+thread_local std::string t_last_error;
+
+vgx_status set_error(vgx_status s, const std::string& msg) {
+  t_last_error = msg;
+  return s;
+}
+
+void throw_hip(hipError_t e, const char* what, const char* file, int line) {
+  std::string m = std::string(what) + " failed: " + hipGetErrorString(e) + " (" + file + ":" +
+                  std::to_string(line) + ")";
+  if (e == hipErrorOutOfMemory) throw Error(VGX_OUT_OF_MEMORY, m);
+  throw Error(VGX_DEVICE_ERROR, m);
+}
+
+void* dev_alloc(size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(VGX_OUT_OF_MEMORY, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+  }
+  return p;
+}
+
+void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+HostBuf::~HostBuf() {
+  if (p) (void)hipHostFree(p);
+}
+
+void HostBuf::ensure(size_t b) {
+  if (b <= bytes) return;
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  size_t c = b < 4096 ? 4096 : b + b / 4;
+  VGX_HIP(hipHostMalloc(&p, c, hipHostMallocDefault));
+  bytes = c;
+}
+
+static inline uint32_t slack_for(uint32_t len) { return len == 0 ? 0u : (len >> 3) + 2u; }
+static inline uint32_t relocated_cap(uint32_t need) { return need + (need >> 2) + 4u; }
+
+static int bits_for(uint64_t n) {
+  int b = 1;
+  while (b < 32 && (1ull << b) < n) ++b;
+  return b;
+}
+
+constexpr uint32_t kBigRow = 4096;
+constexpr int kBlock = 256;
+
+static unsigned grid_for(uint64_t items, unsigned per_block = kBlock, unsigned cap = 65535u * 4u) {
+  uint64_t g = (items + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}
+
+// ------------------------------------------------------------------ kernels
+
+__global__ void k_expand_keys(const uint32_t* __restrict__ s, const uint32_t* __restrict__ d, uint64_t m,
+                              int undirected, int swap, unsigned long long* __restrict__ keys,
+                              uint32_t* __restrict__ vals) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < m;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t u = s[i], v = d[i];
+    if (swap) {
+      uint32_t t = u;
+      u = v;
+      v = t;
+    }
+    bool loop = u == v;
+    // Both arcs of an undirected op sit next to each other so that the
+    // (stable) radix sort keeps every arc's ops in batch order.
+    const uint64_t o = undirected ? 2 * i : i;
+    keys[o] = loop ? kInvalidKey : ((static_cast<unsigned long long>(u) << 32) | v);
+    if (vals) vals[o] = static_cast<uint32_t>(i);
+    if (undirected) {
+      keys[o + 1] = loop ? kInvalidKey : ((static_cast<unsigned long long>(v) << 32) | u);
+      if (vals) vals[o + 1] = static_cast<uint32_t>(i);
+    }
+  }
+}
+
+// Row bounds of a sorted, unique key array: first index and one-past-last.
+__global__ void k_row_bounds(const unsigned long long* __restrict__ keys, uint64_t M, uint32_t* __restrict__ rstart,
+                             uint32_t* __restrict__ rend) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < M;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    unsigned long long k = keys[i];
+    if (k == kInvalidKey) continue;
+    uint32_t u = static_cast<uint32_t>(k >> 32);
+    if (i == 0 || static_cast<uint32_t>(keys[i - 1] >> 32) != u) rstart[u] = static_cast<uint32_t>(i);
+    if (i + 1 == M || keys[i + 1] == kInvalidKey || static_cast<uint32_t>(keys[i + 1] >> 32) != u)
+      rend[u] = static_cast<uint32_t>(i + 1);
+  }
+}
+
+__global__ void k_row_caps(const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ rend, uint64_t n,
+                           uint32_t* __restrict__ cap, uint32_t* __restrict__ deg, uint2* __restrict__ row) {
+  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; u < n;
+       u += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t len = rend[u] - rstart[u];
+    cap[u] = len == 0 ? 0u : len + (len >> 3) + 2u;
+    deg[u] = len;
+    row[u].y = len;
+  }
+}
+
+__global__ void k_row_begin(const uint32_t* __restrict__ beg, uint64_t n, uint2* __restrict__ row) {
+  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; u < n;
+       u += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    row[u].x = beg[u];
+}
+
+__global__ void k_scatter(const unsigned long long* __restrict__ keys, uint64_t M, const uint32_t* __restrict__ rstart,
+                          const uint2* __restrict__ row, uint32_t* __restrict__ arena) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < M;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    unsigned long long k = keys[i];
+    if (k == kInvalidKey) continue;
+    uint32_t u = static_cast<uint32_t>(k >> 32);
+    arena[row[u].x + (static_cast<uint32_t>(i) - rstart[u])] = static_cast<uint32_t>(k);
+  }
+}
+
+__device__ __forceinline__ uint32_t row_lower_bound(const uint32_t* arena, uint2 rb, uint32_t v) {
+  uint32_t lo = 0, hi = rb.y;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if ((arena[rb.x + mid] & kIdMask) < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Decide the action of every final arc op (the last op on its arc wins).
+__global__ void k_resolve(const unsigned long long* __restrict__ skeys, const uint32_t* __restrict__ svals,
+                          uint64_t M, const uint8_t* __restrict__ add, const uint2* __restrict__ row,
+                          const uint32_t* __restrict__ arena, uint64_t n, uint8_t* __restrict__ act,
+                          uint32_t* __restrict__ pos, uint8_t* __restrict__ fflag, uint8_t* __restrict__ iflag) {
+  for (uint64_t p = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p < M;
+       p += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    unsigned long long key = skeys[p];
+    bool fin = key != kInvalidKey && (p + 1 == M || skeys[p + 1] != key);
+    uint8_t f = 0, ins = 0;
+    if (fin) {
+      uint32_t u = static_cast<uint32_t>(key >> 32), v = static_cast<uint32_t>(key);
+      bool want = add[svals[p]] != 0;
+      uint2 rb = u < n ? row[u] : make_uint2(0u, 0u);
+      uint32_t i = row_lower_bound(arena, rb, v);
+      bool found = i < rb.y && (arena[rb.x + i] & kIdMask) == v;
+      bool tomb = found && (arena[rb.x + i] & kTomb);
+      uint8_t a;
+      if (!want) a = (found && !tomb) ? kActDel : kActNopDel;
+      else if (found) a = tomb ? kActUndel : kActNopAdd;
+      else a = kActIns;
+      act[p] = a;
+      pos[p] = found ? rb.x + i : 0u;
+      f = 1;
+      ins = a == kActIns;
+    }
+    fflag[p] = f;
+    iflag[p] = ins;
+  }
+}
+
+// Per insert-run planning: segment starts record the run length and the
+// arena slots a relocation will take.
+__global__ void k_plan_inserts(const uint32_t* __restrict__ ins, const unsigned long long* __restrict__ nins_p,
+                               const unsigned long long* __restrict__ skeys, const uint2* __restrict__ row,
+                               const uint32_t* __restrict__ cap, uint32_t* __restrict__ segk,
+                               uint32_t* __restrict__ big, GraphCounters* ctr, int dir) {
+  const uint64_t nins = *nins_p;
+  for (uint64_t j = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; j < nins;
+       j += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t u = static_cast<uint32_t>(skeys[ins[j]] >> 32);
+    if (j > 0 && static_cast<uint32_t>(skeys[ins[j - 1]] >> 32) == u) {
+      segk[j] = 0;
+      continue;
+    }
+    uint64_t e = j + 1;
+    while (e < nins && static_cast<uint32_t>(skeys[ins[e]] >> 32) == u) ++e;
+    uint32_t k = static_cast<uint32_t>(e - j);
+    segk[j] = k;
+    uint32_t len = row[u].y, c = cap[u];
+    uint32_t need = len + k;
+    bool is_big = need > kBigRow;
+    if (need > c || is_big) {
+      uint32_t nc = need + (need >> 2) + 4u;
+      atomicAdd(&ctr->need[dir], static_cast<unsigned long long>(nc));
+    }
+    if (is_big) {
+      unsigned long long slot = atomicAdd(&ctr->nbig[dir], 1ull);
+      big[slot] = static_cast<uint32_t>(j);
+    }
+  }
+}
+
+__device__ __forceinline__ void block_add(unsigned long long* dst, long long v) {
+  if (v != 0) atomicAdd(dst, static_cast<unsigned long long>(v));
+}
+
+__global__ void k_apply_flip(const uint32_t* __restrict__ fin, const unsigned long long* __restrict__ nfin_p,
+                             const unsigned long long* __restrict__ skeys, const uint8_t* __restrict__ act,
+                             const uint32_t* __restrict__ pos, uint32_t* __restrict__ arena,
+                             uint32_t* __restrict__ deg, GraphCounters* ctr, int dir, int undelete) {
+  const uint64_t nf = *nfin_p;
+  long long live = 0, tomb = 0;
+  for (uint64_t j = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; j < nf;
+       j += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t p = fin[j];
+    uint8_t a = act[p];
+    uint32_t u = static_cast<uint32_t>(skeys[p] >> 32);
+    if (!undelete && a == kActDel) {
+      arena[pos[p]] |= kTomb;
+      atomicSub(&deg[u], 1u);
+      --live;
+      ++tomb;
+    } else if (undelete && a == kActUndel) {
+      arena[pos[p]] &= kIdMask;
+      atomicAdd(&deg[u], 1u);
+      ++live;
+      --tomb;
+    }
+  }
+  live = wave_sum(live);
+  tomb = wave_sum(tomb);
+  if (lane_id() == 0) {
+    block_add(&ctr->live_delta[dir], live);
+    block_add(&ctr->tomb_delta[dir], tomb);
+  }
+}
+
+__global__ void k_insert_small(const uint32_t* __restrict__ ins, const unsigned long long* __restrict__ nins_p,
+                               const uint32_t* __restrict__ segk, const unsigned long long* __restrict__ skeys,
+                               uint2* __restrict__ row, uint32_t* __restrict__ cap, uint32_t* __restrict__ arena,
+                               uint32_t* __restrict__ deg, GraphCounters* ctr, int dir) {
+  const uint64_t nins = *nins_p;
+  long long added = 0, reloc = 0;
+  for (uint64_t j = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; j < nins;
+       j += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t k = segk[j];
+    if (k == 0) continue;
+    uint32_t u = static_cast<uint32_t>(skeys[ins[j]] >> 32);
+    uint2 rb = row[u];
+    uint32_t len = rb.y, c = cap[u];
+    if (len + k > kBigRow) continue;  // merged by k_insert_big
+    if (len + k <= c) {
+      int i = static_cast<int>(len) - 1, t = static_cast<int>(k) - 1, w = static_cast<int>(len + k) - 1;
+      while (t >= 0) {
+        uint32_t vt = static_cast<uint32_t>(skeys[ins[j + t]]);
+        if (i >= 0 && (arena[rb.x + i] & kIdMask) > vt) {
+          arena[rb.x + w] = arena[rb.x + i];
+          --i;
+        } else {
+          arena[rb.x + w] = vt;
+          --t;
+        }
+        --w;
+      }
+      row[u].y = len + k;
+    } else {
+      uint32_t need = len + k;
+      uint32_t nc = need + (need >> 2) + 4u;
+      uint32_t nb = static_cast<uint32_t>(atomicAdd(&ctr->bump[dir], static_cast<unsigned long long>(nc)));
+      uint32_t i = 0, t = 0, w = 0;
+      while (w < need) {
+        uint32_t vt = t < k ? static_cast<uint32_t>(skeys[ins[j + t]]) : kInf;
+        if (i < len && (arena[rb.x + i] & kIdMask) < vt) {
+          arena[nb + w] = arena[rb.x + i];
+          ++i;
+        } else {
+          arena[nb + w] = vt;
+          ++t;
+        }
+        ++w;
+      }
+      row[u] = make_uint2(nb, need);
+      cap[u] = nc;
+      ++reloc;
+    }
+    atomicAdd(&deg[u], k);
+    added += k;
+  }
+  added = wave_sum(added);
+  reloc = wave_sum(reloc);
+  if (lane_id() == 0) {
+    block_add(&ctr->live_delta[dir], added);
+    block_add(&ctr->stored_delta[dir], added);
+    block_add(&ctr->relocated[dir], reloc);
+  }
+}
+
+// One workgroup per large row: merge-path merge of the old row and its
+// sorted inserts into a fresh arena range.
+__global__ void k_insert_big(const uint32_t* __restrict__ big, const uint32_t* __restrict__ ins,
+                             const uint32_t* __restrict__ segk, const unsigned long long* __restrict__ skeys,
+                             uint2* __restrict__ row, uint32_t* __restrict__ cap, uint32_t* __restrict__ arena,
+                             uint32_t* __restrict__ deg, GraphCounters* ctr, int dir) {
+  __shared__ uint32_t s_nb;
+  const uint32_t j = big[blockIdx.x];
+  const uint32_t k = segk[j];
+  const uint32_t u = static_cast<uint32_t>(skeys[ins[j]] >> 32);
+  const uint2 rb = row[u];
+  const uint32_t len = rb.y, need = len + k;
+  const uint32_t nc = need + (need >> 2) + 4u;
+  if (threadIdx.x == 0)
+    s_nb = static_cast<uint32_t>(atomicAdd(&ctr->bump[dir], static_cast<unsigned long long>(nc)));
+  __syncthreads();
+  const uint32_t nb = s_nb;
+  const uint32_t per = (need + blockDim.x - 1) / blockDim.x;
+  uint32_t o0 = threadIdx.x * per;
+  uint32_t o1 = o0 + per < need ? o0 + per : need;
+  if (o0 < need) {
+    // merge path split: number of old-row entries among the first o0 outputs
+    uint32_t lo = o0 > k ? o0 - k : 0u, hi = o0 < len ? o0 : len;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      uint32_t a = arena[rb.x + mid] & kIdMask;
+      uint32_t b = static_cast<uint32_t>(skeys[ins[j + (o0 - mid - 1)]]);
+      if (a < b) lo = mid + 1;
+      else hi = mid;
+    }
+    uint32_t i = lo, t = o0 - lo;
+    for (uint32_t w = o0; w < o1; ++w) {
+      uint32_t vt = t < k ? static_cast<uint32_t>(skeys[ins[j + t]]) : kInf;
+      if (i < len && (arena[rb.x + i] & kIdMask) < vt) {
+        arena[nb + w] = arena[rb.x + i];
+        ++i;
+      } else {
+        arena[nb + w] = vt;
+        ++t;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    row[u] = make_uint2(nb, need);
+    cap[u] = nc;
+    atomicAdd(&deg[u], k);
+    atomicAdd(&ctr->live_delta[dir], static_cast<unsigned long long>(k));
+    atomicAdd(&ctr->stored_delta[dir], static_cast<unsigned long long>(k));
+    atomicAdd(&ctr->relocated[dir], 1ull);
+  }
+}
+
+// Copy the live entries of every row to out + off[u] (order preserved);
+// one wave per row.  Optionally writes the new row table for compaction.
+__global__ void k_pack_rows(const uint2* __restrict__ row, const uint32_t* __restrict__ arena, uint64_t n,
+                            const uint32_t* __restrict__ off, uint32_t* __restrict__ out,
+                            uint2* __restrict__ new_row, const uint32_t* __restrict__ deg) {
+  const uint32_t W = gridDim.x * (blockDim.x / kWave);
+  const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  for (uint64_t u = gw; u < n; u += W) {
+    uint2 rb = row[u];
+    uint32_t o = off[u];
+    for (uint32_t i = 0; i < rb.y; i += kWave) {
+      bool in = i + lane < rb.y;
+      uint32_t x = in ? arena[rb.x + i + lane] : kTomb;
+      bool keep = !(x & kTomb);
+      unsigned long long b = __ballot(keep);
+      if (keep) out[o + lanes_below(b)] = x;
+      o += __popcll(b);
+    }
+    if (new_row != nullptr && lane == 0) new_row[u] = make_uint2(off[u], deg[u]);
+  }
+}
+
+__global__ void k_compact_caps(const uint32_t* __restrict__ deg, uint64_t n, uint32_t* __restrict__ cap) {
+  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; u < n;
+       u += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t d = deg[u];
+    cap[u] = d == 0 ? 0u : d + (d >> 3) + 2u;
+  }
+}
+
+// ------------------------------------------------------------ host helpers
+
+struct DeviceGuard {
+  explicit DeviceGuard(int dev) { VGX_HIP(hipSetDevice(dev)); }
+};
+
+template <class F>
+static void with_temp(vgx_graph* g, DevBuf<uint8_t>& tmp, F&& f) {
+  size_t bytes = 0;
+  VGX_HIP(f(static_cast<void*>(nullptr), bytes));
+  tmp.ensure(bytes + 16);
+  VGX_HIP(f(static_cast<void*>(tmp.p), bytes));
+  (void)g;
+}
+
+void graph_sync_counters(vgx_graph* g) {
+  VGX_HIP(hipMemcpyAsync(g->h_ctr, g->d_ctr, sizeof(GraphCounters), hipMemcpyDeviceToHost, g->stream));
+  VGX_HIP(hipStreamSynchronize(g->stream));
+}
+
+static void reset_deltas(vgx_graph* g) {
+  // The bump pointers survive; every other counter restarts at zero.
+  VGX_HIP(hipMemsetAsync(&g->d_ctr->need, 0, sizeof(GraphCounters) - offsetof(GraphCounters, need), g->stream));
+}
+
+void graph_grow_vertices(vgx_graph* g, uint64_t n_new) {
+  if (n_new <= g->n) return;
+  if (n_new > (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "vertex ids must be below 2^31");
+  if (n_new > g->ncap) {
+    uint64_t c = std::max<uint64_t>(n_new, g->ncap * 2);
+    c = std::max<uint64_t>(c, 64);
+    for (int d = 0; d < g->ndir(); ++d) {
+      Adjacency& a = g->adj[d];
+      a.row.grow_copy(c, g->ncap, g->stream);
+      a.cap.grow_copy(c, g->ncap, g->stream);
+      a.deg.grow_copy(c, g->ncap, g->stream);
+      VGX_HIP(hipMemsetAsync(a.row.p + g->ncap, 0, (c - g->ncap) * sizeof(uint2), g->stream));
+      VGX_HIP(hipMemsetAsync(a.cap.p + g->ncap, 0, (c - g->ncap) * sizeof(uint32_t), g->stream));
+      VGX_HIP(hipMemsetAsync(a.deg.p + g->ncap, 0, (c - g->ncap) * sizeof(uint32_t), g->stream));
+    }
+    g->ncap = c;
+  }
+  g->n = n_new;
+}
+
+static void ensure_arena(vgx_graph* g, int d, uint64_t extra) {
+  Adjacency& a = g->adj[d];
+  if (a.used + extra <= a.acap) return;
+  uint64_t want = std::max<uint64_t>(a.acap + a.acap / 2, a.used + extra + (1u << 16));
+  if (want >= (1ull << 32)) throw Error(VGX_OUT_OF_MEMORY, "row arena exceeds 2^32 entries");
+  a.arena.grow_copy(want, a.used, g->stream);
+  a.acap = want;
+}
+
+// Canonical build from an arc-key array (already mirrored/swapped).
+static void build_direction(vgx_graph* g, int d, DevBuf<unsigned long long>& keys, DevBuf<unsigned long long>& skeys,
+                            uint64_t M) {
+  Adjacency& a = g->adj[d];
+  hipStream_t s = g->stream;
+  const uint64_t n = g->n;
+  int end_bit = 32 + bits_for(n);
+  if (M) {
+    with_temp(g, g->bs[0].tmp, [&](void* t, size_t& b) {
+      return hipcub::DeviceRadixSort::SortKeys(t, b, keys.p, skeys.p, static_cast<int>(M), 0, end_bit, s);
+    });
+  }
+  // unique (invalid keys sort last and collapse to one entry)
+  DevBuf<unsigned long long> nsel;
+  nsel.ensure(1);
+  if (M) {
+    with_temp(g, g->bs[0].tmp, [&](void* t, size_t& b) {
+      return hipcub::DeviceSelect::Unique(t, b, skeys.p, keys.p, nsel.p, static_cast<int>(M), s);
+    });
+  }
+  unsigned long long U = 0;
+  if (M) {
+    VGX_HIP(hipMemcpyAsync(&U, nsel.p, sizeof(U), hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    unsigned long long last = 0;
+    if (U) {
+      VGX_HIP(hipMemcpyAsync(&last, keys.p + U - 1, sizeof(last), hipMemcpyDeviceToHost, s));
+      VGX_HIP(hipStreamSynchronize(s));
+      if (last == kInvalidKey) --U;
+    }
+  }
+  if (U >= (1ull << 32)) throw Error(VGX_OUT_OF_MEMORY, "more than 2^32 arcs per direction");
+  DevBuf<uint32_t> rstart, rend, beg;
+  rstart.ensure(n + 1);
+  rend.ensure(n + 1);
+  beg.ensure(n + 1);
+  VGX_HIP(hipMemsetAsync(rstart.p, 0, (n + 1) * sizeof(uint32_t), s));
+  VGX_HIP(hipMemsetAsync(rend.p, 0, (n + 1) * sizeof(uint32_t), s));
+  if (U) {
+    k_row_bounds<<<grid_for(U), kBlock, 0, s>>>(keys.p, U, rstart.p, rend.p);
+    VGX_LAUNCH_CHECK();
+  }
+  if (n) {
+    k_row_caps<<<grid_for(n), kBlock, 0, s>>>(rstart.p, rend.p, n, a.cap.p, a.deg.p, a.row.p);
+    VGX_LAUNCH_CHECK();
+    with_temp(g, g->bs[0].tmp, [&](void* t, size_t& b) {
+      return hipcub::DeviceScan::InclusiveSum(t, b, a.cap.p, beg.p + 1, static_cast<int>(n), s);
+    });
+    VGX_HIP(hipMemsetAsync(beg.p, 0, sizeof(uint32_t), s));
+  }
+  uint32_t total = 0;
+  if (n) {
+    VGX_HIP(hipMemcpyAsync(&total, beg.p + n, sizeof(total), hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+  }
+  uint64_t acap = static_cast<uint64_t>(total) + static_cast<uint64_t>(total) / 4 + (1u << 16);
+  a.arena.release();
+  a.arena.ensure(acap);
+  a.acap = a.arena.n;
+  if (n) {
+    k_row_begin<<<grid_for(n), kBlock, 0, s>>>(beg.p, n, a.row.p);
+    VGX_LAUNCH_CHECK();
+  }
+  if (U) {
+    k_scatter<<<grid_for(U), kBlock, 0, s>>>(keys.p, U, rstart.p, a.row.p, a.arena.p);
+    VGX_LAUNCH_CHECK();
+  }
+  a.used = total;
+  a.stored = U;
+  a.live = U;
+  a.relocated = 0;
+  unsigned long long bump = total;
+  VGX_HIP(hipMemcpyAsync(&g->d_ctr->bump[d], &bump, sizeof(bump), hipMemcpyHostToDevice, s));
+  VGX_HIP(hipStreamSynchronize(s));
+}
+
+static void bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t* dst, uint64_t m) {
+  uint64_t maxid = 0;
+  bool any = false;
+  for (uint64_t i = 0; i < m; ++i) {
+    if (src[i] == dst[i]) continue;
+    any = true;
+    maxid = std::max<uint64_t>(maxid, std::max(src[i], dst[i]));
+  }
+  if (any) graph_grow_vertices(g, maxid + 1);
+  hipStream_t s = g->stream;
+  DevBuf<uint32_t> ds, dd;
+  ds.ensure(m + 1);
+  dd.ensure(m + 1);
+  if (m) {
+    VGX_HIP(hipMemcpyAsync(ds.p, src, m * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    VGX_HIP(hipMemcpyAsync(dd.p, dst, m * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  }
+  for (int d = 0; d < g->ndir(); ++d) {
+    const uint64_t M = g->directed ? m : 2 * m;
+    DevBuf<unsigned long long> keys, skeys;
+    keys.ensure(M + 1);
+    skeys.ensure(M + 1);
+    if (m) {
+      k_expand_keys<<<grid_for(m), kBlock, 0, s>>>(ds.p, dd.p, m, g->directed ? 0 : 1, d == 1 ? 1 : 0, keys.p,
+                                                    nullptr);
+      VGX_LAUNCH_CHECK();
+    }
+    build_direction(g, d, keys, skeys, M);
+  }
+  g->version += 1;
+}
+
+// ------------------------------------------------------------ batch apply
+
+void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k) {
+  if (k >= (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "update batch too large");
+  uint64_t maxid = 0;
+  bool any = false;
+  for (uint64_t i = 0; i < k; ++i) {
+    if (src[i] == dst[i]) continue;
+    any = true;
+    uint32_t mx = std::max(src[i], dst[i]);
+    if (mx > maxid) maxid = mx;
+  }
+  if (any && maxid >= (1ull << 31) - 1) throw Error(VGX_INVALID_ARGUMENT, "vertex ids must be below 2^31 - 1");
+  if (any) graph_grow_vertices(g, maxid + 1);
+  hipStream_t s = g->stream;
+  // one pinned staging copy: src | dst | add
+  g->stage.ensure(k * 9 + 64);
+  auto* hs = static_cast<uint32_t*>(g->stage.p);
+  std::memcpy(hs, src, k * 4);
+  std::memcpy(hs + k, dst, k * 4);
+  std::memcpy(reinterpret_cast<uint8_t*>(hs + 2 * k), add, k);
+  BatchScratch& b0 = g->bs[0];
+  b0.src.ensure(2 * k + 64);
+  b0.add.ensure(k + 64);
+  VGX_HIP(hipMemcpyAsync(b0.src.p, hs, 8 * k, hipMemcpyHostToDevice, s));
+  VGX_HIP(hipMemcpyAsync(b0.add.p, hs + 2 * k, k, hipMemcpyHostToDevice, s));
+  const uint32_t* d_src = b0.src.p;
+  const uint32_t* d_dst = b0.src.p + k;
+  reset_deltas(g);
+  const int end_bit = 32 + bits_for(g->n);
+  for (int d = 0; d < g->ndir(); ++d) {
+    BatchScratch& b = g->bs[d];
+    Adjacency& a = g->adj[d];
+    const uint64_t M = g->directed ? k : 2 * k;
+    b.k = k;
+    b.m = M;
+    b.keys.ensure(M + 1);
+    b.skeys.ensure(M + 1);
+    b.vals.ensure(M + 1);
+    b.svals.ensure(M + 1);
+    b.act.ensure(M + 1);
+    b.pos.ensure(M + 1);
+    b.fin.ensure(M + 1);
+    b.ins.ensure(M + 1);
+    b.segk.ensure(M + 1);
+    b.big.ensure(M + 1);
+    b.flags.ensure(2 * M + 2);
+    k_expand_keys<<<grid_for(k), kBlock, 0, s>>>(d_src, d_dst, k, g->directed ? 0 : 1, d == 1 ? 1 : 0, b.keys.p,
+                                                  b.vals.p);
+    VGX_LAUNCH_CHECK();
+    with_temp(g, b.tmp, [&](void* t, size_t& bytes) {
+      return hipcub::DeviceRadixSort::SortPairs(t, bytes, b.keys.p, b.skeys.p, b.vals.p, b.svals.p,
+                                                static_cast<int>(M), 0, end_bit, s);
+    });
+    uint8_t* fflag = b.flags.p;
+    uint8_t* iflag = b.flags.p + M + 1;
+    k_resolve<<<grid_for(M), kBlock, 0, s>>>(b.skeys.p, b.svals.p, M, b0.add.p, a.row.p, a.arena.p, g->n,
+                                             b.act.p, b.pos.p, fflag, iflag);
+    VGX_LAUNCH_CHECK();
+    hipcub::CountingInputIterator<uint32_t> it(0);
+    with_temp(g, b.tmp, [&](void* t, size_t& bytes) {
+      return hipcub::DeviceSelect::Flagged(t, bytes, it, fflag, b.fin.p, &g->d_ctr->nfinal[d],
+                                           static_cast<int>(M), s);
+    });
+    with_temp(g, b.tmp, [&](void* t, size_t& bytes) {
+      return hipcub::DeviceSelect::Flagged(t, bytes, it, iflag, b.ins.p, &g->d_ctr->nins[d],
+                                           static_cast<int>(M), s);
+    });
+    k_plan_inserts<<<grid_for(M), kBlock, 0, s>>>(b.ins.p, &g->d_ctr->nins[d], b.skeys.p, a.row.p, a.cap.p,
+                                                  b.segk.p, b.big.p, g->d_ctr, d);
+    VGX_LAUNCH_CHECK();
+  }
+  graph_sync_counters(g);
+  for (int d = 0; d < g->ndir(); ++d) {
+    BatchScratch& b = g->bs[d];
+    b.nfinal = g->h_ctr->nfinal[d];
+    b.nins = g->h_ctr->nins[d];
+    b.nbig = g->h_ctr->nbig[d];
+    ensure_arena(g, d, g->h_ctr->need[d]);
+  }
+}
+
+void graph_apply_deletions(vgx_graph* g) {
+  for (int d = 0; d < g->ndir(); ++d) {
+    BatchScratch& b = g->bs[d];
+    Adjacency& a = g->adj[d];
+    if (!b.nfinal) continue;
+    k_apply_flip<<<grid_for(b.nfinal), kBlock, 0, g->stream>>>(b.fin.p, &g->d_ctr->nfinal[d], b.skeys.p, b.act.p,
+                                                               b.pos.p, a.arena.p, a.deg.p, g->d_ctr, d, 0);
+    VGX_LAUNCH_CHECK();
+  }
+}
+
+void graph_apply_additions(vgx_graph* g) {
+  for (int d = 0; d < g->ndir(); ++d) {
+    BatchScratch& b = g->bs[d];
+    Adjacency& a = g->adj[d];
+    if (b.nfinal) {
+      k_apply_flip<<<grid_for(b.nfinal), kBlock, 0, g->stream>>>(b.fin.p, &g->d_ctr->nfinal[d], b.skeys.p,
+                                                                 b.act.p, b.pos.p, a.arena.p, a.deg.p, g->d_ctr, d,
+                                                                 1);
+      VGX_LAUNCH_CHECK();
+    }
+    if (b.nins) {
+      k_insert_small<<<grid_for(b.nins), kBlock, 0, g->stream>>>(b.ins.p, &g->d_ctr->nins[d], b.segk.p,
+                                                                 b.skeys.p, a.row.p, a.cap.p, a.arena.p, a.deg.p,
+                                                                 g->d_ctr, d);
+      VGX_LAUNCH_CHECK();
+    }
+    if (b.nbig) {
+      k_insert_big<<<static_cast<unsigned>(b.nbig), kBlock, 0, g->stream>>>(b.big.p, b.ins.p, b.segk.p, b.skeys.p,
+                                                                           a.row.p, a.cap.p, a.arena.p, a.deg.p,
+                                                                           g->d_ctr, d);
+      VGX_LAUNCH_CHECK();
+    }
+  }
+}
+
+static void compact_direction(vgx_graph* g, int d);
+
+void graph_finish_batch(vgx_graph* g) {
+  graph_sync_counters(g);
+  bool need_compact = false;
+  for (int d = 0; d < g->ndir(); ++d) {
+    Adjacency& a = g->adj[d];
+    a.live += static_cast<long long>(g->h_ctr->live_delta[d]);
+    a.stored += static_cast<long long>(g->h_ctr->stored_delta[d]);
+    a.relocated += g->h_ctr->relocated[d];
+    a.used = g->h_ctr->bump[d];
+    // Storage maintenance (the role of automatic_storage_maintenance,
+    // dynamic_graph.hpp:791-802): repack once tombstones or abandoned
+    // relocation space outweigh half of the live arcs.
+    uint64_t tomb = a.stored - a.live;
+    if (tomb > a.live / 2 + 4096 || a.used > 2 * (a.live + g->n) + (1u << 20)) need_compact = true;
+  }
+  g->version += 1;
+  if (need_compact)
+    for (int d = 0; d < g->ndir(); ++d) compact_direction(g, d);
+}
+
+static void graph_apply(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k) {
+  if (k == 0) return;
+  graph_stage(g, src, dst, add, k);
+  graph_apply_deletions(g);
+  graph_apply_additions(g);
+  graph_finish_batch(g);
+}
+
+// ------------------------------------------------------ snapshots, compaction
+
+static void pack_direction(vgx_graph* g, int d, DevBuf<uint32_t>& off, DevBuf<uint32_t>& out, uint2* new_row,
+                           const uint32_t* caps) {
+  Adjacency& a = g->adj[d];
+  hipStream_t s = g->stream;
+  const uint64_t n = g->n;
+  off.ensure(n + 2);
+  VGX_HIP(hipMemsetAsync(off.p, 0, sizeof(uint32_t), s));
+  if (n) {
+    with_temp(g, g->bs[0].tmp, [&](void* t, size_t& b) {
+      return hipcub::DeviceScan::InclusiveSum(t, b, caps ? caps : a.deg.p, off.p + 1, static_cast<int>(n), s);
+    });
+  }
+  uint32_t total = 0;
+  if (n) {
+    VGX_HIP(hipMemcpyAsync(&total, off.p + n, sizeof(total), hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+  }
+  uint64_t want = caps ? static_cast<uint64_t>(total) + total / 4 + (1u << 16) : static_cast<uint64_t>(total) + 1;
+  out.ensure(want);
+  if (n) {
+    k_pack_rows<<<grid_for(n, 4, 16384), kBlock, 0, s>>>(a.row.p, a.arena.p, n, off.p, out.p, new_row, a.deg.p);
+    VGX_LAUNCH_CHECK();
+  }
+  if (caps) {
+    a.used = total;
+  }
+}
+
+static void compact_direction(vgx_graph* g, int d) {
+  Adjacency& a = g->adj[d];
+  hipStream_t s = g->stream;
+  const uint64_t n = g->n;
+  if (n == 0) return;
+  k_compact_caps<<<grid_for(n), kBlock, 0, s>>>(a.deg.p, n, a.cap.p);
+  VGX_LAUNCH_CHECK();
+  DevBuf<uint32_t> off, out;
+  DevBuf<uint2> nrow;
+  nrow.ensure(g->ncap);
+  VGX_HIP(hipMemsetAsync(nrow.p, 0, g->ncap * sizeof(uint2), s));
+  pack_direction(g, d, off, out, nrow.p, a.cap.p);
+  VGX_HIP(hipStreamSynchronize(s));
+  a.arena.swap(out);
+  a.acap = a.arena.n;
+  a.row.swap(nrow);
+  a.stored = a.live;
+  a.relocated = 0;
+  unsigned long long bump = a.used;
+  VGX_HIP(hipMemcpyAsync(&g->d_ctr->bump[d], &bump, sizeof(bump), hipMemcpyHostToDevice, s));
+  VGX_HIP(hipStreamSynchronize(s));
+}
+
+const CsrSnapshot& graph_snapshot(vgx_graph* g, int dir) {
+  int d = g->directed ? dir : 0;
+  CsrSnapshot& sn = g->snap[d];
+  if (sn.version == g->version && sn.n == g->n) return sn;
+  pack_direction(g, d, sn.off, sn.adj, nullptr, nullptr);
+  VGX_HIP(hipStreamSynchronize(g->stream));
+  sn.version = g->version;
+  sn.n = g->n;
+  sn.arcs = g->adj[d].live;
+  return sn;
+}
+
+}  // namespace vgx
+
+using namespace vgx;
+
+#define VGX_API_BEGIN try {
+#define VGX_API_END                                                   \
+  }                                                                   \
+  catch (const vgx::Error& e) {                                       \
+    return vgx::set_error(e.code, e.what());                          \
+  }                                                                   \
+  catch (const std::bad_alloc&) {                                     \
+    return vgx::set_error(VGX_OUT_OF_MEMORY, "host allocation failed"); \
+  }                                                                   \
+  catch (const std::exception& e) {                                   \
+    return vgx::set_error(VGX_INVALID_ARGUMENT, e.what());            \
+  }                                                                   \
+  vgx::t_last_error.clear();                                          \
+  return VGX_OK;
+
+extern "C" {
+
+const char* vgx_last_error(void) { return vgx::t_last_error.c_str(); }
+
+const char* vgx_version(void) { return "velographx-amd 0.1.0 (gfx950)"; }
+
+int vgx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+vgx_status vgx_graph_create(uint64_t vertices, int directed, vgx_graph** out) {
+  VGX_API_BEGIN
+  if (!out) throw Error(VGX_INVALID_ARGUMENT, "out is null");
+  if (vertices > (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "vertex ids must be below 2^31");
+  if (vgx_device_count() == 0) throw Error(VGX_DEVICE_ERROR, "no HIP device visible");
+  auto* g = new vgx_graph();
+  try {
+    VGX_HIP(hipGetDevice(&g->device));
+    VGX_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    g->directed = directed != 0;
+    g->d_ctr = static_cast<GraphCounters*>(dev_alloc(sizeof(GraphCounters)));
+    VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->h_ctr), sizeof(GraphCounters), hipHostMallocDefault));
+    VGX_HIP(hipMemsetAsync(g->d_ctr, 0, sizeof(GraphCounters), g->stream));
+    std::memset(g->h_ctr, 0, sizeof(GraphCounters));
+    graph_grow_vertices(g, vertices);
+    for (int d = 0; d < g->ndir(); ++d) {
+      g->adj[d].arena.ensure(1u << 16);
+      g->adj[d].acap = g->adj[d].arena.n;
+    }
+    VGX_HIP(hipStreamSynchronize(g->stream));
+  } catch (...) {
+    vgx_graph_destroy(g);
+    throw;
+  }
+  *out = g;
+  VGX_API_END
+}
+
+vgx_status vgx_graph_destroy(vgx_graph* g) {
+  VGX_API_BEGIN
+  if (!g) return VGX_OK;
+  (void)hipSetDevice(g->device);
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  if (g->d_ctr) dev_free(g->d_ctr);
+  if (g->h_ctr) (void)hipHostFree(g->h_ctr);
+  hipStream_t s = g->stream;
+  delete g;
+  if (s) (void)hipStreamDestroy(s);
+  VGX_API_END
+}
+
+vgx_status vgx_graph_bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t* dst, uint64_t m) {
+  VGX_API_BEGIN
+  if (!g || (m && (!src || !dst))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  DeviceGuard dg(g->device);
+  bulk_load(g, src, dst, m);
+  VGX_API_END
+}
+
+vgx_status vgx_graph_apply(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k) {
+  VGX_API_BEGIN
+  if (!g || (k && (!src || !dst || !add))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  DeviceGuard dg(g->device);
+  graph_apply(g, src, dst, add, k);
+  VGX_API_END
+}
+
+vgx_status vgx_graph_info(const vgx_graph* g, uint64_t* vertex_count, uint64_t* version, int* directed,
+                          uint64_t* edge_count_directed) {
+  VGX_API_BEGIN
+  if (!g) throw Error(VGX_INVALID_ARGUMENT, "null graph");
+  if (vertex_count) *vertex_count = g->n;
+  if (version) *version = g->version;
+  if (directed) *directed = g->directed ? 1 : 0;
+  if (edge_count_directed) *edge_count_directed = g->adj[0].live;
+  VGX_API_END
+}
+
+static std::vector<uint32_t> host_row(vgx_graph* g, int d, uint32_t u) {
+  Adjacency& a = g->adj[d];
+  uint2 rb;
+  VGX_HIP(hipMemcpyAsync(&rb, a.row.p + u, sizeof(rb), hipMemcpyDeviceToHost, g->stream));
+  VGX_HIP(hipStreamSynchronize(g->stream));
+  std::vector<uint32_t> v(rb.y);
+  if (rb.y) {
+    VGX_HIP(hipMemcpyAsync(v.data(), a.arena.p + rb.x, rb.y * sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+    VGX_HIP(hipStreamSynchronize(g->stream));
+  }
+  return v;
+}
+
+vgx_status vgx_graph_neighbors(vgx_graph* g, uint32_t u, int in_direction, uint32_t* out, uint64_t cap,
+                               uint64_t* count) {
+  VGX_API_BEGIN
+  if (!g || !count) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  DeviceGuard dg(g->device);
+  *count = 0;
+  if (u >= g->n) return VGX_OK;  // rows beyond the graph are empty (dynamic_graph.hpp:180-181)
+  int d = (in_direction && g->directed) ? 1 : 0;
+  auto v = host_row(g, d, u);
+  uint64_t c = 0;
+  for (uint32_t x : v) {
+    if (x & kTomb) continue;
+    if (out && c < cap) out[c] = x;
+    ++c;
+  }
+  *count = c;
+  VGX_API_END
+}
+
+vgx_status vgx_graph_has_edge(vgx_graph* g, uint32_t u, uint32_t v, int* present) {
+  VGX_API_BEGIN
+  if (!g || !present) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  DeviceGuard dg(g->device);
+  *present = 0;
+  if (u >= g->n || v > kIdMask) return VGX_OK;
+  auto r = host_row(g, 0, u);
+  auto it = std::lower_bound(r.begin(), r.end(), v, [](uint32_t a, uint32_t b) { return (a & kIdMask) < b; });
+  *present = (it != r.end() && (*it & kIdMask) == v && !(*it & kTomb)) ? 1 : 0;
+  VGX_API_END
+}
+
+vgx_status vgx_graph_degrees(vgx_graph* g, int in_direction, uint32_t* out) {
+  VGX_API_BEGIN
+  if (!g || (!out && g->n)) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  DeviceGuard dg(g->device);
+  int d = (in_direction && g->directed) ? 1 : 0;
+  if (g->n) {
+    VGX_HIP(hipMemcpyAsync(out, g->adj[d].deg.p, g->n * sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+    VGX_HIP(hipStreamSynchronize(g->stream));
+  }
+  VGX_API_END
+}
+
+vgx_status vgx_graph_compact(vgx_graph* g) {
+  VGX_API_BEGIN
+  if (!g) throw Error(VGX_INVALID_ARGUMENT, "null graph");
+  DeviceGuard dg(g->device);
+  for (int d = 0; d < g->ndir(); ++d) compact_direction(g, d);
+  VGX_API_END
+}
+
+vgx_status vgx_graph_storage_info(vgx_graph* g, vgx_graph_storage* out) {
+  VGX_API_BEGIN
+  if (!g || !out) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  std::memset(out, 0, sizeof(*out));
+  for (int d = 0; d < g->ndir(); ++d) {
+    const Adjacency& a = g->adj[d];
+    out->arena_entries += a.acap;
+    out->arena_used += a.used;
+    out->tombstones += a.stored - a.live;
+    out->relocated_rows += a.relocated;
+    out->storage_bytes += a.acap * 4 + g->ncap * (sizeof(uint2) + 8);
+  }
+  VGX_API_END
+}
+
+vgx_status vgx_graph_export_csr(vgx_graph* g, int in_direction, uint64_t* off, uint32_t* adj) {
+  VGX_API_BEGIN
+  if (!g || !off) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  DeviceGuard dg(g->device);
+  const CsrSnapshot& sn = graph_snapshot(g, in_direction ? 1 : 0);
+  std::vector<uint32_t> o(sn.n + 1);
+  VGX_HIP(hipMemcpyAsync(o.data(), sn.off.p, (sn.n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+  VGX_HIP(hipStreamSynchronize(g->stream));
+  for (uint64_t i = 0; i <= sn.n; ++i) off[i] = o[i];
+  if (o[sn.n] && adj) {
+    VGX_HIP(hipMemcpyAsync(adj, sn.adj.p, static_cast<size_t>(o[sn.n]) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           g->stream));
+    VGX_HIP(hipStreamSynchronize(g->stream));
+  }
+  VGX_API_END
+}
+
+vgx_status vgx_graph_csr_device(vgx_graph* g, int in_direction, const uint32_t** off_dev, const uint32_t** adj_dev,
+                                uint64_t* vertices, uint64_t* arcs) {
+  VGX_API_BEGIN
+  if (!g) throw Error(VGX_INVALID_ARGUMENT, "null graph");
+  DeviceGuard dg(g->device);
+  const CsrSnapshot& sn = graph_snapshot(g, in_direction ? 1 : 0);
+  if (off_dev) *off_dev = sn.off.p;
+  if (adj_dev) *adj_dev = sn.adj.p;
+  if (vertices) *vertices = sn.n;
+  if (arcs) *arcs = sn.arcs;
+  VGX_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,982 @@
+// Pull PageRank and localized repair for gfx950.
+//
+// Reference: include/velographx/incremental/pagerank.hpp
+//   full_solve :277-321   next[v] = (1-d)/n + d*dangling/n + d*sum rank[u]/outdeg[u]
+//                         (dangling mass spread uniformly, L-inf stop)
+//   apply      :47-191    full recompute when n changed, when dangling vertices
+//                         existed, or when an endpoint became dangling;
+//                         otherwise Jacobi sweeps over the active set
+//   validate_against_full / apply_validated :204-258
+// plus the static push pagerank of src/algorithms.cpp:56-81 (L1 stop).
+//
+// One iteration is one launch over the packed in-CSR snapshot.  Vertices
+// are split by in-degree: rows of <= 32 arcs are summed by one lane, rows up
+// to 4096 by one wave, larger rows in 4096-arc chunks by several waves whose
+// partial sums the last workgroup combines.  The last workgroup also reduces
+// the per-workgroup residual and dangling partials in a fixed order, sets
+// the next iteration's dangling share and the convergence flag, so the host
+// only polls every few iterations.  FP64 reproduces the reference's
+// arithmetic; FP32 keeps ranks and sums in float with Kahan-compensated
+// lane sums.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "vgx_internal.hpp"
+
+namespace vgx {
+vgx_status set_error(vgx_status s, const std::string& msg);
+extern thread_local std::string t_last_error;
+
+constexpr int kPrBlock = 256;
+constexpr uint32_t kSmallRow = 32;
+constexpr uint32_t kChunk = 4096;
+
+struct PrState {
+  double dangling_share;  // d * dangling / n for the next iteration
+  double l1, linf;
+  unsigned long long iterations;
+  uint32_t done;
+  uint32_t blocks_done;
+  uint32_t stop_l1;  // 1: stop when l1 < tol, 0: when linf <= tol
+  uint32_t pad;
+  double tol;
+  double teleport;
+  double damping;
+  unsigned long long max_iterations;
+};
+
+struct PrPartial {
+  double l1, linf, dangling;
+};
+
+}  // namespace vgx
+
+struct vgx_pagerank {
+  vgx_graph* g = nullptr;
+  double damping = 0.85;
+  int precision = 64;
+  uint64_t n = 0;
+  bool have_values = false;
+  // per graph version
+  uint64_t lists_version = ~0ull;
+  vgx::DevBuf<uint32_t> slist, mlist, hlist;
+  vgx::DevBuf<uint32_t> hchunk_v, hchunk_off;  // chunk -> H index, chunk -> arc start
+  vgx::DevBuf<uint32_t> hchunk_first;          // H index -> first chunk (size nh+1)
+  uint64_t ns = 0, nm = 0, nh = 0, nchunks = 0;
+  vgx::DevBuf<uint8_t> rank[2], contrib[2], hpart, ref_rank[2];
+  vgx::DevBuf<vgx::PrPartial> partials;
+  vgx::PrState* d_st = nullptr;
+  vgx::PrState* h_st = nullptr;
+  int cur = 0;  // buffer holding the current ranks
+  vgx_pagerank_stats stats{};
+  // localized repair workspace
+  vgx::DevBuf<uint32_t> act_flag, ever_flag, alist[2], counters;
+  vgx::DevBuf<uint8_t> upd;
+};
+
+namespace vgx {
+
+template <class T>
+struct Acc;
+template <>
+struct Acc<double> {
+  double s = 0.0;
+  __device__ void add(double x) { s += x; }
+  __device__ double get() const { return s; }
+};
+template <>
+struct Acc<float> {
+  float s = 0.f, c = 0.f;
+  __device__ void add(float x) {
+    float y = x - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ float get() const { return s; }
+};
+
+struct PrArgs {
+  const uint32_t* off;
+  const uint32_t* adj;
+  const uint32_t* outdeg;
+  const uint32_t* slist;
+  const uint32_t* mlist;
+  const uint32_t* hlist;
+  const uint32_t* hchunk_v;
+  const uint32_t* hchunk_off;
+  const uint32_t* hchunk_first;
+  uint32_t ns, nm, nh, nchunks;
+  uint32_t bs, bm;  // workgroups for S and M (H uses the rest)
+  uint32_t n;
+  void* rank0;
+  void* rank1;
+  void* contrib0;
+  void* contrib1;
+  void* hpart;
+  PrPartial* partials;
+  PrState* st;
+};
+
+template <class T>
+__device__ __forceinline__ T finish_vertex(const PrArgs& a, uint32_t v, T incoming, T base, const T* rank, T* next,
+                                           T* cnext, double& l1, double& linf, double& dang) {
+  const T updated = base + static_cast<T>(a.st->damping) * incoming;
+  const T delta = updated > rank[v] ? updated - rank[v] : rank[v] - updated;
+  next[v] = updated;
+  const uint32_t od = a.outdeg[v];
+  cnext[v] = od ? updated / static_cast<T>(od) : T(0);
+  if (!od) dang += static_cast<double>(updated);
+  l1 += static_cast<double>(delta);
+  linf = linf > static_cast<double>(delta) ? linf : static_cast<double>(delta);
+  return updated;
+}
+
+template <class T>
+__device__ void block_reduce3(double& a, double& b, double& c, double* sm) {
+  a = wave_sum(a);
+  b = wave_max(b);
+  c = wave_sum(c);
+  const unsigned w = threadIdx.x / kWave;
+  if (lane_id() == 0) {
+    sm[w * 3 + 0] = a;
+    sm[w * 3 + 1] = b;
+    sm[w * 3 + 2] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (unsigned i = 1; i < blockDim.x / kWave; ++i) {
+      a += sm[i * 3 + 0];
+      b = b > sm[i * 3 + 1] ? b : sm[i * 3 + 1];
+      c += sm[i * 3 + 2];
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kPrBlock) void k_pr_iter(PrArgs a) {
+  __shared__ double sm[3 * (kPrBlock / kWave)];
+  __shared__ int s_last;
+  PrState* st = a.st;
+  if (st->done) return;
+  const int par = static_cast<int>(st->iterations & 1ull);
+  const T* rank = static_cast<const T*>(par ? a.rank1 : a.rank0);
+  T* next = static_cast<T*>(par ? a.rank0 : a.rank1);
+  const T* contrib = static_cast<const T*>(par ? a.contrib1 : a.contrib0);
+  T* cnext = static_cast<T*>(par ? a.contrib0 : a.contrib1);
+  T* hpart = static_cast<T*>(a.hpart);
+  const T base = static_cast<T>(st->teleport + st->dangling_share);
+  double l1 = 0.0, linf = 0.0, dang = 0.0;
+  const uint32_t b = blockIdx.x;
+  if (b < a.bs) {
+    const uint32_t i = b * kPrBlock + threadIdx.x;
+    if (i < a.ns) {
+      const uint32_t v = a.slist[i];
+      Acc<T> acc;
+      for (uint32_t e = a.off[v]; e < a.off[v + 1]; ++e) acc.add(contrib[a.adj[e]]);
+      finish_vertex<T>(a, v, acc.get(), base, rank, next, cnext, l1, linf, dang);
+    }
+  } else if (b < a.bs + a.bm) {
+    const uint32_t i = (b - a.bs) * (kPrBlock / kWave) + threadIdx.x / kWave;
+    if (i < a.nm) {
+      const uint32_t v = a.mlist[i];
+      Acc<T> acc;
+      for (uint32_t e = a.off[v] + lane_id(); e < a.off[v + 1]; e += kWave) acc.add(contrib[a.adj[e]]);
+      T s = wave_sum(acc.get());
+      if (lane_id() == 0) finish_vertex<T>(a, v, s, base, rank, next, cnext, l1, linf, dang);
+    }
+  } else {
+    const uint32_t c = (b - a.bs - a.bm) * (kPrBlock / kWave) + threadIdx.x / kWave;
+    if (c < a.nchunks) {
+      const uint32_t hv = a.hlist[a.hchunk_v[c]];
+      const uint32_t e0 = a.hchunk_off[c];
+      const uint32_t e1 = min(e0 + kChunk, a.off[hv + 1]);
+      Acc<T> acc;
+      for (uint32_t e = e0 + lane_id(); e < e1; e += kWave) acc.add(contrib[a.adj[e]]);
+      T s = wave_sum(acc.get());
+      if (lane_id() == 0) hpart[c] = s;
+    }
+  }
+  block_reduce3<T>(l1, linf, dang, sm);
+  if (threadIdx.x == 0) {
+    a.partials[b] = PrPartial{l1, linf, dang};
+    __threadfence();
+    s_last = atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  // last workgroup: finish the large rows, then reduce in a fixed order
+  double hl1 = 0.0, hlinf = 0.0, hdang = 0.0;
+  for (uint32_t h = threadIdx.x; h < a.nh; h += blockDim.x) {
+    const uint32_t v = a.hlist[h];
+    Acc<T> acc;
+    for (uint32_t c = a.hchunk_first[h]; c < a.hchunk_first[h + 1]; ++c)
+      acc.add(__hip_atomic_load(&hpart[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    finish_vertex<T>(a, v, acc.get(), base, rank, next, cnext, hl1, hlinf, hdang);
+  }
+  block_reduce3<T>(hl1, hlinf, hdang, sm);
+  if (threadIdx.x == 0) {
+    double L1 = 0.0, LI = 0.0, D = 0.0;
+    for (uint32_t i = 0; i < gridDim.x; ++i) {
+      PrPartial p;
+      p.l1 = __hip_atomic_load(&a.partials[i].l1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      p.linf = __hip_atomic_load(&a.partials[i].linf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      p.dangling = __hip_atomic_load(&a.partials[i].dangling, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      L1 += p.l1;
+      LI = LI > p.linf ? LI : p.linf;
+      D += p.dangling;
+    }
+    L1 += hl1;
+    LI = LI > hlinf ? LI : hlinf;
+    D += hdang;
+    st->l1 = L1;
+    st->linf = LI;
+    st->dangling_share = st->damping * D / static_cast<double>(a.n);
+    st->iterations += 1;
+    const bool conv = st->stop_l1 ? (L1 < st->tol) : (LI <= st->tol);
+    if (conv || st->iterations >= st->max_iterations) st->done = conv ? 1u : 2u;
+    st->blocks_done = 0;
+    __threadfence();
+  }
+}
+
+// rank = 1/n (or keep), contrib = rank/outdeg, dangling partials.
+template <class T>
+__global__ __launch_bounds__(kPrBlock) void k_pr_init(PrArgs a, int warm) {
+  __shared__ double sm[3 * (kPrBlock / kWave)];
+  __shared__ int s_last;
+  PrState* st = a.st;
+  T* rank = static_cast<T*>(a.rank0);
+  T* contrib = static_cast<T*>(a.contrib0);
+  double dang = 0.0, z1 = 0.0, z2 = 0.0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
+    T r = warm ? rank[v] : static_cast<T>(1.0 / static_cast<double>(a.n));
+    rank[v] = r;
+    const uint32_t od = a.outdeg[v];
+    contrib[v] = od ? r / static_cast<T>(od) : T(0);
+    if (!od) dang += static_cast<double>(r);
+  }
+  block_reduce3<T>(z1, z2, dang, sm);
+  if (threadIdx.x == 0) {
+    a.partials[blockIdx.x] = PrPartial{0.0, 0.0, dang};
+    __threadfence();
+    s_last = atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last && threadIdx.x == 0) {
+    __threadfence();
+    double D = 0.0;
+    for (uint32_t i = 0; i < gridDim.x; ++i)
+      D += __hip_atomic_load(&a.partials[i].dangling, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    st->dangling_share = st->damping * D / static_cast<double>(a.n);
+    st->blocks_done = 0;
+    __threadfence();
+  }
+}
+
+// Split vertices by in-degree.
+__global__ void k_pr_classify(const uint32_t* off, uint32_t n, uint32_t* slist, uint32_t* mlist, uint32_t* hlist,
+                              unsigned long long* cnt) {
+  for (uint32_t v0 = blockIdx.x * blockDim.x + threadIdx.x - lane_id(); v0 < n; v0 += gridDim.x * blockDim.x) {
+    const uint32_t v = v0 + lane_id();
+    uint32_t d = v < n ? off[v + 1] - off[v] : 0u;
+    bool s = v < n && d <= kSmallRow;
+    bool m = v < n && d > kSmallRow && d <= kChunk;
+    bool h = v < n && d > kChunk;
+    unsigned long long bs = __ballot(s), bm = __ballot(m), bh = __ballot(h);
+    unsigned long long base_s = 0, base_m = 0, base_h = 0;
+    if (lane_id() == 0) {
+      if (bs) base_s = atomicAdd(&cnt[0], static_cast<unsigned long long>(__popcll(bs)));
+      if (bm) base_m = atomicAdd(&cnt[1], static_cast<unsigned long long>(__popcll(bm)));
+      if (bh) base_h = atomicAdd(&cnt[2], static_cast<unsigned long long>(__popcll(bh)));
+    }
+    base_s = __shfl(base_s, 0, kWave);
+    base_m = __shfl(base_m, 0, kWave);
+    base_h = __shfl(base_h, 0, kWave);
+    if (s) slist[base_s + lanes_below(bs)] = v;
+    if (m) mlist[base_m + lanes_below(bm)] = v;
+    if (h) hlist[base_h + lanes_below(bh)] = v;
+  }
+}
+
+template <class T>
+__global__ void k_to_double(const T* src, double* dst, uint64_t n) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    dst[i] = static_cast<double>(src[i]);
+}
+
+template <class T>
+__global__ void k_from_double(const double* src, T* dst, uint64_t n) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    dst[i] = static_cast<T>(src[i]);
+}
+
+// ------------------------------------------------- localized Jacobi repair
+
+struct LocalArgs {
+  const uint2* in_row;
+  const uint32_t* in_arena;
+  const uint2* out_row;
+  const uint32_t* out_arena;
+  const uint32_t* outdeg;
+  uint32_t* act_flag;   // epoch-stamped "in next active set"
+  uint32_t* ever_flag;  // 1 once ever active
+  uint32_t* alist_cur;
+  uint32_t* alist_next;
+  unsigned long long* cnt;  // [0] next active count, [1] ever count, [2] l1 bits, [3] linf bits
+  uint32_t epoch;
+  uint32_t n;
+};
+
+__device__ __forceinline__ void activate(const LocalArgs& a, uint32_t w, bool want) {
+  bool push = false;
+  if (want && w < a.n) {
+    push = atomicExch(&a.act_flag[w], a.epoch) != a.epoch;
+    if (push && atomicExch(&a.ever_flag[w], 1u) == 0u) atomicAdd(&a.cnt[1], 1ull);
+  }
+  unsigned long long bb = __ballot(push);
+  if (bb) {
+    unsigned long long base = 0;
+    if (lane_id() == 0) base = atomicAdd(&a.cnt[0], static_cast<unsigned long long>(__popcll(bb)));
+    base = __shfl(base, 0, kWave);
+    if (push) a.alist_next[base + lanes_below(bb)] = w;
+  }
+}
+
+// Initial active set: endpoints and the out-neighbours of src (and of dst
+// when undirected), pagerank.hpp:112-117.  One wave per update.
+__global__ void k_local_seed(LocalArgs a, const uint32_t* src, const uint32_t* dst, uint64_t k, int undirected) {
+  const uint64_t W = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave; i < k; i += W) {
+    const uint32_t s = src[i], d = dst[i];
+    activate(a, lane_id() == 0 ? s : d, lane_id() < 2);
+    for (int side = 0; side < (undirected ? 2 : 1); ++side) {
+      const uint32_t u = side ? d : s;
+      if (u >= a.n) continue;
+      const uint2 rb = a.out_row[u];
+      for (uint32_t j = 0; j < rb.y; j += kWave) {
+        uint32_t w = j + lane_id() < rb.y ? a.out_arena[rb.x + j + lane_id()] : kTomb;
+        activate(a, w & kIdMask, !(w & kTomb));
+      }
+    }
+  }
+}
+
+// One Jacobi sweep over the active list (pagerank.hpp:157-173): wave per
+// vertex; updated values go to upd[i] and are written back afterwards.
+template <class T>
+__global__ void k_local_sweep(LocalArgs a, const T* rank, T* upd, uint64_t na, double base, double damping,
+                              double tol) {
+  const uint64_t W = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+  double l1 = 0.0, linf = 0.0;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave; i < na; i += W) {
+    const uint32_t v = a.alist_cur[i];
+    const uint2 rb = a.in_row[v];
+    Acc<T> acc;
+    for (uint32_t j = lane_id(); j < rb.y; j += kWave) {
+      const uint32_t u = a.in_arena[rb.x + j];
+      if (u & kTomb) continue;
+      const uint32_t od = a.outdeg[u];
+      if (od) acc.add(rank[u] / static_cast<T>(od));
+    }
+    const T incoming = wave_sum(acc.get());
+    const T updated = static_cast<T>(base + damping * static_cast<double>(incoming));
+    const double delta = fabs(static_cast<double>(updated) - static_cast<double>(rank[v]));
+    if (lane_id() == 0) {
+      upd[i] = updated;
+      l1 += delta;
+      linf = linf > delta ? linf : delta;
+    }
+    const bool spread = delta > tol;
+    if (spread) {
+      const uint2 ob = a.out_row[v];
+      for (uint32_t j = 0; j < ob.y; j += kWave) {
+        uint32_t w = j + lane_id() < ob.y ? a.out_arena[ob.x + j + lane_id()] : kTomb;
+        activate(a, w & kIdMask, !(w & kTomb));
+      }
+    }
+  }
+  l1 = wave_sum(l1);
+  linf = wave_max(linf);
+  if (lane_id() == 0) {
+    atomicAdd(reinterpret_cast<double*>(&a.cnt[2]), l1);
+    // non-negative doubles order like their bit patterns
+    atomicMax(&a.cnt[3], static_cast<unsigned long long>(__double_as_longlong(linf)));
+  }
+}
+
+template <class T>
+__global__ void k_local_writeback(const uint32_t* alist, const T* upd, uint64_t na, T* rank) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < na;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    rank[alist[i]] = upd[i];
+}
+
+__global__ void k_any_dangling(const uint32_t* src, const uint32_t* dst, uint64_t k, int undirected,
+                               const uint32_t* outdeg, uint32_t n, unsigned long long* flag) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < k;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t s = src[i], d = dst[i];
+    bool dang = s < n && outdeg[s] == 0;
+    if (undirected && d != s && d < n && outdeg[d] == 0) dang = true;
+    if (dang) atomicOr(flag, 1ull);
+  }
+}
+
+__global__ void k_count_zero(const uint32_t* deg, uint64_t n, unsigned long long* out) {
+  unsigned long long c = 0;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    c += deg[i] == 0;
+  c = wave_sum(c);
+  if (lane_id() == 0 && c) atomicAdd(out, c);
+}
+
+template <class T>
+__global__ void k_diff(const T* a, const double* ref, uint64_t n, double* out2) {
+  double l1 = 0.0, linf = 0.0;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    double e = fabs(static_cast<double>(a[i]) - ref[i]);
+    l1 += e;
+    linf = linf > e ? linf : e;
+  }
+  l1 = wave_sum(l1);
+  linf = wave_max(linf);
+  if (lane_id() == 0) {
+    atomicAdd(&out2[0], l1);
+    atomicMax(reinterpret_cast<unsigned long long*>(&out2[1]),
+              static_cast<unsigned long long>(__double_as_longlong(linf)));
+  }
+}
+
+// ---------------------------------------------------------------- host side
+
+static unsigned gblocks(uint64_t items, unsigned per = kPrBlock, unsigned cap = 4096) {
+  uint64_t b = (items + per - 1) / per;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return static_cast<unsigned>(b);
+}
+
+static size_t elt(const vgx_pagerank* p) { return p->precision == 32 ? 4 : 8; }
+
+static void build_lists(vgx_pagerank* p) {
+  vgx_graph* g = p->g;
+  const CsrSnapshot& sn = graph_snapshot(g, 1);
+  if (p->lists_version == g->version && p->n == g->n) return;
+  hipStream_t s = g->stream;
+  const uint64_t n = g->n;
+  p->slist.ensure(n + 1);
+  p->mlist.ensure(n + 1);
+  p->hlist.ensure(n + 1);
+  p->counters.ensure(16);
+  auto* cnt = reinterpret_cast<unsigned long long*>(p->counters.p);
+  VGX_HIP(hipMemsetAsync(cnt, 0, 3 * 8, s));
+  if (n) {
+    k_pr_classify<<<gblocks(n), kPrBlock, 0, s>>>(sn.off.p, static_cast<uint32_t>(n), p->slist.p, p->mlist.p,
+                                                  p->hlist.p, cnt);
+    VGX_LAUNCH_CHECK();
+  }
+  unsigned long long c[3];
+  VGX_HIP(hipMemcpyAsync(c, cnt, sizeof(c), hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipStreamSynchronize(s));
+  p->ns = c[0];
+  p->nm = c[1];
+  p->nh = c[2];
+  // chunk table of the large rows (host; there are few of them)
+  std::vector<uint32_t> hl(p->nh), offs(n + 1);
+  if (p->nh) {
+    VGX_HIP(hipMemcpyAsync(hl.data(), p->hlist.p, p->nh * 4, hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipMemcpyAsync(offs.data(), sn.off.p, (n + 1) * 4, hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+  }
+  std::vector<uint32_t> cv, co, first(p->nh + 1, 0);
+  for (uint64_t h = 0; h < p->nh; ++h) {
+    first[h] = static_cast<uint32_t>(cv.size());
+    for (uint32_t e = offs[hl[h]]; e < offs[hl[h] + 1]; e += kChunk) {
+      cv.push_back(static_cast<uint32_t>(h));
+      co.push_back(e);
+    }
+  }
+  first[p->nh] = static_cast<uint32_t>(cv.size());
+  p->nchunks = cv.size();
+  p->hchunk_v.ensure(cv.size() + 1);
+  p->hchunk_off.ensure(co.size() + 1);
+  p->hchunk_first.ensure(first.size());
+  if (!cv.empty()) {
+    VGX_HIP(hipMemcpyAsync(p->hchunk_v.p, cv.data(), cv.size() * 4, hipMemcpyHostToDevice, s));
+    VGX_HIP(hipMemcpyAsync(p->hchunk_off.p, co.data(), co.size() * 4, hipMemcpyHostToDevice, s));
+  }
+  VGX_HIP(hipMemcpyAsync(p->hchunk_first.p, first.data(), first.size() * 4, hipMemcpyHostToDevice, s));
+  p->hpart.ensure((cv.size() + 1) * 8);
+  VGX_HIP(hipStreamSynchronize(s));
+  p->lists_version = g->version;
+}
+
+static PrArgs pr_args(vgx_pagerank* p, DevBuf<uint8_t>* rk) {
+  vgx_graph* g = p->g;
+  const CsrSnapshot& sn = graph_snapshot(g, 1);
+  PrArgs a{};
+  a.off = sn.off.p;
+  a.adj = sn.adj.p;
+  a.outdeg = g->adj[0].deg.p;
+  a.slist = p->slist.p;
+  a.mlist = p->mlist.p;
+  a.hlist = p->hlist.p;
+  a.hchunk_v = p->hchunk_v.p;
+  a.hchunk_off = p->hchunk_off.p;
+  a.hchunk_first = p->hchunk_first.p;
+  a.ns = static_cast<uint32_t>(p->ns);
+  a.nm = static_cast<uint32_t>(p->nm);
+  a.nh = static_cast<uint32_t>(p->nh);
+  a.nchunks = static_cast<uint32_t>(p->nchunks);
+  a.bs = static_cast<uint32_t>((p->ns + kPrBlock - 1) / kPrBlock);
+  a.bm = static_cast<uint32_t>((p->nm + 3) / 4);
+  a.n = static_cast<uint32_t>(g->n);
+  a.rank0 = rk[0].p;
+  a.rank1 = rk[1].p;
+  a.contrib0 = p->contrib[0].p;
+  a.contrib1 = p->contrib[1].p;
+  a.hpart = p->hpart.p;
+  a.partials = p->partials.p;
+  a.st = p->d_st;
+  return a;
+}
+
+// Power iteration into rk[0..1]; the result ends in rk[result index].
+static int full_solve(vgx_pagerank* p, DevBuf<uint8_t>* rk, int precision, uint64_t max_iterations, double tol,
+                      bool stop_l1, bool warm) {
+  vgx_graph* g = p->g;
+  hipStream_t s = g->stream;
+  const uint64_t n = g->n;
+  const size_t es = precision == 32 ? 4 : 8;
+  build_lists(p);
+  rk[0].ensure(n * es + 64);
+  rk[1].ensure(n * es + 64);
+  p->contrib[0].ensure(n * es + 64);
+  p->contrib[1].ensure(n * es + 64);
+  PrState h{};
+  h.damping = p->damping;
+  h.teleport = n ? (1.0 - p->damping) / static_cast<double>(n) : 0.0;
+  h.tol = tol;
+  h.stop_l1 = stop_l1 ? 1 : 0;
+  h.max_iterations = max_iterations;
+  h.done = (n == 0 || max_iterations == 0) ? 1u : 0u;
+  VGX_HIP(hipMemcpyAsync(p->d_st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+  PrArgs a = pr_args(p, rk);
+  const unsigned grid = a.bs + a.bm + static_cast<uint32_t>((p->nchunks + 3) / 4);
+  const unsigned ginit = gblocks(n, kPrBlock, 2048);
+  p->partials.ensure(std::max<unsigned>(grid, ginit) + 1);
+  a.partials = p->partials.p;
+  if (n == 0) {
+    VGX_HIP(hipStreamSynchronize(s));
+    *p->h_st = h;
+    p->h_st->done = 1;
+    return 0;
+  }
+  if (precision == 32) k_pr_init<float><<<ginit, kPrBlock, 0, s>>>(a, warm ? 1 : 0);
+  else k_pr_init<double><<<ginit, kPrBlock, 0, s>>>(a, warm ? 1 : 0);
+  VGX_LAUNCH_CHECK();
+  uint64_t launched = 0;
+  uint32_t chunk = 4;
+  while (launched < max_iterations) {
+    uint64_t c = std::min<uint64_t>(chunk, max_iterations - launched);
+    for (uint64_t i = 0; i < c; ++i) {
+      if (precision == 32) k_pr_iter<float><<<grid, kPrBlock, 0, s>>>(a);
+      else k_pr_iter<double><<<grid, kPrBlock, 0, s>>>(a);
+      VGX_LAUNCH_CHECK();
+    }
+    launched += c;
+    VGX_HIP(hipMemcpyAsync(p->h_st, p->d_st, sizeof(PrState), hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    if (p->h_st->done) break;
+    chunk = std::min<uint32_t>(chunk * 2, 16);
+  }
+  return static_cast<int>(p->h_st->iterations & 1ull);
+}
+
+static void count_dangling(vgx_pagerank* p) {
+  vgx_graph* g = p->g;
+  p->counters.ensure(16);
+  auto* cnt = reinterpret_cast<unsigned long long*>(p->counters.p);
+  VGX_HIP(hipMemsetAsync(cnt + 4, 0, 8, g->stream));
+  if (g->n) {
+    k_count_zero<<<gblocks(g->n), kPrBlock, 0, g->stream>>>(g->adj[0].deg.p, g->n, cnt + 4);
+    VGX_LAUNCH_CHECK();
+  }
+  unsigned long long c = 0;
+  VGX_HIP(hipMemcpyAsync(&c, cnt + 4, 8, hipMemcpyDeviceToHost, g->stream));
+  VGX_HIP(hipStreamSynchronize(g->stream));
+  p->stats.dangling_vertices = c;
+}
+
+// recompute(), pagerank.hpp:193-202
+static void recompute(vgx_pagerank* p, uint64_t max_iterations, double tol, bool stop_l1 = false, bool warm = false) {
+  int r = full_solve(p, p->rank, p->precision, max_iterations, tol, stop_l1, warm);
+  p->cur = r;
+  p->n = p->g->n;
+  p->have_values = true;
+  count_dangling(p);
+  p->stats.last_repaired_vertices = p->g->n;
+  p->stats.last_repair_iterations = p->h_st->iterations;
+  p->stats.last_residual_l1 = p->h_st->l1;
+  p->stats.last_residual_linf = p->h_st->linf;
+  p->stats.last_full_recompute_converged = p->h_st->done == 1 || p->g->n == 0;
+  p->stats.last_used_full_recompute = 1;
+}
+
+static void reset_local_stats(vgx_pagerank* p) {
+  p->stats.last_repaired_vertices = 0;
+  p->stats.last_repair_iterations = 0;
+  p->stats.last_residual_l1 = 0.0;
+  p->stats.last_residual_linf = 0.0;
+}
+
+template <class T>
+static void local_repair(vgx_pagerank* p, const uint32_t* d_src, const uint32_t* d_dst, uint64_t k,
+                         uint64_t local_iterations, double tol, uint64_t fallback_limit) {
+  vgx_graph* g = p->g;
+  hipStream_t s = g->stream;
+  const uint64_t n = g->n;
+  p->act_flag.ensure(n + 1);
+  p->ever_flag.ensure(n + 1);
+  p->alist[0].ensure(n + 1);
+  p->alist[1].ensure(n + 1);
+  p->upd.ensure((n + 1) * sizeof(T));
+  VGX_HIP(hipMemsetAsync(p->act_flag.p, 0, (n + 1) * 4, s));
+  VGX_HIP(hipMemsetAsync(p->ever_flag.p, 0, (n + 1) * 4, s));
+  auto* cnt = reinterpret_cast<unsigned long long*>(p->counters.p);
+  VGX_HIP(hipMemsetAsync(cnt, 0, 4 * 8, s));
+  LocalArgs a{};
+  a.in_row = g->in().row.p;
+  a.in_arena = g->in().arena.p;
+  a.out_row = g->adj[0].row.p;
+  a.out_arena = g->adj[0].arena.p;
+  a.outdeg = g->adj[0].deg.p;
+  a.act_flag = p->act_flag.p;
+  a.ever_flag = p->ever_flag.p;
+  a.cnt = cnt;
+  a.n = static_cast<uint32_t>(n);
+  uint32_t epoch = 1;
+  int cur = 0;
+  a.epoch = epoch;
+  a.alist_cur = p->alist[1].p;
+  a.alist_next = p->alist[0].p;
+  k_local_seed<<<gblocks(k, 4, 2048), kPrBlock, 0, s>>>(a, d_src, d_dst, k, g->directed ? 0 : 1);
+  VGX_LAUNCH_CHECK();
+  unsigned long long h[4];
+  VGX_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipStreamSynchronize(s));
+  uint64_t active = h[0];
+  if (active == 0) {
+    reset_local_stats(p);
+    return;
+  }
+  if (active >= fallback_limit) {
+    recompute(p, 200, 1e-12);
+    return;
+  }
+  const double base = (1.0 - p->damping) / static_cast<double>(n);
+  reset_local_stats(p);
+  p->stats.last_full_recompute_converged = 0;
+  p->stats.last_used_full_recompute = 0;
+  T* rank = reinterpret_cast<T*>(p->rank[p->cur].p);
+  for (uint64_t it = 0; it < local_iterations && active != 0; ++it) {
+    p->stats.last_repair_iterations += 1;
+    ++epoch;
+    a.epoch = epoch;
+    a.alist_cur = p->alist[cur].p;
+    a.alist_next = p->alist[cur ^ 1].p;
+    VGX_HIP(hipMemsetAsync(cnt, 0, 8, s));
+    VGX_HIP(hipMemsetAsync(cnt + 2, 0, 16, s));
+    k_local_sweep<T><<<gblocks(active, 4, 4096), kPrBlock, 0, s>>>(a, rank, reinterpret_cast<T*>(p->upd.p), active,
+                                                                 base, p->damping, tol);
+    VGX_LAUNCH_CHECK();
+    k_local_writeback<T><<<gblocks(active), kPrBlock, 0, s>>>(a.alist_cur, reinterpret_cast<T*>(p->upd.p), active,
+                                                              rank);
+    VGX_LAUNCH_CHECK();
+    VGX_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    active = h[0];
+    double l1, linf;
+    std::memcpy(&l1, &h[2], 8);
+    std::memcpy(&linf, &h[3], 8);
+    p->stats.last_residual_l1 = l1;
+    p->stats.last_residual_linf = linf;
+    cur ^= 1;
+    if (h[1] >= fallback_limit) {
+      recompute(p, 200, 1e-12);
+      return;
+    }
+  }
+  p->stats.last_repaired_vertices = h[1];
+}
+
+static void apply(vgx_pagerank* p, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
+                  uint64_t local_iterations, double tol, double frac) {
+  vgx_graph* g = p->g;
+  if (k == 0) {
+    reset_local_stats(p);
+    return;
+  }
+  const uint64_t prev_n = p->n;
+  const bool had_dangling = p->stats.dangling_vertices != 0;
+  graph_stage(g, src, dst, add, k);
+  graph_apply_deletions(g);
+  graph_apply_additions(g);
+  graph_finish_batch(g);
+  const uint64_t n = g->n;
+  p->stats.last_used_full_recompute = 0;
+  if (n == 0) {
+    reset_local_stats(p);
+    p->stats.last_full_recompute_converged = 1;
+    p->stats.dangling_vertices = 0;
+    p->n = 0;
+    return;
+  }
+  if (n != prev_n || had_dangling) {
+    recompute(p, 200, 1e-12);
+    return;
+  }
+  hipStream_t s = g->stream;
+  const uint32_t* d_src = g->bs[0].src.p;
+  const uint32_t* d_dst = g->bs[0].src.p + k;
+  p->counters.ensure(16);
+  auto* cnt = reinterpret_cast<unsigned long long*>(p->counters.p);
+  VGX_HIP(hipMemsetAsync(cnt + 5, 0, 8, s));
+  k_any_dangling<<<gblocks(k), kPrBlock, 0, s>>>(d_src, d_dst, k, g->directed ? 0 : 1, g->adj[0].deg.p,
+                                                 static_cast<uint32_t>(n), cnt + 5);
+  VGX_LAUNCH_CHECK();
+  unsigned long long any = 0;
+  VGX_HIP(hipMemcpyAsync(&any, cnt + 5, 8, hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipStreamSynchronize(s));
+  if (any) {
+    recompute(p, 200, 1e-12);
+    return;
+  }
+  p->stats.dangling_vertices = 0;
+  double fl = frac * static_cast<double>(n);
+  uint64_t limit = static_cast<uint64_t>(fl > 1.0 ? fl : 1.0);
+  if (p->precision == 32) local_repair<float>(p, d_src, d_dst, k, local_iterations, tol, limit);
+  else local_repair<double>(p, d_src, d_dst, k, local_iterations, tol, limit);
+}
+
+static vgx_pagerank_stats validate(vgx_pagerank* p, uint64_t ref_it, double ref_tol, double l1_tol, double linf_tol,
+                                   double out[6], bool& ref_conv, uint64_t& ref_iters) {
+  // FP64 reference solve (the reference's full_solve) into ref_rank.
+  int r = full_solve(p, p->ref_rank, 64, ref_it, ref_tol, false, false);
+  ref_conv = p->h_st->done == 1 || p->g->n == 0;
+  ref_iters = p->h_st->iterations;
+  out[2] = p->h_st->l1;
+  out[3] = p->h_st->linf;
+  vgx_graph* g = p->g;
+  hipStream_t s = g->stream;
+  p->counters.ensure(16);
+  auto* d2 = reinterpret_cast<double*>(p->counters.p) + 6;
+  VGX_HIP(hipMemsetAsync(d2, 0, 16, s));
+  const uint64_t n = g->n;
+  if (p->n != n || !p->have_values) {
+    out[0] = out[1] = INFINITY;
+  } else {
+    if (n) {
+      const double* ref = reinterpret_cast<const double*>(p->ref_rank[r].p);
+      if (p->precision == 32)
+        k_diff<float><<<gblocks(n), kPrBlock, 0, s>>>(reinterpret_cast<float*>(p->rank[p->cur].p), ref, n, d2);
+      else
+        k_diff<double><<<gblocks(n), kPrBlock, 0, s>>>(reinterpret_cast<double*>(p->rank[p->cur].p), ref, n, d2);
+      VGX_LAUNCH_CHECK();
+    }
+    double h2[2] = {0, 0};
+    VGX_HIP(hipMemcpyAsync(h2, d2, 16, hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    out[0] = h2[0];
+    out[1] = h2[1];
+  }
+  out[4] = r;
+  (void)l1_tol;
+  (void)linf_tol;
+  return p->stats;
+}
+
+}  // namespace vgx
+
+using namespace vgx;
+
+#define VGX_API_BEGIN try {
+#define VGX_API_END                                                   \
+  }                                                                   \
+  catch (const vgx::Error& e) {                                       \
+    return vgx::set_error(e.code, e.what());                          \
+  }                                                                   \
+  catch (const std::bad_alloc&) {                                     \
+    return vgx::set_error(VGX_OUT_OF_MEMORY, "host allocation failed"); \
+  }                                                                   \
+  catch (const std::exception& e) {                                   \
+    return vgx::set_error(VGX_INVALID_ARGUMENT, e.what());            \
+  }                                                                   \
+  vgx::t_last_error.clear();                                          \
+  return VGX_OK;
+
+extern "C" {
+
+vgx_status vgx_pagerank_create(vgx_graph* g, double damping, int precision, vgx_pagerank** out) {
+  VGX_API_BEGIN
+  if (!g || !out) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  if (!(damping >= 0.0 && damping <= 1.0)) throw Error(VGX_INVALID_ARGUMENT, "PageRank damping must be in [0, 1]");
+  if (precision != 32 && precision != 64) throw Error(VGX_INVALID_ARGUMENT, "precision must be 32 or 64");
+  VGX_HIP(hipSetDevice(g->device));
+  auto* p = new vgx_pagerank();
+  try {
+    p->g = g;
+    p->damping = damping;
+    p->precision = precision;
+    p->d_st = static_cast<PrState*>(dev_alloc(sizeof(PrState)));
+    VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_st), sizeof(PrState), hipHostMallocDefault));
+    std::memset(p->h_st, 0, sizeof(PrState));
+    p->counters.ensure(16);
+    recompute(p, 200, 1e-12);
+  } catch (...) {
+    vgx_pagerank_destroy(p);
+    throw;
+  }
+  *out = p;
+  VGX_API_END
+}
+
+vgx_status vgx_pagerank_destroy(vgx_pagerank* p) {
+  VGX_API_BEGIN
+  if (!p) return VGX_OK;
+  if (p->g) (void)hipStreamSynchronize(p->g->stream);
+  if (p->d_st) dev_free(p->d_st);
+  if (p->h_st) (void)hipHostFree(p->h_st);
+  delete p;
+  VGX_API_END
+}
+
+vgx_status vgx_pagerank_recompute(vgx_pagerank* p, uint64_t max_iterations, double tol) {
+  VGX_API_BEGIN
+  if (!p) throw Error(VGX_INVALID_ARGUMENT, "null pagerank");
+  VGX_HIP(hipSetDevice(p->g->device));
+  recompute(p, max_iterations, tol);
+  VGX_API_END
+}
+
+vgx_status vgx_pagerank_solve_l1(vgx_pagerank* p, uint64_t max_iterations, double l1_tol, int warm_start) {
+  VGX_API_BEGIN
+  if (!p) throw Error(VGX_INVALID_ARGUMENT, "null pagerank");
+  VGX_HIP(hipSetDevice(p->g->device));
+  const bool warm = warm_start && p->have_values && p->n == p->g->n;
+  if (warm && p->cur != 0) {
+    // warm start reads rank[0]
+    VGX_HIP(hipMemcpyAsync(p->rank[0].p, p->rank[p->cur].p, p->n * elt(p), hipMemcpyDeviceToDevice, p->g->stream));
+    p->cur = 0;
+  }
+  recompute(p, max_iterations, l1_tol, true, warm);
+  VGX_API_END
+}
+
+vgx_status vgx_pagerank_apply(vgx_pagerank* p, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
+                              uint64_t k, uint64_t local_iterations, double tol, double full_fallback_fraction) {
+  VGX_API_BEGIN
+  if (!p || (k && (!src || !dst || !add))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  VGX_HIP(hipSetDevice(p->g->device));
+  apply(p, src, dst, add, k, local_iterations, tol, full_fallback_fraction);
+  VGX_API_END
+}
+
+vgx_status vgx_pagerank_values(vgx_pagerank* p, double* out, uint64_t cap, uint64_t* count) {
+  VGX_API_BEGIN
+  if (!p || !count) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  VGX_HIP(hipSetDevice(p->g->device));
+  const uint64_t n = p->have_values ? p->n : 0;
+  *count = n;
+  const uint64_t c = std::min(cap, n);
+  if (out && c) {
+    hipStream_t s = p->g->stream;
+    if (p->precision == 64) {
+      VGX_HIP(hipMemcpyAsync(out, p->rank[p->cur].p, c * 8, hipMemcpyDeviceToHost, s));
+    } else {
+      DevBuf<double> tmp;
+      tmp.ensure(c);
+      k_to_double<float><<<gblocks(c), kPrBlock, 0, s>>>(reinterpret_cast<float*>(p->rank[p->cur].p), tmp.p, c);
+      VGX_LAUNCH_CHECK();
+      VGX_HIP(hipMemcpyAsync(out, tmp.p, c * 8, hipMemcpyDeviceToHost, s));
+      VGX_HIP(hipStreamSynchronize(s));
+    }
+    VGX_HIP(hipStreamSynchronize(s));
+  }
+  VGX_API_END
+}
+
+vgx_status vgx_pagerank_stats_get(vgx_pagerank* p, vgx_pagerank_stats* out) {
+  VGX_API_BEGIN
+  if (!p || !out) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  *out = p->stats;
+  VGX_API_END
+}
+
+// validate_against_full (pagerank.hpp:204-232); out_errors = {l1, linf,
+// reference residual l1, reference residual linf}.
+vgx_status vgx_pagerank_validate(vgx_pagerank* p, uint64_t reference_max_iterations, double reference_tol,
+                                 double* out_errors, uint64_t* reference_iterations, int* reference_converged) {
+  VGX_API_BEGIN
+  if (!p || !out_errors) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  VGX_HIP(hipSetDevice(p->g->device));
+  double o[6];
+  bool conv = false;
+  uint64_t it = 0;
+  validate(p, reference_max_iterations, reference_tol, 0, 0, o, conv, it);
+  for (int i = 0; i < 4; ++i) out_errors[i] = o[i];
+  if (reference_iterations) *reference_iterations = it;
+  if (reference_converged) *reference_converged = conv ? 1 : 0;
+  VGX_API_END
+}
+
+// Install the FP64 reference solve as the current values (apply_validated's
+// fallback, pagerank.hpp:246-256).
+vgx_status vgx_pagerank_install_reference(vgx_pagerank* p, uint64_t reference_max_iterations, double reference_tol) {
+  VGX_API_BEGIN
+  if (!p) throw Error(VGX_INVALID_ARGUMENT, "null pagerank");
+  VGX_HIP(hipSetDevice(p->g->device));
+  int r = full_solve(p, p->ref_rank, 64, reference_max_iterations, reference_tol, false, false);
+  vgx_graph* g = p->g;
+  const uint64_t n = g->n;
+  p->rank[0].ensure(n * elt(p) + 64);
+  if (n) {
+    if (p->precision == 64) {
+      VGX_HIP(hipMemcpyAsync(p->rank[0].p, p->ref_rank[r].p, n * 8, hipMemcpyDeviceToDevice, g->stream));
+    } else {
+      k_from_double<float><<<gblocks(n), kPrBlock, 0, g->stream>>>(
+          reinterpret_cast<const double*>(p->ref_rank[r].p), reinterpret_cast<float*>(p->rank[0].p), n);
+      VGX_LAUNCH_CHECK();
+    }
+  }
+  p->cur = 0;
+  p->n = n;
+  p->have_values = true;
+  count_dangling(p);
+  p->stats.last_repaired_vertices = n;
+  p->stats.last_repair_iterations = p->h_st->iterations;
+  p->stats.last_residual_l1 = p->h_st->l1;
+  p->stats.last_residual_linf = p->h_st->linf;
+  p->stats.last_full_recompute_converged = p->h_st->done == 1 || n == 0;
+  VGX_HIP(hipStreamSynchronize(g->stream));
+  VGX_API_END
+}
+
+// Device pointer to the current ranks (float32 or float64 per precision).
+vgx_status vgx_pagerank_values_device(vgx_pagerank* p, const void** dev, uint64_t* count, int* precision) {
+  VGX_API_BEGIN
+  if (!p || !dev) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  *dev = p->rank[p->cur].p;
+  if (count) *count = p->have_values ? p->n : 0;
+  if (precision) *precision = p->precision;
+  VGX_API_END
+}
+
+}  // extern "C"

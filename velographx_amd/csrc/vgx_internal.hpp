@@ -1,0 +1,288 @@
+// Internal definitions shared by the gfx950 kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "vgx.h"
+
+namespace vgx {
+
+// Row entries are vertex ids below 2^31; bit 31 marks a deleted (tombstoned)
+// entry that stays in place so rows keep their sorted order.
+constexpr uint32_t kTomb = 0x80000000u;
+constexpr uint32_t kIdMask = 0x7FFFFFFFu;
+constexpr uint32_t kInf = 0xFFFFFFFFu;
+constexpr uint64_t kInvalidKey = ~0ull;
+// Packed frontier counter: vertex count in the high 26 bits, edge prefix in
+// the low 38 bits, so one 64-bit atomic reserves a queue slot and its edge
+// offset consistently.
+constexpr int kEdgeBits = 38;
+constexpr unsigned long long kEdgeMask = (1ull << kEdgeBits) - 1;
+
+struct Error : std::runtime_error {
+  vgx_status code;
+  Error(vgx_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line);
+
+#define VGX_HIP(x)                                                  \
+  do {                                                              \
+    hipError_t e_ = (x);                                            \
+    if (e_ != hipSuccess) ::vgx::throw_hip(e_, #x, __FILE__, __LINE__); \
+  } while (0)
+
+#define VGX_LAUNCH_CHECK() VGX_HIP(hipGetLastError())
+
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+
+// Owning device buffer that grows on demand (contents are not preserved by
+// ensure(); use grow_copy() when they must be).
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) dev_free(p);
+    p = nullptr;
+    n = 0;
+  }
+  void ensure(size_t m) {
+    if (m <= n) return;
+    release();
+    size_t c = m < 1024 ? 1024 : m;
+    p = static_cast<T*>(dev_alloc(c * sizeof(T)));
+    n = c;
+  }
+  void grow_copy(size_t m, size_t used, hipStream_t s) {
+    if (m <= n) return;
+    size_t c = m < 1024 ? 1024 : m;
+    T* q = static_cast<T*>(dev_alloc(c * sizeof(T)));
+    if (used) VGX_HIP(hipMemcpyAsync(q, p, used * sizeof(T), hipMemcpyDeviceToDevice, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    release();
+    p = q;
+    n = c;
+  }
+  void swap(DevBuf& o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
+};
+
+// Pinned host staging buffer.
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~HostBuf();
+  void ensure(size_t b);
+};
+
+// One direction of adjacency: a row table {beg, len} into an id arena.
+// len counts stored entries including tombstones; cap >= len is the slot
+// count reserved for the row.  Rows are sorted by id (tombstones included).
+struct Adjacency {
+  DevBuf<uint2> row;      // [ncap]
+  DevBuf<uint32_t> cap;   // [ncap]
+  DevBuf<uint32_t> deg;   // [ncap] live degree
+  DevBuf<uint32_t> arena; // [acap]
+  uint64_t acap = 0;       // arena capacity (entries)
+  uint64_t used = 0;       // host mirror of the bump pointer
+  uint64_t stored = 0;     // sum of len (live + tombstones)
+  uint64_t live = 0;       // live arcs
+  uint64_t relocated = 0;  // rows moved since the last compaction
+};
+
+// Device-side counters shared with the host through one small copy.
+struct GraphCounters {
+  unsigned long long bump[2];        // arena bump pointer per direction
+  unsigned long long need[2];        // arena slots requested by relocations
+  unsigned long long live_delta[2];  // signed changes, as two's complement
+  unsigned long long tomb_delta[2];
+  unsigned long long stored_delta[2];
+  unsigned long long relocated[2];
+  unsigned long long nfinal[2];
+  unsigned long long nins[2];
+  unsigned long long nbig[2];
+};
+
+// Staged update batch (device side).
+struct BatchScratch {
+  DevBuf<uint32_t> src, dst;
+  DevBuf<uint8_t> add;
+  DevBuf<unsigned long long> keys, skeys;
+  DevBuf<uint32_t> vals, svals;
+  DevBuf<uint8_t> act;      // per sorted arc: action code
+  DevBuf<uint32_t> pos;     // per sorted arc: arena slot of the match
+  DevBuf<uint32_t> fin;     // sorted positions of final arcs
+  DevBuf<uint32_t> ins;     // sorted positions of inserts
+  DevBuf<uint32_t> segk;    // per insert index: inserts in its row (segment start) or 0
+  DevBuf<uint32_t> big;     // insert indices of rows merged block-wide
+  DevBuf<uint8_t> flags;
+  DevBuf<uint8_t> tmp;      // library temp storage
+  uint64_t k = 0, m = 0;    // ops, arcs
+  uint64_t nfinal = 0, nins = 0, nbig = 0;
+};
+
+enum ActCode : uint8_t { kActNopDel = 0, kActDel = 1, kActNopAdd = 2, kActUndel = 3, kActIns = 4 };
+
+struct CsrSnapshot {
+  DevBuf<uint32_t> off;  // [n+1]
+  DevBuf<uint32_t> adj;  // [arcs]
+  uint64_t version = ~0ull;
+  uint64_t n = 0, arcs = 0;
+};
+
+}  // namespace vgx
+
+struct vgx_graph {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool directed = false;
+  uint64_t n = 0;
+  uint64_t ncap = 0;
+  uint64_t version = 0;
+  vgx::Adjacency adj[2];  // [0] out, [1] in (unused when undirected)
+  vgx::GraphCounters* d_ctr = nullptr;
+  vgx::GraphCounters* h_ctr = nullptr;  // pinned
+  vgx::BatchScratch bs[2];
+  vgx::HostBuf stage;
+  vgx::CsrSnapshot snap[2];
+  int ndir() const { return directed ? 2 : 1; }
+  vgx::Adjacency& in() { return directed ? adj[1] : adj[0]; }
+};
+
+namespace vgx {
+
+// Graph internals used by the algorithm files.
+void graph_grow_vertices(vgx_graph* g, uint64_t n_new);
+// Stages a batch: uploads, sorts, resolves actions; grows vertices; sizes
+// the arena for the inserts.  Does not change adjacency.
+void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k);
+void graph_apply_deletions(vgx_graph* g);
+void graph_apply_additions(vgx_graph* g);
+void graph_finish_batch(vgx_graph* g);
+const CsrSnapshot& graph_snapshot(vgx_graph* g, int dir);
+void graph_sync_counters(vgx_graph* g);
+
+// -------------------------------------------------------------- wave helpers
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+template <class T>
+__device__ __forceinline__ T wave_inclusive_scan(T x) {
+  const unsigned lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    T y = __shfl_up(x, d, kWave);
+    if (lane >= static_cast<unsigned>(d)) x += y;
+  }
+  return x;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+  return x;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_max(T x) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    T y = __shfl_xor(x, d, kWave);
+    x = x > y ? x : y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ unsigned long long atomic_load_u64(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_load_u32(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave-aggregated push of (v, len) into a packed frontier: one 64-bit atomic
+// per wave reserves consecutive queue slots and their edge prefixes.  Must be
+// called by all lanes of the wave.
+__device__ __forceinline__ void wave_push(unsigned long long* ctr, uint32_t* queue, unsigned long long* pre,
+                                          bool want, uint32_t v, uint32_t len) {
+  unsigned long long ball = __ballot(want);
+  if (ball == 0) return;
+  uint32_t e = want ? len : 0u;
+  unsigned long long einc = wave_inclusive_scan<unsigned long long>(e);
+  unsigned long long etot = __shfl(einc, kWave - 1, kWave);
+  uint32_t cnt = __popcll(ball);
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(ctr, (static_cast<unsigned long long>(cnt) << kEdgeBits) + etot);
+  base = __shfl(base, 0, kWave);
+  if (want) {
+    uint32_t slot = static_cast<uint32_t>(base >> kEdgeBits) + lanes_below(ball);
+    queue[slot] = v;
+    pre[slot] = (base & kEdgeMask) + einc - e;
+  }
+}
+
+// Load-balanced edge loop over a packed frontier (queue, pre, count F, edge
+// total E): each wave takes a contiguous edge range and, for every edge,
+// calls fn(item_index, vertex, row, arena_slot, lane_valid).  fn is called
+// by all lanes (valid=false for padding) so it may use wave intrinsics.
+template <class Fn>
+__device__ __forceinline__ void lbs_edges(const uint32_t* queue, const unsigned long long* pre, uint32_t F,
+                                          unsigned long long E, const uint2* row, uint32_t gw, uint32_t W,
+                                          Fn&& fn) {
+  if (E == 0) return;
+  unsigned long long S = (E + W - 1) / W;
+  S = (S + kWave - 1) & ~static_cast<unsigned long long>(kWave - 1);
+  unsigned long long a = static_cast<unsigned long long>(gw) * S;
+  if (a >= E) return;
+  unsigned long long b = a + S < E ? a + S : E;
+  // first item whose prefix <= a
+  uint32_t lo = 0, hi = F;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (pre[mid] <= a) lo = mid;
+    else hi = mid;
+  }
+  uint32_t i0 = lo;
+  const unsigned lane = lane_id();
+  for (unsigned long long e = a; e < b; e += kWave) {
+    unsigned long long g = e + lane;
+    bool valid = g < b;
+    uint32_t l = i0, h = (F - i0 > static_cast<uint32_t>(kWave) + 1) ? i0 + kWave + 1 : F;
+    if (valid) {
+      while (h - l > 1) {
+        uint32_t mid = (l + h) >> 1;
+        if (pre[mid] <= g) l = mid;
+        else h = mid;
+      }
+    }
+    uint32_t v = valid ? queue[l] : 0u;
+    uint2 rb = valid ? row[v] : make_uint2(0u, 0u);
+    uint32_t slot = valid ? rb.x + static_cast<uint32_t>(g - pre[l]) : 0u;
+    fn(l, v, rb, slot, valid);
+    i0 = __shfl(l, kWave - 1, kWave);
+    if (!__shfl(static_cast<int>(valid), kWave - 1, kWave)) break;
+  }
+}
+
+}  // namespace vgx

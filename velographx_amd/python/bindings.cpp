@@ -1,0 +1,334 @@
+// pybind11 module mirroring the reference's python/bindings.cpp:38-52 for
+// the traversal path (Graph, UpdateBatch, IncrementalBFS,
+// IncrementalPageRank, PageRankValidation, from_numpy_edges /
+// from_scipy_csr / from_arrow_columns / from_arrow_table), backed by the
+// gfx950 engine through the C++ facade.  Same names, argument names,
+// defaults and return types; additions are marked as such.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "velographx_amd/pagerank.hpp"
+#include "velographx_amd/velographx.hpp"
+
+namespace py = pybind11;
+using velographx::DynamicGraph;
+using velographx::IncrementalBFS;
+using velographx::IncrementalPageRank;
+using velographx::PageRankValidation;
+using velographx::UpdateBatch;
+using velographx::VertexId;
+
+namespace {
+
+using U32Array = py::array_t<std::uint32_t, py::array::c_style | py::array::forcecast>;
+
+// bindings.cpp:21-25: an UpdateBatch of additions applied to Graph(0, directed).
+DynamicGraph graph_from_numpy_edges(const U32Array& edges, bool directed) {
+  const auto info = edges.request();
+  if (info.ndim != 2 || info.shape[1] != 2) throw std::invalid_argument("edge array must have shape (N, 2)");
+  DynamicGraph graph(0, directed);
+  const auto m = static_cast<std::size_t>(info.shape[0]);
+  const auto* data = static_cast<const std::uint32_t*>(info.ptr);
+  std::vector<std::uint32_t> s(m), d(m);
+  std::vector<std::uint8_t> a(m, 1);
+  for (std::size_t i = 0; i < m; ++i) {
+    s[i] = data[2 * i];
+    d[i] = data[2 * i + 1];
+  }
+  {
+    py::gil_scoped_release nogil;
+    velographx::vgx_check(vgx_graph_apply(graph.handle(), s.data(), d.data(), a.data(), m));
+  }
+  return graph;
+}
+
+// bindings.cpp:26-32
+DynamicGraph graph_from_scipy_csr(const py::object& csr, bool directed) {
+  if (!py::hasattr(csr, "indptr") || !py::hasattr(csr, "indices") || !py::hasattr(csr, "shape"))
+    throw std::invalid_argument("expected a SciPy CSR-like object with indptr, indices, and shape");
+  const auto shape = csr.attr("shape").cast<py::tuple>();
+  if (shape.size() != 2 || shape[0].cast<py::ssize_t>() != shape[1].cast<py::ssize_t>())
+    throw std::invalid_argument("CSR graph matrix must be square");
+  const auto vertices = static_cast<std::size_t>(shape[0].cast<py::ssize_t>());
+  auto indptr = py::array_t<std::int64_t, py::array::c_style | py::array::forcecast>(csr.attr("indptr"));
+  auto indices = py::array_t<std::int64_t, py::array::c_style | py::array::forcecast>(csr.attr("indices"));
+  const auto p = indptr.unchecked<1>();
+  const auto idx = indices.unchecked<1>();
+  if (static_cast<std::size_t>(p.shape(0)) != vertices + 1) throw std::invalid_argument("invalid CSR indptr length");
+  DynamicGraph graph(vertices, directed);
+  std::vector<std::uint32_t> s, d;
+  for (std::size_t u = 0; u < vertices; ++u) {
+    const auto begin = p(static_cast<py::ssize_t>(u)), end = p(static_cast<py::ssize_t>(u + 1));
+    if (begin < 0 || end < begin || end > idx.shape(0)) throw std::invalid_argument("invalid CSR offsets");
+    for (auto pos = begin; pos < end; ++pos) {
+      const auto v = idx(static_cast<py::ssize_t>(pos));
+      if (v < 0 || static_cast<std::size_t>(v) >= vertices)
+        throw std::invalid_argument("CSR contains out-of-range vertex id");
+      if (!directed && static_cast<std::size_t>(v) < u) continue;
+      s.push_back(static_cast<std::uint32_t>(u));
+      d.push_back(static_cast<std::uint32_t>(v));
+    }
+  }
+  std::vector<std::uint8_t> a(s.size(), 1);
+  {
+    py::gil_scoped_release nogil;
+    velographx::vgx_check(vgx_graph_apply(graph.handle(), s.data(), d.data(), a.data(), s.size()));
+  }
+  return graph;
+}
+
+py::array arrow_column_to_numpy(const py::object& column) {
+  if (!py::hasattr(column, "to_numpy")) throw std::invalid_argument("expected an Arrow-like column exposing to_numpy()");
+  return py::array::ensure(column.attr("to_numpy")(py::arg("zero_copy_only") = false));
+}
+
+// bindings.cpp:34
+DynamicGraph graph_from_arrow_columns(const py::object& src_column, const py::object& dst_column, bool directed) {
+  U32Array src(arrow_column_to_numpy(src_column));
+  U32Array dst(arrow_column_to_numpy(dst_column));
+  const auto s = src.unchecked<1>();
+  const auto d = dst.unchecked<1>();
+  if (s.shape(0) != d.shape(0)) throw std::invalid_argument("Arrow source and destination columns must have equal length");
+  DynamicGraph graph(0, directed);
+  const auto m = static_cast<std::size_t>(s.shape(0));
+  std::vector<std::uint32_t> sv(m), dv(m);
+  for (std::size_t i = 0; i < m; ++i) {
+    sv[i] = s(static_cast<py::ssize_t>(i));
+    dv[i] = d(static_cast<py::ssize_t>(i));
+  }
+  std::vector<std::uint8_t> a(m, 1);
+  velographx::vgx_check(vgx_graph_apply(graph.handle(), sv.data(), dv.data(), a.data(), m));
+  return graph;
+}
+
+// bindings.cpp:35
+DynamicGraph graph_from_arrow_table(const py::object& table, const std::string& src_name, const std::string& dst_name,
+                                    bool directed) {
+  if (!py::hasattr(table, "column")) throw std::invalid_argument("expected an Arrow-like table exposing column(name)");
+  return graph_from_arrow_columns(table.attr("column")(src_name), table.attr("column")(dst_name), directed);
+}
+
+template <class T>
+py::array_t<T> to_numpy(std::vector<T>&& v) {
+  auto* heap = new std::vector<T>(std::move(v));
+  py::capsule owner(heap, [](void* p) { delete static_cast<std::vector<T>*>(p); });
+  return py::array_t<T>(static_cast<py::ssize_t>(heap->size()), heap->data(), owner);
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_velographx, m) {
+  m.doc() = "VeloGraphX traversal path on AMD Instinct MI355X (gfx950)";
+
+  py::class_<UpdateBatch>(m, "UpdateBatch")
+      .def(py::init<>())
+      .def("add", &UpdateBatch::add, py::arg("u"), py::arg("v"), py::arg("timestamp") = 0)
+      .def("remove", &UpdateBatch::remove, py::arg("u"), py::arg("v"), py::arg("timestamp") = 0)
+      // additions: size and bulk construction from arrays
+      .def("__len__", [](const UpdateBatch& b) { return b.updates.size(); })
+      .def_static(
+          "from_arrays",
+          [](const U32Array& src, const U32Array& dst, const py::array_t<std::uint8_t, py::array::forcecast>& add) {
+            const auto s = src.unchecked<1>();
+            const auto d = dst.unchecked<1>();
+            const auto a = add.unchecked<1>();
+            if (s.shape(0) != d.shape(0) || s.shape(0) != a.shape(0))
+              throw std::invalid_argument("src, dst and add must have equal length");
+            UpdateBatch b;
+            b.updates.reserve(static_cast<std::size_t>(s.shape(0)));
+            for (py::ssize_t i = 0; i < s.shape(0); ++i) b.updates.push_back({s(i), d(i), a(i) != 0, 0});
+            return b;
+          },
+          py::arg("src"), py::arg("dst"), py::arg("add"));
+
+  py::class_<DynamicGraph>(m, "Graph")
+      .def(py::init<std::size_t, bool>(), py::arg("vertices") = 0, py::arg("directed") = false)
+      .def("add_edge", &DynamicGraph::add_edge)
+      .def("remove_edge", &DynamicGraph::remove_edge)
+      .def("apply", &DynamicGraph::apply, py::call_guard<py::gil_scoped_release>())
+      .def("neighbors", &DynamicGraph::neighbors)
+      .def_property_readonly("version", &DynamicGraph::version)
+      .def_property_readonly("vertex_count", &DynamicGraph::vertex_count)
+      .def("compact", &DynamicGraph::compact)
+      // additions
+      .def_property_readonly("directed", &DynamicGraph::directed)
+      .def_property_readonly("edge_count_directed", &DynamicGraph::edge_count_directed)
+      .def("in_neighbors", &DynamicGraph::in_neighbors)
+      .def("has_edge", &DynamicGraph::has_edge)
+      .def("degrees", [](const DynamicGraph& g, bool in_dir) { return to_numpy(g.degrees(in_dir)); },
+           py::arg("in_direction") = false)
+      .def("bulk_load_edges",
+           [](DynamicGraph& g, const U32Array& edges) {
+             const auto info = edges.request();
+             if (info.ndim != 2 || info.shape[1] != 2) throw std::invalid_argument("edge array must have shape (N, 2)");
+             const auto m2 = static_cast<std::size_t>(info.shape[0]);
+             const auto* data = static_cast<const std::uint32_t*>(info.ptr);
+             std::vector<std::uint32_t> s(m2), d(m2);
+             for (std::size_t i = 0; i < m2; ++i) {
+               s[i] = data[2 * i];
+               d[i] = data[2 * i + 1];
+             }
+             py::gil_scoped_release nogil;
+             g.bulk_load_edges(s.data(), d.data(), m2);
+           },
+           py::arg("edges"))
+      .def("bulk_load_arrays",
+           [](DynamicGraph& g, const U32Array& src, const U32Array& dst) {
+             if (src.size() != dst.size()) throw std::invalid_argument("src and dst must have equal length");
+             const auto* s = src.data();
+             const auto* d = dst.data();
+             const auto m2 = static_cast<std::size_t>(src.size());
+             py::gil_scoped_release nogil;
+             g.bulk_load_edges(s, d, m2);
+           },
+           py::arg("src"), py::arg("dst"))
+      .def("apply_arrays",
+           [](DynamicGraph& g, const U32Array& src, const U32Array& dst,
+              const py::array_t<std::uint8_t, py::array::c_style | py::array::forcecast>& add) {
+             if (src.size() != dst.size() || src.size() != add.size())
+               throw std::invalid_argument("src, dst and add must have equal length");
+             const auto* s = src.data();
+             const auto* d = dst.data();
+             const auto* a = add.data();
+             const auto k = static_cast<std::size_t>(src.size());
+             py::gil_scoped_release nogil;
+             velographx::vgx_check(vgx_graph_apply(g.handle(), s, d, a, k));
+           },
+           py::arg("src"), py::arg("dst"), py::arg("add"))
+      .def("export_csr",
+           [](const DynamicGraph& g, bool in_dir) {
+             std::vector<std::uint64_t> off(g.vertex_count() + 1);
+             std::vector<std::uint32_t> adj(g.edge_count_directed());
+             velographx::vgx_check(vgx_graph_export_csr(g.handle(), in_dir ? 1 : 0, off.data(), adj.data()));
+             return py::make_tuple(to_numpy(std::move(off)), to_numpy(std::move(adj)));
+           },
+           py::arg("in_direction") = false)
+      .def("storage",
+           [](const DynamicGraph& g) {
+             auto s = g.storage();
+             py::dict d;
+             d["arena_entries"] = s.arena_entries;
+             d["arena_used"] = s.arena_used;
+             d["tombstones"] = s.tombstones;
+             d["storage_bytes"] = s.storage_bytes;
+             d["relocated_rows"] = s.relocated_rows;
+             return d;
+           })
+      .def_property_readonly("_handle", [](const DynamicGraph& g) { return reinterpret_cast<std::uintptr_t>(g.handle()); });
+
+  py::class_<PageRankValidation>(m, "PageRankValidation")
+      .def_readonly("l1_error", &PageRankValidation::l1_error)
+      .def_readonly("linf_error", &PageRankValidation::linf_error)
+      .def_readonly("local_residual_l1", &PageRankValidation::local_residual_l1)
+      .def_readonly("local_residual_linf", &PageRankValidation::local_residual_linf)
+      .def_readonly("reference_residual_l1", &PageRankValidation::reference_residual_l1)
+      .def_readonly("reference_residual_linf", &PageRankValidation::reference_residual_linf)
+      .def_readonly("reference_iterations", &PageRankValidation::reference_iterations)
+      .def_readonly("reference_converged", &PageRankValidation::reference_converged)
+      .def_readonly("within_tolerance", &PageRankValidation::within_tolerance)
+      .def_readonly("fallback_applied", &PageRankValidation::fallback_applied);
+
+  py::class_<IncrementalBFS>(m, "IncrementalBFS")
+      .def(py::init<DynamicGraph&, VertexId, double>(), py::arg("graph"), py::arg("source"),
+           py::arg("deletion_fallback_fraction") = 0.35, py::keep_alive<1, 2>())
+      .def_property_readonly("distances", [](const IncrementalBFS& x) { return x.distances(); })
+      .def("apply", &IncrementalBFS::apply, py::call_guard<py::gil_scoped_release>())
+      .def("recompute", &IncrementalBFS::recompute, py::call_guard<py::gil_scoped_release>())
+      .def_property_readonly_static("unreachable", [](py::object) { return IncrementalBFS::unreachable; })
+      // additions: diagnostics of bfs.hpp:24-27 and a zero-list-copy view
+      .def_property_readonly("reachable_count", &IncrementalBFS::reachable_count)
+      .def_property_readonly("last_deletion_candidates", &IncrementalBFS::last_deletion_candidates)
+      .def_property_readonly("last_affected_vertices", &IncrementalBFS::last_affected_vertices)
+      .def_property_readonly("last_used_full_recompute", &IncrementalBFS::last_used_full_recompute)
+      .def("distances_numpy", [](const IncrementalBFS& x) { return to_numpy(x.distances()); })
+      .def("apply_arrays",
+           [](IncrementalBFS& b, const U32Array& src, const U32Array& dst,
+              const py::array_t<std::uint8_t, py::array::c_style | py::array::forcecast>& add) {
+             if (src.size() != dst.size() || src.size() != add.size())
+               throw std::invalid_argument("src, dst and add must have equal length");
+             const auto* s = src.data();
+             const auto* d = dst.data();
+             const auto* a = add.data();
+             const auto k = static_cast<std::size_t>(src.size());
+             py::gil_scoped_release nogil;
+             velographx::vgx_check(vgx_bfs_apply(b.handle(), s, d, a, k));
+           },
+           py::arg("src"), py::arg("dst"), py::arg("add"))
+      .def("stats",
+           [](const IncrementalBFS& b) {
+             auto s = b.stats();
+             py::dict d;
+             d["reachable_count"] = s.reachable_count;
+             d["last_deletion_candidates"] = s.last_deletion_candidates;
+             d["last_affected_vertices"] = s.last_affected_vertices;
+             d["last_used_full_recompute"] = s.last_used_full_recompute != 0;
+             d["levels"] = s.levels;
+             d["bottom_up_levels"] = s.bottom_up_levels;
+             d["last_relaxed_vertices"] = s.last_relaxed_vertices;
+             return d;
+           })
+      .def("set_tuning",
+           [](IncrementalBFS& b, double alpha, double beta, std::uint32_t blocks, std::uint32_t levels_per_sync) {
+             vgx_bfs_tuning t{alpha, beta, blocks, levels_per_sync};
+             velographx::vgx_check(vgx_bfs_set_tuning(b.handle(), &t));
+           },
+           py::arg("alpha") = 15.0, py::arg("beta") = 18.0, py::arg("blocks") = 0, py::arg("levels_per_sync") = 8)
+      .def_property_readonly("_handle", [](const IncrementalBFS& b) { return reinterpret_cast<std::uintptr_t>(b.handle()); });
+
+  py::class_<IncrementalPageRank>(m, "IncrementalPageRank")
+      .def(py::init<DynamicGraph&, double, int>(), py::arg("graph"), py::arg("damping") = 0.85,
+           py::arg("precision") = 64, py::keep_alive<1, 2>())
+      .def_property_readonly("values", [](const IncrementalPageRank& x) { return x.values(); })
+      .def("apply", &IncrementalPageRank::apply, py::arg("batch"), py::arg("local_iterations") = 24,
+           py::arg("tol") = 1e-9, py::arg("full_fallback_fraction") = 0.60, py::call_guard<py::gil_scoped_release>())
+      .def("recompute", &IncrementalPageRank::recompute, py::arg("max_iterations") = 200, py::arg("tol") = 1e-12,
+           py::call_guard<py::gil_scoped_release>())
+      .def("validate_against_full", &IncrementalPageRank::validate_against_full,
+           py::arg("reference_max_iterations") = 500, py::arg("reference_tol") = 1e-12, py::arg("l1_tolerance") = 1e-6,
+           py::arg("linf_tolerance") = 1e-7, py::call_guard<py::gil_scoped_release>())
+      .def("apply_validated", &IncrementalPageRank::apply_validated, py::arg("batch"), py::arg("local_iterations") = 64,
+           py::arg("local_tol") = 1e-10, py::arg("full_fallback_fraction") = 0.95,
+           py::arg("reference_max_iterations") = 500, py::arg("reference_tol") = 1e-12, py::arg("l1_tolerance") = 1e-6,
+           py::arg("linf_tolerance") = 1e-7, py::call_guard<py::gil_scoped_release>())
+      .def_property_readonly("last_repaired_vertices", &IncrementalPageRank::last_repaired_vertices)
+      .def_property_readonly("last_repair_iterations", &IncrementalPageRank::last_repair_iterations)
+      .def_property_readonly("last_residual_l1", &IncrementalPageRank::last_residual_l1)
+      .def_property_readonly("last_residual_linf", &IncrementalPageRank::last_residual_linf)
+      .def_property_readonly("last_full_recompute_converged", &IncrementalPageRank::last_full_recompute_converged)
+      // additions
+      .def("values_numpy", [](const IncrementalPageRank& x) { return to_numpy(x.values()); })
+      .def("solve_l1", &IncrementalPageRank::solve_l1, py::arg("max_iterations") = 200, py::arg("l1_tol") = 1e-7,
+           py::arg("warm_start") = false, py::call_guard<py::gil_scoped_release>())
+      .def_property_readonly("last_used_full_recompute", &IncrementalPageRank::last_used_full_recompute)
+      .def_property_readonly("precision", &IncrementalPageRank::precision);
+
+  m.def("from_numpy_edges", &graph_from_numpy_edges, py::arg("edges"), py::arg("directed") = false);
+  m.def("from_scipy_csr", &graph_from_scipy_csr, py::arg("csr"), py::arg("directed") = false);
+  m.def("from_arrow_columns", &graph_from_arrow_columns, py::arg("src"), py::arg("dst"), py::arg("directed") = false);
+  m.def("from_arrow_table", &graph_from_arrow_table, py::arg("table"), py::arg("src") = "src", py::arg("dst") = "dst",
+        py::arg("directed") = false);
+
+  // additions: workload generator and device info
+  m.def(
+      "rmat_edges",
+      [](std::uint32_t scale, std::uint32_t edgefactor, std::uint64_t seed) {
+        const std::size_t mm = static_cast<std::size_t>(edgefactor) << scale;
+        py::array_t<std::uint32_t> s(static_cast<py::ssize_t>(mm)), d(static_cast<py::ssize_t>(mm));
+        auto* sp = s.mutable_data();
+        auto* dp = d.mutable_data();
+        {
+          py::gil_scoped_release nogil;
+          velographx::vgx_check(vgx_rmat_generate(scale, edgefactor, seed, sp, dp));
+        }
+        return py::make_tuple(s, d);
+      },
+      py::arg("scale"), py::arg("edgefactor") = 16, py::arg("seed") = 1);
+  m.def("device_count", &vgx_device_count);
+  m.def("version", &vgx_version);
+}
