@@ -138,8 +138,15 @@ typedef struct {
   uint32_t levels;                 /* BFS levels of the last full recompute */
   uint32_t bottom_up_levels;
   uint64_t last_relaxed_vertices;  /* vertices whose distance changed in repair */
+  /* With profiling on: level-kernel launches and their summed duration
+   * (HIP events on the graph's stream) since the last reset. */
+  uint64_t level_launches;
+  double level_kernel_ms;
 } vgx_bfs_stats;
 vgx_status vgx_bfs_stats_get(vgx_bfs* b, vgx_bfs_stats* out);
+/* Time every level-kernel launch with HIP events (on = 1) and reset the
+ * accumulated launch count and duration. */
+vgx_status vgx_bfs_set_profiling(vgx_bfs* b, int on);
 
 /* Direction-optimizing switch and launch shape (autotuned). */
 typedef struct {

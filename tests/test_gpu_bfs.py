@@ -246,6 +246,25 @@ def test_rmat_s16_full_bfs_and_10_batches(vx, orc):
     assert g.edge_count_directed == mg.edge_count_directed()
 
 
+@pytest.mark.parametrize("fan", [63, 64, 65, 127, 128, 300, 5000])
+def test_top_down_chunk_boundaries(vx, orc, fan):
+    """Frontiers made of one-edge rows: a 64-edge chunk can then start
+    exactly at a row boundary and hold 64 row starts (regression test for
+    the load-balanced search window)."""
+    edges = [(0, i) for i in range(1, fan + 1)] + [(i, fan + i) for i in range(1, fan + 1)]
+    edges += [(fan + i, 2 * fan + i) for i in range(1, fan + 1, 3)]
+    g = vx.Graph(0, True)
+    g.bulk_load_edges(np.array(edges, dtype=np.uint32))
+    mg = orc.ModelGraph(0, True)
+    mg.bulk_load_edges(edges)
+    full, _ = orc.model_bfs(mg, 0)
+    bfs = vx.IncrementalBFS(g, 0)
+    for blocks in (0, 1, 2, 3):
+        bfs.set_tuning(alpha=1e-9, beta=1.0, blocks=blocks, levels_per_sync=2)
+        bfs.recompute()
+        assert np.array_equal(bfs.distances_numpy(), full), blocks
+
+
 def test_direction_switch_levels(vx, orc):
     """Force bottom-up from the first level and top-down only: both agree
     with the oracle."""

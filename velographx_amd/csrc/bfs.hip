@@ -6,11 +6,11 @@
 // bfs_distances in src/algorithms.cpp:11-27.  BFS distances are unique, so
 // every traversal order below yields the reference's vector bit for bit.
 //
-// Full BFS: one kernel launch per level on a fixed grid.  The direction of
-// each level is chosen on the device by the last workgroup of the previous
-// level (Beamer's rule: top-down -> bottom-up when m_f > m_u / alpha,
-// bottom-up -> top-down when n_f < n / beta), so the host only checks for
-// completion every few levels.
+// Full BFS: one kernel launch per level on a fixed grid.  Each workgroup
+// derives the level's direction from the previous level's counters
+// (Beamer's rule: top-down -> bottom-up when m_f > m_u / alpha, bottom-up ->
+// top-down when n_f < n / beta); no workgroup waits for another, and the
+// host only checks for completion every few levels.
 //   top-down : the frontier is a packed queue (vertex, edge prefix); every
 //              wave takes an equal slice of the frontier's edges (load-
 //              balanced search), claims new vertices with one atomicOr on
@@ -43,20 +43,35 @@ extern thread_local std::string t_last_error;
 
 constexpr int kBfsBlock = 256;
 constexpr uint32_t kWavesPerBlock = kBfsBlock / kWave;
+// Bottom-up rows up to this length are scanned by one lane; longer rows by
+// the whole wave.
+constexpr uint32_t kBottomUpShortRow = 32;
+
+// Per-level record, kept in a ring of 4 indexed by level & 3.  Level L's
+// kernel accumulates the frontier of level L+1 into rec[(L+1)&3] (q, found)
+// and workgroup 0 hands over the direction and visited-edge sum it used;
+// rec[(L+2)&3] is cleared for level L+1.  Every workgroup of level L
+// derives the same direction from rec[L&3], so no workgroup waits for
+// another and no grid-wide fence is needed within a level.
+// The atomically updated counters and the leader-written hand-over live in
+// separate 128-byte lines.
+struct alignas(128) LevelRec {
+  unsigned long long q;      // packed (count << 38 | edges) of the queued frontier
+  unsigned long long found;  // frontier size, zero-length rows included
+};
+struct alignas(128) LevelCtl {
+  unsigned long long visited_edges;  // edges of vertices visited before this level
+  uint32_t prev_mode;                // direction used by the previous level
+};
 
 struct BfsState {
-  uint32_t level;
-  uint32_t mode;  // 0 top-down, 1 bottom-up
-  uint32_t done;
-  uint32_t blocks_done;
-  unsigned long long q[2];      // packed frontier counters by level parity
-  unsigned long long found[2];  // discovered vertices (incl. zero-length rows)
+  LevelRec rec[4];
+  LevelCtl ctl[4];
   unsigned long long reached;
-  unsigned long long visited_edges;
   unsigned long long total_edges;
-  uint32_t n;
-  uint32_t levels;
+  uint32_t done_level;  // first level with an empty frontier (~0u while running)
   uint32_t bu_levels;
+  uint32_t n;
   uint32_t pad;
   float alpha;
   float beta;
@@ -109,6 +124,10 @@ struct vgx_bfs {
   vgx::IncState* d_inc = nullptr;
   vgx::IncState* h_inc = nullptr;
   uint32_t epoch = 0;
+  bool profiling = false;
+  std::vector<hipEvent_t> events;
+  uint64_t level_launches = 0;
+  double level_ms = 0.0;
 };
 
 namespace vgx {
@@ -134,44 +153,71 @@ struct LevelArgs {
 
 __global__ void k_bfs_setup(LevelArgs a, uint32_t root, unsigned long long total_edges, float alpha, float beta) {
   BfsState* st = a.st;
-  st->level = 0;
-  st->mode = 0;
-  st->blocks_done = 0;
-  st->q[0] = st->q[1] = 0;
-  st->found[0] = st->found[1] = 0;
+  for (int i = 0; i < 4; ++i) {
+    st->rec[i].q = 0;
+    st->rec[i].found = 0;
+    st->ctl[i].visited_edges = 0;
+    st->ctl[i].prev_mode = 0;
+  }
   st->total_edges = total_edges;
   st->n = a.n;
-  st->levels = 0;
   st->bu_levels = 0;
   st->alpha = alpha;
   st->beta = beta;
-  if (root >= a.n) {
-    st->done = 1;
-    st->reached = 0;
-    st->visited_edges = 0;
-    return;
-  }
-  st->done = 0;
+  st->done_level = ~0u;
+  st->reached = 0;
+  if (root >= a.n) return;  // an empty level-0 frontier ends the traversal
   a.dist[root] = 0;
   a.visited[root >> 6] |= 1ull << (root & 63);
   a.bm[root >> 6] |= 1ull << (root & 63);
-  uint32_t len = a.row[root].y;
-  st->found[0] = 1;
-  st->reached = 1;
-  st->visited_edges = len;
+  const uint32_t len = a.row[root].y;
+  st->rec[0].found = 1;
   if (len) {
     a.queue0[0] = root;
     a.pre0[0] = 0;
-    st->q[0] = (1ull << kEdgeBits) | len;
+    st->rec[0].q = (1ull << kEdgeBits) | len;
   }
 }
 
-__global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a) {
-  __shared__ int s_last;
+__device__ __forceinline__ bool in_frontier(const unsigned long long* bm, uint32_t x) {
+  return !(x & kTomb) && ((bm[x >> 6] >> (x & 63)) & 1ull);
+}
+
+// One BFS level.  `level` is the launch index passed by the host.
+__global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t level) {
   BfsState* st = a.st;
-  if (st->done) return;
-  const uint32_t level = st->level;
-  const uint32_t mode = st->mode;
+  const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
+  // Launches enqueued past the last level must not read the ring: its slots
+  // then hold the counters of earlier levels.
+  if (st->done_level <= level) return;
+  const unsigned long long qc = st->rec[level & 3].q;
+  const unsigned long long nf = st->rec[level & 3].found;
+  if (nf == 0) {
+    if (leader) atomicMin(&st->done_level, level);
+    return;
+  }
+  // Direction of this level, derived identically by every workgroup.
+  const unsigned long long mf = qc & kEdgeMask;
+  const unsigned long long seen = st->ctl[level & 3].visited_edges + mf;
+  uint32_t mode = 0;
+  if (level > 0) {
+    const uint32_t prev = st->ctl[level & 3].prev_mode;
+    if (prev == 0) {
+      const unsigned long long mu = st->total_edges > seen ? st->total_edges - seen : 0;
+      mode = static_cast<double>(mf) * st->alpha > static_cast<double>(mu) ? 1u : 0u;
+    } else {
+      mode = static_cast<double>(nf) * st->beta < static_cast<double>(st->n) ? 0u : 1u;
+    }
+  }
+  LevelRec* rn = &st->rec[(level + 1) & 3];
+  if (leader) {
+    st->ctl[(level + 1) & 3].prev_mode = mode;
+    st->ctl[(level + 1) & 3].visited_edges = seen;
+    st->rec[(level + 2) & 3].q = 0;
+    st->rec[(level + 2) & 3].found = 0;
+    atomicAdd(&st->reached, nf);
+    if (mode) atomicAdd(&st->bu_levels, 1u);
+  }
   const int cur = level & 1, nxt = cur ^ 1;
   const uint32_t words = a.words;
   unsigned long long* bm_cur = a.bm + static_cast<uint64_t>(level % 3) * words;
@@ -186,14 +232,17 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a) {
   uint32_t* qn = nxt ? a.queue1 : a.queue0;
   unsigned long long* pn = nxt ? a.pre1 : a.pre0;
   const uint32_t next_level = level + 1;
+  __shared__ WaveStage s_stage[kWavesPerBlock];
+  __shared__ unsigned long long s_found[kWavesPerBlock];
+  WaveStage& sb = s_stage[threadIdx.x / kWave];
+  uint32_t nbuf = 0;
+  unsigned long long found_local = 0;
 
   if (mode == 0) {
-    const unsigned long long qc = st->q[cur];
     const uint32_t F = static_cast<uint32_t>(qc >> kEdgeBits);
-    const unsigned long long E = qc & kEdgeMask;
     const uint32_t* qcur = cur ? a.queue1 : a.queue0;
     const unsigned long long* pcur = cur ? a.pre1 : a.pre0;
-    lbs_edges(qcur, pcur, F, E, a.row, gw, W,
+    lbs_edges(qcur, pcur, F, mf, a.row, gw, W,
               [&](uint32_t, uint32_t, uint2, uint32_t slot, bool valid) {
                 uint32_t w = valid ? a.arena[slot] : kTomb;
                 bool newly = false;
@@ -211,9 +260,8 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a) {
                   atomicOr(bm_next + (w >> 6), 1ull << (w & 63));
                   wl = a.row[w].y;
                 }
-                unsigned long long nb = __ballot(newly);
-                if (nb && lane == 0) atomicAdd(&st->found[nxt], static_cast<unsigned long long>(__popcll(nb)));
-                wave_push(&st->q[nxt], qn, pn, newly && wl > 0, w, wl);
+                found_local += __popcll(__ballot(newly));
+                stage_push(sb, nbuf, &rn->q, qn, pn, newly && wl > 0, w, wl);
               });
   } else {
     for (uint32_t wi = gw; wi < words; wi += W) {
@@ -222,62 +270,66 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a) {
       const uint32_t v = wi * 64 + lane;
       const bool unv = v < a.n && !((vw >> lane) & 1ull);
       bool found = false;
-      uint32_t len = 0;
-      if (unv) {
-        const uint2 rb = a.in_row[v];
-        len = a.row[v].y;
-        for (uint32_t j = 0; j < rb.y; ++j) {
-          uint32_t x = a.in_arena[rb.x + j];
-          if (x & kTomb) continue;
-          if ((bm_cur[x >> 6] >> (x & 63)) & 1ull) {
+      const uint2 rb = unv ? a.in_row[v] : make_uint2(0u, 0u);
+      const bool long_row = rb.y > kBottomUpShortRow;
+      if (unv && !long_row) {
+        // short rows: one lane each, four independent loads in flight
+        const uint32_t* p = a.in_arena + rb.x;
+        uint32_t j = 0;
+        for (; j + 4 <= rb.y; j += 4) {
+          const uint32_t x0 = p[j], x1 = p[j + 1], x2 = p[j + 2], x3 = p[j + 3];
+          if (in_frontier(bm_cur, x0) | in_frontier(bm_cur, x1) | in_frontier(bm_cur, x2) |
+              in_frontier(bm_cur, x3)) {
             found = true;
             break;
           }
         }
+        if (!found) {
+          for (; j < rb.y; ++j) {
+            if (in_frontier(bm_cur, p[j])) {
+              found = true;
+              break;
+            }
+          }
+        }
+      }
+      // long rows: the whole wave scans them one after another, 64 entries
+      // per step, and stops at the first parent in the frontier
+      unsigned long long lm = __ballot(long_row);
+      while (lm) {
+        const int l = __ffsll(static_cast<long long>(lm)) - 1;
+        lm &= lm - 1;
+        const uint32_t rx = __shfl(rb.x, l, kWave), ry = __shfl(rb.y, l, kWave);
+        bool hit = false;
+        for (uint32_t j = 0; j < ry; j += kWave) {
+          const uint32_t x = j + lane < ry ? a.in_arena[rx + j + lane] : kTomb;
+          if (__ballot(in_frontier(bm_cur, x))) {
+            hit = true;
+            break;
+          }
+        }
+        if (static_cast<int>(lane) == l) found = hit;
       }
       const unsigned long long m = __ballot(found);
       if (m) {
+        const uint32_t len = found ? a.row[v].y : 0u;
         if (lane == 0) {
           bm_next[wi] = m;
           a.visited[wi] = vw | m;
-          atomicAdd(&st->found[nxt], static_cast<unsigned long long>(__popcll(m)));
         }
         if (found) a.dist[v] = next_level;
-        wave_push(&st->q[nxt], qn, pn, found && len > 0, v, len);
+        found_local += __popcll(m);
+        stage_push(sb, nbuf, &rn->q, qn, pn, found && len > 0, v, len);
       }
     }
   }
-
+  stage_flush(sb, nbuf, &rn->q, qn, pn);
+  if (lane == 0) s_found[threadIdx.x / kWave] = found_local;
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
-    s_last = atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (s_last && threadIdx.x == 0) {
-    __threadfence();
-    const unsigned long long qv = atomic_load_u64(&st->q[nxt]);
-    const unsigned long long fv = atomic_load_u64(&st->found[nxt]);
-    const unsigned long long mf = qv & kEdgeMask;
-    st->reached += fv;
-    st->visited_edges += mf;
-    st->levels += 1;
-    if (mode == 1) st->bu_levels += 1;
-    uint32_t next_mode = mode;
-    if (fv == 0) {
-      st->done = 1;
-    } else if (mode == 0) {
-      const unsigned long long mu = st->total_edges > st->visited_edges ? st->total_edges - st->visited_edges : 0;
-      if (static_cast<double>(mf) * st->alpha > static_cast<double>(mu)) next_mode = 1;
-    } else {
-      if (static_cast<double>(fv) * st->beta < static_cast<double>(st->n)) next_mode = 0;
-    }
-    st->mode = next_mode;
-    st->q[cur] = 0;
-    st->found[cur] = 0;
-    st->level = next_level;
-    st->blocks_done = 0;
-    __threadfence();
+    unsigned long long f = 0;
+    for (uint32_t i = 0; i < kWavesPerBlock; ++i) f += s_found[i];
+    if (f) atomicAdd(&rn->found, f);
   }
 }
 
@@ -684,22 +736,40 @@ static void bfs_full(vgx_bfs* b) {
     uint64_t waves = std::max<uint64_t>(words, g->adj[0].stored / 512);
     blocks = static_cast<unsigned>(std::min<uint64_t>(2048, std::max<uint64_t>(1, (waves + 3) / 4)));
   }
+  // The first chunk covers the depth of the previous traversal (one launch
+  // beyond it detects completion); later chunks shrink.
   uint32_t chunk = std::max<uint32_t>(1, b->tune.levels_per_sync);
+  if (b->levels) chunk = std::max<uint32_t>(chunk, b->levels + 1);
   uint32_t launched = 0;
   for (;;) {
+    if (b->profiling && b->events.size() < 2 * static_cast<size_t>(chunk)) {
+      size_t old = b->events.size();
+      b->events.resize(2 * static_cast<size_t>(chunk));
+      for (size_t i = old; i < b->events.size(); ++i) VGX_HIP(hipEventCreate(&b->events[i]));
+    }
     for (uint32_t i = 0; i < chunk; ++i) {
-      k_bfs_level<<<blocks, kBfsBlock, 0, s>>>(a);
+      if (b->profiling) VGX_HIP(hipEventRecord(b->events[2 * i], s));
+      k_bfs_level<<<blocks, kBfsBlock, 0, s>>>(a, launched + i);
       VGX_LAUNCH_CHECK();
+      if (b->profiling) VGX_HIP(hipEventRecord(b->events[2 * i + 1], s));
     }
     launched += chunk;
     VGX_HIP(hipMemcpyAsync(b->h_st, b->d_st, sizeof(BfsState), hipMemcpyDeviceToHost, s));
     VGX_HIP(hipStreamSynchronize(s));
-    if (b->h_st->done) break;
+    if (b->profiling) {
+      for (uint32_t i = 0; i < chunk; ++i) {
+        float ms = 0.f;
+        VGX_HIP(hipEventElapsedTime(&ms, b->events[2 * i], b->events[2 * i + 1]));
+        b->level_ms += ms;
+      }
+      b->level_launches += chunk;
+    }
+    if (b->h_st->done_level != ~0u) break;
     if (launched > n + 2) throw Error(VGX_DEVICE_ERROR, "BFS did not terminate");
     chunk = std::max<uint32_t>(2, chunk / 2);
   }
   b->reachable = b->h_st->reached;
-  b->levels = b->h_st->levels;
+  b->levels = b->h_st->done_level;
   b->bu_levels = b->h_st->bu_levels;
 }
 
@@ -908,6 +978,7 @@ vgx_status vgx_bfs_destroy(vgx_bfs* b) {
   if (b->d_inc) dev_free(b->d_inc);
   if (b->h_st) (void)hipHostFree(b->h_st);
   if (b->h_inc) (void)hipHostFree(b->h_inc);
+  for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
   delete b;
   VGX_API_END
 }
@@ -959,6 +1030,17 @@ vgx_status vgx_bfs_stats_get(vgx_bfs* b, vgx_bfs_stats* out) {
   out->levels = b->levels;
   out->bottom_up_levels = b->bu_levels;
   out->last_relaxed_vertices = b->last_relaxed;
+  out->level_launches = b->level_launches;
+  out->level_kernel_ms = b->level_ms;
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_set_profiling(vgx_bfs* b, int on) {
+  VGX_API_BEGIN
+  if (!b) throw Error(VGX_INVALID_ARGUMENT, "null bfs");
+  b->profiling = on != 0;
+  b->level_launches = 0;
+  b->level_ms = 0.0;
   VGX_API_END
 }
 

@@ -83,13 +83,16 @@ class Mt19937 {
     idx_ = kN;
   }
 
+  static uint32_t mix(uint32_t a, uint32_t b, uint32_t c) {
+    const uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+    return c ^ (y >> 1) ^ ((0u - (y & 1u)) & 0x9908b0dfu);
+  }
+
   void twist() {
-    for (int i = 0; i < kN; ++i) {
-      uint32_t y = (mt_[i] & 0x80000000u) | (mt_[(i + 1) % kN] & 0x7fffffffu);
-      uint32_t v = mt_[(i + kM) % kN] ^ (y >> 1);
-      if (y & 1u) v ^= 0x9908b0dfu;
-      mt_[i] = v;
-    }
+    int i = 0;
+    for (; i < kN - kM; ++i) mt_[i] = mix(mt_[i], mt_[i + 1], mt_[i + kM]);
+    for (; i < kN - 1; ++i) mt_[i] = mix(mt_[i], mt_[i + 1], mt_[i + kM - kN]);
+    mt_[kN - 1] = mix(mt_[kN - 1], mt_[0], mt_[kM - 1]);
     idx_ = 0;
   }
 
@@ -114,16 +117,13 @@ extern "C" vgx_status vgx_rmat_generate(uint32_t scale, uint32_t edgefactor, uin
   for (uint64_t e = 0; e < m; ++e) {
     uint64_t u = 0, v = 0;
     for (uint64_t step = n >> 1; step; step >>= 1) {
+      // quadrant A: none, B (t0 <= r < t1): v, C (t1 <= r < t2): u, D: both;
+      // written branch-free because the outcome is unpredictable.
       const double r = rng.random();
-      if (r < t0) {
-      } else if (r < t1) {
-        v += step;
-      } else if (r < t2) {
-        u += step;
-      } else {
-        u += step;
-        v += step;
-      }
+      const uint64_t bu = r >= t1;
+      const uint64_t bv = static_cast<uint64_t>(r >= t0) & static_cast<uint64_t>(r < t1 || r >= t2);
+      u += step & (0 - bu);
+      v += step & (0 - bv);
     }
     src[e] = static_cast<uint32_t>(u);
     dst[e] = static_cast<uint32_t>(v);

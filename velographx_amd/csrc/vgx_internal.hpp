@@ -263,26 +263,90 @@ __device__ __forceinline__ void lbs_edges(const uint32_t* queue, const unsigned 
     if (pre[mid] <= a) lo = mid;
     else hi = mid;
   }
+  // Invariant: item i0 contains edge e (pre[i0] <= e < pre[i0 + 1]).
   uint32_t i0 = lo;
   const unsigned lane = lane_id();
   for (unsigned long long e = a; e < b; e += kWave) {
-    unsigned long long g = e + lane;
-    bool valid = g < b;
-    uint32_t l = i0, h = (F - i0 > static_cast<uint32_t>(kWave) + 1) ? i0 + kWave + 1 : F;
-    if (valid) {
-      while (h - l > 1) {
-        uint32_t mid = (l + h) >> 1;
-        if (pre[mid] <= g) l = mid;
-        else h = mid;
-      }
-    }
-    uint32_t v = valid ? queue[l] : 0u;
-    uint2 rb = valid ? row[v] : make_uint2(0u, 0u);
-    uint32_t slot = valid ? rb.x + static_cast<uint32_t>(g - pre[l]) : 0u;
-    fn(l, v, rb, slot, valid);
-    i0 = __shfl(l, kWave - 1, kWave);
+    // Lane k holds item i0 + k of the window.  Every item has at least one
+    // edge and item i0 + 1 starts after e, so the items starting inside this
+    // 64-edge chunk are among the next 63; their start offsets, OR-ed into
+    // one mask, give each lane its item with a popcount instead of a search.
+    const uint32_t idx = i0 + lane;
+    const bool in_q = idx < F;
+    const unsigned long long pk = in_q ? pre[idx] : ~0ull;
+    const uint32_t qk = in_q ? queue[idx] : 0u;
+    unsigned long long bit = (lane > 0 && pk < e + kWave) ? (1ull << (pk - e)) : 0ull;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) bit |= __shfl_xor(bit, d, kWave);
+    const unsigned long long g = e + lane;
+    const bool valid = g < b;
+    const uint32_t k = __popcll(bit & ((2ull << lane) - 1ull));
+    const uint32_t v = __shfl(qk, static_cast<int>(k), kWave);
+    const unsigned long long pv = __shfl(pk, static_cast<int>(k), kWave);
+    const uint2 rb = valid ? row[v] : make_uint2(0u, 0u);
+    const uint32_t slot = valid ? rb.x + static_cast<uint32_t>(g - pv) : 0u;
+    fn(i0 + k, v, rb, slot, valid);
     if (!__shfl(static_cast<int>(valid), kWave - 1, kWave)) break;
+    // advance to the item containing edge e + 64: items of the window that
+    // start at or before it, plus item i0 + 64 when all 63 others do
+    uint32_t adv = __popcll(__ballot(lane > 0 && pk <= e + kWave));
+    if (adv == kWave - 1 && i0 + kWave < F && pre[i0 + kWave] <= e + kWave) adv = kWave;
+    i0 += adv;
   }
+}
+
+// Per-wave staging of frontier pushes in LDS: pushes land in the wave's
+// buffer and one packed atomic per flush reserves their queue slots and
+// edge prefixes, instead of one atomic on a hot counter per push.
+constexpr uint32_t kStage = 512;
+struct WaveStage {
+  uint32_t v[kStage];
+  uint32_t len[kStage];
+};
+
+__device__ __forceinline__ void stage_flush(WaveStage& sb, uint32_t& nbuf, unsigned long long* ctr,
+                                            uint32_t* queue, unsigned long long* pre) {
+  if (nbuf == 0) return;
+  __builtin_amdgcn_wave_barrier();
+  const unsigned lane = lane_id();
+  unsigned long long total = 0;
+  for (uint32_t c = 0; c < nbuf; c += kWave) {
+    const uint32_t l = c + lane < nbuf ? sb.len[c + lane] : 0u;
+    total += wave_sum<unsigned long long>(l);
+  }
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(ctr, (static_cast<unsigned long long>(nbuf) << kEdgeBits) + total);
+  base = __shfl(base, 0, kWave);
+  unsigned long long slot = base >> kEdgeBits;
+  unsigned long long eoff = base & kEdgeMask;
+  for (uint32_t c = 0; c < nbuf; c += kWave) {
+    const bool in = c + lane < nbuf;
+    const uint32_t l = in ? sb.len[c + lane] : 0u;
+    const unsigned long long inc = wave_inclusive_scan<unsigned long long>(l);
+    if (in) {
+      queue[slot + c + lane] = sb.v[c + lane];
+      pre[slot + c + lane] = eoff + inc - l;
+    }
+    eoff += __shfl(inc, kWave - 1, kWave);
+  }
+  __builtin_amdgcn_wave_barrier();
+  nbuf = 0;
+}
+
+// Stage (v, len) for the lanes that want it; flushes when nearly full.
+// Called by all lanes of the wave.
+__device__ __forceinline__ void stage_push(WaveStage& sb, uint32_t& nbuf, unsigned long long* ctr,
+                                           uint32_t* queue, unsigned long long* pre, bool want, uint32_t v,
+                                           uint32_t len) {
+  const unsigned long long ball = __ballot(want);
+  if (!ball) return;
+  if (want) {
+    const uint32_t p = nbuf + lanes_below(ball);
+    sb.v[p] = v;
+    sb.len[p] = len;
+  }
+  nbuf += __popcll(ball);
+  if (nbuf > kStage - kWave) stage_flush(sb, nbuf, ctr, queue, pre);
 }
 
 }  // namespace vgx

@@ -271,8 +271,13 @@ PYBIND11_MODULE(_velographx, m) {
              d["levels"] = s.levels;
              d["bottom_up_levels"] = s.bottom_up_levels;
              d["last_relaxed_vertices"] = s.last_relaxed_vertices;
+             d["level_launches"] = s.level_launches;
+             d["level_kernel_ms"] = s.level_kernel_ms;
              return d;
            })
+      .def("set_profiling",
+           [](IncrementalBFS& b, bool on) { velographx::vgx_check(vgx_bfs_set_profiling(b.handle(), on ? 1 : 0)); },
+           py::arg("on") = true)
       .def("set_tuning",
            [](IncrementalBFS& b, double alpha, double beta, std::uint32_t blocks, std::uint32_t levels_per_sync) {
              vgx_bfs_tuning t{alpha, beta, blocks, levels_per_sync};

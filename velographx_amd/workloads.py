@@ -74,9 +74,12 @@ def canonical_edges(src, dst, directed: bool):
 class BatchStream:
     """Random update batches over a live edge set kept on the host.
 
-    Membership of an edge is tested against the sorted initial key array
-    (minus deleted keys) plus the set of keys inserted since; deletions pick
-    uniformly among the live edges (swap-remove on a dense edge array).
+    Membership is tested against the sorted initial key array minus the
+    sorted array of deleted keys, plus the sorted array of keys inserted
+    since.  Inserts are non-edges of the pre-batch graph, deletions are live
+    edges of the pre-batch graph (swap-remove on a dense edge array), so the
+    two sets are disjoint and the shuffled batch order cannot change the
+    final state of any edge.
     """
 
     def __init__(self, u, v, n: int, directed: bool, seed: int = 1):
@@ -87,74 +90,73 @@ class BatchStream:
         self.ev = np.array(v, dtype=np.uint32)
         self.size = self.eu.size
         self.base = np.sort((self.eu.astype(np.uint64) << np.uint64(32)) | self.ev.astype(np.uint64))
-        self.deleted = set()
-        self.inserted = set()
+        self.deleted = np.zeros(0, np.uint64)
+        self.inserted = np.zeros(0, np.uint64)
 
-    def _key(self, u, v):
-        if not self.directed and v < u:
-            u, v = v, u
-        return (int(u) << 32) | int(v)
+    def _keys(self, a, b):
+        a = a.astype(np.uint64)
+        b = b.astype(np.uint64)
+        if not self.directed:
+            a, b = np.minimum(a, b), np.maximum(a, b)
+        return (a << np.uint64(32)) | b
 
-    def _present(self, key):
-        if key in self.inserted:
-            return True
-        if key in self.deleted:
-            return False
-        i = np.searchsorted(self.base, np.uint64(key))
-        return bool(i < self.base.size and int(self.base[i]) == key)
-
-    def _push(self, u, v):
-        if self.size == self.eu.size:
-            grow = max(1024, self.size // 2)
-            self.eu = np.concatenate([self.eu, np.zeros(grow, np.uint32)])
-            self.ev = np.concatenate([self.ev, np.zeros(grow, np.uint32)])
-        self.eu[self.size] = u
-        self.ev[self.size] = v
-        self.size += 1
+    def _present(self, keys):
+        i = np.minimum(np.searchsorted(self.base, keys), max(self.base.size - 1, 0))
+        in_base = self.base[i] == keys if self.base.size else np.zeros(keys.size, bool)
+        return np.isin(keys, self.inserted) | (in_base & ~np.isin(keys, self.deleted))
 
     def next(self, k: int, insert_fraction: float = 0.8):
         n_ins = int(round(k * insert_fraction))
         n_del = min(k - n_ins, self.size)
-        src, dst, add = [], [], []
-        # inserts: uniform random non-edges u != v
-        while len(src) < n_ins:
-            need = (n_ins - len(src)) * 2 + 16
-            cu = self.rng.below(need, self.n)
-            cv = self.rng.below(need, self.n)
-            for a, b in zip(cu.tolist(), cv.tolist()):
-                if a == b:
-                    continue
-                key = self._key(a, b)
-                if self._present(key):
-                    continue
-                self.inserted.add(key)
-                self.deleted.discard(key)
-                self._push(min(a, b) if not self.directed else a, max(a, b) if not self.directed else b)
-                src.append(a)
-                dst.append(b)
-                add.append(1)
-                if len(src) == n_ins:
-                    break
-        # deletes: uniform over the live edges (after this batch's inserts)
-        picks = self.rng.below(n_del, 1 << 62)
-        for r in picks.tolist():
-            if self.size == 0:
-                break
+        # inserts: uniform random non-edges u != v of the pre-batch graph
+        ins_u = np.zeros(0, np.uint32)
+        ins_v = np.zeros(0, np.uint32)
+        ins_k = np.zeros(0, np.uint64)
+        while ins_u.size < n_ins:
+            need = (n_ins - ins_u.size) * 2 + 16
+            cu = self.rng.below(need, self.n).astype(np.uint32)
+            cv = self.rng.below(need, self.n).astype(np.uint32)
+            ok = cu != cv
+            cu, cv = cu[ok], cv[ok]
+            ck = self._keys(cu, cv)
+            ok = ~self._present(ck) & ~np.isin(ck, ins_k)
+            cu, cv, ck = cu[ok], cv[ok], ck[ok]
+            _, first = np.unique(ck, return_index=True)
+            first = np.sort(first)[: n_ins - ins_u.size]
+            ins_u = np.concatenate([ins_u, cu[first]])
+            ins_v = np.concatenate([ins_v, cv[first]])
+            ins_k = np.concatenate([ins_k, ck[first]])
+        # deletes: uniform over the live edges of the pre-batch graph
+        del_u = np.zeros(n_del, np.uint32)
+        del_v = np.zeros(n_del, np.uint32)
+        picks = self.rng.below(n_del, 1 << 62).tolist()
+        for j, r in enumerate(picks):
             i = r % self.size
-            a, b = int(self.eu[i]), int(self.ev[i])
+            del_u[j], del_v[j] = self.eu[i], self.ev[i]
             self.size -= 1
             self.eu[i] = self.eu[self.size]
             self.ev[i] = self.ev[self.size]
-            key = self._key(a, b)
-            self.inserted.discard(key)
-            self.deleted.add(key)
-            src.append(a)
-            dst.append(b)
-            add.append(0)
+        del_k = self._keys(del_u, del_v)
+        # commit: live array, inserted / deleted key sets
+        grow = self.size + ins_u.size - self.eu.size
+        if grow > 0:
+            extra = max(grow, 1 << 16)
+            self.eu = np.concatenate([self.eu, np.zeros(extra, np.uint32)])
+            self.ev = np.concatenate([self.ev, np.zeros(extra, np.uint32)])
+        lo = np.minimum(ins_u, ins_v) if not self.directed else ins_u
+        hi = np.maximum(ins_u, ins_v) if not self.directed else ins_v
+        self.eu[self.size:self.size + ins_u.size] = lo
+        self.ev[self.size:self.size + ins_u.size] = hi
+        self.size += ins_u.size
+        was_inserted = np.isin(del_k, self.inserted)
+        self.inserted = np.union1d(np.setdiff1d(self.inserted, del_k[was_inserted]), ins_k)
+        self.deleted = np.union1d(np.setdiff1d(self.deleted, ins_k), del_k[~was_inserted])
+        src = np.concatenate([ins_u, del_u])
+        dst = np.concatenate([ins_v, del_v])
+        add = np.concatenate([np.ones(ins_u.size, np.uint8), np.zeros(n_del, np.uint8)])
         # interleave inserts and deletes deterministically
-        order = np.argsort(self.rng.draw(len(src)), kind="stable")
-        return (np.asarray(src, dtype=np.uint32)[order], np.asarray(dst, dtype=np.uint32)[order],
-                np.asarray(add, dtype=np.uint8)[order])
+        order = np.argsort(self.rng.draw(src.size), kind="stable")
+        return src[order], dst[order], add[order]
 
 
 def log_uniform_sizes(count: int, lo: float = 1.0, hi: float = 5.0, seed: int = 1):
