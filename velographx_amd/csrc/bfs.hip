@@ -43,9 +43,9 @@ extern thread_local std::string t_last_error;
 
 constexpr int kBfsBlock = 256;
 constexpr uint32_t kWavesPerBlock = kBfsBlock / kWave;
-// Bottom-up rows up to this length are scanned by one lane; longer rows by
-// the whole wave.
-constexpr uint32_t kBottomUpShortRow = 32;
+// Bottom-up phase A checks the first kHead parents of a vertex from the
+// dense head array; the rest of longer rows is scanned by phase B.
+constexpr uint32_t kHead = 4;
 
 // Per-level record, kept in a ring of 4 indexed by level & 3.  Level L's
 // kernel accumulates the frontier of level L+1 into rec[(L+1)&3] (q, found)
@@ -64,9 +64,15 @@ struct alignas(128) LevelCtl {
   uint32_t prev_mode;                // direction used by the previous level
 };
 
+// Packed counter of the bottom-up pending list of a level (same ring).
+struct alignas(128) PendCtr {
+  unsigned long long q;
+};
+
 struct BfsState {
   LevelRec rec[4];
   LevelCtl ctl[4];
+  PendCtr pend[4];
   unsigned long long reached;
   unsigned long long total_edges;
   uint32_t done_level;  // first level with an empty frontier (~0u while running)
@@ -99,6 +105,10 @@ struct BfsWork {
   DevBuf<unsigned long long> bm;  // 3 bitmaps
   DevBuf<uint32_t> queue[2];
   DevBuf<unsigned long long> pre[2];
+  DevBuf<uint32_t> pend;
+  DevBuf<unsigned long long> ppre;
+  DevBuf<uint4> head;
+  uint64_t head_version = ~0ull, head_n = 0;
   // incremental
   DevBuf<uint32_t> lost, support, flags, stamp;
   DevBuf<uint32_t> tlist, tflist, alist, aall, tall, slist;
@@ -146,10 +156,26 @@ struct LevelArgs {
   uint32_t* queue1;
   unsigned long long* pre0;
   unsigned long long* pre1;
+  uint32_t* pend;            // bottom-up rows left for phase B
+  unsigned long long* ppre;
   BfsState* st;
   uint32_t n;
   uint32_t words;
+  const uint4* head;         // first kHead in-neighbours of every vertex
 };
+
+// head[v] = the first kHead entries of v's in-row (tombstone bits kept;
+// missing entries are tombstones).
+__global__ void k_build_head(const uint2* __restrict__ in_row, const uint32_t* __restrict__ in_arena, uint32_t n,
+                             uint4* __restrict__ head) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint2 rb = in_row[v];
+    uint32_t h[kHead];
+#pragma unroll
+    for (uint32_t t = 0; t < kHead; ++t) h[t] = t < rb.y ? in_arena[rb.x + t] : kTomb;
+    head[v] = make_uint4(h[0], h[1], h[2], h[3]);
+  }
+}
 
 __global__ void k_bfs_setup(LevelArgs a, uint32_t root, unsigned long long total_edges, float alpha, float beta) {
   BfsState* st = a.st;
@@ -158,6 +184,7 @@ __global__ void k_bfs_setup(LevelArgs a, uint32_t root, unsigned long long total
     st->rec[i].found = 0;
     st->ctl[i].visited_edges = 0;
     st->ctl[i].prev_mode = 0;
+    st->pend[i].q = 0;
   }
   st->total_edges = total_edges;
   st->n = a.n;
@@ -215,6 +242,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
     st->ctl[(level + 1) & 3].visited_edges = seen;
     st->rec[(level + 2) & 3].q = 0;
     st->rec[(level + 2) & 3].found = 0;
+    st->pend[(level + 2) & 3].q = 0;
     atomicAdd(&st->reached, nf);
     if (mode) atomicAdd(&st->bu_levels, 1u);
   }
@@ -233,9 +261,11 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
   unsigned long long* pn = nxt ? a.pre1 : a.pre0;
   const uint32_t next_level = level + 1;
   __shared__ WaveStage s_stage[kWavesPerBlock];
+  __shared__ WaveStage s_stage2[kWavesPerBlock];
   __shared__ unsigned long long s_found[kWavesPerBlock];
   WaveStage& sb = s_stage[threadIdx.x / kWave];
-  uint32_t nbuf = 0;
+  WaveStage& sb2 = s_stage2[threadIdx.x / kWave];
+  uint32_t nbuf = 0, nbuf2 = 0;
   unsigned long long found_local = 0;
 
   if (mode == 0) {
@@ -264,52 +294,26 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
                 stage_push(sb, nbuf, &rn->q, qn, pn, newly && wl > 0, w, wl);
               });
   } else {
+    // Phase A: one lane per vertex checks the first kHead parents, read from
+    // the dense head array (16 bytes per vertex, coalesced across the wave)
+    // instead of the row itself.  Rows are sorted by id, so these are the
+    // vertex's lowest-id, on R-MAT highest-degree, parents.
     for (uint32_t wi = gw; wi < words; wi += W) {
       const unsigned long long vw = a.visited[wi];
       if (vw == ~0ull) continue;
       const uint32_t v = wi * 64 + lane;
       const bool unv = v < a.n && !((vw >> lane) & 1ull);
-      bool found = false;
-      const uint2 rb = unv ? a.in_row[v] : make_uint2(0u, 0u);
-      const bool long_row = rb.y > kBottomUpShortRow;
-      if (unv && !long_row) {
-        // short rows: one lane each, four independent loads in flight
-        const uint32_t* p = a.in_arena + rb.x;
-        uint32_t j = 0;
-        for (; j + 4 <= rb.y; j += 4) {
-          const uint32_t x0 = p[j], x1 = p[j + 1], x2 = p[j + 2], x3 = p[j + 3];
-          if (in_frontier(bm_cur, x0) | in_frontier(bm_cur, x1) | in_frontier(bm_cur, x2) |
-              in_frontier(bm_cur, x3)) {
-            found = true;
-            break;
-          }
-        }
-        if (!found) {
-          for (; j < rb.y; ++j) {
-            if (in_frontier(bm_cur, p[j])) {
-              found = true;
-              break;
-            }
-          }
-        }
+      uint4 h = make_uint4(kTomb, kTomb, kTomb, kTomb);
+      uint32_t len_in = 0;
+      if (unv) {
+        h = a.head[v];
+        len_in = a.in_row[v].y;
       }
-      // long rows: the whole wave scans them one after another, 64 entries
-      // per step, and stops at the first parent in the frontier
-      unsigned long long lm = __ballot(long_row);
-      while (lm) {
-        const int l = __ffsll(static_cast<long long>(lm)) - 1;
-        lm &= lm - 1;
-        const uint32_t rx = __shfl(rb.x, l, kWave), ry = __shfl(rb.y, l, kWave);
-        bool hit = false;
-        for (uint32_t j = 0; j < ry; j += kWave) {
-          const uint32_t x = j + lane < ry ? a.in_arena[rx + j + lane] : kTomb;
-          if (__ballot(in_frontier(bm_cur, x))) {
-            hit = true;
-            break;
-          }
-        }
-        if (static_cast<int>(lane) == l) found = hit;
-      }
+      const bool found = in_frontier(bm_cur, h.x) | in_frontier(bm_cur, h.y) | in_frontier(bm_cur, h.z) |
+                         in_frontier(bm_cur, h.w);
+      // the rest of longer rows goes to phase B (k_bfs_bottom_up_tail)
+      stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, unv && !found && len_in > kHead, v,
+                 len_in - kHead);
       const unsigned long long m = __ballot(found);
       if (m) {
         const uint32_t len = found ? a.row[v].y : 0u;
@@ -323,6 +327,62 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
       }
     }
   }
+  stage_flush(sb, nbuf, &rn->q, qn, pn);
+  stage_flush(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre);
+  if (lane == 0) s_found[threadIdx.x / kWave] = found_local;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long f = 0;
+    for (uint32_t i = 0; i < kWavesPerBlock; ++i) f += s_found[i];
+    if (f) atomicAdd(&rn->found, f);
+  }
+}
+
+// Bottom-up phase B of a level: the parents of pending rows beyond the first
+// kBottomUpHead are scanned edge-balanced and coalesced; a parent in the
+// frontier claims the row's vertex through the visited bitmap.  Exits at
+// once when the level ran top-down or left nothing pending.
+__global__ __launch_bounds__(kBfsBlock) void k_bfs_bottom_up_tail(LevelArgs a, uint32_t level) {
+  BfsState* st = a.st;
+  if (st->done_level <= level) return;
+  if (st->rec[level & 3].found == 0) return;
+  if (st->ctl[(level + 1) & 3].prev_mode != 1) return;
+  const unsigned long long pq = st->pend[level & 3].q;
+  if (pq == 0) return;
+  __shared__ WaveStage s_stage[kWavesPerBlock];
+  __shared__ unsigned long long s_found[kWavesPerBlock];
+  WaveStage& sb = s_stage[threadIdx.x / kWave];
+  uint32_t nbuf = 0;
+  unsigned long long found_local = 0;
+  const uint32_t words = a.words;
+  const unsigned long long* bm_cur = a.bm + static_cast<uint64_t>(level % 3) * words;
+  unsigned long long* bm_next = a.bm + static_cast<uint64_t>((level + 1) % 3) * words;
+  LevelRec* rn = &st->rec[(level + 1) & 3];
+  const int nxt = (level & 1) ^ 1;
+  uint32_t* qn = nxt ? a.queue1 : a.queue0;
+  unsigned long long* pn = nxt ? a.pre1 : a.pre0;
+  const uint32_t next_level = level + 1;
+  const uint32_t W = gridDim.x * kWavesPerBlock;
+  const uint32_t gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  lbs_edges(a.pend, a.ppre, static_cast<uint32_t>(pq >> kEdgeBits), pq & kEdgeMask, a.in_row, gw, W,
+            [&](uint32_t, uint32_t v, uint2, uint32_t slot, bool valid) {
+              const uint32_t x = valid ? a.in_arena[slot + kHead] : kTomb;
+              bool newly = false;
+              uint32_t len = 0;
+              if (in_frontier(bm_cur, x)) {
+                const unsigned long long bit = 1ull << (v & 63);
+                unsigned long long* wp = a.visited + (v >> 6);
+                if (!(*wp & bit)) newly = !(atomicOr(wp, bit) & bit);
+              }
+              if (newly) {
+                a.dist[v] = next_level;
+                atomicOr(bm_next + (v >> 6), 1ull << (v & 63));
+                len = a.row[v].y;
+              }
+              found_local += __popcll(__ballot(newly));
+              stage_push(sb, nbuf, &rn->q, qn, pn, newly && len > 0, v, len);
+            });
   stage_flush(sb, nbuf, &rn->q, qn, pn);
   if (lane == 0) s_found[threadIdx.x / kWave] = found_local;
   __syncthreads();
@@ -667,6 +727,10 @@ static void ensure_work(vgx_bfs* b) {
     w.queue[i].ensure(c);
     w.pre[i].ensure(c);
   }
+  {
+    w.pend.ensure(c);
+    w.ppre.ensure(c);
+  }
   // incremental workspace: zero-initialised counters and flags
   uint64_t old = w.lost.n ? w.cap : 0;
   w.lost.grow_copy(c, old, g->stream);
@@ -709,9 +773,12 @@ static LevelArgs level_args(vgx_bfs* b) {
   a.queue1 = b->w.queue[1].p;
   a.pre0 = b->w.pre[0].p;
   a.pre1 = b->w.pre[1].p;
+  a.pend = b->w.pend.p;
+  a.ppre = b->w.ppre.p;
   a.st = b->d_st;
   a.n = static_cast<uint32_t>(g->n);
   a.words = static_cast<uint32_t>((g->n + 63) / 64);
+  a.head = b->w.head.p;
   return a;
 }
 
@@ -722,6 +789,17 @@ static void bfs_full(vgx_bfs* b) {
   const uint64_t n = g->n;
   b->dist_len = n;
   const uint64_t words = (n + 63) / 64;
+  // the dense head array follows the graph's version
+  if (b->w.head_version != g->version || b->w.head_n != n) {
+    b->w.head.ensure(g->ncap + 1);
+    if (n) {
+      k_build_head<<<blocks_for(n, kBfsBlock, 4096), kBfsBlock, 0, s>>>(g->in().row.p, g->in().arena.p,
+                                                                       static_cast<uint32_t>(n), b->w.head.p);
+      VGX_LAUNCH_CHECK();
+    }
+    b->w.head_version = g->version;
+    b->w.head_n = n;
+  }
   if (n) {
     VGX_HIP(hipMemsetAsync(b->w.dist.p, 0xFF, n * sizeof(uint32_t), s));
     VGX_HIP(hipMemsetAsync(b->w.visited.p, 0, words * 8, s));
@@ -750,6 +828,8 @@ static void bfs_full(vgx_bfs* b) {
     for (uint32_t i = 0; i < chunk; ++i) {
       if (b->profiling) VGX_HIP(hipEventRecord(b->events[2 * i], s));
       k_bfs_level<<<blocks, kBfsBlock, 0, s>>>(a, launched + i);
+      VGX_LAUNCH_CHECK();
+      k_bfs_bottom_up_tail<<<blocks, kBfsBlock, 0, s>>>(a, launched + i);
       VGX_LAUNCH_CHECK();
       if (b->profiling) VGX_HIP(hipEventRecord(b->events[2 * i + 1], s));
     }

@@ -298,7 +298,7 @@ __device__ __forceinline__ void lbs_edges(const uint32_t* queue, const unsigned 
 // Per-wave staging of frontier pushes in LDS: pushes land in the wave's
 // buffer and one packed atomic per flush reserves their queue slots and
 // edge prefixes, instead of one atomic on a hot counter per push.
-constexpr uint32_t kStage = 512;
+constexpr uint32_t kStage = 256;
 struct WaveStage {
   uint32_t v[kStage];
   uint32_t len[kStage];

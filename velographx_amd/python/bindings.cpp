@@ -284,6 +284,17 @@ PYBIND11_MODULE(_velographx, m) {
              velographx::vgx_check(vgx_bfs_set_tuning(b.handle(), &t));
            },
            py::arg("alpha") = 15.0, py::arg("beta") = 18.0, py::arg("blocks") = 0, py::arg("levels_per_sync") = 8)
+      .def("get_tuning",
+           [](const IncrementalBFS& b) {
+             vgx_bfs_tuning t{};
+             velographx::vgx_check(vgx_bfs_get_tuning(b.handle(), &t));
+             py::dict d;
+             d["alpha"] = t.alpha;
+             d["beta"] = t.beta;
+             d["blocks"] = t.blocks;
+             d["levels_per_sync"] = t.levels_per_sync;
+             return d;
+           })
       .def_property_readonly("_handle", [](const IncrementalBFS& b) { return reinterpret_cast<std::uintptr_t>(b.handle()); });
 
   py::class_<IncrementalPageRank>(m, "IncrementalPageRank")
