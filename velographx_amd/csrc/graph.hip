@@ -77,7 +77,9 @@ static int bits_for(uint64_t n) {
   return b;
 }
 
-constexpr uint32_t kBigRow = 4096;
+// A row takes its inserts in place on one thread while it stays this short
+// and fits its slack; otherwise a workgroup merges it into a fresh range.
+constexpr uint32_t kBigRow = 32;
 constexpr int kBlock = 256;
 
 static unsigned grid_for(uint64_t items, unsigned per_block = kBlock, unsigned cap = 65535u * 4u) {
@@ -216,7 +218,7 @@ __global__ void k_plan_inserts(const uint32_t* __restrict__ ins, const unsigned 
     segk[j] = k;
     uint32_t len = row[u].y, c = cap[u];
     uint32_t need = len + k;
-    bool is_big = need > kBigRow;
+    bool is_big = need > kBigRow || need > c;
     if (need > c || is_big) {
       uint32_t nc = need + (need >> 2) + 4u;
       atomicAdd(&ctr->need[dir], static_cast<unsigned long long>(nc));
@@ -268,7 +270,7 @@ __global__ void k_insert_small(const uint32_t* __restrict__ ins, const unsigned 
                                uint2* __restrict__ row, uint32_t* __restrict__ cap, uint32_t* __restrict__ arena,
                                uint32_t* __restrict__ deg, GraphCounters* ctr, int dir) {
   const uint64_t nins = *nins_p;
-  long long added = 0, reloc = 0;
+  long long added = 0;
   for (uint64_t j = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; j < nins;
        j += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     uint32_t k = segk[j];
@@ -276,50 +278,28 @@ __global__ void k_insert_small(const uint32_t* __restrict__ ins, const unsigned 
     uint32_t u = static_cast<uint32_t>(skeys[ins[j]] >> 32);
     uint2 rb = row[u];
     uint32_t len = rb.y, c = cap[u];
-    if (len + k > kBigRow) continue;  // merged by k_insert_big
-    if (len + k <= c) {
-      int i = static_cast<int>(len) - 1, t = static_cast<int>(k) - 1, w = static_cast<int>(len + k) - 1;
-      while (t >= 0) {
-        uint32_t vt = static_cast<uint32_t>(skeys[ins[j + t]]);
-        if (i >= 0 && (arena[rb.x + i] & kIdMask) > vt) {
-          arena[rb.x + w] = arena[rb.x + i];
-          --i;
-        } else {
-          arena[rb.x + w] = vt;
-          --t;
-        }
-        --w;
+    if (len + k > kBigRow || len + k > c) continue;  // merged by k_insert_big
+    // backward in-place merge into the row's slack
+    int i = static_cast<int>(len) - 1, t = static_cast<int>(k) - 1, w = static_cast<int>(len + k) - 1;
+    while (t >= 0) {
+      uint32_t vt = static_cast<uint32_t>(skeys[ins[j + t]]);
+      if (i >= 0 && (arena[rb.x + i] & kIdMask) > vt) {
+        arena[rb.x + w] = arena[rb.x + i];
+        --i;
+      } else {
+        arena[rb.x + w] = vt;
+        --t;
       }
-      row[u].y = len + k;
-    } else {
-      uint32_t need = len + k;
-      uint32_t nc = need + (need >> 2) + 4u;
-      uint32_t nb = static_cast<uint32_t>(atomicAdd(&ctr->bump[dir], static_cast<unsigned long long>(nc)));
-      uint32_t i = 0, t = 0, w = 0;
-      while (w < need) {
-        uint32_t vt = t < k ? static_cast<uint32_t>(skeys[ins[j + t]]) : kInf;
-        if (i < len && (arena[rb.x + i] & kIdMask) < vt) {
-          arena[nb + w] = arena[rb.x + i];
-          ++i;
-        } else {
-          arena[nb + w] = vt;
-          ++t;
-        }
-        ++w;
-      }
-      row[u] = make_uint2(nb, need);
-      cap[u] = nc;
-      ++reloc;
+      --w;
     }
+    row[u].y = len + k;
     atomicAdd(&deg[u], k);
     added += k;
   }
   added = wave_sum(added);
-  reloc = wave_sum(reloc);
   if (lane_id() == 0) {
     block_add(&ctr->live_delta[dir], added);
     block_add(&ctr->stored_delta[dir], added);
-    block_add(&ctr->relocated[dir], reloc);
   }
 }
 
