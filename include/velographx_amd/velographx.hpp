@@ -216,6 +216,51 @@ class IncrementalBFS {
   vgx_bfs* b_ = nullptr;
 };
 
+// Addition without a reference counterpart: up to 64 roots traversed in one
+// pass (the reference runs independent roots on threads,
+// benchmarks/thread_scaling_bfs.cpp:78-108).
+class MultiSourceBFS {
+ public:
+  MultiSourceBFS(DynamicGraph& g, const std::vector<VertexId>& roots) : n_(g.vertex_count()), k_(roots.size()) {
+    vgx_check(vgx_msbfs_create(g.handle(), roots.data(), static_cast<std::uint32_t>(roots.size()), &m_));
+  }
+  ~MultiSourceBFS() {
+    if (m_) vgx_msbfs_destroy(m_);
+  }
+  MultiSourceBFS(const MultiSourceBFS&) = delete;
+  MultiSourceBFS& operator=(const MultiSourceBFS&) = delete;
+
+  void run(std::size_t vertex_count) {
+    n_ = vertex_count;
+    vgx_check(vgx_msbfs_run(m_));
+  }
+  [[nodiscard]] std::vector<std::uint32_t> distances(std::uint32_t r) const {
+    std::vector<std::uint32_t> d(n_);
+    vgx_check(vgx_msbfs_distances(m_, r, d.data(), d.size()));
+    return d;
+  }
+  [[nodiscard]] std::vector<std::uint8_t> levels_u8() const {
+    std::vector<std::uint8_t> d(n_ * k_);
+    vgx_check(vgx_msbfs_levels_u8(m_, d.data(), d.size()));
+    return d;
+  }
+  [[nodiscard]] std::vector<std::uint64_t> reached() const {
+    std::vector<std::uint64_t> r(k_);
+    vgx_check(vgx_msbfs_reached(m_, r.data(), static_cast<std::uint32_t>(k_)));
+    return r;
+  }
+  [[nodiscard]] std::uint32_t level_count() const {
+    std::uint32_t l = 0;
+    vgx_check(vgx_msbfs_level_count(m_, &l));
+    return l;
+  }
+  [[nodiscard]] std::size_t root_count() const noexcept { return k_; }
+
+ private:
+  vgx_msbfs* m_ = nullptr;
+  std::size_t n_ = 0, k_ = 0;
+};
+
 // pagerank.hpp:16-27
 struct PageRankValidation {
   double l1_error{0.0};

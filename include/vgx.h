@@ -209,6 +209,8 @@ vgx_status vgx_msbfs_distances(vgx_msbfs* m, uint32_t r, uint32_t* out, uint64_t
 /* Root-major uint8 levels (255 = unreachable) of all roots: out[k * vertex_count]. */
 vgx_status vgx_msbfs_levels_u8(vgx_msbfs* m, uint8_t* out, uint64_t cap);
 vgx_status vgx_msbfs_reached(vgx_msbfs* m, uint64_t* reached_per_root, uint32_t cap);
+/* Levels stored by the last run (deepest distance + 1). */
+vgx_status vgx_msbfs_level_count(vgx_msbfs* m, uint32_t* levels);
 
 /* ------------------------------------------ weighted personalized PageRank */
 

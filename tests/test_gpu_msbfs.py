@@ -1,0 +1,119 @@
+"""GPU parity: multi-source BFS (up to 64 roots per pass).  Every root's
+distances are bit-exact against the CPU oracle's single-root BFS and against
+the single-root GPU path; the uint8 level arrays equal the distances with
+255 for unreachable."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+INF = 0xFFFFFFFF
+
+
+@pytest.fixture(scope="module")
+def vx():
+    import velographx_amd as vx
+
+    assert vx.device_count() > 0, "no gfx950 device visible"
+    return vx
+
+
+def u8_of(dist):
+    out = dist.astype(np.int64)
+    out[dist == INF] = 255
+    return out.astype(np.uint8)
+
+
+def check_all_roots(ms, g, mg, orc, roots):
+    ms.run(g)
+    off, adj = mg.csr()
+    lv = ms.levels_u8()
+    assert lv.shape == (len(roots), mg.n)
+    reached = ms.reached()
+    depth = 0
+    for r, root in enumerate(roots):
+        full, cnt = orc.bfs(mg.n, off, adj, int(root))
+        got = ms.distances(r)
+        assert np.array_equal(got, full), (r, int(root), int((got != full).sum()))
+        assert np.array_equal(lv[r], u8_of(full)), r
+        assert reached[r] == cnt
+        if cnt:
+            depth = max(depth, int(full[full != INF].max()) + 1)
+    assert ms.level_count == depth
+
+
+def test_tiny_directed_and_duplicates(vx, orc):
+    edges = [(0, 1), (1, 2), (2, 3), (3, 1), (4, 0), (5, 5)]
+    g = vx.Graph(0, True)
+    g.bulk_load_edges(np.array(edges, dtype=np.uint32))
+    mg = orc.ModelGraph(0, True)
+    mg.bulk_load_edges(edges)
+    roots = [0, 0, 4, 3, 1, 99]  # a repeated root and one past the last vertex
+    ms = vx.MultiSourceBFS(g, roots)
+    check_all_roots(ms, g, mg, orc, roots)
+
+
+@pytest.mark.parametrize("k", [1, 7, 63, 64])
+def test_rmat_s14_roots(vx, orc, k):
+    s, d = vx.rmat_edges(14, 16, 1)
+    g = vx.Graph(1 << 14, False)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(1 << 14, False)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    roots = np.random.default_rng(k).integers(0, 1 << 14, size=k).astype(np.uint32)
+    ms = vx.MultiSourceBFS(g, roots.tolist())
+    check_all_roots(ms, g, mg, orc, roots)
+
+
+def test_directed_rmat_long_rows(vx, orc):
+    """Directed R-MAT: pulls run over in-rows, hub in-rows exceed one wave."""
+    s, d = vx.rmat_edges(13, 24, 3)
+    g = vx.Graph(1 << 13, True)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(1 << 13, True)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    assert max(np.diff(mg.csr(reverse=True)[0])) > 64
+    roots = list(range(0, 64 * 97, 97))
+    check_all_roots(vx.MultiSourceBFS(g, roots), g, mg, orc, roots)
+
+
+def test_after_updates_matches_single_root(vx, orc):
+    """After update batches (tombstones, relocated rows) the 64-root pass
+    equals 64 single-root GPU BFS runs."""
+    from velographx_amd import workloads as wl
+
+    s, d = vx.rmat_edges(12, 16, 2)
+    g = vx.Graph(1 << 12, False)
+    g.bulk_load_arrays(s, d)
+    eu, ev = wl.canonical_edges(s, d, False)
+    stream = wl.BatchStream(eu, ev, 1 << 12, False, seed=4)
+    for _ in range(5):
+        g.apply_arrays(*stream.next(2000, 0.8))
+    roots = np.random.default_rng(9).integers(0, 1 << 12, size=64).tolist()
+    ms = vx.MultiSourceBFS(g, roots)
+    ms.run(g)
+    for r, root in enumerate(roots):
+        single = vx.IncrementalBFS(g, root).distances_numpy()
+        assert np.array_equal(ms.distances(r), single), r
+
+
+def test_path_deeper_than_u8(vx):
+    """A 300-vertex path: uint32 distances stay exact, the uint8 export
+    refuses (depth beyond 254)."""
+    n = 300
+    g = vx.Graph(0, False)
+    g.bulk_load_edges(np.array([(i, i + 1) for i in range(n - 1)], dtype=np.uint32))
+    ms = vx.MultiSourceBFS(g, [0, n - 1])
+    ms.run(g)
+    assert np.array_equal(ms.distances(0), np.arange(n, dtype=np.uint32))
+    assert np.array_equal(ms.distances(1), np.arange(n - 1, -1, -1).astype(np.uint32))
+    with pytest.raises(Exception):
+        ms.levels_u8()
+
+
+def test_root_count_limits(vx):
+    g = vx.Graph(4, False)
+    with pytest.raises(ValueError):
+        vx.MultiSourceBFS(g, [])
+    with pytest.raises(ValueError):
+        vx.MultiSourceBFS(g, list(range(65)))

@@ -2,7 +2,9 @@
 
 Drop-in for the reference's ``velographx`` Python module on the traversal
 path: ``Graph``, ``UpdateBatch``, ``IncrementalBFS``, ``IncrementalPageRank``,
-``PageRankValidation`` and the ``from_*`` ingestion helpers, backed by
+``PageRankValidation`` and the ``from_*`` ingestion helpers, plus
+``MultiSourceBFS`` (up to 64 roots per pass) and, in ``velographx_amd.autograd``,
+differentiable weighted personalized PageRank, backed by
 hand-written HIP kernels (``libvgx.so``) through a C++ facade.
 
 ``import torch`` comes first on purpose: torch ships its own HIP runtime,
@@ -17,6 +19,7 @@ from ._velographx import (  # noqa: F401
     Graph,
     IncrementalBFS,
     IncrementalPageRank,
+    MultiSourceBFS,
     PageRankValidation,
     UpdateBatch,
     device_count,
@@ -31,5 +34,5 @@ from ._velographx import (  # noqa: F401
 __all__ = [
     "Graph", "UpdateBatch", "IncrementalBFS", "IncrementalPageRank", "PageRankValidation",
     "from_numpy_edges", "from_scipy_csr", "from_arrow_columns", "from_arrow_table",
-    "rmat_edges", "device_count", "version",
+    "rmat_edges", "device_count", "version", "MultiSourceBFS",
 ]

@@ -143,6 +143,14 @@ struct CsrSnapshot {
   uint64_t n = 0, arcs = 0;
 };
 
+// Scratch of the weighted personalized PageRank kernels.
+struct PprWork {
+  DevBuf<float> pn;    // normalised weights, out-arc order
+  DevBuf<float> pin;   // the same, in-arc order
+  DevBuf<float> wsum;  // out-weight sums
+  DevBuf<float> tmp;   // ping-pong n x K
+};
+
 }  // namespace vgx
 
 struct vgx_graph {
@@ -158,6 +166,7 @@ struct vgx_graph {
   vgx::BatchScratch bs[2];
   vgx::HostBuf stage;
   vgx::CsrSnapshot snap[2];
+  vgx::PprWork ppr;
   int ndir() const { return directed ? 2 : 1; }
   vgx::Adjacency& in() { return directed ? adj[1] : adj[0]; }
 };

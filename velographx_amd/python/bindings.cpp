@@ -324,6 +324,58 @@ PYBIND11_MODULE(_velographx, m) {
       .def_property_readonly("last_used_full_recompute", &IncrementalPageRank::last_used_full_recompute)
       .def_property_readonly("precision", &IncrementalPageRank::precision);
 
+  // addition: batched multi-root BFS
+  py::class_<velographx::MultiSourceBFS>(m, "MultiSourceBFS")
+      .def(py::init([](DynamicGraph& g, const std::vector<VertexId>& roots) {
+             return new velographx::MultiSourceBFS(g, roots);
+           }),
+           py::arg("graph"), py::arg("roots"), py::keep_alive<1, 2>())
+      .def("run",
+           [](velographx::MultiSourceBFS& m, const DynamicGraph& g) {
+             const auto n = g.vertex_count();
+             py::gil_scoped_release nogil;
+             m.run(n);
+           },
+           py::arg("graph"))
+      .def("distances", [](const velographx::MultiSourceBFS& m, std::uint32_t r) { return to_numpy(m.distances(r)); })
+      .def("levels_u8",
+           [](const velographx::MultiSourceBFS& m) {
+             auto v = m.levels_u8();
+             const auto k = static_cast<py::ssize_t>(m.root_count());
+             auto arr = to_numpy(std::move(v));
+             return arr.reshape({k, arr.size() / std::max<py::ssize_t>(k, 1)});
+           })
+      .def("reached", [](const velographx::MultiSourceBFS& m) { return to_numpy(m.reached()); })
+      .def_property_readonly("level_count", &velographx::MultiSourceBFS::level_count);
+
+  // addition: weighted personalized PageRank kernels on device pointers
+  m.def(
+      "ppr_forward",
+      [](const DynamicGraph& g, std::uintptr_t w, std::uintptr_t s, std::uintptr_t x, std::uint32_t K, double damping,
+         std::uint32_t iterations, std::uintptr_t stream) {
+        py::gil_scoped_release nogil;
+        velographx::vgx_check(vgx_ppr_forward(g.handle(), reinterpret_cast<const float*>(w),
+                                              reinterpret_cast<const float*>(s), reinterpret_cast<float*>(x), K,
+                                              damping, iterations, reinterpret_cast<void*>(stream)));
+      },
+      py::arg("graph"), py::arg("w_ptr"), py::arg("s_ptr"), py::arg("x_ptr"), py::arg("K"), py::arg("damping"),
+      py::arg("iterations"), py::arg("stream") = 0);
+  m.def(
+      "ppr_backward",
+      [](const DynamicGraph& g, std::uintptr_t w, std::uintptr_t x, std::uintptr_t c, std::uintptr_t lam,
+         std::uintptr_t gs, std::uintptr_t gw, std::uint32_t K, double damping, std::uint32_t iterations,
+         std::uintptr_t stream) {
+        py::gil_scoped_release nogil;
+        velographx::vgx_check(vgx_ppr_backward(g.handle(), reinterpret_cast<const float*>(w),
+                                               reinterpret_cast<const float*>(x), reinterpret_cast<const float*>(c),
+                                               reinterpret_cast<float*>(lam), reinterpret_cast<float*>(gs),
+                                               reinterpret_cast<float*>(gw), K, damping, iterations,
+                                               reinterpret_cast<void*>(stream)));
+      },
+      py::arg("graph"), py::arg("w_ptr"), py::arg("x_ptr"), py::arg("c_ptr"), py::arg("lambda_ptr"),
+      py::arg("grad_s_ptr"), py::arg("grad_w_ptr"), py::arg("K"), py::arg("damping"), py::arg("iterations"),
+      py::arg("stream") = 0);
+
   m.def("from_numpy_edges", &graph_from_numpy_edges, py::arg("edges"), py::arg("directed") = false);
   m.def("from_scipy_csr", &graph_from_scipy_csr, py::arg("csr"), py::arg("directed") = false);
   m.def("from_arrow_columns", &graph_from_arrow_columns, py::arg("src"), py::arg("dst"), py::arg("directed") = false);
