@@ -1,26 +1,41 @@
 #!/usr/bin/env python3
 """Benchmark of the traversal hot path on one MI355X.
 
-Workload (the single-GPU configuration the headline metric is quoted on):
-undirected R-MAT scale 22, edge factor 16, A/B/C/D = .57/.19/.19/.05, seed 1
-(4,194,304 vertices, ~64.2M edges after loop/duplicate removal).
+Headline workload: undirected R-MAT scale 22, edge factor 16,
+A/B/C/D = .57/.19/.19/.05, seed 1 (4,194,304 vertices, ~64.2M edges after
+loop/duplicate removal).
 
 * step  = one direction-optimizing BFS from one of 16 seeded roots in the
           largest component (answer-ready: distances in HBM, stream synced);
-* value = BFS GTEPS over the K timed steps (Graph500: m_c / t per root,
-          harmonic mean; every root lies in the same component, so this is
-          sum(m_c) / sum(t));
-* then 100 update batches of 10,000 ops (80% inserts of random non-edges,
-  20% deletions of live edges) are applied and repaired to exact distances,
-  with a full consolidation every 20 batches -> repairs/s, update-ops/s;
+* value = plain single-root BFS GTEPS over the K timed steps (Graph500:
+          m_c / t; every root lies in the same component, so the harmonic
+          mean is sum(m_c) / sum(t));
 * roofline: algorithmic bytes of one BFS (4*A_c + 12*V_c) over the summed
   duration of its level-kernel launches, timed with HIP events on the
   graph's stream;
-* cpu_baseline: the reference engine (oracle/_ref/vgx_ref, compiled from
-  the reference sources) or, where it is absent, the oracle's queue BFS,
-  timed on this host on a bounded sample of the same roots.
+* cpu_baseline: the reference engine (oracle/_ref/vgx_ref, compiled from the
+  reference sources) or, where it is absent, the oracle's queue BFS, timed
+  on this host on a bounded sample of the same roots.
 
-Prints one JSON line.
+Secondary legs, each in its own object of the same JSON line:
+* batched   -- 64 roots per multi-source pass on the same graph (aggregate
+               GTEPS, batched / plain ratio, bit-exact check of the first 16
+               roots against the plain path);
+* repairs   -- 100 update batches of 10,000 ops (80% inserts of random
+               non-edges, 20% deletions of live edges) repaired to exact
+               distances, consolidation every 20 batches; repairs/s, with
+               the reference engine's IncrementalBFS::apply timed on a
+               bounded sample of the same batches;
+* contrast  -- Erdos-Renyi, 4,194,304 vertices, mean degree 16;
+* pagerank  -- directed R-MAT scale 22: FP32 pull PageRank to an L1
+               residual below 1e-7, then 50 batches of 10,000 ops with a
+               warm-started FP32 re-solve; L1 / max-abs error against the
+               FP64 solve on a second copy of the graph, per batch;
+* ppr       -- directed weighted R-MAT scale 20, 32 one-hot teleport
+               columns: weighted personalized PageRank forward + adjoint
+               through torch.autograd, 100 sweeps each.
+
+Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -40,6 +55,7 @@ sys.path.insert(0, ROOT)
 METRIC = "BFS GTEPS + update-batch repairs/s at R-MAT s22; batched vs plain, 1 MI355X"
 HBM_PEAK_GBPS = 8000.0
 INF = 0xFFFFFFFF
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "vgx_ref")
 
 
 def log(msg):
@@ -54,32 +70,51 @@ def parse():
     p.add_argument("--scale", type=int, default=22)
     p.add_argument("--edgefactor", type=int, default=16)
     p.add_argument("--roots", type=int, default=16)
+    p.add_argument("--batched-roots", type=int, default=64)
     p.add_argument("--batches", type=int, default=100)
     p.add_argument("--batch-size", type=int, default=10000)
     p.add_argument("--consolidate-every", type=int, default=20)
     p.add_argument("--cpu-budget", type=float, default=20.0)
+    p.add_argument("--pr-batches", type=int, default=50)
+    p.add_argument("--ppr-scale", type=int, default=20)
+    p.add_argument("--ppr-iterations", type=int, default=100)
+    p.add_argument("--ppr-reps", type=int, default=5)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-repairs", action="store_true")
+    p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--no-contrast", action="store_true")
+    p.add_argument("--no-pagerank", action="store_true")
+    p.add_argument("--no-ppr", action="store_true")
+    p.add_argument("--only-bfs", action="store_true", help="plain BFS leg only (for profiling)")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                   help="only run 5 device-to-device copies of BYTES (HBM counter calibration)")
     return p.parse_args()
+
+
+# ----------------------------------------------------------------- CPU legs
+
+
+def _edges_file(td, s, d):
+    path = os.path.join(td, "edges.bin")
+    np.stack([s, d], 1).astype(np.uint32).tofile(path)
+    return path
 
 
 def cpu_baseline(s, d, n, roots, m_c, budget):
     """Time the reference engine's BFS (one thread) on a bounded sample."""
-    ref = os.path.join(ROOT, "oracle", "_ref", "vgx_ref")
-    cores = 1
-    if os.path.exists(ref):
+    times = []
+    if os.path.exists(REF_BIN):
         with tempfile.TemporaryDirectory() as td:
-            path = os.path.join(td, "edges.bin")
-            np.stack([s, d], 1).astype(np.uint32).tofile(path)
-            cmd = [ref, "bench-bfs", path, str(n), "0", str(budget)] + [str(int(r)) for r in roots]
-            out = subprocess.run(cmd, capture_output=True, text=True, timeout=600, check=True).stdout
-        times = []
+            cmd = [REF_BIN, "bench-bfs", _edges_file(td, s, d), str(n), "0", str(budget)]
+            cmd += [str(int(r)) for r in roots]
+            out = subprocess.run(cmd, capture_output=True, text=True, timeout=900, check=True).stdout
         for line in out.splitlines():
             w = line.split()
             if w and w[0] == "root":
                 times.append(float(w[2]))
         kind = "reference"
-        sample = f"IncrementalBFS(graph, root).recompute() of the reference engine on the same graph, {len(times)} roots"
+        sample = (f"IncrementalBFS(graph, root) construction (a full queue BFS) of the reference engine on the "
+                  f"same graph, first {len(times)} of the same roots, one thread")
     else:
         from oracle import oracle as orc
 
@@ -87,7 +122,6 @@ def cpu_baseline(s, d, n, roots, m_c, budget):
         g = orc.ModelGraph(n, False)
         g.bulk_load_edges(np.stack([s, d], 1))
         off, adj = g.csr()
-        times = []
         t_all = time.perf_counter()
         for r in roots:
             t0 = time.perf_counter()
@@ -96,12 +130,283 @@ def cpu_baseline(s, d, n, roots, m_c, budget):
             if time.perf_counter() - t_all > budget:
                 break
         kind = "port"
-        sample = f"oracle queue BFS (C restatement) on the same graph, {len(times)} roots"
+        sample = f"oracle queue BFS (C restatement) on the same graph, first {len(times)} of the same roots"
     if not times:
         return None
-    gteps = m_c * len(times) / sum(times) / 1e9
-    return {"value": gteps, "unit": "GTEPS", "cores": cores, "kind": kind, "sample": sample,
-            "ms_per_bfs": 1e3 * sum(times) / len(times)}
+    return {"value": m_c * len(times) / sum(times) / 1e9, "unit": "GTEPS", "cores": 1, "kind": kind,
+            "sample": sample, "ms_per_bfs": 1e3 * sum(times) / len(times)}
+
+
+def cpu_repairs(s, d, n, root, batches, budget):
+    """The reference engine's IncrementalBFS::apply on the first batches of
+    the same stream (one thread, bounded by ``budget`` seconds)."""
+    if not os.path.exists(REF_BIN):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        bpath = os.path.join(td, "batches.bin")
+        with open(bpath, "wb") as f:
+            for bs, bd, ba in batches:
+                np.array([bs.size], np.uint32).tofile(f)
+                np.stack([bs, bd, ba.astype(np.uint32)], 1).astype(np.uint32).tofile(f)
+        cmd = [REF_BIN, "bench-repair", _edges_file(td, s, d), str(n), "0", str(int(root)), bpath, str(budget)]
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=1200, check=True).stdout
+    t = [float(line.split()[1]) for line in out.splitlines() if line.startswith("batch ")]
+    full = sum(int(line.split()[3]) for line in out.splitlines() if line.startswith("batch "))
+    if not t:
+        return None
+    return {"repairs_per_s": len(t) / sum(t), "mean_batch_ms": 1e3 * sum(t) / len(t), "batches": len(t),
+            "full_recompute_batches": full, "cores": 1, "kind": "reference",
+            "sample": f"IncrementalBFS::apply of the reference engine on the first {len(t)} batches"}
+
+
+def cpu_pagerank(s, d, n, iterations):
+    if not os.path.exists(REF_BIN):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        cmd = [REF_BIN, "bench-pr", _edges_file(td, s, d), str(n), "1", str(iterations)]
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=900, check=True).stdout
+    w = [line.split() for line in out.splitlines() if line.startswith("pr ")][0]
+    secs, its = float(w[1]), int(w[2])
+    return {"ms_per_iteration": 1e3 * secs / max(its, 1), "iterations": its, "cores": 1, "kind": "reference",
+            "sample": f"IncrementalPageRank::recompute of the reference engine (FP64), {its} sweeps"}
+
+
+# ----------------------------------------------------------------- GPU legs
+
+
+def timed_bfs(vx, g, roots, steps):
+    """K answer-ready BFS recomputes over the given roots, with level-kernel
+    HIP-event profiling.  Returns (seconds per step list, elapsed, objects)."""
+    import torch
+
+    objs = [vx.IncrementalBFS(g, int(r)) for r in roots]
+    for o in objs:
+        o.set_profiling(True)
+    torch.cuda.synchronize()
+    times = []
+    t_start = time.perf_counter()
+    for k in range(steps):
+        o = objs[k % len(objs)]
+        t0 = time.perf_counter()
+        o.recompute()
+        times.append(time.perf_counter() - t0)
+    torch.cuda.synchronize()
+    return times, time.perf_counter() - t_start, objs
+
+
+def component(vx, wl, g):
+    lcc = wl.largest_component_from_hub(vx, g)
+    deg = g.degrees()
+    arcs_c = int(deg[lcc].astype(np.int64).sum())
+    return lcc, arcs_c
+
+
+def leg_batched(vx, wl, g, lcc, m_c, k, plain_objs, plain_gteps):
+    import torch
+
+    roots = wl.roots_in_component(lcc, k, seed=1)
+    ms = vx.MultiSourceBFS(g, roots.tolist())
+    tuned = ms.autotune(2)  # outside the timed region
+    log(f"batched autotune: {tuned}")
+    ms.run(g)  # warm-up
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ms.run(g)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    exact = all(np.array_equal(ms.distances(i), o.distances_numpy()) for i, o in enumerate(plain_objs[:k]))
+    reached = ms.reached()
+    agg = m_c * k / t / 1e9
+    out = {"roots": k, "ms_per_pass": 1e3 * t, "aggregate_gteps": agg, "batched_vs_plain": agg / plain_gteps,
+           "tuning": {"blocks": tuned["blocks"], "alpha": tuned["alpha"]},
+           "levels": int(ms.level_count), "exact_vs_plain": bool(exact),
+           "all_roots_reach_component": bool(np.all(reached == lcc.size))}
+    log(f"batched: {out}")
+    return out
+
+
+def leg_repairs(vx, wl, g, s, d, n, root, args):
+    import torch
+
+    t0 = time.time()
+    eu, ev = wl.canonical_edges(s, d, False)
+    stream = wl.BatchStream(eu, ev, n, False, seed=1)
+    batches = [stream.next(args.batch_size, 0.8) for _ in range(args.batches)]
+    log(f"generated {len(batches)} batches in {time.time() - t0:.1f}s")
+    inc = vx.IncrementalBFS(g, int(root))
+    batch_t = []
+    full = 0
+    exact = True
+    for i, (bs, bd, ba) in enumerate(batches):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        inc.apply_arrays(bs, bd, ba)
+        if args.consolidate_every and (i + 1) % args.consolidate_every == 0:
+            g.compact()
+        batch_t.append(time.perf_counter() - t0)
+        full += int(inc.last_used_full_recompute)
+        if i % 10 == 9 or i == len(batches) - 1:  # verification outside the timer
+            ver = vx.IncrementalBFS(g, int(root))
+            if not np.array_equal(ver.distances_numpy(), inc.distances_numpy()):
+                exact = False
+            del ver
+    tot = sum(batch_t)
+    out = {"repairs_per_s": len(batch_t) / tot, "update_ops_per_s": len(batch_t) * args.batch_size / tot,
+           "mean_batch_ms": 1e3 * tot / len(batch_t), "p50_batch_ms": 1e3 * float(np.median(batch_t)),
+           "full_recompute_batches": full, "exact": exact, "consolidate_every": args.consolidate_every}
+    log(f"repairs: {out}")
+    return out, batches
+
+
+def leg_contrast(vx, wl, steps):
+    n = 1 << 22
+    t0 = time.time()
+    u, v = wl.erdos_renyi(n, 8 * n, seed=1)
+    g = vx.Graph(n, False)
+    g.bulk_load_arrays(u, v)
+    lcc, arcs_c = component(vx, wl, g)
+    roots = wl.roots_in_component(lcc, 16, seed=1)
+    log(f"ER n={n}: {g.edge_count_directed} arcs, LCC {lcc.size}, built in {time.time() - t0:.1f}s")
+    times, elapsed, objs = timed_bfs(vx, g, roots, steps)
+    kernel_ms = sum(o.stats()["level_kernel_ms"] for o in objs) / steps
+    bytes_bfs = 4.0 * arcs_c + 12.0 * lcc.size
+    out = {"workload": "Erdos-Renyi n=4194304, 33554432 uniform pairs (mean degree 16), seed 1",
+           "arcs": int(g.edge_count_directed), "lcc_vertices": int(lcc.size),
+           "gteps": arcs_c / 2.0 * steps / sum(times) / 1e9, "ms_per_bfs": 1e3 * elapsed / steps,
+           "roofline_frac": bytes_bfs / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+           "levels": objs[0].stats()["levels"], "bottom_up_levels": objs[0].stats()["bottom_up_levels"]}
+    log(f"contrast: {out}")
+    del objs, g
+    return out
+
+
+def leg_pagerank(vx, wl, args, no_cpu):
+    import torch
+
+    scale = args.scale
+    n = 1 << scale
+    t0 = time.time()
+    s, d = vx.rmat_edges(scale, args.edgefactor, 1)
+    g32 = vx.Graph(n, True)
+    g32.bulk_load_arrays(s, d)
+    g64 = vx.Graph(n, True)
+    g64.bulk_load_arrays(s, d)
+    arcs = int(g32.edge_count_directed)
+    log(f"PageRank graph: directed R-MAT s{scale}, {arcs} arcs, built in {time.time() - t0:.1f}s")
+    pr32 = vx.IncrementalPageRank(g32, 0.85, 32)
+    pr64 = vx.IncrementalPageRank(g64, 0.85, 64)
+    pr32.solve_l1(200, 1e-7, False)  # warm-up (snapshot, allocations)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pr32.solve_l1(200, 1e-7, False)
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t0
+    iters = int(pr32.last_repair_iterations)
+    pr64.solve_l1(1000, 1e-11, False)
+    err_l1 = []
+    err_max = []
+
+    def errors():
+        a = pr32.values_numpy()
+        b = pr64.values_numpy()
+        err_l1.append(float(np.abs(a - b).sum()))
+        err_max.append(float(np.abs(a - b).max()))
+
+    errors()
+    bytes_it = 8.0 * arcs + 20.0 * n
+    out = {"workload": f"directed R-MAT s{scale} ef{args.edgefactor} seed 1, d=0.85, FP32, L1 residual < 1e-7",
+           "arcs": arcs, "solve_ms": 1e3 * t_solve, "iterations": iters,
+           "ms_per_iteration": 1e3 * t_solve / max(iters, 1),
+           "gbps_per_iteration_wall": bytes_it * iters / t_solve / 1e9,
+           "roofline_frac_wall": bytes_it * iters / t_solve / 1e9 / HBM_PEAK_GBPS,
+           "residual_l1": float(pr32.last_residual_l1)}
+    if args.pr_batches > 0:
+        eu, ev = wl.canonical_edges(s, d, True)
+        stream = wl.BatchStream(eu, ev, n, True, seed=2)
+        bt = []
+        its = []
+        for _ in range(args.pr_batches):
+            bs, bd, ba = stream.next(args.batch_size, 0.8)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            g32.apply_arrays(bs, bd, ba)
+            pr32.solve_l1(200, 1e-7, True)
+            torch.cuda.synchronize()
+            bt.append(time.perf_counter() - t0)
+            its.append(int(pr32.last_repair_iterations))
+            g64.apply_arrays(bs, bd, ba)
+            pr64.solve_l1(1000, 1e-11, True)
+            errors()
+        out.update({"batches": args.pr_batches, "batch_size": args.batch_size,
+                    "mean_batch_ms": 1e3 * float(np.mean(bt)), "batches_per_s": len(bt) / sum(bt),
+                    "mean_batch_iterations": float(np.mean(its))})
+    out.update({"fp64_reference": "GPU FP64 solve to L1 residual 1e-11 on a second copy of the graph",
+                "max_l1_error": max(err_l1), "max_abs_error": max(err_max),
+                "within_1e-6": bool(max(err_l1) <= 1e-6 and max(err_max) <= 1e-6)})
+    if not no_cpu:
+        out["cpu_reference"] = cpu_pagerank(s, d, n, 3)
+    log(f"pagerank: {out}")
+    del pr32, pr64, g32, g64
+    return out
+
+
+def leg_ppr(vx, args):
+    import torch
+
+    from velographx_amd import autograd as ag
+
+    scale = args.ppr_scale
+    n = 1 << scale
+    s, d = vx.rmat_edges(scale, args.edgefactor, 1)
+    g = vx.Graph(n, True)
+    g.bulk_load_arrays(s, d)
+    arcs = int(g.edge_count_directed)
+    K = 32
+    rng = np.random.default_rng(1)
+    w = torch.tensor((1.0 - rng.random(arcs)).astype(np.float32), device="cuda", requires_grad=True)
+    S = torch.zeros((n, K), device="cuda")
+    S[torch.tensor(rng.integers(0, n, K), device="cuda"), torch.arange(K, device="cuda")] = 1.0
+    S.requires_grad_(True)
+    C = torch.tensor(rng.standard_normal((n, K)).astype(np.float32), device="cuda")
+    T = args.ppr_iterations
+
+    def step():
+        w.grad = None
+        S.grad = None
+        x = ag.personalized_pagerank(g, w, S, 0.85, T)
+        (x * C).sum().backward()
+        return x
+
+    step()
+    torch.cuda.synchronize()
+    t_f = []
+    t_all = []
+    for _ in range(args.ppr_reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            ag.personalized_pagerank(g, w.detach(), S.detach(), 0.85, T)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        t_f.append(t1 - t0)
+        t_all.append(time.perf_counter() - t1)
+    tf = float(np.median(t_f))
+    tfb = float(np.median(t_all))
+    fwd_bytes = T * (8.0 * arcs + 4.0 * n + 12.0 * K * n)
+    bwd_bytes = fwd_bytes + 12.0 * arcs + 8.0 * K * n
+    out = {"workload": f"directed R-MAT s{scale} ef{args.edgefactor}, weights U(0,1] row-normalised, "
+                       f"K={K} one-hot teleport columns, {T} sweeps, random linear loss",
+           "arcs": arcs, "fwd_ms": 1e3 * tf, "fwd_bwd_ms": 1e3 * tfb, "fwd_bwd_per_s": 1.0 / tfb,
+           "fwd_gbps": fwd_bytes / tf / 1e9, "fwd_bwd_gbps": (fwd_bytes + bwd_bytes) / tfb / 1e9,
+           "roofline_frac_fwd_bwd": (fwd_bytes + bwd_bytes) / tfb / 1e9 / HBM_PEAK_GBPS,
+           "grad_w_finite": bool(torch.isfinite(w.grad).all().item()),
+           "grad_s_finite": bool(torch.isfinite(S.grad).all().item())}
+    log(f"ppr: {out}")
+    return out
 
 
 def main():
@@ -110,6 +415,19 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+
+        a = torch.ones(args.calibrate_copy, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy, "copies": 5}), flush=True)
+        return
+    if args.only_bfs:
+        args.no_cpu = args.no_repairs = args.no_batched = args.no_contrast = True
+        args.no_pagerank = args.no_ppr = True
     import torch
 
     import velographx_amd as vx
@@ -124,33 +442,16 @@ def main():
     t0 = time.time()
     g.bulk_load_arrays(s, d)
     log(f"bulk load: {g.edge_count_directed} arcs in {time.time() - t0:.2f}s")
-    lcc = wl.largest_component_from_hub(vx, g)
+    lcc, arcs_c = component(vx, wl, g)
     roots = wl.roots_in_component(lcc, args.roots, seed=1)
-    deg = g.degrees()
-    arcs_c = int(deg[lcc].astype(np.int64).sum())
     m_c = arcs_c / 2.0
     v_c = int(lcc.size)
     bytes_bfs = 4.0 * arcs_c + 12.0 * v_c
     log(f"LCC: {v_c} vertices, {arcs_c} arcs; bytes/BFS {bytes_bfs / 1e6:.1f} MB")
 
-    bfs = vx.IncrementalBFS(g, int(roots[0]))
     for i in range(args.warmup):
-        bfs_w = vx.IncrementalBFS(g, int(roots[i % len(roots)]))
-        del bfs_w
-    # timed BFS steps, each answer-ready
-    objs = [vx.IncrementalBFS(g, int(r)) for r in roots]
-    for o in objs:
-        o.set_profiling(True)
-    torch.cuda.synchronize()
-    times = []
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        o = objs[k % len(objs)]
-        t0 = time.perf_counter()
-        o.recompute()
-        times.append(time.perf_counter() - t0)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
+        vx.IncrementalBFS(g, int(roots[i % len(roots)]))
+    times, elapsed, objs = timed_bfs(vx, g, roots, args.steps)
     launches = sum(o.stats()["level_launches"] for o in objs)
     kernel_ms = sum(o.stats()["level_kernel_ms"] for o in objs)
     st0 = objs[0].stats()
@@ -162,7 +463,8 @@ def main():
     achieved = bytes_bfs / (kernel_ms_per_bfs * 1e-3) / 1e9
     log(f"BFS: {ms_per_step:.3f} ms/step, {gteps:.1f} GTEPS, kernel {kernel_ms_per_bfs:.3f} ms/BFS "
         f"over {launches / args.steps:.1f} launches, levels {st0['levels']} (bottom-up {st0['bottom_up_levels']})")
-    # parity spot check against a top-down-only traversal of the same root
+    # exactness spot check: the direction-optimizing result equals a
+    # top-down-only traversal of the same root
     chk = vx.IncrementalBFS(g, int(roots[0]))
     ref_d = chk.distances_numpy()
     chk.set_tuning(alpha=1e-9, beta=1.0, blocks=0, levels_per_sync=8)
@@ -185,7 +487,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"R-MAT s{args.scale} ef{args.edgefactor} undirected seed 1: direction-optimizing BFS "
-                        f"from {args.roots} LCC roots, then {args.batches} x {args.batch_size}-op update batches",
+                        f"from {args.roots} LCC roots one at a time; {args.batched_roots}-root batched pass; "
+                        f"{args.batches} x {args.batch_size}-op update batches",
             "vertices": n, "arcs": int(g.edge_count_directed), "lcc_vertices": v_c, "lcc_arcs": arcs_c,
             "roots": args.roots, "batches": args.batches, "batch_size": args.batch_size,
         },
@@ -196,7 +499,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": None,
-            "kernel": "k_bfs_level",
+            "kernel": "k_bfs_level + k_bfs_bottom_up_tail",
             "bytes_per_bfs": bytes_bfs,
             "avg_launch_us": 1e3 * kernel_ms / max(launches, 1),
             "launches_per_bfs": launches / args.steps,
@@ -205,47 +508,28 @@ def main():
         "bfs": {"levels": st0["levels"], "bottom_up_levels": st0["bottom_up_levels"],
                 "exact_vs_top_down": exact_bfs},
     }
-
+    if not args.no_batched:
+        result["batched"] = leg_batched(vx, wl, g, lcc, m_c, args.batched_roots, objs, gteps)
+    del objs
+    batches = None
     if not args.no_repairs and args.batches > 0:
-        t0 = time.time()
-        eu, ev = wl.canonical_edges(s, d, False)
-        stream = wl.BatchStream(eu, ev, n, False, seed=1)
-        batches = [stream.next(args.batch_size, 0.8) for _ in range(args.batches)]
-        log(f"generated {len(batches)} batches in {time.time() - t0:.1f}s")
-        inc = vx.IncrementalBFS(g, int(roots[0]))
-        batch_t = []
-        full = 0
-        exact = True
-        for i, (bs, bd, ba) in enumerate(batches):
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            inc.apply_arrays(bs, bd, ba)
-            if args.consolidate_every and (i + 1) % args.consolidate_every == 0:
-                g.compact()
-            batch_t.append(time.perf_counter() - t0)
-            full += int(inc.last_used_full_recompute)
-            if i % 10 == 9 or i == len(batches) - 1:
-                ver = vx.IncrementalBFS(g, int(roots[0]))
-                if not np.array_equal(ver.distances_numpy(), inc.distances_numpy()):
-                    exact = False
-                del ver
-        tot = sum(batch_t)
-        result["repairs"] = {
-            "repairs_per_s": len(batch_t) / tot,
-            "update_ops_per_s": len(batch_t) * args.batch_size / tot,
-            "mean_batch_ms": 1e3 * tot / len(batch_t),
-            "p50_batch_ms": 1e3 * float(np.median(batch_t)),
-            "full_recompute_batches": full,
-            "exact": exact,
-            "consolidate_every": args.consolidate_every,
-        }
-        log(f"repairs: {result['repairs']}")
-
+        result["repairs"], batches = leg_repairs(vx, wl, g, s, d, n, roots[0], args)
+    del g
     if not args.no_cpu:
         t0 = time.time()
-        cb = cpu_baseline(s, d, n, roots, m_c, args.cpu_budget)
-        result["cpu_baseline"] = cb
-        log(f"cpu baseline in {time.time() - t0:.1f}s: {cb}")
+        result["cpu_baseline"] = cpu_baseline(s, d, n, roots, m_c, args.cpu_budget)
+        log(f"cpu baseline in {time.time() - t0:.1f}s: {result['cpu_baseline']}")
+        if batches is not None:
+            cr = cpu_repairs(s, d, n, roots[0], batches, args.cpu_budget)
+            if cr is not None:
+                result["repairs"]["cpu_reference"] = cr
+                log(f"cpu repairs: {cr}")
+    if not args.no_contrast:
+        result["contrast"] = leg_contrast(vx, wl, args.steps)
+    if not args.no_pagerank:
+        result["pagerank"] = leg_pagerank(vx, wl, args, args.no_cpu)
+    if not args.no_ppr:
+        result["ppr"] = leg_ppr(vx, args)
     print(json.dumps(result), flush=True)
 
 

@@ -255,6 +255,7 @@ class MultiSourceBFS {
     return l;
   }
   [[nodiscard]] std::size_t root_count() const noexcept { return k_; }
+  [[nodiscard]] vgx_msbfs* handle() const noexcept { return m_; }
 
  private:
   vgx_msbfs* m_ = nullptr;

@@ -157,6 +157,14 @@ typedef struct {
 } vgx_bfs_tuning;
 vgx_status vgx_bfs_set_tuning(vgx_bfs* b, const vgx_bfs_tuning* t);
 vgx_status vgx_bfs_get_tuning(vgx_bfs* b, vgx_bfs_tuning* t);
+/* Autotuner (no reference counterpart): times answer-ready full BFS runs
+ * from this object's source, `reps` per candidate, by coordinate descent
+ * over alpha {5, 10, 15, 25, 40}, beta {12, 18, 24, 48}, workgroups
+ * {auto, 512, 1024, 2048, 4096} and levels per host check {4, 8, 16};
+ * installs the fastest setting and returns it with its mean time.  The
+ * distances afterwards are those of the last run (unchanged: BFS distances
+ * do not depend on the tuning). */
+vgx_status vgx_bfs_autotune(vgx_bfs* b, uint32_t reps, vgx_bfs_tuning* best, double* best_ms);
 
 /* --------------------------------------------------------------- PageRank */
 
@@ -211,6 +219,65 @@ vgx_status vgx_msbfs_levels_u8(vgx_msbfs* m, uint8_t* out, uint64_t cap);
 vgx_status vgx_msbfs_reached(vgx_msbfs* m, uint64_t* reached_per_root, uint32_t cap);
 /* Levels stored by the last run (deepest distance + 1). */
 vgx_status vgx_msbfs_level_count(vgx_msbfs* m, uint32_t* levels);
+/* Tuning (no reference counterpart): workgroups of the short-row kernels
+ * (0 = auto) and the direction switch (push while the frontier's out-arcs
+ * are fewer than arcs / alpha).  The autotuner times `reps` runs per
+ * candidate, alpha over {2, 4, 8, 16, 32} then workgroups over {auto, 512,
+ * 1024, 2048, 4096, 8192}, and keeps the fastest. */
+vgx_status vgx_msbfs_set_tuning(vgx_msbfs* m, uint32_t blocks, double alpha);
+vgx_status vgx_msbfs_autotune(vgx_msbfs* m, uint32_t reps, uint32_t* best_blocks, double* best_alpha,
+                              double* best_ms);
+
+/* -------------------------------------- repair-vs-recompute selection */
+
+/* Adaptive selector of the reference's publication policy
+ * (benchmarks/publication_policy_bfs.cpp:162-233 decision, :297-330 model
+ * update, constants benchmarks/adaptive_policy_bfs.cpp:21-31).  One record
+ * per batch: the features, the prediction, the decision and its outcome. */
+typedef struct {
+  double update_fraction;                  /* ops / directed arcs before the batch */
+  double shallow_parent_deletion_fraction; /* first batch only (:126-148) */
+  double predicted_incremental_us;
+  double predicted_full_us;
+  double execution_us;                     /* answer-ready cost of the arm that ran */
+  double decision_us;
+  uint64_t affected_vertices;              /* incremental arm: invalidated vertices */
+  int first_batch;
+  int chose_full;
+  int internal_fallback;                   /* incremental arm fell back to a full BFS */
+  int reason;                              /* see vgx_policy_reason_name */
+} vgx_selector_trace;
+
+typedef struct vgx_policy vgx_policy;
+typedef struct vgx_selector vgx_selector;
+
+/* Name of a decision reason ("large_preflight_full", ...). */
+const char* vgx_policy_reason_name(int reason);
+
+/* The decision model alone (host; replays recorded features):
+ * create with the graph's vertex count, the measured cost of the initial
+ * full BFS and its reachable fraction; decide() fills predicted_*,
+ * chose_full and reason from update_fraction, shallow_parent_deletion_fraction
+ * and first_batch; observe() takes the same record with execution_us,
+ * internal_fallback and affected_vertices filled in. */
+vgx_status vgx_policy_create(uint64_t vertices, double initial_full_us, double initial_reachable_fraction,
+                             vgx_policy** out);
+vgx_status vgx_policy_destroy(vgx_policy* p);
+vgx_status vgx_policy_decide(vgx_policy* p, vgx_selector_trace* t);
+vgx_status vgx_policy_observe(vgx_policy* p, const vgx_selector_trace* t);
+
+/* The selector bound to an IncrementalBFS, as the reference harness runs it
+ * (publication_policy_bfs.cpp:243-256): create() times one full BFS as the
+ * initial full-arm sample; apply() computes the features on the GPU (update
+ * fraction from the device arc count, shallow-parent deletions from the
+ * device distances), decides, runs either graph apply + full BFS or the
+ * localized repair, times it answer-ready and updates the model.  The BFS
+ * object's own fallback fraction still applies inside the repair arm (the
+ * reference harness constructs it with 2.0 to disable it). */
+vgx_status vgx_selector_create(vgx_bfs* b, vgx_selector** out);
+vgx_status vgx_selector_destroy(vgx_selector* s);
+vgx_status vgx_selector_apply(vgx_selector* s, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
+                              uint64_t k, vgx_selector_trace* trace);
 
 /* ------------------------------------------ weighted personalized PageRank */
 

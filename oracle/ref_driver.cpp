@@ -20,6 +20,7 @@
 //   pr_recompute <iters> <tol>      pr.recompute(iters, tol)
 //   pr_validate <it> <tol> <l1> <linf>
 //   neighbors <u>
+// Bench modes: bench-bfs, bench-pr, bench-repair (see their comments).
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -189,10 +190,67 @@ int bench_bfs(int argc, char** argv) {
   return 0;
 }
 
+// bench-pr <edges.bin> <n> <directed> <iterations>
+// Times IncrementalPageRank(graph, 0.85) followed by recompute(iterations,
+// 0.0): exactly <iterations> pull sweeps.  Prints
+// "pr <seconds> <iterations>".
+int bench_pr(char** argv) {
+  const auto edges = read_edges_bin(argv[2]);
+  const std::size_t n = std::stoull(argv[3]);
+  const bool directed = std::stoi(argv[4]) != 0;
+  const std::size_t iters = std::stoull(argv[5]);
+  DynamicGraph g(n, directed);
+  g.bulk_load_edges(edges);
+  IncrementalPageRank pr(g, 0.85);
+  const auto t0 = Clock::now();
+  pr.recompute(iters, 0.0);
+  const auto t1 = Clock::now();
+  std::printf("pr %.9f %zu\n", std::chrono::duration<double>(t1 - t0).count(), pr.last_repair_iterations());
+  return 0;
+}
+
+// bench-repair <edges.bin> <n> <directed> <root> <batches.bin> <budget_s>
+// batches.bin: repeated records {u32 k; k x (u32 src, u32 dst, u32 add)}.
+// Applies each batch through IncrementalBFS::apply (one thread) until the
+// budget is spent.  Prints "batch <seconds> <k> <full_recompute>" per batch.
+int bench_repair(char** argv) {
+  const auto edges = read_edges_bin(argv[2]);
+  const std::size_t n = std::stoull(argv[3]);
+  const bool directed = std::stoi(argv[4]) != 0;
+  const VertexId root = static_cast<VertexId>(std::stoul(argv[5]));
+  const double budget = std::stod(argv[7]);
+  std::ifstream f(argv[6], std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open batch file");
+  DynamicGraph g(n, directed);
+  g.bulk_load_edges(edges);
+  IncrementalBFS bfs(g, root);
+  const auto t_all = Clock::now();
+  std::uint32_t k = 0;
+  while (f.read(reinterpret_cast<char*>(&k), 4)) {
+    std::vector<std::uint32_t> raw(static_cast<std::size_t>(k) * 3);
+    f.read(reinterpret_cast<char*>(raw.data()), static_cast<std::streamsize>(raw.size() * 4));
+    UpdateBatch b;
+    for (std::uint32_t i = 0; i < k; ++i) {
+      if (raw[3 * i + 2]) b.add(raw[3 * i], raw[3 * i + 1]);
+      else b.remove(raw[3 * i], raw[3 * i + 1]);
+    }
+    const auto t0 = Clock::now();
+    bfs.apply(b);
+    const auto t1 = Clock::now();
+    std::printf("batch %.9f %u %d\n", std::chrono::duration<double>(t1 - t0).count(), k,
+                bfs.last_used_full_recompute() ? 1 : 0);
+    std::fflush(stdout);
+    if (std::chrono::duration<double>(Clock::now() - t_all).count() > budget) break;
+  }
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   if (argc >= 2 && std::strcmp(argv[1], "bench-bfs") == 0 && argc >= 7) return bench_bfs(argc, argv);
+  if (argc == 6 && std::strcmp(argv[1], "bench-pr") == 0) return bench_pr(argv);
+  if (argc == 8 && std::strcmp(argv[1], "bench-repair") == 0) return bench_repair(argv);
   if (argc >= 3 && std::strcmp(argv[1], "scenario") == 0) {
     std::ifstream f(argv[2]);
     return run_scenario(f);

@@ -117,3 +117,33 @@ def test_root_count_limits(vx):
         vx.MultiSourceBFS(g, [])
     with pytest.raises(ValueError):
         vx.MultiSourceBFS(g, list(range(65)))
+
+
+@pytest.mark.parametrize("alpha", [1e-9, 4.0, 1e9])
+@pytest.mark.parametrize("directed", [False, True])
+def test_push_and_pull_levels_agree(vx, orc, alpha, directed):
+    """alpha = 1e-9: every level pushes; 1e9: only the roots' level pushes,
+    the rest pull; 4: mixed.  All give the oracle's distances."""
+    s, d = vx.rmat_edges(13, 16, 5)
+    g = vx.Graph(1 << 13, directed)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(1 << 13, directed)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    roots = np.random.default_rng(3).integers(0, 1 << 13, size=64).tolist()
+    ms = vx.MultiSourceBFS(g, roots)
+    for blocks in (0, 1, 7):
+        ms.set_tuning(blocks=blocks, alpha=alpha)
+        check_all_roots(ms, g, mg, orc, roots)
+
+
+def test_autotune_keeps_results(vx, orc):
+    s, d = vx.rmat_edges(12, 16, 6)
+    g = vx.Graph(1 << 12, False)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(1 << 12, False)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    roots = list(range(0, 4096, 64))
+    ms = vx.MultiSourceBFS(g, roots)
+    best = ms.autotune(1)
+    assert best["alpha"] > 0 and best["ms"] > 0
+    check_all_roots(ms, g, mg, orc, roots)

@@ -16,6 +16,8 @@ no CPU fallback.
 import torch  # noqa: F401  (see module docstring)
 
 from ._velographx import (  # noqa: F401
+    AdaptivePolicy,
+    AdaptiveSelector,
     Graph,
     IncrementalBFS,
     IncrementalPageRank,
@@ -34,5 +36,5 @@ from ._velographx import (  # noqa: F401
 __all__ = [
     "Graph", "UpdateBatch", "IncrementalBFS", "IncrementalPageRank", "PageRankValidation",
     "from_numpy_edges", "from_scipy_csr", "from_arrow_columns", "from_arrow_table",
-    "rmat_edges", "device_count", "version", "MultiSourceBFS",
+    "rmat_edges", "device_count", "version", "MultiSourceBFS", "AdaptivePolicy", "AdaptiveSelector",
 ]

@@ -33,8 +33,10 @@
 //      launch).  The fixpoint satisfies d(w) <= d(u)+1 on every arc with
 //      realisable upper bounds, hence equals the BFS distances.
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
+#include "policy.hpp"
 #include "vgx_internal.hpp"
 
 namespace vgx {
@@ -1006,6 +1008,32 @@ static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, cons
   VGX_HIP(hipStreamSynchronize(s));
 }
 
+// ------------------------------------------------------ adaptive selector
+
+// Shallow shortest-parent deletions of a batch on the pre-batch distances
+// (benchmarks/adaptive_policy_bfs.cpp:126-148): a deletion (u, v) counts
+// when d(u) + 1 = d(v) <= 3, and for undirected graphs also when
+// d(v) + 1 = d(u) <= 3.
+constexpr uint32_t kShallowDeletionDepth = 3;
+
+__global__ void k_shallow_deletions(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                                    const uint8_t* __restrict__ add, uint64_t k, const uint32_t* __restrict__ dist,
+                                    uint64_t n, int undirected, unsigned long long* count) {
+  unsigned long long c = 0;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < k;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (add[i]) continue;
+    const uint32_t u = src[i], v = dst[i];
+    if (u >= n || v >= n) continue;
+    const uint32_t du = dist[u], dv = dist[v];
+    if (du == kInf || dv == kInf) continue;
+    if (du + 1 == dv && dv <= kShallowDeletionDepth) ++c;
+    if (undirected && dv + 1 == du && du <= kShallowDeletionDepth) ++c;
+  }
+  c = wave_sum(c);
+  if (lane_id() == 0 && c) atomicAdd(count, c);
+}
+
 }  // namespace vgx
 
 using namespace vgx;
@@ -1024,6 +1052,76 @@ using namespace vgx;
   }                                                                   \
   vgx::t_last_error.clear();                                          \
   return VGX_OK;
+
+struct vgx_policy {
+  vgx::AdaptivePolicy model;
+};
+
+struct vgx_selector {
+  vgx_bfs* b = nullptr;
+  vgx::AdaptivePolicy model{0, 0.0, 0.0};
+  bool first_batch = true;
+  vgx::DevBuf<uint8_t> ops;  // staged first batch: src | dst | add
+  unsigned long long* d_cnt = nullptr;
+  unsigned long long* h_cnt = nullptr;
+};
+
+namespace {
+
+using SelClock = std::chrono::steady_clock;
+
+double us_since(SelClock::time_point t0) {
+  return std::chrono::duration<double, std::micro>(SelClock::now() - t0).count();
+}
+
+vgx::PolicyFeatures features_of(const vgx_selector_trace& t) {
+  vgx::PolicyFeatures f;
+  f.update_fraction = t.update_fraction;
+  f.shallow_parent_deletion_fraction = t.shallow_parent_deletion_fraction;
+  f.first_batch = t.first_batch != 0;
+  return f;
+}
+
+void decide_into(const vgx::AdaptivePolicy& m, vgx_selector_trace* t) {
+  const vgx::PolicyDecision d = m.decide(features_of(*t));
+  t->chose_full = d.full ? 1 : 0;
+  t->reason = d.reason;
+  t->predicted_incremental_us = d.predicted_incremental_us;
+  t->predicted_full_us = d.predicted_full_us;
+}
+
+void observe_from(vgx::AdaptivePolicy& m, const vgx_selector_trace& t) {
+  vgx::PolicyDecision d;
+  d.full = t.chose_full != 0;
+  d.reason = t.reason;
+  d.predicted_incremental_us = t.predicted_incremental_us;
+  d.predicted_full_us = t.predicted_full_us;
+  m.observe(features_of(t), d, t.internal_fallback != 0, t.execution_us, t.affected_vertices);
+}
+
+// Fraction of the batch's ops that delete a shallow shortest-parent arc,
+// evaluated on the device distances before the batch.
+double shallow_fraction(vgx_selector* sel, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
+                        uint64_t k) {
+  vgx_bfs* b = sel->b;
+  hipStream_t s = b->g->stream;
+  sel->ops.ensure(k * 9 + 16);
+  uint32_t* d_src = reinterpret_cast<uint32_t*>(sel->ops.p);
+  uint32_t* d_dst = d_src + k;
+  uint8_t* d_add = reinterpret_cast<uint8_t*>(d_dst + k);
+  VGX_HIP(hipMemcpyAsync(d_src, src, k * 4, hipMemcpyHostToDevice, s));
+  VGX_HIP(hipMemcpyAsync(d_dst, dst, k * 4, hipMemcpyHostToDevice, s));
+  VGX_HIP(hipMemcpyAsync(d_add, add, k, hipMemcpyHostToDevice, s));
+  VGX_HIP(hipMemsetAsync(sel->d_cnt, 0, 8, s));
+  k_shallow_deletions<<<blocks_for(k), kBfsBlock, 0, s>>>(d_src, d_dst, d_add, k, b->w.dist.p, b->dist_len,
+                                                          b->g->directed ? 0 : 1, sel->d_cnt);
+  VGX_LAUNCH_CHECK();
+  VGX_HIP(hipMemcpyAsync(sel->h_cnt, sel->d_cnt, 8, hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipStreamSynchronize(s));
+  return static_cast<double>(*sel->h_cnt) / static_cast<double>(std::max<uint64_t>(1, k));
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1137,6 +1235,155 @@ vgx_status vgx_bfs_get_tuning(vgx_bfs* b, vgx_bfs_tuning* t) {
   VGX_API_BEGIN
   if (!b || !t) throw Error(VGX_INVALID_ARGUMENT, "null argument");
   *t = b->tune;
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_autotune(vgx_bfs* b, uint32_t reps, vgx_bfs_tuning* best, double* best_ms) {
+  VGX_API_BEGIN
+  if (!b) throw Error(VGX_INVALID_ARGUMENT, "null bfs");
+  if (reps == 0) reps = 1;
+  VGX_HIP(hipSetDevice(b->g->device));
+  const bool prof = b->profiling;
+  b->profiling = false;
+  auto time_of = [&](const vgx_bfs_tuning& t) {
+    b->tune = t;
+    bfs_full(b);  // settles the chunk length for this setting
+    const auto t0 = SelClock::now();
+    for (uint32_t r = 0; r < reps; ++r) bfs_full(b);
+    return us_since(t0) / 1e3 / reps;
+  };
+  vgx_bfs_tuning cur = b->tune;
+  double cur_ms = time_of(cur);
+  auto sweep = [&](auto setter, const auto& values) {
+    for (auto v : values) {
+      vgx_bfs_tuning t = cur;
+      setter(t, v);
+      const double ms = time_of(t);
+      if (ms < cur_ms) {
+        cur_ms = ms;
+        cur = t;
+      }
+    }
+  };
+  const double alphas[] = {5.0, 10.0, 15.0, 25.0, 40.0};
+  const double betas[] = {12.0, 18.0, 24.0, 48.0};
+  const uint32_t blocks[] = {0u, 512u, 1024u, 2048u, 4096u};
+  const uint32_t lps[] = {4u, 8u, 16u};
+  sweep([](vgx_bfs_tuning& t, double v) { t.alpha = v; }, alphas);
+  sweep([](vgx_bfs_tuning& t, double v) { t.beta = v; }, betas);
+  sweep([](vgx_bfs_tuning& t, uint32_t v) { t.blocks = v; }, blocks);
+  sweep([](vgx_bfs_tuning& t, uint32_t v) { t.levels_per_sync = v; }, lps);
+  b->tune = cur;
+  b->profiling = prof;
+  if (best) *best = cur;
+  if (best_ms) *best_ms = cur_ms;
+  VGX_API_END
+}
+
+const char* vgx_policy_reason_name(int reason) { return vgx::policy_reason_name(reason); }
+
+vgx_status vgx_policy_create(uint64_t vertices, double initial_full_us, double initial_reachable_fraction,
+                             vgx_policy** out) {
+  VGX_API_BEGIN
+  if (!out) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  if (!(initial_full_us >= 0.0)) throw Error(VGX_INVALID_ARGUMENT, "initial_full_us must be non-negative");
+  *out = new vgx_policy{vgx::AdaptivePolicy(vertices, initial_full_us, initial_reachable_fraction)};
+  VGX_API_END
+}
+
+vgx_status vgx_policy_destroy(vgx_policy* p) {
+  VGX_API_BEGIN
+  delete p;
+  VGX_API_END
+}
+
+vgx_status vgx_policy_decide(vgx_policy* p, vgx_selector_trace* t) {
+  VGX_API_BEGIN
+  if (!p || !t) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  decide_into(p->model, t);
+  VGX_API_END
+}
+
+vgx_status vgx_policy_observe(vgx_policy* p, const vgx_selector_trace* t) {
+  VGX_API_BEGIN
+  if (!p || !t) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  observe_from(p->model, *t);
+  VGX_API_END
+}
+
+vgx_status vgx_selector_create(vgx_bfs* b, vgx_selector** out) {
+  VGX_API_BEGIN
+  if (!b || !out) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  VGX_HIP(hipSetDevice(b->g->device));
+  auto* sel = new vgx_selector();
+  try {
+    sel->b = b;
+    sel->d_cnt = static_cast<unsigned long long*>(dev_alloc(8));
+    VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&sel->h_cnt), 8, hipHostMallocDefault));
+    // the initial full BFS is the full arm's first sample (harness :59-65)
+    const auto t0 = SelClock::now();
+    bfs_full(b);
+    const double full_us = us_since(t0);
+    const uint64_t n = b->g->n;
+    const double reach = static_cast<double>(b->reachable) / static_cast<double>(std::max<uint64_t>(1, n));
+    sel->model = vgx::AdaptivePolicy(n, full_us, reach);
+  } catch (...) {
+    vgx_selector_destroy(sel);
+    throw;
+  }
+  *out = sel;
+  VGX_API_END
+}
+
+vgx_status vgx_selector_destroy(vgx_selector* sel) {
+  VGX_API_BEGIN
+  if (!sel) return VGX_OK;
+  if (sel->b) (void)hipStreamSynchronize(sel->b->g->stream);
+  if (sel->d_cnt) dev_free(sel->d_cnt);
+  if (sel->h_cnt) (void)hipHostFree(sel->h_cnt);
+  delete sel;
+  VGX_API_END
+}
+
+vgx_status vgx_selector_apply(vgx_selector* sel, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
+                              uint64_t k, vgx_selector_trace* trace) {
+  VGX_API_BEGIN
+  if (!sel || (k && (!src || !dst || !add))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  vgx_bfs* b = sel->b;
+  vgx_graph* g = b->g;
+  VGX_HIP(hipSetDevice(g->device));
+  vgx_selector_trace t{};
+  const auto t_dec = SelClock::now();
+  t.first_batch = sel->first_batch ? 1 : 0;
+  t.update_fraction =
+      static_cast<double>(k) / static_cast<double>(std::max<uint64_t>(1, g->adj[0].live));
+  t.shallow_parent_deletion_fraction = (sel->first_batch && k) ? shallow_fraction(sel, src, dst, add, k) : 0.0;
+  decide_into(sel->model, &t);
+  t.decision_us = us_since(t_dec);
+  const auto t_exec = SelClock::now();
+  if (t.chose_full) {
+    // graph.apply(batch); bfs.recompute()  (harness :245-248)
+    if (k) {
+      graph_stage(g, src, dst, add, k);
+      graph_apply_deletions(g);
+      graph_apply_additions(g);
+      graph_finish_batch(g);
+    }
+    bfs_full(b);
+    b->last_candidates = 0;
+    b->last_affected = 0;
+    b->last_relaxed = 0;
+    b->last_full = true;
+  } else {
+    bfs_apply(b, src, dst, add, k);
+    t.affected_vertices = b->last_affected;
+    t.internal_fallback = b->last_full ? 1 : 0;
+  }
+  VGX_HIP(hipStreamSynchronize(g->stream));
+  t.execution_us = us_since(t_exec);
+  observe_from(sel->model, t);
+  sel->first_batch = false;
+  if (trace) *trace = t;
   VGX_API_END
 }
 

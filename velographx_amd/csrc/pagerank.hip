@@ -31,6 +31,9 @@ extern thread_local std::string t_last_error;
 constexpr int kPrBlock = 256;
 constexpr uint32_t kSmallRow = 32;
 constexpr uint32_t kChunk = 4096;
+// Workgroups per in-degree class and launch; each class loops grid-stride,
+// so the last workgroup reduces at most 3 x 1024 partials.
+constexpr uint64_t kMaxClassBlocks = 1024;
 
 struct PrState {
   double dangling_share;  // d * dangling / n for the next iteration
@@ -169,17 +172,17 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_iter(PrArgs a) {
   const T base = static_cast<T>(st->teleport + st->dangling_share);
   double l1 = 0.0, linf = 0.0, dang = 0.0;
   const uint32_t b = blockIdx.x;
+  // Each class is covered grid-stride by its own range of workgroups.
   if (b < a.bs) {
-    const uint32_t i = b * kPrBlock + threadIdx.x;
-    if (i < a.ns) {
+    for (uint32_t i = b * kPrBlock + threadIdx.x; i < a.ns; i += a.bs * kPrBlock) {
       const uint32_t v = a.slist[i];
       Acc<T> acc;
       for (uint32_t e = a.off[v]; e < a.off[v + 1]; ++e) acc.add(contrib[a.adj[e]]);
       finish_vertex<T>(a, v, acc.get(), base, rank, next, cnext, l1, linf, dang);
     }
   } else if (b < a.bs + a.bm) {
-    const uint32_t i = (b - a.bs) * (kPrBlock / kWave) + threadIdx.x / kWave;
-    if (i < a.nm) {
+    for (uint32_t i = (b - a.bs) * (kPrBlock / kWave) + threadIdx.x / kWave; i < a.nm;
+         i += a.bm * (kPrBlock / kWave)) {
       const uint32_t v = a.mlist[i];
       Acc<T> acc;
       for (uint32_t e = a.off[v] + lane_id(); e < a.off[v + 1]; e += kWave) acc.add(contrib[a.adj[e]]);
@@ -187,8 +190,9 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_iter(PrArgs a) {
       if (lane_id() == 0) finish_vertex<T>(a, v, s, base, rank, next, cnext, l1, linf, dang);
     }
   } else {
-    const uint32_t c = (b - a.bs - a.bm) * (kPrBlock / kWave) + threadIdx.x / kWave;
-    if (c < a.nchunks) {
+    const uint32_t bh = gridDim.x - a.bs - a.bm;
+    for (uint32_t c = (b - a.bs - a.bm) * (kPrBlock / kWave) + threadIdx.x / kWave; c < a.nchunks;
+         c += bh * (kPrBlock / kWave)) {
       const uint32_t hv = a.hlist[a.hchunk_v[c]];
       const uint32_t e0 = a.hchunk_off[c];
       const uint32_t e1 = min(e0 + kChunk, a.off[hv + 1]);
@@ -216,21 +220,18 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_iter(PrArgs a) {
       acc.add(__hip_atomic_load(&hpart[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     finish_vertex<T>(a, v, acc.get(), base, rank, next, cnext, hl1, hlinf, hdang);
   }
+  // per-workgroup partials: thread t takes t, t + 256, ... in order, then
+  // the fixed-order block reduction (the result does not depend on timing)
+  for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+    hl1 += __hip_atomic_load(&a.partials[i].l1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double pl = __hip_atomic_load(&a.partials[i].linf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    hlinf = hlinf > pl ? hlinf : pl;
+    hdang += __hip_atomic_load(&a.partials[i].dangling, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
   block_reduce3<T>(hl1, hlinf, hdang, sm);
   if (threadIdx.x == 0) {
-    double L1 = 0.0, LI = 0.0, D = 0.0;
-    for (uint32_t i = 0; i < gridDim.x; ++i) {
-      PrPartial p;
-      p.l1 = __hip_atomic_load(&a.partials[i].l1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      p.linf = __hip_atomic_load(&a.partials[i].linf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      p.dangling = __hip_atomic_load(&a.partials[i].dangling, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      L1 += p.l1;
-      LI = LI > p.linf ? LI : p.linf;
-      D += p.dangling;
-    }
-    L1 += hl1;
-    LI = LI > hlinf ? LI : hlinf;
-    D += hdang;
+    const double L1 = hl1, LI = hlinf, D = hdang;
     st->l1 = L1;
     st->linf = LI;
     st->dangling_share = st->damping * D / static_cast<double>(a.n);
@@ -265,11 +266,14 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_init(PrArgs a, int warm) {
     s_last = atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1;
   }
   __syncthreads();
-  if (s_last && threadIdx.x == 0) {
-    __threadfence();
-    double D = 0.0;
-    for (uint32_t i = 0; i < gridDim.x; ++i)
-      D += __hip_atomic_load(&a.partials[i].dangling, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!s_last) return;
+  __threadfence();
+  double D = 0.0, y1 = 0.0, y2 = 0.0;
+  for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+    D += __hip_atomic_load(&a.partials[i].dangling, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  block_reduce3<T>(y1, y2, D, sm);
+  if (threadIdx.x == 0) {
     st->dangling_share = st->damping * D / static_cast<double>(a.n);
     st->blocks_done = 0;
     __threadfence();
@@ -535,8 +539,8 @@ static PrArgs pr_args(vgx_pagerank* p, DevBuf<uint8_t>* rk) {
   a.nm = static_cast<uint32_t>(p->nm);
   a.nh = static_cast<uint32_t>(p->nh);
   a.nchunks = static_cast<uint32_t>(p->nchunks);
-  a.bs = static_cast<uint32_t>((p->ns + kPrBlock - 1) / kPrBlock);
-  a.bm = static_cast<uint32_t>((p->nm + 3) / 4);
+  a.bs = static_cast<uint32_t>(std::min<uint64_t>((p->ns + kPrBlock - 1) / kPrBlock, kMaxClassBlocks));
+  a.bm = static_cast<uint32_t>(std::min<uint64_t>((p->nm + 3) / 4, kMaxClassBlocks));
   a.n = static_cast<uint32_t>(g->n);
   a.rank0 = rk[0].p;
   a.rank1 = rk[1].p;
@@ -569,7 +573,8 @@ static int full_solve(vgx_pagerank* p, DevBuf<uint8_t>* rk, int precision, uint6
   h.done = (n == 0 || max_iterations == 0) ? 1u : 0u;
   VGX_HIP(hipMemcpyAsync(p->d_st, &h, sizeof(h), hipMemcpyHostToDevice, s));
   PrArgs a = pr_args(p, rk);
-  const unsigned grid = a.bs + a.bm + static_cast<uint32_t>((p->nchunks + 3) / 4);
+  const unsigned grid =
+      a.bs + a.bm + static_cast<uint32_t>(std::min<uint64_t>((p->nchunks + 3) / 4, kMaxClassBlocks));
   const unsigned ginit = gblocks(n, kPrBlock, 2048);
   p->partials.ensure(std::max<unsigned>(grid, ginit) + 1);
   a.partials = p->partials.p;

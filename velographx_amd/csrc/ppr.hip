@@ -14,9 +14,16 @@
 // are indexed by the arcs of the graph's out-CSR snapshot
 // (vgx_graph_csr_device / vgx_graph_export_csr order).
 //
-// One wave per vertex; the two half-waves take one arc each per step and
-// their 32 lanes the K <= 32 columns, so every arc's row of X is one
-// coalesced 128-byte load.
+// Work is split into chunks of at most 512 arcs of one row (PprPlan), one
+// wave per chunk, so a hub row of R-MAT (tens of thousands of arcs) spreads
+// over many waves instead of serialising one.  Rows of several chunks get
+// per-chunk partials that a second pass adds in chunk order: every sum has
+// a fixed order and the results do not depend on scheduling.
+// In a sweep the two half-waves take one arc each per step (four steps in
+// flight) and their 32 lanes the K <= 32 columns, so an arc's row of X is
+// one coalesced 128-byte load.
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 
 #include "vgx_internal.hpp"
@@ -26,98 +33,230 @@ vgx_status set_error(vgx_status s, const std::string& msg);
 extern thread_local std::string t_last_error;
 
 constexpr int kPprBlock = 256;
+constexpr uint32_t kPprChunk = 512;
+constexpr uint32_t kPprWaves = kPprBlock / kWave;
 
-// W_u and normalised weights in out-arc order.
-__global__ void k_ppr_normalise(const uint32_t* __restrict__ off, const float* __restrict__ w, uint32_t n,
-                                float* __restrict__ pn, float* __restrict__ wsum) {
-  const uint32_t W = gridDim.x * (blockDim.x / kWave);
-  for (uint32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += W) {
-    const uint32_t b = off[u], e = off[u + 1];
+// ------------------------------------------------------------ chunk plans
+
+__global__ void k_ppr_chunk_count(const uint32_t* __restrict__ off, uint32_t n, uint32_t* __restrict__ cnt) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint32_t len = off[v + 1] - off[v];
+    cnt[v] = len > kPprChunk ? (len + kPprChunk - 1) / kPprChunk : 1u;
+  }
+}
+
+__global__ void k_ppr_chunk_fill(const uint32_t* __restrict__ row_chunk, uint32_t n, uint32_t* __restrict__ chunk_row) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    for (uint32_t c = row_chunk[v]; c < row_chunk[v + 1]; ++c) chunk_row[c] = v;
+}
+
+// Arc range of chunk c of row v.
+__device__ __forceinline__ void chunk_range(const uint32_t* off, const uint32_t* row_chunk, uint32_t v, uint32_t c,
+                                            uint32_t& b, uint32_t& e) {
+  b = off[v] + (c - row_chunk[v]) * kPprChunk;
+  e = min(b + kPprChunk, off[v + 1]);
+}
+
+// --------------------------------------------------------- normalisation
+
+// Per-chunk sums of the out-weights.
+__global__ void k_ppr_wsum_part(const uint32_t* __restrict__ off, const uint32_t* __restrict__ row_chunk,
+                                const uint32_t* __restrict__ chunk_row, uint32_t nchunks,
+                                const float* __restrict__ w, float* __restrict__ cpart) {
+  for (uint32_t c = blockIdx.x * kPprWaves + threadIdx.x / kWave; c < nchunks; c += gridDim.x * kPprWaves) {
+    const uint32_t u = chunk_row[c];
+    uint32_t b, e;
+    chunk_range(off, row_chunk, u, c, b, e);
     float s = 0.f;
     for (uint32_t i = b + lane_id(); i < e; i += kWave) s += w[i];
     s = wave_sum(s);
+    if (lane_id() == 0) cpart[c] = s;
+  }
+}
+
+// W_u = chunk sums of row u in chunk order.
+__global__ void k_ppr_wsum_row(const uint32_t* __restrict__ row_chunk, uint32_t n, const float* __restrict__ cpart,
+                               float* __restrict__ wsum) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (uint32_t c = row_chunk[u]; c < row_chunk[u + 1]; ++c) s += cpart[c];
+    wsum[u] = s;
+  }
+}
+
+__global__ void k_ppr_normalise(const uint32_t* __restrict__ off, const uint32_t* __restrict__ row_chunk,
+                                const uint32_t* __restrict__ chunk_row, uint32_t nchunks,
+                                const float* __restrict__ w, const float* __restrict__ wsum,
+                                float* __restrict__ pn) {
+  for (uint32_t c = blockIdx.x * kPprWaves + threadIdx.x / kWave; c < nchunks; c += gridDim.x * kPprWaves) {
+    const uint32_t u = chunk_row[c];
+    uint32_t b, e;
+    chunk_range(off, row_chunk, u, c, b, e);
+    const float s = wsum[u];
     for (uint32_t i = b + lane_id(); i < e; i += kWave) pn[i] = s > 0.f ? w[i] / s : 0.f;
-    if (lane_id() == 0) wsum[u] = s;
   }
 }
 
 // Position of each in-arc (v <- u) in the out-CSR: binary search of v in
-// u's sorted out-row.
+// u's sorted out-row, one lane per in-arc.
 __global__ void k_ppr_in_to_out(const uint32_t* __restrict__ in_off, const uint32_t* __restrict__ in_adj,
-                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj, uint32_t n,
+                                const uint32_t* __restrict__ row_chunk, const uint32_t* __restrict__ chunk_row,
+                                uint32_t nchunks, const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
                                 const float* __restrict__ pn, float* __restrict__ pin) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    for (uint32_t e = in_off[v]; e < in_off[v + 1]; ++e) {
-      const uint32_t u = in_adj[e];
+  for (uint32_t c = blockIdx.x * kPprWaves + threadIdx.x / kWave; c < nchunks; c += gridDim.x * kPprWaves) {
+    const uint32_t v = chunk_row[c];
+    uint32_t b, e;
+    chunk_range(in_off, row_chunk, v, c, b, e);
+    for (uint32_t i = b + lane_id(); i < e; i += kWave) {
+      const uint32_t u = in_adj[i];
       uint32_t lo = off[u], hi = off[u + 1];
       while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if (adj[mid] < v) lo = mid + 1;
         else hi = mid;
       }
-      pin[e] = pn[lo];
+      pin[i] = pn[lo];
     }
   }
 }
 
-// y[v,:] = a * b[v,:] + d * sum_e p[e] * x[nbr_e,:] over the row of v.
+// ------------------------------------------------------------------ sweeps
+
+// Over one chunk of row v: sum_e p[e] * x[nbr_e, col].  Half-wave h takes
+// arcs b+h, b+h+2, ...; four of them are loaded before any is used.
+__device__ __forceinline__ float chunk_gather(const uint32_t* __restrict__ nbr, const float* __restrict__ p,
+                                              const float* __restrict__ x, uint32_t b, uint32_t e, uint32_t K,
+                                              unsigned half, unsigned col) {
+  float acc = 0.f;
+  const bool on = col < K;
+  uint32_t i = b + half;
+  for (; i + 6 < e; i += 8) {
+    const uint32_t u0 = nbr[i], u1 = nbr[i + 2], u2 = nbr[i + 4], u3 = nbr[i + 6];
+    const float p0 = p[i], p1 = p[i + 2], p2 = p[i + 4], p3 = p[i + 6];
+    if (on) {
+      const float x0 = x[static_cast<uint64_t>(u0) * K + col];
+      const float x1 = x[static_cast<uint64_t>(u1) * K + col];
+      const float x2 = x[static_cast<uint64_t>(u2) * K + col];
+      const float x3 = x[static_cast<uint64_t>(u3) * K + col];
+      acc += p0 * x0;
+      acc += p1 * x1;
+      acc += p2 * x2;
+      acc += p3 * x3;
+    }
+  }
+  for (; i < e; i += 2) {
+    const uint32_t u = nbr[i];
+    const float pw = p[i];
+    if (on) acc += pw * x[static_cast<uint64_t>(u) * K + col];
+  }
+  return acc + __shfl_xor(acc, 32, kWave);
+}
+
+// y[v,:] = a * b[v,:] + d * sum_e p[e] * x[nbr_e,:]; rows of several chunks
+// leave per-chunk partial rows for k_ppr_finish.
 __global__ __launch_bounds__(kPprBlock) void k_ppr_sweep(const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ nbr,
-                                                         const float* __restrict__ p, uint32_t n, uint32_t K,
-                                                         float a, const float* __restrict__ b, float d,
-                                                         const float* __restrict__ x, float* __restrict__ y) {
-  const uint32_t W = gridDim.x * (blockDim.x / kWave);
+                                                         const float* __restrict__ p,
+                                                         const uint32_t* __restrict__ row_chunk,
+                                                         const uint32_t* __restrict__ chunk_row, uint32_t nchunks,
+                                                         uint32_t K, float a, const float* __restrict__ b, float d,
+                                                         const float* __restrict__ x, float* __restrict__ y,
+                                                         float* __restrict__ part) {
   const unsigned lane = lane_id();
   const unsigned half = lane >> 5, col = lane & 31;
-  for (uint32_t v = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; v < n; v += W) {
-    const uint32_t s = off[v], e = off[v + 1];
-    float acc = 0.f;
-    for (uint32_t i = s + half; i < e; i += 2) {
-      const uint32_t u = nbr[i];
-      const float pw = p[i];
-      if (col < K) acc += pw * x[static_cast<uint64_t>(u) * K + col];
-    }
-    acc += __shfl_xor(acc, 32, kWave);
+  for (uint32_t c = blockIdx.x * kPprWaves + threadIdx.x / kWave; c < nchunks; c += gridDim.x * kPprWaves) {
+    const uint32_t v = chunk_row[c];
+    const bool single = row_chunk[v + 1] - row_chunk[v] == 1;
+    uint32_t s, e;
+    chunk_range(off, row_chunk, v, c, s, e);
+    const float acc = chunk_gather(nbr, p, x, s, e, K, half, col);
     if (half == 0 && col < K) {
-      const uint64_t o = static_cast<uint64_t>(v) * K + col;
-      y[o] = a * b[o] + d * acc;
+      if (single) {
+        const uint64_t o = static_cast<uint64_t>(v) * K + col;
+        y[o] = a * b[o] + d * acc;
+      } else {
+        part[static_cast<uint64_t>(c) * K + col] = acc;
+      }
     }
   }
 }
 
-// grad_w over the out-CSR: first grad_P per arc and the row's weighted sum,
-// then the normalisation's chain rule.
-__global__ __launch_bounds__(kPprBlock) void k_ppr_edge_grad(const uint32_t* __restrict__ off,
-                                                             const uint32_t* __restrict__ adj,
-                                                             const float* __restrict__ pn,
-                                                             const float* __restrict__ wsum, uint32_t n, uint32_t K,
-                                                             float d, const float* __restrict__ x,
-                                                             const float* __restrict__ lam,
-                                                             float* __restrict__ gw) {
-  const uint32_t W = gridDim.x * (blockDim.x / kWave);
+// Rows of several chunks: add the partial rows in chunk order.
+__global__ __launch_bounds__(kPprBlock) void k_ppr_finish(const uint32_t* __restrict__ row_chunk, uint32_t n,
+                                                          uint32_t K, float a, const float* __restrict__ b, float d,
+                                                          const float* __restrict__ part, float* __restrict__ y) {
   const unsigned lane = lane_id();
-  const unsigned half = lane >> 5, col = lane & 31;
-  for (uint32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += W) {
-    const uint32_t s = off[u], e = off[u + 1];
-    if (s == e) continue;
-    const float xu = col < K ? x[static_cast<uint64_t>(u) * K + col] : 0.f;
-    float t = 0.f;
-    for (uint32_t i0 = s; i0 < e; i0 += 2) {
-      const uint32_t i = i0 + half;
-      float prod = 0.f;
-      if (i < e && col < K) prod = xu * lam[static_cast<uint64_t>(adj[i]) * K + col];
-#pragma unroll
-      for (int m = 16; m > 0; m >>= 1) prod += __shfl_xor(prod, m, kWave);
-      const float gp = d * prod;
-      if (i < e && col == 0) gw[i] = gp;
-      if (i < e) t += pn[i] * gp;
+  const uint32_t W = gridDim.x * kPprWaves;
+  for (uint32_t v0 = (blockIdx.x * kPprWaves + threadIdx.x / kWave) * kWave; v0 < n; v0 += W * kWave) {
+    const uint32_t v = v0 + lane;
+    const bool multi = v < n && row_chunk[v + 1] - row_chunk[v] > 1;
+    unsigned long long m = __ballot(multi);
+    while (m) {
+      const int l = __ffsll(static_cast<long long>(m)) - 1;
+      m &= m - 1;
+      const uint32_t r = v0 + l;
+      if (lane < K) {
+        float acc = 0.f;
+        for (uint32_t c = row_chunk[r]; c < row_chunk[r + 1]; ++c) acc += part[static_cast<uint64_t>(c) * K + lane];
+        const uint64_t o = static_cast<uint64_t>(r) * K + lane;
+        y[o] = a * b[o] + d * acc;
+      }
     }
-    // t is the same on the 32 lanes of a half; add the two halves
-    t = __shfl(t, 0, kWave) + __shfl(t, 32, kWave);
-    // the lanes below read grad_P values other lanes of this wave stored
-    __threadfence_block();
+  }
+}
+
+// ---------------------------------------------------------- edge gradient
+
+// Per out-arc i = (u -> v): G = d <X[u,:], L[v,:]>, stored in gw; the
+// chunk's sum of P_uv * G goes to cpart.  One lane per arc; X[u,:] is the
+// same for the whole chunk and sits in registers.
+__global__ __launch_bounds__(kPprBlock) void k_ppr_edge_dot(const uint32_t* __restrict__ off,
+                                                            const uint32_t* __restrict__ adj,
+                                                            const float* __restrict__ pn,
+                                                            const uint32_t* __restrict__ row_chunk,
+                                                            const uint32_t* __restrict__ chunk_row, uint32_t nchunks,
+                                                            uint32_t K, float d, const float* __restrict__ x,
+                                                            const float* __restrict__ lam, float* __restrict__ gw,
+                                                            float* __restrict__ cpart) {
+  for (uint32_t c = blockIdx.x * kPprWaves + threadIdx.x / kWave; c < nchunks; c += gridDim.x * kPprWaves) {
+    const uint32_t u = chunk_row[c];
+    uint32_t b, e;
+    chunk_range(off, row_chunk, u, c, b, e);
+    float xu[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) xu[k] = k < static_cast<int>(K) ? x[static_cast<uint64_t>(u) * K + k] : 0.f;
+    float t = 0.f;
+    for (uint32_t i = b + lane_id(); i < e; i += kWave) {
+      const float* lv = lam + static_cast<uint64_t>(adj[i]) * K;
+      float dot = 0.f;
+#pragma unroll
+      for (int k = 0; k < 32; ++k)
+        if (k < static_cast<int>(K)) dot += xu[k] * lv[k];
+      const float g = d * dot;
+      gw[i] = g;
+      t += pn[i] * g;
+    }
+    t = wave_sum(t);
+    if (lane_id() == 0) cpart[c] = t;
+  }
+}
+
+// grad_w_uv = (G_uv - sum over u's chunks of cpart, in chunk order) / W_u.
+__global__ __launch_bounds__(kPprBlock) void k_ppr_edge_norm(const uint32_t* __restrict__ off,
+                                                             const uint32_t* __restrict__ row_chunk,
+                                                             const uint32_t* __restrict__ chunk_row,
+                                                             uint32_t nchunks, const float* __restrict__ wsum,
+                                                             const float* __restrict__ cpart,
+                                                             float* __restrict__ gw) {
+  for (uint32_t c = blockIdx.x * kPprWaves + threadIdx.x / kWave; c < nchunks; c += gridDim.x * kPprWaves) {
+    const uint32_t u = chunk_row[c];
+    uint32_t b, e;
+    chunk_range(off, row_chunk, u, c, b, e);
+    if (b == e) continue;
+    float t = 0.f;
+    for (uint32_t j = row_chunk[u]; j < row_chunk[u + 1]; ++j) t += cpart[j];
     const float ws = wsum[u];
-    for (uint32_t i = s + lane; i < e; i += kWave) gw[i] = ws > 0.f ? (gw[i] - t) / ws : 0.f;
+    for (uint32_t i = b + lane_id(); i < e; i += kWave) gw[i] = ws > 0.f ? (gw[i] - t) / ws : 0.f;
   }
 }
 
@@ -136,6 +275,36 @@ static unsigned ppr_blocks(uint64_t items, unsigned per, unsigned cap = 8192) {
 
 static PprWork& ppr_work(vgx_graph* g) { return g->ppr; }
 
+// Chunk plan of one snapshot, rebuilt when the snapshot changes.
+static const PprPlan& ppr_plan(vgx_graph* g, int dir, const CsrSnapshot& sn, hipStream_t s) {
+  PprWork& pw = ppr_work(g);
+  PprPlan& pl = pw.plan[dir];
+  if (pl.version == sn.version && pl.n == sn.n && pl.row_chunk.p) return pl;
+  const uint32_t n = static_cast<uint32_t>(sn.n);
+  pl.row_chunk.ensure(sn.n + 2);
+  VGX_HIP(hipMemsetAsync(pl.row_chunk.p, 0, sizeof(uint32_t), s));
+  uint32_t total = 0;
+  if (n) {
+    DevBuf<uint32_t> cnt;
+    cnt.ensure(sn.n + 1);
+    k_ppr_chunk_count<<<ppr_blocks(n, kPprBlock), kPprBlock, 0, s>>>(sn.off.p, n, cnt.p);
+    VGX_LAUNCH_CHECK();
+    size_t bytes = 0;
+    VGX_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, cnt.p, pl.row_chunk.p + 1, static_cast<int>(n), s));
+    pw.scan.ensure(bytes + 16);
+    VGX_HIP(hipcub::DeviceScan::InclusiveSum(pw.scan.p, bytes, cnt.p, pl.row_chunk.p + 1, static_cast<int>(n), s));
+    VGX_HIP(hipMemcpyAsync(&total, pl.row_chunk.p + n, sizeof(total), hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    pl.chunk_row.ensure(static_cast<uint64_t>(total) + 1);
+    k_ppr_chunk_fill<<<ppr_blocks(n, kPprBlock), kPprBlock, 0, s>>>(pl.row_chunk.p, n, pl.chunk_row.p);
+    VGX_LAUNCH_CHECK();
+  }
+  pl.nchunks = total;
+  pl.version = sn.version;
+  pl.n = sn.n;
+  return pl;
+}
+
 // Normalised weights (and their in-arc copy) on the caller's stream, after
 // the graph's CSR snapshots are complete.
 static void ppr_prepare(vgx_graph* g, const float* w, hipStream_t s, bool need_in) {
@@ -143,21 +312,56 @@ static void ppr_prepare(vgx_graph* g, const float* w, hipStream_t s, bool need_i
   const CsrSnapshot* in = need_in ? &graph_snapshot(g, 1) : nullptr;
   VGX_HIP(hipStreamSynchronize(g->stream));
   PprWork& pw = ppr_work(g);
+  const PprPlan& po = ppr_plan(g, 0, out, s);
   const uint32_t n = static_cast<uint32_t>(out.n);
   pw.pn.ensure(out.arcs + 1);
   pw.wsum.ensure(out.n + 1);
+  pw.cpart.ensure(po.nchunks + 1);
   if (n) {
-    k_ppr_normalise<<<ppr_blocks(n, kPprBlock / kWave), kPprBlock, 0, s>>>(out.off.p, w, n, pw.pn.p, pw.wsum.p);
+    const unsigned gc = ppr_blocks(po.nchunks, kPprWaves);
+    k_ppr_wsum_part<<<gc, kPprBlock, 0, s>>>(out.off.p, po.row_chunk.p, po.chunk_row.p,
+                                             static_cast<uint32_t>(po.nchunks), w, pw.cpart.p);
+    VGX_LAUNCH_CHECK();
+    k_ppr_wsum_row<<<ppr_blocks(n, kPprBlock), kPprBlock, 0, s>>>(po.row_chunk.p, n, pw.cpart.p, pw.wsum.p);
+    VGX_LAUNCH_CHECK();
+    k_ppr_normalise<<<gc, kPprBlock, 0, s>>>(out.off.p, po.row_chunk.p, po.chunk_row.p,
+                                             static_cast<uint32_t>(po.nchunks), w, pw.wsum.p, pw.pn.p);
     VGX_LAUNCH_CHECK();
   }
   if (in) {
+    const PprPlan& pi = ppr_plan(g, 1, *in, s);
     pw.pin.ensure(in->arcs + 1);
     if (n) {
-      k_ppr_in_to_out<<<ppr_blocks(n, kPprBlock), kPprBlock, 0, s>>>(in->off.p, in->adj.p, out.off.p, out.adj.p, n,
-                                                                      pw.pn.p, pw.pin.p);
+      k_ppr_in_to_out<<<ppr_blocks(pi.nchunks, kPprWaves), kPprBlock, 0, s>>>(
+          in->off.p, in->adj.p, pi.row_chunk.p, pi.chunk_row.p, static_cast<uint32_t>(pi.nchunks), out.off.p,
+          out.adj.p, pw.pn.p, pw.pin.p);
       VGX_LAUNCH_CHECK();
     }
   }
+}
+
+// `iterations` sweeps y <- a*b + d * (P-gather of cur) over one direction,
+// ping-ponging between `io` and the scratch buffer; the result ends in io.
+static void ppr_sweeps(vgx_graph* g, const CsrSnapshot& sn, const PprPlan& pl, const float* p, uint32_t K, float a,
+                       const float* b, float d, uint32_t iterations, float* io, hipStream_t s) {
+  PprWork& pw = ppr_work(g);
+  const uint32_t n = static_cast<uint32_t>(sn.n);
+  const uint64_t nk = static_cast<uint64_t>(n) * K;
+  pw.tmp.ensure(nk + 1);
+  pw.part.ensure(pl.nchunks * K + 1);
+  const unsigned gs = ppr_blocks(pl.nchunks, kPprWaves);
+  const unsigned gf = ppr_blocks((static_cast<uint64_t>(n) + kWave - 1) / kWave, kPprWaves, 2048);
+  float* cur = io;
+  float* nxt = pw.tmp.p;
+  for (uint32_t it = 0; it < iterations; ++it) {
+    k_ppr_sweep<<<gs, kPprBlock, 0, s>>>(sn.off.p, sn.adj.p, p, pl.row_chunk.p, pl.chunk_row.p,
+                                         static_cast<uint32_t>(pl.nchunks), K, a, b, d, cur, nxt, pw.part.p);
+    VGX_LAUNCH_CHECK();
+    k_ppr_finish<<<gf, kPprBlock, 0, s>>>(pl.row_chunk.p, n, K, a, b, d, pw.part.p, nxt);
+    VGX_LAUNCH_CHECK();
+    std::swap(cur, nxt);
+  }
+  if (cur != io) VGX_HIP(hipMemcpyAsync(io, cur, nk * 4, hipMemcpyDeviceToDevice, s));
 }
 
 }  // namespace vgx
@@ -193,23 +397,13 @@ vgx_status vgx_ppr_forward(vgx_graph* g, const float* w_dev, const float* s_dev,
   ppr_prepare(g, w_dev, s, true);
   const CsrSnapshot& in = graph_snapshot(g, 1);
   PprWork& pw = ppr_work(g);
-  const uint32_t n = static_cast<uint32_t>(in.n);
-  const uint64_t nk = static_cast<uint64_t>(n) * K;
-  pw.tmp.ensure(nk + 1);
+  const uint64_t nk = in.n * K;
   const float a = static_cast<float>(1.0 - damping), d = static_cast<float>(damping);
   if (nk) {
-    // X_0 = (1-d) S, then X <- (1-d) S + d P^T X
+    // X_0 = (1-d) S, then X <- (1-d) S + d P^T X over the in-CSR
     k_scale<<<ppr_blocks(nk, 256), 256, 0, s>>>(s_dev, a, nk, x_dev);
     VGX_LAUNCH_CHECK();
-    float* cur = x_dev;
-    float* nxt = pw.tmp.p;
-    for (uint32_t it = 0; it < iterations; ++it) {
-      k_ppr_sweep<<<ppr_blocks(n, kPprBlock / kWave), kPprBlock, 0, s>>>(in.off.p, in.adj.p, pw.pin.p, n, K, a,
-                                                                         s_dev, d, cur, nxt);
-      VGX_LAUNCH_CHECK();
-      std::swap(cur, nxt);
-    }
-    if (cur != x_dev) VGX_HIP(hipMemcpyAsync(x_dev, cur, nk * 4, hipMemcpyDeviceToDevice, s));
+    ppr_sweeps(g, in, pw.plan[1], pw.pin.p, K, a, s_dev, d, iterations, x_dev, s);
   }
   VGX_HIP(hipStreamSynchronize(s));
   VGX_API_END
@@ -227,30 +421,26 @@ vgx_status vgx_ppr_backward(vgx_graph* g, const float* w_dev, const float* x_dev
   ppr_prepare(g, w_dev, s, false);
   const CsrSnapshot& out = graph_snapshot(g, 0);
   PprWork& pw = ppr_work(g);
-  const uint32_t n = static_cast<uint32_t>(out.n);
-  const uint64_t nk = static_cast<uint64_t>(n) * K;
-  pw.tmp.ensure(nk + 1);
+  const PprPlan& po = pw.plan[0];
+  const uint64_t nk = out.n * K;
   const float d = static_cast<float>(damping);
   if (nk) {
     // L_0 = C, then L <- C + d P L over the out-CSR
     VGX_HIP(hipMemcpyAsync(lambda_dev, c_dev, nk * 4, hipMemcpyDeviceToDevice, s));
-    float* cur = lambda_dev;
-    float* nxt = pw.tmp.p;
-    for (uint32_t it = 0; it < iterations; ++it) {
-      k_ppr_sweep<<<ppr_blocks(n, kPprBlock / kWave), kPprBlock, 0, s>>>(out.off.p, out.adj.p, pw.pn.p, n, K, 1.f,
-                                                                         c_dev, d, cur, nxt);
-      VGX_LAUNCH_CHECK();
-      std::swap(cur, nxt);
-    }
-    if (cur != lambda_dev) VGX_HIP(hipMemcpyAsync(lambda_dev, cur, nk * 4, hipMemcpyDeviceToDevice, s));
+    ppr_sweeps(g, out, po, pw.pn.p, K, 1.f, c_dev, d, iterations, lambda_dev, s);
     if (grad_s_dev) {
       k_scale<<<ppr_blocks(nk, 256), 256, 0, s>>>(lambda_dev, static_cast<float>(1.0 - damping), nk, grad_s_dev);
       VGX_LAUNCH_CHECK();
     }
     if (grad_w_dev && out.arcs) {
-      k_ppr_edge_grad<<<ppr_blocks(n, kPprBlock / kWave), kPprBlock, 0, s>>>(out.off.p, out.adj.p, pw.pn.p,
-                                                                             pw.wsum.p, n, K, d, x_dev, lambda_dev,
-                                                                             grad_w_dev);
+      const unsigned gc = ppr_blocks(po.nchunks, kPprWaves);
+      k_ppr_edge_dot<<<gc, kPprBlock, 0, s>>>(out.off.p, out.adj.p, pw.pn.p, po.row_chunk.p, po.chunk_row.p,
+                                              static_cast<uint32_t>(po.nchunks), K, d, x_dev, lambda_dev,
+                                              grad_w_dev, pw.cpart.p);
+      VGX_LAUNCH_CHECK();
+      k_ppr_edge_norm<<<gc, kPprBlock, 0, s>>>(out.off.p, po.row_chunk.p, po.chunk_row.p,
+                                               static_cast<uint32_t>(po.nchunks), pw.wsum.p, pw.cpart.p,
+                                               grad_w_dev);
       VGX_LAUNCH_CHECK();
     }
   }

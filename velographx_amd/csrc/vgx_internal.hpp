@@ -143,12 +143,27 @@ struct CsrSnapshot {
   uint64_t n = 0, arcs = 0;
 };
 
+// Chunk decomposition of one CSR direction: row v owns chunks
+// [row_chunk[v], row_chunk[v+1]) of at most kPprChunk arcs each (an empty
+// row owns one empty chunk), so hub rows spread over many waves.
+struct PprPlan {
+  DevBuf<uint32_t> row_chunk;  // [n+1]
+  DevBuf<uint32_t> chunk_row;  // [nchunks]
+  uint64_t nchunks = 0;
+  uint64_t version = ~0ull;
+  uint64_t n = 0;
+};
+
 // Scratch of the weighted personalized PageRank kernels.
 struct PprWork {
-  DevBuf<float> pn;    // normalised weights, out-arc order
-  DevBuf<float> pin;   // the same, in-arc order
-  DevBuf<float> wsum;  // out-weight sums
-  DevBuf<float> tmp;   // ping-pong n x K
+  DevBuf<float> pn;     // normalised weights, out-arc order
+  DevBuf<float> pin;    // the same, in-arc order
+  DevBuf<float> wsum;   // out-weight sums
+  DevBuf<float> tmp;    // ping-pong n x K
+  DevBuf<float> part;   // per-chunk partial rows, nchunks x K
+  DevBuf<float> cpart;  // per-chunk scalar partials
+  DevBuf<uint8_t> scan; // hipcub temporary storage
+  PprPlan plan[2];      // [0] out-CSR, [1] in-CSR
 };
 
 }  // namespace vgx

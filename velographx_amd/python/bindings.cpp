@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "velographx_amd/pagerank.hpp"
+#include "velographx_amd/selector.hpp"
 #include "velographx_amd/velographx.hpp"
 
 namespace py = pybind11;
@@ -27,6 +28,40 @@ using velographx::VertexId;
 namespace {
 
 using U32Array = py::array_t<std::uint32_t, py::array::c_style | py::array::forcecast>;
+
+py::dict trace_to_dict(const vgx_selector_trace& t) {
+  py::dict d;
+  d["update_fraction"] = t.update_fraction;
+  d["shallow_parent_deletion_fraction"] = t.shallow_parent_deletion_fraction;
+  d["predicted_incremental_us"] = t.predicted_incremental_us;
+  d["predicted_full_us"] = t.predicted_full_us;
+  d["execution_us"] = t.execution_us;
+  d["decision_us"] = t.decision_us;
+  d["affected_vertices"] = t.affected_vertices;
+  d["first_batch"] = t.first_batch != 0;
+  d["chose_full"] = t.chose_full != 0;
+  d["internal_fallback"] = t.internal_fallback != 0;
+  d["reason"] = velographx::policy_reason_name(t.reason);
+  d["reason_code"] = t.reason;
+  return d;
+}
+
+vgx_selector_trace trace_from_dict(const py::dict& d) {
+  vgx_selector_trace t{};
+  auto get = [&](const char* k, double def) { return d.contains(k) ? d[k].cast<double>() : def; };
+  t.update_fraction = get("update_fraction", 0.0);
+  t.shallow_parent_deletion_fraction = get("shallow_parent_deletion_fraction", 0.0);
+  t.predicted_incremental_us = get("predicted_incremental_us", 0.0);
+  t.predicted_full_us = get("predicted_full_us", 0.0);
+  t.execution_us = get("execution_us", 0.0);
+  t.decision_us = get("decision_us", 0.0);
+  t.affected_vertices = d.contains("affected_vertices") ? d["affected_vertices"].cast<std::uint64_t>() : 0;
+  t.first_batch = d.contains("first_batch") && d["first_batch"].cast<bool>() ? 1 : 0;
+  t.chose_full = d.contains("chose_full") && d["chose_full"].cast<bool>() ? 1 : 0;
+  t.internal_fallback = d.contains("internal_fallback") && d["internal_fallback"].cast<bool>() ? 1 : 0;
+  t.reason = d.contains("reason_code") ? d["reason_code"].cast<int>() : 0;
+  return t;
+}
 
 // bindings.cpp:21-25: an UpdateBatch of additions applied to Graph(0, directed).
 DynamicGraph graph_from_numpy_edges(const U32Array& edges, bool directed) {
@@ -295,6 +330,24 @@ PYBIND11_MODULE(_velographx, m) {
              d["levels_per_sync"] = t.levels_per_sync;
              return d;
            })
+      .def(
+          "autotune",
+          [](IncrementalBFS& b, std::uint32_t reps) {
+            vgx_bfs_tuning t{};
+            double ms = 0.0;
+            {
+              py::gil_scoped_release nogil;
+              velographx::vgx_check(vgx_bfs_autotune(b.handle(), reps, &t, &ms));
+            }
+            py::dict d;
+            d["alpha"] = t.alpha;
+            d["beta"] = t.beta;
+            d["blocks"] = t.blocks;
+            d["levels_per_sync"] = t.levels_per_sync;
+            d["ms"] = ms;
+            return d;
+          },
+          py::arg("reps") = 3)
       .def_property_readonly("_handle", [](const IncrementalBFS& b) { return reinterpret_cast<std::uintptr_t>(b.handle()); });
 
   py::class_<IncrementalPageRank>(m, "IncrementalPageRank")
@@ -324,6 +377,58 @@ PYBIND11_MODULE(_velographx, m) {
       .def_property_readonly("last_used_full_recompute", &IncrementalPageRank::last_used_full_recompute)
       .def_property_readonly("precision", &IncrementalPageRank::precision);
 
+  // addition: adaptive repair-vs-recompute selection
+  py::class_<velographx::AdaptivePolicy>(m, "AdaptivePolicy")
+      .def(py::init<std::uint64_t, double, double>(), py::arg("vertices"), py::arg("initial_full_us"),
+           py::arg("initial_reachable_fraction"))
+      .def(
+          "decide",
+          [](const velographx::AdaptivePolicy& p, double update_fraction, double shallow, bool first_batch) {
+            vgx_selector_trace t{};
+            t.update_fraction = update_fraction;
+            t.shallow_parent_deletion_fraction = shallow;
+            t.first_batch = first_batch ? 1 : 0;
+            p.decide(t);
+            return trace_to_dict(t);
+          },
+          py::arg("update_fraction"), py::arg("shallow_parent_deletion_fraction") = 0.0,
+          py::arg("first_batch") = false)
+      .def(
+          "observe", [](velographx::AdaptivePolicy& p, const py::dict& trace) { p.observe(trace_from_dict(trace)); },
+          py::arg("trace"));
+  py::class_<velographx::AdaptiveSelector>(m, "AdaptiveSelector")
+      .def(py::init([](IncrementalBFS& b) { return new velographx::AdaptiveSelector(b); }), py::arg("bfs"),
+           py::keep_alive<1, 2>())
+      .def("apply",
+           [](velographx::AdaptiveSelector& s, const UpdateBatch& batch) {
+             vgx_selector_trace t;
+             {
+               py::gil_scoped_release nogil;
+               t = s.apply(batch);
+             }
+             return trace_to_dict(t);
+           })
+      .def(
+          "apply_arrays",
+          [](velographx::AdaptiveSelector& s, const U32Array& src, const U32Array& dst,
+             const py::array_t<std::uint8_t, py::array::c_style | py::array::forcecast>& add) {
+            if (src.size() != dst.size() || src.size() != add.size())
+              throw std::invalid_argument("src, dst and add must have equal length");
+            const auto* sp = src.data();
+            const auto* dp = dst.data();
+            const auto* ap = add.data();
+            const auto k = static_cast<std::size_t>(src.size());
+            vgx_selector_trace t;
+            {
+              py::gil_scoped_release nogil;
+              t = s.apply(sp, dp, ap, k);
+            }
+            return trace_to_dict(t);
+          },
+          py::arg("src"), py::arg("dst"), py::arg("add"))
+      .def_property_readonly("last_trace", [](const velographx::AdaptiveSelector& s) { return trace_to_dict(s.last_trace()); });
+  m.def("policy_reason_name", &velographx::policy_reason_name);
+
   // addition: batched multi-root BFS
   py::class_<velographx::MultiSourceBFS>(m, "MultiSourceBFS")
       .def(py::init([](DynamicGraph& g, const std::vector<VertexId>& roots) {
@@ -346,7 +451,29 @@ PYBIND11_MODULE(_velographx, m) {
              return arr.reshape({k, arr.size() / std::max<py::ssize_t>(k, 1)});
            })
       .def("reached", [](const velographx::MultiSourceBFS& m) { return to_numpy(m.reached()); })
-      .def_property_readonly("level_count", &velographx::MultiSourceBFS::level_count);
+      .def_property_readonly("level_count", &velographx::MultiSourceBFS::level_count)
+      .def(
+          "set_tuning",
+          [](velographx::MultiSourceBFS& m, std::uint32_t blocks, double alpha) {
+            velographx::vgx_check(vgx_msbfs_set_tuning(m.handle(), blocks, alpha));
+          },
+          py::arg("blocks") = 0, py::arg("alpha") = 4.0)
+      .def(
+          "autotune",
+          [](velographx::MultiSourceBFS& m, std::uint32_t reps) {
+            std::uint32_t blocks = 0;
+            double alpha = 0.0, ms = 0.0;
+            {
+              py::gil_scoped_release nogil;
+              velographx::vgx_check(vgx_msbfs_autotune(m.handle(), reps, &blocks, &alpha, &ms));
+            }
+            py::dict d;
+            d["blocks"] = blocks;
+            d["alpha"] = alpha;
+            d["ms"] = ms;
+            return d;
+          },
+          py::arg("reps") = 3);
 
   // addition: weighted personalized PageRank kernels on device pointers
   m.def(
