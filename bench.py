@@ -12,7 +12,9 @@ loop/duplicate removal).
           mean is sum(m_c) / sum(t));
 * roofline: algorithmic bytes of one BFS (4*A_c + 12*V_c) over the summed
   duration of its level-kernel launches, timed with HIP events on the
-  graph's stream;
+  graph's stream; traffic = HBM bytes per BFS from rocprofv3 FETCH_SIZE and
+  WRITE_SIZE passes over a child run, each scaled by its reading for a
+  1 GiB copy;
 * cpu_baseline: the reference engine (oracle/_ref/vgx_ref, compiled from the
   reference sources) or, where it is absent, the oracle's queue BFS, timed
   on this host on a bounded sample of the same roots.
@@ -88,6 +90,9 @@ def parse():
     p.add_argument("--only-bfs", action="store_true", help="plain BFS leg only (for profiling)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                    help="only run 5 device-to-device copies of BYTES (HBM counter calibration)")
+    p.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
+    p.add_argument("--pmc-child", action="store_true",
+                   help="(internal) profiled child: a 1 GiB copy, then the plain BFS roots")
     return p.parse_args()
 
 
@@ -169,6 +174,101 @@ def cpu_pagerank(s, d, n, iterations):
     secs, its = float(w[1]), int(w[2])
     return {"ms_per_iteration": 1e3 * secs / max(its, 1), "iterations": its, "cores": 1, "kind": "reference",
             "sample": f"IncrementalPageRank::recompute of the reference engine (FP64), {its} sweeps"}
+
+
+# ------------------------------------------------------ HBM traffic (PMC)
+
+COPY_BYTES = 1 << 30
+
+
+def _counter_rows(directory):
+    import csv
+
+    for root, _, files in os.walk(directory):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                with open(os.path.join(root, f)) as fh:
+                    return sorted(csv.DictReader(fh), key=lambda r: int(r["Dispatch_Id"]))
+    return []
+
+
+def _per_bfs(rows):
+    """Counter sums (KiB) of the BFS kernels of each traversal, split at
+    k_bfs_setup (one per traversal)."""
+    groups = []
+    for r in rows:
+        name = r["Kernel_Name"]
+        if "k_bfs_setup" in name:
+            groups.append(0.0)
+        elif groups and ("k_bfs_level" in name or "k_bfs_bottom_up_tail" in name):
+            groups[-1] += float(r["Counter_Value"])
+    return groups
+
+
+def measure_traffic(args, steps):
+    """HBM bytes per plain BFS from two rocprofv3 --pmc passes (FETCH_SIZE,
+    WRITE_SIZE) over a child run of this script.  Each counter is scaled by
+    its own reading for a 1 GiB device copy made first in the same child,
+    i.e. bytes = counter * (2^30 / counter_of_copy); the median over the
+    child's last `steps` traversals is reported."""
+    import shutil
+    import signal
+
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        with tempfile.TemporaryDirectory(dir="/tmp") as td:
+            cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", td, "-o", "p", "--", sys.executable,
+                   os.path.abspath(__file__), "--pmc-child", "--scale", str(args.scale), "--edgefactor",
+                   str(args.edgefactor), "--roots", str(args.roots), "--steps", str(steps)]
+            env = dict(os.environ, TMPDIR="/tmp")
+            proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                                    start_new_session=True)
+            try:
+                _, err = proc.communicate(timeout=300)
+            except subprocess.TimeoutExpired:
+                os.killpg(proc.pid, signal.SIGKILL)
+                proc.communicate()
+                log(f"traffic pass {counter} timed out")
+                return None
+            rows = _counter_rows(td)
+        copies = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        groups = _per_bfs(rows)[-steps:]
+        if not copies or not groups:
+            log(f"traffic pass {counter}: no data (rc={proc.returncode})")
+            return None
+        # the calibration copy is by far the largest copy dispatch of the child
+        scale = (COPY_BYTES / 1024.0) / max(copies)
+        out[counter] = {"kib_per_bfs": float(np.median(groups)), "copy_scale": scale,
+                        "bytes_per_bfs": float(np.median(groups)) * 1024.0 * scale}
+    return out
+
+
+def pmc_child(args):
+    """Profiled child: a 1 GiB device copy (counter calibration), then the
+    plain BFS of the headline workload, one recompute per root."""
+    import torch
+
+    import velographx_amd as vx
+    from velographx_amd import workloads as wl
+
+    a = torch.ones(COPY_BYTES, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    n = 1 << args.scale
+    s, d = vx.rmat_edges(args.scale, args.edgefactor, 1)
+    g = vx.Graph(n, False)
+    g.bulk_load_arrays(s, d)
+    lcc, _ = component(vx, wl, g)
+    roots = wl.roots_in_component(lcc, args.roots, seed=1)
+    objs = [vx.IncrementalBFS(g, int(r)) for r in roots]
+    for k in range(args.steps):
+        objs[k % len(objs)].recompute()
+    torch.cuda.synchronize()
 
 
 # ----------------------------------------------------------------- GPU legs
@@ -425,6 +525,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy, "copies": 5}), flush=True)
         return
+    if args.pmc_child:
+        pmc_child(args)
+        return
     if args.only_bfs:
         args.no_cpu = args.no_repairs = args.no_batched = args.no_contrast = True
         args.no_pagerank = args.no_ppr = True
@@ -530,6 +633,16 @@ def main():
         result["pagerank"] = leg_pagerank(vx, wl, args, args.no_cpu)
     if not args.no_ppr:
         result["ppr"] = leg_ppr(vx, args)
+    if not args.no_traffic:
+        t0 = time.time()
+        tr = measure_traffic(args, 8)
+        log(f"traffic passes in {time.time() - t0:.1f}s: {tr}")
+        if tr:
+            traffic = tr["FETCH_SIZE"]["bytes_per_bfs"] + tr["WRITE_SIZE"]["bytes_per_bfs"]
+            rf = result["roofline"]
+            rf["traffic"] = traffic
+            rf["traffic_detail"] = tr
+            rf["traffic_frac"] = traffic / (kernel_ms_per_bfs * 1e-3) / 1e9 / HBM_PEAK_GBPS
     print(json.dumps(result), flush=True)
 
 

@@ -356,8 +356,14 @@ __global__ void k_insert_big(const uint32_t* __restrict__ big, const uint32_t* _
   }
 }
 
+// Rows longer than this many stored entries are packed in chunks of
+// kPackLong by separate waves (k_pack_long_*), so an R-MAT hub row does not
+// serialise one wave.
+constexpr uint32_t kPackLong = 4096;
+
 // Copy the live entries of every row to out + off[u] (order preserved);
-// one wave per row.  Optionally writes the new row table for compaction.
+// one wave per row; rows longer than kPackLong are left to the chunked
+// kernels.  Optionally writes the new row table for compaction.
 __global__ void k_pack_rows(const uint2* __restrict__ row, const uint32_t* __restrict__ arena, uint64_t n,
                             const uint32_t* __restrict__ off, uint32_t* __restrict__ out,
                             uint2* __restrict__ new_row, const uint32_t* __restrict__ deg) {
@@ -367,6 +373,8 @@ __global__ void k_pack_rows(const uint2* __restrict__ row, const uint32_t* __res
   for (uint64_t u = gw; u < n; u += W) {
     uint2 rb = row[u];
     uint32_t o = off[u];
+    if (new_row != nullptr && lane == 0) new_row[u] = make_uint2(off[u], deg[u]);
+    if (rb.y > kPackLong) continue;
     for (uint32_t i = 0; i < rb.y; i += kWave) {
       bool in = i + lane < rb.y;
       uint32_t x = in ? arena[rb.x + i + lane] : kTomb;
@@ -375,7 +383,61 @@ __global__ void k_pack_rows(const uint2* __restrict__ row, const uint32_t* __res
       if (keep) out[o + lanes_below(b)] = x;
       o += __popcll(b);
     }
-    if (new_row != nullptr && lane == 0) new_row[u] = make_uint2(off[u], deg[u]);
+  }
+}
+
+__global__ void k_collect_long_rows(const uint2* __restrict__ row, uint64_t n, uint32_t* __restrict__ list,
+                                    unsigned long long* count) {
+  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; u < n;
+       u += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    if (row[u].y > kPackLong) list[atomicAdd(count, 1ull)] = static_cast<uint32_t>(u);
+}
+
+__global__ void k_gather_lens(const uint2* __restrict__ row, const uint32_t* __restrict__ list, uint64_t nl,
+                              uint32_t* __restrict__ lens) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < nl;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    lens[i] = row[list[i]].y;
+}
+
+// Live entries of chunk c (row chunk_u[c], entries [j*kPackLong, ...)).
+__global__ void k_pack_long_count(const uint2* __restrict__ row, const uint32_t* __restrict__ arena,
+                                  const uint32_t* __restrict__ chunk_u, const uint32_t* __restrict__ chunk_j,
+                                  uint32_t nch, uint32_t* __restrict__ live) {
+  const uint32_t W = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nch; c += W) {
+    const uint2 rb = row[chunk_u[c]];
+    const uint32_t b = chunk_j[c] * kPackLong;
+    const uint32_t e = min(b + kPackLong, rb.y);
+    uint32_t k = 0;
+    for (uint32_t i = b + lane_id(); i < e; i += kWave) k += (arena[rb.x + i] & kTomb) ? 0u : 1u;
+    k = wave_sum(k);
+    if (lane_id() == 0) live[c] = k;
+  }
+}
+
+// Copy chunk c to off[u] + the live entries of the row's earlier chunks.
+__global__ void k_pack_long_write(const uint2* __restrict__ row, const uint32_t* __restrict__ arena,
+                                  const uint32_t* __restrict__ off, const uint32_t* __restrict__ chunk_u,
+                                  const uint32_t* __restrict__ chunk_j, const uint32_t* __restrict__ live,
+                                  uint32_t nch, uint32_t* __restrict__ out) {
+  const uint32_t W = gridDim.x * (blockDim.x / kWave);
+  const unsigned lane = lane_id();
+  for (uint32_t c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nch; c += W) {
+    const uint32_t u = chunk_u[c], j = chunk_j[c];
+    const uint2 rb = row[u];
+    uint32_t o = off[u];
+    for (uint32_t q = c - j; q < c; ++q) o += live[q];  // the row's chunks are consecutive
+    const uint32_t b = j * kPackLong;
+    const uint32_t e = min(b + kPackLong, rb.y);
+    for (uint32_t i = b; i < e; i += kWave) {
+      const bool in = i + lane < e;
+      const uint32_t x = in ? arena[rb.x + i + lane] : kTomb;
+      const bool keep = !(x & kTomb);
+      const unsigned long long m = __ballot(keep);
+      if (keep) out[o + lanes_below(m)] = x;
+      o += __popcll(m);
+    }
   }
 }
 
@@ -699,6 +761,50 @@ static void graph_apply(vgx_graph* g, const uint32_t* src, const uint32_t* dst, 
 
 // ------------------------------------------------------ snapshots, compaction
 
+// Chunked copy of the rows longer than kPackLong (there are few: the chunk
+// table is built on the host).
+static void pack_long_rows(vgx_graph* g, Adjacency& a, const uint32_t* off, uint32_t* out) {
+  hipStream_t s = g->stream;
+  const uint64_t n = g->n;
+  DevBuf<uint32_t> list;
+  DevBuf<unsigned long long> cnt;
+  list.ensure(n + 1);
+  cnt.ensure(1);
+  VGX_HIP(hipMemsetAsync(cnt.p, 0, 8, s));
+  k_collect_long_rows<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(a.row.p, n, list.p, cnt.p);
+  VGX_LAUNCH_CHECK();
+  unsigned long long nl = 0;
+  VGX_HIP(hipMemcpyAsync(&nl, cnt.p, 8, hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipStreamSynchronize(s));
+  if (nl == 0) return;
+  DevBuf<uint32_t> lens;
+  lens.ensure(nl);
+  k_gather_lens<<<grid_for(nl, kBlock, 1024), kBlock, 0, s>>>(a.row.p, list.p, nl, lens.p);
+  VGX_LAUNCH_CHECK();
+  std::vector<uint32_t> rows(nl), len(nl);
+  VGX_HIP(hipMemcpyAsync(rows.data(), list.p, nl * 4, hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipMemcpyAsync(len.data(), lens.p, nl * 4, hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipStreamSynchronize(s));
+  std::vector<uint32_t> cu, cj;
+  for (uint64_t r = 0; r < nl; ++r)
+    for (uint32_t j = 0; static_cast<uint64_t>(j) * kPackLong < len[r]; ++j) {
+      cu.push_back(rows[r]);
+      cj.push_back(j);
+    }
+  const uint32_t nch = static_cast<uint32_t>(cu.size());
+  DevBuf<uint32_t> du, dj, live;
+  du.ensure(nch);
+  dj.ensure(nch);
+  live.ensure(nch);
+  VGX_HIP(hipMemcpyAsync(du.p, cu.data(), nch * 4, hipMemcpyHostToDevice, s));
+  VGX_HIP(hipMemcpyAsync(dj.p, cj.data(), nch * 4, hipMemcpyHostToDevice, s));
+  k_pack_long_count<<<grid_for(nch, 4, 4096), kBlock, 0, s>>>(a.row.p, a.arena.p, du.p, dj.p, nch, live.p);
+  VGX_LAUNCH_CHECK();
+  k_pack_long_write<<<grid_for(nch, 4, 4096), kBlock, 0, s>>>(a.row.p, a.arena.p, off, du.p, dj.p, live.p, nch, out);
+  VGX_LAUNCH_CHECK();
+  VGX_HIP(hipStreamSynchronize(s));
+}
+
 static void pack_direction(vgx_graph* g, int d, DevBuf<uint32_t>& off, DevBuf<uint32_t>& out, uint2* new_row,
                            const uint32_t* caps) {
   Adjacency& a = g->adj[d];
@@ -721,6 +827,7 @@ static void pack_direction(vgx_graph* g, int d, DevBuf<uint32_t>& off, DevBuf<ui
   if (n) {
     k_pack_rows<<<grid_for(n, 4, 16384), kBlock, 0, s>>>(a.row.p, a.arena.p, n, off.p, out.p, new_row, a.deg.p);
     VGX_LAUNCH_CHECK();
+    pack_long_rows(g, a, off.p, out.p);
   }
   if (caps) {
     a.used = total;

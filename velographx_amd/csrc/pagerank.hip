@@ -18,6 +18,8 @@
 // only polls every few iterations.  FP64 reproduces the reference's
 // arithmetic; FP32 keeps ranks and sums in float with Kahan-compensated
 // lane sums.
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -76,6 +78,7 @@ struct vgx_pagerank {
   // localized repair workspace
   vgx::DevBuf<uint32_t> act_flag, ever_flag, alist[2], counters;
   vgx::DevBuf<uint8_t> upd;
+  vgx::DevBuf<uint8_t> scan_tmp;
 };
 
 namespace vgx {
@@ -280,28 +283,59 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_init(PrArgs a, int warm) {
   }
 }
 
-// Split vertices by in-degree.
-__global__ void k_pr_classify(const uint32_t* off, uint32_t n, uint32_t* slist, uint32_t* mlist, uint32_t* hlist,
-                              unsigned long long* cnt) {
-  for (uint32_t v0 = blockIdx.x * blockDim.x + threadIdx.x - lane_id(); v0 < n; v0 += gridDim.x * blockDim.x) {
-    const uint32_t v = v0 + lane_id();
-    uint32_t d = v < n ? off[v + 1] - off[v] : 0u;
-    bool s = v < n && d <= kSmallRow;
-    bool m = v < n && d > kSmallRow && d <= kChunk;
-    bool h = v < n && d > kChunk;
-    unsigned long long bs = __ballot(s), bm = __ballot(m), bh = __ballot(h);
-    unsigned long long base_s = 0, base_m = 0, base_h = 0;
-    if (lane_id() == 0) {
-      if (bs) base_s = atomicAdd(&cnt[0], static_cast<unsigned long long>(__popcll(bs)));
-      if (bm) base_m = atomicAdd(&cnt[1], static_cast<unsigned long long>(__popcll(bm)));
-      if (bh) base_h = atomicAdd(&cnt[2], static_cast<unsigned long long>(__popcll(bh)));
-    }
-    base_s = __shfl(base_s, 0, kWave);
-    base_m = __shfl(base_m, 0, kWave);
-    base_h = __shfl(base_h, 0, kWave);
-    if (s) slist[base_s + lanes_below(bs)] = v;
-    if (m) mlist[base_m + lanes_below(bm)] = v;
-    if (h) hlist[base_h + lanes_below(bh)] = v;
+// Split vertices by in-degree into three lists kept in vertex order.  Wave
+// w owns 64-vertex groups [w*gpw, (w+1)*gpw): k_pr_class_count counts its
+// members per class, an exclusive scan over [S counts | M counts | H counts]
+// gives every wave its write positions, k_pr_class_write fills the lists.
+// No atomics, and the lists come out sorted.
+constexpr uint32_t kClassWaves = 4096;
+
+__device__ __forceinline__ void class_ballots(const uint32_t* off, uint32_t n, uint32_t g, unsigned long long& bs,
+                                              unsigned long long& bm, unsigned long long& bh) {
+  const uint32_t v = g * kWave + lane_id();
+  const uint32_t d = v < n ? off[v + 1] - off[v] : 0u;
+  bs = __ballot(v < n && d <= kSmallRow);
+  bm = __ballot(v < n && d > kSmallRow && d <= kChunk);
+  bh = __ballot(v < n && d > kChunk);
+}
+
+__global__ void k_pr_class_count(const uint32_t* __restrict__ off, uint32_t n, uint32_t gpw,
+                                 uint32_t* __restrict__ counts) {
+  const uint32_t w = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const uint32_t groups = (n + kWave - 1) / kWave;
+  uint32_t cs = 0, cm = 0, ch = 0;
+  for (uint32_t g = w * gpw; g < min((w + 1) * gpw, groups); ++g) {
+    unsigned long long bs, bm, bh;
+    class_ballots(off, n, g, bs, bm, bh);
+    cs += __popcll(bs);
+    cm += __popcll(bm);
+    ch += __popcll(bh);
+  }
+  if (lane_id() == 0) {
+    counts[w] = cs;
+    counts[kClassWaves + w] = cm;
+    counts[2 * kClassWaves + w] = ch;
+  }
+}
+
+__global__ void k_pr_class_write(const uint32_t* __restrict__ off, uint32_t n, uint32_t gpw,
+                                 const uint32_t* __restrict__ bases, uint32_t* __restrict__ slist,
+                                 uint32_t* __restrict__ mlist, uint32_t* __restrict__ hlist) {
+  const uint32_t w = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const uint32_t groups = (n + kWave - 1) / kWave;
+  uint32_t os = bases[w];
+  uint32_t om = bases[kClassWaves + w] - bases[kClassWaves];
+  uint32_t oh = bases[2 * kClassWaves + w] - bases[2 * kClassWaves];
+  for (uint32_t g = w * gpw; g < min((w + 1) * gpw, groups); ++g) {
+    unsigned long long bs, bm, bh;
+    class_ballots(off, n, g, bs, bm, bh);
+    const uint32_t v = g * kWave + lane_id();
+    if ((bs >> lane_id()) & 1ull) slist[os + lanes_below(bs)] = v;
+    if ((bm >> lane_id()) & 1ull) mlist[om + lanes_below(bm)] = v;
+    if ((bh >> lane_id()) & 1ull) hlist[oh + lanes_below(bh)] = v;
+    os += __popcll(bs);
+    om += __popcll(bm);
+    oh += __popcll(bh);
   }
 }
 
@@ -478,17 +512,34 @@ static void build_lists(vgx_pagerank* p) {
   p->slist.ensure(n + 1);
   p->mlist.ensure(n + 1);
   p->hlist.ensure(n + 1);
-  p->counters.ensure(16);
-  auto* cnt = reinterpret_cast<unsigned long long*>(p->counters.p);
-  VGX_HIP(hipMemsetAsync(cnt, 0, 3 * 8, s));
+  uint32_t c[3] = {0, 0, 0};
   if (n) {
-    k_pr_classify<<<gblocks(n), kPrBlock, 0, s>>>(sn.off.p, static_cast<uint32_t>(n), p->slist.p, p->mlist.p,
-                                                  p->hlist.p, cnt);
+    const uint32_t W = kClassWaves;
+    const uint32_t groups = static_cast<uint32_t>((n + kWave - 1) / kWave);
+    const uint32_t gpw = (groups + W - 1) / W;
+    DevBuf<uint32_t> counts, bases;
+    counts.ensure(3 * W);
+    bases.ensure(3 * W);
+    const unsigned blocks = W / (kPrBlock / kWave);
+    k_pr_class_count<<<blocks, kPrBlock, 0, s>>>(sn.off.p, static_cast<uint32_t>(n), gpw, counts.p);
     VGX_LAUNCH_CHECK();
+    size_t bytes = 0;
+    VGX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, counts.p, bases.p, static_cast<int>(3 * W), s));
+    p->scan_tmp.ensure(bytes + 16);
+    VGX_HIP(hipcub::DeviceScan::ExclusiveSum(p->scan_tmp.p, bytes, counts.p, bases.p, static_cast<int>(3 * W), s));
+    k_pr_class_write<<<blocks, kPrBlock, 0, s>>>(sn.off.p, static_cast<uint32_t>(n), gpw, bases.p, p->slist.p,
+                                                 p->mlist.p, p->hlist.p);
+    VGX_LAUNCH_CHECK();
+    uint32_t tail[4];
+    VGX_HIP(hipMemcpyAsync(&tail[0], bases.p + W, 4, hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipMemcpyAsync(&tail[1], bases.p + 2 * W, 4, hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipMemcpyAsync(&tail[2], bases.p + 3 * W - 1, 4, hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipMemcpyAsync(&tail[3], counts.p + 3 * W - 1, 4, hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    c[0] = tail[0];
+    c[1] = tail[1] - tail[0];
+    c[2] = tail[2] + tail[3] - tail[1];
   }
-  unsigned long long c[3];
-  VGX_HIP(hipMemcpyAsync(c, cnt, sizeof(c), hipMemcpyDeviceToHost, s));
-  VGX_HIP(hipStreamSynchronize(s));
   p->ns = c[0];
   p->nm = c[1];
   p->nh = c[2];
@@ -611,7 +662,7 @@ static void count_dangling(vgx_pagerank* p) {
   auto* cnt = reinterpret_cast<unsigned long long*>(p->counters.p);
   VGX_HIP(hipMemsetAsync(cnt + 4, 0, 8, g->stream));
   if (g->n) {
-    k_count_zero<<<gblocks(g->n), kPrBlock, 0, g->stream>>>(g->adj[0].deg.p, g->n, cnt + 4);
+    k_count_zero<<<gblocks(g->n, kPrBlock, 512), kPrBlock, 0, g->stream>>>(g->adj[0].deg.p, g->n, cnt + 4);
     VGX_LAUNCH_CHECK();
   }
   unsigned long long c = 0;

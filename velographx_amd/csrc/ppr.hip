@@ -17,11 +17,12 @@
 // Work is split into chunks of at most 512 arcs of one row (PprPlan), one
 // wave per chunk, so a hub row of R-MAT (tens of thousands of arcs) spreads
 // over many waves instead of serialising one.  Rows of several chunks get
-// per-chunk partials that a second pass adds in chunk order: every sum has
-// a fixed order and the results do not depend on scheduling.
-// In a sweep the two half-waves take one arc each per step (four steps in
-// flight) and their 32 lanes the K <= 32 columns, so an arc's row of X is
-// one coalesced 128-byte load.
+// per-chunk partials that a second pass adds in a fixed order: every sum
+// has a fixed order and the results do not depend on scheduling.
+// In a sweep with K = 32 and 16-byte aligned buffers, eight lanes hold one
+// 128-byte row of X as float4s and a wave gathers eight arcs per load;
+// otherwise the two half-waves take one arc each per step and their 32
+// lanes the K <= 32 columns.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -152,8 +153,53 @@ __device__ __forceinline__ float chunk_gather(const uint32_t* __restrict__ nbr, 
   return acc + __shfl_xor(acc, 32, kWave);
 }
 
+// The same for K = 32 with 16-byte loads: lane group g = lane / 8 takes arcs
+// b+g, b+g+8, ...; lane s = lane % 8 of the group holds columns 4s..4s+3, so
+// one load instruction fetches eight 128-byte rows.  Two steps are in flight.
+// Returns the chunk's sums on lanes 0..7 (columns 4*lane..4*lane+3).
+__device__ __forceinline__ float4 chunk_gather32(const uint32_t* __restrict__ nbr, const float* __restrict__ p,
+                                                 const float* __restrict__ x, uint32_t b, uint32_t e,
+                                                 unsigned lane) {
+  const unsigned grp = lane >> 3, sub = lane & 7;
+  const float4* x4 = reinterpret_cast<const float4*>(x);
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t i = b + grp;
+  for (; i + 8 < e; i += 16) {
+    const uint32_t u0 = nbr[i], u1 = nbr[i + 8];
+    const float p0 = p[i], p1 = p[i + 8];
+    const float4 a0 = x4[static_cast<uint64_t>(u0) * 8 + sub];
+    const float4 a1 = x4[static_cast<uint64_t>(u1) * 8 + sub];
+    acc.x += p0 * a0.x;
+    acc.y += p0 * a0.y;
+    acc.z += p0 * a0.z;
+    acc.w += p0 * a0.w;
+    acc.x += p1 * a1.x;
+    acc.y += p1 * a1.y;
+    acc.z += p1 * a1.z;
+    acc.w += p1 * a1.w;
+  }
+  if (i < e) {
+    const uint32_t u0 = nbr[i];
+    const float p0 = p[i];
+    const float4 a0 = x4[static_cast<uint64_t>(u0) * 8 + sub];
+    acc.x += p0 * a0.x;
+    acc.y += p0 * a0.y;
+    acc.z += p0 * a0.z;
+    acc.w += p0 * a0.w;
+  }
+#pragma unroll
+  for (int d = 8; d < kWave; d <<= 1) {
+    acc.x += __shfl_xor(acc.x, d, kWave);
+    acc.y += __shfl_xor(acc.y, d, kWave);
+    acc.z += __shfl_xor(acc.z, d, kWave);
+    acc.w += __shfl_xor(acc.w, d, kWave);
+  }
+  return acc;
+}
+
 // y[v,:] = a * b[v,:] + d * sum_e p[e] * x[nbr_e,:]; rows of several chunks
 // leave per-chunk partial rows for k_ppr_finish.
+template <bool kVec32>
 __global__ __launch_bounds__(kPprBlock) void k_ppr_sweep(const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ nbr,
                                                          const float* __restrict__ p,
@@ -169,38 +215,61 @@ __global__ __launch_bounds__(kPprBlock) void k_ppr_sweep(const uint32_t* __restr
     const bool single = row_chunk[v + 1] - row_chunk[v] == 1;
     uint32_t s, e;
     chunk_range(off, row_chunk, v, c, s, e);
-    const float acc = chunk_gather(nbr, p, x, s, e, K, half, col);
-    if (half == 0 && col < K) {
-      if (single) {
-        const uint64_t o = static_cast<uint64_t>(v) * K + col;
-        y[o] = a * b[o] + d * acc;
-      } else {
-        part[static_cast<uint64_t>(c) * K + col] = acc;
+    if (kVec32) {
+      const float4 acc = chunk_gather32(nbr, p, x, s, e, lane);
+      if (lane < 8) {
+        if (single) {
+          const uint64_t o = static_cast<uint64_t>(v) * 8 + lane;
+          const float4 bb = reinterpret_cast<const float4*>(b)[o];
+          reinterpret_cast<float4*>(y)[o] =
+              make_float4(a * bb.x + d * acc.x, a * bb.y + d * acc.y, a * bb.z + d * acc.z, a * bb.w + d * acc.w);
+        } else {
+          reinterpret_cast<float4*>(part)[static_cast<uint64_t>(c) * 8 + lane] = acc;
+        }
+      }
+    } else {
+      const float acc = chunk_gather(nbr, p, x, s, e, K, half, col);
+      if (half == 0 && col < K) {
+        if (single) {
+          const uint64_t o = static_cast<uint64_t>(v) * K + col;
+          y[o] = a * b[o] + d * acc;
+        } else {
+          part[static_cast<uint64_t>(c) * K + col] = acc;
+        }
       }
     }
   }
 }
 
-// Rows of several chunks: add the partial rows in chunk order.
-__global__ __launch_bounds__(kPprBlock) void k_ppr_finish(const uint32_t* __restrict__ row_chunk, uint32_t n,
+__global__ void k_ppr_multi_collect(const uint32_t* __restrict__ row_chunk, uint32_t n, uint32_t* __restrict__ multi,
+                                    unsigned long long* count) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    if (row_chunk[v + 1] - row_chunk[v] > 1) multi[atomicAdd(count, 1ull)] = v;
+}
+
+// Rows of several chunks, one wave each: half-wave h adds the partial rows
+// of chunks h, h+2, ... (four in flight), then the halves are combined.
+__global__ __launch_bounds__(kPprBlock) void k_ppr_finish(const uint32_t* __restrict__ row_chunk,
+                                                          const uint32_t* __restrict__ multi, uint32_t nmulti,
                                                           uint32_t K, float a, const float* __restrict__ b, float d,
                                                           const float* __restrict__ part, float* __restrict__ y) {
   const unsigned lane = lane_id();
-  const uint32_t W = gridDim.x * kPprWaves;
-  for (uint32_t v0 = (blockIdx.x * kPprWaves + threadIdx.x / kWave) * kWave; v0 < n; v0 += W * kWave) {
-    const uint32_t v = v0 + lane;
-    const bool multi = v < n && row_chunk[v + 1] - row_chunk[v] > 1;
-    unsigned long long m = __ballot(multi);
-    while (m) {
-      const int l = __ffsll(static_cast<long long>(m)) - 1;
-      m &= m - 1;
-      const uint32_t r = v0 + l;
-      if (lane < K) {
-        float acc = 0.f;
-        for (uint32_t c = row_chunk[r]; c < row_chunk[r + 1]; ++c) acc += part[static_cast<uint64_t>(c) * K + lane];
-        const uint64_t o = static_cast<uint64_t>(r) * K + lane;
-        y[o] = a * b[o] + d * acc;
-      }
+  const unsigned half = lane >> 5, col = lane & 31;
+  for (uint32_t j = blockIdx.x * kPprWaves + threadIdx.x / kWave; j < nmulti; j += gridDim.x * kPprWaves) {
+    const uint32_t r = multi[j];
+    const uint32_t c0 = row_chunk[r], c1 = row_chunk[r + 1];
+    float acc = 0.f;
+    if (col < K) {
+      uint32_t c = c0 + half;
+      for (; c + 6 < c1; c += 8)
+        acc += (part[static_cast<uint64_t>(c) * K + col] + part[static_cast<uint64_t>(c + 2) * K + col]) +
+               (part[static_cast<uint64_t>(c + 4) * K + col] + part[static_cast<uint64_t>(c + 6) * K + col]);
+      for (; c < c1; c += 2) acc += part[static_cast<uint64_t>(c) * K + col];
+    }
+    acc += __shfl_xor(acc, 32, kWave);
+    if (half == 0 && col < K) {
+      const uint64_t o = static_cast<uint64_t>(r) * K + col;
+      y[o] = a * b[o] + d * acc;
     }
   }
 }
@@ -217,7 +286,7 @@ __global__ __launch_bounds__(kPprBlock) void k_ppr_edge_dot(const uint32_t* __re
                                                             const uint32_t* __restrict__ chunk_row, uint32_t nchunks,
                                                             uint32_t K, float d, const float* __restrict__ x,
                                                             const float* __restrict__ lam, float* __restrict__ gw,
-                                                            float* __restrict__ cpart) {
+                                                            float* __restrict__ cpart, int vec32) {
   for (uint32_t c = blockIdx.x * kPprWaves + threadIdx.x / kWave; c < nchunks; c += gridDim.x * kPprWaves) {
     const uint32_t u = chunk_row[c];
     uint32_t b, e;
@@ -229,9 +298,18 @@ __global__ __launch_bounds__(kPprBlock) void k_ppr_edge_dot(const uint32_t* __re
     for (uint32_t i = b + lane_id(); i < e; i += kWave) {
       const float* lv = lam + static_cast<uint64_t>(adj[i]) * K;
       float dot = 0.f;
+      if (vec32) {
+        const float4* l4 = reinterpret_cast<const float4*>(lv);
 #pragma unroll
-      for (int k = 0; k < 32; ++k)
-        if (k < static_cast<int>(K)) dot += xu[k] * lv[k];
+        for (int q = 0; q < 8; ++q) {
+          const float4 z = l4[q];
+          dot += xu[4 * q] * z.x + xu[4 * q + 1] * z.y + xu[4 * q + 2] * z.z + xu[4 * q + 3] * z.w;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 32; ++k)
+          if (k < static_cast<int>(K)) dot += xu[k] * lv[k];
+      }
       const float g = d * dot;
       gw[i] = g;
       t += pn[i] * g;
@@ -298,6 +376,21 @@ static const PprPlan& ppr_plan(vgx_graph* g, int dir, const CsrSnapshot& sn, hip
     pl.chunk_row.ensure(static_cast<uint64_t>(total) + 1);
     k_ppr_chunk_fill<<<ppr_blocks(n, kPprBlock), kPprBlock, 0, s>>>(pl.row_chunk.p, n, pl.chunk_row.p);
     VGX_LAUNCH_CHECK();
+    // rows of several chunks (their order in the list does not matter: each
+    // row's sum runs over its own chunks in a fixed order)
+    const uint64_t most = std::min<uint64_t>(n, total / 2 + 1);
+    pl.multi.ensure(most + 1);
+    DevBuf<unsigned long long> cnt64;
+    cnt64.ensure(1);
+    VGX_HIP(hipMemsetAsync(cnt64.p, 0, 8, s));
+    k_ppr_multi_collect<<<ppr_blocks(n, kPprBlock), kPprBlock, 0, s>>>(pl.row_chunk.p, n, pl.multi.p, cnt64.p);
+    VGX_LAUNCH_CHECK();
+    unsigned long long nm = 0;
+    VGX_HIP(hipMemcpyAsync(&nm, cnt64.p, 8, hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    pl.nmulti = nm;
+  } else {
+    pl.nmulti = 0;
   }
   pl.nchunks = total;
   pl.version = sn.version;
@@ -350,15 +443,26 @@ static void ppr_sweeps(vgx_graph* g, const CsrSnapshot& sn, const PprPlan& pl, c
   pw.tmp.ensure(nk + 1);
   pw.part.ensure(pl.nchunks * K + 1);
   const unsigned gs = ppr_blocks(pl.nchunks, kPprWaves);
-  const unsigned gf = ppr_blocks((static_cast<uint64_t>(n) + kWave - 1) / kWave, kPprWaves, 2048);
+  const unsigned gf = ppr_blocks(pl.nmulti, kPprWaves);
+  // 16-byte rows need K = 32 and 16-byte aligned X, S and output buffers
+  const bool vec = K == 32 && ((reinterpret_cast<uintptr_t>(io) | reinterpret_cast<uintptr_t>(b) |
+                                reinterpret_cast<uintptr_t>(pw.tmp.p) | reinterpret_cast<uintptr_t>(pw.part.p)) &
+                               15) == 0;
   float* cur = io;
   float* nxt = pw.tmp.p;
   for (uint32_t it = 0; it < iterations; ++it) {
-    k_ppr_sweep<<<gs, kPprBlock, 0, s>>>(sn.off.p, sn.adj.p, p, pl.row_chunk.p, pl.chunk_row.p,
-                                         static_cast<uint32_t>(pl.nchunks), K, a, b, d, cur, nxt, pw.part.p);
+    if (vec)
+      k_ppr_sweep<true><<<gs, kPprBlock, 0, s>>>(sn.off.p, sn.adj.p, p, pl.row_chunk.p, pl.chunk_row.p,
+                                                 static_cast<uint32_t>(pl.nchunks), K, a, b, d, cur, nxt, pw.part.p);
+    else
+      k_ppr_sweep<false><<<gs, kPprBlock, 0, s>>>(sn.off.p, sn.adj.p, p, pl.row_chunk.p, pl.chunk_row.p,
+                                                  static_cast<uint32_t>(pl.nchunks), K, a, b, d, cur, nxt, pw.part.p);
     VGX_LAUNCH_CHECK();
-    k_ppr_finish<<<gf, kPprBlock, 0, s>>>(pl.row_chunk.p, n, K, a, b, d, pw.part.p, nxt);
-    VGX_LAUNCH_CHECK();
+    if (pl.nmulti) {
+      k_ppr_finish<<<gf, kPprBlock, 0, s>>>(pl.row_chunk.p, pl.multi.p, static_cast<uint32_t>(pl.nmulti), K, a, b,
+                                            d, pw.part.p, nxt);
+      VGX_LAUNCH_CHECK();
+    }
     std::swap(cur, nxt);
   }
   if (cur != io) VGX_HIP(hipMemcpyAsync(io, cur, nk * 4, hipMemcpyDeviceToDevice, s));
@@ -436,7 +540,8 @@ vgx_status vgx_ppr_backward(vgx_graph* g, const float* w_dev, const float* x_dev
       const unsigned gc = ppr_blocks(po.nchunks, kPprWaves);
       k_ppr_edge_dot<<<gc, kPprBlock, 0, s>>>(out.off.p, out.adj.p, pw.pn.p, po.row_chunk.p, po.chunk_row.p,
                                               static_cast<uint32_t>(po.nchunks), K, d, x_dev, lambda_dev,
-                                              grad_w_dev, pw.cpart.p);
+                                              grad_w_dev, pw.cpart.p,
+                                              K == 32 && (reinterpret_cast<uintptr_t>(lambda_dev) & 15) == 0);
       VGX_LAUNCH_CHECK();
       k_ppr_edge_norm<<<gc, kPprBlock, 0, s>>>(out.off.p, po.row_chunk.p, po.chunk_row.p,
                                                static_cast<uint32_t>(po.nchunks), pw.wsum.p, pw.cpart.p,

@@ -149,7 +149,9 @@ struct CsrSnapshot {
 struct PprPlan {
   DevBuf<uint32_t> row_chunk;  // [n+1]
   DevBuf<uint32_t> chunk_row;  // [nchunks]
+  DevBuf<uint32_t> multi;      // rows of more than one chunk
   uint64_t nchunks = 0;
+  uint64_t nmulti = 0;
   uint64_t version = ~0ull;
   uint64_t n = 0;
 };
