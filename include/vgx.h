@@ -147,6 +147,19 @@ vgx_status vgx_bfs_stats_get(vgx_bfs* b, vgx_bfs_stats* out);
 /* Time every level-kernel launch with HIP events (on = 1) and reset the
  * accumulated launch count and duration. */
 vgx_status vgx_bfs_set_profiling(vgx_bfs* b, int on);
+/* Per-level record of the last full traversal (diagnostics; no reference
+ * counterpart): direction, frontier vertices and out-edges, bottom-up rows
+ * left for the second phase, and the level's kernel time when profiling is
+ * on.  At most 64 levels are recorded. */
+typedef struct {
+  uint32_t bottom_up;  /* 0 top-down (queue), 1 bottom-up, 2 top-down from the frontier bitmap */
+  uint64_t frontier_vertices;
+  uint64_t frontier_edges;
+  uint64_t pending_vertices;
+  uint64_t pending_edges;
+  double kernel_ms;
+} vgx_bfs_level;
+vgx_status vgx_bfs_level_log(vgx_bfs* b, vgx_bfs_level* out, uint32_t cap, uint32_t* count);
 
 /* Direction-optimizing switch and launch shape (autotuned). */
 typedef struct {

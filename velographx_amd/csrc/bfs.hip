@@ -57,16 +57,32 @@ constexpr uint32_t kHead = 4;
 // another and no grid-wide fence is needed within a level.
 // The atomically updated counters and the leader-written hand-over live in
 // separate 128-byte lines.
-struct alignas(128) LevelRec {
-  unsigned long long q;      // packed (count << 38 | edges) of the queued frontier
-  unsigned long long found;  // frontier size, zero-length rows included
+constexpr uint32_t kLogLevels = 64;
+
+// The frontier size is added by every workgroup; it is striped over
+// kFoundStripes cache lines (workgroup b adds to stripe b % kFoundStripes)
+// so that a level's final additions do not serialise on one address.
+constexpr uint32_t kFoundStripes = 16;
+struct alignas(128) FoundStripe {
+  unsigned long long v;
 };
+struct alignas(128) LevelRec {
+  unsigned long long q;                   // packed (count << 38 | edges) of the queued frontier
+  FoundStripe found[kFoundStripes];       // frontier size, zero-length rows included
+};
+
+__device__ __forceinline__ unsigned long long found_total(const LevelRec& r) {
+  unsigned long long t = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kFoundStripes; ++i) t += r.found[i].v;
+  return t;
+}
 struct alignas(128) LevelCtl {
   unsigned long long visited_edges;  // edges of vertices visited before this level
   uint32_t prev_mode;                // direction used by the previous level
 };
 
-// Packed counter of the bottom-up pending list of a level (same ring).
+// Rows a bottom-up level leaves for phase B (same ring).
 struct alignas(128) PendCtr {
   unsigned long long q;
 };
@@ -83,6 +99,11 @@ struct BfsState {
   uint32_t pad;
   float alpha;
   float beta;
+  // per-level record for diagnostics (levels < kLogLevels)
+  uint32_t log_mode[kLogLevels];
+  unsigned long long log_nf[kLogLevels];
+  unsigned long long log_mf[kLogLevels];
+  unsigned long long log_pend[kLogLevels];  // packed count << 38 | edges left for phase B
 };
 
 // Counters of the incremental repair.
@@ -107,9 +128,10 @@ struct BfsWork {
   DevBuf<unsigned long long> bm;  // 3 bitmaps
   DevBuf<uint32_t> queue[2];
   DevBuf<unsigned long long> pre[2];
-  DevBuf<uint32_t> pend;
-  DevBuf<unsigned long long> ppre;
+  DevBuf<uint32_t> pend;           // bottom-up rows left for phase B
+  DevBuf<unsigned long long> ppre; // their edge prefixes
   DevBuf<uint4> head;
+  DevBuf<unsigned long long> zin;  // vertices without live in-arcs (and ids >= n)
   uint64_t head_version = ~0ull, head_n = 0;
   // incremental
   DevBuf<uint32_t> lost, support, flags, stamp;
@@ -140,6 +162,7 @@ struct vgx_bfs {
   std::vector<hipEvent_t> events;
   uint64_t level_launches = 0;
   double level_ms = 0.0;
+  float last_level_ms[64] = {};  // per launch of the last traversal (profiling)
 };
 
 namespace vgx {
@@ -179,11 +202,24 @@ __global__ void k_build_head(const uint2* __restrict__ in_row, const uint32_t* _
   }
 }
 
+// Bit v set when v has no live in-arc (or v >= n).  Such a vertex can only
+// be reached as the root, so every traversal starts with these bits already
+// in the visited bitmap and the bottom-up sweeps skip them.
+__global__ void k_build_zin(const uint32_t* __restrict__ in_deg, uint32_t n, uint32_t words,
+                            unsigned long long* __restrict__ zin) {
+  const uint32_t W = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t wi = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; wi < words; wi += W) {
+    const uint32_t v = wi * kWave + lane_id();
+    const unsigned long long m = __ballot(v >= n || in_deg[v] == 0);
+    if (lane_id() == 0) zin[wi] = m;
+  }
+}
+
 __global__ void k_bfs_setup(LevelArgs a, uint32_t root, unsigned long long total_edges, float alpha, float beta) {
   BfsState* st = a.st;
   for (int i = 0; i < 4; ++i) {
     st->rec[i].q = 0;
-    st->rec[i].found = 0;
+    for (uint32_t j = 0; j < kFoundStripes; ++j) st->rec[i].found[j].v = 0;
     st->ctl[i].visited_edges = 0;
     st->ctl[i].prev_mode = 0;
     st->pend[i].q = 0;
@@ -200,7 +236,7 @@ __global__ void k_bfs_setup(LevelArgs a, uint32_t root, unsigned long long total
   a.visited[root >> 6] |= 1ull << (root & 63);
   a.bm[root >> 6] |= 1ull << (root & 63);
   const uint32_t len = a.row[root].y;
-  st->rec[0].found = 1;
+  st->rec[0].found[0].v = 1;
   if (len) {
     a.queue0[0] = root;
     a.pre0[0] = 0;
@@ -220,7 +256,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
   // then hold the counters of earlier levels.
   if (st->done_level <= level) return;
   const unsigned long long qc = st->rec[level & 3].q;
-  const unsigned long long nf = st->rec[level & 3].found;
+  const unsigned long long nf = found_total(st->rec[level & 3]);
   if (nf == 0) {
     if (leader) atomicMin(&st->done_level, level);
     return;
@@ -228,14 +264,16 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
   // Direction of this level, derived identically by every workgroup.
   const unsigned long long mf = qc & kEdgeMask;
   const unsigned long long seen = st->ctl[level & 3].visited_edges + mf;
+  // 0: top-down from the queue, 1: bottom-up, 2: top-down from the frontier
+  // bitmap (after a bottom-up level, which builds no queue)
   uint32_t mode = 0;
   if (level > 0) {
     const uint32_t prev = st->ctl[level & 3].prev_mode;
-    if (prev == 0) {
+    if (prev != 1) {
       const unsigned long long mu = st->total_edges > seen ? st->total_edges - seen : 0;
       mode = static_cast<double>(mf) * st->alpha > static_cast<double>(mu) ? 1u : 0u;
     } else {
-      mode = static_cast<double>(nf) * st->beta < static_cast<double>(st->n) ? 0u : 1u;
+      mode = static_cast<double>(nf) * st->beta < static_cast<double>(st->n) ? 2u : 1u;
     }
   }
   LevelRec* rn = &st->rec[(level + 1) & 3];
@@ -243,10 +281,16 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
     st->ctl[(level + 1) & 3].prev_mode = mode;
     st->ctl[(level + 1) & 3].visited_edges = seen;
     st->rec[(level + 2) & 3].q = 0;
-    st->rec[(level + 2) & 3].found = 0;
+    for (uint32_t j = 0; j < kFoundStripes; ++j) st->rec[(level + 2) & 3].found[j].v = 0;
     st->pend[(level + 2) & 3].q = 0;
     atomicAdd(&st->reached, nf);
-    if (mode) atomicAdd(&st->bu_levels, 1u);
+    if (mode == 1) atomicAdd(&st->bu_levels, 1u);
+    if (level < kLogLevels) {
+      st->log_mode[level] = mode;
+      st->log_nf[level] = nf;
+      st->log_mf[level] = mf;
+      st->log_pend[level] = 0;
+    }
   }
   const int cur = level & 1, nxt = cur ^ 1;
   const uint32_t words = a.words;
@@ -270,99 +314,136 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
   uint32_t nbuf = 0, nbuf2 = 0;
   unsigned long long found_local = 0;
 
-  if (mode == 0) {
+  // Claim w for the next level if nobody has; queue it with its out-degree.
+  auto claim = [&](uint32_t w, bool consider) {
+    bool newly = false;
+    uint32_t wl = 0;
+    if (consider && !(w & kTomb)) {
+      const unsigned long long bit = 1ull << (w & 63);
+      unsigned long long* wp = a.visited + (w >> 6);
+      if (!(*wp & bit)) newly = !(atomicOr(wp, bit) & bit);
+    }
+    if (newly) {
+      a.dist[w] = next_level;
+      atomicOr(bm_next + (w >> 6), 1ull << (w & 63));
+      wl = a.row[w].y;
+    }
+    found_local += __popcll(__ballot(newly));
+    stage_push(sb, nbuf, &rn->q, qn, pn, newly && wl > 0, w, wl);
+  };
+
+  if (mode == 2) {
+    // Frontier vertices from the bitmap: rows of up to 32 entries are
+    // expanded by their own lane, longer rows by the whole wave.
+    for (uint32_t wi = gw; wi < words; wi += W) {
+      const unsigned long long fw = bm_cur[wi];
+      if (fw == 0) continue;
+      const uint32_t u = wi * 64 + lane;
+      const bool mine = (fw >> lane) & 1ull;
+      const uint2 rb = mine ? a.row[u] : make_uint2(0u, 0u);
+      const bool longr = rb.y > 32;
+      const uint32_t cnt = longr ? 0u : rb.y;
+      uint32_t most = cnt;
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) most = max(most, static_cast<uint32_t>(__shfl_xor(static_cast<int>(most), d, kWave)));
+      for (uint32_t j = 0; j < most; ++j) {
+        const bool act = j < cnt;
+        claim(act ? a.arena[rb.x + j] : kTomb, act);
+      }
+      unsigned long long lm = __ballot(longr);
+      while (lm) {
+        const int l = __ffsll(static_cast<long long>(lm)) - 1;
+        lm &= lm - 1;
+        const uint32_t bx = __shfl(rb.x, l, kWave), by = __shfl(rb.y, l, kWave);
+        for (uint32_t j = 0; j < by; j += kWave) {
+          const bool act = j + lane < by;
+          claim(act ? a.arena[bx + j + lane] : kTomb, act);
+        }
+      }
+    }
+  } else if (mode == 0) {
     const uint32_t F = static_cast<uint32_t>(qc >> kEdgeBits);
     const uint32_t* qcur = cur ? a.queue1 : a.queue0;
     const unsigned long long* pcur = cur ? a.pre1 : a.pre0;
     lbs_edges(qcur, pcur, F, mf, a.row, gw, W,
               [&](uint32_t, uint32_t, uint2, uint32_t slot, bool valid) {
-                uint32_t w = valid ? a.arena[slot] : kTomb;
-                bool newly = false;
-                uint32_t wl = 0;
-                if (!(w & kTomb)) {
-                  unsigned long long bit = 1ull << (w & 63);
-                  unsigned long long* wp = a.visited + (w >> 6);
-                  if (!(*wp & bit)) {
-                    unsigned long long old = atomicOr(wp, bit);
-                    newly = !(old & bit);
-                  }
-                }
-                if (newly) {
-                  a.dist[w] = next_level;
-                  atomicOr(bm_next + (w >> 6), 1ull << (w & 63));
-                  wl = a.row[w].y;
-                }
-                found_local += __popcll(__ballot(newly));
-                stage_push(sb, nbuf, &rn->q, qn, pn, newly && wl > 0, w, wl);
+                claim(valid ? a.arena[slot] : kTomb, valid);
               });
   } else {
     // Phase A: one lane per vertex checks the first kHead parents, read from
     // the dense head array (16 bytes per vertex, coalesced across the wave)
     // instead of the row itself.  Rows are sorted by id, so these are the
-    // vertex's lowest-id, on R-MAT highest-degree, parents.
-    for (uint32_t wi = gw; wi < words; wi += W) {
-      const unsigned long long vw = a.visited[wi];
-      if (vw == ~0ull) continue;
-      const uint32_t v = wi * 64 + lane;
-      const bool unv = v < a.n && !((vw >> lane) & 1ull);
-      uint4 h = make_uint4(kTomb, kTomb, kTomb, kTomb);
-      uint32_t len_in = 0;
-      if (unv) {
-        h = a.head[v];
-        len_in = a.in_row[v].y;
-      }
-      const bool found = in_frontier(bm_cur, h.x) | in_frontier(bm_cur, h.y) | in_frontier(bm_cur, h.z) |
-                         in_frontier(bm_cur, h.w);
+    // vertex's lowest-id, on R-MAT highest-degree, parents.  Each wave takes
+    // two words per step and issues both words' loads before using them.
+    const uint4 none = make_uint4(kTomb, kTomb, kTomb, kTomb);
+    for (uint32_t wi = gw; wi < words; wi += 2 * W) {
+      const uint32_t wj = wi + W;
+      const bool has_j = wj < words;
+      const unsigned long long vw0 = a.visited[wi];
+      const unsigned long long vw1 = has_j ? a.visited[wj] : ~0ull;
+      if ((vw0 & vw1) == ~0ull) continue;
+      const uint32_t v0 = wi * 64 + lane, v1 = wj * 64 + lane;
+      const bool unv0 = v0 < a.n && !((vw0 >> lane) & 1ull);
+      const bool unv1 = has_j && v1 < a.n && !((vw1 >> lane) & 1ull);
+      const uint4 h0 = unv0 ? a.head[v0] : none;
+      const uint4 h1 = unv1 ? a.head[v1] : none;
+      const bool f0 = in_frontier(bm_cur, h0.x) | in_frontier(bm_cur, h0.y) | in_frontier(bm_cur, h0.z) |
+                      in_frontier(bm_cur, h0.w);
+      const bool f1 = in_frontier(bm_cur, h1.x) | in_frontier(bm_cur, h1.y) | in_frontier(bm_cur, h1.z) |
+                      in_frontier(bm_cur, h1.w);
       // the rest of longer rows goes to phase B (k_bfs_bottom_up_tail)
-      stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, unv && !found && len_in > kHead, v,
-                 len_in - kHead);
-      const unsigned long long m = __ballot(found);
-      if (m) {
-        const uint32_t len = found ? a.row[v].y : 0u;
-        if (lane == 0) {
-          bm_next[wi] = m;
-          a.visited[wi] = vw | m;
+      const uint32_t li0 = (unv0 && !f0) ? a.in_row[v0].y : 0u;
+      const uint32_t li1 = (unv1 && !f1) ? a.in_row[v1].y : 0u;
+      stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, li0 > kHead, v0, li0 - kHead);
+      stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, li1 > kHead, v1, li1 - kHead);
+      // Bottom-up levels build no queue: the next level reads the bitmap.
+      const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1);
+      if (lane == 0) {
+        if (m0) {
+          bm_next[wi] = m0;
+          a.visited[wi] = vw0 | m0;
         }
-        if (found) a.dist[v] = next_level;
-        found_local += __popcll(m);
-        stage_push(sb, nbuf, &rn->q, qn, pn, found && len > 0, v, len);
+        if (m1) {
+          bm_next[wj] = m1;
+          a.visited[wj] = vw1 | m1;
+        }
       }
+      if (f0) a.dist[v0] = next_level;
+      if (f1) a.dist[v1] = next_level;
+      found_local += __popcll(m0) + __popcll(m1);
     }
   }
-  stage_flush(sb, nbuf, &rn->q, qn, pn);
-  stage_flush(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre);
+  // one atomic per workgroup and list for the last staged pushes
+  __shared__ unsigned long long s_flush[2 * kWavesPerBlock + 1];
+  stage_flush_block(sb, nbuf, &rn->q, qn, pn, s_flush);
+  stage_flush_block(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, s_flush);
   if (lane == 0) s_found[threadIdx.x / kWave] = found_local;
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long f = 0;
     for (uint32_t i = 0; i < kWavesPerBlock; ++i) f += s_found[i];
-    if (f) atomicAdd(&rn->found, f);
+    if (f) atomicAdd(&rn->found[blockIdx.x % kFoundStripes].v, f);
   }
 }
 
 // Bottom-up phase B of a level: the parents of pending rows beyond the first
-// kBottomUpHead are scanned edge-balanced and coalesced; a parent in the
-// frontier claims the row's vertex through the visited bitmap.  Exits at
-// once when the level ran top-down or left nothing pending.
+// kHead are scanned edge-balanced and coalesced; a parent in the frontier
+// claims the row's vertex through the visited bitmap.  Exits at once when
+// the level ran top-down or left nothing pending.
 __global__ __launch_bounds__(kBfsBlock) void k_bfs_bottom_up_tail(LevelArgs a, uint32_t level) {
   BfsState* st = a.st;
   if (st->done_level <= level) return;
-  if (st->rec[level & 3].found == 0) return;
+  if (found_total(st->rec[level & 3]) == 0) return;
   if (st->ctl[(level + 1) & 3].prev_mode != 1) return;
   const unsigned long long pq = st->pend[level & 3].q;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && level < kLogLevels) st->log_pend[level] = pq;
   if (pq == 0) return;
-  __shared__ WaveStage s_stage[kWavesPerBlock];
   __shared__ unsigned long long s_found[kWavesPerBlock];
-  WaveStage& sb = s_stage[threadIdx.x / kWave];
-  uint32_t nbuf = 0;
   unsigned long long found_local = 0;
   const uint32_t words = a.words;
   const unsigned long long* bm_cur = a.bm + static_cast<uint64_t>(level % 3) * words;
   unsigned long long* bm_next = a.bm + static_cast<uint64_t>((level + 1) % 3) * words;
   LevelRec* rn = &st->rec[(level + 1) & 3];
-  const int nxt = (level & 1) ^ 1;
-  uint32_t* qn = nxt ? a.queue1 : a.queue0;
-  unsigned long long* pn = nxt ? a.pre1 : a.pre0;
   const uint32_t next_level = level + 1;
   const uint32_t W = gridDim.x * kWavesPerBlock;
   const uint32_t gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
@@ -370,9 +451,19 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_bottom_up_tail(LevelArgs a, u
   lbs_edges(a.pend, a.ppre, static_cast<uint32_t>(pq >> kEdgeBits), pq & kEdgeMask, a.in_row, gw, W,
             [&](uint32_t, uint32_t v, uint2, uint32_t slot, bool valid) {
               const uint32_t x = valid ? a.in_arena[slot + kHead] : kTomb;
+              // A window holds consecutive edges of a few rows; of the lanes
+              // that find a parent of the same vertex only the lowest claims
+              // it, so the wave sends one atomic per vertex, not one per hit.
+              const bool hit = in_frontier(bm_cur, x);
+              const unsigned long long hits = __ballot(hit);
+              const uint32_t vprev = __shfl_up(v, 1, kWave);
+              const unsigned long long starts = __ballot(lane == 0 || vprev != v);
+              const unsigned long long below = (1ull << lane) - 1;
+              const unsigned long long seg = starts & (below | (1ull << lane));
+              const int first = 63 - __clzll(static_cast<long long>(seg));
+              const unsigned long long seg_below = below & ~((1ull << first) - 1);
               bool newly = false;
-              uint32_t len = 0;
-              if (in_frontier(bm_cur, x)) {
+              if (hit && !(hits & seg_below)) {
                 const unsigned long long bit = 1ull << (v & 63);
                 unsigned long long* wp = a.visited + (v >> 6);
                 if (!(*wp & bit)) newly = !(atomicOr(wp, bit) & bit);
@@ -380,18 +471,15 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_bottom_up_tail(LevelArgs a, u
               if (newly) {
                 a.dist[v] = next_level;
                 atomicOr(bm_next + (v >> 6), 1ull << (v & 63));
-                len = a.row[v].y;
               }
               found_local += __popcll(__ballot(newly));
-              stage_push(sb, nbuf, &rn->q, qn, pn, newly && len > 0, v, len);
             });
-  stage_flush(sb, nbuf, &rn->q, qn, pn);
   if (lane == 0) s_found[threadIdx.x / kWave] = found_local;
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long f = 0;
     for (uint32_t i = 0; i < kWavesPerBlock; ++i) f += s_found[i];
-    if (f) atomicAdd(&rn->found, f);
+    if (f) atomicAdd(&rn->found[blockIdx.x % kFoundStripes].v, f);
   }
 }
 
@@ -724,14 +812,14 @@ static void ensure_work(vgx_bfs* b) {
   w.dist.grow_copy(c, b->dist_len, g->stream);
   uint64_t words = (c + 63) / 64;
   w.visited.ensure(words);
+  w.pend.ensure(c);
+  w.ppre.ensure(c);
   w.bm.ensure(3 * words);
   for (int i = 0; i < 2; ++i) {
     w.queue[i].ensure(c);
     w.pre[i].ensure(c);
   }
   {
-    w.pend.ensure(c);
-    w.ppre.ensure(c);
   }
   // incremental workspace: zero-initialised counters and flags
   uint64_t old = w.lost.n ? w.cap : 0;
@@ -775,28 +863,33 @@ static LevelArgs level_args(vgx_bfs* b) {
   a.queue1 = b->w.queue[1].p;
   a.pre0 = b->w.pre[0].p;
   a.pre1 = b->w.pre[1].p;
-  a.pend = b->w.pend.p;
-  a.ppre = b->w.ppre.p;
   a.st = b->d_st;
   a.n = static_cast<uint32_t>(g->n);
   a.words = static_cast<uint32_t>((g->n + 63) / 64);
   a.head = b->w.head.p;
+  a.pend = b->w.pend.p;
+  a.ppre = b->w.ppre.p;
   return a;
 }
 
 static void bfs_full(vgx_bfs* b) {
   vgx_graph* g = b->g;
   ensure_work(b);
+  std::fill(std::begin(b->last_level_ms), std::end(b->last_level_ms), 0.f);
   hipStream_t s = g->stream;
   const uint64_t n = g->n;
   b->dist_len = n;
   const uint64_t words = (n + 63) / 64;
-  // the dense head array follows the graph's version
+  // the dense head array and the zero-in-degree bitmap follow the graph's version
   if (b->w.head_version != g->version || b->w.head_n != n) {
     b->w.head.ensure(g->ncap + 1);
+    b->w.zin.ensure(words + 1);
     if (n) {
       k_build_head<<<blocks_for(n, kBfsBlock, 4096), kBfsBlock, 0, s>>>(g->in().row.p, g->in().arena.p,
                                                                        static_cast<uint32_t>(n), b->w.head.p);
+      VGX_LAUNCH_CHECK();
+      k_build_zin<<<blocks_for(words, kWavesPerBlock, 4096), kBfsBlock, 0, s>>>(
+          g->in().deg.p, static_cast<uint32_t>(n), static_cast<uint32_t>(words), b->w.zin.p);
       VGX_LAUNCH_CHECK();
     }
     b->w.head_version = g->version;
@@ -804,7 +897,7 @@ static void bfs_full(vgx_bfs* b) {
   }
   if (n) {
     VGX_HIP(hipMemsetAsync(b->w.dist.p, 0xFF, n * sizeof(uint32_t), s));
-    VGX_HIP(hipMemsetAsync(b->w.visited.p, 0, words * 8, s));
+    VGX_HIP(hipMemcpyAsync(b->w.visited.p, b->w.zin.p, words * 8, hipMemcpyDeviceToDevice, s));
     VGX_HIP(hipMemsetAsync(b->w.bm.p, 0, 3 * words * 8, s));
   }
   LevelArgs a = level_args(b);
@@ -813,8 +906,10 @@ static void bfs_full(vgx_bfs* b) {
   VGX_LAUNCH_CHECK();
   unsigned blocks = b->tune.blocks;
   if (blocks == 0) {
+    // about one resident round of workgroups: more waves only add flushes
+    // on the level counters (measured at R-MAT scale 22: 1024 beats 2048)
     uint64_t waves = std::max<uint64_t>(words, g->adj[0].stored / 512);
-    blocks = static_cast<unsigned>(std::min<uint64_t>(2048, std::max<uint64_t>(1, (waves + 3) / 4)));
+    blocks = static_cast<unsigned>(std::min<uint64_t>(1024, std::max<uint64_t>(1, (waves + 3) / 4)));
   }
   // The first chunk covers the depth of the previous traversal (one launch
   // beyond it detects completion); later chunks shrink.
@@ -843,6 +938,7 @@ static void bfs_full(vgx_bfs* b) {
         float ms = 0.f;
         VGX_HIP(hipEventElapsedTime(&ms, b->events[2 * i], b->events[2 * i + 1]));
         b->level_ms += ms;
+        if (launched - chunk + i < kLogLevels) b->last_level_ms[launched - chunk + i] = ms;
       }
       b->level_launches += chunk;
     }
@@ -1277,6 +1373,25 @@ vgx_status vgx_bfs_autotune(vgx_bfs* b, uint32_t reps, vgx_bfs_tuning* best, dou
   b->profiling = prof;
   if (best) *best = cur;
   if (best_ms) *best_ms = cur_ms;
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_level_log(vgx_bfs* b, vgx_bfs_level* out, uint32_t cap, uint32_t* count) {
+  VGX_API_BEGIN
+  if (!b || !count) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  VGX_HIP(hipSetDevice(b->g->device));
+  VGX_HIP(hipMemcpyAsync(b->h_st, b->d_st, sizeof(BfsState), hipMemcpyDeviceToHost, b->g->stream));
+  VGX_HIP(hipStreamSynchronize(b->g->stream));
+  const uint32_t levels = std::min<uint32_t>(b->levels, kLogLevels);
+  *count = levels;
+  for (uint32_t i = 0; i < levels && i < cap && out; ++i) {
+    out[i].bottom_up = b->h_st->log_mode[i];
+    out[i].frontier_vertices = b->h_st->log_nf[i];
+    out[i].frontier_edges = b->h_st->log_mf[i];
+    out[i].pending_vertices = b->h_st->log_pend[i] >> kEdgeBits;
+    out[i].pending_edges = b->h_st->log_pend[i] & kEdgeMask;
+    out[i].kernel_ms = b->last_level_ms[i];
+  }
   VGX_API_END
 }
 

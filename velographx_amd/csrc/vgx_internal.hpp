@@ -361,6 +361,53 @@ __device__ __forceinline__ void stage_flush(WaveStage& sb, uint32_t& nbuf, unsig
 
 // Stage (v, len) for the lanes that want it; flushes when nearly full.
 // Called by all lanes of the wave.
+// Final flush of all waves of a workgroup with one atomic for the whole
+// workgroup (instead of one per wave).  Every thread of the workgroup must
+// call it; `sm` is a __shared__ array of 2 * waves + 1 words.
+__device__ __forceinline__ void stage_flush_block(WaveStage& sb, uint32_t& nbuf, unsigned long long* ctr,
+                                                  uint32_t* queue, unsigned long long* pre, unsigned long long* sm) {
+  const unsigned lane = lane_id();
+  const uint32_t wv = threadIdx.x / kWave, waves = blockDim.x / kWave;
+  unsigned long long total = 0;
+  for (uint32_t c = 0; c < nbuf; c += kWave) {
+    const uint32_t l = c + lane < nbuf ? sb.len[c + lane] : 0u;
+    total += wave_sum<unsigned long long>(l);
+  }
+  if (lane == 0) {
+    sm[2 * wv] = nbuf;
+    sm[2 * wv + 1] = total;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long cnt = 0, edges = 0;
+    for (uint32_t i = 0; i < waves; ++i) {
+      cnt += sm[2 * i];
+      edges += sm[2 * i + 1];
+    }
+    sm[2 * waves] = cnt ? atomicAdd(ctr, (cnt << kEdgeBits) + edges) : 0ull;
+  }
+  __syncthreads();
+  const unsigned long long base = sm[2 * waves];
+  unsigned long long slot = base >> kEdgeBits;
+  unsigned long long eoff = base & kEdgeMask;
+  for (uint32_t i = 0; i < wv; ++i) {
+    slot += sm[2 * i];
+    eoff += sm[2 * i + 1];
+  }
+  for (uint32_t c = 0; c < nbuf; c += kWave) {
+    const bool in = c + lane < nbuf;
+    const uint32_t l = in ? sb.len[c + lane] : 0u;
+    const unsigned long long inc = wave_inclusive_scan<unsigned long long>(l);
+    if (in) {
+      queue[slot + c + lane] = sb.v[c + lane];
+      pre[slot + c + lane] = eoff + inc - l;
+    }
+    eoff += __shfl(inc, kWave - 1, kWave);
+  }
+  __syncthreads();
+  nbuf = 0;
+}
+
 __device__ __forceinline__ void stage_push(WaveStage& sb, uint32_t& nbuf, unsigned long long* ctr,
                                            uint32_t* queue, unsigned long long* pre, bool want, uint32_t v,
                                            uint32_t len) {

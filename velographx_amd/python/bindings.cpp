@@ -330,6 +330,25 @@ PYBIND11_MODULE(_velographx, m) {
              d["levels_per_sync"] = t.levels_per_sync;
              return d;
            })
+      .def("level_log",
+           [](const IncrementalBFS& b) {
+             vgx_bfs_level lv[64];
+             std::uint32_t cnt = 0;
+             velographx::vgx_check(vgx_bfs_level_log(b.handle(), lv, 64, &cnt));
+             py::list out;
+             for (std::uint32_t i = 0; i < cnt && i < 64; ++i) {
+               py::dict d;
+               d["bottom_up"] = lv[i].bottom_up == 1;
+               d["mode"] = lv[i].bottom_up == 1 ? "bottom-up" : (lv[i].bottom_up == 2 ? "top-down-bitmap" : "top-down");
+               d["frontier_vertices"] = lv[i].frontier_vertices;
+               d["frontier_edges"] = lv[i].frontier_edges;
+               d["pending_vertices"] = lv[i].pending_vertices;
+               d["pending_edges"] = lv[i].pending_edges;
+               d["kernel_ms"] = lv[i].kernel_ms;
+               out.append(d);
+             }
+             return out;
+           })
       .def(
           "autotune",
           [](IncrementalBFS& b, std::uint32_t reps) {
