@@ -283,3 +283,31 @@ def test_direction_switch_levels(vx, orc):
             st = bfs.stats()
             if alpha > 1e8:
                 assert st["bottom_up_levels"] >= 1
+
+
+def test_level_shape_changes_between_traversals(vx, orc):
+    """Launch sizes follow the previous traversal's level shape; when the
+    graph changes shape (a long path gains dense shortcuts, then loses them)
+    the next traversal still has to be exact, including bottom-up levels
+    that the previous shape predicted to be top-down."""
+    n = 3000
+    path = [(i, i + 1) for i in range(n - 1)]
+    g = vx.Graph(0, False)
+    g.bulk_load_edges(np.array(path, dtype=np.uint32))
+    mg = orc.ModelGraph(0, False)
+    mg.bulk_load_edges(path)
+    bfs = vx.IncrementalBFS(g, 0)
+    bfs.recompute()
+    rng = np.random.default_rng(7)
+    hubs = [(1, v) for v in range(3, n, 2)] + [(2, v) for v in range(4, n, 2)]
+    extra = [tuple(int(x) for x in e) for e in rng.integers(0, n, size=(20000, 2)) if e[0] != e[1]]
+    for ops, add in ((hubs + extra, 1), (hubs, 0)):
+        s = np.array([o[0] for o in ops], np.uint32)
+        d = np.array([o[1] for o in ops], np.uint32)
+        a = np.full(len(ops), add, np.uint8)
+        g.apply_arrays(s, d, a)
+        mg.apply(s, d, a)
+        full, _ = orc.model_bfs(mg, 0)
+        for _ in range(2):
+            bfs.recompute()
+            assert np.array_equal(bfs.distances_numpy(), full)

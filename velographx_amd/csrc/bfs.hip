@@ -163,6 +163,10 @@ struct vgx_bfs {
   uint64_t level_launches = 0;
   double level_ms = 0.0;
   float last_level_ms[64] = {};  // per launch of the last traversal (profiling)
+  // shape of the last full traversal, used to size the next one's launches
+  uint32_t prof_levels = 0;
+  uint32_t prof_mode[64] = {};
+  uint64_t prof_mf[64] = {};
 };
 
 namespace vgx {
@@ -248,8 +252,39 @@ __device__ __forceinline__ bool in_frontier(const unsigned long long* bm, uint32
   return !(x & kTomb) && ((bm[x >> 6] >> (x & 63)) & 1ull);
 }
 
-// One BFS level.  `level` is the launch index passed by the host.
-__global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t level) {
+// Whether a parent of v beyond the head is in the frontier: rows with up to
+// 64 more entries on the lane itself, longer rows by the whole wave, first
+// hit ends the scan.  Called by the whole wave; `want` marks the lanes asking.
+__device__ __forceinline__ bool rest_in_frontier(const LevelArgs& a, const unsigned long long* bm_cur, uint2 rb,
+                                                 bool want) {
+  bool found = false;
+  const bool mine = want && rb.y - kHead <= kWave;
+  if (mine)
+    for (uint32_t j = kHead; j < rb.y; ++j)
+      if (in_frontier(bm_cur, a.in_arena[rb.x + j])) {
+        found = true;
+        break;
+      }
+  unsigned long long lm = __ballot(want && !mine);
+  while (lm) {
+    const int l = __ffsll(static_cast<long long>(lm)) - 1;
+    lm &= lm - 1;
+    const uint32_t bx = __shfl(rb.x, l, kWave), by = __shfl(rb.y, l, kWave);
+    for (uint32_t j = kHead; j < by; j += kWave) {
+      const bool hit = j + lane_id() < by && in_frontier(bm_cur, a.in_arena[bx + j + lane_id()]);
+      if (__ballot(hit)) {
+        if (static_cast<int>(lane_id()) == l) found = true;
+        break;
+      }
+    }
+  }
+  return found;
+}
+
+// One BFS level.  `level` is the launch index passed by the host.  With
+// `inline_tail` set (the host predicted a top-down level and launches no
+// tail kernel after it), a bottom-up level finishes its rows itself.
+__global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t level, int inline_tail) {
   BfsState* st = a.st;
   const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
   // Launches enqueued past the last level must not read the ring: its slots
@@ -387,15 +422,20 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
       const bool unv1 = has_j && v1 < a.n && !((vw1 >> lane) & 1ull);
       const uint4 h0 = unv0 ? a.head[v0] : none;
       const uint4 h1 = unv1 ? a.head[v1] : none;
-      const bool f0 = in_frontier(bm_cur, h0.x) | in_frontier(bm_cur, h0.y) | in_frontier(bm_cur, h0.z) |
-                      in_frontier(bm_cur, h0.w);
-      const bool f1 = in_frontier(bm_cur, h1.x) | in_frontier(bm_cur, h1.y) | in_frontier(bm_cur, h1.z) |
-                      in_frontier(bm_cur, h1.w);
+      bool f0 = in_frontier(bm_cur, h0.x) | in_frontier(bm_cur, h0.y) | in_frontier(bm_cur, h0.z) |
+                in_frontier(bm_cur, h0.w);
+      bool f1 = in_frontier(bm_cur, h1.x) | in_frontier(bm_cur, h1.y) | in_frontier(bm_cur, h1.z) |
+                in_frontier(bm_cur, h1.w);
       // the rest of longer rows goes to phase B (k_bfs_bottom_up_tail)
-      const uint32_t li0 = (unv0 && !f0) ? a.in_row[v0].y : 0u;
-      const uint32_t li1 = (unv1 && !f1) ? a.in_row[v1].y : 0u;
-      stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, li0 > kHead, v0, li0 - kHead);
-      stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, li1 > kHead, v1, li1 - kHead);
+      const uint2 r0 = (unv0 && !f0) ? a.in_row[v0] : make_uint2(0u, 0u);
+      const uint2 r1 = (unv1 && !f1) ? a.in_row[v1] : make_uint2(0u, 0u);
+      if (inline_tail) {
+        f0 = f0 || rest_in_frontier(a, bm_cur, r0, r0.y > kHead);
+        f1 = f1 || rest_in_frontier(a, bm_cur, r1, r1.y > kHead);
+      } else {
+        stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, r0.y > kHead, v0, r0.y - kHead);
+        stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, r1.y > kHead, v1, r1.y - kHead);
+      }
       // Bottom-up levels build no queue: the next level reads the bitmap.
       const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1);
       if (lane == 0) {
@@ -909,8 +949,32 @@ static void bfs_full(vgx_bfs* b) {
     // about one resident round of workgroups: more waves only add flushes
     // on the level counters (measured at R-MAT scale 22: 1024 beats 2048)
     uint64_t waves = std::max<uint64_t>(words, g->adj[0].stored / 512);
-    blocks = static_cast<unsigned>(std::min<uint64_t>(1024, std::max<uint64_t>(1, (waves + 3) / 4)));
+    blocks = static_cast<unsigned>(std::min<uint64_t>(1536, std::max<uint64_t>(1, (waves + 3) / 4)));
   }
+  // With automatic sizing, a level that ran top-down from a small queue in
+  // the previous traversal of this object gets a grid sized to its edges,
+  // and the tail kernel a token grid where that level was not bottom-up:
+  // dispatching ~1500 idle workgroups costs several microseconds per launch.
+  // Every kernel is grid-stride, so a wrong guess costs time, never results.
+  const bool adaptive = b->tune.blocks == 0 && b->prof_levels > 0;
+  // tb == 0: no tail kernel, the level kernel finishes bottom-up rows inline.
+  auto grids = [&](uint32_t L, unsigned& lb, unsigned& tb) {
+    lb = blocks;
+    tb = blocks;
+    if (!adaptive) return;
+    if (L >= b->prof_levels) {
+      lb = std::min(blocks, 4u);
+      tb = 0;
+      return;
+    }
+    if (L >= kLogLevels) return;
+    if (b->prof_mode[L] != 1) tb = 0;
+    if (b->prof_mode[L] == 0) {
+      const uint64_t windows = (b->prof_mf[L] + kWave - 1) / kWave;  // 64-edge windows
+      const uint64_t want = (windows + 16 * kWavesPerBlock - 1) / (16 * kWavesPerBlock);
+      lb = static_cast<unsigned>(std::min<uint64_t>(blocks, std::max<uint64_t>(16, want)));
+    }
+  };
   // The first chunk covers the depth of the previous traversal (one launch
   // beyond it detects completion); later chunks shrink.
   uint32_t chunk = std::max<uint32_t>(1, b->tune.levels_per_sync);
@@ -924,10 +988,14 @@ static void bfs_full(vgx_bfs* b) {
     }
     for (uint32_t i = 0; i < chunk; ++i) {
       if (b->profiling) VGX_HIP(hipEventRecord(b->events[2 * i], s));
-      k_bfs_level<<<blocks, kBfsBlock, 0, s>>>(a, launched + i);
+      unsigned lb, tb;
+      grids(launched + i, lb, tb);
+      k_bfs_level<<<lb, kBfsBlock, 0, s>>>(a, launched + i, tb == 0 ? 1 : 0);
       VGX_LAUNCH_CHECK();
-      k_bfs_bottom_up_tail<<<blocks, kBfsBlock, 0, s>>>(a, launched + i);
-      VGX_LAUNCH_CHECK();
+      if (tb) {
+        k_bfs_bottom_up_tail<<<tb, kBfsBlock, 0, s>>>(a, launched + i);
+        VGX_LAUNCH_CHECK();
+      }
       if (b->profiling) VGX_HIP(hipEventRecord(b->events[2 * i + 1], s));
     }
     launched += chunk;
@@ -949,6 +1017,11 @@ static void bfs_full(vgx_bfs* b) {
   b->reachable = b->h_st->reached;
   b->levels = b->h_st->done_level;
   b->bu_levels = b->h_st->bu_levels;
+  b->prof_levels = b->levels;
+  for (uint32_t L = 0; L < b->levels && L < kLogLevels; ++L) {
+    b->prof_mode[L] = b->h_st->log_mode[L];
+    b->prof_mf[L] = b->h_st->log_mf[L];
+  }
 }
 
 static IncArgs inc_args(vgx_bfs* b) {
