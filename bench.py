@@ -29,6 +29,10 @@ Secondary legs, each in its own object of the same JSON line:
                the reference engine's IncrementalBFS::apply timed on a
                bounded sample of the same batches;
 * contrast  -- Erdos-Renyi, 4,194,304 vertices, mean degree 16;
+* batching  -- R-MAT scale 20: 64 roots per hipGraph-captured multi-source
+               pass (autotuned), then 256 update batches of log-uniform size
+               10 .. 100,000, each applied and answered by a replay; sampled
+               roots checked bit-exact against the single-root path;
 * pagerank  -- directed R-MAT scale 22: FP32 pull PageRank to an L1
                residual below 1e-7, then 50 batches of 10,000 ops with a
                warm-started FP32 re-solve; L1 / max-abs error against the
@@ -78,6 +82,9 @@ def parse():
     p.add_argument("--consolidate-every", type=int, default=20)
     p.add_argument("--cpu-budget", type=float, default=20.0)
     p.add_argument("--pr-batches", type=int, default=50)
+    p.add_argument("--batch-scale", type=int, default=20)
+    p.add_argument("--uneven-batches", type=int, default=256)
+    p.add_argument("--no-batching", action="store_true")
     p.add_argument("--ppr-scale", type=int, default=20)
     p.add_argument("--ppr-iterations", type=int, default=100)
     p.add_argument("--ppr-reps", type=int, default=5)
@@ -327,6 +334,78 @@ def leg_batched(vx, wl, g, lcc, m_c, k, plain_objs, plain_gteps):
     return out
 
 
+def leg_batching(vx, wl, args):
+    """R-MAT scale 20: 64 roots per captured multi-source pass, then 256
+    update batches of log-uniform size (10 .. 100,000 ops, 80/20), each
+    applied to the graph and answered by replaying the captured pass."""
+    import torch
+
+    scale = args.batch_scale
+    n = 1 << scale
+    s, d = vx.rmat_edges(scale, args.edgefactor, 1)
+    g = vx.Graph(n, False)
+    g.bulk_load_arrays(s, d)
+    lcc, arcs_c = component(vx, wl, g)
+    roots = wl.roots_in_component(lcc, 64, seed=1).tolist()
+    ms = vx.MultiSourceBFS(g, roots)
+    tuned = ms.autotune(2)
+    ms.set_graph(True)
+    ms.run(g)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ms.run(g)
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - t0) / reps
+    ms.set_graph(False)
+    ms.run(g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ms.run(g)
+    torch.cuda.synchronize()
+    t_eager = (time.perf_counter() - t0) / reps
+    ms.set_graph(True)
+    sizes = wl.log_uniform_sizes(args.uneven_batches, 1.0, 5.0, seed=1)
+    eu, ev = wl.canonical_edges(s, d, False)
+    stream = wl.BatchStream(eu, ev, n, False, seed=3)
+    batches = [stream.next(int(k), 0.8) for k in sizes]
+    t_apply, t_pass = [], []
+    exact = True
+    checked = 0
+    singles = [vx.IncrementalBFS(g, int(r)) for r in roots[:8]]
+    for i, (bs, bd, ba) in enumerate(batches):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.apply_arrays(bs, bd, ba)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ms.run(g)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_apply.append(t1 - t0)
+        t_pass.append(t2 - t1)
+        if i % 32 == 31 or i == len(batches) - 1:  # verification outside the timer
+            for j, o in enumerate(singles):
+                o.recompute()
+                exact &= bool(np.array_equal(ms.distances(j), o.distances_numpy()))
+            checked += 1
+    ops = int(sum(int(k) for k in sizes))
+    out = {"workload": f"undirected R-MAT s{scale} ef{args.edgefactor} seed 1, 64 LCC roots per captured pass; "
+                       f"{len(batches)} batches of log-uniform size 10..100000 (80/20)",
+           "arcs": int(g.edge_count_directed), "pass_ms_graph": 1e3 * t_graph, "pass_ms_eager": 1e3 * t_eager,
+           "aggregate_gteps_graph": arcs_c / 2.0 * 64 / t_graph / 1e9,
+           "tuning": {"blocks": tuned["blocks"], "alpha": tuned["alpha"]},
+           "batches": len(batches), "update_ops": ops,
+           "mean_apply_ms": 1e3 * float(np.mean(t_apply)), "mean_pass_ms": 1e3 * float(np.mean(t_pass)),
+           "batches_per_s": len(batches) / (sum(t_apply) + sum(t_pass)),
+           "graph_replays": int(ms.stats()["graph_replays"]),
+           "exact_vs_single_root": exact, "verified_batches": checked, "verified_roots_per_check": len(singles)}
+    log(f"batching: {out}")
+    return out
+
+
 def leg_repairs(vx, wl, g, s, d, n, root, args):
     import torch
 
@@ -530,7 +609,7 @@ def main():
         return
     if args.only_bfs:
         args.no_cpu = args.no_repairs = args.no_batched = args.no_contrast = True
-        args.no_pagerank = args.no_ppr = True
+        args.no_pagerank = args.no_ppr = args.no_batching = True
     import torch
 
     import velographx_amd as vx
@@ -629,6 +708,8 @@ def main():
                 log(f"cpu repairs: {cr}")
     if not args.no_contrast:
         result["contrast"] = leg_contrast(vx, wl, args.steps)
+    if not args.no_batching:
+        result["batching"] = leg_batching(vx, wl, args)
     if not args.no_pagerank:
         result["pagerank"] = leg_pagerank(vx, wl, args, args.no_cpu)
     if not args.no_ppr:

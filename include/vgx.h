@@ -238,6 +238,15 @@ vgx_status vgx_msbfs_level_count(vgx_msbfs* m, uint32_t* levels);
  * candidate, alpha over {2, 4, 8, 16, 32} then workgroups over {auto, 512,
  * 1024, 2048, 4096, 8192}, and keeps the fastest. */
 vgx_status vgx_msbfs_set_tuning(vgx_msbfs* m, uint32_t blocks, double alpha);
+/* Run the pass as a captured hipGraph (no reference counterpart): the
+ * launch program of the first 16 levels is captured once and replayed by
+ * later runs, also after the graph changed (its arguments live in device
+ * memory and are rewritten before each replay); a change of vertex count or
+ * size class captures again.  Results are identical to the eager path. */
+vgx_status vgx_msbfs_set_graph(vgx_msbfs* m, int on);
+/* Levels that ran in the push direction in the last run, and hipGraph
+ * replays so far. */
+vgx_status vgx_msbfs_stats(vgx_msbfs* m, uint32_t* push_levels, uint64_t* graph_replays);
 vgx_status vgx_msbfs_autotune(vgx_msbfs* m, uint32_t reps, uint32_t* best_blocks, double* best_alpha,
                               double* best_ms);
 

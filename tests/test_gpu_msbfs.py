@@ -147,3 +147,42 @@ def test_autotune_keeps_results(vx, orc):
     best = ms.autotune(1)
     assert best["alpha"] > 0 and best["ms"] > 0
     check_all_roots(ms, g, mg, orc, roots)
+
+
+def test_graph_replay_across_updates(vx, orc):
+    """The captured launch program is replayed after every update batch
+    (uneven sizes); each replay equals the oracle for every root."""
+    from velographx_amd import workloads as wl
+
+    s, d = vx.rmat_edges(12, 16, 4)
+    g = vx.Graph(1 << 12, False)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(1 << 12, False)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    roots = np.random.default_rng(2).integers(0, 1 << 12, size=64).tolist()
+    ms = vx.MultiSourceBFS(g, roots)
+    ms.set_graph(True)
+    check_all_roots(ms, g, mg, orc, roots)
+    eu, ev = wl.canonical_edges(s, d, False)
+    stream = wl.BatchStream(eu, ev, 1 << 12, False, seed=8)
+    for k in (10, 3000, 150, 20000, 40):
+        bs, bd, ba = stream.next(k, 0.8)
+        g.apply_arrays(bs, bd, ba)
+        mg.apply(bs, bd, ba)
+        check_all_roots(ms, g, mg, orc, roots)
+    assert ms.stats()["graph_replays"] >= 6
+
+
+def test_graph_mode_deep_path(vx):
+    """Deeper than one captured program (16 levels): the pass continues
+    eagerly from where the replay stopped."""
+    n = 100
+    g = vx.Graph(0, False)
+    g.bulk_load_edges(np.array([(i, i + 1) for i in range(n - 1)], dtype=np.uint32))
+    ms = vx.MultiSourceBFS(g, [0, 50])
+    ms.set_graph(True)
+    for _ in range(2):
+        ms.run(g)
+        assert np.array_equal(ms.distances(0), np.arange(n, dtype=np.uint32))
+        assert np.array_equal(ms.distances(1), np.abs(np.arange(n) - 50).astype(np.uint32))
+        assert ms.level_count == n

@@ -18,8 +18,17 @@
 // vertices per wave); longer rows are cut into chunks of 1024 arcs, one wave
 // each, so R-MAT hubs (which cluster at low ids) spread over many waves.
 // OR is commutative and idempotent, so the result does not depend on the
-// order of the atomics.  A separate pass folds next into seen and counts the
-// new (vertex, root) pairs and the next frontier's out-arcs.
+// order of the atomics.
+//
+// The pass is a fixed program of launches that the device steers by
+// itself: every level runs push/pull kernels that return at once unless the
+// level's direction (decided on the device at the end of the previous level)
+// is theirs, a finalize kernel that folds the level into seen and counts new
+// pairs and next-frontier arcs, and a one-wave decide kernel that picks the
+// next direction or marks the pass done.  Kernel arguments are one device
+// block rewritten before each pass, so the same launch program can be
+// captured once in a hipGraph and replayed for every pass while the graph
+// changes underneath (only its size class forces a new capture).
 // Per-root uint8 levels (255 = unreachable) are written at the end by a
 // transpose that keeps the byte stores coalesced.
 #include <hipcub/hipcub.hpp>
@@ -38,27 +47,52 @@ constexpr int kMsBlock = 256;
 constexpr uint32_t kMsWaves = kMsBlock / kWave;
 constexpr uint32_t kMsShortRow = 32;
 constexpr uint32_t kMsChunk = 1024;
+constexpr uint32_t kMsStripes = 32;
+// Levels one launch program covers; deeper traversals continue with
+// further programs from where the previous one stopped.
+constexpr uint32_t kMsProgram = 16;
 
 // Level counters, striped over kMsStripes cache lines (workgroup b adds to
-// stripe b % kMsStripes; the host sums the stripes), so the finalize pass
-// does not serialise on one address.
-constexpr uint32_t kMsStripes = 32;
+// stripe b % kMsStripes; the decide kernel sums the stripes).
 struct alignas(128) MsStripe {
   unsigned long long new_pairs;      // (vertex, root) pairs reached this level
   unsigned long long frontier_arcs;  // out-arcs of the vertices reached this level
 };
-struct MsCounters {
+struct MsLevel {
   MsStripe s[kMsStripes];
-  unsigned long long pairs() const {
-    unsigned long long t = 0;
-    for (const auto& x : s) t += x.new_pairs;
-    return t;
-  }
-  unsigned long long arcs() const {
-    unsigned long long t = 0;
-    for (const auto& x : s) t += x.frontier_arcs;
-    return t;
-  }
+};
+
+// Device-side control of one pass.
+struct MsCtl {
+  uint32_t dir[kMsProgram + 1];  // per program level: 0 push, 1 pull
+  uint32_t done;                 // set once a level found nothing new
+  uint32_t levels;               // stored levels (valid once done)
+  uint32_t push_levels;
+  uint32_t pad;
+  MsLevel ctr[kMsProgram];
+};
+
+// Everything a level kernel needs, in device memory (rewritten per pass).
+struct MsArgs {
+  const uint32_t* out_off;
+  const uint32_t* out_adj;
+  const uint32_t* out_rc;  // long-row chunk plan of the out-CSR
+  const uint32_t* out_cr;
+  const uint32_t* in_off;
+  const uint32_t* in_adj;
+  const uint32_t* in_rc;
+  const uint32_t* in_cr;
+  uint32_t out_nch, in_nch;
+  unsigned long long* seen;
+  unsigned long long* newbits;  // level l of this program at newbits + (base + l) * n
+  const uint32_t* roots;
+  uint32_t n, k;
+  uint32_t base;  // absolute level of program level 0
+  uint32_t pad;
+  unsigned long long all;
+  double arcs;
+  double alpha;
+  MsCtl* ctl;
 };
 
 // Chunks of the rows longer than kMsShortRow in one snapshot: row v owns
@@ -83,27 +117,61 @@ struct vgx_msbfs {
   vgx::DevBuf<unsigned long long> seen;
   vgx::DevBuf<unsigned long long> newbits;  // [levels_cap][n]
   uint64_t levels_cap = 0;
-  vgx::MsCounters* d_ctr = nullptr;
-  vgx::MsCounters* h_ctr = nullptr;
+  vgx::DevBuf<uint32_t> droots;
+  vgx::MsArgs* d_args = nullptr;
+  vgx::MsCtl* d_ctl = nullptr;
+  vgx::MsCtl* h_ctl = nullptr;  // pinned
   std::vector<uint64_t> reached;  // per root
   uint32_t blocks = 0;            // short-row kernel workgroups, 0 = auto
   double alpha = 4.0;             // push while frontier arcs < arcs / alpha
   vgx::MsPlan plan[2];            // [0] out-rows (push), [1] in-rows (pull)
   vgx::DevBuf<uint8_t> scan;
   uint32_t push_levels = 0;
+  // captured launch program
+  bool use_graph = false;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  uint64_t graph_key[4] = {0, 0, 0, 0};
+  uint64_t replays = 0;
 };
 
 namespace vgx {
 
-__global__ void k_ms_init(unsigned long long* seen, unsigned long long* nb0, uint64_t n, const uint32_t* roots,
-                          uint32_t k) {
+__device__ __forceinline__ unsigned long long* level_ptr(const MsArgs& a, uint32_t l) {
+  return a.newbits + static_cast<uint64_t>(a.base + l) * a.n;
+}
+
+// Program start: at absolute level 0 the roots' bits; later programs
+// continue from the last level of the previous one.  Clears the level
+// counters and the first `next` array, and sets the first direction.
+__global__ void k_ms_start(const MsArgs* __restrict__ pa, int fresh, uint32_t first_dir) {
+  const MsArgs a = *pa;
+  const uint64_t n = a.n;
+  unsigned long long* nb0 = level_ptr(a, 0);
+  unsigned long long* nb1 = level_ptr(a, 1);
   for (uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; v < n;
        v += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    unsigned long long m = 0;
-    for (uint32_t r = 0; r < k; ++r)
-      if (roots[r] == v) m |= 1ull << r;
-    seen[v] = m;
-    nb0[v] = m;
+    if (fresh) {
+      unsigned long long m = 0;
+      for (uint32_t r = 0; r < a.k; ++r)
+        if (a.roots[r] == v) m |= 1ull << r;
+      a.seen[v] = m;
+      nb0[v] = m;
+    }
+    nb1[v] = 0;
+  }
+  if (blockIdx.x == 0) {
+    MsCtl* c = a.ctl;
+    for (uint32_t i = threadIdx.x; i < kMsProgram * kMsStripes; i += blockDim.x) {
+      c->ctr[i / kMsStripes].s[i % kMsStripes].new_pairs = 0;
+      c->ctr[i / kMsStripes].s[i % kMsStripes].frontier_arcs = 0;
+    }
+    if (threadIdx.x == 0) {
+      c->dir[0] = first_dir;
+      c->done = 0;
+      c->levels = 0;
+      c->push_levels = 0;
+    }
   }
 }
 
@@ -119,20 +187,26 @@ __global__ void k_ms_chunk_fill(const uint32_t* __restrict__ row_chunk, uint32_t
     for (uint32_t c = row_chunk[v]; c < row_chunk[v + 1]; ++c) chunk_row[c] = v;
 }
 
+// Whether program level l runs in direction `dir`.
+__device__ __forceinline__ bool level_runs(const MsArgs& a, uint32_t l, uint32_t dir) {
+  const MsCtl* c = a.ctl;
+  return !c->done && c->dir[l] == dir;
+}
+
 // push, short rows: lane per frontier vertex u.
-__global__ __launch_bounds__(kMsBlock) void k_ms_push_short(const uint32_t* __restrict__ off,
-                                                             const uint32_t* __restrict__ adj, uint32_t n,
-                                                             const unsigned long long* __restrict__ frontier,
-                                                             const unsigned long long* __restrict__ seen,
-                                                             unsigned long long* next) {
-  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(kMsBlock) void k_ms_push_short(const MsArgs* __restrict__ pa, uint32_t l) {
+  const MsArgs a = *pa;
+  if (!level_runs(a, l, 0)) return;
+  const unsigned long long* frontier = level_ptr(a, l);
+  unsigned long long* next = level_ptr(a, l + 1);
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < a.n; u += gridDim.x * blockDim.x) {
     const unsigned long long f = frontier[u];
     if (!f) continue;
-    const uint32_t b = off[u], e = off[u + 1];
+    const uint32_t b = a.out_off[u], e = a.out_off[u + 1];
     if (e - b > kMsShortRow) continue;
     for (uint32_t i = b; i < e; ++i) {
-      const uint32_t v = adj[i];
-      const unsigned long long m = f & ~seen[v];
+      const uint32_t v = a.out_adj[i];
+      const unsigned long long m = f & ~a.seen[v];
       // bits only ever get added to next[v]: skip the atomic when they are there
       if (m & ~next[v]) atomicOr(&next[v], m);
     }
@@ -140,22 +214,20 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_push_short(const uint32_t* __re
 }
 
 // push, long rows: wave per chunk, lane per arc.
-__global__ __launch_bounds__(kMsBlock) void k_ms_push_long(const uint32_t* __restrict__ off,
-                                                            const uint32_t* __restrict__ adj,
-                                                            const uint32_t* __restrict__ row_chunk,
-                                                            const uint32_t* __restrict__ chunk_row, uint32_t nchunks,
-                                                            const unsigned long long* __restrict__ frontier,
-                                                            const unsigned long long* __restrict__ seen,
-                                                            unsigned long long* next) {
-  for (uint32_t c = blockIdx.x * kMsWaves + threadIdx.x / kWave; c < nchunks; c += gridDim.x * kMsWaves) {
-    const uint32_t u = chunk_row[c];
+__global__ __launch_bounds__(kMsBlock) void k_ms_push_long(const MsArgs* __restrict__ pa, uint32_t l) {
+  const MsArgs a = *pa;
+  if (!level_runs(a, l, 0)) return;
+  const unsigned long long* frontier = level_ptr(a, l);
+  unsigned long long* next = level_ptr(a, l + 1);
+  for (uint32_t c = blockIdx.x * kMsWaves + threadIdx.x / kWave; c < a.out_nch; c += gridDim.x * kMsWaves) {
+    const uint32_t u = a.out_cr[c];
     const unsigned long long f = frontier[u];
     if (!f) continue;
-    const uint32_t b = off[u] + (c - row_chunk[u]) * kMsChunk;
-    const uint32_t e = min(b + kMsChunk, off[u + 1]);
+    const uint32_t b = a.out_off[u] + (c - a.out_rc[u]) * kMsChunk;
+    const uint32_t e = min(b + kMsChunk, a.out_off[u + 1]);
     for (uint32_t i = b + lane_id(); i < e; i += kWave) {
-      const uint32_t v = adj[i];
-      const unsigned long long m = f & ~seen[v];
+      const uint32_t v = a.out_adj[i];
+      const unsigned long long m = f & ~a.seen[v];
       if (m & ~next[v]) atomicOr(&next[v], m);
     }
   }
@@ -163,17 +235,18 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_push_long(const uint32_t* __res
 
 // pull, short rows: lane per unfinished vertex v, early exit once every
 // root is accounted for.
-__global__ __launch_bounds__(kMsBlock) void k_ms_pull_short(const uint32_t* __restrict__ off,
-                                                             const uint32_t* __restrict__ adj, uint32_t n,
-                                                             unsigned long long all,
-                                                             const unsigned long long* __restrict__ frontier,
-                                                             const unsigned long long* __restrict__ seen,
-                                                             unsigned long long* __restrict__ next) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    const unsigned long long s = seen[v];
+__global__ __launch_bounds__(kMsBlock) void k_ms_pull_short(const MsArgs* __restrict__ pa, uint32_t l) {
+  const MsArgs a = *pa;
+  if (!level_runs(a, l, 1)) return;
+  const unsigned long long* frontier = level_ptr(a, l);
+  unsigned long long* next = level_ptr(a, l + 1);
+  const unsigned long long all = a.all;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
+    const unsigned long long s = a.seen[v];
     if (s == all) continue;
-    const uint32_t b = off[v], e = off[v + 1];
+    const uint32_t b = a.in_off[v], e = a.in_off[v + 1];
     if (e - b > kMsShortRow) continue;
+    const uint32_t* adj = a.in_adj;
     unsigned long long acc = 0;
     uint32_t i = b;
     for (; i + 4 <= e; i += 4) {
@@ -194,26 +267,24 @@ __device__ __forceinline__ unsigned long long wave_or(unsigned long long x) {
 }
 
 // pull, long rows: wave per chunk; the early-exit test runs every 4 steps.
-__global__ __launch_bounds__(kMsBlock) void k_ms_pull_long(const uint32_t* __restrict__ off,
-                                                            const uint32_t* __restrict__ adj,
-                                                            const uint32_t* __restrict__ row_chunk,
-                                                            const uint32_t* __restrict__ chunk_row, uint32_t nchunks,
-                                                            unsigned long long all,
-                                                            const unsigned long long* __restrict__ frontier,
-                                                            const unsigned long long* __restrict__ seen,
-                                                            unsigned long long* __restrict__ next) {
-  for (uint32_t c = blockIdx.x * kMsWaves + threadIdx.x / kWave; c < nchunks; c += gridDim.x * kMsWaves) {
-    const uint32_t v = chunk_row[c];
-    const unsigned long long s = seen[v];
+__global__ __launch_bounds__(kMsBlock) void k_ms_pull_long(const MsArgs* __restrict__ pa, uint32_t l) {
+  const MsArgs a = *pa;
+  if (!level_runs(a, l, 1)) return;
+  const unsigned long long* frontier = level_ptr(a, l);
+  unsigned long long* next = level_ptr(a, l + 1);
+  const unsigned long long all = a.all;
+  for (uint32_t c = blockIdx.x * kMsWaves + threadIdx.x / kWave; c < a.in_nch; c += gridDim.x * kMsWaves) {
+    const uint32_t v = a.in_cr[c];
+    const unsigned long long s = a.seen[v];
     if (s == all) continue;
-    const uint32_t b = off[v] + (c - row_chunk[v]) * kMsChunk;
-    const uint32_t e = min(b + kMsChunk, off[v + 1]);
+    const uint32_t b = a.in_off[v] + (c - a.in_rc[v]) * kMsChunk;
+    const uint32_t e = min(b + kMsChunk, a.in_off[v + 1]);
     unsigned long long acc = 0;
     for (uint32_t i0 = b; i0 < e; i0 += 4 * kWave) {
 #pragma unroll
       for (uint32_t j = 0; j < 4; ++j) {
         const uint32_t i = i0 + j * kWave + lane_id();
-        if (i < e) acc |= frontier[adj[i]];
+        if (i < e) acc |= frontier[a.in_adj[i]];
       }
       acc = wave_or(acc);
       if ((acc | s) == all) break;
@@ -223,18 +294,21 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_pull_long(const uint32_t* __res
   }
 }
 
-// Fold the level into seen; count new pairs and the next frontier's arcs.
-__global__ __launch_bounds__(kMsBlock) void k_ms_finalize(const uint32_t* __restrict__ out_off, uint32_t n,
-                                                           unsigned long long* __restrict__ seen,
-                                                           const unsigned long long* __restrict__ next,
-                                                           MsCounters* ctr) {
+// Fold level l into seen; count new pairs and the next frontier's arcs;
+// clear the array the next level will fill.
+__global__ __launch_bounds__(kMsBlock) void k_ms_finalize(const MsArgs* __restrict__ pa, uint32_t l) {
+  const MsArgs a = *pa;
+  if (a.ctl->done) return;
+  const unsigned long long* next = level_ptr(a, l + 1);
+  unsigned long long* after = level_ptr(a, l + 2);
   unsigned long long pairs = 0, arcs = 0;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
+    after[v] = 0;
     const unsigned long long nw = next[v];
     if (!nw) continue;
-    seen[v] |= nw;
+    a.seen[v] |= nw;
     pairs += __popcll(nw);
-    arcs += out_off[v + 1] - out_off[v];
+    arcs += a.out_off[v + 1] - a.out_off[v];
   }
   __shared__ unsigned long long sp[kMsWaves], sa[kMsWaves];
   pairs = wave_sum(pairs);
@@ -249,9 +323,32 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_finalize(const uint32_t* __rest
       pairs += sp[w];
       arcs += sa[w];
     }
-    MsStripe& st = ctr->s[blockIdx.x % kMsStripes];
+    MsStripe& st = a.ctl->ctr[l].s[blockIdx.x % kMsStripes];
     if (pairs) atomicAdd(&st.new_pairs, pairs);
     if (arcs) atomicAdd(&st.frontier_arcs, arcs);
+  }
+}
+
+// One wave: sum level l's stripes; finish the pass or set the direction of
+// level l + 1.
+__global__ void k_ms_decide(const MsArgs* __restrict__ pa, uint32_t l) {
+  const MsArgs a = *pa;
+  MsCtl* c = a.ctl;
+  if (c->done) return;
+  const unsigned lane = lane_id();
+  unsigned long long pairs = lane < kMsStripes ? c->ctr[l].s[lane].new_pairs : 0ull;
+  unsigned long long arcs = lane < kMsStripes ? c->ctr[l].s[lane].frontier_arcs : 0ull;
+  pairs = wave_sum(pairs);
+  arcs = wave_sum(arcs);
+  if (lane == 0) {
+    if (c->dir[l] == 0) c->push_levels += 1;
+    if (pairs == 0) {
+      c->done = 1;
+      c->levels = a.base + l + 1;
+    } else {
+      c->levels = a.base + l + 2;
+      c->dir[l + 1] = static_cast<double>(arcs) * a.alpha < a.arcs ? 0u : 1u;
+    }
   }
 }
 
@@ -310,9 +407,17 @@ static unsigned ms_blocks(uint64_t items, unsigned per = kMsBlock, unsigned cap 
   return static_cast<unsigned>(b);
 }
 
+// Grid of a chunk kernel: the next power of two of the waves it needs, so
+// small changes of the chunk count keep the launch program unchanged.
+static unsigned ms_chunk_blocks(uint64_t nchunks) {
+  uint64_t b = 1;
+  while (b * kMsWaves < nchunks && b < 8192) b <<= 1;
+  return static_cast<unsigned>(b);
+}
+
 static void ms_ensure_levels(vgx_msbfs* m, uint64_t need) {
   if (need <= m->levels_cap) return;
-  uint64_t cap = std::max<uint64_t>(need, m->levels_cap ? m->levels_cap * 2 : 16);
+  uint64_t cap = std::max<uint64_t>(need, m->levels_cap ? m->levels_cap * 2 : 32);
   m->newbits.grow_copy(cap * m->n + 1, m->levels_cap * m->n, m->g->stream);
   m->levels_cap = cap;
 }
@@ -346,6 +451,39 @@ static const MsPlan& ms_plan(vgx_msbfs* m, int dir, const CsrSnapshot& sn, hipSt
   return pl;
 }
 
+struct MsGrids {
+  unsigned start, shrt, out_long, in_long, fin;
+};
+
+// The launch program of kMsProgram levels.  Only kernel launches: it is
+// also what gets captured into the hipGraph.
+static void ms_program(const vgx_msbfs* m, const MsGrids& gr, int fresh, uint32_t first_dir, hipStream_t s) {
+  const MsArgs* pa = m->d_args;
+  k_ms_start<<<gr.start, kMsBlock, 0, s>>>(pa, fresh, first_dir);
+  VGX_LAUNCH_CHECK();
+  for (uint32_t l = 0; l < kMsProgram; ++l) {
+    k_ms_push_short<<<gr.shrt, kMsBlock, 0, s>>>(pa, l);
+    VGX_LAUNCH_CHECK();
+    k_ms_push_long<<<gr.out_long, kMsBlock, 0, s>>>(pa, l);
+    VGX_LAUNCH_CHECK();
+    k_ms_pull_short<<<gr.shrt, kMsBlock, 0, s>>>(pa, l);
+    VGX_LAUNCH_CHECK();
+    k_ms_pull_long<<<gr.in_long, kMsBlock, 0, s>>>(pa, l);
+    VGX_LAUNCH_CHECK();
+    k_ms_finalize<<<gr.fin, kMsBlock, 0, s>>>(pa, l);
+    VGX_LAUNCH_CHECK();
+    k_ms_decide<<<1, kWave, 0, s>>>(pa, l);
+    VGX_LAUNCH_CHECK();
+  }
+}
+
+static void ms_drop_graph(vgx_msbfs* m) {
+  if (m->exec) (void)hipGraphExecDestroy(m->exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->exec = nullptr;
+  m->graph = nullptr;
+}
+
 static void ms_run(vgx_msbfs* m) {
   vgx_graph* g = m->g;
   hipStream_t s = g->stream;
@@ -357,61 +495,79 @@ static void ms_run(vgx_msbfs* m) {
   if (n != m->n) {  // the per-level arrays are laid out with stride n
     m->levels_cap = 0;
     m->newbits.release();
+    ms_drop_graph(m);
   }
   m->n = n;
   m->seen.ensure(n + 1);
-  ms_ensure_levels(m, 16);
-  const unsigned long long all = m->k == 64 ? ~0ull : ((1ull << m->k) - 1);
-  DevBuf<uint32_t> droots;
-  droots.ensure(m->k + 1);
-  VGX_HIP(hipMemcpyAsync(droots.p, m->roots.data(), m->k * 4, hipMemcpyHostToDevice, s));
+  m->droots.ensure(m->k + 1);
+  VGX_HIP(hipMemcpyAsync(m->droots.p, m->roots.data(), m->k * 4, hipMemcpyHostToDevice, s));
   m->levels = 0;
   m->push_levels = 0;
-  if (n) {
-    k_ms_init<<<ms_blocks(n), kMsBlock, 0, s>>>(m->seen.p, m->newbits.p, n, droots.p, m->k);
-    VGX_LAUNCH_CHECK();
+  if (n == 0) {
+    m->ran = true;
+    return;
   }
-  m->levels = 1;
-  const uint32_t nn = static_cast<uint32_t>(n);
-  const unsigned bshort = m->blocks ? m->blocks : ms_blocks(n, kMsBlock, 4096);
-  const unsigned bout = ms_blocks(po.nchunks, kMsWaves, 8192);
-  const unsigned bin = ms_blocks(pi.nchunks, kMsWaves, 8192);
-  const double arcs = static_cast<double>(out.arcs);
-  unsigned long long frontier_arcs = 0;  // unknown for the roots: push
-  bool first = true;
-  while (n) {
-    ms_ensure_levels(m, m->levels + 1);
-    const unsigned long long* cur = m->newbits.p + (m->levels - 1) * n;
-    unsigned long long* nxt = m->newbits.p + m->levels * n;
-    VGX_HIP(hipMemsetAsync(nxt, 0, n * 8, s));
-    VGX_HIP(hipMemsetAsync(m->d_ctr, 0, sizeof(MsCounters), s));
-    const bool push = first || static_cast<double>(frontier_arcs) * m->alpha < arcs;
-    if (push) {
-      k_ms_push_short<<<bshort, kMsBlock, 0, s>>>(out.off.p, out.adj.p, nn, cur, m->seen.p, nxt);
-      VGX_LAUNCH_CHECK();
-      if (po.nchunks) {
-        k_ms_push_long<<<bout, kMsBlock, 0, s>>>(out.off.p, out.adj.p, po.row_chunk.p, po.chunk_row.p,
-                                                 static_cast<uint32_t>(po.nchunks), cur, m->seen.p, nxt);
-        VGX_LAUNCH_CHECK();
+  MsGrids gr;
+  gr.start = ms_blocks(n, kMsBlock, 4096);
+  gr.shrt = m->blocks ? m->blocks : ms_blocks(n, kMsBlock, 4096);
+  gr.out_long = ms_chunk_blocks(po.nchunks);
+  gr.in_long = ms_chunk_blocks(pi.nchunks);
+  gr.fin = ms_blocks(n, kMsBlock, 1024);
+  // Programs of kMsProgram levels; the first starts from the roots, the
+  // next ones from the last level of the previous program.
+  uint32_t base = 0;
+  uint32_t first_dir = 0;
+  for (;;) {
+    ms_ensure_levels(m, base + kMsProgram + 2);
+    MsArgs a{};
+    a.out_off = out.off.p;
+    a.out_adj = out.adj.p;
+    a.out_rc = po.row_chunk.p;
+    a.out_cr = po.chunk_row.p;
+    a.out_nch = static_cast<uint32_t>(po.nchunks);
+    a.in_off = in.off.p;
+    a.in_adj = in.adj.p;
+    a.in_rc = pi.row_chunk.p;
+    a.in_cr = pi.chunk_row.p;
+    a.in_nch = static_cast<uint32_t>(pi.nchunks);
+    a.seen = m->seen.p;
+    a.newbits = m->newbits.p;
+    a.roots = m->droots.p;
+    a.n = static_cast<uint32_t>(n);
+    a.k = m->k;
+    a.base = base;
+    a.all = m->k == 64 ? ~0ull : ((1ull << m->k) - 1);
+    a.arcs = static_cast<double>(out.arcs);
+    a.alpha = m->alpha;
+    a.ctl = m->d_ctl;
+    VGX_HIP(hipMemcpyAsync(m->d_args, &a, sizeof(a), hipMemcpyHostToDevice, s));
+    const bool fresh = base == 0;
+    if (m->use_graph && fresh) {
+      const uint64_t key[4] = {n, (static_cast<uint64_t>(gr.shrt) << 32) | gr.fin,
+                               (static_cast<uint64_t>(gr.out_long) << 32) | gr.in_long, gr.start};
+      if (!m->exec || !std::equal(key, key + 4, m->graph_key)) {
+        ms_drop_graph(m);
+        VGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        ms_program(m, gr, 1, 0, s);
+        VGX_HIP(hipStreamEndCapture(s, &m->graph));
+        VGX_HIP(hipGraphInstantiate(&m->exec, m->graph, nullptr, nullptr, 0));
+        std::copy(key, key + 4, m->graph_key);
       }
-      m->push_levels += 1;
+      VGX_HIP(hipGraphLaunch(m->exec, s));
+      m->replays += 1;
     } else {
-      k_ms_pull_short<<<bshort, kMsBlock, 0, s>>>(in.off.p, in.adj.p, nn, all, cur, m->seen.p, nxt);
-      VGX_LAUNCH_CHECK();
-      if (pi.nchunks) {
-        k_ms_pull_long<<<bin, kMsBlock, 0, s>>>(in.off.p, in.adj.p, pi.row_chunk.p, pi.chunk_row.p,
-                                                static_cast<uint32_t>(pi.nchunks), all, cur, m->seen.p, nxt);
-        VGX_LAUNCH_CHECK();
-      }
+      ms_program(m, gr, fresh ? 1 : 0, first_dir, s);
     }
-    k_ms_finalize<<<ms_blocks(n, kMsBlock, 1024), kMsBlock, 0, s>>>(out.off.p, nn, m->seen.p, nxt, m->d_ctr);
-    VGX_LAUNCH_CHECK();
-    VGX_HIP(hipMemcpyAsync(m->h_ctr, m->d_ctr, sizeof(MsCounters), hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipMemcpyAsync(m->h_ctl, m->d_ctl, sizeof(MsCtl), hipMemcpyDeviceToHost, s));
     VGX_HIP(hipStreamSynchronize(s));
-    first = false;
-    if (m->h_ctr->pairs() == 0) break;
-    frontier_arcs = m->h_ctr->arcs();
-    m->levels += 1;
+    m->push_levels += m->h_ctl->push_levels;
+    if (m->h_ctl->done) {
+      m->levels = m->h_ctl->levels;
+      break;
+    }
+    // deeper than one program: continue from the last level written
+    first_dir = m->h_ctl->dir[kMsProgram];
+    base += kMsProgram;
   }
   m->ran = true;
   m->reached.assign(m->k, 0);
@@ -448,8 +604,9 @@ vgx_status vgx_msbfs_create(vgx_graph* g, const uint32_t* roots, uint32_t k, vgx
     m->g = g;
     m->roots.assign(roots, roots + k);
     m->k = k;
-    m->d_ctr = static_cast<MsCounters*>(dev_alloc(sizeof(MsCounters)));
-    VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&m->h_ctr), sizeof(MsCounters), hipHostMallocDefault));
+    m->d_args = static_cast<MsArgs*>(dev_alloc(sizeof(MsArgs)));
+    m->d_ctl = static_cast<MsCtl*>(dev_alloc(sizeof(MsCtl)));
+    VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&m->h_ctl), sizeof(MsCtl), hipHostMallocDefault));
   } catch (...) {
     vgx_msbfs_destroy(m);
     throw;
@@ -462,8 +619,10 @@ vgx_status vgx_msbfs_destroy(vgx_msbfs* m) {
   VGX_API_BEGIN
   if (!m) return VGX_OK;
   if (m->g) (void)hipStreamSynchronize(m->g->stream);
-  if (m->d_ctr) dev_free(m->d_ctr);
-  if (m->h_ctr) (void)hipHostFree(m->h_ctr);
+  ms_drop_graph(m);
+  if (m->d_args) dev_free(m->d_args);
+  if (m->d_ctl) dev_free(m->d_ctl);
+  if (m->h_ctl) (void)hipHostFree(m->h_ctl);
   delete m;
   VGX_API_END
 }
@@ -473,6 +632,22 @@ vgx_status vgx_msbfs_run(vgx_msbfs* m) {
   if (!m) throw Error(VGX_INVALID_ARGUMENT, "null msbfs");
   VGX_HIP(hipSetDevice(m->g->device));
   ms_run(m);
+  VGX_API_END
+}
+
+vgx_status vgx_msbfs_set_graph(vgx_msbfs* m, int on) {
+  VGX_API_BEGIN
+  if (!m) throw Error(VGX_INVALID_ARGUMENT, "null msbfs");
+  m->use_graph = on != 0;
+  if (!m->use_graph) ms_drop_graph(m);
+  VGX_API_END
+}
+
+vgx_status vgx_msbfs_stats(vgx_msbfs* m, uint32_t* push_levels, uint64_t* graph_replays) {
+  VGX_API_BEGIN
+  if (!m) throw Error(VGX_INVALID_ARGUMENT, "null msbfs");
+  if (push_levels) *push_levels = m->push_levels;
+  if (graph_replays) *graph_replays = m->replays;
   VGX_API_END
 }
 
@@ -549,7 +724,7 @@ vgx_status vgx_msbfs_autotune(vgx_msbfs* m, uint32_t reps, uint32_t* best_blocks
   if (reps == 0) reps = 1;
   VGX_HIP(hipSetDevice(m->g->device));
   auto time_run = [&]() {
-    ms_run(m);  // allocations and plans settle
+    ms_run(m);  // allocations, plans and the captured program settle
     const auto t0 = std::chrono::steady_clock::now();
     for (uint32_t r = 0; r < reps; ++r) ms_run(m);
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;

@@ -472,6 +472,20 @@ PYBIND11_MODULE(_velographx, m) {
       .def("reached", [](const velographx::MultiSourceBFS& m) { return to_numpy(m.reached()); })
       .def_property_readonly("level_count", &velographx::MultiSourceBFS::level_count)
       .def(
+          "set_graph",
+          [](velographx::MultiSourceBFS& m, bool on) { velographx::vgx_check(vgx_msbfs_set_graph(m.handle(), on)); },
+          py::arg("on") = true)
+      .def("stats",
+           [](const velographx::MultiSourceBFS& m) {
+             std::uint32_t pl = 0;
+             std::uint64_t rep = 0;
+             velographx::vgx_check(vgx_msbfs_stats(m.handle(), &pl, &rep));
+             py::dict d;
+             d["push_levels"] = pl;
+             d["graph_replays"] = rep;
+             return d;
+           })
+      .def(
           "set_tuning",
           [](velographx::MultiSourceBFS& m, std::uint32_t blocks, double alpha) {
             velographx::vgx_check(vgx_msbfs_set_tuning(m.handle(), blocks, alpha));
