@@ -7,12 +7,14 @@ loop/duplicate removal).
 
 * step  = one direction-optimizing BFS from one of 16 seeded roots in the
           largest component (answer-ready: distances in HBM, stream synced);
+          the warm-up recomputes every root twice, after which a root's
+          repeated traversal replays its captured launch program;
 * value = plain single-root BFS GTEPS over the K timed steps (Graph500:
           m_c / t; every root lies in the same component, so the harmonic
           mean is sum(m_c) / sum(t));
-* roofline: algorithmic bytes of one BFS (4*A_c + 12*V_c) over the summed
-  duration of its level-kernel launches, timed with HIP events on the
-  graph's stream; traffic = HBM bytes per BFS from rocprofv3 FETCH_SIZE and
+* roofline: algorithmic bytes of one BFS (4*A_c + 12*V_c) over its GPU
+  time, measured in the timed region with HIP events on the graph's stream
+  around every traversal (init, setup and level kernels); traffic = HBM bytes per BFS from rocprofv3 FETCH_SIZE and
   WRITE_SIZE passes over a child run, each scaled by its reading for a
   1 GiB copy;
 * cpu_baseline: the reference engine (oracle/_ref/vgx_ref, compiled from the
@@ -72,7 +74,8 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=16)
-    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--warmup", type=int, default=32,
+                   help="untimed recomputes over the same root objects (two per root capture their launch programs)")
     p.add_argument("--scale", type=int, default=22)
     p.add_argument("--edgefactor", type=int, default=16)
     p.add_argument("--roots", type=int, default=16)
@@ -200,14 +203,14 @@ def _counter_rows(directory):
 
 
 def _per_bfs(rows):
-    """Counter sums (KiB) of the BFS kernels of each traversal, split at
-    k_bfs_setup (one per traversal)."""
+    """Counter sums (KiB) of the kernels of each traversal (init, setup,
+    level and tail kernels), split at k_bfs_init (one per traversal)."""
     groups = []
     for r in rows:
         name = r["Kernel_Name"]
-        if "k_bfs_setup" in name:
-            groups.append(0.0)
-        elif groups and ("k_bfs_level" in name or "k_bfs_bottom_up_tail" in name):
+        if "k_bfs_init" in name:
+            groups.append(float(r["Counter_Value"]))
+        elif groups and any(k in name for k in ("k_bfs_setup", "k_bfs_level", "k_bfs_bottom_up_tail")):
             groups[-1] += float(r["Counter_Value"])
     return groups
 
@@ -273,7 +276,7 @@ def pmc_child(args):
     lcc, _ = component(vx, wl, g)
     roots = wl.roots_in_component(lcc, args.roots, seed=1)
     objs = [vx.IncrementalBFS(g, int(r)) for r in roots]
-    for k in range(args.steps):
+    for k in range(2 * len(objs) + args.steps):
         objs[k % len(objs)].recompute()
     torch.cuda.synchronize()
 
@@ -281,14 +284,21 @@ def pmc_child(args):
 # ----------------------------------------------------------------- GPU legs
 
 
-def timed_bfs(vx, g, roots, steps):
-    """K answer-ready BFS recomputes over the given roots, with level-kernel
-    HIP-event profiling.  Returns (seconds per step list, elapsed, objects)."""
+def timed_bfs(vx, g, roots, steps, warmup=0, objs=None):
+    """K answer-ready BFS recomputes over the given roots after `warmup`
+    untimed ones on the same objects, with level-kernel HIP-event profiling
+    in a separate pass.  Returns (seconds per step list, elapsed, objects,
+    kernel ms per BFS, launches per BFS)."""
     import torch
 
-    objs = [vx.IncrementalBFS(g, int(r)) for r in roots]
+    if objs is None:
+        objs = [vx.IncrementalBFS(g, int(r)) for r in roots]
     for o in objs:
-        o.set_profiling(True)
+        o.set_profiling(2)  # GPU time of every whole traversal (replay-compatible)
+    for k in range(warmup):
+        objs[k % len(objs)].recompute()
+    for o in objs:
+        o.set_profiling(2)  # reset the counters: only the timed traversals count
     torch.cuda.synchronize()
     times = []
     t_start = time.perf_counter()
@@ -298,7 +308,21 @@ def timed_bfs(vx, g, roots, steps):
         o.recompute()
         times.append(time.perf_counter() - t0)
     torch.cuda.synchronize()
-    return times, time.perf_counter() - t_start, objs
+    elapsed = time.perf_counter() - t_start
+    # GPU time per BFS over the timed region: HIP events on the graph's
+    # stream around each traversal (its init, setup and level kernels)
+    kernel_ms = sum(o.stats()["traversal_ms"] for o in objs) / steps
+    # launches per BFS: one more pass with events around every level launch
+    for o in objs:
+        o.set_profiling(1)
+    for k in range(len(objs)):
+        objs[k].recompute()
+    torch.cuda.synchronize()
+    launches = sum(o.stats()["level_launches"] for o in objs) / len(objs)
+    level_ms = sum(o.stats()["level_kernel_ms"] for o in objs) / len(objs)
+    for o in objs:
+        o.set_profiling(0)
+    return times, elapsed, objs, kernel_ms, launches, level_ms
 
 
 def component(vx, wl, g):
@@ -448,8 +472,7 @@ def leg_contrast(vx, wl, steps):
     lcc, arcs_c = component(vx, wl, g)
     roots = wl.roots_in_component(lcc, 16, seed=1)
     log(f"ER n={n}: {g.edge_count_directed} arcs, LCC {lcc.size}, built in {time.time() - t0:.1f}s")
-    times, elapsed, objs = timed_bfs(vx, g, roots, steps)
-    kernel_ms = sum(o.stats()["level_kernel_ms"] for o in objs) / steps
+    times, elapsed, objs, kernel_ms, _, _ = timed_bfs(vx, g, roots, steps, 2 * len(roots))
     bytes_bfs = 4.0 * arcs_c + 12.0 * lcc.size
     out = {"workload": "Erdos-Renyi n=4194304, 33554432 uniform pairs (mean degree 16), seed 1",
            "arcs": int(g.edge_count_directed), "lcc_vertices": int(lcc.size),
@@ -631,20 +654,17 @@ def main():
     bytes_bfs = 4.0 * arcs_c + 12.0 * v_c
     log(f"LCC: {v_c} vertices, {arcs_c} arcs; bytes/BFS {bytes_bfs / 1e6:.1f} MB")
 
-    for i in range(args.warmup):
-        vx.IncrementalBFS(g, int(roots[i % len(roots)]))
-    times, elapsed, objs = timed_bfs(vx, g, roots, args.steps)
-    launches = sum(o.stats()["level_launches"] for o in objs)
-    kernel_ms = sum(o.stats()["level_kernel_ms"] for o in objs)
+    times, elapsed, objs, kernel_ms_per_bfs, launches_per_bfs, level_ms_eager = timed_bfs(
+        vx, g, roots, args.steps, args.warmup)
     st0 = objs[0].stats()
     reached = {o.stats()["reachable_count"] for o in objs}
     assert reached == {v_c}, reached
     gteps = m_c * args.steps / sum(times) / 1e9
     ms_per_step = 1e3 * elapsed / args.steps
-    kernel_ms_per_bfs = kernel_ms / args.steps
     achieved = bytes_bfs / (kernel_ms_per_bfs * 1e-3) / 1e9
     log(f"BFS: {ms_per_step:.3f} ms/step, {gteps:.1f} GTEPS, kernel {kernel_ms_per_bfs:.3f} ms/BFS "
-        f"over {launches / args.steps:.1f} launches, levels {st0['levels']} (bottom-up {st0['bottom_up_levels']})")
+        f"over {launches_per_bfs:.1f} launches, levels {st0['levels']} (bottom-up {st0['bottom_up_levels']}), "
+        f"graph replays {st0['graph_replays']}")
     # exactness spot check: the direction-optimizing result equals a
     # top-down-only traversal of the same root
     chk = vx.IncrementalBFS(g, int(roots[0]))
@@ -683,8 +703,9 @@ def main():
             "traffic": None,
             "kernel": "k_bfs_level + k_bfs_bottom_up_tail",
             "bytes_per_bfs": bytes_bfs,
-            "avg_launch_us": 1e3 * kernel_ms / max(launches, 1),
-            "launches_per_bfs": launches / args.steps,
+            "gpu_ms_per_bfs": kernel_ms_per_bfs,
+            "avg_launch_us": 1e3 * level_ms_eager / max(launches_per_bfs, 1),
+            "launches_per_bfs": launches_per_bfs,
             "wall_frac": bytes_bfs / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
         },
         "bfs": {"levels": st0["levels"], "bottom_up_levels": st0["bottom_up_levels"],

@@ -142,10 +142,16 @@ typedef struct {
    * (HIP events on the graph's stream) since the last reset. */
   uint64_t level_launches;
   double level_kernel_ms;
+  uint64_t graph_replays;  /* traversals replayed from a captured hipGraph */
+  double traversal_ms;     /* profiling mode 2: GPU time of the traversals since enabled */
+  uint64_t traversals;
 } vgx_bfs_stats;
 vgx_status vgx_bfs_stats_get(vgx_bfs* b, vgx_bfs_stats* out);
 /* Time every level-kernel launch with HIP events (on = 1) and reset the
  * accumulated launch count and duration. */
+/* 0: off; 1: HIP events around every level launch (level_kernel_ms; runs
+ * eagerly); 2: events around every whole traversal (traversal_ms; keeps the
+ * hipGraph replay). */
 vgx_status vgx_bfs_set_profiling(vgx_bfs* b, int on);
 /* Per-level record of the last full traversal (diagnostics; no reference
  * counterpart): direction, frontier vertices and out-edges, bottom-up rows

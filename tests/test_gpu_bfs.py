@@ -311,3 +311,32 @@ def test_level_shape_changes_between_traversals(vx, orc):
         for _ in range(2):
             bfs.recompute()
             assert np.array_equal(bfs.distances_numpy(), full)
+
+
+def test_replayed_traversals_stay_exact(vx, orc):
+    """A traversal that repeats the previous launch program is replayed from
+    a captured hipGraph; replays stay exact, also after deletions change the
+    graph under an unchanged program."""
+    from velographx_amd import workloads as wl
+
+    s, d = vx.rmat_edges(14, 16, 9)
+    g = vx.Graph(1 << 14, False)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(1 << 14, False)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    bfs = vx.IncrementalBFS(g, 3)
+    full, _ = orc.model_bfs(mg, 3)
+    for _ in range(4):
+        bfs.recompute()
+        assert np.array_equal(bfs.distances_numpy(), full)
+    assert bfs.stats()["graph_replays"] >= 1
+    eu, ev = wl.canonical_edges(s, d, False)
+    stream = wl.BatchStream(eu, ev, 1 << 14, False, seed=11)
+    for _ in range(3):
+        bs, bd, ba = stream.next(500, 0.0)  # deletions only
+        g.apply_arrays(bs, bd, ba)
+        mg.apply(bs, bd, ba)
+        full, _ = orc.model_bfs(mg, 3)
+        for _ in range(3):
+            bfs.recompute()
+            assert np.array_equal(bfs.distances_numpy(), full)

@@ -158,7 +158,7 @@ struct vgx_bfs {
   vgx::IncState* d_inc = nullptr;
   vgx::IncState* h_inc = nullptr;
   uint32_t epoch = 0;
-  bool profiling = false;
+  int profiling = 0;  // 1: events around every level, 2: around every traversal
   std::vector<hipEvent_t> events;
   uint64_t level_launches = 0;
   double level_ms = 0.0;
@@ -167,6 +167,16 @@ struct vgx_bfs {
   uint32_t prof_levels = 0;
   uint32_t prof_mode[64] = {};
   uint64_t prof_mf[64] = {};
+  // replay of a repeated traversal's launch program
+  bool use_graph = true;
+  std::vector<uint64_t> gkey;
+  hipGraph_t ggraph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  uint64_t graph_replays = 0;
+  // whole-traversal GPU time (profiling mode 2): events around each traversal
+  hipEvent_t tev[2] = {nullptr, nullptr};
+  double traversal_ms = 0.0;
+  uint64_t traversals = 0;
 };
 
 namespace vgx {
@@ -912,6 +922,29 @@ static LevelArgs level_args(vgx_bfs* b) {
   return a;
 }
 
+// Start of a traversal in one launch: all distances unreachable, visited =
+// the zero-in-degree bitmap, the three frontier bitmaps clear.
+__global__ void k_bfs_init(uint32_t* __restrict__ dist, uint64_t n, unsigned long long* __restrict__ visited,
+                           const unsigned long long* __restrict__ zin, uint64_t words,
+                           unsigned long long* __restrict__ bm) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+    dist[i] = kInf;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < words; i += stride) {
+    visited[i] = zin[i];
+    bm[i] = 0;
+    bm[words + i] = 0;
+    bm[2 * words + i] = 0;
+  }
+}
+
+static void drop_bfs_graph(vgx_bfs* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->ggraph) (void)hipGraphDestroy(b->ggraph);
+  b->gexec = nullptr;
+  b->ggraph = nullptr;
+}
+
 static void bfs_full(vgx_bfs* b) {
   vgx_graph* g = b->g;
   ensure_work(b);
@@ -935,27 +968,20 @@ static void bfs_full(vgx_bfs* b) {
     b->w.head_version = g->version;
     b->w.head_n = n;
   }
-  if (n) {
-    VGX_HIP(hipMemsetAsync(b->w.dist.p, 0xFF, n * sizeof(uint32_t), s));
-    VGX_HIP(hipMemcpyAsync(b->w.visited.p, b->w.zin.p, words * 8, hipMemcpyDeviceToDevice, s));
-    VGX_HIP(hipMemsetAsync(b->w.bm.p, 0, 3 * words * 8, s));
-  }
   LevelArgs a = level_args(b);
-  k_bfs_setup<<<1, 1, 0, s>>>(a, b->source, g->adj[0].stored, static_cast<float>(b->tune.alpha),
-                              static_cast<float>(b->tune.beta));
-  VGX_LAUNCH_CHECK();
   unsigned blocks = b->tune.blocks;
   if (blocks == 0) {
     // about one resident round of workgroups: more waves only add flushes
-    // on the level counters (measured at R-MAT scale 22: 1024 beats 2048)
+    // on the level counters (measured at R-MAT scale 22)
     uint64_t waves = std::max<uint64_t>(words, g->adj[0].stored / 512);
     blocks = static_cast<unsigned>(std::min<uint64_t>(1536, std::max<uint64_t>(1, (waves + 3) / 4)));
   }
   // With automatic sizing, a level that ran top-down from a small queue in
   // the previous traversal of this object gets a grid sized to its edges,
-  // and the tail kernel a token grid where that level was not bottom-up:
-  // dispatching ~1500 idle workgroups costs several microseconds per launch.
-  // Every kernel is grid-stride, so a wrong guess costs time, never results.
+  // and no tail kernel where that level was not bottom-up (the level kernel
+  // then finishes bottom-up rows inline): dispatching ~1500 idle workgroups
+  // costs microseconds per launch.  Every kernel is grid-stride, so a wrong
+  // guess costs time, never results.
   const bool adaptive = b->tune.blocks == 0 && b->prof_levels > 0;
   // tb == 0: no tail kernel, the level kernel finishes bottom-up rows inline.
   auto grids = [&](uint32_t L, unsigned& lb, unsigned& tb) {
@@ -979,29 +1005,100 @@ static void bfs_full(vgx_bfs* b) {
   // beyond it detects completion); later chunks shrink.
   uint32_t chunk = std::max<uint32_t>(1, b->tune.levels_per_sync);
   if (b->levels) chunk = std::max<uint32_t>(chunk, b->levels + 1);
-  uint32_t launched = 0;
-  for (;;) {
-    if (b->profiling && b->events.size() < 2 * static_cast<size_t>(chunk)) {
+  const unsigned long long total_edges = g->adj[0].stored;
+  const float alpha = static_cast<float>(b->tune.alpha), beta = static_cast<float>(b->tune.beta);
+  const unsigned ginit = blocks_for(std::max<uint64_t>(n, 1), kBfsBlock, 2048);
+  auto launch_levels = [&](uint32_t first, uint32_t count, bool events) {
+    if (events && b->events.size() < 2 * static_cast<size_t>(count)) {
       size_t old = b->events.size();
-      b->events.resize(2 * static_cast<size_t>(chunk));
+      b->events.resize(2 * static_cast<size_t>(count));
       for (size_t i = old; i < b->events.size(); ++i) VGX_HIP(hipEventCreate(&b->events[i]));
     }
-    for (uint32_t i = 0; i < chunk; ++i) {
-      if (b->profiling) VGX_HIP(hipEventRecord(b->events[2 * i], s));
+    for (uint32_t i = 0; i < count; ++i) {
+      if (events) VGX_HIP(hipEventRecord(b->events[2 * i], s));
       unsigned lb, tb;
-      grids(launched + i, lb, tb);
-      k_bfs_level<<<lb, kBfsBlock, 0, s>>>(a, launched + i, tb == 0 ? 1 : 0);
+      grids(first + i, lb, tb);
+      k_bfs_level<<<lb, kBfsBlock, 0, s>>>(a, first + i, tb == 0 ? 1 : 0);
       VGX_LAUNCH_CHECK();
       if (tb) {
-        k_bfs_bottom_up_tail<<<tb, kBfsBlock, 0, s>>>(a, launched + i);
+        k_bfs_bottom_up_tail<<<tb, kBfsBlock, 0, s>>>(a, first + i);
         VGX_LAUNCH_CHECK();
       }
-      if (b->profiling) VGX_HIP(hipEventRecord(b->events[2 * i + 1], s));
+      if (events) VGX_HIP(hipEventRecord(b->events[2 * i + 1], s));
+    }
+  };
+  auto launch_start = [&]() {
+    if (n) {
+      k_bfs_init<<<ginit, kBfsBlock, 0, s>>>(b->w.dist.p, n, b->w.visited.p, b->w.zin.p, words, b->w.bm.p);
+      VGX_LAUNCH_CHECK();
+    }
+    k_bfs_setup<<<1, 1, 0, s>>>(a, b->source, total_edges, alpha, beta);
+    VGX_LAUNCH_CHECK();
+  };
+  // A traversal that repeats the previous one's launch program exactly
+  // (same buffers, root, graph size and level shape) replays it from a
+  // hipGraph captured on the second occurrence; anything else runs eagerly.
+  const bool per_level = b->profiling == 1;
+  const bool whole = b->profiling == 2;
+  if (whole && !b->tev[0]) {
+    VGX_HIP(hipEventCreate(&b->tev[0]));
+    VGX_HIP(hipEventCreate(&b->tev[1]));
+  }
+  if (whole) VGX_HIP(hipEventRecord(b->tev[0], s));
+  bool replayed = false;
+  if (b->use_graph && !per_level) {
+    std::vector<uint64_t> key;
+    key.reserve(sizeof(LevelArgs) / 8 + 8 + 2 * chunk);
+    const uint64_t* ab = reinterpret_cast<const uint64_t*>(&a);
+    static_assert(sizeof(LevelArgs) % 8 == 0, "LevelArgs is hashed as 8-byte words");
+    key.assign(ab, ab + sizeof(LevelArgs) / 8);
+    key.push_back(n);
+    key.push_back(total_edges);
+    key.push_back(b->source);
+    key.push_back(chunk);
+    key.push_back((static_cast<uint64_t>(__builtin_bit_cast(uint32_t, alpha)) << 32) |
+                  __builtin_bit_cast(uint32_t, beta));
+    key.push_back(reinterpret_cast<uint64_t>(b->w.dist.p));
+    key.push_back(reinterpret_cast<uint64_t>(b->w.zin.p));
+    for (uint32_t i = 0; i < chunk; ++i) {
+      unsigned lb, tb;
+      grids(i, lb, tb);
+      key.push_back((static_cast<uint64_t>(lb) << 32) | tb);
+    }
+    if (key == b->gkey) {
+      if (!b->gexec) {
+        VGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        launch_start();
+        launch_levels(0, chunk, false);
+        VGX_HIP(hipStreamEndCapture(s, &b->ggraph));
+        VGX_HIP(hipGraphInstantiate(&b->gexec, b->ggraph, nullptr, nullptr, 0));
+      }
+      VGX_HIP(hipGraphLaunch(b->gexec, s));
+      b->graph_replays += 1;
+      replayed = true;
+    } else {
+      drop_bfs_graph(b);
+      b->gkey = std::move(key);
+    }
+  }
+  uint32_t launched = 0;
+  bool first = true;
+  for (;;) {
+    if (!(first && replayed)) {
+      if (first) launch_start();
+      launch_levels(launched, chunk, per_level);
     }
     launched += chunk;
+    if (whole) VGX_HIP(hipEventRecord(b->tev[1], s));
     VGX_HIP(hipMemcpyAsync(b->h_st, b->d_st, sizeof(BfsState), hipMemcpyDeviceToHost, s));
     VGX_HIP(hipStreamSynchronize(s));
-    if (b->profiling) {
+    if (whole) {
+      float ms = 0.f;
+      VGX_HIP(hipEventElapsedTime(&ms, b->tev[0], b->tev[1]));
+      b->traversal_ms += ms;  // over all chunks of this traversal
+      VGX_HIP(hipEventRecord(b->tev[0], s));
+    }
+    if (per_level && !(first && replayed)) {
       for (uint32_t i = 0; i < chunk; ++i) {
         float ms = 0.f;
         VGX_HIP(hipEventElapsedTime(&ms, b->events[2 * i], b->events[2 * i + 1]));
@@ -1010,10 +1107,12 @@ static void bfs_full(vgx_bfs* b) {
       }
       b->level_launches += chunk;
     }
+    first = false;
     if (b->h_st->done_level != ~0u) break;
     if (launched > n + 2) throw Error(VGX_DEVICE_ERROR, "BFS did not terminate");
     chunk = std::max<uint32_t>(2, chunk / 2);
   }
+  if (whole) b->traversals += 1;
   b->reachable = b->h_st->reached;
   b->levels = b->h_st->done_level;
   b->bu_levels = b->h_st->bu_levels;
@@ -1326,6 +1425,9 @@ vgx_status vgx_bfs_destroy(vgx_bfs* b) {
   if (b->h_st) (void)hipHostFree(b->h_st);
   if (b->h_inc) (void)hipHostFree(b->h_inc);
   for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+  drop_bfs_graph(b);
+  for (hipEvent_t e : b->tev)
+    if (e) (void)hipEventDestroy(e);
   delete b;
   VGX_API_END
 }
@@ -1379,15 +1481,21 @@ vgx_status vgx_bfs_stats_get(vgx_bfs* b, vgx_bfs_stats* out) {
   out->last_relaxed_vertices = b->last_relaxed;
   out->level_launches = b->level_launches;
   out->level_kernel_ms = b->level_ms;
+  out->graph_replays = b->graph_replays;
+  out->traversal_ms = b->traversal_ms;
+  out->traversals = b->traversals;
   VGX_API_END
 }
 
 vgx_status vgx_bfs_set_profiling(vgx_bfs* b, int on) {
   VGX_API_BEGIN
   if (!b) throw Error(VGX_INVALID_ARGUMENT, "null bfs");
-  b->profiling = on != 0;
+  if (on < 0 || on > 2) throw Error(VGX_INVALID_ARGUMENT, "profiling mode is 0, 1 or 2");
+  b->profiling = on;
   b->level_launches = 0;
   b->level_ms = 0.0;
+  b->traversal_ms = 0.0;
+  b->traversals = 0;
   VGX_API_END
 }
 
@@ -1412,8 +1520,8 @@ vgx_status vgx_bfs_autotune(vgx_bfs* b, uint32_t reps, vgx_bfs_tuning* best, dou
   if (!b) throw Error(VGX_INVALID_ARGUMENT, "null bfs");
   if (reps == 0) reps = 1;
   VGX_HIP(hipSetDevice(b->g->device));
-  const bool prof = b->profiling;
-  b->profiling = false;
+  const int prof = b->profiling;
+  b->profiling = 0;
   auto time_of = [&](const vgx_bfs_tuning& t) {
     b->tune = t;
     bfs_full(b);  // settles the chunk length for this setting

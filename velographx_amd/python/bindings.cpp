@@ -308,11 +308,14 @@ PYBIND11_MODULE(_velographx, m) {
              d["last_relaxed_vertices"] = s.last_relaxed_vertices;
              d["level_launches"] = s.level_launches;
              d["level_kernel_ms"] = s.level_kernel_ms;
+             d["graph_replays"] = s.graph_replays;
+             d["traversal_ms"] = s.traversal_ms;
+             d["traversals"] = s.traversals;
              return d;
            })
       .def("set_profiling",
-           [](IncrementalBFS& b, bool on) { velographx::vgx_check(vgx_bfs_set_profiling(b.handle(), on ? 1 : 0)); },
-           py::arg("on") = true)
+           [](IncrementalBFS& b, int mode) { velographx::vgx_check(vgx_bfs_set_profiling(b.handle(), mode)); },
+           py::arg("mode") = 1)
       .def("set_tuning",
            [](IncrementalBFS& b, double alpha, double beta, std::uint32_t blocks, std::uint32_t levels_per_sync) {
              vgx_bfs_tuning t{alpha, beta, blocks, levels_per_sync};
