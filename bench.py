@@ -26,9 +26,11 @@ Secondary legs, each in its own object of the same JSON line:
                GTEPS, batched / plain ratio, bit-exact check of the first 16
                roots against the plain path);
 * repairs   -- 100 update batches of 10,000 ops (80% inserts of random
-               non-edges, 20% deletions of live edges) repaired to exact
-               distances, consolidation every 20 batches; repairs/s, with
-               the reference engine's IncrementalBFS::apply timed on a
+               non-edges, 20% deletions of live edges) answered with exact
+               distances, the adaptive selector choosing localized repair or
+               full recompute per batch, consolidation every 20 batches;
+               repairs/s, next to the always-repair and always-recompute arms
+               and the reference engine's IncrementalBFS::apply timed on a
                bounded sample of the same batches;
 * contrast  -- Erdos-Renyi, 4,194,304 vertices, mean degree 16;
 * batching  -- R-MAT scale 20: 64 roots per hipGraph-captured multi-source
@@ -430,35 +432,73 @@ def leg_batching(vx, wl, args):
     return out
 
 
-def leg_repairs(vx, wl, g, s, d, n, root, args):
+def _run_repairs(vx, g, root, batches, args, policy):
+    """One pass over the update stream with the given arm policy:
+    "adaptive" (the selector picks repair or recompute per batch, the
+    reference harness's set-up with the in-object fallback disabled),
+    "incremental" (always the localized repair, in-object fallback 0.35) or
+    "full" (graph apply + full recompute).  Consolidation every
+    `consolidate_every` batches inside the timer; verification against a
+    fresh BFS every 10 batches outside it."""
+    import collections
+
     import torch
 
-    t0 = time.time()
-    eu, ev = wl.canonical_edges(s, d, False)
-    stream = wl.BatchStream(eu, ev, n, False, seed=1)
-    batches = [stream.next(args.batch_size, 0.8) for _ in range(args.batches)]
-    log(f"generated {len(batches)} batches in {time.time() - t0:.1f}s")
-    inc = vx.IncrementalBFS(g, int(root))
-    batch_t = []
+    inc = vx.IncrementalBFS(g, int(root), 2.0 if policy == "adaptive" else 0.35)
+    sel = vx.AdaptiveSelector(inc) if policy == "adaptive" else None
+    batch_t, reasons = [], collections.Counter()
     full = 0
     exact = True
     for i, (bs, bd, ba) in enumerate(batches):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        inc.apply_arrays(bs, bd, ba)
+        if policy == "adaptive":
+            tr = sel.apply_arrays(bs, bd, ba)
+        elif policy == "incremental":
+            inc.apply_arrays(bs, bd, ba)
+        else:
+            g.apply_arrays(bs, bd, ba)
+            inc.recompute()
         if args.consolidate_every and (i + 1) % args.consolidate_every == 0:
             g.compact()
         batch_t.append(time.perf_counter() - t0)
-        full += int(inc.last_used_full_recompute)
-        if i % 10 == 9 or i == len(batches) - 1:  # verification outside the timer
+        if policy == "adaptive":
+            reasons[tr["reason"]] += 1
+            full += int(tr["chose_full"] or tr["internal_fallback"])
+        elif policy == "incremental":
+            full += int(inc.last_used_full_recompute)
+        else:
+            full += 1
+        if i % 10 == 9 or i == len(batches) - 1:
             ver = vx.IncrementalBFS(g, int(root))
-            if not np.array_equal(ver.distances_numpy(), inc.distances_numpy()):
-                exact = False
+            exact &= bool(np.array_equal(ver.distances_numpy(), inc.distances_numpy()))
             del ver
     tot = sum(batch_t)
     out = {"repairs_per_s": len(batch_t) / tot, "update_ops_per_s": len(batch_t) * args.batch_size / tot,
            "mean_batch_ms": 1e3 * tot / len(batch_t), "p50_batch_ms": 1e3 * float(np.median(batch_t)),
-           "full_recompute_batches": full, "exact": exact, "consolidate_every": args.consolidate_every}
+           "full_recompute_batches": full, "exact": exact}
+    if reasons:
+        out["decisions"] = dict(reasons)
+    return out
+
+
+def leg_repairs(vx, wl, g, s, d, n, root, args):
+    t0 = time.time()
+    eu, ev = wl.canonical_edges(s, d, False)
+    stream = wl.BatchStream(eu, ev, n, False, seed=1)
+    batches = [stream.next(args.batch_size, 0.8) for _ in range(args.batches)]
+    log(f"generated {len(batches)} batches in {time.time() - t0:.1f}s")
+    out = _run_repairs(vx, g, root, batches, args, "adaptive")
+    out["policy"] = "adaptive selector (publication-preflight-v1), consolidation every %d" % args.consolidate_every
+    out["consolidate_every"] = args.consolidate_every
+    # the two fixed arms on fresh copies of the same graph, for comparison
+    for policy in ("incremental", "full"):
+        g2 = vx.Graph(n, False)
+        g2.bulk_load_arrays(s, d)
+        r = _run_repairs(vx, g2, root, batches, args, policy)
+        out[f"always_{policy}"] = {k: r[k] for k in ("repairs_per_s", "mean_batch_ms", "full_recompute_batches",
+                                                     "exact")}
+        del g2
     log(f"repairs: {out}")
     return out, batches
 

@@ -128,6 +128,42 @@ def test_row_relocation_and_big_rows(vx, orc):
     assert st["relocated_rows"] > 0
 
 
+@pytest.mark.parametrize("directed", [False, True])
+def test_insert_paths_mid_relocation_and_long_rows(vx, orc, directed):
+    """Rows of every insert path: lane merges (<= 32 entries), in-place
+    workgroup merges of longer rows with slack (tiles shifted from the right,
+    with deletions present as tombstones), relocations of rows that outgrow
+    their slots, of rows above the in-place limit (20,000 entries) and of
+    rows taking more than 1,024 inserts at once; adjacency equals the model
+    after every batch."""
+    n = 60000
+    rng = np.random.default_rng(5)
+    base = [(0, v) for v in range(2, 40000, 2)]          # 19,999 entries: above the in-place limit
+    base += [(1, v) for v in range(10, 3000, 3)]          # ~1,000 entries: in place while slack lasts
+    base += [(2, v) for v in range(100, 140)]             # 40 entries
+    base += [(u, u + 1) for u in range(3, 50000, 7)]      # short rows
+    g = vx.Graph(n, directed)
+    m = orc.ModelGraph(n, directed)
+    g.bulk_load_edges(np.array(base, dtype=np.uint32))
+    m.bulk_load_edges(base)
+    check_graph(g, m)
+    plans = [
+        [(0, 1, 1), (0, 39999, 1), (1, 11, 1), (2, 99, 1), (2, 141, 1)],
+        [(1, int(v), 1) for v in rng.integers(3000, 60000, size=60)] + [(1, 13, 0), (1, 16, 0)],
+        [(1, int(v), 1) for v in rng.integers(3000, 60000, size=400)],      # outgrows: relocation
+        [(2, int(v), 1) for v in rng.integers(0, 60000, size=3000)],        # > 1,024 inserts in one row
+        [(0, int(v), 1) for v in rng.integers(40000, 60000, size=50)] + [(0, 4, 0)],
+        [(int(u), int(v), 1) for u, v in rng.integers(0, 60000, size=(5000, 2))],
+    ]
+    for ops in plans:
+        s_, d_, a_ = ops_arrays(ops)
+        g.apply_arrays(s_, d_, a_)
+        m.apply(s_, d_, a_)
+        check_graph(g, m)
+    full, _ = orc.model_bfs(m, 0)
+    assert np.array_equal(vx.IncrementalBFS(g, 0).distances_numpy(), full)
+
+
 # ------------------------------------------------------------ BFS
 
 

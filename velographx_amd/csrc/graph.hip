@@ -69,7 +69,7 @@ void HostBuf::ensure(size_t b) {
 }
 
 static inline uint32_t slack_for(uint32_t len) { return len == 0 ? 0u : (len >> 3) + 2u; }
-static inline uint32_t relocated_cap(uint32_t need) { return need + (need >> 2) + 4u; }
+__host__ __device__ static inline uint32_t relocated_cap(uint32_t need) { return need + (need >> 2) + 4u; }
 
 static int bits_for(uint64_t n) {
   int b = 1;
@@ -78,8 +78,14 @@ static int bits_for(uint64_t n) {
 }
 
 // A row takes its inserts in place on one thread while it stays this short
-// and fits its slack; otherwise a workgroup merges it into a fresh range.
+// and fits its slack; a longer row that fits is merged in place by one
+// workgroup (right-to-left tiles) up to kInPlaceMax entries; rows that
+// outgrow their slots or are longer are rewritten at the bump pointer by as
+// many workgroups as they have kRelocChunk-entry chunks.
 constexpr uint32_t kBigRow = 32;
+constexpr uint32_t kInPlaceMax = 16384;
+constexpr uint32_t kMidInsMax = 1024;  // most inserts one in-place merge takes
+constexpr uint32_t kRelocChunk = 2048;
 constexpr int kBlock = 256;
 
 static unsigned grid_for(uint64_t items, unsigned per_block = kBlock, unsigned cap = 65535u * 4u) {
@@ -203,7 +209,7 @@ __global__ void k_resolve(const unsigned long long* __restrict__ skeys, const ui
 __global__ void k_plan_inserts(const uint32_t* __restrict__ ins, const unsigned long long* __restrict__ nins_p,
                                const unsigned long long* __restrict__ skeys, const uint2* __restrict__ row,
                                const uint32_t* __restrict__ cap, uint32_t* __restrict__ segk,
-                               uint32_t* __restrict__ big, GraphCounters* ctr, int dir) {
+                               uint32_t* __restrict__ big, uint32_t* __restrict__ mid, GraphCounters* ctr, int dir) {
   const uint64_t nins = *nins_p;
   for (uint64_t j = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; j < nins;
        j += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
@@ -218,14 +224,18 @@ __global__ void k_plan_inserts(const uint32_t* __restrict__ ins, const unsigned 
     segk[j] = k;
     uint32_t len = row[u].y, c = cap[u];
     uint32_t need = len + k;
-    bool is_big = need > kBigRow || need > c;
-    if (need > c || is_big) {
-      uint32_t nc = need + (need >> 2) + 4u;
+    // a row that outgrows its slots, or is too long to shift in place by one
+    // workgroup, is rewritten at the bump pointer by many workgroups; a row
+    // of more than kBigRow entries that fits is merged in place
+    const bool reloc = need > c || len > kInPlaceMax || (need > kBigRow && k > kMidInsMax);
+    if (reloc) {
+      uint32_t nc = relocated_cap(need);
       atomicAdd(&ctr->need[dir], static_cast<unsigned long long>(nc));
-    }
-    if (is_big) {
       unsigned long long slot = atomicAdd(&ctr->nbig[dir], 1ull);
       big[slot] = static_cast<uint32_t>(j);
+    } else if (need > kBigRow) {
+      unsigned long long slot = atomicAdd(&ctr->nmid[dir], 1ull);
+      mid[slot] = static_cast<uint32_t>(j);
     }
   }
 }
@@ -305,54 +315,211 @@ __global__ void k_insert_small(const uint32_t* __restrict__ ins, const unsigned 
 
 // One workgroup per large row: merge-path merge of the old row and its
 // sorted inserts into a fresh arena range.
-__global__ void k_insert_big(const uint32_t* __restrict__ big, const uint32_t* __restrict__ ins,
-                             const uint32_t* __restrict__ segk, const unsigned long long* __restrict__ skeys,
-                             uint2* __restrict__ row, uint32_t* __restrict__ cap, uint32_t* __restrict__ arena,
-                             uint32_t* __restrict__ deg, GraphCounters* ctr, int dir) {
-  __shared__ uint32_t s_nb;
-  const uint32_t j = big[blockIdx.x];
+// Rows of kBigRow < len + k <= cap, len <= kInPlaceMax, k <= kMidMaxIns:
+// one workgroup finds every insert's final place in the old row first, then
+// shifts the old entries right in tiles taken from the right end (a tile is
+// read whole before it is written, and its writes only reach its own
+// positions or positions the tile to its right has vacated), then writes
+// the inserted entries into the gaps.
+constexpr uint32_t kMidTile = 4 * 256;
+constexpr uint32_t kMidMaxIns = kMidInsMax;
+__global__ __launch_bounds__(256) void k_insert_mid(const uint32_t* __restrict__ mid,
+                                                    const uint32_t* __restrict__ ins,
+                                                    const uint32_t* __restrict__ segk,
+                                                    const unsigned long long* __restrict__ skeys,
+                                                    uint2* __restrict__ row, uint32_t* __restrict__ arena,
+                                                    uint32_t* __restrict__ deg, GraphCounters* ctr, int dir) {
+  __shared__ uint32_t s_pos[kMidMaxIns];
+  __shared__ uint32_t s_ins[kMidMaxIns];
+  const uint32_t j = mid[blockIdx.x];
   const uint32_t k = segk[j];
   const uint32_t u = static_cast<uint32_t>(skeys[ins[j]] >> 32);
   const uint2 rb = row[u];
-  const uint32_t len = rb.y, need = len + k;
-  const uint32_t nc = need + (need >> 2) + 4u;
-  if (threadIdx.x == 0)
-    s_nb = static_cast<uint32_t>(atomicAdd(&ctr->bump[dir], static_cast<unsigned long long>(nc)));
+  const uint32_t len = rb.y;
+  uint32_t* a = arena + rb.x;
+  for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) s_ins[t] = static_cast<uint32_t>(skeys[ins[j + t]]);
   __syncthreads();
-  const uint32_t nb = s_nb;
-  const uint32_t per = (need + blockDim.x - 1) / blockDim.x;
-  uint32_t o0 = threadIdx.x * per;
-  uint32_t o1 = o0 + per < need ? o0 + per : need;
-  if (o0 < need) {
-    // merge path split: number of old-row entries among the first o0 outputs
-    uint32_t lo = o0 > k ? o0 - k : 0u, hi = o0 < len ? o0 : len;
+  // final place of insert t: t + old entries below it
+  for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) {
+    const uint32_t x = s_ins[t];
+    uint32_t lo = 0, hi = len;
     while (lo < hi) {
-      uint32_t mid = (lo + hi) >> 1;
-      uint32_t a = arena[rb.x + mid] & kIdMask;
-      uint32_t b = static_cast<uint32_t>(skeys[ins[j + (o0 - mid - 1)]]);
-      if (a < b) lo = mid + 1;
-      else hi = mid;
+      const uint32_t m = (lo + hi) >> 1;
+      if ((a[m] & kIdMask) < x) lo = m + 1;
+      else hi = m;
     }
-    uint32_t i = lo, t = o0 - lo;
-    for (uint32_t w = o0; w < o1; ++w) {
-      uint32_t vt = t < k ? static_cast<uint32_t>(skeys[ins[j + t]]) : kInf;
-      if (i < len && (arena[rb.x + i] & kIdMask) < vt) {
-        arena[nb + w] = arena[rb.x + i];
-        ++i;
-      } else {
-        arena[nb + w] = vt;
-        ++t;
-      }
-    }
+    s_pos[t] = t + lo;
   }
   __syncthreads();
+  // old entry i moves to i + (inserts below it)
+  auto rank_ins = [&](uint32_t x) {
+    uint32_t lo = 0, hi = k;
+    while (lo < hi) {
+      const uint32_t m = (lo + hi) >> 1;
+      if (s_ins[m] < x) lo = m + 1;
+      else hi = m;
+    }
+    return lo;
+  };
+  const uint32_t tiles = (len + kMidTile - 1) / kMidTile;
+  for (uint32_t t = tiles; t-- > 0;) {
+    uint32_t v[4], dst[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t i = t * kMidTile + q * 256 + threadIdx.x;
+      dst[q] = ~0u;
+      v[q] = 0;
+      if (i < len) {
+        v[q] = a[i];
+        dst[q] = i + rank_ins(v[q] & kIdMask);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (dst[q] != ~0u) a[dst[q]] = v[q];
+    __syncthreads();
+  }
+  for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) a[s_pos[t]] = s_ins[t];
   if (threadIdx.x == 0) {
-    row[u] = make_uint2(nb, need);
-    cap[u] = nc;
+    row[u].y = len + k;
     atomicAdd(&deg[u], k);
     atomicAdd(&ctr->live_delta[dir], static_cast<unsigned long long>(k));
     atomicAdd(&ctr->stored_delta[dir], static_cast<unsigned long long>(k));
-    atomicAdd(&ctr->relocated[dir], 1ull);
+  }
+}
+
+// Relocation plan (one workgroup): for relocated row b (insert index
+// big[b]) record {u, j, k, old begin, old len, new begin}, its first output
+// chunk in plan[8 * nbig + b] (the total after the last), give it its new
+// range at the bump pointer and switch the row table to it.  The old range
+// stays intact for k_reloc_copy.
+__global__ __launch_bounds__(256) void k_reloc_plan(const uint32_t* __restrict__ big, uint32_t nbig,
+                                                    const uint32_t* __restrict__ ins,
+                                                    const uint32_t* __restrict__ segk,
+                                                    const unsigned long long* __restrict__ skeys,
+                                                    uint2* __restrict__ row, uint32_t* __restrict__ cap,
+                                                    uint32_t* __restrict__ deg, uint32_t* __restrict__ plan,
+                                                    GraphCounters* ctr, int dir) {
+  __shared__ unsigned long long s_scan[2][256];
+  __shared__ unsigned long long s_base[2];
+  if (threadIdx.x == 0) {
+    s_base[0] = ctr->bump[dir];  // arena offset
+    s_base[1] = 0;               // chunk offset
+  }
+  __syncthreads();
+  uint32_t* first = plan + 8ull * nbig;
+  for (uint32_t b0 = 0; b0 < nbig; b0 += 256) {
+    const uint32_t b = b0 + threadIdx.x;
+    uint32_t j = 0, k = 0, u = 0, nc = 0, chunks = 0, need = 0;
+    uint2 rb = make_uint2(0u, 0u);
+    if (b < nbig) {
+      j = big[b];
+      k = segk[j];
+      u = static_cast<uint32_t>(skeys[ins[j]] >> 32);
+      rb = row[u];
+      need = rb.y + k;
+      nc = relocated_cap(need);
+      chunks = (need + kRelocChunk - 1) / kRelocChunk;
+    }
+    s_scan[0][threadIdx.x] = nc;
+    s_scan[1][threadIdx.x] = chunks;
+    __syncthreads();
+    for (uint32_t d = 1; d < 256; d <<= 1) {  // inclusive scans of nc and chunks
+      unsigned long long x0 = 0, x1 = 0;
+      if (threadIdx.x >= d) {
+        x0 = s_scan[0][threadIdx.x - d];
+        x1 = s_scan[1][threadIdx.x - d];
+      }
+      __syncthreads();
+      s_scan[0][threadIdx.x] += x0;
+      s_scan[1][threadIdx.x] += x1;
+      __syncthreads();
+    }
+    if (b < nbig) {
+      const unsigned long long nb = s_base[0] + s_scan[0][threadIdx.x] - nc;
+      const unsigned long long cf = s_base[1] + s_scan[1][threadIdx.x] - chunks;
+      uint32_t* p = plan + 8ull * b;
+      p[0] = u;
+      p[1] = j;
+      p[2] = k;
+      p[3] = rb.x;
+      p[4] = rb.y;
+      p[5] = static_cast<uint32_t>(nb);
+      first[b] = static_cast<uint32_t>(cf);
+      row[u] = make_uint2(static_cast<uint32_t>(nb), need);
+      cap[u] = nc;
+      atomicAdd(&deg[u], k);
+    }
+    __syncthreads();
+    if (threadIdx.x == 255) {
+      s_base[0] += s_scan[0][255];
+      s_base[1] += s_scan[1][255];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    first[nbig] = static_cast<uint32_t>(s_base[1]);
+    ctr->bump[dir] = s_base[0];
+  }
+  // live and stored arcs grow by the inserts of every relocated row
+  unsigned long long ksum = 0;
+  for (uint32_t b = threadIdx.x; b < nbig; b += 256) ksum += plan[8ull * b + 2];
+  ksum = wave_sum(ksum);
+  if (lane_id() == 0 && ksum) {
+    atomicAdd(&ctr->live_delta[dir], ksum);
+    atomicAdd(&ctr->stored_delta[dir], ksum);
+  }
+  if (threadIdx.x == 0) atomicAdd(&ctr->relocated[dir], static_cast<unsigned long long>(nbig));
+}
+
+// Output chunk c of a relocated row: the merge of the old row and its
+// inserts over outputs [cj * kRelocChunk, ...), each thread a contiguous
+// eighth found by a merge-path split.  Workgroups loop over the chunks.
+__global__ __launch_bounds__(256) void k_reloc_copy(const uint32_t* __restrict__ plan, uint32_t nbig,
+                                                    const uint32_t* __restrict__ ins,
+                                                    const unsigned long long* __restrict__ skeys,
+                                                    uint32_t* __restrict__ arena) {
+  const uint32_t* first = plan + 8ull * nbig;
+  const uint32_t total = first[nbig];
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    // relocated row owning chunk c: last b with first[b] <= c
+    uint32_t lo = 0, hi = nbig;
+    while (hi - lo > 1) {
+      const uint32_t m = (lo + hi) >> 1;
+      if (first[m] <= c) lo = m;
+      else hi = m;
+    }
+    const uint32_t* p = plan + 8ull * lo;
+    const uint32_t j = p[1], k = p[2], ox = p[3], len = p[4], nb = p[5];
+    const uint32_t need = len + k;
+    const uint32_t c0 = (c - first[lo]) * kRelocChunk;
+    const uint32_t c1 = min(c0 + kRelocChunk, need);
+    constexpr uint32_t per = kRelocChunk / 256;
+    const uint32_t o0 = c0 + threadIdx.x * per;
+    const uint32_t o1 = min(o0 + per, c1);
+    if (o0 < c1) {
+      // merge path: old entries among the first o0 outputs
+      uint32_t l = o0 > k ? o0 - k : 0u, h = o0 < len ? o0 : len;
+      while (l < h) {
+        const uint32_t m = (l + h) >> 1;
+        const uint32_t av = arena[ox + m] & kIdMask;
+        const uint32_t bv = static_cast<uint32_t>(skeys[ins[j + (o0 - m - 1)]]);
+        if (av < bv) l = m + 1;
+        else h = m;
+      }
+      uint32_t i = l, t = o0 - l;
+      for (uint32_t w = o0; w < o1; ++w) {
+        const uint32_t vt = t < k ? static_cast<uint32_t>(skeys[ins[j + t]]) : kInf;
+        if (i < len && (arena[ox + i] & kIdMask) < vt) {
+          arena[nb + w] = arena[ox + i];
+          ++i;
+        } else {
+          arena[nb + w] = vt;
+          ++t;
+        }
+      }
+    }
   }
 }
 
@@ -657,6 +824,7 @@ void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const u
     b.ins.ensure(M + 1);
     b.segk.ensure(M + 1);
     b.big.ensure(M + 1);
+    b.mid.ensure(M + 1);
     b.flags.ensure(2 * M + 2);
     k_expand_keys<<<grid_for(k), kBlock, 0, s>>>(d_src, d_dst, k, g->directed ? 0 : 1, d == 1 ? 1 : 0, b.keys.p,
                                                   b.vals.p);
@@ -680,7 +848,7 @@ void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const u
                                            static_cast<int>(M), s);
     });
     k_plan_inserts<<<grid_for(M), kBlock, 0, s>>>(b.ins.p, &g->d_ctr->nins[d], b.skeys.p, a.row.p, a.cap.p,
-                                                  b.segk.p, b.big.p, g->d_ctr, d);
+                                                  b.segk.p, b.big.p, b.mid.p, g->d_ctr, d);
     VGX_LAUNCH_CHECK();
   }
   graph_sync_counters(g);
@@ -689,6 +857,8 @@ void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const u
     b.nfinal = g->h_ctr->nfinal[d];
     b.nins = g->h_ctr->nins[d];
     b.nbig = g->h_ctr->nbig[d];
+    b.nmid = g->h_ctr->nmid[d];
+    b.need = g->h_ctr->need[d];
     ensure_arena(g, d, g->h_ctr->need[d]);
   }
 }
@@ -720,10 +890,21 @@ void graph_apply_additions(vgx_graph* g) {
                                                                  g->d_ctr, d);
       VGX_LAUNCH_CHECK();
     }
+    if (b.nmid) {
+      k_insert_mid<<<static_cast<unsigned>(b.nmid), 256, 0, g->stream>>>(b.mid.p, b.ins.p, b.segk.p, b.skeys.p,
+                                                                        a.row.p, a.arena.p, a.deg.p, g->d_ctr, d);
+      VGX_LAUNCH_CHECK();
+    }
     if (b.nbig) {
-      k_insert_big<<<static_cast<unsigned>(b.nbig), kBlock, 0, g->stream>>>(b.big.p, b.ins.p, b.segk.p, b.skeys.p,
-                                                                           a.row.p, a.cap.p, a.arena.p, a.deg.p,
-                                                                           g->d_ctr, d);
+      b.reloc.ensure(9 * b.nbig + 2);
+      k_reloc_plan<<<1, 256, 0, g->stream>>>(b.big.p, static_cast<uint32_t>(b.nbig), b.ins.p, b.segk.p, b.skeys.p,
+                                             a.row.p, a.cap.p, a.deg.p, b.reloc.p, g->d_ctr, d);
+      VGX_LAUNCH_CHECK();
+      // chunks <= sum of new capacities / chunk + one per row
+      const uint64_t chunks = b.need / kRelocChunk + b.nbig;
+      const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(4096, std::max<uint64_t>(1, chunks)));
+      k_reloc_copy<<<grid, 256, 0, g->stream>>>(b.reloc.p, static_cast<uint32_t>(b.nbig), b.ins.p, b.skeys.p,
+                                                a.arena.p);
       VGX_LAUNCH_CHECK();
     }
   }

@@ -113,7 +113,8 @@ struct GraphCounters {
   unsigned long long relocated[2];
   unsigned long long nfinal[2];
   unsigned long long nins[2];
-  unsigned long long nbig[2];
+  unsigned long long nbig[2];        // rows relocated (outgrow their slots, or very long)
+  unsigned long long nmid[2];        // rows of more than 32 entries merged in place
 };
 
 // Staged update batch (device side).
@@ -127,11 +128,13 @@ struct BatchScratch {
   DevBuf<uint32_t> fin;     // sorted positions of final arcs
   DevBuf<uint32_t> ins;     // sorted positions of inserts
   DevBuf<uint32_t> segk;    // per insert index: inserts in its row (segment start) or 0
-  DevBuf<uint32_t> big;     // insert indices of rows merged block-wide
+  DevBuf<uint32_t> big;     // insert indices of rows relocated
+  DevBuf<uint32_t> mid;     // insert indices of rows merged in place by a workgroup
+  DevBuf<uint32_t> reloc;   // relocation plan: 8 words per relocated row + chunk prefix
   DevBuf<uint8_t> flags;
   DevBuf<uint8_t> tmp;      // library temp storage
   uint64_t k = 0, m = 0;    // ops, arcs
-  uint64_t nfinal = 0, nins = 0, nbig = 0;
+  uint64_t nfinal = 0, nins = 0, nbig = 0, nmid = 0, need = 0;
 };
 
 enum ActCode : uint8_t { kActNopDel = 0, kActDel = 1, kActNopAdd = 2, kActUndel = 3, kActIns = 4 };
