@@ -14,9 +14,9 @@ loop/duplicate removal).
           mean is sum(m_c) / sum(t));
 * roofline: algorithmic bytes of one BFS (4*A_c + 12*V_c) over its GPU
   time, measured in the timed region with HIP events on the graph's stream
-  around every traversal (init, setup and level kernels); traffic = HBM bytes per BFS from rocprofv3 FETCH_SIZE and
-  WRITE_SIZE passes over a child run, each scaled by its reading for a
-  1 GiB copy;
+  around every traversal (init and level kernels); traffic = HBM bytes per
+  BFS from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over a child run, each
+  scaled by its reading for a 1 GiB copy;
 * cpu_baseline: the reference engine (oracle/_ref/vgx_ref, compiled from the
   reference sources) or, where it is absent, the oracle's queue BFS, timed
   on this host on a bounded sample of the same roots.
@@ -205,14 +205,14 @@ def _counter_rows(directory):
 
 
 def _per_bfs(rows):
-    """Counter sums (KiB) of the kernels of each traversal (init, setup,
-    level and tail kernels), split at k_bfs_init (one per traversal)."""
+    """Counter sums (KiB) of the kernels of each traversal (init, level
+    and tail kernels), split at k_bfs_init (one per traversal)."""
     groups = []
     for r in rows:
         name = r["Kernel_Name"]
         if "k_bfs_init" in name:
             groups.append(float(r["Counter_Value"]))
-        elif groups and any(k in name for k in ("k_bfs_setup", "k_bfs_level", "k_bfs_bottom_up_tail")):
+        elif groups and any(k in name for k in ("k_bfs_level", "k_bfs_bottom_up_tail")):
             groups[-1] += float(r["Counter_Value"])
     return groups
 
@@ -312,7 +312,7 @@ def timed_bfs(vx, g, roots, steps, warmup=0, objs=None):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     # GPU time per BFS over the timed region: HIP events on the graph's
-    # stream around each traversal (its init, setup and level kernels)
+    # stream around each traversal (its init and level kernels)
     kernel_ms = sum(o.stats()["traversal_ms"] for o in objs) / steps
     # launches per BFS: one more pass with events around every level launch
     for o in objs:
@@ -741,7 +741,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": None,
-            "kernel": "one traversal: k_bfs_init, k_bfs_setup, k_bfs_level, k_bfs_bottom_up_tail",
+            "kernel": "one traversal: k_bfs_init, k_bfs_level, k_bfs_bottom_up_tail",
             "bytes_per_bfs": bytes_bfs,
             "gpu_ms_per_bfs": kernel_ms_per_bfs,
             "avg_launch_us": 1e3 * level_ms_eager / max(launches_per_bfs, 1),

@@ -48,6 +48,11 @@ constexpr uint32_t kWavesPerBlock = kBfsBlock / kWave;
 // Bottom-up phase A checks the first kHead parents of a vertex from the
 // dense head array; the rest of longer rows is scanned by phase B.
 constexpr uint32_t kHead = 4;
+// A bottom-up level whose previous traversal left at most this many rows
+// (and edges) beyond the head finishes them inside the level kernel instead
+// of a phase-B launch.
+constexpr uint64_t kInlineRows = 2048;
+constexpr uint64_t kInlineEdges = 65536;
 
 // Per-level record, kept in a ring of 4 indexed by level & 3.  Level L's
 // kernel accumulates the frontier of level L+1 into rec[(L+1)&3] (q, found)
@@ -167,6 +172,7 @@ struct vgx_bfs {
   uint32_t prof_levels = 0;
   uint32_t prof_mode[64] = {};
   uint64_t prof_mf[64] = {};
+  uint64_t prof_pend[64] = {};  // packed rows << kEdgeBits | edges left for phase B
   // replay of a repeated traversal's launch program
   bool use_graph = true;
   std::vector<uint64_t> gkey;
@@ -226,35 +232,6 @@ __global__ void k_build_zin(const uint32_t* __restrict__ in_deg, uint32_t n, uin
     const uint32_t v = wi * kWave + lane_id();
     const unsigned long long m = __ballot(v >= n || in_deg[v] == 0);
     if (lane_id() == 0) zin[wi] = m;
-  }
-}
-
-__global__ void k_bfs_setup(LevelArgs a, uint32_t root, unsigned long long total_edges, float alpha, float beta) {
-  BfsState* st = a.st;
-  for (int i = 0; i < 4; ++i) {
-    st->rec[i].q = 0;
-    for (uint32_t j = 0; j < kFoundStripes; ++j) st->rec[i].found[j].v = 0;
-    st->ctl[i].visited_edges = 0;
-    st->ctl[i].prev_mode = 0;
-    st->pend[i].q = 0;
-  }
-  st->total_edges = total_edges;
-  st->n = a.n;
-  st->bu_levels = 0;
-  st->alpha = alpha;
-  st->beta = beta;
-  st->done_level = ~0u;
-  st->reached = 0;
-  if (root >= a.n) return;  // an empty level-0 frontier ends the traversal
-  a.dist[root] = 0;
-  a.visited[root >> 6] |= 1ull << (root & 63);
-  a.bm[root >> 6] |= 1ull << (root & 63);
-  const uint32_t len = a.row[root].y;
-  st->rec[0].found[0].v = 1;
-  if (len) {
-    a.queue0[0] = root;
-    a.pre0[0] = 0;
-    st->rec[0].q = (1ull << kEdgeBits) | len;
   }
 }
 
@@ -334,8 +311,8 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
       st->log_mode[level] = mode;
       st->log_nf[level] = nf;
       st->log_mf[level] = mf;
-      st->log_pend[level] = 0;
     }
+    if (level + 1 < kLogLevels) st->log_pend[level + 1] = 0;
   }
   const int cur = level & 1, nxt = cur ^ 1;
   const uint32_t words = a.words;
@@ -344,7 +321,9 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
   unsigned long long* bm_clear = a.bm + static_cast<uint64_t>((level + 2) % 3) * words;
   const uint32_t gthreads = gridDim.x * blockDim.x;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  for (uint32_t i = gtid; i < words; i += gthreads) bm_clear[i] = 0;
+  // level 0 starts with the bitmaps cleared by the init kernel
+  if (level > 0)
+    for (uint32_t i = gtid; i < words; i += gthreads) bm_clear[i] = 0;
   const uint32_t W = gridDim.x * kWavesPerBlock;
   const uint32_t gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   const unsigned lane = lane_id();
@@ -357,7 +336,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
   WaveStage& sb = s_stage[threadIdx.x / kWave];
   WaveStage& sb2 = s_stage2[threadIdx.x / kWave];
   uint32_t nbuf = 0, nbuf2 = 0;
-  unsigned long long found_local = 0;
+  unsigned long long found_local = 0, pend_local = 0;
 
   // Claim w for the next level if nobody has; queue it with its out-degree.
   auto claim = [&](uint32_t w, bool consider) {
@@ -440,6 +419,10 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
       const uint2 r0 = (unv0 && !f0) ? a.in_row[v0] : make_uint2(0u, 0u);
       const uint2 r1 = (unv1 && !f1) ? a.in_row[v1] : make_uint2(0u, 0u);
       if (inline_tail) {
+        // recorded like phase B's list, so that the next traversal of this
+        // object can choose between the two
+        if (r0.y > kHead) pend_local += (1ull << kEdgeBits) + (r0.y - kHead);
+        if (r1.y > kHead) pend_local += (1ull << kEdgeBits) + (r1.y - kHead);
         f0 = f0 || rest_in_frontier(a, bm_cur, r0, r0.y > kHead);
         f1 = f1 || rest_in_frontier(a, bm_cur, r1, r1.y > kHead);
       } else {
@@ -468,11 +451,23 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
   stage_flush_block(sb, nbuf, &rn->q, qn, pn, s_flush);
   stage_flush_block(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, s_flush);
   if (lane == 0) s_found[threadIdx.x / kWave] = found_local;
+  const bool count_pend = mode == 1 && inline_tail && level < kLogLevels;
+  __shared__ unsigned long long s_pend[kWavesPerBlock];
+  if (count_pend) {
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) pend_local += __shfl_xor(pend_local, d, kWave);
+    if (lane == 0) s_pend[threadIdx.x / kWave] = pend_local;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long f = 0;
     for (uint32_t i = 0; i < kWavesPerBlock; ++i) f += s_found[i];
     if (f) atomicAdd(&rn->found[blockIdx.x % kFoundStripes].v, f);
+    if (count_pend) {
+      unsigned long long p = 0;
+      for (uint32_t i = 0; i < kWavesPerBlock; ++i) p += s_pend[i];
+      if (p) atomicAdd(&st->log_pend[level], p);
+    }
   }
 }
 
@@ -922,19 +917,53 @@ static LevelArgs level_args(vgx_bfs* b) {
   return a;
 }
 
-// Start of a traversal in one launch: all distances unreachable, visited =
-// the zero-in-degree bitmap, the three frontier bitmaps clear.
-__global__ void k_bfs_init(uint32_t* __restrict__ dist, uint64_t n, unsigned long long* __restrict__ visited,
-                           const unsigned long long* __restrict__ zin, uint64_t words,
-                           unsigned long long* __restrict__ bm) {
+// Start of a traversal in one launch: distances to infinity, the visited
+// bitmap from the zero-in-degree bitmap, the frontier bitmaps cleared, and
+// the root placed (distance 0, visited, level-0 frontier and queue).
+// Workgroup 0 resets the level counters.  A root outside the graph leaves
+// the level-0 frontier empty, which ends the traversal at once.
+__global__ void k_bfs_init(LevelArgs a, const unsigned long long* __restrict__ zin, uint32_t root,
+                           unsigned long long total_edges, float alpha, float beta) {
+  const uint64_t n = a.n, words = a.words;
+  const bool has_root = root < n;
+  const uint64_t rw = has_root ? root >> 6 : ~0ull;
+  const unsigned long long rbit = 1ull << (root & 63);
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
-    dist[i] = kInf;
-  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < words; i += stride) {
-    visited[i] = zin[i];
-    bm[i] = 0;
-    bm[words + i] = 0;
-    bm[2 * words + i] = 0;
+  const uint64_t t0 = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  for (uint64_t i = t0; i < n; i += stride) a.dist[i] = i == root ? 0u : kInf;
+  for (uint64_t i = t0; i < words; i += stride) {
+    const unsigned long long r = i == rw ? rbit : 0ull;
+    a.visited[i] = zin[i] | r;
+    a.bm[i] = r;
+    a.bm[words + i] = 0;
+    a.bm[2 * words + i] = 0;
+  }
+  if (blockIdx.x != 0) return;
+  BfsState* st = a.st;
+  for (uint32_t t = threadIdx.x; t < 4 * kFoundStripes; t += blockDim.x) st->rec[t / kFoundStripes].found[t % kFoundStripes].v = 0;
+  if (threadIdx.x < 4) {
+    st->rec[threadIdx.x].q = 0;
+    st->ctl[threadIdx.x].visited_edges = 0;
+    st->ctl[threadIdx.x].prev_mode = 0;
+    st->pend[threadIdx.x].q = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  st->total_edges = total_edges;
+  st->n = a.n;
+  st->bu_levels = 0;
+  st->alpha = alpha;
+  st->beta = beta;
+  st->done_level = ~0u;
+  st->reached = 0;
+  st->log_pend[0] = 0;
+  if (!has_root) return;
+  st->rec[0].found[0].v = 1;
+  const uint32_t len = a.row[root].y;
+  if (len) {
+    a.queue0[0] = root;
+    a.pre0[0] = 0;
+    st->rec[0].q = (1ull << kEdgeBits) | len;
   }
 }
 
@@ -994,17 +1023,20 @@ static void bfs_full(vgx_bfs* b) {
       return;
     }
     if (L >= kLogLevels) return;
-    if (b->prof_mode[L] != 1) tb = 0;
+    // no tail where the level was not bottom-up or left few rows for phase B
+    if (b->prof_mode[L] != 1 || ((b->prof_pend[L] >> kEdgeBits) <= kInlineRows &&
+                                  (b->prof_pend[L] & kEdgeMask) <= kInlineEdges))
+      tb = 0;
     if (b->prof_mode[L] == 0) {
       const uint64_t windows = (b->prof_mf[L] + kWave - 1) / kWave;  // 64-edge windows
       const uint64_t want = (windows + 16 * kWavesPerBlock - 1) / (16 * kWavesPerBlock);
       lb = static_cast<unsigned>(std::min<uint64_t>(blocks, std::max<uint64_t>(16, want)));
     }
   };
-  // The first chunk covers the depth of the previous traversal (one launch
-  // beyond it detects completion); later chunks shrink.
-  uint32_t chunk = std::max<uint32_t>(1, b->tune.levels_per_sync);
-  if (b->levels) chunk = std::max<uint32_t>(chunk, b->levels + 1);
+  // The first chunk covers the depth of the previous traversal; the host
+  // sees completion in the counters of the level after the chunk, so no
+  // launch is spent on detecting it.  Later chunks shrink.
+  uint32_t chunk = b->levels ? b->levels : std::max<uint32_t>(1, b->tune.levels_per_sync);
   const unsigned long long total_edges = g->adj[0].stored;
   const float alpha = static_cast<float>(b->tune.alpha), beta = static_cast<float>(b->tune.beta);
   const unsigned ginit = blocks_for(std::max<uint64_t>(n, 1), kBfsBlock, 2048);
@@ -1028,11 +1060,7 @@ static void bfs_full(vgx_bfs* b) {
     }
   };
   auto launch_start = [&]() {
-    if (n) {
-      k_bfs_init<<<ginit, kBfsBlock, 0, s>>>(b->w.dist.p, n, b->w.visited.p, b->w.zin.p, words, b->w.bm.p);
-      VGX_LAUNCH_CHECK();
-    }
-    k_bfs_setup<<<1, 1, 0, s>>>(a, b->source, total_edges, alpha, beta);
+    k_bfs_init<<<ginit, kBfsBlock, 0, s>>>(a, b->w.zin.p, b->source, total_edges, alpha, beta);
     VGX_LAUNCH_CHECK();
   };
   // A traversal that repeats the previous one's launch program exactly
@@ -1081,7 +1109,7 @@ static void bfs_full(vgx_bfs* b) {
       b->gkey = std::move(key);
     }
   }
-  uint32_t launched = 0;
+  uint32_t launched = 0, done_level = ~0u;
   bool first = true;
   for (;;) {
     if (!(first && replayed)) {
@@ -1108,18 +1136,26 @@ static void bfs_full(vgx_bfs* b) {
       b->level_launches += chunk;
     }
     first = false;
-    if (b->h_st->done_level != ~0u) break;
+    done_level = b->h_st->done_level;
+    if (done_level == ~0u) {
+      // the last launched level found nothing: the traversal is complete
+      unsigned long long f = 0;
+      for (uint32_t j = 0; j < kFoundStripes; ++j) f += b->h_st->rec[launched & 3].found[j].v;
+      if (f == 0) done_level = launched;
+    }
+    if (done_level != ~0u) break;
     if (launched > n + 2) throw Error(VGX_DEVICE_ERROR, "BFS did not terminate");
     chunk = std::max<uint32_t>(2, chunk / 2);
   }
   if (whole) b->traversals += 1;
   b->reachable = b->h_st->reached;
-  b->levels = b->h_st->done_level;
+  b->levels = done_level;
   b->bu_levels = b->h_st->bu_levels;
   b->prof_levels = b->levels;
   for (uint32_t L = 0; L < b->levels && L < kLogLevels; ++L) {
     b->prof_mode[L] = b->h_st->log_mode[L];
     b->prof_mf[L] = b->h_st->log_mf[L];
+    b->prof_pend[L] = b->h_st->log_pend[L];
   }
 }
 
