@@ -53,6 +53,10 @@ constexpr uint32_t kHead = 4;
 // of a phase-B launch.
 constexpr uint64_t kInlineRows = 2048;
 constexpr uint64_t kInlineEdges = 65536;
+// Top-down levels sized from the previous traversal get about this many
+// 64-edge windows per wave (at least 16 workgroups): each window is a chain
+// of dependent loads, so heavy levels want many waves, light ones few.
+constexpr uint64_t kTdWindowsPerWave = 2;
 
 // Per-level record, kept in a ring of 4 indexed by level & 3.  Level L's
 // kernel accumulates the frontier of level L+1 into rec[(L+1)&3] (q, found)
@@ -1029,7 +1033,8 @@ static void bfs_full(vgx_bfs* b) {
       tb = 0;
     if (b->prof_mode[L] == 0) {
       const uint64_t windows = (b->prof_mf[L] + kWave - 1) / kWave;  // 64-edge windows
-      const uint64_t want = (windows + 16 * kWavesPerBlock - 1) / (16 * kWavesPerBlock);
+      const uint64_t per_block = kTdWindowsPerWave * kWavesPerBlock;
+      const uint64_t want = (windows + per_block - 1) / per_block;
       lb = static_cast<unsigned>(std::min<uint64_t>(blocks, std::max<uint64_t>(16, want)));
     }
   };

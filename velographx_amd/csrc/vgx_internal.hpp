@@ -285,16 +285,24 @@ __device__ __forceinline__ void lbs_edges(const uint32_t* queue, const unsigned 
   unsigned long long a = static_cast<unsigned long long>(gw) * S;
   if (a >= E) return;
   unsigned long long b = a + S < E ? a + S : E;
-  // first item whose prefix <= a
+  const unsigned lane = lane_id();
+  // Last item whose prefix <= a, by a 64-way search: the wave samples 64
+  // evenly spaced prefixes of [lo, hi) and keeps the stretch after the last
+  // one <= a, so F items take ceil(log64 F) dependent loads instead of
+  // log2 F.  Prefixes strictly increase, so the lanes that pass are a prefix
+  // of the wave; lane 0 always passes (pre[lo] <= a).
   uint32_t lo = 0, hi = F;
   while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (pre[mid] <= a) lo = mid;
-    else hi = mid;
+    const uint32_t step = (hi - lo + kWave - 1) / kWave;
+    const uint32_t idx = lo + lane * step;
+    const bool ok = idx < hi && pre[idx] <= a;
+    const unsigned long long m = __ballot(ok);
+    const uint32_t last = 63 - __clzll(static_cast<long long>(m));
+    lo += last * step;
+    hi = lo + step < hi ? lo + step : hi;
   }
   // Invariant: item i0 contains edge e (pre[i0] <= e < pre[i0 + 1]).
   uint32_t i0 = lo;
-  const unsigned lane = lane_id();
   for (unsigned long long e = a; e < b; e += kWave) {
     // Lane k holds item i0 + k of the window.  Every item has at least one
     // edge and item i0 + 1 starts after e, so the items starting inside this
