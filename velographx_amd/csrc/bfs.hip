@@ -115,18 +115,25 @@ struct BfsState {
   unsigned long long log_pend[kLogLevels];  // packed count << 38 | edges left for phase B
 };
 
-// Counters of the incremental repair.
+// Counters of the incremental repair.  The lists of the invalidation
+// rounds come in two parities (round r uses r & 1) and the repair frontier
+// in a ring of three, so that a kernel can clear the counters the next
+// round appends to while this round's are still being read: the host
+// launches several rounds between synchronisations and every kernel takes
+// its list sizes from here.
 struct IncState {
-  unsigned long long tq;        // touched list (plain count)
-  unsigned long long tfq;       // first-touched list, packed (in-rows)
-  unsigned long long aq;        // newly affected, packed (out-rows)
+  unsigned long long tq[2];     // touched list (plain count)
+  unsigned long long tfq[2];    // first-touched list, packed (in-rows)
+  unsigned long long aq[2];     // newly affected, packed (out-rows)
   unsigned long long aall;      // all affected (plain count)
   unsigned long long tall;      // all touched (plain count)
   unsigned long long candidates;
-  unsigned long long fq[2];     // repair frontier, packed (out-rows)
+  unsigned long long fq[3];     // repair frontier, packed (out-rows)
   unsigned long long sq;        // seed list, packed (in-rows)
   unsigned long long became_reachable;
   unsigned long long relaxed;
+  uint32_t fallback;            // set once more than `limit` vertices are affected
+  uint32_t pad;
 };
 
 enum : uint32_t { kFAffected = 1u, kFSupport = 2u, kFCand = 4u };
@@ -547,10 +554,15 @@ struct IncArgs {
   IncState* st;
   uint32_t epoch;
   uint32_t source;
+  uint64_t lcap;  // slots of one parity of the round lists
 };
 
-// touch(): record that v received a loss (or is a candidate) this round.
-__device__ __forceinline__ void touch(const IncArgs& a, uint32_t v, uint32_t* tlist, uint32_t* tall,
+// Workgroups of the repair kernels whose list sizes live on the device.
+constexpr unsigned kRepairBlocks = 512;
+
+// touch(): record that v received a loss (or is a candidate) in the round
+// whose lists have parity `par`.
+__device__ __forceinline__ void touch(const IncArgs& a, int par, uint32_t v, uint32_t* tlist, uint32_t* tall,
                                       uint32_t* tflist, unsigned long long* tfpre, bool want) {
   bool t_new = false, first = false;
   uint32_t inlen = 0;
@@ -563,12 +575,13 @@ __device__ __forceinline__ void touch(const IncArgs& a, uint32_t v, uint32_t* tl
       inlen = a.in_row[v].y;
     }
   }
+  const uint64_t off = par * a.lcap;
   unsigned long long bt = __ballot(t_new);
   if (bt) {
     unsigned long long base = 0;
-    if (lane_id() == 0) base = atomicAdd(&a.st->tq, static_cast<unsigned long long>(__popcll(bt)));
+    if (lane_id() == 0) base = atomicAdd(&a.st->tq[par], static_cast<unsigned long long>(__popcll(bt)));
     base = __shfl(base, 0, kWave);
-    if (t_new) tlist[base + lanes_below(bt)] = v;
+    if (t_new) tlist[off + base + lanes_below(bt)] = v;
   }
   unsigned long long bf = __ballot(first);
   if (bf) {
@@ -577,10 +590,11 @@ __device__ __forceinline__ void touch(const IncArgs& a, uint32_t v, uint32_t* tl
     base = __shfl(base, 0, kWave);
     if (first) tall[base + lanes_below(bf)] = v;
   }
-  wave_push(&a.st->tfq, tflist, tfpre, first && inlen > 0, v, inlen);
+  wave_push(&a.st->tfq[par], tflist + off, tfpre + off, first && inlen > 0, v, inlen);
 }
 
-// Heads of deleted arcs that were shortest-parent arcs (bfs.hpp:52-63, 242-245).
+// Heads of deleted arcs that were shortest-parent arcs (bfs.hpp:52-63, 242-245),
+// touched for round 0.
 __global__ void k_inc_candidates(IncArgs a, const uint32_t* fin, uint64_t nfin, const unsigned long long* skeys,
                                  const uint8_t* act, uint32_t* tlist, uint32_t* tall, uint32_t* tflist,
                                  unsigned long long* tfpre) {
@@ -603,16 +617,21 @@ __global__ void k_inc_candidates(IncArgs a, const uint32_t* fin, uint64_t nfin, 
         }
       }
     }
-    touch(a, v, tlist, tall, tflist, tfpre, want);
+    touch(a, 0, v, tlist, tall, tflist, tfpre, want);
   }
 }
 
-// support[v] += live in-arcs p->v with d(p) + 1 == d(v).
-__global__ void k_inc_support(IncArgs a, const uint32_t* tflist, const unsigned long long* tfpre) {
-  const unsigned long long qc = a.st->tfq;
+// Round of parity par, step 1: support[v] += live in-arcs p->v with
+// d(p) + 1 == d(v), over the vertices touched for the first time.  Clears
+// the affected counter this round's evaluation appends to.
+__global__ void k_inc_support(IncArgs a, int par, const uint32_t* tflist, const unsigned long long* tfpre) {
+  if (a.st->fallback) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.st->aq[par] = 0;
+  const unsigned long long qc = a.st->tfq[par];
   const uint32_t W = gridDim.x * kWavesPerBlock;
   const uint32_t gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  lbs_edges(tflist, tfpre, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.in_row, gw, W,
+  const uint64_t off = par * a.lcap;
+  lbs_edges(tflist + off, tfpre + off, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.in_row, gw, W,
             [&](uint32_t, uint32_t v, uint2, uint32_t slot, bool valid) {
               if (!valid) return;
               uint32_t p = a.in_arena[slot];
@@ -622,9 +641,17 @@ __global__ void k_inc_support(IncArgs a, const uint32_t* tflist, const unsigned 
             });
 }
 
-// v is affected once every live shortest parent is lost.
-__global__ void k_inc_evaluate(IncArgs a, const uint32_t* tlist, uint64_t nt, uint32_t* alist,
+// Step 2: v is affected once every live shortest parent is lost.  Clears the
+// touched counters of the next parity, which step 3 appends to.
+__global__ void k_inc_evaluate(IncArgs a, int par, const uint32_t* tlist, uint32_t* alist,
                                unsigned long long* apre, uint32_t* aall) {
+  if (a.st->fallback) return;
+  const uint64_t nt = a.st->tq[par];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.st->tq[par ^ 1] = 0;
+    a.st->tfq[par ^ 1] = 0;
+  }
+  const uint64_t off = par * a.lcap;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   for (uint64_t j0 = base - lane_id(); j0 < nt; j0 += stride) {
@@ -632,7 +659,7 @@ __global__ void k_inc_evaluate(IncArgs a, const uint32_t* tlist, uint64_t nt, ui
     bool aff = false;
     uint32_t v = 0, len = 0;
     if (j < nt) {
-      v = tlist[j];
+      v = tlist[off + j];
       uint32_t f = a.flags[v];
       if (!(f & kFAffected) && a.lost[v] >= a.support[v]) {
         a.flags[v] = f | kFAffected;
@@ -647,17 +674,26 @@ __global__ void k_inc_evaluate(IncArgs a, const uint32_t* tlist, uint64_t nt, ui
       base2 = __shfl(base2, 0, kWave);
       if (aff) aall[base2 + lanes_below(b)] = v;
     }
-    wave_push(&a.st->aq, alist, apre, aff && len > 0, v, len);
+    wave_push(&a.st->aq[par], alist + off, apre + off, aff && len > 0, v, len);
   }
 }
 
-// Affected u makes its live children at d(u)+1 lose a parent (bfs.hpp:247-257).
-__global__ void k_inc_emit(IncArgs a, const uint32_t* alist, const unsigned long long* apre, uint32_t* tlist,
-                           uint32_t* tall, uint32_t* tflist, unsigned long long* tfpre) {
-  const unsigned long long qc = a.st->aq;
+// Step 3: affected u makes its live children at d(u)+1 lose a parent
+// (bfs.hpp:247-257); they are touched for the next round (a.epoch is that
+// round's).  Past `limit` affected vertices the repair stops here and falls
+// back to a full traversal.
+__global__ void k_inc_emit(IncArgs a, int par, uint64_t limit, const uint32_t* alist, const unsigned long long* apre,
+                           uint32_t* tlist, uint32_t* tall, uint32_t* tflist, unsigned long long* tfpre) {
+  if (a.st->fallback) return;
+  if (a.st->aall > limit) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.st->fallback = 1;
+    return;
+  }
+  const unsigned long long qc = a.st->aq[par];
   const uint32_t W = gridDim.x * kWavesPerBlock;
   const uint32_t gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  lbs_edges(alist, apre, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.out_row, gw, W,
+  const uint64_t off = par * a.lcap;
+  lbs_edges(alist + off, apre + off, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.out_row, gw, W,
             [&](uint32_t, uint32_t u, uint2, uint32_t slot, bool valid) {
               bool want = false;
               uint32_t w = 0;
@@ -673,12 +709,12 @@ __global__ void k_inc_emit(IncArgs a, const uint32_t* alist, const unsigned long
                   w = 0;
                 }
               }
-              touch(a, w, tlist, tall, tflist, tfpre, want);
+              touch(a, par ^ 1, w, tlist, tall, tflist, tfpre, want);
             });
 }
 
-__global__ void k_inc_reset_affected(IncArgs a, const uint32_t* aall, uint64_t na, uint32_t* slist,
-                                     unsigned long long* spre) {
+__global__ void k_inc_reset_affected(IncArgs a, const uint32_t* aall, uint32_t* slist, unsigned long long* spre) {
+  const uint64_t na = a.st->aall;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   for (uint64_t j0 = base - lane_id(); j0 < na; j0 += stride) {
@@ -711,7 +747,7 @@ __global__ void k_inc_seed(IncArgs a, const uint32_t* slist, const unsigned long
             });
 }
 
-__device__ __forceinline__ void frontier_push(const IncArgs& a, int slot_par, uint32_t* fq, unsigned long long* fp,
+__device__ __forceinline__ void frontier_push(const IncArgs& a, int slot, uint32_t* fq, unsigned long long* fp,
                                               uint32_t v, bool want) {
   bool push = false;
   uint32_t len = 0;
@@ -719,11 +755,11 @@ __device__ __forceinline__ void frontier_push(const IncArgs& a, int slot_par, ui
     push = atomicExch(&a.stamp[v], a.epoch) != a.epoch;
     if (push) len = a.out_row[v].y;
   }
-  wave_push(&a.st->fq[slot_par], fq, fp, push && len > 0, v, len);
+  wave_push(&a.st->fq[slot], fq, fp, push && len > 0, v, len);
 }
 
-__global__ void k_inc_collect_seeds(IncArgs a, const uint32_t* aall, uint64_t na, uint32_t* fq,
-                                    unsigned long long* fp) {
+__global__ void k_inc_collect_seeds(IncArgs a, const uint32_t* aall, uint32_t* fq, unsigned long long* fp) {
+  const uint64_t na = a.st->aall;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   unsigned long long reach = 0;
@@ -782,10 +818,14 @@ __global__ void k_inc_relax_additions(IncArgs a, const uint32_t* fin, uint64_t n
   }
 }
 
-// propagate_decreases (bfs.hpp:341-354) as one Bellman-Ford round.
-__global__ void k_inc_propagate(IncArgs a, int par, const uint32_t* fq_cur, const unsigned long long* fp_cur,
+// propagate_decreases (bfs.hpp:341-354) as one Bellman-Ford round: round r
+// expands frontier r % 3 into (r + 1) % 3 and clears (r + 2) % 3, which
+// round r + 1 appends to (a.epoch is this round's).
+__global__ void k_inc_propagate(IncArgs a, uint32_t r, const uint32_t* fq_cur, const unsigned long long* fp_cur,
                                 uint32_t* fq_next, unsigned long long* fp_next) {
-  const unsigned long long qc = a.st->fq[par];
+  const uint32_t cur = r % 3, nxt = (r + 1) % 3;
+  const unsigned long long qc = a.st->fq[cur];
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.st->fq[(r + 2) % 3] = 0;
   const uint32_t W = gridDim.x * kWavesPerBlock;
   const uint32_t gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   unsigned long long reach = 0, relaxed = 0;
@@ -809,7 +849,7 @@ __global__ void k_inc_propagate(IncArgs a, int par, const uint32_t* fq_cur, cons
                   w = 0;
                 }
               }
-              frontier_push(a, par ^ 1, fq_next, fp_next, w, want);
+              frontier_push(a, nxt, fq_next, fp_next, w, want);
             });
   reach = wave_sum(reach);
   relaxed = wave_sum(relaxed);
@@ -819,7 +859,8 @@ __global__ void k_inc_propagate(IncArgs a, int par, const uint32_t* fq_cur, cons
   }
 }
 
-__global__ void k_inc_cleanup(IncArgs a, const uint32_t* tall, uint64_t nt) {
+__global__ void k_inc_cleanup(IncArgs a, const uint32_t* tall) {
+  const uint64_t nt = a.st->tall;
   for (uint64_t j = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; j < nt;
        j += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     uint32_t v = tall[j];
@@ -880,8 +921,11 @@ static void ensure_work(vgx_bfs* b) {
   VGX_HIP(hipMemsetAsync(w.support.p + old, 0, (c - old) * 4, g->stream));
   VGX_HIP(hipMemsetAsync(w.flags.p + old, 0, (c - old) * 4, g->stream));
   VGX_HIP(hipMemsetAsync(w.stamp.p + old, 0, (c - old) * 4, g->stream));
-  for (auto* l : {&w.tlist, &w.tflist, &w.alist, &w.aall, &w.tall, &w.slist}) l->ensure(c);
-  for (auto* l : {&w.tfpre, &w.apre, &w.spre}) l->ensure(c);
+  // the lists of the invalidation rounds have two parities of c slots
+  for (auto* l : {&w.tlist, &w.tflist, &w.alist}) l->ensure(2 * c);
+  for (auto* l : {&w.tfpre, &w.apre}) l->ensure(2 * c);
+  for (auto* l : {&w.aall, &w.tall, &w.slist}) l->ensure(c);
+  w.spre.ensure(c);
   w.cap = c;
 }
 
@@ -1179,21 +1223,31 @@ static IncArgs inc_args(vgx_bfs* b) {
   a.st = b->d_inc;
   a.epoch = b->epoch;
   a.source = b->source;
+  a.lcap = b->w.cap;
   return a;
 }
 
-static void next_epoch(vgx_bfs* b) {
-  b->epoch += 1;
-  if (b->epoch == 0) {
+// Reserve `count` fresh stamp epochs and return the first; the stamps are
+// cleared when the 32-bit epoch would wrap.
+static uint32_t take_epochs(vgx_bfs* b, uint32_t count) {
+  if (b->epoch > 0xFFFFFFFFu - count - 1) {
     VGX_HIP(hipMemsetAsync(b->w.stamp.p, 0, b->w.cap * 4, b->g->stream));
-    b->epoch = 1;
+    b->epoch = 0;
   }
+  const uint32_t first = b->epoch + 1;
+  b->epoch += count;
+  return first;
 }
 
 static void inc_sync(vgx_bfs* b) {
   VGX_HIP(hipMemcpyAsync(b->h_inc, b->d_inc, sizeof(IncState), hipMemcpyDeviceToHost, b->g->stream));
   VGX_HIP(hipStreamSynchronize(b->g->stream));
 }
+
+// Invalidation rounds and propagation rounds launched between two host
+// synchronisations; a round with nothing to do exits at once.
+constexpr uint32_t kInvalidationRounds = 2;
+constexpr uint32_t kPropagationRounds = 4;
 
 static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k) {
   b->last_candidates = 0;
@@ -1215,44 +1269,36 @@ static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, cons
   BatchScratch& bs = g->bs[0];
 
   // ---- invalidation cascade on the graph without the deleted arcs
-  next_epoch(b);
   IncArgs a = inc_args(b);
-  bool fallback = false;
+  a.epoch = take_epochs(b, 1);
   if (bs.nfinal) {
     k_inc_candidates<<<blocks_for(bs.nfinal), kBfsBlock, 0, s>>>(a, bs.fin.p, bs.nfinal, bs.skeys.p, bs.act.p,
                                                                  w.tlist.p, w.tall.p, w.tflist.p, w.tfpre.p);
     VGX_LAUNCH_CHECK();
   }
-  inc_sync(b);
-  b->last_candidates = b->h_inc->candidates;
-  uint64_t nt = b->h_inc->tq;
-  unsigned long long tfq = b->h_inc->tfq;
-  while (nt) {
-    if (tfq) {
-      k_inc_support<<<lbs_blocks(tfq & kEdgeMask), kBfsBlock, 0, s>>>(a, w.tflist.p, w.tfpre.p);
+  bool fallback = false;
+  for (uint32_t r0 = 0;; r0 += kInvalidationRounds) {
+    const uint32_t e0 = take_epochs(b, kInvalidationRounds);
+    for (uint32_t i = 0; i < kInvalidationRounds; ++i) {
+      const int par = static_cast<int>((r0 + i) & 1);
+      k_inc_support<<<kRepairBlocks, kBfsBlock, 0, s>>>(a, par, w.tflist.p, w.tfpre.p);
+      VGX_LAUNCH_CHECK();
+      k_inc_evaluate<<<kRepairBlocks, kBfsBlock, 0, s>>>(a, par, w.tlist.p, w.alist.p, w.apre.p, w.aall.p);
+      VGX_LAUNCH_CHECK();
+      IncArgs an = a;
+      an.epoch = e0 + i;  // the next round's touches
+      k_inc_emit<<<kRepairBlocks, kBfsBlock, 0, s>>>(an, par, limit, w.alist.p, w.apre.p, w.tlist.p, w.tall.p,
+                                                     w.tflist.p, w.tfpre.p);
       VGX_LAUNCH_CHECK();
     }
-    k_inc_evaluate<<<blocks_for(nt), kBfsBlock, 0, s>>>(a, w.tlist.p, nt, w.alist.p, w.apre.p, w.aall.p);
-    VGX_LAUNCH_CHECK();
     inc_sync(b);
-    if (b->h_inc->aall > limit) {
+    b->last_candidates = b->h_inc->candidates;
+    if (b->h_inc->fallback || b->h_inc->aall > limit) {
       fallback = true;
       break;
     }
-    unsigned long long aq = b->h_inc->aq;
-    if (aq == 0) break;
-    // next round
-    next_epoch(b);
-    a = inc_args(b);
-    unsigned long long zero3[3] = {0, 0, 0};
-    VGX_HIP(hipMemcpyAsync(&b->d_inc->tq, zero3, 2 * sizeof(unsigned long long), hipMemcpyHostToDevice, s));
-    k_inc_emit<<<lbs_blocks(aq & kEdgeMask), kBfsBlock, 0, s>>>(a, w.alist.p, w.apre.p, w.tlist.p, w.tall.p,
-                                                                w.tflist.p, w.tfpre.p);
-    VGX_LAUNCH_CHECK();
-    VGX_HIP(hipMemsetAsync(&b->d_inc->aq, 0, sizeof(unsigned long long), s));
-    inc_sync(b);
-    nt = b->h_inc->tq;
-    tfq = b->h_inc->tfq;
+    if (b->h_inc->tq[(r0 + kInvalidationRounds) & 1] == 0) break;  // nothing touched for the next round
+    if (r0 > n_pre + 2) throw Error(VGX_DEVICE_ERROR, "invalidation did not terminate");
   }
   const uint64_t n_aff = b->h_inc->aall;
   b->last_affected = n_aff;
@@ -1263,29 +1309,22 @@ static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, cons
   extend_dist(b);
   a = inc_args(b);
   if (fallback) {
-    inc_sync(b);
-    if (b->h_inc->tall) {
-      k_inc_cleanup<<<blocks_for(b->h_inc->tall), kBfsBlock, 0, s>>>(a, w.tall.p, b->h_inc->tall);
-      VGX_LAUNCH_CHECK();
-    }
+    k_inc_cleanup<<<kRepairBlocks, kBfsBlock, 0, s>>>(a, w.tall.p);
+    VGX_LAUNCH_CHECK();
     bfs_full(b);
     b->last_full = true;
     return;
   }
 
-  // ---- re-relaxation
-  next_epoch(b);
-  a = inc_args(b);
+  // ---- re-relaxation: seeds from unaffected in-neighbours and the added
+  // arcs, then Bellman-Ford rounds, several per synchronisation
+  a.epoch = take_epochs(b, 1);
   if (n_aff) {
-    k_inc_reset_affected<<<blocks_for(n_aff), kBfsBlock, 0, s>>>(a, w.aall.p, n_aff, w.slist.p, w.spre.p);
+    k_inc_reset_affected<<<blocks_for(n_aff), kBfsBlock, 0, s>>>(a, w.aall.p, w.slist.p, w.spre.p);
     VGX_LAUNCH_CHECK();
-    inc_sync(b);
-    unsigned long long sq = b->h_inc->sq;
-    if (sq) {
-      k_inc_seed<<<lbs_blocks(sq & kEdgeMask), kBfsBlock, 0, s>>>(a, w.slist.p, w.spre.p);
-      VGX_LAUNCH_CHECK();
-    }
-    k_inc_collect_seeds<<<blocks_for(n_aff), kBfsBlock, 0, s>>>(a, w.aall.p, n_aff, w.queue[0].p, w.pre[0].p);
+    k_inc_seed<<<kRepairBlocks, kBfsBlock, 0, s>>>(a, w.slist.p, w.spre.p);
+    VGX_LAUNCH_CHECK();
+    k_inc_collect_seeds<<<blocks_for(n_aff), kBfsBlock, 0, s>>>(a, w.aall.p, w.queue[0].p, w.pre[0].p);
     VGX_LAUNCH_CHECK();
   }
   // The staged batch of the out-direction still lists every final arc.
@@ -1294,27 +1333,25 @@ static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, cons
                                                                       bs.act.p, w.queue[0].p, w.pre[0].p);
     VGX_LAUNCH_CHECK();
   }
-  inc_sync(b);
-  int par = 0;
-  unsigned long long fq = b->h_inc->fq[0];
-  while (fq) {
-    next_epoch(b);
-    a = inc_args(b);
-    VGX_HIP(hipMemsetAsync(&b->d_inc->fq[par ^ 1], 0, sizeof(unsigned long long), s));
-    k_inc_propagate<<<lbs_blocks(fq & kEdgeMask), kBfsBlock, 0, s>>>(a, par, w.queue[par].p, w.pre[par].p,
-                                                                     w.queue[par ^ 1].p, w.pre[par ^ 1].p);
-    VGX_LAUNCH_CHECK();
+  for (uint32_t r0 = 0;; r0 += kPropagationRounds) {
+    const uint32_t e0 = take_epochs(b, kPropagationRounds);
+    for (uint32_t i = 0; i < kPropagationRounds; ++i) {
+      const uint32_t r = r0 + i;
+      IncArgs ar = a;
+      ar.epoch = e0 + i;
+      k_inc_propagate<<<kRepairBlocks, kBfsBlock, 0, s>>>(ar, r, w.queue[r & 1].p, w.pre[r & 1].p,
+                                                          w.queue[(r + 1) & 1].p, w.pre[(r + 1) & 1].p);
+      VGX_LAUNCH_CHECK();
+    }
     inc_sync(b);
-    par ^= 1;
-    fq = b->h_inc->fq[par];
+    if (b->h_inc->fq[(r0 + kPropagationRounds) % 3] == 0) break;
+    if (r0 > n_pre + 2) throw Error(VGX_DEVICE_ERROR, "propagation did not terminate");
   }
-  if (b->h_inc->tall) {
-    k_inc_cleanup<<<blocks_for(b->h_inc->tall), kBfsBlock, 0, s>>>(a, w.tall.p, b->h_inc->tall);
-    VGX_LAUNCH_CHECK();
-  }
+  // scratch flags only: the distances are final at the synchronisation above
+  k_inc_cleanup<<<kRepairBlocks, kBfsBlock, 0, s>>>(a, w.tall.p);
+  VGX_LAUNCH_CHECK();
   b->reachable = b->reachable - n_aff + b->h_inc->became_reachable;
   b->last_relaxed = b->h_inc->relaxed;
-  VGX_HIP(hipStreamSynchronize(s));
 }
 
 // ------------------------------------------------------ adaptive selector

@@ -389,88 +389,96 @@ __global__ __launch_bounds__(256) void k_insert_mid(const uint32_t* __restrict__
   }
 }
 
-// Relocation plan (one workgroup): for relocated row b (insert index
-// big[b]) record {u, j, k, old begin, old len, new begin}, its first output
-// chunk in plan[8 * nbig + b] (the total after the last), give it its new
-// range at the bump pointer and switch the row table to it.  The old range
-// stays intact for k_reloc_copy.
-__global__ __launch_bounds__(256) void k_reloc_plan(const uint32_t* __restrict__ big, uint32_t nbig,
-                                                    const uint32_t* __restrict__ ins,
-                                                    const uint32_t* __restrict__ segk,
-                                                    const unsigned long long* __restrict__ skeys,
-                                                    uint2* __restrict__ row, uint32_t* __restrict__ cap,
-                                                    uint32_t* __restrict__ deg, uint32_t* __restrict__ plan,
-                                                    GraphCounters* ctr, int dir) {
-  __shared__ unsigned long long s_scan[2][256];
-  __shared__ unsigned long long s_base[2];
-  if (threadIdx.x == 0) {
-    s_base[0] = ctr->bump[dir];  // arena offset
-    s_base[1] = 0;               // chunk offset
+// Relocation plan, step 1 (a thread per relocated row b, insert index
+// big[b]): plan[8b ..] = {u, j, k, old begin, old len, -, new capacity,
+// output chunks}.
+__global__ void k_reloc_prep(const uint32_t* __restrict__ big, uint32_t nbig, const uint32_t* __restrict__ ins,
+                             const uint32_t* __restrict__ segk, const unsigned long long* __restrict__ skeys,
+                             const uint2* __restrict__ row, uint32_t* __restrict__ plan) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nbig; b += gridDim.x * blockDim.x) {
+    const uint32_t j = big[b];
+    const uint32_t k = segk[j];
+    const uint32_t u = static_cast<uint32_t>(skeys[ins[j]] >> 32);
+    const uint2 rb = row[u];
+    const uint32_t need = rb.y + k;
+    uint4* p = reinterpret_cast<uint4*>(plan + 8ull * b);
+    p[0] = make_uint4(u, j, k, rb.x);
+    p[1] = make_uint4(rb.y, 0u, relocated_cap(need), (need + kRelocChunk - 1) / kRelocChunk);
+  }
+}
+
+// Relocation plan, step 2 (one workgroup of kPlanThreads, each a contiguous
+// run of rows): exclusive scans of the new capacities and chunk counts in
+// row order give every relocated row its new range at the bump pointer
+// (plan[8b + 5]) and its first output chunk (plan[8 * nbig + b], the total
+// after the last); the row table switches to the new range.  The old range
+// stays intact for k_reloc_copy.  Deterministic: the layout depends only on
+// the order of the relocated rows.
+constexpr uint32_t kPlanThreads = 1024;
+__global__ __launch_bounds__(kPlanThreads) void k_reloc_plan(uint32_t nbig, uint2* __restrict__ row,
+                                                             uint32_t* __restrict__ cap, uint32_t* __restrict__ deg,
+                                                             uint32_t* __restrict__ plan, GraphCounters* ctr,
+                                                             int dir) {
+  constexpr uint32_t kW = kPlanThreads / kWave;
+  __shared__ unsigned long long s_nc[kW], s_ch[kW], s_k[kW], s_bump;
+  if (threadIdx.x == 0) s_bump = ctr->bump[dir];  // thread 0 moves the bump pointer at the end
+  const uint32_t per = (nbig + kPlanThreads - 1) / kPlanThreads;
+  const uint32_t b0 = min(nbig, threadIdx.x * per), b1 = min(nbig, b0 + per);
+  unsigned long long nc = 0, ch = 0, ks = 0;
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint4 q0 = reinterpret_cast<const uint4*>(plan + 8ull * b)[0];
+    const uint4 q1 = reinterpret_cast<const uint4*>(plan + 8ull * b)[1];
+    ks += q0.z;
+    nc += q1.z;
+    ch += q1.w;
+  }
+  const unsigned lane = lane_id();
+  const uint32_t wv = threadIdx.x / kWave;
+  const unsigned long long inc_nc = wave_inclusive_scan<unsigned long long>(nc);
+  const unsigned long long inc_ch = wave_inclusive_scan<unsigned long long>(ch);
+  const unsigned long long ksum = wave_sum<unsigned long long>(ks);
+  if (lane == kWave - 1) {
+    s_nc[wv] = inc_nc;
+    s_ch[wv] = inc_ch;
+    s_k[wv] = ksum;
   }
   __syncthreads();
+  unsigned long long bnc = s_bump, bch = 0, tnc = 0, tch = 0, tk = 0;
+  for (uint32_t w = 0; w < kW; ++w) {
+    if (w < wv) {
+      bnc += s_nc[w];
+      bch += s_ch[w];
+    }
+    tnc += s_nc[w];
+    tch += s_ch[w];
+    tk += s_k[w];
+  }
+  bnc += inc_nc - nc;
+  bch += inc_ch - ch;
   uint32_t* first = plan + 8ull * nbig;
-  for (uint32_t b0 = 0; b0 < nbig; b0 += 256) {
-    const uint32_t b = b0 + threadIdx.x;
-    uint32_t j = 0, k = 0, u = 0, nc = 0, chunks = 0, need = 0;
-    uint2 rb = make_uint2(0u, 0u);
-    if (b < nbig) {
-      j = big[b];
-      k = segk[j];
-      u = static_cast<uint32_t>(skeys[ins[j]] >> 32);
-      rb = row[u];
-      need = rb.y + k;
-      nc = relocated_cap(need);
-      chunks = (need + kRelocChunk - 1) / kRelocChunk;
-    }
-    s_scan[0][threadIdx.x] = nc;
-    s_scan[1][threadIdx.x] = chunks;
-    __syncthreads();
-    for (uint32_t d = 1; d < 256; d <<= 1) {  // inclusive scans of nc and chunks
-      unsigned long long x0 = 0, x1 = 0;
-      if (threadIdx.x >= d) {
-        x0 = s_scan[0][threadIdx.x - d];
-        x1 = s_scan[1][threadIdx.x - d];
-      }
-      __syncthreads();
-      s_scan[0][threadIdx.x] += x0;
-      s_scan[1][threadIdx.x] += x1;
-      __syncthreads();
-    }
-    if (b < nbig) {
-      const unsigned long long nb = s_base[0] + s_scan[0][threadIdx.x] - nc;
-      const unsigned long long cf = s_base[1] + s_scan[1][threadIdx.x] - chunks;
-      uint32_t* p = plan + 8ull * b;
-      p[0] = u;
-      p[1] = j;
-      p[2] = k;
-      p[3] = rb.x;
-      p[4] = rb.y;
-      p[5] = static_cast<uint32_t>(nb);
-      first[b] = static_cast<uint32_t>(cf);
-      row[u] = make_uint2(static_cast<uint32_t>(nb), need);
-      cap[u] = nc;
-      atomicAdd(&deg[u], k);
-    }
-    __syncthreads();
-    if (threadIdx.x == 255) {
-      s_base[0] += s_scan[0][255];
-      s_base[1] += s_scan[1][255];
-    }
-    __syncthreads();
+  for (uint32_t b = b0; b < b1; ++b) {
+    uint32_t* p = plan + 8ull * b;
+    const uint4 q0 = reinterpret_cast<const uint4*>(p)[0];
+    const uint4 q1 = reinterpret_cast<const uint4*>(p)[1];
+    const uint32_t u = q0.x, k = q0.z;
+    p[5] = static_cast<uint32_t>(bnc);
+    first[b] = static_cast<uint32_t>(bch);
+    row[u] = make_uint2(static_cast<uint32_t>(bnc), q1.x + k);
+    cap[u] = q1.z;
+    atomicAdd(&deg[u], k);
+    bnc += q1.z;
+    bch += q1.w;
   }
   if (threadIdx.x == 0) {
-    first[nbig] = static_cast<uint32_t>(s_base[1]);
-    ctr->bump[dir] = s_base[0];
+    first[nbig] = static_cast<uint32_t>(tch);
+    ctr->bump[dir] = s_bump + tnc;
+    // live and stored arcs grow by the inserts of every relocated row
+    if (tk) {
+      atomicAdd(&ctr->live_delta[dir], tk);
+      atomicAdd(&ctr->stored_delta[dir], tk);
+    }
+    atomicAdd(&ctr->relocated[dir], static_cast<unsigned long long>(nbig));
   }
-  // live and stored arcs grow by the inserts of every relocated row
-  unsigned long long ksum = 0;
-  for (uint32_t b = threadIdx.x; b < nbig; b += 256) ksum += plan[8ull * b + 2];
-  ksum = wave_sum(ksum);
-  if (lane_id() == 0 && ksum) {
-    atomicAdd(&ctr->live_delta[dir], ksum);
-    atomicAdd(&ctr->stored_delta[dir], ksum);
-  }
-  if (threadIdx.x == 0) atomicAdd(&ctr->relocated[dir], static_cast<unsigned long long>(nbig));
 }
 
 // Output chunk c of a relocated row: the merge of the old row and its
@@ -897,8 +905,11 @@ void graph_apply_additions(vgx_graph* g) {
     }
     if (b.nbig) {
       b.reloc.ensure(9 * b.nbig + 2);
-      k_reloc_plan<<<1, 256, 0, g->stream>>>(b.big.p, static_cast<uint32_t>(b.nbig), b.ins.p, b.segk.p, b.skeys.p,
-                                             a.row.p, a.cap.p, a.deg.p, b.reloc.p, g->d_ctr, d);
+      k_reloc_prep<<<grid_for(b.nbig), kBlock, 0, g->stream>>>(b.big.p, static_cast<uint32_t>(b.nbig), b.ins.p,
+                                                                b.segk.p, b.skeys.p, a.row.p, b.reloc.p);
+      VGX_LAUNCH_CHECK();
+      k_reloc_plan<<<1, kPlanThreads, 0, g->stream>>>(static_cast<uint32_t>(b.nbig), a.row.p, a.cap.p, a.deg.p,
+                                                      b.reloc.p, g->d_ctr, d);
       VGX_LAUNCH_CHECK();
       // chunks <= sum of new capacities / chunk + one per row
       const uint64_t chunks = b.need / kRelocChunk + b.nbig;
