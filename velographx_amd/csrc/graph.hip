@@ -81,9 +81,11 @@ static int bits_for(uint64_t n) {
 // and fits its slack; a longer row that fits is merged in place by one
 // workgroup (right-to-left tiles) up to kInPlaceMax entries; rows that
 // outgrow their slots or are longer are rewritten at the bump pointer by as
-// many workgroups as they have kRelocChunk-entry chunks.
+// many workgroups as they have kRelocChunk-entry chunks.  The in-place
+// merge walks its tiles one after another, so rows beyond four tiles are
+// faster rewritten by parallel chunks.
 constexpr uint32_t kBigRow = 32;
-constexpr uint32_t kInPlaceMax = 16384;
+constexpr uint32_t kInPlaceMax = 4096;
 constexpr uint32_t kMidInsMax = 1024;  // most inserts one in-place merge takes
 constexpr uint32_t kRelocChunk = 2048;
 constexpr int kBlock = 256;
@@ -389,119 +391,77 @@ __global__ __launch_bounds__(256) void k_insert_mid(const uint32_t* __restrict__
   }
 }
 
-// Relocation plan, step 1 (a thread per relocated row b, insert index
-// big[b]): plan[8b ..] = {u, j, k, old begin, old len, -, new capacity,
-// output chunks}.
-__global__ void k_reloc_prep(const uint32_t* __restrict__ big, uint32_t nbig, const uint32_t* __restrict__ ins,
+// Relocation plan (a thread per relocated row b, insert index big[b]):
+// plan[8b ..] = {u, j, k, old begin, old len, new begin, -, first output
+// chunk}.  Each wave reserves its rows' new ranges and output chunks with
+// one atomicAdd on the bump pointer and one on the chunk counter (rows of a
+// wave are contiguous in both), and chunk_row[c] names the row that owns
+// output chunk c.  The row table switches to the new range; the old range
+// stays intact for k_reloc_copy.
+__global__ void k_reloc_plan(const uint32_t* __restrict__ big, uint32_t nbig, const uint32_t* __restrict__ ins,
                              const uint32_t* __restrict__ segk, const unsigned long long* __restrict__ skeys,
-                             const uint2* __restrict__ row, uint32_t* __restrict__ plan) {
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nbig; b += gridDim.x * blockDim.x) {
-    const uint32_t j = big[b];
-    const uint32_t k = segk[j];
-    const uint32_t u = static_cast<uint32_t>(skeys[ins[j]] >> 32);
-    const uint2 rb = row[u];
-    const uint32_t need = rb.y + k;
-    uint4* p = reinterpret_cast<uint4*>(plan + 8ull * b);
-    p[0] = make_uint4(u, j, k, rb.x);
-    p[1] = make_uint4(rb.y, 0u, relocated_cap(need), (need + kRelocChunk - 1) / kRelocChunk);
-  }
-}
-
-// Relocation plan, step 2 (one workgroup of kPlanThreads, each a contiguous
-// run of rows): exclusive scans of the new capacities and chunk counts in
-// row order give every relocated row its new range at the bump pointer
-// (plan[8b + 5]) and its first output chunk (plan[8 * nbig + b], the total
-// after the last); the row table switches to the new range.  The old range
-// stays intact for k_reloc_copy.  Deterministic: the layout depends only on
-// the order of the relocated rows.
-constexpr uint32_t kPlanThreads = 1024;
-__global__ __launch_bounds__(kPlanThreads) void k_reloc_plan(uint32_t nbig, uint2* __restrict__ row,
-                                                             uint32_t* __restrict__ cap, uint32_t* __restrict__ deg,
-                                                             uint32_t* __restrict__ plan, GraphCounters* ctr,
-                                                             int dir) {
-  constexpr uint32_t kW = kPlanThreads / kWave;
-  __shared__ unsigned long long s_nc[kW], s_ch[kW], s_k[kW], s_bump;
-  if (threadIdx.x == 0) s_bump = ctr->bump[dir];  // thread 0 moves the bump pointer at the end
-  const uint32_t per = (nbig + kPlanThreads - 1) / kPlanThreads;
-  const uint32_t b0 = min(nbig, threadIdx.x * per), b1 = min(nbig, b0 + per);
-  unsigned long long nc = 0, ch = 0, ks = 0;
-  for (uint32_t b = b0; b < b1; ++b) {
-    const uint4 q0 = reinterpret_cast<const uint4*>(plan + 8ull * b)[0];
-    const uint4 q1 = reinterpret_cast<const uint4*>(plan + 8ull * b)[1];
-    ks += q0.z;
-    nc += q1.z;
-    ch += q1.w;
-  }
-  const unsigned lane = lane_id();
-  const uint32_t wv = threadIdx.x / kWave;
-  const unsigned long long inc_nc = wave_inclusive_scan<unsigned long long>(nc);
-  const unsigned long long inc_ch = wave_inclusive_scan<unsigned long long>(ch);
-  const unsigned long long ksum = wave_sum<unsigned long long>(ks);
-  if (lane == kWave - 1) {
-    s_nc[wv] = inc_nc;
-    s_ch[wv] = inc_ch;
-    s_k[wv] = ksum;
-  }
-  __syncthreads();
-  unsigned long long bnc = s_bump, bch = 0, tnc = 0, tch = 0, tk = 0;
-  for (uint32_t w = 0; w < kW; ++w) {
-    if (w < wv) {
-      bnc += s_nc[w];
-      bch += s_ch[w];
+                             uint2* __restrict__ row, uint32_t* __restrict__ cap, uint32_t* __restrict__ deg,
+                             uint32_t* __restrict__ plan, uint32_t* __restrict__ chunk_row, GraphCounters* ctr,
+                             int dir) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t b0 = blockIdx.x * blockDim.x + threadIdx.x - lane_id(); b0 < nbig; b0 += stride) {
+    const uint32_t b = b0 + lane_id();
+    const bool in = b < nbig;
+    uint32_t j = 0, k = 0, u = 0, nc = 0, chunks = 0;
+    uint2 rb = make_uint2(0u, 0u);
+    if (in) {
+      j = big[b];
+      k = segk[j];
+      u = static_cast<uint32_t>(skeys[ins[j]] >> 32);
+      rb = row[u];
+      const uint32_t need = rb.y + k;
+      nc = relocated_cap(need);
+      chunks = (need + kRelocChunk - 1) / kRelocChunk;
     }
-    tnc += s_nc[w];
-    tch += s_ch[w];
-    tk += s_k[w];
-  }
-  bnc += inc_nc - nc;
-  bch += inc_ch - ch;
-  uint32_t* first = plan + 8ull * nbig;
-  for (uint32_t b = b0; b < b1; ++b) {
-    uint32_t* p = plan + 8ull * b;
-    const uint4 q0 = reinterpret_cast<const uint4*>(p)[0];
-    const uint4 q1 = reinterpret_cast<const uint4*>(p)[1];
-    const uint32_t u = q0.x, k = q0.z;
-    p[5] = static_cast<uint32_t>(bnc);
-    first[b] = static_cast<uint32_t>(bch);
-    row[u] = make_uint2(static_cast<uint32_t>(bnc), q1.x + k);
-    cap[u] = q1.z;
-    atomicAdd(&deg[u], k);
-    bnc += q1.z;
-    bch += q1.w;
-  }
-  if (threadIdx.x == 0) {
-    first[nbig] = static_cast<uint32_t>(tch);
-    ctr->bump[dir] = s_bump + tnc;
-    // live and stored arcs grow by the inserts of every relocated row
-    if (tk) {
-      atomicAdd(&ctr->live_delta[dir], tk);
-      atomicAdd(&ctr->stored_delta[dir], tk);
+    const unsigned long long inc_nc = wave_inclusive_scan<unsigned long long>(nc);
+    const unsigned long long inc_ch = wave_inclusive_scan<unsigned long long>(chunks);
+    const unsigned long long ksum = wave_sum<unsigned long long>(k);
+    const unsigned long long rows = __popcll(__ballot(in));
+    unsigned long long base_nc = 0, base_ch = 0;
+    if (lane_id() == kWave - 1) {
+      base_nc = atomicAdd(&ctr->bump[dir], inc_nc);
+      base_ch = atomicAdd(&ctr->reloc_chunks[dir], inc_ch);
+      // live and stored arcs grow by the inserts of every relocated row
+      atomicAdd(&ctr->live_delta[dir], ksum);
+      atomicAdd(&ctr->stored_delta[dir], ksum);
+      atomicAdd(&ctr->relocated[dir], rows);
     }
-    atomicAdd(&ctr->relocated[dir], static_cast<unsigned long long>(nbig));
+    base_nc = __shfl(base_nc, kWave - 1, kWave);
+    base_ch = __shfl(base_ch, kWave - 1, kWave);
+    if (in) {
+      const uint32_t nb = static_cast<uint32_t>(base_nc + inc_nc - nc);
+      const uint32_t cf = static_cast<uint32_t>(base_ch + inc_ch - chunks);
+      uint4* p = reinterpret_cast<uint4*>(plan + 8ull * b);
+      p[0] = make_uint4(u, j, k, rb.x);
+      p[1] = make_uint4(rb.y, nb, 0u, cf);
+      row[u] = make_uint2(nb, rb.y + k);
+      cap[u] = nc;
+      atomicAdd(&deg[u], k);
+      for (uint32_t c = 0; c < chunks; ++c) chunk_row[cf + c] = b;
+    }
   }
 }
 
 // Output chunk c of a relocated row: the merge of the old row and its
 // inserts over outputs [cj * kRelocChunk, ...), each thread a contiguous
 // eighth found by a merge-path split.  Workgroups loop over the chunks.
-__global__ __launch_bounds__(256) void k_reloc_copy(const uint32_t* __restrict__ plan, uint32_t nbig,
+__global__ __launch_bounds__(256) void k_reloc_copy(const uint32_t* __restrict__ plan,
+                                                    const uint32_t* __restrict__ chunk_row,
+                                                    const GraphCounters* __restrict__ ctr, int dir,
                                                     const uint32_t* __restrict__ ins,
                                                     const unsigned long long* __restrict__ skeys,
                                                     uint32_t* __restrict__ arena) {
-  const uint32_t* first = plan + 8ull * nbig;
-  const uint32_t total = first[nbig];
+  const uint32_t total = static_cast<uint32_t>(ctr->reloc_chunks[dir]);
   for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
-    // relocated row owning chunk c: last b with first[b] <= c
-    uint32_t lo = 0, hi = nbig;
-    while (hi - lo > 1) {
-      const uint32_t m = (lo + hi) >> 1;
-      if (first[m] <= c) lo = m;
-      else hi = m;
-    }
-    const uint32_t* p = plan + 8ull * lo;
-    const uint32_t j = p[1], k = p[2], ox = p[3], len = p[4], nb = p[5];
+    const uint32_t* p = plan + 8ull * chunk_row[c];
+    const uint32_t j = p[1], k = p[2], ox = p[3], len = p[4], nb = p[5], cf = p[7];
     const uint32_t need = len + k;
-    const uint32_t c0 = (c - first[lo]) * kRelocChunk;
+    const uint32_t c0 = (c - cf) * kRelocChunk;
     const uint32_t c1 = min(c0 + kRelocChunk, need);
     constexpr uint32_t per = kRelocChunk / 256;
     const uint32_t o0 = c0 + threadIdx.x * per;
@@ -904,17 +864,17 @@ void graph_apply_additions(vgx_graph* g) {
       VGX_LAUNCH_CHECK();
     }
     if (b.nbig) {
-      b.reloc.ensure(9 * b.nbig + 2);
-      k_reloc_prep<<<grid_for(b.nbig), kBlock, 0, g->stream>>>(b.big.p, static_cast<uint32_t>(b.nbig), b.ins.p,
-                                                                b.segk.p, b.skeys.p, a.row.p, b.reloc.p);
-      VGX_LAUNCH_CHECK();
-      k_reloc_plan<<<1, kPlanThreads, 0, g->stream>>>(static_cast<uint32_t>(b.nbig), a.row.p, a.cap.p, a.deg.p,
-                                                      b.reloc.p, g->d_ctr, d);
-      VGX_LAUNCH_CHECK();
-      // chunks <= sum of new capacities / chunk + one per row
+      // plan: 8 words per relocated row, then the owner row of every output
+      // chunk (at most need / chunk + one per row)
       const uint64_t chunks = b.need / kRelocChunk + b.nbig;
+      b.reloc.ensure(8 * b.nbig + chunks + 1);
+      uint32_t* chunk_row = b.reloc.p + 8 * b.nbig;
+      k_reloc_plan<<<grid_for(b.nbig), kBlock, 0, g->stream>>>(b.big.p, static_cast<uint32_t>(b.nbig), b.ins.p,
+                                                                b.segk.p, b.skeys.p, a.row.p, a.cap.p, a.deg.p,
+                                                                b.reloc.p, chunk_row, g->d_ctr, d);
+      VGX_LAUNCH_CHECK();
       const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(4096, std::max<uint64_t>(1, chunks)));
-      k_reloc_copy<<<grid, 256, 0, g->stream>>>(b.reloc.p, static_cast<uint32_t>(b.nbig), b.ins.p, b.skeys.p,
+      k_reloc_copy<<<grid, 256, 0, g->stream>>>(b.reloc.p, chunk_row, g->d_ctr, d, b.ins.p, b.skeys.p,
                                                 a.arena.p);
       VGX_LAUNCH_CHECK();
     }

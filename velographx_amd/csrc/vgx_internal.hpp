@@ -115,6 +115,7 @@ struct GraphCounters {
   unsigned long long nins[2];
   unsigned long long nbig[2];        // rows relocated (outgrow their slots, or very long)
   unsigned long long nmid[2];        // rows of more than 32 entries merged in place
+  unsigned long long reloc_chunks[2];  // output chunks of the relocated rows
 };
 
 // Staged update batch (device side).
