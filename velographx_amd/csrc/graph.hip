@@ -500,7 +500,7 @@ constexpr uint32_t kPackLong = 4096;
 // one wave per row; rows longer than kPackLong are left to the chunked
 // kernels.  Optionally writes the new row table for compaction.
 __global__ void k_pack_rows(const uint2* __restrict__ row, const uint32_t* __restrict__ arena, uint64_t n,
-                            const uint32_t* __restrict__ off, uint32_t* __restrict__ out,
+                            const uint32_t* __restrict__ off, uint32_t* __restrict__ out, uint64_t out_cap,
                             uint2* __restrict__ new_row, const uint32_t* __restrict__ deg) {
   const uint32_t W = gridDim.x * (blockDim.x / kWave);
   const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
@@ -515,7 +515,8 @@ __global__ void k_pack_rows(const uint2* __restrict__ row, const uint32_t* __res
       uint32_t x = in ? arena[rb.x + i + lane] : kTomb;
       bool keep = !(x & kTomb);
       unsigned long long b = __ballot(keep);
-      if (keep) out[o + lanes_below(b)] = x;
+      const uint64_t at = static_cast<uint64_t>(o) + lanes_below(b);
+      if (keep && at < out_cap) out[at] = x;  // the host checks the total
       o += __popcll(b);
     }
   }
@@ -528,17 +529,49 @@ __global__ void k_collect_long_rows(const uint2* __restrict__ row, uint64_t n, u
     if (row[u].y > kPackLong) list[atomicAdd(count, 1ull)] = static_cast<uint32_t>(u);
 }
 
-__global__ void k_gather_lens(const uint2* __restrict__ row, const uint32_t* __restrict__ list, uint64_t nl,
-                              uint32_t* __restrict__ lens) {
-  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < nl;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-    lens[i] = row[list[i]].y;
+// Chunk table of the long rows (one workgroup of kTableThreads, each a
+// contiguous run of rows): chunk c covers entries [chunk_j[c] * kPackLong,
+// ...) of row chunk_u[c]; a row's chunks are consecutive.  counts[0] holds
+// the number of long rows, counts[1] receives the number of chunks.
+constexpr uint32_t kTableThreads = 1024;
+__global__ __launch_bounds__(kTableThreads) void k_long_chunk_table(const uint2* __restrict__ row,
+                                                                    const uint32_t* __restrict__ list,
+                                                                    unsigned long long* counts,
+                                                                    uint32_t* __restrict__ chunk_u,
+                                                                    uint32_t* __restrict__ chunk_j) {
+  constexpr uint32_t kW = kTableThreads / kWave;
+  __shared__ unsigned long long s_tot[kW];
+  const uint32_t nl = static_cast<uint32_t>(counts[0]);
+  const uint32_t per = (nl + kTableThreads - 1) / kTableThreads;
+  const uint32_t r0 = min(nl, threadIdx.x * per), r1 = min(nl, r0 + per);
+  unsigned long long mine = 0;
+  for (uint32_t r = r0; r < r1; ++r) mine += (row[list[r]].y + kPackLong - 1) / kPackLong;
+  const unsigned long long inc = wave_inclusive_scan<unsigned long long>(mine);
+  const uint32_t wv = threadIdx.x / kWave;
+  if (lane_id() == kWave - 1) s_tot[wv] = inc;
+  __syncthreads();
+  unsigned long long base = inc - mine, total = 0;
+  for (uint32_t w = 0; w < kW; ++w) {
+    if (w < wv) base += s_tot[w];
+    total += s_tot[w];
+  }
+  for (uint32_t r = r0; r < r1; ++r) {
+    const uint32_t u = list[r];
+    const uint32_t nc = (row[u].y + kPackLong - 1) / kPackLong;
+    for (uint32_t q = 0; q < nc; ++q) {
+      chunk_u[base + q] = u;
+      chunk_j[base + q] = q;
+    }
+    base += nc;
+  }
+  if (threadIdx.x == 0) counts[1] = total;
 }
 
 // Live entries of chunk c (row chunk_u[c], entries [j*kPackLong, ...)).
 __global__ void k_pack_long_count(const uint2* __restrict__ row, const uint32_t* __restrict__ arena,
                                   const uint32_t* __restrict__ chunk_u, const uint32_t* __restrict__ chunk_j,
-                                  uint32_t nch, uint32_t* __restrict__ live) {
+                                  const unsigned long long* counts, uint32_t* __restrict__ live) {
+  const uint32_t nch = static_cast<uint32_t>(counts[1]);
   const uint32_t W = gridDim.x * (blockDim.x / kWave);
   for (uint32_t c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nch; c += W) {
     const uint2 rb = row[chunk_u[c]];
@@ -555,7 +588,8 @@ __global__ void k_pack_long_count(const uint2* __restrict__ row, const uint32_t*
 __global__ void k_pack_long_write(const uint2* __restrict__ row, const uint32_t* __restrict__ arena,
                                   const uint32_t* __restrict__ off, const uint32_t* __restrict__ chunk_u,
                                   const uint32_t* __restrict__ chunk_j, const uint32_t* __restrict__ live,
-                                  uint32_t nch, uint32_t* __restrict__ out) {
+                                  const unsigned long long* counts, uint32_t* __restrict__ out, uint64_t out_cap) {
+  const uint32_t nch = static_cast<uint32_t>(counts[1]);
   const uint32_t W = gridDim.x * (blockDim.x / kWave);
   const unsigned lane = lane_id();
   for (uint32_t c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nch; c += W) {
@@ -570,10 +604,20 @@ __global__ void k_pack_long_write(const uint2* __restrict__ row, const uint32_t*
       const uint32_t x = in ? arena[rb.x + i + lane] : kTomb;
       const bool keep = !(x & kTomb);
       const unsigned long long m = __ballot(keep);
-      if (keep) out[o + lanes_below(m)] = x;
+      const uint64_t at = static_cast<uint64_t>(o) + lanes_below(m);
+      if (keep && at < out_cap) out[at] = x;  // the host checks the total
       o += __popcll(m);
     }
   }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+// bump pointer of direction dir := the packed total off[n]
+__global__ void k_set_bump(const uint32_t* off, uint64_t n, GraphCounters* ctr, int dir) {
+  if (threadIdx.x == 0) ctr->bump[dir] = off[n];
 }
 
 __global__ void k_compact_caps(const uint32_t* __restrict__ deg, uint64_t n, uint32_t* __restrict__ cap) {
@@ -913,50 +957,39 @@ static void graph_apply(vgx_graph* g, const uint32_t* src, const uint32_t* dst, 
 
 // ------------------------------------------------------ snapshots, compaction
 
-// Chunked copy of the rows longer than kPackLong (there are few: the chunk
-// table is built on the host).
-static void pack_long_rows(vgx_graph* g, Adjacency& a, const uint32_t* off, uint32_t* out) {
+// Chunked copy of the rows longer than kPackLong; the chunk table is built
+// on the device, so packing needs no host round trip.  `stored` bounds the
+// entries of all rows (the arena's size does).
+static void pack_long_rows(vgx_graph* g, Adjacency& a, const uint32_t* off, uint32_t* out, uint64_t out_cap,
+                           uint64_t stored) {
   hipStream_t s = g->stream;
   const uint64_t n = g->n;
-  DevBuf<uint32_t> list;
-  DevBuf<unsigned long long> cnt;
-  list.ensure(n + 1);
-  cnt.ensure(1);
-  VGX_HIP(hipMemsetAsync(cnt.p, 0, 8, s));
-  k_collect_long_rows<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(a.row.p, n, list.p, cnt.p);
+  PackScratch& pk = g->pk;
+  const uint64_t max_chunks = 2 * (stored / kPackLong) + 2;
+  pk.long_list.ensure(n + 1);
+  pk.chunk_u.ensure(max_chunks);
+  pk.chunk_j.ensure(max_chunks);
+  pk.chunk_live.ensure(max_chunks);
+  pk.counts.ensure(2);
+  k_zero_u64<<<1, 64, 0, s>>>(pk.counts.p, 2);
   VGX_LAUNCH_CHECK();
-  unsigned long long nl = 0;
-  VGX_HIP(hipMemcpyAsync(&nl, cnt.p, 8, hipMemcpyDeviceToHost, s));
-  VGX_HIP(hipStreamSynchronize(s));
-  if (nl == 0) return;
-  DevBuf<uint32_t> lens;
-  lens.ensure(nl);
-  k_gather_lens<<<grid_for(nl, kBlock, 1024), kBlock, 0, s>>>(a.row.p, list.p, nl, lens.p);
+  k_collect_long_rows<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(a.row.p, n, pk.long_list.p, pk.counts.p);
   VGX_LAUNCH_CHECK();
-  std::vector<uint32_t> rows(nl), len(nl);
-  VGX_HIP(hipMemcpyAsync(rows.data(), list.p, nl * 4, hipMemcpyDeviceToHost, s));
-  VGX_HIP(hipMemcpyAsync(len.data(), lens.p, nl * 4, hipMemcpyDeviceToHost, s));
-  VGX_HIP(hipStreamSynchronize(s));
-  std::vector<uint32_t> cu, cj;
-  for (uint64_t r = 0; r < nl; ++r)
-    for (uint32_t j = 0; static_cast<uint64_t>(j) * kPackLong < len[r]; ++j) {
-      cu.push_back(rows[r]);
-      cj.push_back(j);
-    }
-  const uint32_t nch = static_cast<uint32_t>(cu.size());
-  DevBuf<uint32_t> du, dj, live;
-  du.ensure(nch);
-  dj.ensure(nch);
-  live.ensure(nch);
-  VGX_HIP(hipMemcpyAsync(du.p, cu.data(), nch * 4, hipMemcpyHostToDevice, s));
-  VGX_HIP(hipMemcpyAsync(dj.p, cj.data(), nch * 4, hipMemcpyHostToDevice, s));
-  k_pack_long_count<<<grid_for(nch, 4, 4096), kBlock, 0, s>>>(a.row.p, a.arena.p, du.p, dj.p, nch, live.p);
+  k_long_chunk_table<<<1, kTableThreads, 0, s>>>(a.row.p, pk.long_list.p, pk.counts.p, pk.chunk_u.p, pk.chunk_j.p);
   VGX_LAUNCH_CHECK();
-  k_pack_long_write<<<grid_for(nch, 4, 4096), kBlock, 0, s>>>(a.row.p, a.arena.p, off, du.p, dj.p, live.p, nch, out);
+  const unsigned grid = grid_for(max_chunks, 4, 2048);
+  k_pack_long_count<<<grid, kBlock, 0, s>>>(a.row.p, a.arena.p, pk.chunk_u.p, pk.chunk_j.p, pk.counts.p,
+                                            pk.chunk_live.p);
   VGX_LAUNCH_CHECK();
-  VGX_HIP(hipStreamSynchronize(s));
+  k_pack_long_write<<<grid, kBlock, 0, s>>>(a.row.p, a.arena.p, off, pk.chunk_u.p, pk.chunk_j.p, pk.chunk_live.p,
+                                            pk.counts.p, out, out_cap);
+  VGX_LAUNCH_CHECK();
 }
 
+// Pack the live entries of every row of direction d into `out` at the
+// offsets of an inclusive scan of `caps` (the live degrees when null),
+// writing the new row table when asked.  `out` is sized from a host bound,
+// so packing needs no host round trip.
 static void pack_direction(vgx_graph* g, int d, DevBuf<uint32_t>& off, DevBuf<uint32_t>& out, uint2* new_row,
                            const uint32_t* caps) {
   Adjacency& a = g->adj[d];
@@ -969,44 +1002,45 @@ static void pack_direction(vgx_graph* g, int d, DevBuf<uint32_t>& off, DevBuf<ui
       return hipcub::DeviceScan::InclusiveSum(t, b, caps ? caps : a.deg.p, off.p + 1, static_cast<int>(n), s);
     });
   }
-  uint32_t total = 0;
+  // capacities are deg + deg/8 + 2 (k_compact_caps), plus slack for the
+  // next inserts; a snapshot holds exactly the live arcs
+  const uint64_t bound = caps ? a.live + a.live / 8 + 2 * n : a.live;
+  out.ensure(caps ? bound + bound / 8 + (1u << 16) : bound + 1);
   if (n) {
-    VGX_HIP(hipMemcpyAsync(&total, off.p + n, sizeof(total), hipMemcpyDeviceToHost, s));
-    VGX_HIP(hipStreamSynchronize(s));
-  }
-  uint64_t want = caps ? static_cast<uint64_t>(total) + total / 4 + (1u << 16) : static_cast<uint64_t>(total) + 1;
-  out.ensure(want);
-  if (n) {
-    k_pack_rows<<<grid_for(n, 4, 16384), kBlock, 0, s>>>(a.row.p, a.arena.p, n, off.p, out.p, new_row, a.deg.p);
+    k_pack_rows<<<grid_for(n, 4, 16384), kBlock, 0, s>>>(a.row.p, a.arena.p, n, off.p, out.p, out.n, new_row,
+                                                         a.deg.p);
     VGX_LAUNCH_CHECK();
-    pack_long_rows(g, a, off.p, out.p);
-  }
-  if (caps) {
-    a.used = total;
+    pack_long_rows(g, a, off.p, out.p, out.n, a.arena.n);
   }
 }
 
+// Consolidation: every row is repacked without tombstones into the spare
+// arena with capacity deg + deg/8 + 2, and the spare and live arena (and
+// row tables) trade places.  One host synchronisation reads the new bump
+// pointer.
 static void compact_direction(vgx_graph* g, int d) {
   Adjacency& a = g->adj[d];
   hipStream_t s = g->stream;
   const uint64_t n = g->n;
   if (n == 0) return;
+  PackScratch& pk = g->pk;
   k_compact_caps<<<grid_for(n), kBlock, 0, s>>>(a.deg.p, n, a.cap.p);
   VGX_LAUNCH_CHECK();
-  DevBuf<uint32_t> off, out;
-  DevBuf<uint2> nrow;
-  nrow.ensure(g->ncap);
-  VGX_HIP(hipMemsetAsync(nrow.p, 0, g->ncap * sizeof(uint2), s));
-  pack_direction(g, d, off, out, nrow.p, a.cap.p);
-  VGX_HIP(hipStreamSynchronize(s));
-  a.arena.swap(out);
+  pk.spare_row.ensure(g->ncap);
+  VGX_HIP(hipMemsetAsync(pk.spare_row.p, 0, g->ncap * sizeof(uint2), s));
+  pack_direction(g, d, pk.off, pk.spare_arena, pk.spare_row.p, a.cap.p);
+  k_set_bump<<<1, 64, 0, s>>>(pk.off.p, n, g->d_ctr, d);
+  VGX_LAUNCH_CHECK();
+  a.arena.swap(pk.spare_arena);
   a.acap = a.arena.n;
-  a.row.swap(nrow);
+  a.row.swap(pk.spare_row);
   a.stored = a.live;
   a.relocated = 0;
-  unsigned long long bump = a.used;
-  VGX_HIP(hipMemcpyAsync(&g->d_ctr->bump[d], &bump, sizeof(bump), hipMemcpyHostToDevice, s));
+  uint32_t total = 0;
+  VGX_HIP(hipMemcpyAsync(&total, pk.off.p + n, sizeof(total), hipMemcpyDeviceToHost, s));
   VGX_HIP(hipStreamSynchronize(s));
+  if (total > a.arena.n) throw Error(VGX_DEVICE_ERROR, "compaction exceeded its arena bound");
+  a.used = total;
 }
 
 const CsrSnapshot& graph_snapshot(vgx_graph* g, int dir) {
@@ -1014,7 +1048,10 @@ const CsrSnapshot& graph_snapshot(vgx_graph* g, int dir) {
   CsrSnapshot& sn = g->snap[d];
   if (sn.version == g->version && sn.n == g->n) return sn;
   pack_direction(g, d, sn.off, sn.adj, nullptr, nullptr);
+  uint32_t total = 0;
+  if (g->n) VGX_HIP(hipMemcpyAsync(&total, sn.off.p + g->n, sizeof(total), hipMemcpyDeviceToHost, g->stream));
   VGX_HIP(hipStreamSynchronize(g->stream));
+  if (total > sn.adj.n) throw Error(VGX_DEVICE_ERROR, "snapshot exceeded its bound");
   sn.version = g->version;
   sn.n = g->n;
   sn.arcs = g->adj[d].live;

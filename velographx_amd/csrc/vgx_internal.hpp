@@ -174,6 +174,21 @@ struct PprWork {
 
 }  // namespace vgx
 
+namespace vgx {
+// Scratch of snapshot packing and compaction, kept between calls so that a
+// consolidation allocates nothing once the sizes have been seen: the spare
+// arena and row table receive the compacted copy and are swapped with the
+// live ones (the old ones become the spares).
+struct PackScratch {
+  DevBuf<uint32_t> off;          // row offsets of the packed copy (n + 2)
+  DevBuf<uint32_t> spare_arena;
+  DevBuf<uint2> spare_row;
+  DevBuf<uint32_t> long_list;    // rows longer than the chunked-packing limit
+  DevBuf<uint32_t> chunk_u, chunk_j, chunk_live;
+  DevBuf<unsigned long long> counts;  // [0] long rows, [1] chunks
+};
+}  // namespace vgx
+
 struct vgx_graph {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -188,6 +203,7 @@ struct vgx_graph {
   vgx::HostBuf stage;
   vgx::CsrSnapshot snap[2];
   vgx::PprWork ppr;
+  vgx::PackScratch pk;
   int ndir() const { return directed ? 2 : 1; }
   vgx::Adjacency& in() { return directed ? adj[1] : adj[0]; }
 };
