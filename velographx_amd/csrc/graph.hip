@@ -496,28 +496,59 @@ __global__ __launch_bounds__(256) void k_reloc_copy(const uint32_t* __restrict__
 // serialise one wave.
 constexpr uint32_t kPackLong = 4096;
 
-// Copy the live entries of every row to out + off[u] (order preserved);
-// one wave per row; rows longer than kPackLong are left to the chunked
-// kernels.  Optionally writes the new row table for compaction.
+// Copy the live entries of every row to out + off[u] (order preserved).  A
+// wave takes 64 rows at a time, W apart (W = waves in the grid, so the
+// long rows of low ids, which R-MAT clusters, spread over many waves): rows
+// of up to kPackShort entries are copied by their own lane, eight entries
+// per step with all eight loads in flight, longer rows by the whole wave in
+// turn; rows longer than kPackLong are left to the chunked kernels.
+// Optionally writes the new row table for compaction.
+constexpr uint32_t kPackShort = 32;
 __global__ void k_pack_rows(const uint2* __restrict__ row, const uint32_t* __restrict__ arena, uint64_t n,
                             const uint32_t* __restrict__ off, uint32_t* __restrict__ out, uint64_t out_cap,
                             uint2* __restrict__ new_row, const uint32_t* __restrict__ deg) {
-  const uint32_t W = gridDim.x * (blockDim.x / kWave);
-  const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const uint64_t W = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+  const uint64_t gw = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave;
   const unsigned lane = lane_id();
-  for (uint64_t u = gw; u < n; u += W) {
-    uint2 rb = row[u];
-    uint32_t o = off[u];
-    if (new_row != nullptr && lane == 0) new_row[u] = make_uint2(off[u], deg[u]);
-    if (rb.y > kPackLong) continue;
-    for (uint32_t i = 0; i < rb.y; i += kWave) {
-      bool in = i + lane < rb.y;
-      uint32_t x = in ? arena[rb.x + i + lane] : kTomb;
-      bool keep = !(x & kTomb);
-      unsigned long long b = __ballot(keep);
-      const uint64_t at = static_cast<uint64_t>(o) + lanes_below(b);
-      if (keep && at < out_cap) out[at] = x;  // the host checks the total
-      o += __popcll(b);
+  for (uint64_t i0 = 0; gw + W * i0 < n; i0 += kWave) {
+    const uint64_t u = gw + W * (i0 + lane);
+    uint2 rb = make_uint2(0u, 0u);
+    uint32_t o = 0;
+    if (u < n) {
+      rb = row[u];
+      o = off[u];
+      if (new_row != nullptr) new_row[u] = make_uint2(o, deg[u]);
+    }
+    const bool shrt = rb.y <= kPackShort;
+    const uint32_t cnt = shrt ? rb.y : 0u;
+    const uint32_t most = wave_max(cnt);
+    uint64_t at = o;
+    for (uint32_t j = 0; j < most; j += 8) {
+      uint32_t x[8];
+#pragma unroll
+      for (uint32_t q = 0; q < 8; ++q) x[q] = j + q < cnt ? arena[rb.x + j + q] : kTomb;
+#pragma unroll
+      for (uint32_t q = 0; q < 8; ++q)
+        if (!(x[q] & kTomb)) {
+          if (at < out_cap) out[at] = x[q];  // the host checks the total
+          ++at;
+        }
+    }
+    unsigned long long lm = __ballot(!shrt && rb.y <= kPackLong);
+    while (lm) {
+      const int l = __ffsll(static_cast<long long>(lm)) - 1;
+      lm &= lm - 1;
+      const uint32_t bx = __shfl(rb.x, l, kWave), by = __shfl(rb.y, l, kWave);
+      uint64_t ol = __shfl(o, l, kWave);
+      for (uint32_t i = 0; i < by; i += kWave) {
+        const bool in = i + lane < by;
+        const uint32_t x = in ? arena[bx + i + lane] : kTomb;
+        const bool keep = !(x & kTomb);
+        const unsigned long long b = __ballot(keep);
+        const uint64_t pos = ol + lanes_below(b);
+        if (keep && pos < out_cap) out[pos] = x;  // the host checks the total
+        ol += __popcll(b);
+      }
     }
   }
 }
@@ -1004,11 +1035,12 @@ static void pack_direction(vgx_graph* g, int d, DevBuf<uint32_t>& off, DevBuf<ui
   }
   // capacities are deg + deg/8 + 2 (k_compact_caps), plus slack for the
   // next inserts; a snapshot holds exactly the live arcs
-  const uint64_t bound = caps ? a.live + a.live / 8 + 2 * n : a.live;
-  out.ensure(caps ? bound + bound / 8 + (1u << 16) : bound + 1);
+  const uint64_t bound = (caps ? a.live + a.live / 8 + 2 * n : a.live) + 1;
+  // grown with slack, so the small growth of later batches reuses it
+  if (out.n < bound) out.ensure(bound + bound / 8 + (1u << 16));
   if (n) {
-    k_pack_rows<<<grid_for(n, 4, 16384), kBlock, 0, s>>>(a.row.p, a.arena.p, n, off.p, out.p, out.n, new_row,
-                                                         a.deg.p);
+    k_pack_rows<<<grid_for(n, 4 * kWave, 4096), kBlock, 0, s>>>(a.row.p, a.arena.p, n, off.p, out.p, out.n,
+                                                                new_row, a.deg.p);
     VGX_LAUNCH_CHECK();
     pack_long_rows(g, a, off.p, out.p, out.n, a.arena.n);
   }

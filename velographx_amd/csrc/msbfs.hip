@@ -8,7 +8,9 @@
 // when v is at distance L from root r, so every root's distances equal its
 // single-source BFS.
 //
-// Each level runs in one of two directions over the packed CSR snapshots:
+// Each level runs in one of two directions over the graph's live row tables
+// (sorted rows, deleted entries tombstoned, so an update batch needs no
+// repacked copy):
 //   push: every frontier vertex u ORs its new bits, minus what the target
 //         has seen, into next[v] of its out-neighbours (64-bit atomicOr);
 //   pull: every unfinished vertex ORs the frontier masks of its
@@ -73,13 +75,16 @@ struct MsCtl {
 };
 
 // Everything a level kernel needs, in device memory (rewritten per pass).
+// The passes read the graph's live row tables and arenas (rows sorted,
+// deleted entries tombstoned), so an update batch needs no repacked copy.
 struct MsArgs {
-  const uint32_t* out_off;
-  const uint32_t* out_adj;
-  const uint32_t* out_rc;  // long-row chunk plan of the out-CSR
+  const uint2* out_row;
+  const uint32_t* out_arena;
+  const uint32_t* out_deg;  // live out-degrees
+  const uint32_t* out_rc;   // long-row chunk plan of the out-rows
   const uint32_t* out_cr;
-  const uint32_t* in_off;
-  const uint32_t* in_adj;
+  const uint2* in_row;
+  const uint32_t* in_arena;
   const uint32_t* in_rc;
   const uint32_t* in_cr;
   uint32_t out_nch, in_nch;
@@ -95,11 +100,12 @@ struct MsArgs {
   MsCtl* ctl;
 };
 
-// Chunks of the rows longer than kMsShortRow in one snapshot: row v owns
-// chunks [row_chunk[v], row_chunk[v+1]).
+// Chunks of the rows longer than kMsShortRow in one direction's row table:
+// row v owns chunks [row_chunk[v], row_chunk[v+1]).
 struct MsPlan {
   DevBuf<uint32_t> row_chunk;
   DevBuf<uint32_t> chunk_row;
+  DevBuf<uint32_t> cnt;  // chunks per row (scratch of the rebuild)
   uint64_t nchunks = 0;
   uint64_t version = ~0ull;
   uint64_t n = 0;
@@ -175,9 +181,9 @@ __global__ void k_ms_start(const MsArgs* __restrict__ pa, int fresh, uint32_t fi
   }
 }
 
-__global__ void k_ms_chunk_count(const uint32_t* __restrict__ off, uint32_t n, uint32_t* __restrict__ cnt) {
+__global__ void k_ms_chunk_count(const uint2* __restrict__ row, uint32_t n, uint32_t* __restrict__ cnt) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    const uint32_t len = off[v + 1] - off[v];
+    const uint32_t len = row[v].y;
     cnt[v] = len > kMsShortRow ? (len + kMsChunk - 1) / kMsChunk : 0u;
   }
 }
@@ -202,10 +208,11 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_push_short(const MsArgs* __rest
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < a.n; u += gridDim.x * blockDim.x) {
     const unsigned long long f = frontier[u];
     if (!f) continue;
-    const uint32_t b = a.out_off[u], e = a.out_off[u + 1];
-    if (e - b > kMsShortRow) continue;
-    for (uint32_t i = b; i < e; ++i) {
-      const uint32_t v = a.out_adj[i];
+    const uint2 rb = a.out_row[u];
+    if (rb.y > kMsShortRow) continue;
+    for (uint32_t i = rb.x; i < rb.x + rb.y; ++i) {
+      const uint32_t v = a.out_arena[i];
+      if (v & kTomb) continue;
       const unsigned long long m = f & ~a.seen[v];
       // bits only ever get added to next[v]: skip the atomic when they are there
       if (m & ~next[v]) atomicOr(&next[v], m);
@@ -223,14 +230,23 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_push_long(const MsArgs* __restr
     const uint32_t u = a.out_cr[c];
     const unsigned long long f = frontier[u];
     if (!f) continue;
-    const uint32_t b = a.out_off[u] + (c - a.out_rc[u]) * kMsChunk;
-    const uint32_t e = min(b + kMsChunk, a.out_off[u + 1]);
+    const uint2 rb = a.out_row[u];
+    const uint32_t b = rb.x + (c - a.out_rc[u]) * kMsChunk;
+    const uint32_t e = min(b + kMsChunk, rb.x + rb.y);
     for (uint32_t i = b + lane_id(); i < e; i += kWave) {
-      const uint32_t v = a.out_adj[i];
+      const uint32_t v = a.out_arena[i];
+      if (v & kTomb) continue;
       const unsigned long long m = f & ~a.seen[v];
       if (m & ~next[v]) atomicOr(&next[v], m);
     }
   }
+}
+
+// Frontier mask of parent entry x (nothing for a deleted entry, whose id
+// bits still name a vertex).
+__device__ __forceinline__ unsigned long long in_mask(const unsigned long long* frontier, uint32_t x) {
+  const unsigned long long f = frontier[x & kIdMask];
+  return (x & kTomb) ? 0ull : f;
 }
 
 // pull, short rows: lane per unfinished vertex v, early exit once every
@@ -244,17 +260,19 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_pull_short(const MsArgs* __rest
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
     const unsigned long long s = a.seen[v];
     if (s == all) continue;
-    const uint32_t b = a.in_off[v], e = a.in_off[v + 1];
-    if (e - b > kMsShortRow) continue;
-    const uint32_t* adj = a.in_adj;
+    const uint2 rb = a.in_row[v];
+    if (rb.y > kMsShortRow) continue;
+    const uint32_t* adj = a.in_arena;
+    const uint32_t b = rb.x, e = rb.x + rb.y;
     unsigned long long acc = 0;
     uint32_t i = b;
     for (; i + 4 <= e; i += 4) {
-      acc |= frontier[adj[i]] | frontier[adj[i + 1]] | frontier[adj[i + 2]] | frontier[adj[i + 3]];
+      acc |= in_mask(frontier, adj[i]) | in_mask(frontier, adj[i + 1]) | in_mask(frontier, adj[i + 2]) |
+             in_mask(frontier, adj[i + 3]);
       if ((acc | s) == all) break;
     }
     if ((acc | s) != all)
-      for (; i < e; ++i) acc |= frontier[adj[i]];
+      for (; i < e; ++i) acc |= in_mask(frontier, adj[i]);
     const unsigned long long m = acc & ~s;
     if (m) next[v] = m;  // the only writer of a short row's entry
   }
@@ -277,14 +295,15 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_pull_long(const MsArgs* __restr
     const uint32_t v = a.in_cr[c];
     const unsigned long long s = a.seen[v];
     if (s == all) continue;
-    const uint32_t b = a.in_off[v] + (c - a.in_rc[v]) * kMsChunk;
-    const uint32_t e = min(b + kMsChunk, a.in_off[v + 1]);
+    const uint2 rb = a.in_row[v];
+    const uint32_t b = rb.x + (c - a.in_rc[v]) * kMsChunk;
+    const uint32_t e = min(b + kMsChunk, rb.x + rb.y);
     unsigned long long acc = 0;
     for (uint32_t i0 = b; i0 < e; i0 += 4 * kWave) {
 #pragma unroll
       for (uint32_t j = 0; j < 4; ++j) {
         const uint32_t i = i0 + j * kWave + lane_id();
-        if (i < e) acc |= frontier[a.in_adj[i]];
+        if (i < e) acc |= in_mask(frontier, a.in_arena[i]);
       }
       acc = wave_or(acc);
       if ((acc | s) == all) break;
@@ -308,7 +327,7 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_finalize(const MsArgs* __restri
     if (!nw) continue;
     a.seen[v] |= nw;
     pairs += __popcll(nw);
-    arcs += a.out_off[v + 1] - a.out_off[v];
+    arcs += a.out_deg[v];
   }
   __shared__ unsigned long long sp[kMsWaves], sa[kMsWaves];
   pairs = wave_sum(pairs);
@@ -422,18 +441,20 @@ static void ms_ensure_levels(vgx_msbfs* m, uint64_t need) {
   m->levels_cap = cap;
 }
 
-// Long-row chunks of one snapshot, rebuilt when the snapshot changes.
-static const MsPlan& ms_plan(vgx_msbfs* m, int dir, const CsrSnapshot& sn, hipStream_t s) {
+// Long-row chunks of one direction's row table, rebuilt when the graph
+// changes (chunks cover stored entries, tombstones included).
+static const MsPlan& ms_plan(vgx_msbfs* m, int dir, const Adjacency& adj, uint64_t version, uint64_t nv,
+                             hipStream_t s) {
   MsPlan& pl = m->plan[dir];
-  if (pl.version == sn.version && pl.n == sn.n && pl.row_chunk.p) return pl;
-  const uint32_t n = static_cast<uint32_t>(sn.n);
-  pl.row_chunk.ensure(sn.n + 2);
+  if (pl.version == version && pl.n == nv && pl.row_chunk.p) return pl;
+  const uint32_t n = static_cast<uint32_t>(nv);
+  pl.row_chunk.ensure(nv + 2);
   VGX_HIP(hipMemsetAsync(pl.row_chunk.p, 0, sizeof(uint32_t), s));
   uint32_t total = 0;
   if (n) {
-    DevBuf<uint32_t> cnt;
-    cnt.ensure(sn.n + 1);
-    k_ms_chunk_count<<<ms_blocks(n), kMsBlock, 0, s>>>(sn.off.p, n, cnt.p);
+    DevBuf<uint32_t>& cnt = pl.cnt;
+    cnt.ensure(nv + 1);
+    k_ms_chunk_count<<<ms_blocks(n), kMsBlock, 0, s>>>(adj.row.p, n, cnt.p);
     VGX_LAUNCH_CHECK();
     size_t bytes = 0;
     VGX_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, cnt.p, pl.row_chunk.p + 1, static_cast<int>(n), s));
@@ -446,8 +467,8 @@ static const MsPlan& ms_plan(vgx_msbfs* m, int dir, const CsrSnapshot& sn, hipSt
     VGX_LAUNCH_CHECK();
   }
   pl.nchunks = total;
-  pl.version = sn.version;
-  pl.n = sn.n;
+  pl.version = version;
+  pl.n = nv;
   return pl;
 }
 
@@ -487,10 +508,10 @@ static void ms_drop_graph(vgx_msbfs* m) {
 static void ms_run(vgx_msbfs* m) {
   vgx_graph* g = m->g;
   hipStream_t s = g->stream;
-  const CsrSnapshot& out = graph_snapshot(g, 0);
-  const CsrSnapshot& in = graph_snapshot(g, 1);
-  const MsPlan& po = ms_plan(m, 0, out, s);
-  const MsPlan& pi = ms_plan(m, 1, in, s);
+  const Adjacency& out = g->adj[0];
+  const Adjacency& in = g->in();
+  const MsPlan& po = ms_plan(m, 0, out, g->version, g->n, s);
+  const MsPlan& pi = g->directed ? ms_plan(m, 1, in, g->version, g->n, s) : po;  // undirected: one table, one plan
   const uint64_t n = g->n;
   if (n != m->n) {  // the per-level arrays are laid out with stride n
     m->levels_cap = 0;
@@ -520,13 +541,14 @@ static void ms_run(vgx_msbfs* m) {
   for (;;) {
     ms_ensure_levels(m, base + kMsProgram + 2);
     MsArgs a{};
-    a.out_off = out.off.p;
-    a.out_adj = out.adj.p;
+    a.out_row = out.row.p;
+    a.out_arena = out.arena.p;
+    a.out_deg = out.deg.p;
     a.out_rc = po.row_chunk.p;
     a.out_cr = po.chunk_row.p;
     a.out_nch = static_cast<uint32_t>(po.nchunks);
-    a.in_off = in.off.p;
-    a.in_adj = in.adj.p;
+    a.in_row = in.row.p;
+    a.in_arena = in.arena.p;
     a.in_rc = pi.row_chunk.p;
     a.in_cr = pi.chunk_row.p;
     a.in_nch = static_cast<uint32_t>(pi.nchunks);
@@ -537,7 +559,7 @@ static void ms_run(vgx_msbfs* m) {
     a.k = m->k;
     a.base = base;
     a.all = m->k == 64 ? ~0ull : ((1ull << m->k) - 1);
-    a.arcs = static_cast<double>(out.arcs);
+    a.arcs = static_cast<double>(out.live);
     a.alpha = m->alpha;
     a.ctl = m->d_ctl;
     VGX_HIP(hipMemcpyAsync(m->d_args, &a, sizeof(a), hipMemcpyHostToDevice, s));

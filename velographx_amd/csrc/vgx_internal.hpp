@@ -57,10 +57,14 @@ struct DevBuf {
     p = nullptr;
     n = 0;
   }
+  // Capacity for m elements (contents not kept).  A buffer that has to grow
+  // takes a quarter more than asked, so sizes that creep up from call to
+  // call (batches, snapshots) do not reallocate every time.
   void ensure(size_t m) {
     if (m <= n) return;
+    const size_t grown = n ? m + m / 4 : m;
     release();
-    size_t c = m < 1024 ? 1024 : m;
+    size_t c = grown < 1024 ? 1024 : grown;
     p = static_cast<T*>(dev_alloc(c * sizeof(T)));
     n = c;
   }
