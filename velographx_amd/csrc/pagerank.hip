@@ -9,9 +9,10 @@
 //   validate_against_full / apply_validated :204-258
 // plus the static push pagerank of src/algorithms.cpp:56-81 (L1 stop).
 //
-// One iteration is one launch over the packed in-CSR snapshot.  Vertices
-// are split by in-degree: rows of <= 32 arcs are summed by one lane, rows up
-// to 4096 by one wave, larger rows in 4096-arc chunks by several waves whose
+// One iteration is one launch over the graph's live in-rows (deleted
+// entries tombstoned and skipped).  Vertices are split by stored row length:
+// rows of <= 32 entries are summed by one lane, rows up to 4096 by one wave,
+// larger rows in 4096-entry chunks by several waves whose
 // partial sums the last workgroup combines.  The last workgroup also reduces
 // the per-workgroup residual and dangling partials in a fixed order, sets
 // the next iteration's dangling share and the convergence flag, so the host
@@ -68,6 +69,7 @@ struct vgx_pagerank {
   vgx::DevBuf<uint32_t> slist, mlist, hlist;
   vgx::DevBuf<uint32_t> hchunk_v, hchunk_off;  // chunk -> H index, chunk -> arc start
   vgx::DevBuf<uint32_t> hchunk_first;          // H index -> first chunk (size nh+1)
+  vgx::DevBuf<uint32_t> class_counts, class_bases, class_tail;  // list building scratch
   uint64_t ns = 0, nm = 0, nh = 0, nchunks = 0;
   vgx::DevBuf<uint8_t> rank[2], contrib[2], hpart, ref_rank[2];
   vgx::DevBuf<vgx::PrPartial> partials;
@@ -103,9 +105,12 @@ struct Acc<float> {
   __device__ float get() const { return s; }
 };
 
+// The sweeps read the graph's live in-rows (sorted, deleted entries
+// tombstoned), so an update batch needs no repacked copy; a row's sum runs
+// over its live entries in row order whatever tombstones it holds.
 struct PrArgs {
-  const uint32_t* off;
-  const uint32_t* adj;
+  const uint2* row;
+  const uint32_t* arena;
   const uint32_t* outdeg;
   const uint32_t* slist;
   const uint32_t* mlist;
@@ -180,7 +185,11 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_iter(PrArgs a) {
     for (uint32_t i = b * kPrBlock + threadIdx.x; i < a.ns; i += a.bs * kPrBlock) {
       const uint32_t v = a.slist[i];
       Acc<T> acc;
-      for (uint32_t e = a.off[v]; e < a.off[v + 1]; ++e) acc.add(contrib[a.adj[e]]);
+      const uint2 rb = a.row[v];
+      for (uint32_t e = rb.x; e < rb.x + rb.y; ++e) {
+        const uint32_t x = a.arena[e];
+        if (!(x & kTomb)) acc.add(contrib[x]);
+      }
       finish_vertex<T>(a, v, acc.get(), base, rank, next, cnext, l1, linf, dang);
     }
   } else if (b < a.bs + a.bm) {
@@ -188,7 +197,11 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_iter(PrArgs a) {
          i += a.bm * (kPrBlock / kWave)) {
       const uint32_t v = a.mlist[i];
       Acc<T> acc;
-      for (uint32_t e = a.off[v] + lane_id(); e < a.off[v + 1]; e += kWave) acc.add(contrib[a.adj[e]]);
+      const uint2 rb = a.row[v];
+      for (uint32_t e = rb.x + lane_id(); e < rb.x + rb.y; e += kWave) {
+        const uint32_t x = a.arena[e];
+        if (!(x & kTomb)) acc.add(contrib[x]);
+      }
       T s = wave_sum(acc.get());
       if (lane_id() == 0) finish_vertex<T>(a, v, s, base, rank, next, cnext, l1, linf, dang);
     }
@@ -197,10 +210,14 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_iter(PrArgs a) {
     for (uint32_t c = (b - a.bs - a.bm) * (kPrBlock / kWave) + threadIdx.x / kWave; c < a.nchunks;
          c += bh * (kPrBlock / kWave)) {
       const uint32_t hv = a.hlist[a.hchunk_v[c]];
+      const uint2 rb = a.row[hv];
       const uint32_t e0 = a.hchunk_off[c];
-      const uint32_t e1 = min(e0 + kChunk, a.off[hv + 1]);
+      const uint32_t e1 = min(e0 + kChunk, rb.x + rb.y);
       Acc<T> acc;
-      for (uint32_t e = e0 + lane_id(); e < e1; e += kWave) acc.add(contrib[a.adj[e]]);
+      for (uint32_t e = e0 + lane_id(); e < e1; e += kWave) {
+        const uint32_t x = a.arena[e];
+        if (!(x & kTomb)) acc.add(contrib[x]);
+      }
       T s = wave_sum(acc.get());
       if (lane_id() == 0) hpart[c] = s;
     }
@@ -290,23 +307,23 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_init(PrArgs a, int warm) {
 // No atomics, and the lists come out sorted.
 constexpr uint32_t kClassWaves = 4096;
 
-__device__ __forceinline__ void class_ballots(const uint32_t* off, uint32_t n, uint32_t g, unsigned long long& bs,
+__device__ __forceinline__ void class_ballots(const uint2* row, uint32_t n, uint32_t g, unsigned long long& bs,
                                               unsigned long long& bm, unsigned long long& bh) {
   const uint32_t v = g * kWave + lane_id();
-  const uint32_t d = v < n ? off[v + 1] - off[v] : 0u;
+  const uint32_t d = v < n ? row[v].y : 0u;  // stored entries, tombstones included
   bs = __ballot(v < n && d <= kSmallRow);
   bm = __ballot(v < n && d > kSmallRow && d <= kChunk);
   bh = __ballot(v < n && d > kChunk);
 }
 
-__global__ void k_pr_class_count(const uint32_t* __restrict__ off, uint32_t n, uint32_t gpw,
+__global__ void k_pr_class_count(const uint2* __restrict__ row, uint32_t n, uint32_t gpw,
                                  uint32_t* __restrict__ counts) {
   const uint32_t w = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   const uint32_t groups = (n + kWave - 1) / kWave;
   uint32_t cs = 0, cm = 0, ch = 0;
   for (uint32_t g = w * gpw; g < min((w + 1) * gpw, groups); ++g) {
     unsigned long long bs, bm, bh;
-    class_ballots(off, n, g, bs, bm, bh);
+    class_ballots(row, n, g, bs, bm, bh);
     cs += __popcll(bs);
     cm += __popcll(bm);
     ch += __popcll(bh);
@@ -318,7 +335,7 @@ __global__ void k_pr_class_count(const uint32_t* __restrict__ off, uint32_t n, u
   }
 }
 
-__global__ void k_pr_class_write(const uint32_t* __restrict__ off, uint32_t n, uint32_t gpw,
+__global__ void k_pr_class_write(const uint2* __restrict__ row, uint32_t n, uint32_t gpw,
                                  const uint32_t* __restrict__ bases, uint32_t* __restrict__ slist,
                                  uint32_t* __restrict__ mlist, uint32_t* __restrict__ hlist) {
   const uint32_t w = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
@@ -328,7 +345,7 @@ __global__ void k_pr_class_write(const uint32_t* __restrict__ off, uint32_t n, u
   uint32_t oh = bases[2 * kClassWaves + w] - bases[2 * kClassWaves];
   for (uint32_t g = w * gpw; g < min((w + 1) * gpw, groups); ++g) {
     unsigned long long bs, bm, bh;
-    class_ballots(off, n, g, bs, bm, bh);
+    class_ballots(row, n, g, bs, bm, bh);
     const uint32_t v = g * kWave + lane_id();
     if ((bs >> lane_id()) & 1ull) slist[os + lanes_below(bs)] = v;
     if ((bm >> lane_id()) & 1ull) mlist[om + lanes_below(bm)] = v;
@@ -336,6 +353,54 @@ __global__ void k_pr_class_write(const uint32_t* __restrict__ off, uint32_t n, u
     os += __popcll(bs);
     om += __popcll(bm);
     oh += __popcll(bh);
+  }
+}
+
+// Chunk table of the large rows (one workgroup of kHubThreads, each a
+// contiguous run of H rows): hub h owns chunks [first[h], first[h+1]), chunk
+// c starts at arena entry off[c] of row hlist[cv[c]].  tail[0..3] receive
+// the class list sizes (S, M, H) and the chunk count for the host.
+constexpr uint32_t kHubThreads = 1024;
+__global__ __launch_bounds__(kHubThreads) void k_pr_hub_chunks(const uint2* __restrict__ row,
+                                                               const uint32_t* __restrict__ hlist,
+                                                               const uint32_t* __restrict__ counts,
+                                                               const uint32_t* __restrict__ bases,
+                                                               uint32_t* __restrict__ cv, uint32_t* __restrict__ co,
+                                                               uint32_t* __restrict__ first, uint32_t* __restrict__ tail) {
+  constexpr uint32_t kW = kHubThreads / kWave;
+  __shared__ unsigned long long s_tot[kW];
+  const uint32_t W = kClassWaves;
+  const uint32_t ns = bases[W], nm = bases[2 * W] - bases[W];
+  const uint32_t nh = bases[3 * W - 1] + counts[3 * W - 1] - bases[2 * W];
+  const uint32_t per = (nh + kHubThreads - 1) / kHubThreads;
+  const uint32_t h0 = min(nh, threadIdx.x * per), h1 = min(nh, h0 + per);
+  unsigned long long mine = 0;
+  for (uint32_t h = h0; h < h1; ++h) mine += (row[hlist[h]].y + kChunk - 1) / kChunk;
+  const unsigned long long inc = wave_inclusive_scan<unsigned long long>(mine);
+  const uint32_t wv = threadIdx.x / kWave;
+  if (lane_id() == kWave - 1) s_tot[wv] = inc;
+  __syncthreads();
+  unsigned long long base = inc - mine, total = 0;
+  for (uint32_t w = 0; w < kW; ++w) {
+    if (w < wv) base += s_tot[w];
+    total += s_tot[w];
+  }
+  for (uint32_t h = h0; h < h1; ++h) {
+    const uint2 rb = row[hlist[h]];
+    const uint32_t nc = (rb.y + kChunk - 1) / kChunk;
+    first[h] = static_cast<uint32_t>(base);
+    for (uint32_t q = 0; q < nc; ++q) {
+      cv[base + q] = h;
+      co[base + q] = rb.x + q * kChunk;
+    }
+    base += nc;
+  }
+  if (threadIdx.x == 0) {
+    first[nh] = static_cast<uint32_t>(total);
+    tail[0] = ns;
+    tail[1] = nm;
+    tail[2] = nh;
+    tail[3] = static_cast<uint32_t>(total);
   }
 }
 
@@ -505,80 +570,58 @@ static size_t elt(const vgx_pagerank* p) { return p->precision == 32 ? 4 : 8; }
 
 static void build_lists(vgx_pagerank* p) {
   vgx_graph* g = p->g;
-  const CsrSnapshot& sn = graph_snapshot(g, 1);
   if (p->lists_version == g->version && p->n == g->n) return;
   hipStream_t s = g->stream;
   const uint64_t n = g->n;
+  const Adjacency& in = g->in();
   p->slist.ensure(n + 1);
   p->mlist.ensure(n + 1);
   p->hlist.ensure(n + 1);
-  uint32_t c[3] = {0, 0, 0};
+  // at most stored / kChunk + one chunk per large row; large rows exceed kChunk
+  const uint64_t max_chunks = 2 * (in.arena.n / kChunk) + 2;
+  p->hchunk_v.ensure(max_chunks);
+  p->hchunk_off.ensure(max_chunks);
+  p->hchunk_first.ensure(std::min<uint64_t>(n, in.arena.n / kChunk + 1) + 1);
+  p->class_tail.ensure(4);
+  uint32_t tail[4] = {0, 0, 0, 0};
   if (n) {
     const uint32_t W = kClassWaves;
     const uint32_t groups = static_cast<uint32_t>((n + kWave - 1) / kWave);
     const uint32_t gpw = (groups + W - 1) / W;
-    DevBuf<uint32_t> counts, bases;
-    counts.ensure(3 * W);
-    bases.ensure(3 * W);
+    p->class_counts.ensure(3 * W);
+    p->class_bases.ensure(3 * W);
     const unsigned blocks = W / (kPrBlock / kWave);
-    k_pr_class_count<<<blocks, kPrBlock, 0, s>>>(sn.off.p, static_cast<uint32_t>(n), gpw, counts.p);
+    k_pr_class_count<<<blocks, kPrBlock, 0, s>>>(in.row.p, static_cast<uint32_t>(n), gpw, p->class_counts.p);
     VGX_LAUNCH_CHECK();
     size_t bytes = 0;
-    VGX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, counts.p, bases.p, static_cast<int>(3 * W), s));
+    VGX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, p->class_counts.p, p->class_bases.p,
+                                             static_cast<int>(3 * W), s));
     p->scan_tmp.ensure(bytes + 16);
-    VGX_HIP(hipcub::DeviceScan::ExclusiveSum(p->scan_tmp.p, bytes, counts.p, bases.p, static_cast<int>(3 * W), s));
-    k_pr_class_write<<<blocks, kPrBlock, 0, s>>>(sn.off.p, static_cast<uint32_t>(n), gpw, bases.p, p->slist.p,
-                                                 p->mlist.p, p->hlist.p);
+    VGX_HIP(hipcub::DeviceScan::ExclusiveSum(p->scan_tmp.p, bytes, p->class_counts.p, p->class_bases.p,
+                                             static_cast<int>(3 * W), s));
+    k_pr_class_write<<<blocks, kPrBlock, 0, s>>>(in.row.p, static_cast<uint32_t>(n), gpw, p->class_bases.p,
+                                                 p->slist.p, p->mlist.p, p->hlist.p);
     VGX_LAUNCH_CHECK();
-    uint32_t tail[4];
-    VGX_HIP(hipMemcpyAsync(&tail[0], bases.p + W, 4, hipMemcpyDeviceToHost, s));
-    VGX_HIP(hipMemcpyAsync(&tail[1], bases.p + 2 * W, 4, hipMemcpyDeviceToHost, s));
-    VGX_HIP(hipMemcpyAsync(&tail[2], bases.p + 3 * W - 1, 4, hipMemcpyDeviceToHost, s));
-    VGX_HIP(hipMemcpyAsync(&tail[3], counts.p + 3 * W - 1, 4, hipMemcpyDeviceToHost, s));
-    VGX_HIP(hipStreamSynchronize(s));
-    c[0] = tail[0];
-    c[1] = tail[1] - tail[0];
-    c[2] = tail[2] + tail[3] - tail[1];
-  }
-  p->ns = c[0];
-  p->nm = c[1];
-  p->nh = c[2];
-  // chunk table of the large rows (host; there are few of them)
-  std::vector<uint32_t> hl(p->nh), offs(n + 1);
-  if (p->nh) {
-    VGX_HIP(hipMemcpyAsync(hl.data(), p->hlist.p, p->nh * 4, hipMemcpyDeviceToHost, s));
-    VGX_HIP(hipMemcpyAsync(offs.data(), sn.off.p, (n + 1) * 4, hipMemcpyDeviceToHost, s));
+    k_pr_hub_chunks<<<1, kHubThreads, 0, s>>>(in.row.p, p->hlist.p, p->class_counts.p, p->class_bases.p,
+                                              p->hchunk_v.p, p->hchunk_off.p, p->hchunk_first.p,
+                                              p->class_tail.p);
+    VGX_LAUNCH_CHECK();
+    VGX_HIP(hipMemcpyAsync(tail, p->class_tail.p, sizeof(tail), hipMemcpyDeviceToHost, s));
     VGX_HIP(hipStreamSynchronize(s));
   }
-  std::vector<uint32_t> cv, co, first(p->nh + 1, 0);
-  for (uint64_t h = 0; h < p->nh; ++h) {
-    first[h] = static_cast<uint32_t>(cv.size());
-    for (uint32_t e = offs[hl[h]]; e < offs[hl[h] + 1]; e += kChunk) {
-      cv.push_back(static_cast<uint32_t>(h));
-      co.push_back(e);
-    }
-  }
-  first[p->nh] = static_cast<uint32_t>(cv.size());
-  p->nchunks = cv.size();
-  p->hchunk_v.ensure(cv.size() + 1);
-  p->hchunk_off.ensure(co.size() + 1);
-  p->hchunk_first.ensure(first.size());
-  if (!cv.empty()) {
-    VGX_HIP(hipMemcpyAsync(p->hchunk_v.p, cv.data(), cv.size() * 4, hipMemcpyHostToDevice, s));
-    VGX_HIP(hipMemcpyAsync(p->hchunk_off.p, co.data(), co.size() * 4, hipMemcpyHostToDevice, s));
-  }
-  VGX_HIP(hipMemcpyAsync(p->hchunk_first.p, first.data(), first.size() * 4, hipMemcpyHostToDevice, s));
-  p->hpart.ensure((cv.size() + 1) * 8);
-  VGX_HIP(hipStreamSynchronize(s));
+  p->ns = tail[0];
+  p->nm = tail[1];
+  p->nh = tail[2];
+  p->nchunks = tail[3];
+  p->hpart.ensure((p->nchunks + 1) * 8);
   p->lists_version = g->version;
 }
 
 static PrArgs pr_args(vgx_pagerank* p, DevBuf<uint8_t>* rk) {
   vgx_graph* g = p->g;
-  const CsrSnapshot& sn = graph_snapshot(g, 1);
   PrArgs a{};
-  a.off = sn.off.p;
-  a.adj = sn.adj.p;
+  a.row = g->in().row.p;
+  a.arena = g->in().arena.p;
   a.outdeg = g->adj[0].deg.p;
   a.slist = p->slist.p;
   a.mlist = p->mlist.p;
