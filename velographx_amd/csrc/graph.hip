@@ -175,6 +175,60 @@ __device__ __forceinline__ uint32_t row_lower_bound(const uint32_t* arena, uint2
   return lo;
 }
 
+// Both compactions of a staged batch in one workgroup (batches of up to
+// kSelectSmall arc ops; larger ones use the library's device-wide select):
+// positions i with f1[i] != 0 go to out1, those with f2[i] != 0 to out2, in
+// order.  Each thread takes a contiguous run of whole 16-byte words of both
+// flag arrays (16-byte aligned, padded past M), counts, and after a block
+// scan writes its positions.
+constexpr uint32_t kSelectThreads = 1024;
+constexpr uint64_t kSelectSmall = 1u << 17;
+__global__ __launch_bounds__(kSelectThreads) void k_select_two(const uint8_t* __restrict__ f1,
+                                                               const uint8_t* __restrict__ f2, uint32_t M,
+                                                               uint32_t* __restrict__ out1, unsigned long long* n1,
+                                                               uint32_t* __restrict__ out2, unsigned long long* n2) {
+  constexpr uint32_t kW = kSelectThreads / kWave;
+  __shared__ unsigned long long s_tot[kW];
+  const uint32_t per = ((M + kSelectThreads - 1) / kSelectThreads + 15) & ~15u;
+  const uint32_t i0 = min(M, threadIdx.x * per), i1 = min(M, i0 + per);
+  auto bytes = [](const uint4& q, uint32_t b) {
+    const uint32_t w = b < 4 ? q.x : b < 8 ? q.y : b < 12 ? q.z : q.w;
+    return (w >> ((b & 3) * 8)) & 0xFFu;
+  };
+  unsigned long long c = 0;
+  for (uint32_t i = i0; i < i1; i += 16) {
+    const uint4 a1 = *reinterpret_cast<const uint4*>(f1 + i);
+    const uint4 a2 = *reinterpret_cast<const uint4*>(f2 + i);
+#pragma unroll
+    for (uint32_t b = 0; b < 16; ++b)
+      if (i + b < i1) c += (static_cast<unsigned long long>(bytes(a1, b) != 0) << 32) + (bytes(a2, b) != 0);
+  }
+  const unsigned long long inc = wave_inclusive_scan<unsigned long long>(c);
+  const uint32_t wv = threadIdx.x / kWave;
+  if (lane_id() == kWave - 1) s_tot[wv] = inc;
+  __syncthreads();
+  unsigned long long base = inc - c, total = 0;
+  for (uint32_t w = 0; w < kW; ++w) {
+    if (w < wv) base += s_tot[w];
+    total += s_tot[w];
+  }
+  uint32_t o1 = static_cast<uint32_t>(base >> 32), o2 = static_cast<uint32_t>(base);
+  for (uint32_t i = i0; i < i1; i += 16) {
+    const uint4 a1 = *reinterpret_cast<const uint4*>(f1 + i);
+    const uint4 a2 = *reinterpret_cast<const uint4*>(f2 + i);
+#pragma unroll
+    for (uint32_t b = 0; b < 16; ++b) {
+      if (i + b >= i1) break;
+      if (bytes(a1, b)) out1[o1++] = i + b;
+      if (bytes(a2, b)) out2[o2++] = i + b;
+    }
+  }
+  if (threadIdx.x == 0) {
+    *n1 = total >> 32;
+    *n2 = total & 0xFFFFFFFFull;
+  }
+}
+
 // Decide the action of every final arc op (the last op on its arc wins).
 __global__ void k_resolve(const unsigned long long* __restrict__ skeys, const uint32_t* __restrict__ svals,
                           uint64_t M, const uint8_t* __restrict__ add, const uint2* __restrict__ row,
@@ -318,12 +372,10 @@ __global__ void k_insert_small(const uint32_t* __restrict__ ins, const unsigned 
 // One workgroup per large row: merge-path merge of the old row and its
 // sorted inserts into a fresh arena range.
 // Rows of kBigRow < len + k <= cap, len <= kInPlaceMax, k <= kMidMaxIns:
-// one workgroup finds every insert's final place in the old row first, then
-// shifts the old entries right in tiles taken from the right end (a tile is
-// read whole before it is written, and its writes only reach its own
-// positions or positions the tile to its right has vacated), then writes
-// the inserted entries into the gaps.
-constexpr uint32_t kMidTile = 4 * 256;
+// one workgroup reads the whole row and its inserts into LDS, places every
+// old entry at i + (inserts below it) and every insert at t + (old entries
+// below it) by binary searches in LDS, and writes the merged row back over
+// the old one (all sources are in LDS, so the writes cannot overtake reads).
 constexpr uint32_t kMidMaxIns = kMidInsMax;
 __global__ __launch_bounds__(256) void k_insert_mid(const uint32_t* __restrict__ mid,
                                                     const uint32_t* __restrict__ ins,
@@ -331,7 +383,7 @@ __global__ __launch_bounds__(256) void k_insert_mid(const uint32_t* __restrict__
                                                     const unsigned long long* __restrict__ skeys,
                                                     uint2* __restrict__ row, uint32_t* __restrict__ arena,
                                                     uint32_t* __restrict__ deg, GraphCounters* ctr, int dir) {
-  __shared__ uint32_t s_pos[kMidMaxIns];
+  __shared__ uint32_t s_row[kInPlaceMax];
   __shared__ uint32_t s_ins[kMidMaxIns];
   const uint32_t j = mid[blockIdx.x];
   const uint32_t k = segk[j];
@@ -340,49 +392,32 @@ __global__ __launch_bounds__(256) void k_insert_mid(const uint32_t* __restrict__
   const uint32_t len = rb.y;
   uint32_t* a = arena + rb.x;
   for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) s_ins[t] = static_cast<uint32_t>(skeys[ins[j + t]]);
+  for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) s_row[i] = a[i];
   __syncthreads();
-  // final place of insert t: t + old entries below it
-  for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) {
-    const uint32_t x = s_ins[t];
-    uint32_t lo = 0, hi = len;
-    while (lo < hi) {
-      const uint32_t m = (lo + hi) >> 1;
-      if ((a[m] & kIdMask) < x) lo = m + 1;
-      else hi = m;
-    }
-    s_pos[t] = t + lo;
-  }
-  __syncthreads();
-  // old entry i moves to i + (inserts below it)
-  auto rank_ins = [&](uint32_t x) {
+  // old entry i moves to i + (inserts below it); ids keep the row sorted,
+  // deleted entries included
+  for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+    const uint32_t v = s_row[i];
+    const uint32_t x = v & kIdMask;
     uint32_t lo = 0, hi = k;
     while (lo < hi) {
       const uint32_t m = (lo + hi) >> 1;
       if (s_ins[m] < x) lo = m + 1;
       else hi = m;
     }
-    return lo;
-  };
-  const uint32_t tiles = (len + kMidTile - 1) / kMidTile;
-  for (uint32_t t = tiles; t-- > 0;) {
-    uint32_t v[4], dst[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t i = t * kMidTile + q * 256 + threadIdx.x;
-      dst[q] = ~0u;
-      v[q] = 0;
-      if (i < len) {
-        v[q] = a[i];
-        dst[q] = i + rank_ins(v[q] & kIdMask);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (dst[q] != ~0u) a[dst[q]] = v[q];
-    __syncthreads();
+    a[i + lo] = v;
   }
-  for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) a[s_pos[t]] = s_ins[t];
+  // insert t lands at t + (old entries below it)
+  for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) {
+    const uint32_t x = s_ins[t];
+    uint32_t lo = 0, hi = len;
+    while (lo < hi) {
+      const uint32_t m = (lo + hi) >> 1;
+      if ((s_row[m] & kIdMask) < x) lo = m + 1;
+      else hi = m;
+    }
+    a[t + lo] = x;
+  }
   if (threadIdx.x == 0) {
     row[u].y = len + k;
     atomicAdd(&deg[u], k);
@@ -868,7 +903,9 @@ void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const u
     b.segk.ensure(M + 1);
     b.big.ensure(M + 1);
     b.mid.ensure(M + 1);
-    b.flags.ensure(2 * M + 2);
+    // two flag arrays, each 16-byte aligned and padded to whole 16-byte words
+    const uint64_t fstride = (M + 16) & ~static_cast<uint64_t>(15);
+    b.flags.ensure(2 * fstride + 16);
     k_expand_keys<<<grid_for(k), kBlock, 0, s>>>(d_src, d_dst, k, g->directed ? 0 : 1, d == 1 ? 1 : 0, b.keys.p,
                                                   b.vals.p);
     VGX_LAUNCH_CHECK();
@@ -877,19 +914,25 @@ void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const u
                                                 static_cast<int>(M), 0, end_bit, s);
     });
     uint8_t* fflag = b.flags.p;
-    uint8_t* iflag = b.flags.p + M + 1;
+    uint8_t* iflag = b.flags.p + fstride;
     k_resolve<<<grid_for(M), kBlock, 0, s>>>(b.skeys.p, b.svals.p, M, b0.add.p, a.row.p, a.arena.p, g->n,
                                              b.act.p, b.pos.p, fflag, iflag);
     VGX_LAUNCH_CHECK();
-    hipcub::CountingInputIterator<uint32_t> it(0);
-    with_temp(g, b.tmp, [&](void* t, size_t& bytes) {
-      return hipcub::DeviceSelect::Flagged(t, bytes, it, fflag, b.fin.p, &g->d_ctr->nfinal[d],
-                                           static_cast<int>(M), s);
-    });
-    with_temp(g, b.tmp, [&](void* t, size_t& bytes) {
-      return hipcub::DeviceSelect::Flagged(t, bytes, it, iflag, b.ins.p, &g->d_ctr->nins[d],
-                                           static_cast<int>(M), s);
-    });
+    if (M <= kSelectSmall) {
+      k_select_two<<<1, kSelectThreads, 0, s>>>(fflag, iflag, static_cast<uint32_t>(M), b.fin.p,
+                                                &g->d_ctr->nfinal[d], b.ins.p, &g->d_ctr->nins[d]);
+      VGX_LAUNCH_CHECK();
+    } else {
+      hipcub::CountingInputIterator<uint32_t> it(0);
+      with_temp(g, b.tmp, [&](void* t, size_t& bytes) {
+        return hipcub::DeviceSelect::Flagged(t, bytes, it, fflag, b.fin.p, &g->d_ctr->nfinal[d],
+                                             static_cast<int>(M), s);
+      });
+      with_temp(g, b.tmp, [&](void* t, size_t& bytes) {
+        return hipcub::DeviceSelect::Flagged(t, bytes, it, iflag, b.ins.p, &g->d_ctr->nins[d],
+                                             static_cast<int>(M), s);
+      });
+    }
     k_plan_inserts<<<grid_for(M), kBlock, 0, s>>>(b.ins.p, &g->d_ctr->nins[d], b.skeys.p, a.row.p, a.cap.p,
                                                   b.segk.p, b.big.p, b.mid.p, g->d_ctr, d);
     VGX_LAUNCH_CHECK();
