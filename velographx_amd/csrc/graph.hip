@@ -526,27 +526,32 @@ __global__ __launch_bounds__(256) void k_reloc_copy(const uint32_t* __restrict__
   }
 }
 
-// Rows longer than this many stored entries are packed in chunks of
-// kPackLong by separate waves (k_pack_long_*), so an R-MAT hub row does not
-// serialise one wave.
-constexpr uint32_t kPackLong = 4096;
+// Packing (snapshots and compaction).  Rows of up to kPackShort stored
+// entries are copied by k_pack_rows, eight rows side by side per wave;
+// every longer row is cut into kPackChunk-entry chunks, one wave per chunk,
+// from a chunk table built on the device, so the work stays balanced however
+// skewed the degrees are (R-MAT's heavy rows cluster at ids with few bits
+// set).  A chunk of a row without deleted entries (live degree = stored
+// length) is a plain copy to off[u] + its start; otherwise its live
+// entries are counted first and placed after those of the row's earlier
+// chunks (an exclusive scan over all chunks).
+constexpr uint32_t kPackShort = 64;
+constexpr uint32_t kPackGroup = 8;
+constexpr uint32_t kPackChunk = 1024;
 
-// Copy the live entries of every row to out + off[u] (order preserved).  A
-// wave takes 64 rows at a time, W apart (W = waves in the grid, so the
-// long rows of low ids, which R-MAT clusters, spread over many waves): rows
-// of up to kPackShort entries are copied by their own lane, eight entries
-// per step with all eight loads in flight, longer rows by the whole wave in
-// turn; rows longer than kPackLong are left to the chunked kernels.
-// Optionally writes the new row table for compaction.
-constexpr uint32_t kPackShort = 32;
+// Copy the live entries of every row of up to kPackShort entries to
+// out + off[u] (order preserved): groups of eight lanes, eight rows side by
+// side, eight entries per step.  Optionally writes the new row table (every
+// row) for compaction.
 __global__ void k_pack_rows(const uint2* __restrict__ row, const uint32_t* __restrict__ arena, uint64_t n,
                             const uint32_t* __restrict__ off, uint32_t* __restrict__ out, uint64_t out_cap,
                             uint2* __restrict__ new_row, const uint32_t* __restrict__ deg) {
   const uint64_t W = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
   const uint64_t gw = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave;
   const unsigned lane = lane_id();
-  for (uint64_t i0 = 0; gw + W * i0 < n; i0 += kWave) {
-    const uint64_t u = gw + W * (i0 + lane);
+  const unsigned grp = lane / kPackGroup, gl = lane % kPackGroup;
+  for (uint64_t u0 = gw * kWave; u0 < n; u0 += W * kWave) {
+    const uint64_t u = u0 + lane;
     uint2 rb = make_uint2(0u, 0u);
     uint32_t o = 0;
     if (u < n) {
@@ -555,130 +560,113 @@ __global__ void k_pack_rows(const uint2* __restrict__ row, const uint32_t* __res
       if (new_row != nullptr) new_row[u] = make_uint2(o, deg[u]);
     }
     const bool shrt = rb.y <= kPackShort;
-    const uint32_t cnt = shrt ? rb.y : 0u;
-    const uint32_t most = wave_max(cnt);
-    uint64_t at = o;
-    for (uint32_t j = 0; j < most; j += 8) {
-      uint32_t x[8];
-#pragma unroll
-      for (uint32_t q = 0; q < 8; ++q) x[q] = j + q < cnt ? arena[rb.x + j + q] : kTomb;
-#pragma unroll
-      for (uint32_t q = 0; q < 8; ++q)
-        if (!(x[q] & kTomb)) {
-          if (at < out_cap) out[at] = x[q];  // the host checks the total
-          ++at;
-        }
-    }
-    unsigned long long lm = __ballot(!shrt && rb.y <= kPackLong);
-    while (lm) {
-      const int l = __ffsll(static_cast<long long>(lm)) - 1;
-      lm &= lm - 1;
-      const uint32_t bx = __shfl(rb.x, l, kWave), by = __shfl(rb.y, l, kWave);
-      uint64_t ol = __shfl(o, l, kWave);
-      for (uint32_t i = 0; i < by; i += kWave) {
-        const bool in = i + lane < by;
-        const uint32_t x = in ? arena[bx + i + lane] : kTomb;
+    for (unsigned r = 0; r < kWave / kPackGroup; ++r) {
+      // row handled by group grp in this round: lane grp * 8 + r
+      const int src = static_cast<int>(grp * kPackGroup + r);
+      const uint32_t bx = __shfl(rb.x, src, kWave);
+      const uint32_t by = __shfl(shrt ? rb.y : 0u, src, kWave);
+      uint64_t og = __shfl(o, src, kWave);
+      const uint32_t most = wave_max(by);
+      for (uint32_t j = 0; j < most; j += kPackGroup) {
+        const bool in = j + gl < by;
+        const uint32_t x = in ? arena[bx + j + gl] : kTomb;
         const bool keep = !(x & kTomb);
-        const unsigned long long b = __ballot(keep);
-        const uint64_t pos = ol + lanes_below(b);
-        if (keep && pos < out_cap) out[pos] = x;  // the host checks the total
-        ol += __popcll(b);
+        const unsigned long long m = (__ballot(keep) >> (grp * kPackGroup)) & 0xFFull;
+        const uint64_t at = og + __popcll(m & ((1ull << gl) - 1ull));
+        if (keep && at < out_cap) out[at] = x;  // the host checks the total
+        og += __popcll(m);
       }
     }
   }
 }
 
-__global__ void k_collect_long_rows(const uint2* __restrict__ row, uint64_t n, uint32_t* __restrict__ list,
-                                    unsigned long long* count) {
-  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; u < n;
-       u += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-    if (row[u].y > kPackLong) list[atomicAdd(count, 1ull)] = static_cast<uint32_t>(u);
+// chunks[u] = kPackChunk-entry chunks of row u when it is longer than
+// kPackShort (else 0); chunks[n] = 0 closes the scan.
+__global__ void k_pack_chunk_count(const uint2* __restrict__ row, uint64_t n, uint32_t* __restrict__ chunks) {
+  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; u <= n;
+       u += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t len = u < n ? row[u].y : 0u;
+    chunks[u] = len > kPackShort ? (len + kPackChunk - 1) / kPackChunk : 0u;
+  }
 }
 
-// Chunk table of the long rows (one workgroup of kTableThreads, each a
-// contiguous run of rows): chunk c covers entries [chunk_j[c] * kPackLong,
-// ...) of row chunk_u[c]; a row's chunks are consecutive.  counts[0] holds
-// the number of long rows, counts[1] receives the number of chunks.
-constexpr uint32_t kTableThreads = 1024;
-__global__ __launch_bounds__(kTableThreads) void k_long_chunk_table(const uint2* __restrict__ row,
-                                                                    const uint32_t* __restrict__ list,
-                                                                    unsigned long long* counts,
-                                                                    uint32_t* __restrict__ chunk_u,
-                                                                    uint32_t* __restrict__ chunk_j) {
-  constexpr uint32_t kW = kTableThreads / kWave;
-  __shared__ unsigned long long s_tot[kW];
-  const uint32_t nl = static_cast<uint32_t>(counts[0]);
-  const uint32_t per = (nl + kTableThreads - 1) / kTableThreads;
-  const uint32_t r0 = min(nl, threadIdx.x * per), r1 = min(nl, r0 + per);
-  unsigned long long mine = 0;
-  for (uint32_t r = r0; r < r1; ++r) mine += (row[list[r]].y + kPackLong - 1) / kPackLong;
-  const unsigned long long inc = wave_inclusive_scan<unsigned long long>(mine);
-  const uint32_t wv = threadIdx.x / kWave;
-  if (lane_id() == kWave - 1) s_tot[wv] = inc;
-  __syncthreads();
-  unsigned long long base = inc - mine, total = 0;
-  for (uint32_t w = 0; w < kW; ++w) {
-    if (w < wv) base += s_tot[w];
-    total += s_tot[w];
-  }
-  for (uint32_t r = r0; r < r1; ++r) {
-    const uint32_t u = list[r];
-    const uint32_t nc = (row[u].y + kPackLong - 1) / kPackLong;
-    for (uint32_t q = 0; q < nc; ++q) {
-      chunk_u[base + q] = u;
-      chunk_j[base + q] = q;
+// Chunk table: chunk first[u] + q covers entries [q * kPackChunk, ...) of
+// row u; counts[0] receives the number of chunks.
+__global__ void k_pack_chunk_fill(const uint32_t* __restrict__ first, uint64_t n, uint32_t* __restrict__ chunk_u,
+                                  uint32_t* __restrict__ chunk_j, unsigned long long* counts) {
+  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; u <= n;
+       u += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (u == n) {
+      counts[0] = first[n];
+      continue;
     }
-    base += nc;
+    const uint32_t f = first[u], e = first[u + 1];
+    for (uint32_t c = f; c < e; ++c) {
+      chunk_u[c] = static_cast<uint32_t>(u);
+      chunk_j[c] = c - f;
+    }
   }
-  if (threadIdx.x == 0) counts[1] = total;
 }
 
-// Live entries of chunk c (row chunk_u[c], entries [j*kPackLong, ...)).
-__global__ void k_pack_long_count(const uint2* __restrict__ row, const uint32_t* __restrict__ arena,
-                                  const uint32_t* __restrict__ chunk_u, const uint32_t* __restrict__ chunk_j,
-                                  const unsigned long long* counts, uint32_t* __restrict__ live) {
-  const uint32_t nch = static_cast<uint32_t>(counts[1]);
+// Live entries of every chunk slot up to `slots` (0 past the last chunk):
+// counted for rows with deleted entries, the chunk's length otherwise.
+__global__ void k_pack_chunk_live(const uint2* __restrict__ row, const uint32_t* __restrict__ arena,
+                                  const uint32_t* __restrict__ deg, const uint32_t* __restrict__ chunk_u,
+                                  const uint32_t* __restrict__ chunk_j, const unsigned long long* counts,
+                                  uint32_t slots, uint32_t* __restrict__ live) {
+  const uint32_t nch = static_cast<uint32_t>(counts[0]);
   const uint32_t W = gridDim.x * (blockDim.x / kWave);
-  for (uint32_t c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nch; c += W) {
-    const uint2 rb = row[chunk_u[c]];
-    const uint32_t b = chunk_j[c] * kPackLong;
-    const uint32_t e = min(b + kPackLong, rb.y);
+  for (uint32_t c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < slots; c += W) {
     uint32_t k = 0;
-    for (uint32_t i = b + lane_id(); i < e; i += kWave) k += (arena[rb.x + i] & kTomb) ? 0u : 1u;
-    k = wave_sum(k);
+    if (c < nch) {
+      const uint32_t u = chunk_u[c];
+      const uint2 rb = row[u];
+      const uint32_t b = chunk_j[c] * kPackChunk;
+      const uint32_t e = min(b + kPackChunk, rb.y);
+      if (deg[u] == rb.y) {
+        k = e - b;
+      } else {
+        for (uint32_t i = b + lane_id(); i < e; i += kWave) k += (arena[rb.x + i] & kTomb) ? 0u : 1u;
+        k = wave_sum(k);
+      }
+    }
     if (lane_id() == 0) live[c] = k;
   }
 }
 
-// Copy chunk c to off[u] + the live entries of the row's earlier chunks.
-__global__ void k_pack_long_write(const uint2* __restrict__ row, const uint32_t* __restrict__ arena,
-                                  const uint32_t* __restrict__ off, const uint32_t* __restrict__ chunk_u,
-                                  const uint32_t* __restrict__ chunk_j, const uint32_t* __restrict__ live,
-                                  const unsigned long long* counts, uint32_t* __restrict__ out, uint64_t out_cap) {
-  const uint32_t nch = static_cast<uint32_t>(counts[1]);
+// Copy chunk c to off[u] + the live entries of the row's earlier chunks
+// (lscan: exclusive scan of the live counts over all chunks).
+__global__ void k_pack_chunk_write(const uint2* __restrict__ row, const uint32_t* __restrict__ arena,
+                                   const uint32_t* __restrict__ deg, const uint32_t* __restrict__ off,
+                                   const uint32_t* __restrict__ chunk_u, const uint32_t* __restrict__ chunk_j,
+                                   const uint32_t* __restrict__ lscan, const unsigned long long* counts,
+                                   uint32_t* __restrict__ out, uint64_t out_cap) {
+  const uint32_t nch = static_cast<uint32_t>(counts[0]);
   const uint32_t W = gridDim.x * (blockDim.x / kWave);
   const unsigned lane = lane_id();
   for (uint32_t c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nch; c += W) {
     const uint32_t u = chunk_u[c], j = chunk_j[c];
     const uint2 rb = row[u];
-    uint32_t o = off[u];
-    for (uint32_t q = c - j; q < c; ++q) o += live[q];  // the row's chunks are consecutive
-    const uint32_t b = j * kPackLong;
-    const uint32_t e = min(b + kPackLong, rb.y);
+    uint64_t o = static_cast<uint64_t>(off[u]) + (lscan[c] - lscan[c - j]);  // the row's chunks are consecutive
+    const uint32_t b = j * kPackChunk;
+    const uint32_t e = min(b + kPackChunk, rb.y);
+    if (deg[u] == rb.y) {
+      if (o + (e - b) <= out_cap) {  // the host checks the total
+#pragma unroll 4
+        for (uint32_t i = b + lane; i < e; i += kWave) out[o + (i - b)] = arena[rb.x + i];
+      }
+      continue;
+    }
     for (uint32_t i = b; i < e; i += kWave) {
       const bool in = i + lane < e;
       const uint32_t x = in ? arena[rb.x + i + lane] : kTomb;
       const bool keep = !(x & kTomb);
       const unsigned long long m = __ballot(keep);
-      const uint64_t at = static_cast<uint64_t>(o) + lanes_below(m);
+      const uint64_t at = o + lanes_below(m);
       if (keep && at < out_cap) out[at] = x;  // the host checks the total
       o += __popcll(m);
     }
   }
-}
-
-__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
 // bump pointer of direction dir := the packed total off[n]
@@ -1031,39 +1019,45 @@ static void graph_apply(vgx_graph* g, const uint32_t* src, const uint32_t* dst, 
 
 // ------------------------------------------------------ snapshots, compaction
 
-// Chunked copy of the rows longer than kPackLong; the chunk table is built
+// Chunked copy of the rows longer than kPackShort; the chunk table is built
 // on the device, so packing needs no host round trip.  `stored` bounds the
 // entries of all rows (the arena's size does).
-static void pack_long_rows(vgx_graph* g, Adjacency& a, const uint32_t* off, uint32_t* out, uint64_t out_cap,
-                           uint64_t stored) {
+static void pack_chunked_rows(vgx_graph* g, Adjacency& a, const uint32_t* off, uint32_t* out, uint64_t out_cap,
+                              uint64_t stored) {
   hipStream_t s = g->stream;
   const uint64_t n = g->n;
   PackScratch& pk = g->pk;
-  const uint64_t max_chunks = 2 * (stored / kPackLong) + 2;
-  pk.long_list.ensure(n + 1);
+  // rows longer than kPackShort hold more than kPackShort entries each, so
+  // chunks <= stored / kPackChunk + stored / kPackShort
+  const uint64_t max_chunks = stored / kPackChunk + stored / kPackShort + 2;
+  pk.row_chunks.ensure(n + 1);
+  pk.chunk_first.ensure(n + 1);
   pk.chunk_u.ensure(max_chunks);
   pk.chunk_j.ensure(max_chunks);
-  pk.chunk_live.ensure(max_chunks);
+  pk.chunk_live.ensure(max_chunks + 1);
+  pk.chunk_scan.ensure(max_chunks + 1);
   pk.counts.ensure(2);
-  k_zero_u64<<<1, 64, 0, s>>>(pk.counts.p, 2);
+  k_pack_chunk_count<<<grid_for(n + 1, kBlock, 4096), kBlock, 0, s>>>(a.row.p, n, pk.row_chunks.p);
   VGX_LAUNCH_CHECK();
-  k_collect_long_rows<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(a.row.p, n, pk.long_list.p, pk.counts.p);
+  with_temp(g, g->bs[0].tmp, [&](void* t, size_t& b) {
+    return hipcub::DeviceScan::ExclusiveSum(t, b, pk.row_chunks.p, pk.chunk_first.p, static_cast<int>(n + 1), s);
+  });
+  k_pack_chunk_fill<<<grid_for(n + 1, kBlock, 4096), kBlock, 0, s>>>(pk.chunk_first.p, n, pk.chunk_u.p,
+                                                                      pk.chunk_j.p, pk.counts.p);
   VGX_LAUNCH_CHECK();
-  k_long_chunk_table<<<1, kTableThreads, 0, s>>>(a.row.p, pk.long_list.p, pk.counts.p, pk.chunk_u.p, pk.chunk_j.p);
+  const unsigned grid = grid_for(max_chunks, 4, 4096);
+  k_pack_chunk_live<<<grid, kBlock, 0, s>>>(a.row.p, a.arena.p, a.deg.p, pk.chunk_u.p, pk.chunk_j.p, pk.counts.p,
+                                            static_cast<uint32_t>(max_chunks + 1), pk.chunk_live.p);
   VGX_LAUNCH_CHECK();
-  const unsigned grid = grid_for(max_chunks, 4, 2048);
-  k_pack_long_count<<<grid, kBlock, 0, s>>>(a.row.p, a.arena.p, pk.chunk_u.p, pk.chunk_j.p, pk.counts.p,
-                                            pk.chunk_live.p);
-  VGX_LAUNCH_CHECK();
-  k_pack_long_write<<<grid, kBlock, 0, s>>>(a.row.p, a.arena.p, off, pk.chunk_u.p, pk.chunk_j.p, pk.chunk_live.p,
-                                            pk.counts.p, out, out_cap);
+  with_temp(g, g->bs[0].tmp, [&](void* t, size_t& b) {
+    return hipcub::DeviceScan::ExclusiveSum(t, b, pk.chunk_live.p, pk.chunk_scan.p,
+                                            static_cast<int>(max_chunks + 1), s);
+  });
+  k_pack_chunk_write<<<grid, kBlock, 0, s>>>(a.row.p, a.arena.p, a.deg.p, off, pk.chunk_u.p, pk.chunk_j.p,
+                                             pk.chunk_scan.p, pk.counts.p, out, out_cap);
   VGX_LAUNCH_CHECK();
 }
 
-// Pack the live entries of every row of direction d into `out` at the
-// offsets of an inclusive scan of `caps` (the live degrees when null),
-// writing the new row table when asked.  `out` is sized from a host bound,
-// so packing needs no host round trip.
 static void pack_direction(vgx_graph* g, int d, DevBuf<uint32_t>& off, DevBuf<uint32_t>& out, uint2* new_row,
                            const uint32_t* caps) {
   Adjacency& a = g->adj[d];
@@ -1085,7 +1079,7 @@ static void pack_direction(vgx_graph* g, int d, DevBuf<uint32_t>& off, DevBuf<ui
     k_pack_rows<<<grid_for(n, 4 * kWave, 4096), kBlock, 0, s>>>(a.row.p, a.arena.p, n, off.p, out.p, out.n,
                                                                 new_row, a.deg.p);
     VGX_LAUNCH_CHECK();
-    pack_long_rows(g, a, off.p, out.p, out.n, a.arena.n);
+    pack_chunked_rows(g, a, off.p, out.p, out.n, a.arena.n);
   }
 }
 

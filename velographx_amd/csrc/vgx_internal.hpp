@@ -187,9 +187,9 @@ struct PackScratch {
   DevBuf<uint32_t> off;          // row offsets of the packed copy (n + 2)
   DevBuf<uint32_t> spare_arena;
   DevBuf<uint2> spare_row;
-  DevBuf<uint32_t> long_list;    // rows longer than the chunked-packing limit
-  DevBuf<uint32_t> chunk_u, chunk_j, chunk_live;
-  DevBuf<unsigned long long> counts;  // [0] long rows, [1] chunks
+  DevBuf<uint32_t> row_chunks, chunk_first;  // chunks per row and their exclusive scan
+  DevBuf<uint32_t> chunk_u, chunk_j, chunk_live, chunk_scan;
+  DevBuf<unsigned long long> counts;  // [0] chunks
 };
 }  // namespace vgx
 
