@@ -322,17 +322,21 @@ __device__ __forceinline__ void lbs_edges(const uint32_t* queue, const unsigned 
     lo += last * step;
     hi = lo + step < hi ? lo + step : hi;
   }
-  // Invariant: item i0 contains edge e (pre[i0] <= e < pre[i0 + 1]).
+  // Invariant: item i0 contains edge e (pre[i0] <= e < pre[i0 + 1]).  The
+  // items of the next window are loaded before fn runs on this one, so
+  // their latency hides behind fn's own loads.
   uint32_t i0 = lo;
+  unsigned long long pk = ~0ull;
+  uint32_t qk = 0u;
+  if (i0 + lane < F) {
+    pk = pre[i0 + lane];
+    qk = queue[i0 + lane];
+  }
   for (unsigned long long e = a; e < b; e += kWave) {
     // Lane k holds item i0 + k of the window.  Every item has at least one
     // edge and item i0 + 1 starts after e, so the items starting inside this
     // 64-edge chunk are among the next 63; their start offsets, OR-ed into
     // one mask, give each lane its item with a popcount instead of a search.
-    const uint32_t idx = i0 + lane;
-    const bool in_q = idx < F;
-    const unsigned long long pk = in_q ? pre[idx] : ~0ull;
-    const uint32_t qk = in_q ? queue[idx] : 0u;
     unsigned long long bit = (lane > 0 && pk < e + kWave) ? (1ull << (pk - e)) : 0ull;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) bit |= __shfl_xor(bit, d, kWave);
@@ -343,13 +347,23 @@ __device__ __forceinline__ void lbs_edges(const uint32_t* queue, const unsigned 
     const unsigned long long pv = __shfl(pk, static_cast<int>(k), kWave);
     const uint2 rb = valid ? row[v] : make_uint2(0u, 0u);
     const uint32_t slot = valid ? rb.x + static_cast<uint32_t>(g - pv) : 0u;
-    fn(i0 + k, v, rb, slot, valid);
-    if (!__shfl(static_cast<int>(valid), kWave - 1, kWave)) break;
-    // advance to the item containing edge e + 64: items of the window that
-    // start at or before it, plus item i0 + 64 when all 63 others do
+    // the item containing edge e + 64: items of the window that start at or
+    // before it, plus item i0 + 64 when all 63 others do
+    const bool more = e + kWave < b;
     uint32_t adv = __popcll(__ballot(lane > 0 && pk <= e + kWave));
-    if (adv == kWave - 1 && i0 + kWave < F && pre[i0 + kWave] <= e + kWave) adv = kWave;
-    i0 += adv;
+    if (more && adv == kWave - 1 && i0 + kWave < F && pre[i0 + kWave] <= e + kWave) adv = kWave;
+    const uint32_t i1 = i0 + adv;
+    unsigned long long pk1 = ~0ull;
+    uint32_t qk1 = 0u;
+    if (more && i1 + lane < F) {
+      pk1 = pre[i1 + lane];
+      qk1 = queue[i1 + lane];
+    }
+    fn(i0 + k, v, rb, slot, valid);
+    if (!more) break;
+    i0 = i1;
+    pk = pk1;
+    qk = qk1;
   }
 }
 
