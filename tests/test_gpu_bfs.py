@@ -164,6 +164,40 @@ def test_insert_paths_mid_relocation_and_long_rows(vx, orc, directed):
     assert np.array_equal(vx.IncrementalBFS(g, 0).distances_numpy(), full)
 
 
+@pytest.mark.parametrize("directed", [False, True])
+def test_compaction_every_row_class(vx, orc, directed):
+    """Consolidation and snapshots over rows of every packing path: short
+    rows (up to 64 entries, eight-lane groups), rows cut into 1,024-entry
+    chunks with and without deletions (one chunk, several chunks, a row of
+    20,000 entries), empty rows; adjacency equals the model after each of
+    several compactions interleaved with batches, and BFS stays exact."""
+    n = 50000
+    rng = np.random.default_rng(11)
+    lens = {0: 20000, 1: 1024, 2: 1025, 3: 3000, 4: 65, 5: 64, 6: 63, 7: 1, 8: 700}
+    base = []
+    for u, L in lens.items():
+        base += [(u, int(v)) for v in rng.choice(np.arange(100, n), size=L, replace=False)]
+    base += [(int(u), int(u) + 1) for u in range(100, 40000, 3)]
+    g = vx.Graph(n, directed)
+    m = orc.ModelGraph(n, directed)
+    g.bulk_load_edges(np.array(base, dtype=np.uint32))
+    m.bulk_load_edges(base)
+    for step in range(4):
+        dels = [(u, int(v), 0) for u, L in lens.items() if u % 2 == step % 2
+                for v in m.neighbors(u)[: max(1, L // 3)]]
+        adds = [(int(u), int(v), 1) for u, v in rng.integers(0, n, size=(3000, 2))]
+        s_, d_, a_ = ops_arrays(dels + adds)
+        g.apply_arrays(s_, d_, a_)
+        m.apply(s_, d_, a_)
+        check_graph(g, m)          # snapshot packing with tombstones present
+        g.compact()
+        check_graph(g, m)          # consolidated rows
+        st = g.storage()
+        assert st["arena_used"] >= m.edge_count_directed()
+    full, _ = orc.model_bfs(m, 0)
+    assert np.array_equal(vx.IncrementalBFS(g, 0).distances_numpy(), full)
+
+
 # ------------------------------------------------------------ BFS
 
 
