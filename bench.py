@@ -539,7 +539,7 @@ def leg_pagerank(vx, wl, args, no_cpu):
     log(f"PageRank graph: directed R-MAT s{scale}, {arcs} arcs, built in {time.time() - t0:.1f}s")
     pr32 = vx.IncrementalPageRank(g32, 0.85, 32)
     pr64 = vx.IncrementalPageRank(g64, 0.85, 64)
-    pr32.solve_l1(200, 1e-7, False)  # warm-up (snapshot, allocations)
+    pr32.solve_l1(200, 1e-7, False)  # warm-up (class lists, allocations)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     pr32.solve_l1(200, 1e-7, False)
