@@ -89,6 +89,7 @@ __device__ __forceinline__ unsigned long long found_total(const LevelRec& r) {
 struct alignas(128) LevelCtl {
   unsigned long long visited_edges;  // edges of vertices visited before this level
   uint32_t prev_mode;                // direction used by the previous level
+  uint32_t no_queue;                 // the previous level built no frontier queue
 };
 
 // Rows a bottom-up level leaves for phase B (same ring).
@@ -184,6 +185,7 @@ struct vgx_bfs {
   uint32_t prof_mode[64] = {};
   uint64_t prof_mf[64] = {};
   uint64_t prof_pend[64] = {};  // packed rows << kEdgeBits | edges left for phase B
+  uint64_t prof_nf[64] = {};    // frontier vertices
   // replay of a repeated traversal's launch program
   bool use_graph = true;
   std::vector<uint64_t> gkey;
@@ -282,7 +284,14 @@ __device__ __forceinline__ bool rest_in_frontier(const LevelArgs& a, const unsig
 // One BFS level.  `level` is the launch index passed by the host.  With
 // `inline_tail` set (the host predicted a top-down level and launches no
 // tail kernel after it), a bottom-up level finishes its rows itself.
-__global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t level, int inline_tail) {
+// With `build_queue` clear (the host expects a large bottom-up level next)
+// a top-down level claims vertices without queueing them: no row-length
+// load and no queue traffic per new vertex.  The next level then has no
+// frontier arc count: it runs bottom-up when the frontier is large by the
+// bottom-up rule (n_f * beta >= n), else top-down from the frontier bitmap,
+// and counts the frontier's arcs as its size times the mean degree.
+__global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t level, int inline_tail,
+                                                         int build_queue) {
   BfsState* st = a.st;
   const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
   // Launches enqueued past the last level must not read the ring: its slots
@@ -295,16 +304,23 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
     return;
   }
   // Direction of this level, derived identically by every workgroup.
-  const unsigned long long mf = qc & kEdgeMask;
+  const bool no_queue = level > 0 && st->ctl[level & 3].no_queue;
+  const unsigned long long mf =
+      no_queue ? static_cast<unsigned long long>(static_cast<double>(nf) * static_cast<double>(st->total_edges) /
+                                                 static_cast<double>(st->n > 0 ? st->n : 1))
+               : (qc & kEdgeMask);
   const unsigned long long seen = st->ctl[level & 3].visited_edges + mf;
   // 0: top-down from the queue, 1: bottom-up, 2: top-down from the frontier
-  // bitmap (after a bottom-up level, which builds no queue)
+  // bitmap (after a bottom-up level, or a top-down level that built no queue)
   uint32_t mode = 0;
   if (level > 0) {
     const uint32_t prev = st->ctl[level & 3].prev_mode;
     if (prev != 1) {
       const unsigned long long mu = st->total_edges > seen ? st->total_edges - seen : 0;
-      mode = static_cast<double>(mf) * st->alpha > static_cast<double>(mu) ? 1u : 0u;
+      if (no_queue)
+        mode = static_cast<double>(nf) * st->beta >= static_cast<double>(st->n) ? 1u : 2u;
+      else
+        mode = static_cast<double>(mf) * st->alpha > static_cast<double>(mu) ? 1u : 0u;
     } else {
       mode = static_cast<double>(nf) * st->beta < static_cast<double>(st->n) ? 2u : 1u;
     }
@@ -313,6 +329,8 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
   if (leader) {
     st->ctl[(level + 1) & 3].prev_mode = mode;
     st->ctl[(level + 1) & 3].visited_edges = seen;
+    // only top-down levels fill a queue (bottom-up ones never do)
+    st->ctl[(level + 1) & 3].no_queue = (mode != 1 && !build_queue) ? 1u : 0u;
     st->rec[(level + 2) & 3].q = 0;
     for (uint32_t j = 0; j < kFoundStripes; ++j) st->rec[(level + 2) & 3].found[j].v = 0;
     st->pend[(level + 2) & 3].q = 0;
@@ -361,7 +379,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
     if (newly) {
       a.dist[w] = next_level;
       atomicOr(bm_next + (w >> 6), 1ull << (w & 63));
-      wl = a.row[w].y;
+      if (build_queue) wl = a.row[w].y;
     }
     found_local += __popcll(__ballot(newly));
     stage_push(sb, nbuf, &rn->q, qn, pn, newly && wl > 0, w, wl);
@@ -993,6 +1011,7 @@ __global__ void k_bfs_init(LevelArgs a, const unsigned long long* __restrict__ z
     st->rec[threadIdx.x].q = 0;
     st->ctl[threadIdx.x].visited_edges = 0;
     st->ctl[threadIdx.x].prev_mode = 0;
+    st->ctl[threadIdx.x].no_queue = 0;
     st->pend[threadIdx.x].q = 0;
   }
   __syncthreads();
@@ -1082,6 +1101,15 @@ static void bfs_full(vgx_bfs* b) {
       lb = static_cast<unsigned>(std::min<uint64_t>(blocks, std::max<uint64_t>(16, want)));
     }
   };
+  // A level builds its queue unless the previous traversal of this object
+  // ran the next level bottom-up on a frontier large enough that the
+  // bottom-up rule alone picks that direction again (so the choice is stable
+  // from one traversal to the next).
+  auto build_queue = [&](uint32_t L) {
+    if (!adaptive || L + 1 >= b->prof_levels || L + 1 >= kLogLevels) return true;
+    return !(b->prof_mode[L + 1] == 1 &&
+             static_cast<double>(b->prof_nf[L + 1]) * b->tune.beta >= static_cast<double>(n));
+  };
   // The first chunk covers the depth of the previous traversal; the host
   // sees completion in the counters of the level after the chunk, so no
   // launch is spent on detecting it.  Later chunks shrink.
@@ -1099,7 +1127,7 @@ static void bfs_full(vgx_bfs* b) {
       if (events) VGX_HIP(hipEventRecord(b->events[2 * i], s));
       unsigned lb, tb;
       grids(first + i, lb, tb);
-      k_bfs_level<<<lb, kBfsBlock, 0, s>>>(a, first + i, tb == 0 ? 1 : 0);
+      k_bfs_level<<<lb, kBfsBlock, 0, s>>>(a, first + i, tb == 0 ? 1 : 0, build_queue(first + i) ? 1 : 0);
       VGX_LAUNCH_CHECK();
       if (tb) {
         k_bfs_bottom_up_tail<<<tb, kBfsBlock, 0, s>>>(a, first + i);
@@ -1140,7 +1168,7 @@ static void bfs_full(vgx_bfs* b) {
     for (uint32_t i = 0; i < chunk; ++i) {
       unsigned lb, tb;
       grids(i, lb, tb);
-      key.push_back((static_cast<uint64_t>(lb) << 32) | tb);
+      key.push_back((static_cast<uint64_t>(lb) << 32) | (static_cast<uint64_t>(build_queue(i)) << 31) | tb);
     }
     if (key == b->gkey) {
       if (!b->gexec) {
@@ -1205,6 +1233,7 @@ static void bfs_full(vgx_bfs* b) {
     b->prof_mode[L] = b->h_st->log_mode[L];
     b->prof_mf[L] = b->h_st->log_mf[L];
     b->prof_pend[L] = b->h_st->log_pend[L];
+    b->prof_nf[L] = b->h_st->log_nf[L];
   }
 }
 
