@@ -57,6 +57,11 @@ constexpr uint64_t kInlineEdges = 65536;
 // 64-edge windows per wave (at least 16 workgroups): each window is a chain
 // of dependent loads, so heavy levels want many waves, light ones few.
 constexpr uint64_t kTdWindowsPerWave = 2;
+// How a top-down level claims new vertices: with an atomic on the visited
+// bitmap and a queue push (kClaimQueue), with the atomic and no queue
+// (kClaimBitmap), or by storing the distance alone, racing stores of the
+// same value, after which k_bfs_pack derives the frontier (kClaimDistance).
+enum Claims : uint32_t { kClaimQueue = 0, kClaimBitmap = 1, kClaimDistance = 2 };
 
 // Per-level record, kept in a ring of 4 indexed by level & 3.  Level L's
 // kernel accumulates the frontier of level L+1 into rec[(L+1)&3] (q, found)
@@ -284,14 +289,16 @@ __device__ __forceinline__ bool rest_in_frontier(const LevelArgs& a, const unsig
 // One BFS level.  `level` is the launch index passed by the host.  With
 // `inline_tail` set (the host predicted a top-down level and launches no
 // tail kernel after it), a bottom-up level finishes its rows itself.
-// With `build_queue` clear (the host expects a large bottom-up level next)
-// a top-down level claims vertices without queueing them: no row-length
-// load and no queue traffic per new vertex.  The next level then has no
+// `claims` (enum Claims) says how a top-down level claims new vertices.
+// Without a queue (the host expects a large bottom-up level next) there is
+// no row-length load and no queue traffic per new vertex, and where the
+// host launches k_bfs_pack after the level, no atomic either: the pack
+// builds the frontier bitmap from the distances.  The next level then has no
 // frontier arc count: it runs bottom-up when the frontier is large by the
 // bottom-up rule (n_f * beta >= n), else top-down from the frontier bitmap,
 // and counts the frontier's arcs as its size times the mean degree.
 __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t level, int inline_tail,
-                                                         int build_queue) {
+                                                         uint32_t claims) {
   BfsState* st = a.st;
   const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
   // Launches enqueued past the last level must not read the ring: its slots
@@ -330,7 +337,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
     st->ctl[(level + 1) & 3].prev_mode = mode;
     st->ctl[(level + 1) & 3].visited_edges = seen;
     // only top-down levels fill a queue (bottom-up ones never do)
-    st->ctl[(level + 1) & 3].no_queue = (mode != 1 && !build_queue) ? 1u : 0u;
+    st->ctl[(level + 1) & 3].no_queue = (mode != 1 && claims != kClaimQueue) ? 1u : 0u;
     st->rec[(level + 2) & 3].q = 0;
     for (uint32_t j = 0; j < kFoundStripes; ++j) st->rec[(level + 2) & 3].found[j].v = 0;
     st->pend[(level + 2) & 3].q = 0;
@@ -368,7 +375,16 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
   unsigned long long found_local = 0, pend_local = 0;
 
   // Claim w for the next level if nobody has; queue it with its out-degree.
+  // A level that builds no queue claims with the distance alone: every lane
+  // that finds w unvisited stores the same value, with no atomic (a
+  // device-scope atomic is a round trip past the L2), and k_bfs_pack then
+  // derives the frontier bitmap, the visited bits and the count of new
+  // vertices from the distances.
   auto claim = [&](uint32_t w, bool consider) {
+    if (claims == kClaimDistance) {
+      if (consider && !(w & kTomb) && !((a.visited[w >> 6] >> (w & 63)) & 1ull)) a.dist[w] = next_level;
+      return;
+    }
     bool newly = false;
     uint32_t wl = 0;
     if (consider && !(w & kTomb)) {
@@ -379,7 +395,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
     if (newly) {
       a.dist[w] = next_level;
       atomicOr(bm_next + (w >> 6), 1ull << (w & 63));
-      if (build_queue) wl = a.row[w].y;
+      if (claims == kClaimQueue) wl = a.row[w].y;
     }
     found_local += __popcll(__ballot(newly));
     stage_push(sb, nbuf, &rn->q, qn, pn, newly && wl > 0, w, wl);
@@ -497,6 +513,47 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
       for (uint32_t i = 0; i < kWavesPerBlock; ++i) p += s_pend[i];
       if (p) atomicAdd(&st->log_pend[level], p);
     }
+  }
+}
+
+// After a top-down level that built no queue: the vertices it claimed are
+// those at distance level + 1.  One wave per 64-vertex word sets the next
+// frontier bitmap and the visited bits from the distances (coalesced) and
+// counts them for the next level.  Exits at once after any other level.
+__global__ __launch_bounds__(kBfsBlock) void k_bfs_pack(LevelArgs a, uint32_t level) {
+  BfsState* st = a.st;
+  if (st->done_level <= level || !st->ctl[(level + 1) & 3].no_queue) return;
+  const uint32_t words = a.words, next_level = level + 1;
+  unsigned long long* bm_next = a.bm + static_cast<uint64_t>((level + 1) % 3) * words;
+  const uint32_t W = gridDim.x * kWavesPerBlock;
+  const uint32_t gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  unsigned long long found_local = 0;
+  // four words per step, their distance loads issued together
+  for (uint32_t w0 = gw * 4; w0 < words; w0 += W * 4) {
+    uint32_t d[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t v = (w0 + k) * 64 + lane;
+      d[k] = v < a.n ? a.dist[v] : kInf;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const unsigned long long m = __ballot(d[k] == next_level);
+      if (m && lane == 0) {
+        bm_next[w0 + k] = m;
+        a.visited[w0 + k] |= m;
+      }
+      found_local += __popcll(m);
+    }
+  }
+  __shared__ unsigned long long s_found[kWavesPerBlock];
+  if (lane == 0) s_found[threadIdx.x / kWave] = found_local;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long f = 0;
+    for (uint32_t i = 0; i < kWavesPerBlock; ++i) f += s_found[i];
+    if (f) atomicAdd(&st->rec[(level + 1) & 3].found[blockIdx.x % kFoundStripes].v, f);
   }
 }
 
@@ -1105,10 +1162,15 @@ static void bfs_full(vgx_bfs* b) {
   // ran the next level bottom-up on a frontier large enough that the
   // bottom-up rule alone picks that direction again (so the choice is stable
   // from one traversal to the next).
-  auto build_queue = [&](uint32_t L) {
-    if (!adaptive || L + 1 >= b->prof_levels || L + 1 >= kLogLevels) return true;
-    return !(b->prof_mode[L + 1] == 1 &&
-             static_cast<double>(b->prof_nf[L + 1]) * b->tune.beta >= static_cast<double>(n));
+  // Such a level claims by distance where it ran top-down itself, with a
+  // pack launch after it; where it was bottom-up (no claims to pack unless
+  // the direction changes) it keeps the atomic claim and needs no pack.
+  auto claims = [&](uint32_t L) -> uint32_t {
+    if (!adaptive || L + 1 >= b->prof_levels || L + 1 >= kLogLevels) return kClaimQueue;
+    if (!(b->prof_mode[L + 1] == 1 &&
+          static_cast<double>(b->prof_nf[L + 1]) * b->tune.beta >= static_cast<double>(n)))
+      return kClaimQueue;
+    return b->prof_mode[L] == 1 ? kClaimBitmap : kClaimDistance;
   };
   // The first chunk covers the depth of the previous traversal; the host
   // sees completion in the counters of the level after the chunk, so no
@@ -1117,6 +1179,7 @@ static void bfs_full(vgx_bfs* b) {
   const unsigned long long total_edges = g->adj[0].stored;
   const float alpha = static_cast<float>(b->tune.alpha), beta = static_cast<float>(b->tune.beta);
   const unsigned ginit = blocks_for(std::max<uint64_t>(n, 1), kBfsBlock, 2048);
+  const unsigned gpack = blocks_for(std::max<uint64_t>(words, 1), 4 * kWavesPerBlock, 1024);
   auto launch_levels = [&](uint32_t first, uint32_t count, bool events) {
     if (events && b->events.size() < 2 * static_cast<size_t>(count)) {
       size_t old = b->events.size();
@@ -1127,10 +1190,15 @@ static void bfs_full(vgx_bfs* b) {
       if (events) VGX_HIP(hipEventRecord(b->events[2 * i], s));
       unsigned lb, tb;
       grids(first + i, lb, tb);
-      k_bfs_level<<<lb, kBfsBlock, 0, s>>>(a, first + i, tb == 0 ? 1 : 0, build_queue(first + i) ? 1 : 0);
+      const uint32_t cl = claims(first + i);
+      k_bfs_level<<<lb, kBfsBlock, 0, s>>>(a, first + i, tb == 0 ? 1 : 0, cl);
       VGX_LAUNCH_CHECK();
       if (tb) {
         k_bfs_bottom_up_tail<<<tb, kBfsBlock, 0, s>>>(a, first + i);
+        VGX_LAUNCH_CHECK();
+      }
+      if (cl == kClaimDistance) {
+        k_bfs_pack<<<gpack, kBfsBlock, 0, s>>>(a, first + i);
         VGX_LAUNCH_CHECK();
       }
       if (events) VGX_HIP(hipEventRecord(b->events[2 * i + 1], s));
@@ -1168,7 +1236,7 @@ static void bfs_full(vgx_bfs* b) {
     for (uint32_t i = 0; i < chunk; ++i) {
       unsigned lb, tb;
       grids(i, lb, tb);
-      key.push_back((static_cast<uint64_t>(lb) << 32) | (static_cast<uint64_t>(build_queue(i)) << 31) | tb);
+      key.push_back((static_cast<uint64_t>(lb) << 32) | (static_cast<uint64_t>(claims(i)) << 30) | tb);
     }
     if (key == b->gkey) {
       if (!b->gexec) {
