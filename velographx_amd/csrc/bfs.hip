@@ -9,7 +9,9 @@
 // Full BFS: one kernel launch per level on a fixed grid.  Each workgroup
 // derives the level's direction from the previous level's counters
 // (Beamer's rule: top-down -> bottom-up when m_f > m_u / alpha, bottom-up ->
-// top-down when n_f < n / beta); no workgroup waits for another, and the
+// top-down when n_f < n_u / beta, with n_u the vertices with in-arcs still
+// unvisited, which is what a bottom-up pass checks: vertices without
+// in-arcs start visited); no workgroup waits for another, and the
 // host only checks for completion every few levels.
 //   top-down : the frontier is a packed queue (vertex, edge prefix); every
 //              wave takes an equal slice of the frontier's edges (load-
@@ -93,6 +95,9 @@ __device__ __forceinline__ unsigned long long found_total(const LevelRec& r) {
 }
 struct alignas(128) LevelCtl {
   unsigned long long visited_edges;  // edges of vertices visited before this level
+  // vertices with live in-arcs still unvisited when the previous level
+  // started (this level's frontier is still counted in it)
+  unsigned long long unvisited;
   uint32_t prev_mode;                // direction used by the previous level
   uint32_t no_queue;                 // the previous level built no frontier queue
 };
@@ -154,6 +159,7 @@ struct BfsWork {
   DevBuf<unsigned long long> ppre; // their edge prefixes
   DevBuf<uint4> head;
   DevBuf<unsigned long long> zin;  // vertices without live in-arcs (and ids >= n)
+  DevBuf<unsigned long long> zcount;  // bits set in zin
   uint64_t head_version = ~0ull, head_n = 0;
   // incremental
   DevBuf<uint32_t> lost, support, flags, stamp;
@@ -228,9 +234,11 @@ struct LevelArgs {
 };
 
 // head[v] = the first kHead entries of v's in-row (tombstone bits kept;
-// missing entries are tombstones).
+// missing entries are tombstones).  Also clears the count that k_build_zin,
+// launched next on the same stream, adds to.
 __global__ void k_build_head(const uint2* __restrict__ in_row, const uint32_t* __restrict__ in_arena, uint32_t n,
-                             uint4* __restrict__ head) {
+                             uint4* __restrict__ head, unsigned long long* __restrict__ zcount) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *zcount = 0;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     const uint2 rb = in_row[v];
     uint32_t h[kHead];
@@ -243,13 +251,24 @@ __global__ void k_build_head(const uint2* __restrict__ in_row, const uint32_t* _
 // Bit v set when v has no live in-arc (or v >= n).  Such a vertex can only
 // be reached as the root, so every traversal starts with these bits already
 // in the visited bitmap and the bottom-up sweeps skip them.
-__global__ void k_build_zin(const uint32_t* __restrict__ in_deg, uint32_t n, uint32_t words,
-                            unsigned long long* __restrict__ zin) {
-  const uint32_t W = gridDim.x * (blockDim.x / kWave);
-  for (uint32_t wi = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; wi < words; wi += W) {
+__global__ __launch_bounds__(kBfsBlock) void k_build_zin(const uint32_t* __restrict__ in_deg, uint32_t n,
+                                                         uint32_t words, unsigned long long* __restrict__ zin,
+                                                         unsigned long long* __restrict__ zcount) {
+  const uint32_t W = gridDim.x * kWavesPerBlock;
+  unsigned long long c = 0;
+  for (uint32_t wi = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; wi < words; wi += W) {
     const uint32_t v = wi * kWave + lane_id();
     const unsigned long long m = __ballot(v >= n || in_deg[v] == 0);
     if (lane_id() == 0) zin[wi] = m;
+    c += __popcll(m);
+  }
+  __shared__ unsigned long long s_c[kWavesPerBlock];
+  if (lane_id() == 0) s_c[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (uint32_t i = 0; i < kWavesPerBlock; ++i) t += s_c[i];
+    if (t) atomicAdd(zcount, t);
   }
 }
 
@@ -295,7 +314,7 @@ __device__ __forceinline__ bool rest_in_frontier(const LevelArgs& a, const unsig
 // host launches k_bfs_pack after the level, no atomic either: the pack
 // builds the frontier bitmap from the distances.  The next level then has no
 // frontier arc count: it runs bottom-up when the frontier is large by the
-// bottom-up rule (n_f * beta >= n), else top-down from the frontier bitmap,
+// bottom-up rule (n_f * beta >= n_u), else top-down from the frontier bitmap,
 // and counts the frontier's arcs as its size times the mean degree.
 __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t level, int inline_tail,
                                                          uint32_t claims) {
@@ -317,6 +336,9 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
                                                  static_cast<double>(st->n > 0 ? st->n : 1))
                : (qc & kEdgeMask);
   const unsigned long long seen = st->ctl[level & 3].visited_edges + mf;
+  // vertices a bottom-up pass of this level would check
+  const unsigned long long left = st->ctl[level & 3].unvisited;
+  const unsigned long long unvisited = left > nf ? left - nf : 0;
   // 0: top-down from the queue, 1: bottom-up, 2: top-down from the frontier
   // bitmap (after a bottom-up level, or a top-down level that built no queue)
   uint32_t mode = 0;
@@ -325,17 +347,18 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
     if (prev != 1) {
       const unsigned long long mu = st->total_edges > seen ? st->total_edges - seen : 0;
       if (no_queue)
-        mode = static_cast<double>(nf) * st->beta >= static_cast<double>(st->n) ? 1u : 2u;
+        mode = static_cast<double>(nf) * st->beta >= static_cast<double>(unvisited) ? 1u : 2u;
       else
         mode = static_cast<double>(mf) * st->alpha > static_cast<double>(mu) ? 1u : 0u;
     } else {
-      mode = static_cast<double>(nf) * st->beta < static_cast<double>(st->n) ? 2u : 1u;
+      mode = static_cast<double>(nf) * st->beta < static_cast<double>(unvisited) ? 2u : 1u;
     }
   }
   LevelRec* rn = &st->rec[(level + 1) & 3];
   if (leader) {
     st->ctl[(level + 1) & 3].prev_mode = mode;
     st->ctl[(level + 1) & 3].visited_edges = seen;
+    st->ctl[(level + 1) & 3].unvisited = unvisited;
     // only top-down levels fill a queue (bottom-up ones never do)
     st->ctl[(level + 1) & 3].no_queue = (mode != 1 && claims != kClaimQueue) ? 1u : 0u;
     st->rec[(level + 2) & 3].q = 0;
@@ -1045,7 +1068,8 @@ static LevelArgs level_args(vgx_bfs* b) {
 // the root placed (distance 0, visited, level-0 frontier and queue).
 // Workgroup 0 resets the level counters.  A root outside the graph leaves
 // the level-0 frontier empty, which ends the traversal at once.
-__global__ void k_bfs_init(LevelArgs a, const unsigned long long* __restrict__ zin, uint32_t root,
+__global__ void k_bfs_init(LevelArgs a, const unsigned long long* __restrict__ zin,
+                           const unsigned long long* __restrict__ zcount, uint32_t root,
                            unsigned long long total_edges, float alpha, float beta) {
   const uint64_t n = a.n, words = a.words;
   const bool has_root = root < n;
@@ -1069,6 +1093,7 @@ __global__ void k_bfs_init(LevelArgs a, const unsigned long long* __restrict__ z
     st->ctl[threadIdx.x].visited_edges = 0;
     st->ctl[threadIdx.x].prev_mode = 0;
     st->ctl[threadIdx.x].no_queue = 0;
+    st->ctl[threadIdx.x].unvisited = 0;
     st->pend[threadIdx.x].q = 0;
   }
   __syncthreads();
@@ -1081,6 +1106,10 @@ __global__ void k_bfs_init(LevelArgs a, const unsigned long long* __restrict__ z
   st->done_level = ~0u;
   st->reached = 0;
   st->log_pend[0] = 0;
+  // every vertex with in-arcs, and the root (level 0's frontier) either way
+  const unsigned long long cells = words * 64;
+  const unsigned long long z = *zcount;
+  st->ctl[0].unvisited = (cells > z ? cells - z : 0) + ((has_root && (zin[rw] & rbit)) ? 1 : 0);
   if (!has_root) return;
   st->rec[0].found[0].v = 1;
   const uint32_t len = a.row[root].y;
@@ -1110,12 +1139,13 @@ static void bfs_full(vgx_bfs* b) {
   if (b->w.head_version != g->version || b->w.head_n != n) {
     b->w.head.ensure(g->ncap + 1);
     b->w.zin.ensure(words + 1);
+    b->w.zcount.ensure(1);
     if (n) {
-      k_build_head<<<blocks_for(n, kBfsBlock, 4096), kBfsBlock, 0, s>>>(g->in().row.p, g->in().arena.p,
-                                                                       static_cast<uint32_t>(n), b->w.head.p);
+      k_build_head<<<blocks_for(n, kBfsBlock, 4096), kBfsBlock, 0, s>>>(
+          g->in().row.p, g->in().arena.p, static_cast<uint32_t>(n), b->w.head.p, b->w.zcount.p);
       VGX_LAUNCH_CHECK();
       k_build_zin<<<blocks_for(words, kWavesPerBlock, 4096), kBfsBlock, 0, s>>>(
-          g->in().deg.p, static_cast<uint32_t>(n), static_cast<uint32_t>(words), b->w.zin.p);
+          g->in().deg.p, static_cast<uint32_t>(n), static_cast<uint32_t>(words), b->w.zin.p, b->w.zcount.p);
       VGX_LAUNCH_CHECK();
     }
     b->w.head_version = g->version;
@@ -1142,7 +1172,6 @@ static void bfs_full(vgx_bfs* b) {
     tb = blocks;
     if (!adaptive) return;
     if (L >= b->prof_levels) {
-      lb = std::min(blocks, 4u);
       tb = 0;
       return;
     }
@@ -1205,7 +1234,7 @@ static void bfs_full(vgx_bfs* b) {
     }
   };
   auto launch_start = [&]() {
-    k_bfs_init<<<ginit, kBfsBlock, 0, s>>>(a, b->w.zin.p, b->source, total_edges, alpha, beta);
+    k_bfs_init<<<ginit, kBfsBlock, 0, s>>>(a, b->w.zin.p, b->w.zcount.p, b->source, total_edges, alpha, beta);
     VGX_LAUNCH_CHECK();
   };
   // A traversal that repeats the previous one's launch program exactly
