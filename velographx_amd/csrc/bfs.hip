@@ -426,30 +426,39 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
 
   if (mode == 2) {
     // Frontier vertices from the bitmap: rows of up to 32 entries are
-    // expanded by their own lane, longer rows by the whole wave.
-    for (uint32_t wi = gw; wi < words; wi += W) {
-      const unsigned long long fw = bm_cur[wi];
-      if (fw == 0) continue;
-      const uint32_t u = wi * 64 + lane;
-      const bool mine = (fw >> lane) & 1ull;
-      const uint2 rb = mine ? a.row[u] : make_uint2(0u, 0u);
-      const bool longr = rb.y > 32;
-      const uint32_t cnt = longr ? 0u : rb.y;
-      uint32_t most = cnt;
+    // expanded by their own lane, longer rows by the whole wave.  The wave's
+    // words (stride W) are loaded 64 at a time, one per lane, and only the
+    // non-empty ones are visited, so a sparse frontier costs one load.
+    for (uint32_t base = gw; base < words; base += kWave * W) {
+      const uint32_t wk = base + lane * W;
+      const unsigned long long fk = wk < words ? bm_cur[wk] : 0ull;
+      unsigned long long todo = __ballot(fk != 0);
+      while (todo) {
+        const int lw = __ffsll(static_cast<long long>(todo)) - 1;
+        todo &= todo - 1;
+        const uint32_t wi = base + static_cast<uint32_t>(lw) * W;
+        const unsigned long long fw = __shfl(fk, lw, kWave);
+        const uint32_t u = wi * 64 + lane;
+        const bool mine = (fw >> lane) & 1ull;
+        const uint2 rb = mine ? a.row[u] : make_uint2(0u, 0u);
+        const bool longr = rb.y > 32;
+        const uint32_t cnt = longr ? 0u : rb.y;
+        uint32_t most = cnt;
 #pragma unroll
-      for (int d = 1; d < kWave; d <<= 1) most = max(most, static_cast<uint32_t>(__shfl_xor(static_cast<int>(most), d, kWave)));
-      for (uint32_t j = 0; j < most; ++j) {
-        const bool act = j < cnt;
-        claim(act ? a.arena[rb.x + j] : kTomb, act);
-      }
-      unsigned long long lm = __ballot(longr);
-      while (lm) {
-        const int l = __ffsll(static_cast<long long>(lm)) - 1;
-        lm &= lm - 1;
-        const uint32_t bx = __shfl(rb.x, l, kWave), by = __shfl(rb.y, l, kWave);
-        for (uint32_t j = 0; j < by; j += kWave) {
-          const bool act = j + lane < by;
-          claim(act ? a.arena[bx + j + lane] : kTomb, act);
+        for (int d = 1; d < kWave; d <<= 1) most = max(most, static_cast<uint32_t>(__shfl_xor(static_cast<int>(most), d, kWave)));
+        for (uint32_t j = 0; j < most; ++j) {
+          const bool act = j < cnt;
+          claim(act ? a.arena[rb.x + j] : kTomb, act);
+        }
+        unsigned long long lm = __ballot(longr);
+        while (lm) {
+          const int l = __ffsll(static_cast<long long>(lm)) - 1;
+          lm &= lm - 1;
+          const uint32_t bx = __shfl(rb.x, l, kWave), by = __shfl(rb.y, l, kWave);
+          for (uint32_t j = 0; j < by; j += kWave) {
+            const bool act = j + lane < by;
+            claim(act ? a.arena[bx + j + lane] : kTomb, act);
+          }
         }
       }
     }
@@ -465,53 +474,64 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
     // Phase A: one lane per vertex checks the first kHead parents, read from
     // the dense head array (16 bytes per vertex, coalesced across the wave)
     // instead of the row itself.  Rows are sorted by id, so these are the
-    // vertex's lowest-id, on R-MAT highest-degree, parents.  Each wave takes
-    // two words per step and issues both words' loads before using them.
+    // vertex's lowest-id, on R-MAT highest-degree, parents.  The wave's
+    // visited words (stride W) are loaded 64 at a time, one per lane; only
+    // words with unvisited vertices are processed, two per step with both
+    // words' loads issued before use (each word belongs to one wave, so the
+    // loaded values stay current).
     const uint4 none = make_uint4(kTomb, kTomb, kTomb, kTomb);
-    for (uint32_t wi = gw; wi < words; wi += 2 * W) {
-      const uint32_t wj = wi + W;
-      const bool has_j = wj < words;
-      const unsigned long long vw0 = a.visited[wi];
-      const unsigned long long vw1 = has_j ? a.visited[wj] : ~0ull;
-      if ((vw0 & vw1) == ~0ull) continue;
-      const uint32_t v0 = wi * 64 + lane, v1 = wj * 64 + lane;
-      const bool unv0 = v0 < a.n && !((vw0 >> lane) & 1ull);
-      const bool unv1 = has_j && v1 < a.n && !((vw1 >> lane) & 1ull);
-      const uint4 h0 = unv0 ? a.head[v0] : none;
-      const uint4 h1 = unv1 ? a.head[v1] : none;
-      bool f0 = in_frontier(bm_cur, h0.x) | in_frontier(bm_cur, h0.y) | in_frontier(bm_cur, h0.z) |
-                in_frontier(bm_cur, h0.w);
-      bool f1 = in_frontier(bm_cur, h1.x) | in_frontier(bm_cur, h1.y) | in_frontier(bm_cur, h1.z) |
-                in_frontier(bm_cur, h1.w);
-      // the rest of longer rows goes to phase B (k_bfs_bottom_up_tail)
-      const uint2 r0 = (unv0 && !f0) ? a.in_row[v0] : make_uint2(0u, 0u);
-      const uint2 r1 = (unv1 && !f1) ? a.in_row[v1] : make_uint2(0u, 0u);
-      if (inline_tail) {
-        // recorded like phase B's list, so that the next traversal of this
-        // object can choose between the two
-        if (r0.y > kHead) pend_local += (1ull << kEdgeBits) + (r0.y - kHead);
-        if (r1.y > kHead) pend_local += (1ull << kEdgeBits) + (r1.y - kHead);
-        f0 = f0 || rest_in_frontier(a, bm_cur, r0, r0.y > kHead);
-        f1 = f1 || rest_in_frontier(a, bm_cur, r1, r1.y > kHead);
-      } else {
-        stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, r0.y > kHead, v0, r0.y - kHead);
-        stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, r1.y > kHead, v1, r1.y - kHead);
-      }
-      // Bottom-up levels build no queue: the next level reads the bitmap.
-      const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1);
-      if (lane == 0) {
-        if (m0) {
-          bm_next[wi] = m0;
-          a.visited[wi] = vw0 | m0;
+    for (uint32_t base = gw; base < words; base += kWave * W) {
+      const uint32_t wk = base + lane * W;
+      const unsigned long long vk = wk < words ? a.visited[wk] : ~0ull;
+      unsigned long long todo = __ballot(vk != ~0ull);
+      while (todo) {
+        const int l0 = __ffsll(static_cast<long long>(todo)) - 1;
+        todo &= todo - 1;
+        const bool has_j = todo != 0;
+        const int l1 = has_j ? __ffsll(static_cast<long long>(todo)) - 1 : l0;
+        if (has_j) todo &= todo - 1;
+        const uint32_t wi = base + static_cast<uint32_t>(l0) * W, wj = base + static_cast<uint32_t>(l1) * W;
+        const unsigned long long vw0 = __shfl(vk, l0, kWave);
+        const unsigned long long vw1 = has_j ? __shfl(vk, l1, kWave) : ~0ull;
+        const uint32_t v0 = wi * 64 + lane, v1 = wj * 64 + lane;
+        const bool unv0 = v0 < a.n && !((vw0 >> lane) & 1ull);
+        const bool unv1 = has_j && v1 < a.n && !((vw1 >> lane) & 1ull);
+        const uint4 h0 = unv0 ? a.head[v0] : none;
+        const uint4 h1 = unv1 ? a.head[v1] : none;
+        bool f0 = in_frontier(bm_cur, h0.x) | in_frontier(bm_cur, h0.y) | in_frontier(bm_cur, h0.z) |
+                  in_frontier(bm_cur, h0.w);
+        bool f1 = in_frontier(bm_cur, h1.x) | in_frontier(bm_cur, h1.y) | in_frontier(bm_cur, h1.z) |
+                  in_frontier(bm_cur, h1.w);
+        // the rest of longer rows goes to phase B (k_bfs_bottom_up_tail)
+        const uint2 r0 = (unv0 && !f0) ? a.in_row[v0] : make_uint2(0u, 0u);
+        const uint2 r1 = (unv1 && !f1) ? a.in_row[v1] : make_uint2(0u, 0u);
+        if (inline_tail) {
+          // recorded like phase B's list, so that the next traversal of this
+          // object can choose between the two
+          if (r0.y > kHead) pend_local += (1ull << kEdgeBits) + (r0.y - kHead);
+          if (r1.y > kHead) pend_local += (1ull << kEdgeBits) + (r1.y - kHead);
+          f0 = f0 || rest_in_frontier(a, bm_cur, r0, r0.y > kHead);
+          f1 = f1 || rest_in_frontier(a, bm_cur, r1, r1.y > kHead);
+        } else {
+          stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, r0.y > kHead, v0, r0.y - kHead);
+          stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, r1.y > kHead, v1, r1.y - kHead);
         }
-        if (m1) {
-          bm_next[wj] = m1;
-          a.visited[wj] = vw1 | m1;
+        // Bottom-up levels build no queue: the next level reads the bitmap.
+        const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1);
+        if (lane == 0) {
+          if (m0) {
+            bm_next[wi] = m0;
+            a.visited[wi] = vw0 | m0;
+          }
+          if (m1) {
+            bm_next[wj] = m1;
+            a.visited[wj] = vw1 | m1;
+          }
         }
+        if (f0) a.dist[v0] = next_level;
+        if (f1) a.dist[v1] = next_level;
+        found_local += __popcll(m0) + __popcll(m1);
       }
-      if (f0) a.dist[v0] = next_level;
-      if (f1) a.dist[v1] = next_level;
-      found_local += __popcll(m0) + __popcll(m1);
     }
   }
   // one atomic per workgroup and list for the last staged pushes
