@@ -13,7 +13,7 @@
 // unvisited, which is what a bottom-up pass checks: vertices without
 // in-arcs start visited); no workgroup waits for another, and the
 // host only checks for completion every few levels.
-//   top-down : the frontier is a packed queue (vertex, edge prefix); every
+//   top-down : the frontier is a packed queue (row begin, edge prefix); every
 //              wave takes an equal slice of the frontier's edges (load-
 //              balanced search), claims new vertices with one atomicOr on
 //              the visited bitmap and pushes them wave-aggregated.
@@ -153,7 +153,7 @@ struct BfsWork {
   DevBuf<uint32_t> dist;
   DevBuf<unsigned long long> visited;
   DevBuf<unsigned long long> bm;  // 3 bitmaps
-  DevBuf<uint32_t> queue[2];
+  DevBuf<uint32_t> queue[2];       // top-down frontier: row begins of its vertices
   DevBuf<unsigned long long> pre[2];
   DevBuf<uint32_t> pend;           // bottom-up rows left for phase B
   DevBuf<unsigned long long> ppre; // their edge prefixes
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
   uint32_t nbuf = 0, nbuf2 = 0;
   unsigned long long found_local = 0, pend_local = 0;
 
-  // Claim w for the next level if nobody has; queue it with its out-degree.
+  // Claim w for the next level if nobody has; queue its row (begin, length).
   // A level that builds no queue claims with the distance alone: every lane
   // that finds w unvisited stores the same value, with no atomic (a
   // device-scope atomic is a round trip past the L2), and k_bfs_pack then
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
       return;
     }
     bool newly = false;
-    uint32_t wl = 0;
+    uint2 wr = make_uint2(0u, 0u);
     if (consider && !(w & kTomb)) {
       const unsigned long long bit = 1ull << (w & 63);
       unsigned long long* wp = a.visited + (w >> 6);
@@ -418,10 +418,10 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
     if (newly) {
       a.dist[w] = next_level;
       atomicOr(bm_next + (w >> 6), 1ull << (w & 63));
-      if (claims == kClaimQueue) wl = a.row[w].y;
+      if (claims == kClaimQueue) wr = a.row[w];
     }
     found_local += __popcll(__ballot(newly));
-    stage_push(sb, nbuf, &rn->q, qn, pn, newly && wl > 0, w, wl);
+    stage_push(sb, nbuf, &rn->q, qn, pn, newly && wr.y > 0, wr.x, wr.y);
   };
 
   if (mode == 2) {
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
     const uint32_t F = static_cast<uint32_t>(qc >> kEdgeBits);
     const uint32_t* qcur = cur ? a.queue1 : a.queue0;
     const unsigned long long* pcur = cur ? a.pre1 : a.pre0;
-    lbs_edges(qcur, pcur, F, mf, a.row, gw, W,
+    lbs_edges<true>(qcur, pcur, F, mf, a.row, gw, W,
               [&](uint32_t, uint32_t, uint2, uint32_t slot, bool valid) {
                 claim(valid ? a.arena[slot] : kTomb, valid);
               });
@@ -1132,11 +1132,11 @@ __global__ void k_bfs_init(LevelArgs a, const unsigned long long* __restrict__ z
   st->ctl[0].unvisited = (cells > z ? cells - z : 0) + ((has_root && (zin[rw] & rbit)) ? 1 : 0);
   if (!has_root) return;
   st->rec[0].found[0].v = 1;
-  const uint32_t len = a.row[root].y;
-  if (len) {
-    a.queue0[0] = root;
+  const uint2 rr = a.row[root];
+  if (rr.y) {
+    a.queue0[0] = rr.x;
     a.pre0[0] = 0;
-    st->rec[0].q = (1ull << kEdgeBits) | len;
+    st->rec[0].q = (1ull << kEdgeBits) | rr.y;
   }
 }
 

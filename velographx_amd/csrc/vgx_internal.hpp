@@ -296,7 +296,10 @@ __device__ __forceinline__ void wave_push(unsigned long long* ctr, uint32_t* que
 // total E): each wave takes a contiguous edge range and, for every edge,
 // calls fn(item_index, vertex, row, arena_slot, lane_valid).  fn is called
 // by all lanes (valid=false for padding) so it may use wave intrinsics.
-template <class Fn>
+// With Begins set the queue holds each item's row begin instead of its
+// vertex (row is not read; fn gets the begin as both vertex and row.x), so
+// no dependent row load stands between an item and its arena entries.
+template <bool Begins = false, class Fn>
 __device__ __forceinline__ void lbs_edges(const uint32_t* queue, const unsigned long long* pre, uint32_t F,
                                           unsigned long long E, const uint2* row, uint32_t gw, uint32_t W,
                                           Fn&& fn) {
@@ -345,7 +348,11 @@ __device__ __forceinline__ void lbs_edges(const uint32_t* queue, const unsigned 
     const uint32_t k = __popcll(bit & ((2ull << lane) - 1ull));
     const uint32_t v = __shfl(qk, static_cast<int>(k), kWave);
     const unsigned long long pv = __shfl(pk, static_cast<int>(k), kWave);
-    const uint2 rb = valid ? row[v] : make_uint2(0u, 0u);
+    uint2 rb;
+    if constexpr (Begins)
+      rb = make_uint2(v, 0u);
+    else
+      rb = valid ? row[v] : make_uint2(0u, 0u);
     const uint32_t slot = valid ? rb.x + static_cast<uint32_t>(g - pv) : 0u;
     // the item containing edge e + 64: items of the window that start at or
     // before it, plus item i0 + 64 when all 63 others do
