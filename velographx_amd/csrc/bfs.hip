@@ -64,6 +64,8 @@ constexpr uint64_t kTdWindowsPerWave = 2;
 // (kClaimBitmap), or by storing the distance alone, racing stores of the
 // same value, after which k_bfs_pack derives the frontier (kClaimDistance).
 enum Claims : uint32_t { kClaimQueue = 0, kClaimBitmap = 1, kClaimDistance = 2 };
+// Words of the frontier bitmap one wave of k_bfs_pack derives per step.
+constexpr uint32_t kPackWords = 16;
 
 // Per-level record, kept in a ring of 4 indexed by level & 3.  Level L's
 // kernel accumulates the frontier of level L+1 into rec[(L+1)&3] (q, found)
@@ -560,9 +562,9 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
 }
 
 // After a top-down level that built no queue: the vertices it claimed are
-// those at distance level + 1.  One wave per 64-vertex word sets the next
-// frontier bitmap and the visited bits from the distances (coalesced) and
-// counts them for the next level.  Exits at once after any other level.
+// those at distance level + 1.  Waves read the distances (coalesced), set
+// the next frontier bitmap and the visited bits from them and count them for
+// the next level.  Exits at once after any other level.
 __global__ __launch_bounds__(kBfsBlock) void k_bfs_pack(LevelArgs a, uint32_t level) {
   BfsState* st = a.st;
   if (st->done_level <= level || !st->ctl[(level + 1) & 3].no_queue) return;
@@ -572,22 +574,27 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_pack(LevelArgs a, uint32_t le
   const uint32_t gw = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   const unsigned lane = lane_id();
   unsigned long long found_local = 0;
-  // four words per step, their distance loads issued together
-  for (uint32_t w0 = gw * 4; w0 < words; w0 += W * 4) {
-    uint32_t d[4];
+  // kPackWords words per step: their distance loads and visited words are
+  // issued together, and lane k writes word k's results.
+  for (uint32_t w0 = gw * kPackWords; w0 < words; w0 += W * kPackWords) {
+    uint32_t d[kPackWords];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (uint32_t k = 0; k < kPackWords; ++k) {
       const uint32_t v = (w0 + k) * 64 + lane;
       d[k] = v < a.n ? a.dist[v] : kInf;
     }
+    const bool mine = lane < kPackWords && w0 + lane < words;
+    const unsigned long long vis = mine ? a.visited[w0 + lane] : 0ull;
+    unsigned long long mk = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (uint32_t k = 0; k < kPackWords; ++k) {
       const unsigned long long m = __ballot(d[k] == next_level);
-      if (m && lane == 0) {
-        bm_next[w0 + k] = m;
-        a.visited[w0 + k] |= m;
-      }
+      if (lane == k) mk = m;
       found_local += __popcll(m);
+    }
+    if (mine && mk) {
+      bm_next[w0 + lane] = mk;
+      a.visited[w0 + lane] = vis | mk;
     }
   }
   __shared__ unsigned long long s_found[kWavesPerBlock];
@@ -1228,7 +1235,7 @@ static void bfs_full(vgx_bfs* b) {
   const unsigned long long total_edges = g->adj[0].stored;
   const float alpha = static_cast<float>(b->tune.alpha), beta = static_cast<float>(b->tune.beta);
   const unsigned ginit = blocks_for(std::max<uint64_t>(n, 1), kBfsBlock, 2048);
-  const unsigned gpack = blocks_for(std::max<uint64_t>(words, 1), 4 * kWavesPerBlock, 1024);
+  const unsigned gpack = blocks_for(std::max<uint64_t>(words, 1), kPackWords * kWavesPerBlock, 1024);
   auto launch_levels = [&](uint32_t first, uint32_t count, bool events) {
     if (events && b->events.size() < 2 * static_cast<size_t>(count)) {
       size_t old = b->events.size();
