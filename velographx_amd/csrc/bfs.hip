@@ -64,6 +64,11 @@ constexpr uint64_t kTdWindowsPerWave = 2;
 // (kClaimBitmap), or by storing the distance alone, racing stores of the
 // same value, after which k_bfs_pack derives the frontier (kClaimDistance).
 enum Claims : uint32_t { kClaimQueue = 0, kClaimBitmap = 1, kClaimDistance = 2 };
+// The count of zero-in-degree bits is added by every workgroup of
+// k_build_zin into one of kZinStripes partial sums, kStripeWords apart (one
+// 128-byte line each): additions to one address serialise.
+constexpr uint32_t kZinStripes = 16;
+constexpr uint32_t kStripeWords = 16;
 // Words of the frontier bitmap one wave of k_bfs_pack derives per step.
 constexpr uint32_t kPackWords = 16;
 
@@ -161,7 +166,7 @@ struct BfsWork {
   DevBuf<unsigned long long> ppre; // their edge prefixes
   DevBuf<uint4> head;
   DevBuf<unsigned long long> zin;  // vertices without live in-arcs (and ids >= n)
-  DevBuf<unsigned long long> zcount;  // bits set in zin
+  DevBuf<unsigned long long> zcount;  // bits set in zin, in kZinStripes striped partial sums
   uint64_t head_version = ~0ull, head_n = 0;
   // incremental
   DevBuf<uint32_t> lost, support, flags, stamp;
@@ -240,7 +245,7 @@ struct LevelArgs {
 // launched next on the same stream, adds to.
 __global__ void k_build_head(const uint2* __restrict__ in_row, const uint32_t* __restrict__ in_arena, uint32_t n,
                              uint4* __restrict__ head, unsigned long long* __restrict__ zcount) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *zcount = 0;
+  if (blockIdx.x == 0 && threadIdx.x < kZinStripes) zcount[threadIdx.x * kStripeWords] = 0;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     const uint2 rb = in_row[v];
     uint32_t h[kHead];
@@ -270,7 +275,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_build_zin(const uint32_t* __restr
   if (threadIdx.x == 0) {
     unsigned long long t = 0;
     for (uint32_t i = 0; i < kWavesPerBlock; ++i) t += s_c[i];
-    if (t) atomicAdd(zcount, t);
+    if (t) atomicAdd(&zcount[(blockIdx.x % kZinStripes) * kStripeWords], t);
   }
 }
 
@@ -1135,7 +1140,8 @@ __global__ void k_bfs_init(LevelArgs a, const unsigned long long* __restrict__ z
   st->log_pend[0] = 0;
   // every vertex with in-arcs, and the root (level 0's frontier) either way
   const unsigned long long cells = words * 64;
-  const unsigned long long z = *zcount;
+  unsigned long long z = 0;
+  for (uint32_t j = 0; j < kZinStripes; ++j) z += zcount[j * kStripeWords];
   st->ctl[0].unvisited = (cells > z ? cells - z : 0) + ((has_root && (zin[rw] & rbit)) ? 1 : 0);
   if (!has_root) return;
   st->rec[0].found[0].v = 1;
@@ -1166,12 +1172,13 @@ static void bfs_full(vgx_bfs* b) {
   if (b->w.head_version != g->version || b->w.head_n != n) {
     b->w.head.ensure(g->ncap + 1);
     b->w.zin.ensure(words + 1);
-    b->w.zcount.ensure(1);
+    b->w.zcount.ensure(kZinStripes * kStripeWords);
     if (n) {
-      k_build_head<<<blocks_for(n, kBfsBlock, 4096), kBfsBlock, 0, s>>>(
+      // one vertex per thread: each is a chain of two dependent loads
+      k_build_head<<<blocks_for(n, kBfsBlock, 65535), kBfsBlock, 0, s>>>(
           g->in().row.p, g->in().arena.p, static_cast<uint32_t>(n), b->w.head.p, b->w.zcount.p);
       VGX_LAUNCH_CHECK();
-      k_build_zin<<<blocks_for(words, kWavesPerBlock, 4096), kBfsBlock, 0, s>>>(
+      k_build_zin<<<blocks_for(words, kWavesPerBlock, 1024), kBfsBlock, 0, s>>>(
           g->in().deg.p, static_cast<uint32_t>(n), static_cast<uint32_t>(words), b->w.zin.p, b->w.zcount.p);
       VGX_LAUNCH_CHECK();
     }
