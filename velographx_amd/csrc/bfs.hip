@@ -50,6 +50,10 @@ constexpr uint32_t kWavesPerBlock = kBfsBlock / kWave;
 // Bottom-up phase A checks the first kHead parents of a vertex from the
 // dense head array; the rest of longer rows is scanned by phase B.
 constexpr uint32_t kHead = 4;
+// Head entry past the end of a shorter row (tombstone bit set, so it is
+// never in a frontier; distinct from any tombstoned arc, since the graph
+// accepts arc endpoints below 2^31 - 1 only).
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
 // A bottom-up level whose previous traversal left at most this many rows
 // (and edges) beyond the head finishes them inside the level kernel instead
 // of a phase-B launch.
@@ -241,7 +245,8 @@ struct LevelArgs {
 };
 
 // head[v] = the first kHead entries of v's in-row (tombstone bits kept;
-// missing entries are tombstones).  Also clears the count that k_build_zin,
+// missing entries are kNoEntry, so a head whose last entry is kNoEntry
+// holds the whole row).  Also clears the count that k_build_zin,
 // launched next on the same stream, adds to.
 __global__ void k_build_head(const uint2* __restrict__ in_row, const uint32_t* __restrict__ in_arena, uint32_t n,
                              uint4* __restrict__ head, unsigned long long* __restrict__ zcount) {
@@ -250,7 +255,7 @@ __global__ void k_build_head(const uint2* __restrict__ in_row, const uint32_t* _
     const uint2 rb = in_row[v];
     uint32_t h[kHead];
 #pragma unroll
-    for (uint32_t t = 0; t < kHead; ++t) h[t] = t < rb.y ? in_arena[rb.x + t] : kTomb;
+    for (uint32_t t = 0; t < kHead; ++t) h[t] = t < rb.y ? in_arena[rb.x + t] : kNoEntry;
     head[v] = make_uint4(h[0], h[1], h[2], h[3]);
   }
 }
@@ -509,9 +514,10 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
                   in_frontier(bm_cur, h0.w);
         bool f1 = in_frontier(bm_cur, h1.x) | in_frontier(bm_cur, h1.y) | in_frontier(bm_cur, h1.z) |
                   in_frontier(bm_cur, h1.w);
-        // the rest of longer rows goes to phase B (k_bfs_bottom_up_tail)
-        const uint2 r0 = (unv0 && !f0) ? a.in_row[v0] : make_uint2(0u, 0u);
-        const uint2 r1 = (unv1 && !f1) ? a.in_row[v1] : make_uint2(0u, 0u);
+        // the rest of longer rows goes to phase B (k_bfs_bottom_up_tail);
+        // a head that ends in kNoEntry is the whole row
+        const uint2 r0 = (unv0 && !f0 && h0.w != kNoEntry) ? a.in_row[v0] : make_uint2(0u, 0u);
+        const uint2 r1 = (unv1 && !f1 && h1.w != kNoEntry) ? a.in_row[v1] : make_uint2(0u, 0u);
         if (inline_tail) {
           // recorded like phase B's list, so that the next traversal of this
           // object can choose between the two
