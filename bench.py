@@ -741,7 +741,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": None,
-            "kernel": "one traversal: k_bfs_init, k_bfs_level, k_bfs_bottom_up_tail",
+            "kernel": "one traversal: k_bfs_init, k_bfs_level, k_bfs_bottom_up_tail, k_bfs_pack",
             "bytes_per_bfs": bytes_bfs,
             "gpu_ms_per_bfs": kernel_ms_per_bfs,
             "avg_launch_us": 1e3 * level_ms_eager / max(launches_per_bfs, 1),
