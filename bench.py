@@ -205,14 +205,14 @@ def _counter_rows(directory):
 
 
 def _per_bfs(rows):
-    """Counter sums (KiB) of the kernels of each traversal (init, level
-    and tail kernels), split at k_bfs_init (one per traversal)."""
+    """Counter sums (KiB) of the kernels of each traversal (init, level,
+    tail and pack kernels), split at k_bfs_init (one per traversal)."""
     groups = []
     for r in rows:
         name = r["Kernel_Name"]
         if "k_bfs_init" in name:
             groups.append(float(r["Counter_Value"]))
-        elif groups and any(k in name for k in ("k_bfs_level", "k_bfs_bottom_up_tail")):
+        elif groups and any(k in name for k in ("k_bfs_level", "k_bfs_bottom_up_tail", "k_bfs_pack")):
             groups[-1] += float(r["Counter_Value"])
     return groups
 
