@@ -73,6 +73,9 @@ enum Claims : uint32_t { kClaimQueue = 0, kClaimBitmap = 1, kClaimDistance = 2 }
 // 128-byte line each): additions to one address serialise.
 constexpr uint32_t kZinStripes = 16;
 constexpr uint32_t kStripeWords = 16;
+// Top-down levels over at most this many frontier arcs get one 64-edge
+// window per wave instead of kTdWindowsPerWave.
+constexpr uint64_t kLightEdges = 65536;
 // Words of the frontier bitmap one wave of k_bfs_pack derives per step.
 constexpr uint32_t kPackWords = 16;
 
@@ -1222,7 +1225,9 @@ static void bfs_full(vgx_bfs* b) {
       tb = 0;
     if (b->prof_mode[L] == 0) {
       const uint64_t windows = (b->prof_mf[L] + kWave - 1) / kWave;  // 64-edge windows
-      const uint64_t per_block = kTdWindowsPerWave * kWavesPerBlock;
+      // a light level is one chain of round trips per window: one window per wave
+      const uint64_t per_wave = b->prof_mf[L] <= kLightEdges ? 1 : kTdWindowsPerWave;
+      const uint64_t per_block = per_wave * kWavesPerBlock;
       const uint64_t want = (windows + per_block - 1) / per_block;
       lb = static_cast<unsigned>(std::min<uint64_t>(blocks, std::max<uint64_t>(16, want)));
     }
