@@ -513,12 +513,14 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
         const bool unv1 = has_j && v1 < a.n && !((vw1 >> lane) & 1ull);
         const uint4 h0 = unv0 ? a.head[v0] : none;
         const uint4 h1 = unv1 ? a.head[v1] : none;
-        // the first parent, then the other three only for the lanes still
-        // searching (inactive lanes send no address)
+        // the first parent, then the second, then the last two, each only
+        // for the lanes still searching (inactive lanes send no address)
         bool f0 = in_frontier(bm_cur, h0.x);
         bool f1 = in_frontier(bm_cur, h1.x);
-        if (!f0) f0 = in_frontier(bm_cur, h0.y) | in_frontier(bm_cur, h0.z) | in_frontier(bm_cur, h0.w);
-        if (!f1) f1 = in_frontier(bm_cur, h1.y) | in_frontier(bm_cur, h1.z) | in_frontier(bm_cur, h1.w);
+        if (!f0) f0 = in_frontier(bm_cur, h0.y);
+        if (!f1) f1 = in_frontier(bm_cur, h1.y);
+        if (!f0) f0 = in_frontier(bm_cur, h0.z) | in_frontier(bm_cur, h0.w);
+        if (!f1) f1 = in_frontier(bm_cur, h1.z) | in_frontier(bm_cur, h1.w);
         // the rest of longer rows goes to phase B (k_bfs_bottom_up_tail);
         // a head that ends in kNoEntry is the whole row
         const uint2 r0 = (unv0 && !f0 && h0.w != kNoEntry) ? a.in_row[v0] : make_uint2(0u, 0u);
