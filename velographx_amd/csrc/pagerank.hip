@@ -11,7 +11,7 @@
 //
 // One iteration is one launch over the graph's live in-rows (deleted
 // entries tombstoned and skipped).  Vertices are split by stored row length:
-// rows of <= 32 entries are summed by one lane, rows up to 4096 by one wave,
+// rows of <= 32 entries are summed by groups of 8 lanes, rows up to 4096 by one wave,
 // larger rows in 4096-entry chunks by several waves whose
 // partial sums the last workgroup combines.  The last workgroup also reduces
 // the per-workgroup residual and dangling partials in a fixed order, sets
@@ -33,6 +33,8 @@ extern thread_local std::string t_last_error;
 
 constexpr int kPrBlock = 256;
 constexpr uint32_t kSmallRow = 32;
+// Lanes per short row in a sweep.
+constexpr uint32_t kSmallGroup = 8;
 constexpr uint32_t kChunk = 4096;
 // Workgroups per in-degree class and launch; each class loops grid-stride,
 // so the last workgroup reduces at most 3 x 1024 partials.
@@ -182,15 +184,28 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_iter(PrArgs a) {
   const uint32_t b = blockIdx.x;
   // Each class is covered grid-stride by its own range of workgroups.
   if (b < a.bs) {
-    for (uint32_t i = b * kPrBlock + threadIdx.x; i < a.ns; i += a.bs * kPrBlock) {
-      const uint32_t v = a.slist[i];
+    // Short rows by groups of kSmallGroup lanes: a group's loads of one row
+    // are consecutive (coalesced), and a wave's lanes stay busy however the
+    // row lengths vary, where a lane per row would send 64 scattered
+    // addresses per arena load and idle on the wave's shortest rows.
+    constexpr uint32_t G = kSmallGroup;
+    const uint32_t lane = lane_id(), sub = lane % G;
+    const uint32_t gwave = b * (kPrBlock / kWave) + threadIdx.x / kWave;
+    const uint32_t nwaves = a.bs * (kPrBlock / kWave);
+    for (uint32_t i0 = gwave * (kWave / G); i0 < a.ns; i0 += nwaves * (kWave / G)) {
+      const uint32_t i = i0 + lane / G;
+      const bool act = i < a.ns;
+      const uint32_t v = act ? a.slist[i] : 0u;
+      const uint2 rb = act ? a.row[v] : make_uint2(0u, 0u);
       Acc<T> acc;
-      const uint2 rb = a.row[v];
-      for (uint32_t e = rb.x; e < rb.x + rb.y; ++e) {
+      for (uint32_t e = rb.x + sub; e < rb.x + rb.y; e += G) {
         const uint32_t x = a.arena[e];
         if (!(x & kTomb)) acc.add(contrib[x]);
       }
-      finish_vertex<T>(a, v, acc.get(), base, rank, next, cnext, l1, linf, dang);
+      T sum = acc.get();
+#pragma unroll
+      for (uint32_t d = 1; d < G; d <<= 1) sum += __shfl_xor(sum, static_cast<int>(d), kWave);
+      if (act && sub == 0) finish_vertex<T>(a, v, sum, base, rank, next, cnext, l1, linf, dang);
     }
   } else if (b < a.bs + a.bm) {
     for (uint32_t i = (b - a.bs) * (kPrBlock / kWave) + threadIdx.x / kWave; i < a.nm;
@@ -633,7 +648,8 @@ static PrArgs pr_args(vgx_pagerank* p, DevBuf<uint8_t>* rk) {
   a.nm = static_cast<uint32_t>(p->nm);
   a.nh = static_cast<uint32_t>(p->nh);
   a.nchunks = static_cast<uint32_t>(p->nchunks);
-  a.bs = static_cast<uint32_t>(std::min<uint64_t>((p->ns + kPrBlock - 1) / kPrBlock, kMaxClassBlocks));
+  a.bs = static_cast<uint32_t>(
+      std::min<uint64_t>((p->ns + kPrBlock / kSmallGroup - 1) / (kPrBlock / kSmallGroup), kMaxClassBlocks));
   a.bm = static_cast<uint32_t>(std::min<uint64_t>((p->nm + 3) / 4, kMaxClassBlocks));
   a.n = static_cast<uint32_t>(g->n);
   a.rank0 = rk[0].p;
