@@ -410,3 +410,45 @@ def test_replayed_traversals_stay_exact(vx, orc):
         for _ in range(3):
             bfs.recompute()
             assert np.array_equal(bfs.distances_numpy(), full)
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_distance_claims_and_pack_stay_exact(vx, orc, directed):
+    """A top-down level that the previous traversal of the same object saw
+    followed by a large bottom-up level (frontier of at least n/beta
+    vertices) claims by storing distances alone; a pack kernel then rebuilds
+    the frontier bitmap, the visited bits and the frontier size from them.
+    The first traversal's level log shows where that applies; the next
+    traversals (eager, then replayed) must stay bit-exact."""
+    from velographx_amd import workloads as wl
+
+    scale = 16
+    n = 1 << scale
+    s, d = vx.rmat_edges(scale, 16, 3)
+    g = vx.Graph(n, directed)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(n, directed)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    if directed:
+        roots = [int(r) for r in np.argsort(-np.bincount(s, minlength=n))[:16]]
+    else:
+        lcc = wl.largest_component_from_hub(vx, g)
+        roots = [int(r) for r in wl.roots_in_component(lcc, 16, seed=2)]
+    exercised = 0
+    for root in roots:
+        bfs = vx.IncrementalBFS(g, root)
+        bfs.set_profiling(1)
+        bfs.recompute()
+        log = bfs.level_log()
+        full, _ = orc.model_bfs(mg, root)
+        assert np.array_equal(bfs.distances_numpy(), full)
+        hit = any(log[i]["mode"] != "bottom-up" and log[i + 1]["mode"] == "bottom-up"
+                  and log[i + 1]["frontier_vertices"] * 18.0 >= n for i in range(len(log) - 1))
+        if not hit:
+            continue
+        exercised += 1
+        bfs.set_profiling(0)
+        for _ in range(3):
+            bfs.recompute()
+            assert np.array_equal(bfs.distances_numpy(), full), root
+    assert exercised >= 1

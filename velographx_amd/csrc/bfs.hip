@@ -293,7 +293,8 @@ __device__ __forceinline__ bool in_frontier(const unsigned long long* bm, uint32
 
 // Whether a parent of v beyond the head is in the frontier: rows with up to
 // 64 more entries on the lane itself, longer rows by the whole wave, first
-// hit ends the scan.  Called by the whole wave; `want` marks the lanes asking.
+// hit ends the scan.  Must be called by every lane of the wave (all lanes
+// take part in the scan of a long row); `want` marks the lanes asking.
 __device__ __forceinline__ bool rest_in_frontier(const LevelArgs& a, const unsigned long long* bm_cur, uint2 rb,
                                                  bool want) {
   bool found = false;
@@ -530,8 +531,13 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
           // object can choose between the two
           if (r0.y > kHead) pend_local += (1ull << kEdgeBits) + (r0.y - kHead);
           if (r1.y > kHead) pend_local += (1ull << kEdgeBits) + (r1.y - kHead);
-          f0 = f0 || rest_in_frontier(a, bm_cur, r0, r0.y > kHead);
-          f1 = f1 || rest_in_frontier(a, bm_cur, r1, r1.y > kHead);
+          // called by every lane: a long row is scanned by the whole wave
+          // (a short-circuit `f0 || ...` would leave the lanes that already
+          // found a parent out of that scan, and skip the entries they own)
+          const bool g0 = rest_in_frontier(a, bm_cur, r0, r0.y > kHead);
+          const bool g1 = rest_in_frontier(a, bm_cur, r1, r1.y > kHead);
+          f0 = f0 || g0;
+          f1 = f1 || g1;
         } else {
           stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, r0.y > kHead, v0, r0.y - kHead);
           stage_push(sb2, nbuf2, &st->pend[level & 3].q, a.pend, a.ppre, r1.y > kHead, v1, r1.y - kHead);
