@@ -452,3 +452,28 @@ def test_distance_claims_and_pack_stay_exact(vx, orc, directed):
             bfs.recompute()
             assert np.array_equal(bfs.distances_numpy(), full), root
     assert exercised >= 1
+
+
+@pytest.mark.parametrize("scale,seed,directed", [(12, 1, False), (13, 2, True), (14, 4, False), (15, 5, True),
+                                                 (15, 6, False)])
+def test_repeated_traversals_random_graphs(vx, orc, scale, seed, directed):
+    """Every traversal after the first of an object sizes its launches, tail
+    handling and claim modes from the one before; four traversals per root
+    (the later ones replayed) over R-MAT graphs of several sizes, both
+    directions, must all equal the oracle."""
+    n = 1 << scale
+    s, d = vx.rmat_edges(scale, 16, seed)
+    g = vx.Graph(n, directed)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(n, directed)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    rng = np.random.default_rng(seed)
+    hubs = [int(r) for r in np.argsort(-np.bincount(s, minlength=n))[:4]]
+    roots = hubs + [int(r) for r in rng.integers(0, n, size=8)]
+    for root in roots:
+        full, _ = orc.model_bfs(mg, root)
+        bfs = vx.IncrementalBFS(g, root)
+        for t in range(4):
+            if t:
+                bfs.recompute()
+            assert np.array_equal(bfs.distances_numpy(), full), (root, t)
