@@ -16,10 +16,14 @@
 //   top-down : the frontier is a packed queue (row begin, edge prefix); every
 //              wave takes an equal slice of the frontier's edges (load-
 //              balanced search), claims new vertices with one atomicOr on
-//              the visited bitmap and pushes them wave-aggregated.
+//              the visited bitmap and pushes them wave-aggregated.  Before a
+//              large bottom-up level no queue is built, and where the level
+//              itself was top-down last time the claim is the distance alone
+//              (k_bfs_pack derives the frontier from the distances).
 //   bottom-up: every wave owns 64 consecutive vertices (one visited word);
-//              unvisited lanes scan their in-row (parents) until one is in
-//              the frontier bitmap; one ballot gives the next frontier word.
+//              unvisited lanes check the first parents from the head array,
+//              then the rest of their in-row (inline or in
+//              k_bfs_bottom_up_tail); one ballot gives the next frontier word.
 //
 // Incremental repair (apply):
 //   1. deletions are applied first (tombstones), then vertices that lost
