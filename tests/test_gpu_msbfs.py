@@ -186,3 +186,53 @@ def test_graph_mode_deep_path(vx):
         assert np.array_equal(ms.distances(0), np.arange(n, dtype=np.uint32))
         assert np.array_equal(ms.distances(1), np.abs(np.arange(n) - 50).astype(np.uint32))
         assert ms.level_count == n
+
+
+@pytest.mark.parametrize("graph_mode", [True, False])
+def test_program_length_follows_depth(vx, graph_mode):
+    """The launch program is sized from the previous pass's depth: a shallow
+    pass (a star) is followed by a 60-level pass (the star turned into a
+    path), which continues over several short programs, then by the star
+    again; every pass is exact."""
+    n = 60
+    star = np.array([(0, i) for i in range(1, n)], dtype=np.uint32)
+    path = np.array([(i, i + 1) for i in range(n - 1)], dtype=np.uint32)
+    g = vx.Graph(0, False)
+    g.bulk_load_edges(star)
+    ms = vx.MultiSourceBFS(g, [0, n - 1])
+    ms.set_graph(graph_mode)
+
+    def check(edges):
+        ms.run(g)
+        from collections import deque
+
+        adj = [[] for _ in range(n)]
+        for u, v in edges:
+            adj[u].append(v)
+            adj[v].append(u)
+        depth = 0
+        for r, root in enumerate((0, n - 1)):
+            want = np.full(n, INF, dtype=np.uint32)
+            want[root] = 0
+            q = deque([root])
+            while q:
+                u = q.popleft()
+                for v in adj[u]:
+                    if want[v] == INF:
+                        want[v] = want[u] + 1
+                        q.append(v)
+            assert np.array_equal(ms.distances(r), want), r
+            depth = max(depth, int(want.max()) + 1)
+        assert ms.level_count == depth
+
+    for _ in range(2):
+        check(star)
+    ones = np.ones(n - 1, dtype=np.uint8)
+    zeros = np.zeros(n - 1, dtype=np.uint8)
+    g.apply_arrays(np.concatenate([star[:, 0], path[:, 0]]), np.concatenate([star[:, 1], path[:, 1]]),
+                   np.concatenate([zeros, ones]))
+    for _ in range(2):
+        check(path)
+    g.apply_arrays(np.concatenate([path[:, 0], star[:, 0]]), np.concatenate([path[:, 1], star[:, 1]]),
+                   np.concatenate([zeros, ones]))
+    check(star)

@@ -137,7 +137,7 @@ struct vgx_msbfs {
   bool use_graph = false;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
-  uint64_t graph_key[4] = {0, 0, 0, 0};
+  uint64_t graph_key[5] = {0, 0, 0, 0, 0};
   uint64_t replays = 0;
 };
 
@@ -476,13 +476,14 @@ struct MsGrids {
   unsigned start, shrt, out_long, in_long, fin;
 };
 
-// The launch program of kMsProgram levels.  Only kernel launches: it is
-// also what gets captured into the hipGraph.
-static void ms_program(const vgx_msbfs* m, const MsGrids& gr, int fresh, uint32_t first_dir, hipStream_t s) {
+// The launch program of `prog` (at most kMsProgram) levels.  Only kernel
+// launches: it is also what gets captured into the hipGraph.
+static void ms_program(const vgx_msbfs* m, const MsGrids& gr, uint32_t prog, int fresh, uint32_t first_dir,
+                       hipStream_t s) {
   const MsArgs* pa = m->d_args;
   k_ms_start<<<gr.start, kMsBlock, 0, s>>>(pa, fresh, first_dir);
   VGX_LAUNCH_CHECK();
-  for (uint32_t l = 0; l < kMsProgram; ++l) {
+  for (uint32_t l = 0; l < prog; ++l) {
     k_ms_push_short<<<gr.shrt, kMsBlock, 0, s>>>(pa, l);
     VGX_LAUNCH_CHECK();
     k_ms_push_long<<<gr.out_long, kMsBlock, 0, s>>>(pa, l);
@@ -522,6 +523,12 @@ static void ms_run(vgx_msbfs* m) {
   m->seen.ensure(n + 1);
   m->droots.ensure(m->k + 1);
   VGX_HIP(hipMemcpyAsync(m->droots.p, m->roots.data(), m->k * 4, hipMemcpyHostToDevice, s));
+  // Levels per program: the previous pass's depth plus two (levels past
+  // the end of a pass still cost a launch each; the slack keeps a pass one
+  // level deeper within one program), between 4 and kMsProgram; a deeper
+  // pass continues with another program.
+  const uint32_t prog =
+      m->levels ? std::min<uint32_t>(kMsProgram, std::max<uint32_t>(4, m->levels + 2)) : kMsProgram;
   m->levels = 0;
   m->push_levels = 0;
   if (n == 0) {
@@ -534,12 +541,12 @@ static void ms_run(vgx_msbfs* m) {
   gr.out_long = ms_chunk_blocks(po.nchunks);
   gr.in_long = ms_chunk_blocks(pi.nchunks);
   gr.fin = ms_blocks(n, kMsBlock, 1024);
-  // Programs of kMsProgram levels; the first starts from the roots, the
-  // next ones from the last level of the previous program.
+  // Programs of `prog` levels; the first starts from the roots, the next
+  // ones from the last level of the previous program.
   uint32_t base = 0;
   uint32_t first_dir = 0;
   for (;;) {
-    ms_ensure_levels(m, base + kMsProgram + 2);
+    ms_ensure_levels(m, base + prog + 2);
     MsArgs a{};
     a.out_row = out.row.p;
     a.out_arena = out.arena.p;
@@ -565,20 +572,20 @@ static void ms_run(vgx_msbfs* m) {
     VGX_HIP(hipMemcpyAsync(m->d_args, &a, sizeof(a), hipMemcpyHostToDevice, s));
     const bool fresh = base == 0;
     if (m->use_graph && fresh) {
-      const uint64_t key[4] = {n, (static_cast<uint64_t>(gr.shrt) << 32) | gr.fin,
-                               (static_cast<uint64_t>(gr.out_long) << 32) | gr.in_long, gr.start};
-      if (!m->exec || !std::equal(key, key + 4, m->graph_key)) {
+      const uint64_t key[5] = {n, (static_cast<uint64_t>(gr.shrt) << 32) | gr.fin,
+                               (static_cast<uint64_t>(gr.out_long) << 32) | gr.in_long, gr.start, prog};
+      if (!m->exec || !std::equal(key, key + 5, m->graph_key)) {
         ms_drop_graph(m);
         VGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        ms_program(m, gr, 1, 0, s);
+        ms_program(m, gr, prog, 1, 0, s);
         VGX_HIP(hipStreamEndCapture(s, &m->graph));
         VGX_HIP(hipGraphInstantiate(&m->exec, m->graph, nullptr, nullptr, 0));
-        std::copy(key, key + 4, m->graph_key);
+        std::copy(key, key + 5, m->graph_key);
       }
       VGX_HIP(hipGraphLaunch(m->exec, s));
       m->replays += 1;
     } else {
-      ms_program(m, gr, fresh ? 1 : 0, first_dir, s);
+      ms_program(m, gr, prog, fresh ? 1 : 0, first_dir, s);
     }
     VGX_HIP(hipMemcpyAsync(m->h_ctl, m->d_ctl, sizeof(MsCtl), hipMemcpyDeviceToHost, s));
     VGX_HIP(hipStreamSynchronize(s));
@@ -588,8 +595,8 @@ static void ms_run(vgx_msbfs* m) {
       break;
     }
     // deeper than one program: continue from the last level written
-    first_dir = m->h_ctl->dir[kMsProgram];
-    base += kMsProgram;
+    first_dir = m->h_ctl->dir[prog];
+    base += prog;
   }
   m->ran = true;
   m->reached.assign(m->k, 0);
