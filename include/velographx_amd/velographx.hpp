@@ -149,9 +149,25 @@ class DynamicGraph {
     return s;
   }
 
+  // dynamic_graph.hpp:599-606: device bytes held by the graph.
+  [[nodiscard]] std::size_t storage_bytes() const { return storage().storage_bytes; }
+
+  // dynamic_graph.hpp:569-571.  Rows are updated in place here, so the
+  // graph is compact when no row holds a tombstone and no row has moved
+  // since the last bulk load, compaction or snapshot.
+  [[nodiscard]] bool is_compact() const {
+    const auto s = storage();
+    return s.tombstones == 0 && s.relocated_rows == 0;
+  }
+
   [[nodiscard]] vgx_graph* handle() const noexcept { return g_; }
 
+  // Takes ownership of a graph handle made by the C ABI (vgx_graph_snapshot).
+  [[nodiscard]] static DynamicGraph adopt(vgx_graph* g) noexcept { return DynamicGraph(g, Adopt{}); }
+
  private:
+  struct Adopt {};
+  DynamicGraph(vgx_graph* g, Adopt) noexcept : g_(g) {}
   struct Info {
     std::uint64_t n, version, arcs;
     bool directed;

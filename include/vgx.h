@@ -104,6 +104,24 @@ typedef struct {
 } vgx_graph_storage;
 vgx_status vgx_graph_storage_info(vgx_graph* g, vgx_graph_storage* out);
 
+/* consolidate_to_csr_snapshot -- storage/consolidation.hpp:132-163: a new
+ * graph (*out, owned by the caller) holding the current logical graph in
+ * compacted rows (no tombstones, no relocated rows), version 1, on the
+ * source's device; the source graph is not changed. */
+vgx_status vgx_graph_snapshot(vgx_graph* src, vgx_graph** out);
+
+/* The neighbour-latency probe of the steady-state storage benchmark --
+ * benchmarks/steady_state_storage.cpp:61-86 (probe_once / probe_median):
+ * `count` probes of out-rows u_i = (i * 11400714819323198485 mod 2^64) mod
+ * vertex_count, one wave per probe on the device, timed over `repeats`
+ * launches; *ns_per_probe is the median launch time per probe.  *digest is
+ * the sum mod 2^64 of mix(live length) + sum of mix(v) over each probed
+ * row (mix = splitmix64's output function), equal for every graph with the
+ * same logical rows whatever their storage; a digest that differs between
+ * repeats is VGX_DEVICE_ERROR.  An empty graph gives 0 and 0. */
+vgx_status vgx_graph_probe_neighbors(vgx_graph* g, uint64_t count, uint32_t repeats, double* ns_per_probe,
+                                     uint64_t* digest);
+
 /* Export the logical graph as CSR (ascending rows) into host buffers:
  * off[vertex_count+1], adj[edge_count_directed]. */
 vgx_status vgx_graph_export_csr(vgx_graph* g, int in_direction, uint64_t* off, uint32_t* adj);

@@ -20,6 +20,12 @@
 //   pr_recompute <iters> <tol>      pr.recompute(iters, tol)
 //   pr_validate <it> <tol> <l1> <linf>
 //   neighbors <u>
+//   cons_policy <arcs> <latency>    scale_aware_consolidation_policy
+//   cons_eval <cur> <canon> <cur_lat> <canon_lat> <max_s> <max_l>
+//                                   evaluate_consolidation
+//   cons_ctl <min_epochs> <samples> <growth>   new ConsolidationController
+//   cons_observe <sr> <lr> <se> <le> <sc> <epoch>   controller.observe
+//   cons_mark <epoch>               controller.mark_consolidated
 // Bench modes: bench-bfs, bench-pr, bench-repair (see their comments).
 #include <chrono>
 #include <cstdint>
@@ -34,6 +40,7 @@
 
 #include "velographx/incremental/bfs.hpp"
 #include "velographx/incremental/pagerank.hpp"
+#include "velographx/storage/consolidation.hpp"
 #include "velographx/storage/dynamic_graph.hpp"
 
 using velographx::DynamicGraph;
@@ -66,6 +73,7 @@ int run_scenario(std::istream& in) {
   std::unique_ptr<DynamicGraph> g;
   std::unique_ptr<IncrementalBFS> bfs;
   std::unique_ptr<IncrementalPageRank> pr;
+  std::unique_ptr<velographx::ConsolidationController> ctl;
   UpdateBatch batch;
   std::string cmd;
   while (in >> cmd) {
@@ -131,6 +139,38 @@ int run_scenario(std::istream& in) {
       const auto v = pr->validate_against_full(it, tol, l1, linf);
       std::printf("prvalid %.17g %.17g %zu %d %d\n", v.l1_error, v.linf_error, v.reference_iterations,
                   v.reference_converged ? 1 : 0, v.within_tolerance ? 1 : 0);
+    } else if (cmd == "cons_policy") {
+      std::size_t arcs;
+      double latency;
+      in >> arcs >> latency;
+      const auto p = velographx::scale_aware_consolidation_policy(arcs, latency);
+      std::printf("cpolicy %.17g %.17g\n", p.max_storage_growth_ratio, p.max_neighbor_latency_ratio);
+    } else if (cmd == "cons_eval") {
+      std::size_t cur, canon;
+      double cur_lat, canon_lat, max_s, max_l;
+      in >> cur >> canon >> cur_lat >> canon_lat >> max_s >> max_l;
+      const auto sg = velographx::evaluate_consolidation(cur, canon, cur_lat, canon_lat, {max_s, max_l});
+      std::printf("csignal %.17g %.17g %d %d %d\n", sg.storage_growth_ratio, sg.neighbor_latency_ratio,
+                  sg.storage_limit_exceeded ? 1 : 0, sg.latency_limit_exceeded ? 1 : 0,
+                  sg.should_consolidate ? 1 : 0);
+    } else if (cmd == "cons_ctl") {
+      velographx::ConsolidationControllerConfig c;
+      in >> c.min_epochs_between_consolidations >> c.latency_breach_samples >> c.min_storage_growth_for_latency_trigger;
+      ctl = std::make_unique<velographx::ConsolidationController>(c);
+    } else if (cmd == "cons_observe") {
+      velographx::ConsolidationSignal sg;
+      int se, le, sc;
+      std::size_t epoch;
+      in >> sg.storage_growth_ratio >> sg.neighbor_latency_ratio >> se >> le >> sc >> epoch;
+      sg.storage_limit_exceeded = se != 0;
+      sg.latency_limit_exceeded = le != 0;
+      sg.should_consolidate = sc != 0;
+      const bool fire = ctl->observe(sg, epoch);
+      std::printf("cobserve %d %zu %zu\n", fire ? 1 : 0, ctl->latency_breach_streak(), ctl->last_consolidation_epoch());
+    } else if (cmd == "cons_mark") {
+      std::size_t epoch;
+      in >> epoch;
+      ctl->mark_consolidated(epoch);
     } else if (cmd == "neighbors") {
       VertexId u;
       in >> u;

@@ -61,6 +61,12 @@ def parse(text):
             recs.append(("prvalid", {"l1_error": float(w[1]), "linf_error": float(w[2]),
                                      "reference_iterations": int(w[3]), "reference_converged": w[4] == "1",
                                      "within_tolerance": w[5] == "1"}))
+        elif tag == "cpolicy":
+            recs.append(("cpolicy", (float(w[1]), float(w[2]))))
+        elif tag == "csignal":
+            recs.append(("csignal", (float(w[1]), float(w[2]), w[3] == "1", w[4] == "1", w[5] == "1")))
+        elif tag == "cobserve":
+            recs.append(("cobserve", (w[1] == "1", int(w[2]), int(w[3]))))
         elif tag == "neighbors":
             recs.append(("neighbors", [int(x) for x in w[2:]]))
     return recs

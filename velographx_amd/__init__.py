@@ -3,6 +3,8 @@
 Drop-in for the reference's ``velographx`` Python module on the traversal
 path: ``Graph``, ``UpdateBatch``, ``IncrementalBFS``, ``IncrementalPageRank``,
 ``PageRankValidation`` and the ``from_*`` ingestion helpers, plus
+the consolidation policy, controller and device snapshot
+(``ConsolidationController``, ``consolidate_to_csr_snapshot``),
 ``MultiSourceBFS`` (up to 64 roots per pass) and, in ``velographx_amd.autograd``,
 differentiable weighted personalized PageRank, backed by
 hand-written HIP kernels (``libvgx.so``) through a C++ facade.
@@ -18,18 +20,26 @@ import torch  # noqa: F401  (see module docstring)
 from ._velographx import (  # noqa: F401
     AdaptivePolicy,
     AdaptiveSelector,
+    ConsolidationController,
+    ConsolidationControllerConfig,
+    ConsolidationPolicy,
+    ConsolidationSignal,
+    ConsolidationSnapshot,
     Graph,
     IncrementalBFS,
     IncrementalPageRank,
     MultiSourceBFS,
     PageRankValidation,
     UpdateBatch,
+    consolidate_to_csr_snapshot,
     device_count,
+    evaluate_consolidation,
     from_arrow_columns,
     from_arrow_table,
     from_numpy_edges,
     from_scipy_csr,
     rmat_edges,
+    scale_aware_consolidation_policy,
     version,
 )
 
@@ -37,4 +47,7 @@ __all__ = [
     "Graph", "UpdateBatch", "IncrementalBFS", "IncrementalPageRank", "PageRankValidation",
     "from_numpy_edges", "from_scipy_csr", "from_arrow_columns", "from_arrow_table",
     "rmat_edges", "device_count", "version", "MultiSourceBFS", "AdaptivePolicy", "AdaptiveSelector",
+    "ConsolidationPolicy", "ConsolidationSignal", "ConsolidationControllerConfig", "ConsolidationController",
+    "ConsolidationSnapshot", "evaluate_consolidation", "scale_aware_consolidation_policy",
+    "consolidate_to_csr_snapshot",
 ]

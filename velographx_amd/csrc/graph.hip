@@ -1112,6 +1112,74 @@ static void compact_direction(vgx_graph* g, int d) {
   a.used = total;
 }
 
+// Consolidated snapshot: the live rows of `src` packed, without tombstones
+// and with compaction's capacities (deg + deg/8 + 2), into the fresh graph
+// `dst` (same vertex count and directedness).  `src` is only read; all work
+// runs on src's stream and ends with one synchronisation per direction.
+static void snapshot_into(vgx_graph* src, vgx_graph* dst) {
+  const uint64_t n = src->n;
+  hipStream_t s = src->stream;
+  for (int d = 0; d < src->ndir() && n; ++d) {
+    Adjacency& sa = src->adj[d];
+    Adjacency& da = dst->adj[d];
+    k_compact_caps<<<grid_for(n), kBlock, 0, s>>>(sa.deg.p, n, da.cap.p);
+    VGX_LAUNCH_CHECK();
+    VGX_HIP(hipMemcpyAsync(da.deg.p, sa.deg.p, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    pack_direction(src, d, dst->pk.off, da.arena, da.row.p, da.cap.p);
+    k_set_bump<<<1, 64, 0, s>>>(dst->pk.off.p, n, dst->d_ctr, d);
+    VGX_LAUNCH_CHECK();
+    uint32_t total = 0;
+    VGX_HIP(hipMemcpyAsync(&total, dst->pk.off.p + n, sizeof(total), hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    if (total > da.arena.n) throw Error(VGX_DEVICE_ERROR, "snapshot exceeded its arena bound");
+    da.acap = da.arena.n;
+    da.used = total;
+    da.stored = sa.live;
+    da.live = sa.live;
+    da.relocated = 0;
+  }
+  dst->version = 1;  // one bulk load, as a freshly built graph has
+}
+
+// splitmix64's output function (Steele, Lea and Flood, "Fast splittable
+// pseudorandom number generators", OOPSLA 2014): spreads a vertex id over
+// 64 bits so that the sum of the mixes of a row identifies its contents.
+__device__ __forceinline__ unsigned long long probe_mix(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// Neighbour probe: probe i reads row u_i = (i * 0x9E3779B97F4A7C15 mod 2^64)
+// mod n, one wave per probe, and adds mix(live length) + sum of mix(v) over
+// its live entries into *digest (a sum mod 2^64, so independent of order
+// and of how the rows sit in the arena).
+__global__ void k_probe_rows(const uint2* __restrict__ row, const uint32_t* __restrict__ arena, uint64_t n,
+                             uint64_t count, unsigned long long* __restrict__ digest) {
+  const uint64_t W = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+  const unsigned lane = lane_id();
+  unsigned long long h = 0;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave; i < count;
+       i += W) {
+    const uint64_t u = (i * 0x9E3779B97F4A7C15ull) % n;
+    const uint2 rb = row[u];
+    unsigned long long part = 0;
+    uint32_t live = 0;
+    for (uint32_t j = lane; j < rb.y; j += kWave) {
+      const uint32_t x = arena[rb.x + j];
+      if (!(x & kTomb)) {
+        part += probe_mix(x);
+        live += 1;
+      }
+    }
+    part = wave_sum(part);
+    live = wave_sum(live);
+    h += part + probe_mix(live);
+  }
+  if (lane == 0 && h) atomicAdd(digest, h);
+}
+
 const CsrSnapshot& graph_snapshot(vgx_graph* g, int dir) {
   int d = g->directed ? dir : 0;
   CsrSnapshot& sn = g->snap[d];
@@ -1335,6 +1403,68 @@ vgx_status vgx_graph_csr_device(vgx_graph* g, int in_direction, const uint32_t**
   if (adj_dev) *adj_dev = sn.adj.p;
   if (vertices) *vertices = sn.n;
   if (arcs) *arcs = sn.arcs;
+  VGX_API_END
+}
+
+vgx_status vgx_graph_snapshot(vgx_graph* src, vgx_graph** out) {
+  VGX_API_BEGIN
+  if (!src || !out) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  DeviceGuard dg(src->device);
+  vgx_graph* dst = nullptr;
+  const vgx_status st = vgx_graph_create(src->n, src->directed ? 1 : 0, &dst);
+  if (st != VGX_OK) throw Error(st, std::string(vgx_last_error()));
+  try {
+    snapshot_into(src, dst);
+  } catch (...) {
+    vgx_graph_destroy(dst);
+    throw;
+  }
+  *out = dst;
+  VGX_API_END
+}
+
+vgx_status vgx_graph_probe_neighbors(vgx_graph* g, uint64_t count, uint32_t repeats, double* ns_per_probe,
+                                     uint64_t* digest) {
+  VGX_API_BEGIN
+  if (!g || !ns_per_probe || !digest) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  if (count == 0 || repeats == 0) throw Error(VGX_INVALID_ARGUMENT, "count and repeats must be positive");
+  *ns_per_probe = 0.0;
+  *digest = 0;
+  if (g->n == 0) return VGX_OK;
+  DeviceGuard dg(g->device);
+  hipStream_t s = g->stream;
+  DevBuf<unsigned long long> sums;
+  sums.ensure(repeats);
+  VGX_HIP(hipMemsetAsync(sums.p, 0, repeats * sizeof(unsigned long long), s));
+  struct Events {
+    std::vector<hipEvent_t> e;
+    ~Events() {
+      for (hipEvent_t x : e) (void)hipEventDestroy(x);
+    }
+  } ev;
+  ev.e.resize(2 * static_cast<size_t>(repeats), nullptr);
+  for (hipEvent_t& x : ev.e) VGX_HIP(hipEventCreate(&x));
+  const unsigned grid = static_cast<unsigned>(std::min<uint64_t>((count + 3) / 4, 1024));
+  const Adjacency& a = g->adj[0];
+  for (uint32_t r = 0; r < repeats; ++r) {
+    VGX_HIP(hipEventRecord(ev.e[2 * r], s));
+    k_probe_rows<<<grid, kBlock, 0, s>>>(a.row.p, a.arena.p, g->n, count, sums.p + r);
+    VGX_LAUNCH_CHECK();
+    VGX_HIP(hipEventRecord(ev.e[2 * r + 1], s));
+  }
+  std::vector<unsigned long long> h(repeats);
+  VGX_HIP(hipMemcpyAsync(h.data(), sums.p, repeats * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipStreamSynchronize(s));
+  std::vector<double> ns(repeats);
+  for (uint32_t r = 0; r < repeats; ++r) {
+    float ms = 0.f;
+    VGX_HIP(hipEventElapsedTime(&ms, ev.e[2 * r], ev.e[2 * r + 1]));
+    ns[r] = 1e6 * ms / static_cast<double>(count);
+    if (h[r] != h[0]) throw Error(VGX_DEVICE_ERROR, "probe digest changed between repeats");
+  }
+  std::sort(ns.begin(), ns.end());
+  *ns_per_probe = ns[ns.size() / 2];
+  *digest = h[0];
   VGX_API_END
 }
 
