@@ -25,6 +25,10 @@ Secondary legs, each in its own object of the same JSON line:
 * batched   -- 64 roots per multi-source pass on the same graph (aggregate
                GTEPS, batched / plain ratio, bit-exact check of the first 16
                roots against the plain path);
+* consolidation -- device snapshot of the same graph (timed, GB/s), the
+               on-device neighbour probe, and 24 epochs of 1,000,000-op
+               batches under the consolidation controller with snapshot
+               cutover; the reference engine's snapshot and probe beside it;
 * repairs   -- 100 update batches of 10,000 ops (80% inserts of random
                non-edges, 20% deletions of live edges) answered with exact
                distances, the adaptive selector choosing localized repair or
@@ -96,6 +100,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-repairs", action="store_true")
     p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--no-consolidation", action="store_true")
+    p.add_argument("--cons-epochs", type=int, default=24)
+    p.add_argument("--cons-batch", type=int, default=1000000)
     p.add_argument("--no-contrast", action="store_true")
     p.add_argument("--no-pagerank", action="store_true")
     p.add_argument("--no-ppr", action="store_true")
@@ -432,6 +439,107 @@ def leg_batching(vx, wl, args):
     return out
 
 
+def cpu_snapshot(s, d, n, repeats=2):
+    """The reference engine's consolidate_to_csr_snapshot and its 4096-row
+    neighbour probe on the same graph (one thread)."""
+    if not os.path.exists(REF_BIN):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        cmd = [REF_BIN, "bench-snapshot", _edges_file(td, s, d), str(n), "0", str(repeats)]
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=900, check=True).stdout
+    snaps = [float(line.split()[1]) for line in out.splitlines() if line.startswith("snapshot ")]
+    probe = [float(line.split()[1]) for line in out.splitlines() if line.startswith("probe ")]
+    if not snaps:
+        return None
+    return {"snapshot_ms": 1e3 * min(snaps), "probe_ns": probe[0] if probe else None, "cores": 1,
+            "kind": "reference",
+            "sample": f"consolidate_to_csr_snapshot of the reference engine on the bulk-loaded graph "
+                      f"(best of {len(snaps)}), then its 4096-row neighbour probe"}
+
+
+def leg_consolidation(vx, wl, g, s, d, n, args):
+    """Consolidation at the BFS graph's scale: the device snapshot of the
+    bulk-loaded graph (timed; algorithmic bytes = row table, degrees and
+    arena read, packed arena and row table, capacities and degrees
+    written), the on-device neighbour probe (4096 rows, the reference
+    benchmark's default), then a controller-driven steady state on a
+    snapshot copy: per epoch one update batch (80% inserts of random pairs,
+    20% deletions of loaded edges), the storage and
+    probe-latency signal under the reference benchmark's default policy
+    (1.25 / 1.25), the controller's decision and, on a trigger, a snapshot
+    cut over in place of the graph."""
+    import torch
+
+    st = g.storage()
+    arcs = int(g.edge_count_directed)
+    nv = int(g.vertex_count)
+    stored = int(st["arena_used"])
+    snap_bytes = 12.0 * nv + 4.0 * stored + 4.0 * arcs + 16.0 * nv
+    vx.consolidate_to_csr_snapshot(g)  # warm-up: allocator pools
+    t_snap = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        snap = vx.consolidate_to_csr_snapshot(g)
+        t_snap.append(time.perf_counter() - t0)
+        del snap
+    t_best = min(t_snap)
+    probe_ns, _ = g.probe_neighbors(4096, 5)
+    keep = vx.consolidate_to_csr_snapshot(g)
+    h = keep.graph
+    exact = h.probe_neighbors(nv, 1)[1] == g.probe_neighbors(nv, 1)[1]
+    rng = np.random.default_rng(31)
+    n_ins = int(0.8 * args.cons_batch)
+    n_del = args.cons_batch - n_ins
+    policy = vx.ConsolidationPolicy(1.25, 1.25)
+    ctl = vx.ConsolidationController()
+    canonical_bytes = h.storage_bytes()
+    canonical_ns, _ = h.probe_neighbors(4096, 5)
+    t_apply, t_cut = [], []
+    triggers = {"storage": 0, "latency": 0}
+    hw_storage, hw_latency = 1.0, 1.0
+    for epoch in range(1, args.cons_epochs + 1):
+        # inserts of uniform random pairs, deletions of bulk-loaded edges
+        pick = rng.integers(0, s.size, n_del)
+        bs = np.concatenate([rng.integers(0, nv, n_ins).astype(np.uint32), s[pick]])
+        bd = np.concatenate([rng.integers(0, nv, n_ins).astype(np.uint32), d[pick]])
+        ba = np.concatenate([np.ones(n_ins, np.uint8), np.zeros(n_del, np.uint8)])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h.apply_arrays(bs, bd, ba)
+        torch.cuda.synchronize()
+        t_apply.append(time.perf_counter() - t0)
+        ns, _ = h.probe_neighbors(4096, 3)
+        sig = vx.evaluate_consolidation(h.storage_bytes(), canonical_bytes, ns, canonical_ns, policy)
+        hw_storage = max(hw_storage, sig.storage_growth_ratio)
+        hw_latency = max(hw_latency, sig.neighbor_latency_ratio)
+        if ctl.observe(sig, epoch):
+            triggers["storage" if sig.storage_limit_exceeded else "latency"] += 1
+            t0 = time.perf_counter()
+            nxt = vx.consolidate_to_csr_snapshot(h)
+            t_cut.append(time.perf_counter() - t0)
+            exact &= nxt.graph.probe_neighbors(nv, 1)[1] == h.probe_neighbors(nv, 1)[1]
+            keep, h = nxt, nxt.graph
+            canonical_bytes = h.storage_bytes()
+            canonical_ns, _ = h.probe_neighbors(4096, 5)
+            ctl.mark_consolidated(epoch)
+    out = {"workload": f"undirected R-MAT s{args.scale} ef{args.edgefactor} seed 1: device snapshot of the "
+                       f"bulk-loaded graph; {args.cons_epochs} epochs of {args.cons_batch}-op batches (80% random "
+                       f"inserts, 20% deletions of loaded edges) "
+                       f"under the consolidation controller (policy 1.25/1.25)",
+           "arcs": arcs, "snapshot_ms": 1e3 * t_best, "snapshot_bytes": snap_bytes,
+           "snapshot_gbps": snap_bytes / t_best / 1e9, "snapshot_roofline_frac": snap_bytes / t_best / 1e9 / HBM_PEAK_GBPS,
+           "probe_ns_4096": probe_ns, "epochs": args.cons_epochs,
+           "mean_apply_ms": 1e3 * float(np.mean(t_apply)), "consolidations": len(t_cut),
+           "storage_triggers": triggers["storage"], "latency_triggers": triggers["latency"],
+           "mean_cutover_ms": 1e3 * float(np.mean(t_cut)) if t_cut else None,
+           "high_water_storage_ratio": hw_storage, "high_water_latency_ratio": hw_latency,
+           "digest_equal_after_every_cutover": bool(exact)}
+    del keep, h
+    log(f"consolidation: {out}")
+    return out
+
+
 def _run_repairs(vx, g, root, batches, args, policy):
     """One pass over the update stream with the given arm policy:
     "adaptive" (the selector picks repair or recompute per batch, the
@@ -671,7 +779,7 @@ def main():
         pmc_child(args)
         return
     if args.only_bfs:
-        args.no_cpu = args.no_repairs = args.no_batched = args.no_contrast = True
+        args.no_cpu = args.no_repairs = args.no_batched = args.no_contrast = args.no_consolidation = True
         args.no_pagerank = args.no_ppr = args.no_batching = True
     import torch
 
@@ -754,6 +862,8 @@ def main():
     if not args.no_batched:
         result["batched"] = leg_batched(vx, wl, g, lcc, m_c, args.batched_roots, objs, gteps)
     del objs
+    if not args.no_consolidation:
+        result["consolidation"] = leg_consolidation(vx, wl, g, s, d, n, args)
     batches = None
     if not args.no_repairs and args.batches > 0:
         result["repairs"], batches = leg_repairs(vx, wl, g, s, d, n, roots[0], args)
@@ -762,6 +872,11 @@ def main():
         t0 = time.time()
         result["cpu_baseline"] = cpu_baseline(s, d, n, roots, m_c, args.cpu_budget)
         log(f"cpu baseline in {time.time() - t0:.1f}s: {result['cpu_baseline']}")
+        if "consolidation" in result:
+            cs = cpu_snapshot(s, d, n)
+            if cs is not None:
+                result["consolidation"]["cpu_reference"] = cs
+                log(f"cpu snapshot: {cs}")
         if batches is not None:
             cr = cpu_repairs(s, d, n, roots[0], batches, args.cpu_budget)
             if cr is not None:

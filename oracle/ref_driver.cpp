@@ -26,7 +26,8 @@
 //   cons_ctl <min_epochs> <samples> <growth>   new ConsolidationController
 //   cons_observe <sr> <lr> <se> <le> <sc> <epoch>   controller.observe
 //   cons_mark <epoch>               controller.mark_consolidated
-// Bench modes: bench-bfs, bench-pr, bench-repair (see their comments).
+// Bench modes: bench-bfs, bench-pr, bench-repair, bench-snapshot (see their
+// comments).
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -230,6 +231,41 @@ int bench_bfs(int argc, char** argv) {
   return 0;
 }
 
+// bench-snapshot <edges.bin> <n> <directed> <repeats>
+// Builds the graph, then times consolidate_to_csr_snapshot (one thread)
+// <repeats> times, and the neighbour probe of the steady-state storage
+// benchmark (4096 rows, steady_state_storage.cpp:61-72's row sequence).
+// Prints "snapshot <seconds> <arcs> <source_bytes> <consolidated_bytes>"
+// per repeat and "probe <ns_per_probe> <checksum>".
+int bench_snapshot(char** argv) {
+  const auto edges = read_edges_bin(argv[2]);
+  const std::size_t n = std::stoull(argv[3]);
+  const bool directed = std::stoi(argv[4]) != 0;
+  const int repeats = std::stoi(argv[5]);
+  DynamicGraph g(n, directed);
+  g.bulk_load_edges(edges);
+  for (int r = 0; r < repeats; ++r) {
+    const auto t0 = Clock::now();
+    auto snap = velographx::consolidate_to_csr_snapshot(g);
+    const auto t1 = Clock::now();
+    std::printf("snapshot %.9f %zu %zu %zu\n", std::chrono::duration<double>(t1 - t0).count(), snap.directed_edges,
+                snap.source_storage_bytes, snap.consolidated_storage_bytes);
+    std::fflush(stdout);
+  }
+  constexpr std::size_t kProbes = 4096;
+  std::uint64_t sum = 0;
+  const auto t0 = Clock::now();
+  for (std::size_t i = 0; i < kProbes; ++i) {
+    const auto u = static_cast<VertexId>((i * 11400714819323198485ULL) % g.vertex_count());
+    const auto row = g.neighbors(u);
+    sum += row.size();
+    for (const auto v : row) sum += v;
+  }
+  const double ns = std::chrono::duration<double, std::nano>(Clock::now() - t0).count() / kProbes;
+  std::printf("probe %.3f %llu\n", ns, static_cast<unsigned long long>(sum));
+  return 0;
+}
+
 // bench-pr <edges.bin> <n> <directed> <iterations>
 // Times IncrementalPageRank(graph, 0.85) followed by recompute(iterations,
 // 0.0): exactly <iterations> pull sweeps.  Prints
@@ -290,6 +326,7 @@ int bench_repair(char** argv) {
 int main(int argc, char** argv) {
   if (argc >= 2 && std::strcmp(argv[1], "bench-bfs") == 0 && argc >= 7) return bench_bfs(argc, argv);
   if (argc == 6 && std::strcmp(argv[1], "bench-pr") == 0) return bench_pr(argv);
+  if (argc == 6 && std::strcmp(argv[1], "bench-snapshot") == 0) return bench_snapshot(argv);
   if (argc == 8 && std::strcmp(argv[1], "bench-repair") == 0) return bench_repair(argv);
   if (argc >= 3 && std::strcmp(argv[1], "scenario") == 0) {
     std::ifstream f(argv[2]);
