@@ -215,6 +215,14 @@ vgx_status vgx_pagerank_recompute(vgx_pagerank* p, uint64_t max_iterations, doub
 /* Full solve with an L1 stop (the static push pagerank's criterion,
  * src/algorithms.cpp:56-81), optionally warm-started from the current ranks. */
 vgx_status vgx_pagerank_solve_l1(vgx_pagerank* p, uint64_t max_iterations, double l1_tol, int warm_start);
+/* pagerank(const CsrGraph&, damping, max_iterations, tolerance) --
+ * src/algorithms.cpp:56-81, include/velographx/algorithms.hpp:10: FP64 ranks
+ * from 1/n, at most max_iterations sweeps, stop once the L1 change is below
+ * tolerance; no solver object is kept.  Copies min(cap, n) ranks to out,
+ * sets *count = n and, if not null, *iterations to the sweeps run.  Damping
+ * outside [0, 1] is VGX_INVALID_ARGUMENT. */
+vgx_status vgx_pagerank_static(vgx_graph* g, double damping, uint64_t max_iterations, double tolerance, double* out,
+                               uint64_t cap, uint64_t* count, uint64_t* iterations);
 /* apply(batch, local_iterations, tol, full_fallback_fraction) -- pagerank.hpp:47-191. */
 vgx_status vgx_pagerank_apply(vgx_pagerank* p, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
                               uint64_t k, uint64_t local_iterations, double tol, double full_fallback_fraction);

@@ -3,7 +3,8 @@
 Drop-in for the reference's ``velographx`` Python module on the traversal
 path: ``Graph``, ``UpdateBatch``, ``IncrementalBFS``, ``IncrementalPageRank``,
 ``PageRankValidation`` and the ``from_*`` ingestion helpers, plus
-the consolidation policy, controller and device snapshot
+the static ``CsrGraph`` with ``bfs_distances`` and ``pagerank``, the
+consolidation policy, controller and device snapshot
 (``ConsolidationController``, ``consolidate_to_csr_snapshot``),
 ``MultiSourceBFS`` (up to 64 roots per pass) and, in ``velographx_amd.autograd``,
 differentiable weighted personalized PageRank, backed by
@@ -25,12 +26,14 @@ from ._velographx import (  # noqa: F401
     ConsolidationPolicy,
     ConsolidationSignal,
     ConsolidationSnapshot,
+    CsrGraph,
     Graph,
     IncrementalBFS,
     IncrementalPageRank,
     MultiSourceBFS,
     PageRankValidation,
     UpdateBatch,
+    bfs_distances,
     consolidate_to_csr_snapshot,
     device_count,
     evaluate_consolidation,
@@ -38,6 +41,7 @@ from ._velographx import (  # noqa: F401
     from_arrow_table,
     from_numpy_edges,
     from_scipy_csr,
+    pagerank,
     rmat_edges,
     scale_aware_consolidation_policy,
     version,
@@ -49,5 +53,5 @@ __all__ = [
     "rmat_edges", "device_count", "version", "MultiSourceBFS", "AdaptivePolicy", "AdaptiveSelector",
     "ConsolidationPolicy", "ConsolidationSignal", "ConsolidationControllerConfig", "ConsolidationController",
     "ConsolidationSnapshot", "evaluate_consolidation", "scale_aware_consolidation_policy",
-    "consolidate_to_csr_snapshot",
+    "consolidate_to_csr_snapshot", "CsrGraph", "bfs_distances", "pagerank",
 ]

@@ -995,6 +995,41 @@ vgx_status vgx_pagerank_solve_l1(vgx_pagerank* p, uint64_t max_iterations, doubl
   VGX_API_END
 }
 
+vgx_status vgx_pagerank_static(vgx_graph* g, double damping, uint64_t max_iterations, double tolerance, double* out,
+                               uint64_t cap, uint64_t* count, uint64_t* iterations) {
+  VGX_API_BEGIN
+  if (!g || !count) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  if (!(damping >= 0.0 && damping <= 1.0)) throw Error(VGX_INVALID_ARGUMENT, "PageRank damping must be in [0, 1]");
+  *count = 0;
+  if (iterations) *iterations = 0;
+  if (g->n == 0) return VGX_OK;
+  VGX_HIP(hipSetDevice(g->device));
+  // A solver object without the constructor's solve: uniform start, FP64,
+  // L1 stop, then the ranks copied out and the object released.
+  struct Owned {
+    vgx_pagerank* p = new vgx_pagerank();
+    ~Owned() { vgx_pagerank_destroy(p); }
+  } o;
+  vgx_pagerank* p = o.p;
+  p->g = g;
+  p->damping = damping;
+  p->precision = 64;
+  p->d_st = static_cast<PrState*>(dev_alloc(sizeof(PrState)));
+  VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_st), sizeof(PrState), hipHostMallocDefault));
+  std::memset(p->h_st, 0, sizeof(PrState));
+  p->counters.ensure(16);
+  recompute(p, max_iterations, tolerance, true, false);
+  const uint64_t n = g->n;
+  *count = n;
+  if (iterations) *iterations = max_iterations == 0 ? 0 : p->h_st->iterations;
+  const uint64_t c = std::min(cap, n);
+  if (out && c) {
+    VGX_HIP(hipMemcpyAsync(out, p->rank[p->cur].p, c * 8, hipMemcpyDeviceToHost, g->stream));
+    VGX_HIP(hipStreamSynchronize(g->stream));
+  }
+  VGX_API_END
+}
+
 vgx_status vgx_pagerank_apply(vgx_pagerank* p, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
                               uint64_t k, uint64_t local_iterations, double tol, double full_fallback_fraction) {
   VGX_API_BEGIN

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "velographx_amd/consolidation.hpp"
+#include "velographx_amd/csr_graph.hpp"
 #include "velographx_amd/pagerank.hpp"
 #include "velographx_amd/selector.hpp"
 #include "velographx_amd/velographx.hpp"
@@ -270,6 +271,53 @@ PYBIND11_MODULE(_velographx, m) {
           },
           py::arg("count") = 4096, py::arg("repeats") = 5)
       .def_property_readonly("_handle", [](const DynamicGraph& g) { return reinterpret_cast<std::uintptr_t>(g.handle()); });
+
+  // csr_graph.hpp, algorithms.hpp:8-10 (static graph, one-shot algorithms)
+  py::class_<velographx::CsrGraph>(m, "CsrGraph")
+      .def(py::init([](const U32Array& edges, bool directed) {
+             const auto info = edges.request();
+             if (info.size == 0) return std::make_unique<velographx::CsrGraph>(std::vector<velographx::CsrGraph::Edge>{}, directed);
+             if (info.ndim != 2 || info.shape[1] != 2) throw std::invalid_argument("edge array must have shape (N, 2)");
+             const auto m = static_cast<std::size_t>(info.shape[0]);
+             const auto* data = static_cast<const std::uint32_t*>(info.ptr);
+             std::vector<std::uint32_t> s(m), d(m);
+             for (std::size_t i = 0; i < m; ++i) {
+               s[i] = data[2 * i];
+               d[i] = data[2 * i + 1];
+             }
+             py::gil_scoped_release nogil;
+             return std::make_unique<velographx::CsrGraph>(s.data(), d.data(), m, directed);
+           }),
+           py::arg("edges"), py::arg("directed") = false)
+      .def("vertex_count", &velographx::CsrGraph::vertex_count)
+      .def("edge_entry_count", &velographx::CsrGraph::edge_entry_count)
+      .def("directed", &velographx::CsrGraph::directed)
+      .def("neighbors", &velographx::CsrGraph::neighbors, py::arg("v"))
+      .def("in_neighbors", &velographx::CsrGraph::in_neighbors, py::arg("v"))
+      .def("degree", &velographx::CsrGraph::degree, py::arg("v"))
+      .def("has_edge", &velographx::CsrGraph::has_edge, py::arg("u"), py::arg("v"));
+  m.def(
+      "bfs_distances",
+      [](const velographx::CsrGraph& g, VertexId source) {
+        std::vector<std::uint32_t> d;
+        {
+          py::gil_scoped_release nogil;
+          d = velographx::bfs_distances(g, source);
+        }
+        return to_numpy(std::move(d));
+      },
+      py::arg("graph"), py::arg("source"));
+  m.def(
+      "pagerank",
+      [](const velographx::CsrGraph& g, double damping, std::size_t max_iterations, double tolerance) {
+        std::vector<double> r;
+        {
+          py::gil_scoped_release nogil;
+          r = velographx::pagerank(g, damping, max_iterations, tolerance);
+        }
+        return to_numpy(std::move(r));
+      },
+      py::arg("graph"), py::arg("damping") = 0.85, py::arg("max_iterations") = 100, py::arg("tolerance") = 1e-10);
 
   // storage/consolidation.hpp:10-163
   py::class_<velographx::ConsolidationPolicy>(m, "ConsolidationPolicy")
