@@ -29,6 +29,8 @@ Secondary legs, each in its own object of the same JSON line:
                on-device neighbour probe, and 24 epochs of 1,000,000-op
                batches under the consolidation controller with snapshot
                cutover; the reference engine's snapshot and probe beside it;
+* static    -- the static CsrGraph of the same edges: build, bfs_distances,
+               20 pagerank sweeps; the reference's beside it;
 * repairs   -- 100 update batches of 10,000 ops (80% inserts of random
                non-edges, 20% deletions of live edges) answered with exact
                distances, the adaptive selector choosing localized repair or
@@ -101,6 +103,7 @@ def parse():
     p.add_argument("--no-repairs", action="store_true")
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-consolidation", action="store_true")
+    p.add_argument("--no-static", action="store_true")
     p.add_argument("--cons-epochs", type=int, default=24)
     p.add_argument("--cons-batch", type=int, default=1000000)
     p.add_argument("--no-contrast", action="store_true")
@@ -540,6 +543,53 @@ def leg_consolidation(vx, wl, g, s, d, n, args):
     return out
 
 
+def leg_static(vx, s, d, root):
+    """The static CsrGraph of the same edges: construction from the edge
+    array (host copy, upload, device build), bfs_distances from one root
+    (answer returned to the host, as the reference's vector), and exactly 20
+    pagerank sweeps (tolerance 0)."""
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = vx.CsrGraph(np.stack([s, d], 1), False)
+    t_build = time.perf_counter() - t0
+    vx.bfs_distances(g, int(root))
+    t_bfs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        dist = vx.bfs_distances(g, int(root))
+        t_bfs.append(time.perf_counter() - t0)
+    reached = int((dist != 0xFFFFFFFF).sum())
+    vx.pagerank(g, 0.85, 2, 0.0)
+    t0 = time.perf_counter()
+    pr = vx.pagerank(g, 0.85, 20, 0.0)
+    t_pr = time.perf_counter() - t0
+    out = {"workload": "static CsrGraph of the same R-MAT edges: build, bfs_distances from the first root, "
+                       "pagerank(0.85, 20 sweeps, tolerance 0)",
+           "vertices": int(g.vertex_count()), "entries": int(g.edge_entry_count()),
+           "build_ms": 1e3 * t_build, "bfs_distances_ms": 1e3 * min(t_bfs), "reached": reached,
+           "pagerank_ms_per_sweep": 1e3 * t_pr / 20, "rank_sum": float(pr.sum())}
+    del g
+    log(f"static: {out}")
+    return out
+
+
+def cpu_static(s, d, root):
+    """The reference's CsrGraph construction, bfs_distances and 3 pagerank
+    sweeps on the same edges (one thread)."""
+    if not os.path.exists(REF_BIN):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        cmd = [REF_BIN, "bench-static", _edges_file(td, s, d), "0", str(int(root)), "3"]
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=900, check=True).stdout
+    w = [line.split() for line in out.splitlines() if line.startswith("static ")][0]
+    return {"build_ms": 1e3 * float(w[1]), "bfs_distances_ms": 1e3 * float(w[2]),
+            "pagerank_ms_per_sweep": 1e3 * float(w[3]) / 3, "cores": 1, "kind": "reference",
+            "sample": "CsrGraph(edges) construction, bfs_distances from the same root and pagerank(0.85, 3 sweeps, "
+                      "tolerance 0) of the reference engine"}
+
+
 def _run_repairs(vx, g, root, batches, args, policy):
     """One pass over the update stream with the given arm policy:
     "adaptive" (the selector picks repair or recompute per batch, the
@@ -779,7 +829,7 @@ def main():
         pmc_child(args)
         return
     if args.only_bfs:
-        args.no_cpu = args.no_repairs = args.no_batched = args.no_contrast = args.no_consolidation = True
+        args.no_cpu = args.no_repairs = args.no_batched = args.no_contrast = args.no_consolidation = args.no_static = True
         args.no_pagerank = args.no_ppr = args.no_batching = True
     import torch
 
@@ -864,6 +914,8 @@ def main():
     del objs
     if not args.no_consolidation:
         result["consolidation"] = leg_consolidation(vx, wl, g, s, d, n, args)
+    if not args.no_static:
+        result["static"] = leg_static(vx, s, d, roots[0])
     batches = None
     if not args.no_repairs and args.batches > 0:
         result["repairs"], batches = leg_repairs(vx, wl, g, s, d, n, roots[0], args)
@@ -877,6 +929,11 @@ def main():
             if cs is not None:
                 result["consolidation"]["cpu_reference"] = cs
                 log(f"cpu snapshot: {cs}")
+        if "static" in result:
+            cst = cpu_static(s, d, roots[0])
+            if cst is not None:
+                result["static"]["cpu_reference"] = cst
+                log(f"cpu static: {cst}")
         if batches is not None:
             cr = cpu_repairs(s, d, n, roots[0], batches, args.cpu_budget)
             if cr is not None:

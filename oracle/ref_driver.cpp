@@ -26,8 +26,8 @@
 //   cons_ctl <min_epochs> <samples> <growth>   new ConsolidationController
 //   cons_observe <sr> <lr> <se> <le> <sc> <epoch>   controller.observe
 //   cons_mark <epoch>               controller.mark_consolidated
-// Bench modes: bench-bfs, bench-pr, bench-repair, bench-snapshot (see their
-// comments).
+// Bench modes: bench-bfs, bench-pr, bench-repair, bench-snapshot,
+// bench-static (see their comments).
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -39,6 +39,8 @@
 #include <string>
 #include <vector>
 
+#include "velographx/algorithms.hpp"
+#include "velographx/csr_graph.hpp"
 #include "velographx/incremental/bfs.hpp"
 #include "velographx/incremental/pagerank.hpp"
 #include "velographx/storage/consolidation.hpp"
@@ -266,6 +268,28 @@ int bench_snapshot(char** argv) {
   return 0;
 }
 
+// bench-static <edges.bin> <directed> <root> <pr_iterations>
+// Times the CsrGraph construction, bfs_distances from <root> and
+// pagerank(graph, 0.85, <pr_iterations>, 0.0) (one thread).  Prints
+// "static <build_s> <bfs_s> <pagerank_s> <vertices> <entries>".
+int bench_static(char** argv) {
+  const auto edges = read_edges_bin(argv[2]);
+  const bool directed = std::stoi(argv[3]) != 0;
+  const VertexId root = static_cast<VertexId>(std::stoul(argv[4]));
+  const std::size_t iters = std::stoull(argv[5]);
+  const auto t0 = Clock::now();
+  velographx::CsrGraph g(edges, directed);
+  const auto t1 = Clock::now();
+  const auto d = velographx::bfs_distances(g, root);
+  const auto t2 = Clock::now();
+  const auto pr = velographx::pagerank(g, 0.85, iters, 0.0);
+  const auto t3 = Clock::now();
+  std::printf("static %.9f %.9f %.9f %zu %zu\n", std::chrono::duration<double>(t1 - t0).count(),
+              std::chrono::duration<double>(t2 - t1).count(), std::chrono::duration<double>(t3 - t2).count(),
+              g.vertex_count(), g.edge_entry_count());
+  return d.empty() || pr.empty() ? 1 : 0;
+}
+
 // bench-pr <edges.bin> <n> <directed> <iterations>
 // Times IncrementalPageRank(graph, 0.85) followed by recompute(iterations,
 // 0.0): exactly <iterations> pull sweeps.  Prints
@@ -327,6 +351,7 @@ int main(int argc, char** argv) {
   if (argc >= 2 && std::strcmp(argv[1], "bench-bfs") == 0 && argc >= 7) return bench_bfs(argc, argv);
   if (argc == 6 && std::strcmp(argv[1], "bench-pr") == 0) return bench_pr(argv);
   if (argc == 6 && std::strcmp(argv[1], "bench-snapshot") == 0) return bench_snapshot(argv);
+  if (argc == 6 && std::strcmp(argv[1], "bench-static") == 0) return bench_static(argv);
   if (argc == 8 && std::strcmp(argv[1], "bench-repair") == 0) return bench_repair(argv);
   if (argc >= 3 && std::strcmp(argv[1], "scenario") == 0) {
     std::ifstream f(argv[2]);

@@ -1166,11 +1166,22 @@ __global__ void k_probe_rows(const uint2* __restrict__ row, const uint32_t* __re
     const uint2 rb = row[u];
     unsigned long long part = 0;
     uint32_t live = 0;
-    for (uint32_t j = lane; j < rb.y; j += kWave) {
-      const uint32_t x = arena[rb.x + j];
-      if (!(x & kTomb)) {
-        part += probe_mix(x);
-        live += 1;
+    // Eight loads in flight per lane before any is used: a hub row's scan
+    // is bound by load latency, not bandwidth.
+    constexpr uint32_t kUnroll = 8;
+    for (uint32_t j0 = 0; j0 < rb.y; j0 += kUnroll * kWave) {
+      uint32_t x[kUnroll];
+#pragma unroll
+      for (uint32_t q = 0; q < kUnroll; ++q) {
+        const uint32_t j = j0 + q * kWave + lane;
+        x[q] = j < rb.y ? arena[rb.x + j] : kTomb;  // past the row: skipped like a tombstone
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < kUnroll; ++q) {
+        if (!(x[q] & kTomb)) {
+          part += probe_mix(x[q]);
+          live += 1;
+        }
       }
     }
     part = wave_sum(part);
