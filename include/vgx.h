@@ -71,6 +71,14 @@ vgx_status vgx_graph_bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t
  * version += 1 when k > 0. */
 vgx_status vgx_graph_apply(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
                            uint64_t k);
+/* vgx_graph_apply for a batch in device memory (see vgx_bfs_apply_device
+ * for the pointer and stream rules). */
+vgx_status vgx_graph_apply_device(vgx_graph* g, const uint32_t* src_dev, const uint32_t* dst_dev,
+                                  const uint8_t* add_dev, uint64_t k, void* stream);
+
+/* HIP device ordinal the graph lives on (no reference counterpart: the
+ * device of the arrays vgx_*_device and the DLPack views return). */
+vgx_status vgx_graph_device(const vgx_graph* g, int* device);
 
 /* vertex_count / version / directed / edge_count_directed / delta_edge_count
  * -- dynamic_graph.hpp:486-491, 589-597. */
@@ -141,12 +149,22 @@ vgx_status vgx_bfs_recompute(vgx_bfs* b);
  * re-relaxation, or a full recompute past the fallback fraction). */
 vgx_status vgx_bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
                          uint64_t k);
+/* The same batch already in device memory (zero-copy ingestion,
+ * python/bindings.cpp:21-36's from_numpy path without the host copy):
+ * src_dev, dst_dev, add_dev are device pointers of this graph's device,
+ * written on `stream` (a hipStream_t, synchronised first; null = the
+ * graph's own stream).  Results equal vgx_bfs_apply's on the same ops. */
+vgx_status vgx_bfs_apply_device(vgx_bfs* b, const uint32_t* src_dev, const uint32_t* dst_dev, const uint8_t* add_dev,
+                                uint64_t k, void* stream);
 
 /* distances() -- bfs.hpp:23: copies *count (= vertex entries) uint32 values
  * into out (cap entries max). */
 vgx_status vgx_bfs_distances(vgx_bfs* b, uint32_t* out, uint64_t cap, uint64_t* count);
 /* Device pointer to the distance array (valid until the next call). */
 vgx_status vgx_bfs_distances_device(vgx_bfs* b, const uint32_t** dist_dev, uint64_t* count);
+/* The graph a BFS object borrows (no reference counterpart; the device of
+ * its arrays is vgx_graph_device of it). */
+vgx_status vgx_bfs_graph(const vgx_bfs* b, vgx_graph** g);
 
 typedef struct {
   uint64_t reachable_count;        /* bfs.hpp:24 */
@@ -239,6 +257,8 @@ vgx_status vgx_pagerank_validate(vgx_pagerank* p, uint64_t reference_max_iterati
 vgx_status vgx_pagerank_install_reference(vgx_pagerank* p, uint64_t reference_max_iterations, double reference_tol);
 /* Device pointer to the current ranks (float32 or float64 per precision). */
 vgx_status vgx_pagerank_values_device(vgx_pagerank* p, const void** dev, uint64_t* count, int* precision);
+/* The graph a PageRank object borrows (no reference counterpart). */
+vgx_status vgx_pagerank_graph(const vgx_pagerank* p, vgx_graph** g);
 
 typedef struct {
   uint64_t last_repaired_vertices;  /* pagerank.hpp:41-45 */

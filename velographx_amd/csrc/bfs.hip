@@ -1434,7 +1434,8 @@ static void inc_sync(vgx_bfs* b) {
 constexpr uint32_t kInvalidationRounds = 2;
 constexpr uint32_t kPropagationRounds = 4;
 
-static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k) {
+static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
+                      bool device = false, hipStream_t producer = nullptr) {
   b->last_candidates = 0;
   b->last_affected = 0;
   b->last_relaxed = 0;
@@ -1446,7 +1447,8 @@ static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, cons
   uint64_t limit = static_cast<uint64_t>(static_cast<double>(n_pre) * b->fallback);
   if (limit < 1) limit = 1;
 
-  graph_stage(g, src, dst, add, k);  // may grow vertices
+  if (device) graph_stage_device(g, src, dst, add, k, producer);  // may grow vertices
+  else graph_stage(g, src, dst, add, k);
   extend_dist(b);
   BfsWork& w = b->w;
   VGX_HIP(hipMemsetAsync(b->d_inc, 0, sizeof(IncState), s));
@@ -1711,6 +1713,15 @@ vgx_status vgx_bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, c
   VGX_API_END
 }
 
+vgx_status vgx_bfs_apply_device(vgx_bfs* b, const uint32_t* src_dev, const uint32_t* dst_dev, const uint8_t* add_dev,
+                                uint64_t k, void* stream) {
+  VGX_API_BEGIN
+  if (!b || (k && (!src_dev || !dst_dev || !add_dev))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  VGX_HIP(hipSetDevice(b->g->device));
+  bfs_apply(b, src_dev, dst_dev, add_dev, k, true, static_cast<hipStream_t>(stream));
+  VGX_API_END
+}
+
 vgx_status vgx_bfs_distances(vgx_bfs* b, uint32_t* out, uint64_t cap, uint64_t* count) {
   VGX_API_BEGIN
   if (!b || !count) throw Error(VGX_INVALID_ARGUMENT, "null argument");
@@ -1721,6 +1732,13 @@ vgx_status vgx_bfs_distances(vgx_bfs* b, uint32_t* out, uint64_t cap, uint64_t* 
     VGX_HIP(hipMemcpyAsync(out, b->w.dist.p, c * sizeof(uint32_t), hipMemcpyDeviceToHost, b->g->stream));
     VGX_HIP(hipStreamSynchronize(b->g->stream));
   }
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_graph(const vgx_bfs* b, vgx_graph** g) {
+  VGX_API_BEGIN
+  if (!b || !g) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  *g = b->g;
   VGX_API_END
 }
 

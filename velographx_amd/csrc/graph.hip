@@ -846,6 +846,8 @@ static void bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t* dst, ui
 
 // ------------------------------------------------------------ batch apply
 
+static void stage_sorted(vgx_graph* g, uint64_t k);
+
 void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k) {
   if (k >= (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "update batch too large");
   uint64_t maxid = 0;
@@ -870,6 +872,50 @@ void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const u
   b0.add.ensure(k + 64);
   VGX_HIP(hipMemcpyAsync(b0.src.p, hs, 8 * k, hipMemcpyHostToDevice, s));
   VGX_HIP(hipMemcpyAsync(b0.add.p, hs + 2 * k, k, hipMemcpyHostToDevice, s));
+  stage_sorted(g, k);
+}
+
+// Largest endpoint + 1 over the batch's non-loop ops (0 when all are loops).
+__global__ void k_batch_extent(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint64_t k,
+                               unsigned long long* __restrict__ out) {
+  unsigned long long m = 0;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < k;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t u = src[i], v = dst[i];
+    if (u != v) m = max(m, static_cast<unsigned long long>(max(u, v)) + 1ull);
+  }
+  m = wave_max(m);
+  if (lane_id() == 0 && m) atomicMax(out, m);
+}
+
+void graph_stage_device(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
+                        hipStream_t producer) {
+  if (k >= (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "update batch too large");
+  hipStream_t s = g->stream;
+  if (producer && producer != s) VGX_HIP(hipStreamSynchronize(producer));
+  BatchScratch& b0 = g->bs[0];
+  b0.src.ensure(2 * k + 64);
+  b0.add.ensure(k + 64);
+  b0.extent.ensure(1);
+  VGX_HIP(hipMemcpyAsync(b0.src.p, src, 4 * k, hipMemcpyDeviceToDevice, s));
+  VGX_HIP(hipMemcpyAsync(b0.src.p + k, dst, 4 * k, hipMemcpyDeviceToDevice, s));
+  VGX_HIP(hipMemcpyAsync(b0.add.p, add, k, hipMemcpyDeviceToDevice, s));
+  VGX_HIP(hipMemsetAsync(b0.extent.p, 0, sizeof(unsigned long long), s));
+  k_batch_extent<<<grid_for(k, kBlock, 1024), kBlock, 0, s>>>(b0.src.p, b0.src.p + k, k, b0.extent.p);
+  VGX_LAUNCH_CHECK();
+  unsigned long long ext = 0;
+  VGX_HIP(hipMemcpyAsync(&ext, b0.extent.p, sizeof(ext), hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipStreamSynchronize(s));
+  if (ext > 0 && ext - 1 >= (1ull << 31) - 1) throw Error(VGX_INVALID_ARGUMENT, "vertex ids must be below 2^31 - 1");
+  if (ext > 0) graph_grow_vertices(g, ext);
+  stage_sorted(g, k);
+}
+
+// Sort, resolve and plan a batch whose src | dst and add already sit in
+// bs[0] (from either staging path).
+static void stage_sorted(vgx_graph* g, uint64_t k) {
+  hipStream_t s = g->stream;
+  BatchScratch& b0 = g->bs[0];
   const uint32_t* d_src = b0.src.p;
   const uint32_t* d_dst = b0.src.p + k;
   reset_deltas(g);
@@ -1009,9 +1055,11 @@ void graph_finish_batch(vgx_graph* g) {
     for (int d = 0; d < g->ndir(); ++d) compact_direction(g, d);
 }
 
-static void graph_apply(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k) {
+static void graph_apply(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
+                        bool device = false, hipStream_t producer = nullptr) {
   if (k == 0) return;
-  graph_stage(g, src, dst, add, k);
+  if (device) graph_stage_device(g, src, dst, add, k, producer);
+  else graph_stage(g, src, dst, add, k);
   graph_apply_deletions(g);
   graph_apply_additions(g);
   graph_finish_batch(g);
@@ -1294,6 +1342,22 @@ vgx_status vgx_graph_apply(vgx_graph* g, const uint32_t* src, const uint32_t* ds
   if (!g || (k && (!src || !dst || !add))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
   DeviceGuard dg(g->device);
   graph_apply(g, src, dst, add, k);
+  VGX_API_END
+}
+
+vgx_status vgx_graph_apply_device(vgx_graph* g, const uint32_t* src_dev, const uint32_t* dst_dev,
+                                  const uint8_t* add_dev, uint64_t k, void* stream) {
+  VGX_API_BEGIN
+  if (!g || (k && (!src_dev || !dst_dev || !add_dev))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  DeviceGuard dg(g->device);
+  graph_apply(g, src_dev, dst_dev, add_dev, k, true, static_cast<hipStream_t>(stream));
+  VGX_API_END
+}
+
+vgx_status vgx_graph_device(const vgx_graph* g, int* device) {
+  VGX_API_BEGIN
+  if (!g || !device) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  *device = g->device;
   VGX_API_END
 }
 

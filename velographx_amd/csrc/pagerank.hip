@@ -1119,6 +1119,13 @@ vgx_status vgx_pagerank_install_reference(vgx_pagerank* p, uint64_t reference_ma
   VGX_API_END
 }
 
+vgx_status vgx_pagerank_graph(const vgx_pagerank* p, vgx_graph** g) {
+  VGX_API_BEGIN
+  if (!p || !g) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  *g = p->g;
+  VGX_API_END
+}
+
 // Device pointer to the current ranks (float32 or float64 per precision).
 vgx_status vgx_pagerank_values_device(vgx_pagerank* p, const void** dev, uint64_t* count, int* precision) {
   VGX_API_BEGIN

@@ -138,6 +138,7 @@ struct BatchScratch {
   DevBuf<uint32_t> reloc;   // relocation plan: 8 words per relocated row + chunk prefix
   DevBuf<uint8_t> flags;
   DevBuf<uint8_t> tmp;      // library temp storage
+  DevBuf<unsigned long long> extent;  // device batches: largest non-loop endpoint + 1
   uint64_t k = 0, m = 0;    // ops, arcs
   uint64_t nfinal = 0, nins = 0, nbig = 0, nmid = 0, need = 0;
 };
@@ -219,6 +220,11 @@ void graph_grow_vertices(vgx_graph* g, uint64_t n_new);
 // Stages a batch: uploads, sorts, resolves actions; grows vertices; sizes
 // the arena for the inserts.  Does not change adjacency.
 void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k);
+// The same for a batch already in device memory (src, dst, add device
+// pointers), written on `producer` (waited for first when it is not the
+// graph's stream; null = the graph's stream).
+void graph_stage_device(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
+                        hipStream_t producer);
 void graph_apply_deletions(vgx_graph* g);
 void graph_apply_additions(vgx_graph* g);
 void graph_finish_batch(vgx_graph* g);

@@ -158,6 +158,77 @@ py::array_t<T> to_numpy(std::vector<T>&& v) {
   return py::array_t<T>(static_cast<py::ssize_t>(heap->size()), heap->data(), owner);
 }
 
+// DLPack (the published v0 ABI of dlpack.h: DLDevice, DLDataType, DLTensor,
+// DLManagedTensor) for zero-copy views of device arrays, consumed by
+// torch.from_dlpack.  The view keeps the owning Python object alive until
+// the consumer releases it.
+struct DLDevice {
+  std::int32_t device_type;  // 10 = kDLROCM
+  std::int32_t device_id;
+};
+struct DLDataType {
+  std::uint8_t code;  // 0 int, 1 uint, 2 float
+  std::uint8_t bits;
+  std::uint16_t lanes;
+};
+struct DLTensor {
+  void* data;
+  DLDevice device;
+  std::int32_t ndim;
+  DLDataType dtype;
+  std::int64_t* shape;
+  std::int64_t* strides;
+  std::uint64_t byte_offset;
+};
+struct DLManagedTensor {
+  DLTensor dl_tensor;
+  void* manager_ctx;
+  void (*deleter)(DLManagedTensor*);
+};
+
+struct DlpackCtx {
+  std::int64_t shape[1];
+  py::object owner;
+};
+
+void dlpack_delete(DLManagedTensor* t) {
+  auto* ctx = static_cast<DlpackCtx*>(t->manager_ctx);
+  {
+    py::gil_scoped_acquire gil;
+    delete ctx;
+  }
+  delete t;
+}
+
+// A 1-D view of `count` elements at device pointer `data` on `device`.
+py::capsule dlpack_view(void* data, std::uint64_t count, int device, DLDataType dtype, py::object owner) {
+  auto* ctx = new DlpackCtx{{static_cast<std::int64_t>(count)}, std::move(owner)};
+  auto* t = new DLManagedTensor{};
+  t->dl_tensor.data = data;
+  t->dl_tensor.device = DLDevice{10, device};
+  t->dl_tensor.ndim = 1;
+  t->dl_tensor.dtype = dtype;
+  t->dl_tensor.shape = ctx->shape;
+  t->dl_tensor.strides = nullptr;
+  t->dl_tensor.byte_offset = 0;
+  t->manager_ctx = ctx;
+  t->deleter = dlpack_delete;
+  // An unconsumed capsule frees the view itself; a consumer renames it
+  // "used_dltensor" and calls the deleter when done.
+  return py::capsule(t, "dltensor", [](PyObject* cap) {
+    if (PyCapsule_IsValid(cap, "dltensor")) {
+      auto* m = static_cast<DLManagedTensor*>(PyCapsule_GetPointer(cap, "dltensor"));
+      if (m && m->deleter) m->deleter(m);
+    }
+  });
+}
+
+int device_of(vgx_graph* g) {
+  int dev = 0;
+  velographx::vgx_check(vgx_graph_device(g, &dev));
+  return dev;
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_velographx, m) {
@@ -258,6 +329,18 @@ PYBIND11_MODULE(_velographx, m) {
              return d;
            })
       .def("storage_bytes", &DynamicGraph::storage_bytes)
+      .def(
+          "_apply_device",
+          [](DynamicGraph& g, std::uintptr_t src, std::uintptr_t dst, std::uintptr_t add, std::uint64_t k,
+             std::uintptr_t stream) {
+            py::gil_scoped_release nogil;
+            velographx::vgx_check(vgx_graph_apply_device(g.handle(), reinterpret_cast<const std::uint32_t*>(src),
+                                                         reinterpret_cast<const std::uint32_t*>(dst),
+                                                         reinterpret_cast<const std::uint8_t*>(add), k,
+                                                         reinterpret_cast<void*>(stream)));
+          },
+          py::arg("src_ptr"), py::arg("dst_ptr"), py::arg("add_ptr"), py::arg("k"), py::arg("stream") = 0)
+      .def_property_readonly("device", [](const DynamicGraph& g) { return device_of(g.handle()); })
       .def("is_compact", &DynamicGraph::is_compact)
       .def(
           "probe_neighbors",
@@ -402,6 +485,30 @@ PYBIND11_MODULE(_velographx, m) {
       .def_property_readonly("last_affected_vertices", &IncrementalBFS::last_affected_vertices)
       .def_property_readonly("last_used_full_recompute", &IncrementalBFS::last_used_full_recompute)
       .def("distances_numpy", [](const IncrementalBFS& x) { return to_numpy(x.distances()); })
+      .def(
+          "distances_dlpack",
+          [](py::object self) {
+            const auto& b = self.cast<const IncrementalBFS&>();
+            const std::uint32_t* dev = nullptr;
+            std::uint64_t n = 0;
+            velographx::vgx_check(vgx_bfs_distances_device(b.handle(), &dev, &n));
+            vgx_graph* g = nullptr;
+            velographx::vgx_check(vgx_bfs_graph(b.handle(), &g));
+            return dlpack_view(const_cast<std::uint32_t*>(dev), n, device_of(g), DLDataType{1, 32, 1}, self);
+          },
+          "Zero-copy DLPack view (uint32, on the device) of the current distances; valid until the "
+          "object's next apply or recompute.")
+      .def(
+          "_apply_device",
+          [](IncrementalBFS& b, std::uintptr_t src, std::uintptr_t dst, std::uintptr_t add, std::uint64_t k,
+             std::uintptr_t stream) {
+            py::gil_scoped_release nogil;
+            velographx::vgx_check(vgx_bfs_apply_device(b.handle(), reinterpret_cast<const std::uint32_t*>(src),
+                                                       reinterpret_cast<const std::uint32_t*>(dst),
+                                                       reinterpret_cast<const std::uint8_t*>(add), k,
+                                                       reinterpret_cast<void*>(stream)));
+          },
+          py::arg("src_ptr"), py::arg("dst_ptr"), py::arg("add_ptr"), py::arg("k"), py::arg("stream") = 0)
       .def("apply_arrays",
            [](IncrementalBFS& b, const U32Array& src, const U32Array& dst,
               const py::array_t<std::uint8_t, py::array::c_style | py::array::forcecast>& add) {
@@ -496,6 +603,21 @@ PYBIND11_MODULE(_velographx, m) {
       .def(py::init<DynamicGraph&, double, int>(), py::arg("graph"), py::arg("damping") = 0.85,
            py::arg("precision") = 64, py::keep_alive<1, 2>())
       .def_property_readonly("values", [](const IncrementalPageRank& x) { return x.values(); })
+      .def(
+          "values_dlpack",
+          [](py::object self) {
+            const auto& pr = self.cast<const IncrementalPageRank&>();
+            const void* dev = nullptr;
+            std::uint64_t n = 0;
+            int precision = 64;
+            velographx::vgx_check(vgx_pagerank_values_device(pr.handle(), &dev, &n, &precision));
+            vgx_graph* g = nullptr;
+            velographx::vgx_check(vgx_pagerank_graph(pr.handle(), &g));
+            const DLDataType dt{2, static_cast<std::uint8_t>(precision), 1};
+            return dlpack_view(const_cast<void*>(dev), n, device_of(g), dt, self);
+          },
+          "Zero-copy DLPack view (float64, or float32 at precision 32, on the device) of the current "
+          "ranks; valid until the object's next apply or recompute.")
       .def("apply", &IncrementalPageRank::apply, py::arg("batch"), py::arg("local_iterations") = 24,
            py::arg("tol") = 1e-9, py::arg("full_fallback_fraction") = 0.60, py::call_guard<py::gil_scoped_release>())
       .def("recompute", &IncrementalPageRank::recompute, py::arg("max_iterations") = 200, py::arg("tol") = 1e-12,
