@@ -594,8 +594,10 @@ def _run_repairs(vx, g, root, batches, args, policy):
     """One pass over the update stream with the given arm policy:
     "adaptive" (the selector picks repair or recompute per batch, the
     reference harness's set-up with the in-object fallback disabled),
-    "incremental" (always the localized repair, in-object fallback 0.35) or
-    "full" (graph apply + full recompute).  Consolidation every
+    "incremental" (always the localized repair, in-object fallback 0.35),
+    "incremental_device" (the same with every batch uploaded to HBM before
+    the timer and applied from there) or "full" (graph apply + full
+    recompute).  Consolidation every
     `consolidate_every` batches inside the timer; verification against a
     fresh BFS every 10 batches outside it."""
     import collections
@@ -604,6 +606,11 @@ def _run_repairs(vx, g, root, batches, args, policy):
 
     inc = vx.IncrementalBFS(g, int(root), 2.0 if policy == "adaptive" else 0.35)
     sel = vx.AdaptiveSelector(inc) if policy == "adaptive" else None
+    dev = None
+    if policy == "incremental_device":  # batches resident in HBM before the timer
+        dev = [(torch.from_numpy(bs.view(np.int32)).cuda(), torch.from_numpy(bd.view(np.int32)).cuda(),
+                torch.from_numpy(ba.astype(np.uint8)).cuda()) for bs, bd, ba in batches]
+        torch.cuda.synchronize()
     batch_t, reasons = [], collections.Counter()
     full = 0
     exact = True
@@ -614,6 +621,8 @@ def _run_repairs(vx, g, root, batches, args, policy):
             tr = sel.apply_arrays(bs, bd, ba)
         elif policy == "incremental":
             inc.apply_arrays(bs, bd, ba)
+        elif policy == "incremental_device":
+            inc.apply_tensors(*dev[i])
         else:
             g.apply_arrays(bs, bd, ba)
             inc.recompute()
@@ -623,7 +632,7 @@ def _run_repairs(vx, g, root, batches, args, policy):
         if policy == "adaptive":
             reasons[tr["reason"]] += 1
             full += int(tr["chose_full"] or tr["internal_fallback"])
-        elif policy == "incremental":
+        elif policy in ("incremental", "incremental_device"):
             full += int(inc.last_used_full_recompute)
         else:
             full += 1
@@ -650,7 +659,8 @@ def leg_repairs(vx, wl, g, s, d, n, root, args):
     out["policy"] = "adaptive selector (publication-preflight-v1), consolidation every %d" % args.consolidate_every
     out["consolidate_every"] = args.consolidate_every
     # the two fixed arms on fresh copies of the same graph, for comparison
-    for policy in ("incremental", "full"):
+    # ("incremental_device": the same repairs with each batch already in HBM)
+    for policy in ("incremental", "incremental_device", "full"):
         g2 = vx.Graph(n, False)
         g2.bulk_load_arrays(s, d)
         r = _run_repairs(vx, g2, root, batches, args, policy)

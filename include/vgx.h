@@ -12,8 +12,9 @@
  *  - Every call returns vgx_status; on failure vgx_last_error() holds a
  *    message for the calling thread.
  *  - Host pointers ("const uint32_t* src") are read during the call only.
- *    Arguments named *_dev are device pointers (HBM), used on the stream
- *    passed alongside (NULL = the graph's own stream).
+ *    Arguments named *_dev are device pointers (HBM), used on (or, for
+ *    update batches, waited for on) the stream passed alongside, a
+ *    hipStream_t; NULL is the default (null) stream, torch's default.
  *  - Results are answer-ready when a call returns (the call synchronises
  *    its stream), unless the name ends in _async.
  *  - Vertex ids are uint32 below 2^31 (bit 31 marks tombstones on device).
@@ -152,8 +153,9 @@ vgx_status vgx_bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, c
 /* The same batch already in device memory (zero-copy ingestion,
  * python/bindings.cpp:21-36's from_numpy path without the host copy):
  * src_dev, dst_dev, add_dev are device pointers of this graph's device,
- * written on `stream` (a hipStream_t, synchronised first; null = the
- * graph's own stream).  Results equal vgx_bfs_apply's on the same ops. */
+ * written on `stream` (a hipStream_t, synchronised first unless it is the
+ * graph's own; null = the default stream).  Results equal vgx_bfs_apply's
+ * on the same ops. */
 vgx_status vgx_bfs_apply_device(vgx_bfs* b, const uint32_t* src_dev, const uint32_t* dst_dev, const uint8_t* add_dev,
                                 uint64_t k, void* stream);
 

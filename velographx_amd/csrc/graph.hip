@@ -875,33 +875,36 @@ void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const u
   stage_sorted(g, k);
 }
 
-// Largest endpoint + 1 over the batch's non-loop ops (0 when all are loops).
-__global__ void k_batch_extent(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint64_t k,
-                               unsigned long long* __restrict__ out) {
+// Copies a device batch into the staging layout (src | dst, add) and
+// computes the largest endpoint + 1 over its non-loop ops (0 when all are
+// loops) in the same pass.
+__global__ void k_batch_ingest(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                               const uint8_t* __restrict__ add, uint64_t k, uint32_t* __restrict__ sd,
+                               uint8_t* __restrict__ add_out, unsigned long long* __restrict__ extent) {
   unsigned long long m = 0;
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < k;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint32_t u = src[i], v = dst[i];
+    sd[i] = u;
+    sd[k + i] = v;
+    add_out[i] = add[i];
     if (u != v) m = max(m, static_cast<unsigned long long>(max(u, v)) + 1ull);
   }
   m = wave_max(m);
-  if (lane_id() == 0 && m) atomicMax(out, m);
+  if (lane_id() == 0 && m) atomicMax(extent, m);
 }
 
 void graph_stage_device(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
                         hipStream_t producer) {
   if (k >= (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "update batch too large");
   hipStream_t s = g->stream;
-  if (producer && producer != s) VGX_HIP(hipStreamSynchronize(producer));
+  if (producer != s) VGX_HIP(hipStreamSynchronize(producer));  // null: the default stream
   BatchScratch& b0 = g->bs[0];
   b0.src.ensure(2 * k + 64);
   b0.add.ensure(k + 64);
   b0.extent.ensure(1);
-  VGX_HIP(hipMemcpyAsync(b0.src.p, src, 4 * k, hipMemcpyDeviceToDevice, s));
-  VGX_HIP(hipMemcpyAsync(b0.src.p + k, dst, 4 * k, hipMemcpyDeviceToDevice, s));
-  VGX_HIP(hipMemcpyAsync(b0.add.p, add, k, hipMemcpyDeviceToDevice, s));
   VGX_HIP(hipMemsetAsync(b0.extent.p, 0, sizeof(unsigned long long), s));
-  k_batch_extent<<<grid_for(k, kBlock, 1024), kBlock, 0, s>>>(b0.src.p, b0.src.p + k, k, b0.extent.p);
+  k_batch_ingest<<<grid_for(k, kBlock, 1024), kBlock, 0, s>>>(src, dst, add, k, b0.src.p, b0.add.p, b0.extent.p);
   VGX_LAUNCH_CHECK();
   unsigned long long ext = 0;
   VGX_HIP(hipMemcpyAsync(&ext, b0.extent.p, sizeof(ext), hipMemcpyDeviceToHost, s));

@@ -496,7 +496,7 @@ vgx_status vgx_ppr_forward(vgx_graph* g, const float* w_dev, const float* s_dev,
   if (K == 0 || K > 32) throw Error(VGX_INVALID_ARGUMENT, "K must be in [1, 32]");
   if (!(damping >= 0.0 && damping <= 1.0)) throw Error(VGX_INVALID_ARGUMENT, "damping must be in [0, 1]");
   VGX_HIP(hipSetDevice(g->device));
-  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->stream;
+  hipStream_t s = static_cast<hipStream_t>(stream);  // null: the default stream
   if (s_dev == x_dev) throw Error(VGX_INVALID_ARGUMENT, "S and X must not alias");
   ppr_prepare(g, w_dev, s, true);
   const CsrSnapshot& in = graph_snapshot(g, 1);
@@ -520,7 +520,7 @@ vgx_status vgx_ppr_backward(vgx_graph* g, const float* w_dev, const float* x_dev
   if (!g || !w_dev || !x_dev || !c_dev || !lambda_dev) throw Error(VGX_INVALID_ARGUMENT, "null argument");
   if (K == 0 || K > 32) throw Error(VGX_INVALID_ARGUMENT, "K must be in [1, 32]");
   VGX_HIP(hipSetDevice(g->device));
-  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->stream;
+  hipStream_t s = static_cast<hipStream_t>(stream);  // null: the default stream
   if (c_dev == lambda_dev) throw Error(VGX_INVALID_ARGUMENT, "C and Lambda must not alias");
   ppr_prepare(g, w_dev, s, false);
   const CsrSnapshot& out = graph_snapshot(g, 0);

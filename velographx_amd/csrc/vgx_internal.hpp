@@ -222,7 +222,7 @@ void graph_grow_vertices(vgx_graph* g, uint64_t n_new);
 void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k);
 // The same for a batch already in device memory (src, dst, add device
 // pointers), written on `producer` (waited for first when it is not the
-// graph's stream; null = the graph's stream).
+// graph's stream; null = the default stream).
 void graph_stage_device(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
                         hipStream_t producer);
 void graph_apply_deletions(vgx_graph* g);
