@@ -47,6 +47,7 @@ typedef struct vgx_graph vgx_graph;
 typedef struct vgx_bfs vgx_bfs;
 typedef struct vgx_pagerank vgx_pagerank;
 typedef struct vgx_msbfs vgx_msbfs;
+typedef struct vgx_kcore vgx_kcore;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* vgx_last_error(void);
@@ -370,6 +371,28 @@ vgx_status vgx_ppr_forward(vgx_graph* g, const float* w_dev, const float* s_dev,
 vgx_status vgx_ppr_backward(vgx_graph* g, const float* w_dev, const float* x_dev, const float* c_dev,
                             float* lambda_dev, float* grad_s_dev, float* grad_w_dev, uint32_t K,
                             double damping, uint32_t iterations, void* stream);
+
+/* ----------------------------------------------------------------- k-core */
+
+/* BasicIncrementalKCore(graph) -- incremental/kcore.hpp:20-26, 58-63: core
+ * numbers of an undirected graph (a directed one is VGX_INVALID_ARGUMENT,
+ * as the reference throws).  Computed on the device by level-synchronous
+ * peeling; the object borrows the graph. */
+vgx_status vgx_kcore_create(vgx_graph* g, vgx_kcore** out);
+vgx_status vgx_kcore_destroy(vgx_kcore* kc);
+/* recompute() -- kcore.hpp:58-63. */
+vgx_status vgx_kcore_recompute(vgx_kcore* kc);
+/* apply(batch) -- kcore.hpp:31-56: applies the batch to the graph and leaves
+ * the core numbers of the new graph (the reference repairs the touched
+ * components; the values are the same).  k = 0 changes nothing and sets
+ * last_repaired_vertices to 0, as the reference's empty batch does. */
+vgx_status vgx_kcore_apply(vgx_kcore* kc, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k);
+/* core() -- kcore.hpp:28: copies min(cap, n) core numbers, *count = n. */
+vgx_status vgx_kcore_core(vgx_kcore* kc, uint32_t* out, uint64_t cap, uint64_t* count);
+/* last_repaired_vertices() -- kcore.hpp:29 (here: the vertex count after a
+ * non-empty batch or recompute, 0 after an empty batch), and the peeling's
+ * levels and rounds of the last computation. */
+vgx_status vgx_kcore_stats(vgx_kcore* kc, uint64_t* last_repaired_vertices, uint32_t* levels, uint32_t* rounds);
 
 /* ------------------------------------------------------- workload helpers */
 

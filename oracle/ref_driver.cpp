@@ -20,6 +20,8 @@
 //   pr_recompute <iters> <tol>      pr.recompute(iters, tol)
 //   pr_validate <it> <tol> <l1> <linf>
 //   neighbors <u>
+//   sssp_new <source> / sssp_apply  IncrementalSSSP(graph, source) / apply(batch)
+//   kcore_new / kcore_apply         IncrementalKCore(graph) / apply(batch)
 //   cons_policy <arcs> <latency>    scale_aware_consolidation_policy
 //   cons_eval <cur> <canon> <cur_lat> <canon_lat> <max_s> <max_l>
 //                                   evaluate_consolidation
@@ -42,7 +44,9 @@
 #include "velographx/algorithms.hpp"
 #include "velographx/csr_graph.hpp"
 #include "velographx/incremental/bfs.hpp"
+#include "velographx/incremental/kcore.hpp"
 #include "velographx/incremental/pagerank.hpp"
+#include "velographx/incremental/sssp.hpp"
 #include "velographx/storage/consolidation.hpp"
 #include "velographx/storage/dynamic_graph.hpp"
 
@@ -72,11 +76,28 @@ void print_pr(const IncrementalPageRank& pr) {
               pr.last_residual_l1(), pr.last_residual_linf(), pr.last_full_recompute_converged() ? 1 : 0);
 }
 
+void print_sssp(const velographx::IncrementalSSSP& s) {
+  const auto& d = s.distances();
+  std::printf("sdist %zu", d.size());
+  for (auto x : d) std::printf(" %llu", static_cast<unsigned long long>(x));
+  std::printf("\n");
+}
+
+void print_kcore(const velographx::IncrementalKCore& k) {
+  const auto& c = k.core();
+  std::printf("kcore %zu", c.size());
+  for (auto x : c) std::printf(" %u", x);
+  std::printf("\n");
+  std::printf("kcorediag %zu\n", k.last_repaired_vertices());
+}
+
 int run_scenario(std::istream& in) {
   std::unique_ptr<DynamicGraph> g;
   std::unique_ptr<IncrementalBFS> bfs;
   std::unique_ptr<IncrementalPageRank> pr;
   std::unique_ptr<velographx::ConsolidationController> ctl;
+  std::unique_ptr<velographx::IncrementalSSSP> sssp;
+  std::unique_ptr<velographx::IncrementalKCore> kcore;
   UpdateBatch batch;
   std::string cmd;
   while (in >> cmd) {
@@ -86,6 +107,8 @@ int run_scenario(std::istream& in) {
       in >> n >> directed;
       bfs.reset();
       pr.reset();
+      sssp.reset();
+      kcore.reset();
       g = std::make_unique<DynamicGraph>(n, directed != 0);
     } else if (cmd == "bulk") {
       std::size_t m;
@@ -142,6 +165,20 @@ int run_scenario(std::istream& in) {
       const auto v = pr->validate_against_full(it, tol, l1, linf);
       std::printf("prvalid %.17g %.17g %zu %d %d\n", v.l1_error, v.linf_error, v.reference_iterations,
                   v.reference_converged ? 1 : 0, v.within_tolerance ? 1 : 0);
+    } else if (cmd == "kcore_new") {
+      kcore = std::make_unique<velographx::IncrementalKCore>(*g);
+      print_kcore(*kcore);
+    } else if (cmd == "kcore_apply") {
+      kcore->apply(batch);
+      print_kcore(*kcore);
+    } else if (cmd == "sssp_new") {
+      VertexId s;
+      in >> s;
+      sssp = std::make_unique<velographx::IncrementalSSSP>(*g, s);
+      print_sssp(*sssp);
+    } else if (cmd == "sssp_apply") {
+      sssp->apply(batch);
+      print_sssp(*sssp);
     } else if (cmd == "cons_policy") {
       std::size_t arcs;
       double latency;

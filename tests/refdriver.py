@@ -61,6 +61,12 @@ def parse(text):
             recs.append(("prvalid", {"l1_error": float(w[1]), "linf_error": float(w[2]),
                                      "reference_iterations": int(w[3]), "reference_converged": w[4] == "1",
                                      "within_tolerance": w[5] == "1"}))
+        elif tag == "kcore":
+            recs.append(("kcore", np.array([int(x) for x in w[2:]], dtype=np.uint32)))
+        elif tag == "kcorediag":
+            recs.append(("kcorediag", int(w[1])))
+        elif tag == "sdist":
+            recs.append(("sdist", np.array([int(x) for x in w[2:]], dtype=np.uint64)))
         elif tag == "cpolicy":
             recs.append(("cpolicy", (float(w[1]), float(w[2]))))
         elif tag == "csignal":

@@ -29,7 +29,9 @@ from ._velographx import (  # noqa: F401
     CsrGraph,
     Graph,
     IncrementalBFS,
+    IncrementalKCore,
     IncrementalPageRank,
+    IncrementalSSSP,
     MultiSourceBFS,
     PageRankValidation,
     UpdateBatch,
@@ -53,7 +55,7 @@ __all__ = [
     "rmat_edges", "device_count", "version", "MultiSourceBFS", "AdaptivePolicy", "AdaptiveSelector",
     "ConsolidationPolicy", "ConsolidationSignal", "ConsolidationControllerConfig", "ConsolidationController",
     "ConsolidationSnapshot", "evaluate_consolidation", "scale_aware_consolidation_policy",
-    "consolidate_to_csr_snapshot", "CsrGraph", "bfs_distances", "pagerank",
+    "consolidate_to_csr_snapshot", "CsrGraph", "bfs_distances", "pagerank", "IncrementalSSSP", "IncrementalKCore",
 ]
 
 

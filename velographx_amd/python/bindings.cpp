@@ -15,6 +15,8 @@
 
 #include "velographx_amd/consolidation.hpp"
 #include "velographx_amd/csr_graph.hpp"
+#include "velographx_amd/kcore.hpp"
+#include "velographx_amd/sssp.hpp"
 #include "velographx_amd/pagerank.hpp"
 #include "velographx_amd/selector.hpp"
 #include "velographx_amd/velographx.hpp"
@@ -354,6 +356,62 @@ PYBIND11_MODULE(_velographx, m) {
           },
           py::arg("count") = 4096, py::arg("repeats") = 5)
       .def_property_readonly("_handle", [](const DynamicGraph& g) { return reinterpret_cast<std::uintptr_t>(g.handle()); });
+
+  // incremental/sssp.hpp (unweighted SSSP on the BFS engine)
+  py::class_<velographx::IncrementalSSSP>(m, "IncrementalSSSP")
+      .def(py::init<DynamicGraph&, VertexId>(), py::arg("graph"), py::arg("source"), py::keep_alive<1, 2>())
+      .def_property_readonly("distances", &velographx::IncrementalSSSP::distances)
+      .def("distances_numpy", [](const velographx::IncrementalSSSP& x) { return to_numpy(x.distances()); })
+      .def("apply", &velographx::IncrementalSSSP::apply, py::call_guard<py::gil_scoped_release>())
+      .def("recompute", &velographx::IncrementalSSSP::recompute, py::call_guard<py::gil_scoped_release>())
+      .def(
+          "apply_arrays",
+          [](velographx::IncrementalSSSP& x, const U32Array& src, const U32Array& dst,
+             const py::array_t<std::uint8_t, py::array::c_style | py::array::forcecast>& add) {
+            if (src.size() != dst.size() || src.size() != add.size())
+              throw std::invalid_argument("src, dst and add must have equal length");
+            const auto* s = src.data();
+            const auto* d = dst.data();
+            const auto* a = add.data();
+            const auto k = static_cast<std::size_t>(src.size());
+            py::gil_scoped_release nogil;
+            x.apply_arrays(s, d, a, k);
+          },
+          py::arg("src"), py::arg("dst"), py::arg("add"))
+      .def_property_readonly_static("infinity", [](py::object) { return velographx::IncrementalSSSP::infinity; });
+
+  // incremental/kcore.hpp (core numbers by device peeling)
+  py::class_<velographx::IncrementalKCore>(m, "IncrementalKCore")
+      .def(py::init<DynamicGraph&>(), py::arg("graph"), py::keep_alive<1, 2>())
+      .def_property_readonly("core", &velographx::IncrementalKCore::core)
+      .def("core_numpy", [](const velographx::IncrementalKCore& x) { return to_numpy(x.core()); })
+      .def_property_readonly("last_repaired_vertices", &velographx::IncrementalKCore::last_repaired_vertices)
+      .def("apply", &velographx::IncrementalKCore::apply, py::call_guard<py::gil_scoped_release>())
+      .def("recompute", &velographx::IncrementalKCore::recompute, py::call_guard<py::gil_scoped_release>())
+      .def(
+          "apply_arrays",
+          [](velographx::IncrementalKCore& x, const U32Array& src, const U32Array& dst,
+             const py::array_t<std::uint8_t, py::array::c_style | py::array::forcecast>& add) {
+            if (src.size() != dst.size() || src.size() != add.size())
+              throw std::invalid_argument("src, dst and add must have equal length");
+            const auto* s = src.data();
+            const auto* d = dst.data();
+            const auto* a = add.data();
+            const auto k = static_cast<std::size_t>(src.size());
+            py::gil_scoped_release nogil;
+            x.apply_arrays(s, d, a, k);
+          },
+          py::arg("src"), py::arg("dst"), py::arg("add"))
+      .def("stats", [](const velographx::IncrementalKCore& x) {
+        std::uint64_t r = 0;
+        std::uint32_t levels = 0, rounds = 0;
+        velographx::vgx_check(vgx_kcore_stats(x.handle(), &r, &levels, &rounds));
+        py::dict d;
+        d["last_repaired_vertices"] = r;
+        d["levels"] = levels;
+        d["rounds"] = rounds;
+        return d;
+      });
 
   // csr_graph.hpp, algorithms.hpp:8-10 (static graph, one-shot algorithms)
   py::class_<velographx::CsrGraph>(m, "CsrGraph")
