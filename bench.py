@@ -31,6 +31,8 @@ Secondary legs, each in its own object of the same JSON line:
                cutover; the reference engine's snapshot and probe beside it;
 * static    -- the static CsrGraph of the same edges: build, bfs_distances,
                20 pagerank sweeps; the reference's beside it;
+* kcore     -- all core numbers of undirected R-MAT s20 by device peeling;
+               the reference's IncrementalKCore construction beside it;
 * repairs   -- 100 update batches of 10,000 ops (80% inserts of random
                non-edges, 20% deletions of live edges) answered with exact
                distances, the adaptive selector choosing localized repair or
@@ -104,6 +106,7 @@ def parse():
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-consolidation", action="store_true")
     p.add_argument("--no-static", action="store_true")
+    p.add_argument("--no-kcore", action="store_true")
     p.add_argument("--cons-epochs", type=int, default=24)
     p.add_argument("--cons-batch", type=int, default=1000000)
     p.add_argument("--no-contrast", action="store_true")
@@ -575,6 +578,44 @@ def leg_static(vx, s, d, root):
     return out
 
 
+def leg_kcore(vx, args, with_cpu):
+    """Core numbers of undirected R-MAT at the batching scale (s20): the
+    device peeling from the bulk-loaded graph (best of 3), and the
+    reference's IncrementalKCore construction on the same edges."""
+    import torch
+
+    scale = args.batch_scale
+    n = 1 << scale
+    s, d = vx.rmat_edges(scale, args.edgefactor, 1)
+    g = vx.Graph(n, False)
+    g.bulk_load_arrays(s, d)
+    kc = vx.IncrementalKCore(g)  # warm-up
+    t = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        kc.recompute()
+        t.append(time.perf_counter() - t0)
+    core = kc.core_numpy()
+    st = kc.stats()
+    out = {"workload": f"undirected R-MAT s{scale} ef{args.edgefactor} seed 1: all core numbers",
+           "arcs": int(g.edge_count_directed), "ms": 1e3 * min(t), "max_core": int(core.max()),
+           "levels": int(st["levels"]), "rounds": int(st["rounds"]),
+           "arcs_per_s": g.edge_count_directed / min(t)}
+    if with_cpu and os.path.exists(REF_BIN):
+        with tempfile.TemporaryDirectory() as td:
+            cmd = [REF_BIN, "bench-kcore", _edges_file(td, s, d), str(n)]
+            res = subprocess.run(cmd, capture_output=True, text=True, timeout=900, check=True).stdout
+        w = [line.split() for line in res.splitlines() if line.startswith("kcore ")][0]
+        out["cpu_reference"] = {"ms": 1e3 * float(w[1]), "max_core": int(w[2]), "cores": 1, "kind": "reference",
+                                "sample": "IncrementalKCore(graph) construction of the reference engine on the "
+                                          "same edges"}
+        out["max_core_equal"] = int(w[2]) == out["max_core"]
+    del kc, g
+    log(f"kcore: {out}")
+    return out
+
+
 def cpu_static(s, d, root):
     """The reference's CsrGraph construction, bfs_distances and 3 pagerank
     sweeps on the same edges (one thread)."""
@@ -840,6 +881,7 @@ def main():
         return
     if args.only_bfs:
         args.no_cpu = args.no_repairs = args.no_batched = args.no_contrast = args.no_consolidation = args.no_static = True
+        args.no_kcore = True
         args.no_pagerank = args.no_ppr = args.no_batching = True
     import torch
 
@@ -953,6 +995,8 @@ def main():
         result["contrast"] = leg_contrast(vx, wl, args.steps)
     if not args.no_batching:
         result["batching"] = leg_batching(vx, wl, args)
+    if not args.no_kcore:
+        result["kcore"] = leg_kcore(vx, args, not args.no_cpu)
     if not args.no_pagerank:
         result["pagerank"] = leg_pagerank(vx, wl, args, args.no_cpu)
     if not args.no_ppr:

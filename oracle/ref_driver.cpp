@@ -29,7 +29,8 @@
 //   cons_observe <sr> <lr> <se> <le> <sc> <epoch>   controller.observe
 //   cons_mark <epoch>               controller.mark_consolidated
 // Bench modes: bench-bfs, bench-pr, bench-repair, bench-snapshot,
-// bench-static (see their comments).
+// bench-static, bench-kcore (see their comments).
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -327,6 +328,23 @@ int bench_static(char** argv) {
   return d.empty() || pr.empty() ? 1 : 0;
 }
 
+// bench-kcore <edges.bin> <n>
+// Times IncrementalKCore(graph) on the undirected graph (its constructor
+// peels every vertex, one thread).  Prints "kcore <seconds> <max_core>".
+int bench_kcore(char** argv) {
+  const auto edges = read_edges_bin(argv[2]);
+  const std::size_t n = std::stoull(argv[3]);
+  DynamicGraph g(n, false);
+  g.bulk_load_edges(edges);
+  const auto t0 = Clock::now();
+  velographx::IncrementalKCore kc(g);
+  const auto t1 = Clock::now();
+  std::uint32_t mx = 0;
+  for (auto c : kc.core()) mx = std::max(mx, c);
+  std::printf("kcore %.9f %u\n", std::chrono::duration<double>(t1 - t0).count(), mx);
+  return 0;
+}
+
 // bench-pr <edges.bin> <n> <directed> <iterations>
 // Times IncrementalPageRank(graph, 0.85) followed by recompute(iterations,
 // 0.0): exactly <iterations> pull sweeps.  Prints
@@ -389,6 +407,7 @@ int main(int argc, char** argv) {
   if (argc == 6 && std::strcmp(argv[1], "bench-pr") == 0) return bench_pr(argv);
   if (argc == 6 && std::strcmp(argv[1], "bench-snapshot") == 0) return bench_snapshot(argv);
   if (argc == 6 && std::strcmp(argv[1], "bench-static") == 0) return bench_static(argv);
+  if (argc == 4 && std::strcmp(argv[1], "bench-kcore") == 0) return bench_kcore(argv);
   if (argc == 8 && std::strcmp(argv[1], "bench-repair") == 0) return bench_repair(argv);
   if (argc >= 3 && std::strcmp(argv[1], "scenario") == 0) {
     std::ifstream f(argv[2]);
