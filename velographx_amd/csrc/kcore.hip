@@ -26,9 +26,15 @@ extern thread_local std::string t_last_error;
 constexpr uint32_t kKcUnset = 0xFFFFFFFFu;
 constexpr int kKcBlock = 256;
 
+// Peel rounds run several per host check: round r reads its queue length
+// from ring[r % 3], appends to ring[(r + 1) % 3] and clears ring[(r + 2) % 3]
+// (the next round's output; its last reader, round r - 1, has finished).
+// The scan appends the level's first queue to ring[0].
+constexpr uint32_t kKcRoundsPerCheck = 4;
+
 struct KcCounters {
-  unsigned long long queued;     // claimed by the scan (level queue length)
-  unsigned long long next;       // claimed by a peel round
+  unsigned long long ring[3];
+  unsigned long long queued;     // the scan's claims (copied by the level's first round)
   unsigned long long remaining;  // unpeeled vertices left after the scan's claims
   unsigned long long min_deg;    // smallest remaining degree among them
 };
@@ -92,7 +98,7 @@ __global__ __launch_bounds__(kKcBlock) void k_kc_scan(uint32_t k, uint64_t n, co
         mn = min(mn, static_cast<unsigned long long>(d));
       }
     }
-    kc_push(claim, static_cast<uint32_t>(v), q, &ctr->queued);
+    kc_push(claim, static_cast<uint32_t>(v), q, &ctr->ring[0]);
   }
   rem = wave_sum(rem);
   mn = ~wave_max(~mn);  // the wave's minimum
@@ -102,11 +108,17 @@ __global__ __launch_bounds__(kKcBlock) void k_kc_scan(uint32_t k, uint64_t n, co
   }
 }
 
-__global__ __launch_bounds__(kKcBlock) void k_kc_peel(uint32_t k, const uint2* __restrict__ row,
+__global__ __launch_bounds__(kKcBlock) void k_kc_peel(uint32_t k, uint32_t round, const uint2* __restrict__ row,
                                                       const uint32_t* __restrict__ arena,
-                                                      const uint32_t* __restrict__ q, uint64_t count,
-                                                      uint32_t* __restrict__ deg, uint32_t* __restrict__ core,
-                                                      uint32_t* __restrict__ qnext, KcCounters* __restrict__ ctr) {
+                                                      const uint32_t* __restrict__ q, uint32_t* __restrict__ deg,
+                                                      uint32_t* __restrict__ core, uint32_t* __restrict__ qnext,
+                                                      KcCounters* __restrict__ ctr) {
+  const uint64_t count = ctr->ring[round % 3];  // written by the previous launch
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctr->ring[(round + 2) % 3] = 0;
+    if (round == 0) ctr->queued = count;
+  }
+  unsigned long long* next = &ctr->ring[(round + 1) % 3];
   const uint64_t W = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
   const unsigned lane = lane_id();
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave; i < count;
@@ -126,7 +138,7 @@ __global__ __launch_bounds__(kKcBlock) void k_kc_peel(uint32_t k, const uint2* _
           }
         }
       }
-      kc_push(claim, v, qnext, &ctr->next);
+      kc_push(claim, v, qnext, next);
     }
   }
 }
@@ -147,38 +159,41 @@ static void kc_recompute(vgx_kcore* kc) {
   const Adjacency& a = g->adj[0];
   k_kc_init<<<kc_blocks(n, kKcBlock, 4096), kKcBlock, 0, s>>>(a.deg.p, n, kc->deg.p, kc->core.p);
   VGX_LAUNCH_CHECK();
+  // the peel grid is fixed (grid-stride over the device-side queue length)
+  const unsigned peel_blocks = kc_blocks(n, kKcBlock / kWave, 2048);
   uint32_t k = 0;
+  // Per level: the scan and the first rounds go out together and one
+  // readback tells whether the level claimed anything, what remains and the
+  // smallest remaining degree; longer levels continue a few rounds per check.
+  auto rounds = [&](uint32_t& r) {
+    for (uint32_t i = 0; i < kKcRoundsPerCheck; ++i, ++r) {
+      k_kc_peel<<<peel_blocks, kKcBlock, 0, s>>>(k, r, a.row.p, a.arena.p, kc->q[r & 1].p, kc->deg.p, kc->core.p,
+                                                 kc->q[(r + 1) & 1].p, kc->d_ctr);
+      VGX_LAUNCH_CHECK();
+    }
+    kc->rounds += kKcRoundsPerCheck;
+    VGX_HIP(hipMemcpyAsync(kc->h_ctr, kc->d_ctr, sizeof(KcCounters), hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    const uint64_t c = kc->h_ctr->ring[r % 3];
+    if (c > n || kc->h_ctr->queued > n) throw Error(VGX_DEVICE_ERROR, "k-core queue exceeded the vertex count");
+    return c;
+  };
   for (;;) {
-    KcCounters zero{0, 0, 0, ~0ull};
+    KcCounters zero{{0, 0, 0}, 0, 0, ~0ull};
     VGX_HIP(hipMemcpyAsync(kc->d_ctr, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
     k_kc_scan<<<kc_blocks(n, kKcBlock, 4096), kKcBlock, 0, s>>>(k, n, kc->deg.p, kc->core.p, kc->q[0].p, kc->d_ctr);
     VGX_LAUNCH_CHECK();
-    VGX_HIP(hipMemcpyAsync(kc->h_ctr, kc->d_ctr, sizeof(KcCounters), hipMemcpyDeviceToHost, s));
-    VGX_HIP(hipStreamSynchronize(s));
-    uint64_t c = kc->h_ctr->queued;
+    uint32_t r = 0;
+    uint64_t c = rounds(r);
     const uint64_t rem = kc->h_ctr->remaining;
     const unsigned long long mn = kc->h_ctr->min_deg;
-    if (c > n) throw Error(VGX_DEVICE_ERROR, "k-core queue exceeded the vertex count");
-    if (c == 0) {
+    if (kc->h_ctr->queued == 0) {
       if (rem == 0) break;
       k = static_cast<uint32_t>(mn);  // nothing at this level: the next one with vertices
       continue;
     }
     kc->levels += 1;
-    int cur = 0;
-    while (c) {
-      VGX_HIP(hipMemsetAsync(&kc->d_ctr->next, 0, sizeof(unsigned long long), s));
-      const unsigned blocks = kc_blocks(c, kKcBlock / kWave, 8192);
-      k_kc_peel<<<blocks, kKcBlock, 0, s>>>(k, a.row.p, a.arena.p, kc->q[cur].p, c, kc->deg.p, kc->core.p,
-                                            kc->q[cur ^ 1].p, kc->d_ctr);
-      VGX_LAUNCH_CHECK();
-      VGX_HIP(hipMemcpyAsync(kc->h_ctr, kc->d_ctr, sizeof(KcCounters), hipMemcpyDeviceToHost, s));
-      VGX_HIP(hipStreamSynchronize(s));
-      c = kc->h_ctr->next;
-      if (c > n) throw Error(VGX_DEVICE_ERROR, "k-core queue exceeded the vertex count");
-      cur ^= 1;
-      kc->rounds += 1;
-    }
+    while (c) c = rounds(r);
     if (rem == 0) break;  // the scan left nothing unpeeled beyond this level's queue
     k += 1;
   }
