@@ -100,9 +100,21 @@ __global__ __launch_bounds__(kKcBlock) void k_kc_scan(uint32_t k, uint64_t n, co
     }
     kc_push(claim, static_cast<uint32_t>(v), q, &ctr->ring[0]);
   }
+  // one atomic pair per workgroup: thousands of waves on one counter line
+  // serialise at the L2
+  __shared__ unsigned long long s_rem[kKcBlock / kWave], s_mn[kKcBlock / kWave];
   rem = wave_sum(rem);
   mn = ~wave_max(~mn);  // the wave's minimum
   if (lane_id() == 0) {
+    s_rem[threadIdx.x / kWave] = rem;
+    s_mn[threadIdx.x / kWave] = mn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kKcBlock / kWave; ++w) {
+      rem += s_rem[w];
+      mn = min(mn, s_mn[w]);
+    }
     if (rem) atomicAdd(&ctr->remaining, rem);
     if (mn != ~0ull) atomicMin(&ctr->min_deg, mn);
   }
@@ -160,7 +172,7 @@ static void kc_recompute(vgx_kcore* kc) {
   k_kc_init<<<kc_blocks(n, kKcBlock, 4096), kKcBlock, 0, s>>>(a.deg.p, n, kc->deg.p, kc->core.p);
   VGX_LAUNCH_CHECK();
   // the peel grid is fixed (grid-stride over the device-side queue length)
-  const unsigned peel_blocks = kc_blocks(n, kKcBlock / kWave, 2048);
+  const unsigned peel_blocks = kc_blocks(n, kKcBlock / kWave, 1024);
   uint32_t k = 0;
   // Per level: the scan and the first rounds go out together and one
   // readback tells whether the level claimed anything, what remains and the
@@ -181,7 +193,7 @@ static void kc_recompute(vgx_kcore* kc) {
   for (;;) {
     KcCounters zero{{0, 0, 0}, 0, 0, ~0ull};
     VGX_HIP(hipMemcpyAsync(kc->d_ctr, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
-    k_kc_scan<<<kc_blocks(n, kKcBlock, 4096), kKcBlock, 0, s>>>(k, n, kc->deg.p, kc->core.p, kc->q[0].p, kc->d_ctr);
+    k_kc_scan<<<kc_blocks(n, kKcBlock, 512), kKcBlock, 0, s>>>(k, n, kc->deg.p, kc->core.p, kc->q[0].p, kc->d_ctr);
     VGX_LAUNCH_CHECK();
     uint32_t r = 0;
     uint64_t c = rounds(r);
