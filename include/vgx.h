@@ -48,6 +48,7 @@ typedef struct vgx_bfs vgx_bfs;
 typedef struct vgx_pagerank vgx_pagerank;
 typedef struct vgx_msbfs vgx_msbfs;
 typedef struct vgx_kcore vgx_kcore;
+typedef struct vgx_components vgx_components;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* vgx_last_error(void);
@@ -393,6 +394,31 @@ vgx_status vgx_kcore_core(vgx_kcore* kc, uint32_t* out, uint64_t cap, uint64_t* 
  * non-empty batch or recompute, 0 after an empty batch), and the peeling's
  * levels and rounds of the last computation. */
 vgx_status vgx_kcore_stats(vgx_kcore* kc, uint64_t* last_repaired_vertices, uint32_t* levels, uint32_t* rounds);
+
+/* ------------------------------------------------------------ components */
+
+/* BasicIncrementalComponents(graph) -- incremental/connected_components.hpp:
+ * 19-25: connected components of an undirected graph (a directed one is
+ * VGX_INVALID_ARGUMENT, as the reference throws).  A component's label is
+ * its smallest vertex id (the reference's is a union-find root that depends
+ * on its union order: the partition is the same, the representative may
+ * differ).  Min-label hooking with pointer jumping on the device. */
+vgx_status vgx_components_create(vgx_graph* g, vgx_components** out);
+vgx_status vgx_components_destroy(vgx_components* cc);
+vgx_status vgx_components_recompute(vgx_components* cc);
+/* apply(batch) -- connected_components.hpp:35-97: applies the batch to the
+ * graph and relabels (k = 0 changes nothing and sets last_repaired_vertices
+ * to 0, as the reference's empty batch does). */
+vgx_status vgx_components_apply(vgx_components* cc, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
+                                uint64_t k);
+/* component(v) -- :27-30: VGX_OUT_OF_RANGE for v outside the graph. */
+vgx_status vgx_components_component(vgx_components* cc, uint32_t v, uint32_t* label);
+/* Every vertex's label: min(cap, n) values, *count = n. */
+vgx_status vgx_components_labels(vgx_components* cc, uint32_t* out, uint64_t cap, uint64_t* count);
+/* last_repaired_vertices() -- :31-33 (here: the vertex count after a
+ * non-empty batch or recompute, 0 after an empty batch) and the hooking
+ * rounds of the last computation. */
+vgx_status vgx_components_stats(vgx_components* cc, uint64_t* last_repaired_vertices, uint32_t* rounds);
 
 /* ------------------------------------------------------- workload helpers */
 

@@ -22,6 +22,7 @@
 //   neighbors <u>
 //   sssp_new <source> / sssp_apply  IncrementalSSSP(graph, source) / apply(batch)
 //   kcore_new / kcore_apply         IncrementalKCore(graph) / apply(batch)
+//   cc_new / cc_apply               IncrementalComponents(graph) / apply(batch)
 //   cons_policy <arcs> <latency>    scale_aware_consolidation_policy
 //   cons_eval <cur> <canon> <cur_lat> <canon_lat> <max_s> <max_l>
 //                                   evaluate_consolidation
@@ -45,6 +46,7 @@
 #include "velographx/algorithms.hpp"
 #include "velographx/csr_graph.hpp"
 #include "velographx/incremental/bfs.hpp"
+#include "velographx/incremental/connected_components.hpp"
 #include "velographx/incremental/kcore.hpp"
 #include "velographx/incremental/pagerank.hpp"
 #include "velographx/incremental/sssp.hpp"
@@ -84,6 +86,13 @@ void print_sssp(const velographx::IncrementalSSSP& s) {
   std::printf("\n");
 }
 
+void print_cc(velographx::IncrementalComponents& c, std::size_t n) {
+  std::printf("cclabels %zu", n);
+  for (std::size_t v = 0; v < n; ++v) std::printf(" %u", c.component(static_cast<VertexId>(v)));
+  std::printf("\n");
+  std::printf("ccdiag %zu\n", c.last_repaired_vertices());
+}
+
 void print_kcore(const velographx::IncrementalKCore& k) {
   const auto& c = k.core();
   std::printf("kcore %zu", c.size());
@@ -99,6 +108,7 @@ int run_scenario(std::istream& in) {
   std::unique_ptr<velographx::ConsolidationController> ctl;
   std::unique_ptr<velographx::IncrementalSSSP> sssp;
   std::unique_ptr<velographx::IncrementalKCore> kcore;
+  std::unique_ptr<velographx::IncrementalComponents> cc;
   UpdateBatch batch;
   std::string cmd;
   while (in >> cmd) {
@@ -110,6 +120,7 @@ int run_scenario(std::istream& in) {
       pr.reset();
       sssp.reset();
       kcore.reset();
+      cc.reset();
       g = std::make_unique<DynamicGraph>(n, directed != 0);
     } else if (cmd == "bulk") {
       std::size_t m;
@@ -166,6 +177,12 @@ int run_scenario(std::istream& in) {
       const auto v = pr->validate_against_full(it, tol, l1, linf);
       std::printf("prvalid %.17g %.17g %zu %d %d\n", v.l1_error, v.linf_error, v.reference_iterations,
                   v.reference_converged ? 1 : 0, v.within_tolerance ? 1 : 0);
+    } else if (cmd == "cc_new") {
+      cc = std::make_unique<velographx::IncrementalComponents>(*g);
+      print_cc(*cc, g->vertex_count());
+    } else if (cmd == "cc_apply") {
+      cc->apply(batch);
+      print_cc(*cc, g->vertex_count());
     } else if (cmd == "kcore_new") {
       kcore = std::make_unique<velographx::IncrementalKCore>(*g);
       print_kcore(*kcore);

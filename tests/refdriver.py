@@ -61,6 +61,10 @@ def parse(text):
             recs.append(("prvalid", {"l1_error": float(w[1]), "linf_error": float(w[2]),
                                      "reference_iterations": int(w[3]), "reference_converged": w[4] == "1",
                                      "within_tolerance": w[5] == "1"}))
+        elif tag == "cclabels":
+            recs.append(("cclabels", np.array([int(x) for x in w[2:]], dtype=np.uint32)))
+        elif tag == "ccdiag":
+            recs.append(("ccdiag", int(w[1])))
         elif tag == "kcore":
             recs.append(("kcore", np.array([int(x) for x in w[2:]], dtype=np.uint32)))
         elif tag == "kcorediag":

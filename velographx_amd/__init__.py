@@ -29,6 +29,7 @@ from ._velographx import (  # noqa: F401
     CsrGraph,
     Graph,
     IncrementalBFS,
+    IncrementalComponents,
     IncrementalKCore,
     IncrementalPageRank,
     IncrementalSSSP,
@@ -55,7 +56,7 @@ __all__ = [
     "rmat_edges", "device_count", "version", "MultiSourceBFS", "AdaptivePolicy", "AdaptiveSelector",
     "ConsolidationPolicy", "ConsolidationSignal", "ConsolidationControllerConfig", "ConsolidationController",
     "ConsolidationSnapshot", "evaluate_consolidation", "scale_aware_consolidation_policy",
-    "consolidate_to_csr_snapshot", "CsrGraph", "bfs_distances", "pagerank", "IncrementalSSSP", "IncrementalKCore",
+    "consolidate_to_csr_snapshot", "CsrGraph", "bfs_distances", "pagerank", "IncrementalSSSP", "IncrementalKCore", "IncrementalComponents",
 ]
 
 

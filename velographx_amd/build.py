@@ -21,7 +21,7 @@ BUILD = os.path.join(PKG, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libvgx.so")
 
-HIP_SOURCES = ["graph.hip", "bfs.hip", "pagerank.hip", "msbfs.hip", "ppr.hip", "kcore.hip"]
+HIP_SOURCES = ["graph.hip", "bfs.hip", "pagerank.hip", "msbfs.hip", "ppr.hip", "kcore.hip", "components.hip"]
 HOST_SOURCES = ["rmat.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -15,6 +15,7 @@
 
 #include "velographx_amd/consolidation.hpp"
 #include "velographx_amd/csr_graph.hpp"
+#include "velographx_amd/components.hpp"
 #include "velographx_amd/kcore.hpp"
 #include "velographx_amd/sssp.hpp"
 #include "velographx_amd/pagerank.hpp"
@@ -379,6 +380,38 @@ PYBIND11_MODULE(_velographx, m) {
           },
           py::arg("src"), py::arg("dst"), py::arg("add"))
       .def_property_readonly_static("infinity", [](py::object) { return velographx::IncrementalSSSP::infinity; });
+
+  // incremental/connected_components.hpp (labels: smallest id per component)
+  py::class_<velographx::IncrementalComponents>(m, "IncrementalComponents")
+      .def(py::init<DynamicGraph&>(), py::arg("graph"), py::keep_alive<1, 2>())
+      .def("component", &velographx::IncrementalComponents::component, py::arg("v"))
+      .def("labels_numpy", [](const velographx::IncrementalComponents& x) { return to_numpy(x.labels()); })
+      .def_property_readonly("last_repaired_vertices", &velographx::IncrementalComponents::last_repaired_vertices)
+      .def("apply", &velographx::IncrementalComponents::apply, py::call_guard<py::gil_scoped_release>())
+      .def("recompute", &velographx::IncrementalComponents::recompute, py::call_guard<py::gil_scoped_release>())
+      .def(
+          "apply_arrays",
+          [](velographx::IncrementalComponents& x, const U32Array& src, const U32Array& dst,
+             const py::array_t<std::uint8_t, py::array::c_style | py::array::forcecast>& add) {
+            if (src.size() != dst.size() || src.size() != add.size())
+              throw std::invalid_argument("src, dst and add must have equal length");
+            const auto* s = src.data();
+            const auto* d = dst.data();
+            const auto* a = add.data();
+            const auto k = static_cast<std::size_t>(src.size());
+            py::gil_scoped_release nogil;
+            x.apply_arrays(s, d, a, k);
+          },
+          py::arg("src"), py::arg("dst"), py::arg("add"))
+      .def("stats", [](const velographx::IncrementalComponents& x) {
+        std::uint64_t r = 0;
+        std::uint32_t rounds = 0;
+        velographx::vgx_check(vgx_components_stats(x.handle(), &r, &rounds));
+        py::dict d;
+        d["last_repaired_vertices"] = r;
+        d["rounds"] = rounds;
+        return d;
+      });
 
   // incremental/kcore.hpp (core numbers by device peeling)
   py::class_<velographx::IncrementalKCore>(m, "IncrementalKCore")
