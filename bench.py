@@ -31,8 +31,9 @@ Secondary legs, each in its own object of the same JSON line:
                cutover; the reference engine's snapshot and probe beside it;
 * static    -- the static CsrGraph of the same edges: build, bfs_distances,
                20 pagerank sweeps; the reference's beside it;
-* kcore     -- all core numbers of undirected R-MAT s20 by device peeling;
-               the reference's IncrementalKCore construction beside it;
+* kcore     -- all core numbers and connected components of undirected
+               R-MAT s20 on the device; the reference's IncrementalKCore and
+               IncrementalComponents constructions beside them;
 * repairs   -- 100 update batches of 10,000 ops (80% inserts of random
                non-edges, 20% deletions of live edges) answered with exact
                distances, the adaptive selector choosing localized repair or
@@ -579,9 +580,10 @@ def leg_static(vx, s, d, root):
 
 
 def leg_kcore(vx, args, with_cpu):
-    """Core numbers of undirected R-MAT at the batching scale (s20): the
-    device peeling from the bulk-loaded graph (best of 3), and the
-    reference's IncrementalKCore construction on the same edges."""
+    """Core numbers and connected components of undirected R-MAT at the
+    batching scale (s20): the device peeling and the device labelling of the
+    bulk-loaded graph (best of 3 each), and the reference's IncrementalKCore
+    and IncrementalComponents constructions on the same edges."""
     import torch
 
     scale = args.batch_scale
@@ -598,10 +600,23 @@ def leg_kcore(vx, args, with_cpu):
         t.append(time.perf_counter() - t0)
     core = kc.core_numpy()
     st = kc.stats()
+    del kc
+    cc = vx.IncrementalComponents(g)  # warm-up
+    tc = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cc.recompute()
+        tc.append(time.perf_counter() - t0)
+    labels = cc.labels_numpy()
+    ncomp = int((labels == np.arange(labels.size, dtype=np.uint32)).sum())
+    cc_rounds = int(cc.stats()["rounds"])
+    del cc
     out = {"workload": f"undirected R-MAT s{scale} ef{args.edgefactor} seed 1: all core numbers",
            "arcs": int(g.edge_count_directed), "ms": 1e3 * min(t), "max_core": int(core.max()),
            "levels": int(st["levels"]), "rounds": int(st["rounds"]),
-           "arcs_per_s": g.edge_count_directed / min(t)}
+           "arcs_per_s": g.edge_count_directed / min(t),
+           "components": {"ms": 1e3 * min(tc), "count": ncomp, "rounds": cc_rounds}}
     if with_cpu and os.path.exists(REF_BIN):
         with tempfile.TemporaryDirectory() as td:
             cmd = [REF_BIN, "bench-kcore", _edges_file(td, s, d), str(n)]
@@ -611,7 +626,15 @@ def leg_kcore(vx, args, with_cpu):
                                 "sample": "IncrementalKCore(graph) construction of the reference engine on the "
                                           "same edges"}
         out["max_core_equal"] = int(w[2]) == out["max_core"]
-    del kc, g
+        with tempfile.TemporaryDirectory() as td:
+            cmd = [REF_BIN, "bench-cc", _edges_file(td, s, d), str(n)]
+            res = subprocess.run(cmd, capture_output=True, text=True, timeout=900, check=True).stdout
+        w = [line.split() for line in res.splitlines() if line.startswith("cc ")][0]
+        out["components"]["cpu_reference"] = {"ms": 1e3 * float(w[1]), "count": int(w[2]), "cores": 1,
+                                              "kind": "reference",
+                                              "sample": "IncrementalComponents(graph) construction of the "
+                                                        "reference engine on the same edges"}
+    del g
     log(f"kcore: {out}")
     return out
 

@@ -30,7 +30,7 @@
 //   cons_observe <sr> <lr> <se> <le> <sc> <epoch>   controller.observe
 //   cons_mark <epoch>               controller.mark_consolidated
 // Bench modes: bench-bfs, bench-pr, bench-repair, bench-snapshot,
-// bench-static, bench-kcore (see their comments).
+// bench-static, bench-kcore, bench-cc (see their comments).
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
@@ -362,6 +362,25 @@ int bench_kcore(char** argv) {
   return 0;
 }
 
+// bench-cc <edges.bin> <n>
+// Times IncrementalComponents(graph) on the undirected graph (its
+// constructor unions every arc, one thread).  Prints "cc <seconds> <count>"
+// with the number of components.
+int bench_cc(char** argv) {
+  const auto edges = read_edges_bin(argv[2]);
+  const std::size_t n = std::stoull(argv[3]);
+  DynamicGraph g(n, false);
+  g.bulk_load_edges(edges);
+  const auto t0 = Clock::now();
+  velographx::IncrementalComponents cc(g);
+  const auto t1 = Clock::now();
+  std::size_t roots = 0;
+  for (std::size_t v = 0; v < g.vertex_count(); ++v)
+    if (cc.component(static_cast<VertexId>(v)) == v) ++roots;
+  std::printf("cc %.9f %zu\n", std::chrono::duration<double>(t1 - t0).count(), roots);
+  return 0;
+}
+
 // bench-pr <edges.bin> <n> <directed> <iterations>
 // Times IncrementalPageRank(graph, 0.85) followed by recompute(iterations,
 // 0.0): exactly <iterations> pull sweeps.  Prints
@@ -425,6 +444,7 @@ int main(int argc, char** argv) {
   if (argc == 6 && std::strcmp(argv[1], "bench-snapshot") == 0) return bench_snapshot(argv);
   if (argc == 6 && std::strcmp(argv[1], "bench-static") == 0) return bench_static(argv);
   if (argc == 4 && std::strcmp(argv[1], "bench-kcore") == 0) return bench_kcore(argv);
+  if (argc == 4 && std::strcmp(argv[1], "bench-cc") == 0) return bench_cc(argv);
   if (argc == 8 && std::strcmp(argv[1], "bench-repair") == 0) return bench_repair(argv);
   if (argc >= 3 && std::strcmp(argv[1], "scenario") == 0) {
     std::ifstream f(argv[2]);

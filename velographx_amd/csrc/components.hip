@@ -7,9 +7,10 @@
 // vertex id, so the partition is the reference's and the label is a fixed,
 // order-free representative.
 //
-// Every round: each live arc (u, v) lowers the larger of the two current
-// labels' parent slot to the smaller label (atomicMin), then every vertex
-// jumps to its root.  Parent values only ever decrease and start at the
+// Labels start at the smallest of each vertex and its neighbours (no
+// atomics), jumped to their roots.  Every round: each live arc (u, v) lowers the larger of the two current
+// labels' parent slot to the smaller label (atomicMin, from the row of the
+// endpoint holding the larger label), then every vertex jumps to its root.  Parent values only ever decrease and start at the
 // vertex itself, so parent[v] <= v and the pointers form a forest; a round
 // with no hook ends the loop, and then every vertex of a component points at
 // the component's smallest id (its own parent is itself).
@@ -44,14 +45,29 @@ static unsigned cc_blocks(uint64_t items, unsigned per_block, unsigned cap) {
   return static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(b, cap)));
 }
 
-__global__ void k_cc_init(uint64_t n, uint32_t* __restrict__ parent) {
-  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; v < n;
-       v += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-    parent[v] = static_cast<uint32_t>(v);
+// Start labels without atomics: each vertex points at the smallest of itself
+// and its live neighbours (a vertex of its own component, never above it),
+// one wave per row.
+__global__ __launch_bounds__(256) void k_cc_init(const uint2* __restrict__ row, const uint32_t* __restrict__ arena,
+                                                 uint64_t n, uint32_t* __restrict__ parent) {
+  const uint64_t W = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+  const unsigned lane = lane_id();
+  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += W) {
+    const uint2 rb = row[u];
+    uint32_t m = static_cast<uint32_t>(u);
+    for (uint32_t j = lane; j < rb.y; j += kWave) {
+      const uint32_t x = arena[rb.x + j];
+      if (!(x & kTomb)) m = min(m, x);
+    }
+    m = ~wave_max(~m);  // the wave's minimum
+    if (lane == 0) parent[u] = m;
+  }
 }
 
-// One wave per vertex u: every live arc (u, v) hooks the larger label under
-// the smaller one.
+// One wave per vertex u: a live arc (u, v) whose label is below u's lowers
+// u's label slot to it.  Each undirected arc is stored in both rows, so the
+// row of the endpoint with the larger label does the hook; a plain read
+// first skips the atomic where the slot is already low enough.
 __global__ __launch_bounds__(kCcBlock) void k_cc_hook(const uint2* __restrict__ row, const uint32_t* __restrict__ arena,
                                                       uint64_t n, uint32_t* __restrict__ parent,
                                                       unsigned long long* __restrict__ hooked) {
@@ -65,9 +81,8 @@ __global__ __launch_bounds__(kCcBlock) void k_cc_hook(const uint2* __restrict__ 
       const uint32_t x = arena[rb.x + j];
       if (x & kTomb) continue;
       const uint32_t pv = parent[x];
-      if (pv == pu) continue;
-      const uint32_t hi = max(pu, pv), lo = min(pu, pv);
-      if (atomicMin(&parent[hi], lo) > lo) any = 1;
+      if (pv >= pu || parent[pu] <= pv) continue;
+      if (atomicMin(&parent[pu], pv) > pv) any = 1;
     }
   }
   if (__ballot(any != 0) && lane == 0) atomicAdd(hooked, 1ull);
@@ -122,7 +137,9 @@ static void cc_recompute(vgx_components* cc) {
   cc->hooked.ensure(1);
   if (n == 0) return;
   const Adjacency& a = g->adj[0];
-  k_cc_init<<<cc_blocks(n, kCcBlock, 4096), kCcBlock, 0, s>>>(n, cc->parent.p);
+  k_cc_init<<<cc_blocks(n, kCcBlock / kWave, 2048), kCcBlock, 0, s>>>(a.row.p, a.arena.p, n, cc->parent.p);
+  VGX_LAUNCH_CHECK();
+  k_cc_compress<<<cc_blocks(n, kCcBlock, 2048), kCcBlock, 0, s>>>(n, cc->parent.p);
   VGX_LAUNCH_CHECK();
   // every round either hooks (a label strictly decreases) or ends the loop;
   // labels are bounded below, so the loop ends
