@@ -25,6 +25,7 @@ vgx_status set_error(vgx_status s, const std::string& msg);
 extern thread_local std::string t_last_error;
 
 constexpr int kCcBlock = 256;
+constexpr uint32_t kCcUnroll = 4;
 
 }  // namespace vgx
 
@@ -55,9 +56,17 @@ __global__ __launch_bounds__(256) void k_cc_init(const uint2* __restrict__ row, 
   for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += W) {
     const uint2 rb = row[u];
     uint32_t m = static_cast<uint32_t>(u);
-    for (uint32_t j = lane; j < rb.y; j += kWave) {
-      const uint32_t x = arena[rb.x + j];
-      if (!(x & kTomb)) m = min(m, x);
+    // four loads in flight per lane: hub rows are bound by load latency
+    for (uint32_t j0 = 0; j0 < rb.y; j0 += kCcUnroll * kWave) {
+      uint32_t x[kCcUnroll];
+#pragma unroll
+      for (uint32_t q = 0; q < kCcUnroll; ++q) {
+        const uint32_t j = j0 + q * kWave + lane;
+        x[q] = j < rb.y ? arena[rb.x + j] : kTomb;  // past the row: skipped like a tombstone
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < kCcUnroll; ++q)
+        if (!(x[q] & kTomb)) m = min(m, x[q]);
     }
     m = ~wave_max(~m);  // the wave's minimum
     if (lane == 0) parent[u] = m;
@@ -77,12 +86,19 @@ __global__ __launch_bounds__(kCcBlock) void k_cc_hook(const uint2* __restrict__ 
   for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += W) {
     const uint2 rb = row[u];
     const uint32_t pu = parent[u];
-    for (uint32_t j = lane; j < rb.y; j += kWave) {
-      const uint32_t x = arena[rb.x + j];
-      if (x & kTomb) continue;
-      const uint32_t pv = parent[x];
-      if (pv >= pu || parent[pu] <= pv) continue;
-      if (atomicMin(&parent[pu], pv) > pv) any = 1;
+    for (uint32_t j0 = 0; j0 < rb.y; j0 += kCcUnroll * kWave) {
+      uint32_t pv[kCcUnroll];
+#pragma unroll
+      for (uint32_t q = 0; q < kCcUnroll; ++q) {
+        const uint32_t j = j0 + q * kWave + lane;
+        const uint32_t x = j < rb.y ? arena[rb.x + j] : kTomb;
+        pv[q] = (x & kTomb) ? pu : parent[x];  // past the row or deleted: no hook
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < kCcUnroll; ++q) {
+        if (pv[q] >= pu || parent[pu] <= pv[q]) continue;
+        if (atomicMin(&parent[pu], pv[q]) > pv[q]) any = 1;
+      }
     }
   }
   if (__ballot(any != 0) && lane == 0) atomicAdd(hooked, 1ull);
