@@ -25,6 +25,7 @@ extern thread_local std::string t_last_error;
 
 constexpr uint32_t kKcUnset = 0xFFFFFFFFu;
 constexpr int kKcBlock = 256;
+constexpr uint32_t kKcUnroll = 4;
 
 // Peel rounds run several per host check: round r reads its queue length
 // from ring[r % 3], appends to ring[(r + 1) % 3] and clears ring[(r + 2) % 3]
@@ -136,21 +137,25 @@ __global__ __launch_bounds__(kKcBlock) void k_kc_peel(uint32_t k, uint32_t round
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave; i < count;
        i += W) {
     const uint2 rb = row[q[i]];
-    for (uint32_t j0 = 0; j0 < rb.y; j0 += kWave) {  // uniform across the wave
-      const uint32_t j = j0 + lane;
-      bool claim = false;
-      uint32_t v = 0;
-      if (j < rb.y) {
-        const uint32_t x = arena[rb.x + j];
-        if (!(x & kTomb)) {
-          v = x;
-          if (core[v] == kKcUnset) {
-            const uint32_t before = atomicSub(&deg[v], 1u);
-            if (before - 1u <= k) claim = atomicCAS(&core[v], kKcUnset, k) == kKcUnset;
-          }
-        }
+    // uniform across the wave; four arena loads in flight per lane (hub rows
+    // are bound by load latency)
+    for (uint32_t j0 = 0; j0 < rb.y; j0 += kKcUnroll * kWave) {
+      uint32_t x[kKcUnroll];
+#pragma unroll
+      for (uint32_t t = 0; t < kKcUnroll; ++t) {
+        const uint32_t j = j0 + t * kWave + lane;
+        x[t] = j < rb.y ? arena[rb.x + j] : kTomb;  // past the row: skipped like a tombstone
       }
-      kc_push(claim, v, qnext, next);
+#pragma unroll
+      for (uint32_t t = 0; t < kKcUnroll; ++t) {
+        bool claim = false;
+        const uint32_t v = x[t] & ~kTomb;
+        if (!(x[t] & kTomb) && core[v] == kKcUnset) {
+          const uint32_t before = atomicSub(&deg[v], 1u);
+          if (before - 1u <= k) claim = atomicCAS(&core[v], kKcUnset, k) == kKcUnset;
+        }
+        kc_push(claim, v, qnext, next);
+      }
     }
   }
 }
