@@ -6,15 +6,18 @@ A/B/C/D = .57/.19/.19/.05, seed 1 (4,194,304 vertices, ~64.2M edges after
 loop/duplicate removal).
 
 * step  = one direction-optimizing BFS from one of 16 seeded roots in the
-          largest component (answer-ready: distances in HBM, stream synced);
-          the warm-up recomputes every root twice, after which a root's
-          repeated traversal replays its captured launch program;
+          largest component (answer-ready: distances in HBM, stream synced),
+          the first traversal from that root: one traversal object is
+          re-targeted to the step's root; the warm-up traverses other roots
+          of the component.  No state is kept per root (every launch-shape
+          choice is made on the device from the traversal's own counters),
+          so the figure does not change with --warmup;
 * value = plain single-root BFS GTEPS over the K timed steps (Graph500:
           m_c / t; every root lies in the same component, so the harmonic
           mean is sum(m_c) / sum(t));
 * roofline: algorithmic bytes of one BFS (4*A_c + 12*V_c) over its GPU
   time, measured in the timed region with HIP events on the graph's stream
-  around every traversal (init and level kernels); traffic = HBM bytes per
+  around every traversal (init and step kernels); traffic = HBM bytes per
   BFS from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over a child run, each
   scaled by its reading for a 1 GiB copy;
 * cpu_baseline: the reference engine (oracle/_ref/vgx_ref, compiled from the
@@ -85,8 +88,8 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=16)
-    p.add_argument("--warmup", type=int, default=32,
-                   help="untimed recomputes over the same root objects (two per root capture their launch programs)")
+    p.add_argument("--warmup", type=int, default=4,
+                   help="untimed traversals from other roots of the same component before the timed ones")
     p.add_argument("--scale", type=int, default=22)
     p.add_argument("--edgefactor", type=int, default=16)
     p.add_argument("--roots", type=int, default=16)
@@ -219,14 +222,14 @@ def _counter_rows(directory):
 
 
 def _per_bfs(rows):
-    """Counter sums (KiB) of the kernels of each traversal (init, level,
-    tail and pack kernels), split at k_bfs_init (one per traversal)."""
+    """Counter sums (KiB) of the kernels of each traversal (the init and the
+    step kernels), split at k_bfs_init (one per traversal)."""
     groups = []
     for r in rows:
         name = r["Kernel_Name"]
         if "k_bfs_init" in name:
             groups.append(float(r["Counter_Value"]))
-        elif groups and any(k in name for k in ("k_bfs_level", "k_bfs_bottom_up_tail", "k_bfs_pack")):
+        elif groups and "k_bfs_step" in name:
             groups[-1] += float(r["Counter_Value"])
     return groups
 
@@ -274,7 +277,8 @@ def measure_traffic(args, steps):
 
 def pmc_child(args):
     """Profiled child: a 1 GiB device copy (counter calibration), then the
-    plain BFS of the headline workload, one recompute per root."""
+    plain BFS of the headline workload as the timed region runs it (warm-up
+    roots, then one traversal per root by a re-targeted object)."""
     import torch
 
     import velographx_amd as vx
@@ -291,54 +295,77 @@ def pmc_child(args):
     g.bulk_load_arrays(s, d)
     lcc, _ = component(vx, wl, g)
     roots = wl.roots_in_component(lcc, args.roots, seed=1)
-    objs = [vx.IncrementalBFS(g, int(r)) for r in roots]
-    for k in range(2 * len(objs) + args.steps):
-        objs[k % len(objs)].recompute()
+    warm = warmup_roots(wl, lcc, roots, 4)
+    walker = vx.IncrementalBFS(g, int(warm[0]))
+    for r in warm:
+        walker.set_source(int(r))
+    for k in range(args.steps):
+        walker.set_source(int(roots[k % len(roots)]))
     torch.cuda.synchronize()
 
 
 # ----------------------------------------------------------------- GPU legs
 
 
-def timed_bfs(vx, g, roots, steps, warmup=0, objs=None):
-    """K answer-ready BFS recomputes over the given roots after `warmup`
-    untimed ones on the same objects, with level-kernel HIP-event profiling
-    in a separate pass.  Returns (seconds per step list, elapsed, objects,
-    kernel ms per BFS, launches per BFS)."""
+def timed_bfs(vx, g, roots, steps, warmup, warm_roots):
+    """K answer-ready BFS traversals, step k from roots[k % len(roots)], by
+    one traversal object re-targeted to the step's root (set_source), after
+    `warmup` untimed traversals from other roots (warm_roots).  Nothing is
+    kept per root: the launch program is steered on the device by the
+    traversal's own counters, so the first traversal from a root is the one
+    timed, and a root traversed again runs the same program.  Returns a dict
+    with the wall seconds per step, the elapsed wall time, the GPU ms per
+    traversal (HIP events on the graph's stream around each one), the step
+    launches per traversal and their mean duration (a separate pass with
+    events around every launch), the GPU ms of the same roots traversed a
+    second time in a row, and the traversal object."""
     import torch
 
-    if objs is None:
-        objs = [vx.IncrementalBFS(g, int(r)) for r in roots]
-    for o in objs:
-        o.set_profiling(2)  # GPU time of every whole traversal (replay-compatible)
+    walker = vx.IncrementalBFS(g, int(warm_roots[0]))
     for k in range(warmup):
-        objs[k % len(objs)].recompute()
-    for o in objs:
-        o.set_profiling(2)  # reset the counters: only the timed traversals count
+        walker.set_source(int(warm_roots[k % len(warm_roots)]))
+    walker.set_profiling(2)  # GPU time of every whole traversal; resets the counters
     torch.cuda.synchronize()
     times = []
     t_start = time.perf_counter()
     for k in range(steps):
-        o = objs[k % len(objs)]
+        r = int(roots[k % len(roots)])
         t0 = time.perf_counter()
-        o.recompute()
+        walker.set_source(r)
         times.append(time.perf_counter() - t0)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
-    # GPU time per BFS over the timed region: HIP events on the graph's
-    # stream around each traversal (its init and level kernels)
-    kernel_ms = sum(o.stats()["traversal_ms"] for o in objs) / steps
-    # launches per BFS: one more pass with events around every level launch
-    for o in objs:
-        o.set_profiling(1)
-    for k in range(len(objs)):
-        objs[k].recompute()
+    kernel_ms = walker.stats()["traversal_ms"] / steps
+    # the same roots again, each traversed twice in a row (the second timed):
+    # equal to the first traversal within noise, since nothing is kept
+    walker.set_profiling(0)
+    again = []
+    for r in roots:
+        walker.set_source(int(r))
+        walker.set_profiling(2)
+        walker.recompute()
+        again.append(walker.stats()["traversal_ms"])
+        walker.set_profiling(0)
+    # step launches per traversal and their mean duration
+    walker.set_profiling(1)
+    for r in roots:
+        walker.set_source(int(r))
     torch.cuda.synchronize()
-    launches = sum(o.stats()["level_launches"] for o in objs) / len(objs)
-    level_ms = sum(o.stats()["level_kernel_ms"] for o in objs) / len(objs)
-    for o in objs:
-        o.set_profiling(0)
-    return times, elapsed, objs, kernel_ms, launches, level_ms
+    st = walker.stats()
+    launches = st["level_launches"] / len(roots)
+    launch_ms = st["level_kernel_ms"] / max(st["level_launches"], 1)
+    walker.set_profiling(0)
+    return {"times": times, "elapsed": elapsed, "kernel_ms": kernel_ms, "launches": launches,
+            "launch_ms": launch_ms, "repeat_ms": float(np.mean(again)), "walker": walker}
+
+
+def warmup_roots(wl, lcc, roots, count):
+    """Roots of the same component for the untimed warm-up, none of them a
+    timed root."""
+    cand = wl.roots_in_component(lcc, len(roots) + count + 16, seed=7)
+    timed = set(int(r) for r in roots)
+    out = [int(r) for r in cand if int(r) not in timed]
+    return out[:max(count, 1)]
 
 
 def component(vx, wl, g):
@@ -348,7 +375,7 @@ def component(vx, wl, g):
     return lcc, arcs_c
 
 
-def leg_batched(vx, wl, g, lcc, m_c, k, plain_objs, plain_gteps):
+def leg_batched(vx, wl, g, lcc, m_c, k, plain_dist, plain_gteps):
     import torch
 
     roots = wl.roots_in_component(lcc, k, seed=1)
@@ -363,7 +390,7 @@ def leg_batched(vx, wl, g, lcc, m_c, k, plain_objs, plain_gteps):
         ms.run(g)
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / reps
-    exact = all(np.array_equal(ms.distances(i), o.distances_numpy()) for i, o in enumerate(plain_objs[:k]))
+    exact = all(np.array_equal(ms.distances(i), x) for i, x in enumerate(plain_dist[:k]))
     reached = ms.reached()
     agg = m_c * k / t / 1e9
     out = {"roots": k, "ms_per_pass": 1e3 * t, "aggregate_gteps": agg, "batched_vs_plain": agg / plain_gteps,
@@ -744,15 +771,18 @@ def leg_contrast(vx, wl, steps):
     lcc, arcs_c = component(vx, wl, g)
     roots = wl.roots_in_component(lcc, 16, seed=1)
     log(f"ER n={n}: {g.edge_count_directed} arcs, LCC {lcc.size}, built in {time.time() - t0:.1f}s")
-    times, elapsed, objs, kernel_ms, _, _ = timed_bfs(vx, g, roots, steps, 2 * len(roots))
+    t_bfs = timed_bfs(vx, g, roots, steps, 4, warmup_roots(wl, lcc, roots, 4))
+    times, elapsed, kernel_ms = t_bfs["times"], t_bfs["elapsed"], t_bfs["kernel_ms"]
     bytes_bfs = 4.0 * arcs_c + 12.0 * lcc.size
+    st = t_bfs["walker"].stats()
     out = {"workload": "Erdos-Renyi n=4194304, 33554432 uniform pairs (mean degree 16), seed 1",
            "arcs": int(g.edge_count_directed), "lcc_vertices": int(lcc.size),
            "gteps": arcs_c / 2.0 * steps / sum(times) / 1e9, "ms_per_bfs": 1e3 * elapsed / steps,
+           "gpu_ms_per_bfs": kernel_ms, "bytes_per_bfs": bytes_bfs,
            "roofline_frac": bytes_bfs / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
-           "levels": objs[0].stats()["levels"], "bottom_up_levels": objs[0].stats()["bottom_up_levels"]}
+           "levels": st["levels"], "bottom_up_levels": st["bottom_up_levels"], "steps": st["steps"]}
     log(f"contrast: {out}")
-    del objs, g
+    del t_bfs, g
     return out
 
 
@@ -927,24 +957,30 @@ def main():
     bytes_bfs = 4.0 * arcs_c + 12.0 * v_c
     log(f"LCC: {v_c} vertices, {arcs_c} arcs; bytes/BFS {bytes_bfs / 1e6:.1f} MB")
 
-    times, elapsed, objs, kernel_ms_per_bfs, launches_per_bfs, level_ms_eager = timed_bfs(
-        vx, g, roots, args.steps, args.warmup)
-    st0 = objs[0].stats()
-    reached = {o.stats()["reachable_count"] for o in objs}
+    t_bfs = timed_bfs(vx, g, roots, args.steps, args.warmup, warmup_roots(wl, lcc, roots, args.warmup))
+    times, elapsed, kernel_ms_per_bfs = t_bfs["times"], t_bfs["elapsed"], t_bfs["kernel_ms"]
+    walker = t_bfs["walker"]
+    # every root's distances (outside the timed region): reach, and the
+    # direction-optimizing result against a top-down-only traversal
+    plain_dist = []
+    shapes = []
+    for r in roots:
+        walker.set_source(int(r))
+        plain_dist.append(walker.distances_numpy())
+        st = walker.stats()
+        shapes.append((st["levels"], st["bottom_up_levels"], st["steps"]))
+    reached = {int(np.count_nonzero(x != INF)) for x in plain_dist}
     assert reached == {v_c}, reached
     gteps = m_c * args.steps / sum(times) / 1e9
     ms_per_step = 1e3 * elapsed / args.steps
     achieved = bytes_bfs / (kernel_ms_per_bfs * 1e-3) / 1e9
-    log(f"BFS: {ms_per_step:.3f} ms/step, {gteps:.1f} GTEPS, kernel {kernel_ms_per_bfs:.3f} ms/BFS "
-        f"over {launches_per_bfs:.1f} launches, levels {st0['levels']} (bottom-up {st0['bottom_up_levels']}), "
-        f"graph replays {st0['graph_replays']}")
-    # exactness spot check: the direction-optimizing result equals a
-    # top-down-only traversal of the same root
+    log(f"BFS: {ms_per_step:.3f} ms/step, {gteps:.1f} GTEPS, GPU {kernel_ms_per_bfs:.4f} ms/BFS "
+        f"over {t_bfs['launches']:.1f} step launches; same roots again {t_bfs['repeat_ms']:.4f} ms; "
+        f"levels/bottom-up/steps {shapes}")
     chk = vx.IncrementalBFS(g, int(roots[0]))
-    ref_d = chk.distances_numpy()
-    chk.set_tuning(alpha=1e-9, beta=1.0, blocks=0, levels_per_sync=8)
+    chk.set_tuning(alpha=1e-9, beta=1.0, blocks=0, levels_per_sync=0)
     chk.recompute()
-    exact_bfs = bool(np.array_equal(ref_d, chk.distances_numpy()))
+    exact_bfs = bool(np.array_equal(plain_dist[0], chk.distances_numpy()))
     del chk
 
     result = {
@@ -974,19 +1010,23 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": None,
-            "kernel": "one traversal: k_bfs_init, k_bfs_level, k_bfs_bottom_up_tail, k_bfs_pack",
+            "kernel": "one traversal: k_bfs_init and its k_bfs_step launches (levels, tails, packs)",
             "bytes_per_bfs": bytes_bfs,
             "gpu_ms_per_bfs": kernel_ms_per_bfs,
-            "avg_launch_us": 1e3 * level_ms_eager / max(launches_per_bfs, 1),
-            "launches_per_bfs": launches_per_bfs,
+            "avg_launch_us": 1e3 * t_bfs["launch_ms"],
+            "launches_per_bfs": t_bfs["launches"],
             "wall_frac": bytes_bfs / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
         },
-        "bfs": {"levels": st0["levels"], "bottom_up_levels": st0["bottom_up_levels"],
+        "bfs": {"timed": "first traversal from each root (one object re-targeted per step; nothing is kept per "
+                         "root, the launch program is steered on the device)",
+                "levels": [x[0] for x in shapes], "bottom_up_levels": [x[1] for x in shapes],
+                "steps": [x[2] for x in shapes],
+                "gpu_ms_per_bfs_same_root_again": t_bfs["repeat_ms"],
                 "exact_vs_top_down": exact_bfs},
     }
     if not args.no_batched:
-        result["batched"] = leg_batched(vx, wl, g, lcc, m_c, args.batched_roots, objs, gteps)
-    del objs
+        result["batched"] = leg_batched(vx, wl, g, lcc, m_c, args.batched_roots, plain_dist, gteps)
+    del walker, t_bfs
     if not args.no_consolidation:
         result["consolidation"] = leg_consolidation(vx, wl, g, s, d, n, args)
     if not args.no_static:

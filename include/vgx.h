@@ -146,6 +146,12 @@ vgx_status vgx_bfs_destroy(vgx_bfs* b);
 
 /* BasicIncrementalBFS::recompute -- bfs.hpp:100-120. */
 vgx_status vgx_bfs_recompute(vgx_bfs* b);
+/* A new source for the same object, then recompute (no reference
+ * counterpart: there the source is fixed at construction, bfs.hpp:18-21,
+ * and a new root means a new object).  Reuses the object's device buffers,
+ * so traversals from many roots allocate nothing; like construction, a
+ * source outside the graph gives all-unreachable distances. */
+vgx_status vgx_bfs_set_source(vgx_bfs* b, uint32_t source);
 
 /* BasicIncrementalBFS::apply -- bfs.hpp:29-98: applies the batch to the
  * graph and repairs the distances exactly (localized invalidation and
@@ -182,16 +188,15 @@ typedef struct {
    * (HIP events on the graph's stream) since the last reset. */
   uint64_t level_launches;
   double level_kernel_ms;
-  uint64_t graph_replays;  /* traversals replayed from a captured hipGraph */
+  uint64_t steps;          /* launches the last traversal needed: its levels, tails and packs */
   double traversal_ms;     /* profiling mode 2: GPU time of the traversals since enabled */
   uint64_t traversals;
 } vgx_bfs_stats;
 vgx_status vgx_bfs_stats_get(vgx_bfs* b, vgx_bfs_stats* out);
-/* Time every level-kernel launch with HIP events (on = 1) and reset the
- * accumulated launch count and duration. */
-/* 0: off; 1: HIP events around every level launch (level_kernel_ms; runs
- * eagerly); 2: events around every whole traversal (traversal_ms; keeps the
- * hipGraph replay). */
+/* Profiling, and a reset of the accumulated launch counts and durations:
+ * 0: off; 1: HIP events around every step launch (level_kernel_ms, and
+ * each level's steps summed in vgx_bfs_level_log); 2: events around every
+ * whole traversal (traversal_ms). */
 vgx_status vgx_bfs_set_profiling(vgx_bfs* b, int on);
 /* Per-level record of the last full traversal (diagnostics; no reference
  * counterpart): direction, frontier vertices and out-edges, bottom-up rows
@@ -207,19 +212,27 @@ typedef struct {
 } vgx_bfs_level;
 vgx_status vgx_bfs_level_log(vgx_bfs* b, vgx_bfs_level* out, uint32_t cap, uint32_t* count);
 
-/* Direction-optimizing switch and launch shape (autotuned). */
+/* Direction-optimizing switch and launch shape (autotuned).  Every choice
+ * is made on the device from the traversal's own counters, so the result
+ * and the launch program never depend on an earlier traversal. */
 typedef struct {
   double alpha;          /* top-down -> bottom-up when m_f > m_u / alpha */
-  double beta;           /* bottom-up -> top-down when n_f < n / beta */
-  uint32_t blocks;       /* level-kernel workgroups (0 = auto) */
-  uint32_t levels_per_sync; /* levels enqueued between host checks */
+  double beta;           /* bottom-up -> top-down when n_f < n_u / beta */
+  uint32_t blocks;       /* step-kernel workgroups (0 = auto) */
+  uint32_t levels_per_sync; /* steps enqueued before the first host check (0 = as many as
+                               this graph version's traversals have needed) */
+  double tail_ratio;     /* a bottom-up level leaves rows beyond the head to a tail step
+                            when n_f * tail_ratio < n_u */
+  double claim_ratio;    /* a top-down level claims by distance (pack step after it)
+                            when m_f * claim_ratio >= n_u */
 } vgx_bfs_tuning;
 vgx_status vgx_bfs_set_tuning(vgx_bfs* b, const vgx_bfs_tuning* t);
 vgx_status vgx_bfs_get_tuning(vgx_bfs* b, vgx_bfs_tuning* t);
 /* Autotuner (no reference counterpart): times answer-ready full BFS runs
  * from this object's source, `reps` per candidate, by coordinate descent
  * over alpha {5, 10, 15, 25, 40}, beta {12, 18, 24, 48}, workgroups
- * {auto, 512, 1024, 2048, 4096} and levels per host check {4, 8, 16};
+ * {auto, 512, 1024, 2048}, tail ratio {8, 16, 32, 64, 128} and claim ratio
+ * {2, 3, 5, 8, 12};
  * installs the fastest setting and returns it with its mean time.  The
  * distances afterwards are those of the last run (unchanged: BFS distances
  * do not depend on the tuning). */

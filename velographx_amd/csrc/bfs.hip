@@ -6,24 +6,26 @@
 // bfs_distances in src/algorithms.cpp:11-27.  BFS distances are unique, so
 // every traversal order below yields the reference's vector bit for bit.
 //
-// Full BFS: one kernel launch per level on a fixed grid.  Each workgroup
-// derives the level's direction from the previous level's counters
-// (Beamer's rule: top-down -> bottom-up when m_f > m_u / alpha, bottom-up ->
-// top-down when n_f < n_u / beta, with n_u the vertices with in-arcs still
+// Full BFS: one launch per step on a fixed grid, where a step is a level,
+// the tail (phase B) of a bottom-up level or the pack after a top-down
+// level that claimed by distance.  Each workgroup derives what its step is
+// and the level's direction from what the previous step recorded (Beamer's
+// rule: top-down -> bottom-up when m_f > m_u / alpha, bottom-up -> top-down
+// when n_f < n_u / beta, with n_u the vertices with in-arcs still
 // unvisited, which is what a bottom-up pass checks: vertices without
-// in-arcs start visited); no workgroup waits for another, and the
-// host only checks for completion every few levels.
+// in-arcs start visited); no workgroup waits for another, no decision is
+// taken from an earlier traversal, and the host only checks for completion
+// after the number of steps this graph's traversals have needed.
 //   top-down : the frontier is a packed queue (row begin, edge prefix); every
 //              wave takes an equal slice of the frontier's edges (load-
 //              balanced search), claims new vertices with one atomicOr on
-//              the visited bitmap and pushes them wave-aggregated.  Before a
-//              large bottom-up level no queue is built, and where the level
-//              itself was top-down last time the claim is the distance alone
-//              (k_bfs_pack derives the frontier from the distances).
+//              the visited bitmap and pushes them wave-aggregated.  Before an
+//              expected large bottom-up level no queue is built: the claim
+//              is the distance alone and a pack step derives the frontier.
 //   bottom-up: every wave owns 64 consecutive vertices (one visited word);
 //              unvisited lanes check the first parents from the head array,
-//              then the rest of their in-row (inline or in
-//              k_bfs_bottom_up_tail); one ballot gives the next frontier word.
+//              then the rest of their in-row (inline or in a tail step);
+//              one ballot gives the next frontier word.
 //
 // Incremental repair (apply):
 //   1. deletions are applied first (tombstones), then vertices that lost
@@ -58,29 +60,17 @@ constexpr uint32_t kHead = 4;
 // never in a frontier; distinct from any tombstoned arc, since the graph
 // accepts arc endpoints below 2^31 - 1 only).
 constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
-// A bottom-up level whose previous traversal left at most this many rows
-// (and edges) beyond the head finishes them inside the level kernel instead
-// of a phase-B launch.
-constexpr uint64_t kInlineRows = 2048;
-constexpr uint64_t kInlineEdges = 65536;
-// Top-down levels sized from the previous traversal get about this many
-// 64-edge windows per wave (at least 16 workgroups): each window is a chain
-// of dependent loads, so heavy levels want many waves, light ones few.
-constexpr uint64_t kTdWindowsPerWave = 2;
 // How a top-down level claims new vertices: with an atomic on the visited
-// bitmap and a queue push (kClaimQueue), with the atomic and no queue
-// (kClaimBitmap), or by storing the distance alone, racing stores of the
-// same value, after which k_bfs_pack derives the frontier (kClaimDistance).
-enum Claims : uint32_t { kClaimQueue = 0, kClaimBitmap = 1, kClaimDistance = 2 };
+// bitmap and a queue push (kClaimQueue), or by storing the distance alone,
+// racing stores of the same value, after which a pack step derives the
+// frontier (kClaimDistance).
+enum Claims : uint32_t { kClaimQueue = 0, kClaimDistance = 2 };
 // The count of zero-in-degree bits is added by every workgroup of
 // k_build_zin into one of kZinStripes partial sums, kStripeWords apart (one
 // 128-byte line each): additions to one address serialise.
 constexpr uint32_t kZinStripes = 16;
 constexpr uint32_t kStripeWords = 16;
-// Top-down levels over at most this many frontier arcs get one 64-edge
-// window per wave instead of kTdWindowsPerWave.
-constexpr uint64_t kLightEdges = 65536;
-// Words of the frontier bitmap one wave of k_bfs_pack derives per step.
+// Words of the frontier bitmap one wave of a pack step derives per loop.
 constexpr uint32_t kPackWords = 16;
 
 // Per-level record, kept in a ring of 4 indexed by level & 3.  Level L's
@@ -125,10 +115,47 @@ struct alignas(128) PendCtr {
   unsigned long long q;
 };
 
+// A traversal is a sequence of steps, one launch each: a level, the tail
+// (phase B) of a bottom-up level that left rows pending, or the pack after
+// a top-down level that claimed by distance.  Step i reads what step i - 1
+// did from srec[i & 3] (written by its leader) and the counters it left,
+// and its own leader records step i in srec[(i + 1) & 3], a slot no
+// workgroup of step i reads.
+enum Phase : uint32_t { kPhStart = 0, kPhLevel = 1, kPhTail = 2, kPhPack = 3, kPhDone = 4 };
+struct alignas(128) StepRec {
+  uint32_t phase;
+  uint32_t level;
+  uint32_t mode;         // direction of a level step (0, 1, 2 as in bfs_level)
+  uint32_t claims;       // claim mode of a level step (enum Claims)
+  uint32_t inline_tail;  // a bottom-up level step finished its rows itself
+};
+struct StepPlan {
+  uint32_t phase;
+  uint32_t level;
+};
+// The step after `prev`; `pend_q` is the pending-row counter of prev's level.
+__host__ __device__ inline StepPlan next_step(const StepRec& prev, unsigned long long pend_q) {
+  switch (prev.phase) {
+    case kPhStart:
+      return {kPhLevel, 0u};
+    case kPhLevel:
+      if (prev.mode == 1 && !prev.inline_tail && pend_q != 0) return {kPhTail, prev.level};
+      if (prev.mode != 1 && prev.claims == kClaimDistance) return {kPhPack, prev.level};
+      return {kPhLevel, prev.level + 1};
+    case kPhTail:
+    case kPhPack:
+      return {kPhLevel, prev.level + 1};
+    default:
+      return {kPhDone, prev.level};
+  }
+}
+constexpr uint32_t kLogSteps = 96;
+
 struct BfsState {
   LevelRec rec[4];
   LevelCtl ctl[4];
   PendCtr pend[4];
+  StepRec srec[4];
   unsigned long long reached;
   unsigned long long total_edges;
   uint32_t done_level;  // first level with an empty frontier (~0u while running)
@@ -137,12 +164,27 @@ struct BfsState {
   uint32_t pad;
   float alpha;
   float beta;
+  float tail_ratio;   // bottom-up rows go to a tail step when n_f * tail_ratio < n_u
+  float claim_ratio;  // top-down claims by distance when m_f * claim_ratio >= n_u
+  // phase << 24 | level of every step (steps < kLogSteps), for diagnostics
+  uint32_t log_step[kLogSteps];
   // per-level record for diagnostics (levels < kLogLevels)
   uint32_t log_mode[kLogLevels];
   unsigned long long log_nf[kLogLevels];
   unsigned long long log_mf[kLogLevels];
   unsigned long long log_pend[kLogLevels];  // packed count << 38 | edges left for phase B
 };
+
+__device__ __forceinline__ void write_step(BfsState* st, uint32_t step, uint32_t phase, uint32_t level, uint32_t mode,
+                                           uint32_t claims, uint32_t inline_tail) {
+  StepRec& r = st->srec[(step + 1) & 3];
+  r.phase = phase;
+  r.level = level;
+  r.mode = mode;
+  r.claims = claims;
+  r.inline_tail = inline_tail;
+  if (step < kLogSteps) st->log_step[step] = (phase << 24) | (level & 0xFFFFFFu);
+}
 
 // Counters of the incremental repair.  The lists of the invalidation
 // rounds come in two parities (round r uses r & 1) and the repair frontier
@@ -197,30 +239,19 @@ struct vgx_bfs {
   uint64_t last_candidates = 0, last_affected = 0, last_relaxed = 0;
   bool last_full = false;
   uint32_t levels = 0, bu_levels = 0;
-  vgx_bfs_tuning tune{15.0, 18.0, 0, 8};
+  vgx_bfs_tuning tune{15.0, 18.0, 0, 0, 32.0, 5.0};
   vgx::BfsWork w;
   vgx::BfsState* d_st = nullptr;
   vgx::BfsState* h_st = nullptr;
   vgx::IncState* d_inc = nullptr;
   vgx::IncState* h_inc = nullptr;
   uint32_t epoch = 0;
-  int profiling = 0;  // 1: events around every level, 2: around every traversal
+  int profiling = 0;  // 1: events around every step, 2: around every traversal
   std::vector<hipEvent_t> events;
   uint64_t level_launches = 0;
   double level_ms = 0.0;
-  float last_level_ms[64] = {};  // per launch of the last traversal (profiling)
-  // shape of the last full traversal, used to size the next one's launches
-  uint32_t prof_levels = 0;
-  uint32_t prof_mode[64] = {};
-  uint64_t prof_mf[64] = {};
-  uint64_t prof_pend[64] = {};  // packed rows << kEdgeBits | edges left for phase B
-  uint64_t prof_nf[64] = {};    // frontier vertices
-  // replay of a repeated traversal's launch program
-  bool use_graph = true;
-  std::vector<uint64_t> gkey;
-  hipGraph_t ggraph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  uint64_t graph_replays = 0;
+  float last_level_ms[64] = {};  // per level of the last traversal, its steps summed (profiling)
+  uint32_t steps = 0;            // steps the last traversal needed
   // whole-traversal GPU time (profiling mode 2): events around each traversal
   hipEvent_t tev[2] = {nullptr, nullptr};
   double traversal_ms = 0.0;
@@ -325,28 +356,31 @@ __device__ __forceinline__ bool rest_in_frontier(const LevelArgs& a, const unsig
   return found;
 }
 
-// One BFS level.  `level` is the launch index passed by the host.  With
-// `inline_tail` set (the host predicted a top-down level and launches no
-// tail kernel after it), a bottom-up level finishes its rows itself.
-// `claims` (enum Claims) says how a top-down level claims new vertices.
-// Without a queue (the host expects a large bottom-up level next) there is
-// no row-length load and no queue traffic per new vertex, and where the
-// host launches k_bfs_pack after the level, no atomic either: the pack
-// builds the frontier bitmap from the distances.  The next level then has no
+// One level step.  Its direction follows the rule above from the previous
+// level's counters.  Two launch-shape choices are made here too, by every
+// workgroup alike from the same counters, so that no traversal depends on
+// an earlier one: a top-down level whose frontier arcs are large next to
+// the unvisited vertices (m_f * claim_ratio >= n_u: the next level will
+// most likely run bottom-up over a large frontier) claims by storing the
+// distance alone, with no atomic and no queue, and a pack step derives the
+// next frontier from the distances; a bottom-up level with a frontier small
+// next to the unvisited vertices (n_f * tail_ratio < n_u: many rows will
+// find no parent in their head) leaves the rows beyond the head to a tail
+// step, edge-balanced, and otherwise finishes them itself.  A wrong guess
+// costs time, never results.  Without a queue the next level has no
 // frontier arc count: it runs bottom-up when the frontier is large by the
-// bottom-up rule (n_f * beta >= n_u), else top-down from the frontier bitmap,
-// and counts the frontier's arcs as its size times the mean degree.
-__global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t level, int inline_tail,
-                                                         uint32_t claims) {
+// bottom-up rule (n_f * beta >= n_u), else top-down from the frontier
+// bitmap, and counts the frontier's arcs as its size times the mean degree.
+__device__ __forceinline__ void bfs_level(const LevelArgs& a, uint32_t level, uint32_t step) {
   BfsState* st = a.st;
   const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
-  // Launches enqueued past the last level must not read the ring: its slots
-  // then hold the counters of earlier levels.
-  if (st->done_level <= level) return;
   const unsigned long long qc = st->rec[level & 3].q;
   const unsigned long long nf = found_total(st->rec[level & 3]);
   if (nf == 0) {
-    if (leader) atomicMin(&st->done_level, level);
+    if (leader) {
+      st->done_level = level;
+      write_step(st, step, kPhDone, level, 0u, kClaimQueue, 1u);
+    }
     return;
   }
   // Direction of this level, derived identically by every workgroup.
@@ -374,8 +408,16 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
       mode = static_cast<double>(nf) * st->beta < static_cast<double>(unvisited) ? 2u : 1u;
     }
   }
+  uint32_t claims = kClaimQueue;
+  int inline_tail = 1;
+  if (mode != 1) {
+    if (static_cast<double>(mf) * st->claim_ratio >= static_cast<double>(unvisited)) claims = kClaimDistance;
+  } else {
+    inline_tail = static_cast<double>(nf) * st->tail_ratio >= static_cast<double>(unvisited) ? 1 : 0;
+  }
   LevelRec* rn = &st->rec[(level + 1) & 3];
   if (leader) {
+    write_step(st, step, kPhLevel, level, mode, claims, static_cast<uint32_t>(inline_tail));
     st->ctl[(level + 1) & 3].prev_mode = mode;
     st->ctl[(level + 1) & 3].visited_edges = seen;
     st->ctl[(level + 1) & 3].unvisited = unvisited;
@@ -420,7 +462,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
   // Claim w for the next level if nobody has; queue its row (begin, length).
   // A level that builds no queue claims with the distance alone: every lane
   // that finds w unvisited stores the same value, with no atomic (a
-  // device-scope atomic is a round trip past the L2), and k_bfs_pack then
+  // device-scope atomic is a round trip past the L2), and the pack step then
   // derives the frontier bitmap, the visited bits and the count of new
   // vertices from the distances.
   auto claim = [&](uint32_t w, bool consider) {
@@ -526,13 +568,12 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
         if (!f1) f1 = in_frontier(bm_cur, h1.y);
         if (!f0) f0 = in_frontier(bm_cur, h0.z) | in_frontier(bm_cur, h0.w);
         if (!f1) f1 = in_frontier(bm_cur, h1.z) | in_frontier(bm_cur, h1.w);
-        // the rest of longer rows goes to phase B (k_bfs_bottom_up_tail);
+        // the rest of longer rows goes to phase B (a tail step);
         // a head that ends in kNoEntry is the whole row
         const uint2 r0 = (unv0 && !f0 && h0.w != kNoEntry) ? a.in_row[v0] : make_uint2(0u, 0u);
         const uint2 r1 = (unv1 && !f1 && h1.w != kNoEntry) ? a.in_row[v1] : make_uint2(0u, 0u);
         if (inline_tail) {
-          // recorded like phase B's list, so that the next traversal of this
-          // object can choose between the two
+          // counted like phase B's list, for the level log
           if (r0.y > kHead) pend_local += (1ull << kEdgeBits) + (r0.y - kHead);
           if (r1.y > kHead) pend_local += (1ull << kEdgeBits) + (r1.y - kHead);
           // called by every lane: a long row is scanned by the whole wave
@@ -589,13 +630,12 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_level(LevelArgs a, uint32_t l
   }
 }
 
-// After a top-down level that built no queue: the vertices it claimed are
-// those at distance level + 1.  Waves read the distances (coalesced), set
-// the next frontier bitmap and the visited bits from them and count them for
-// the next level.  Exits at once after any other level.
-__global__ __launch_bounds__(kBfsBlock) void k_bfs_pack(LevelArgs a, uint32_t level) {
+// Pack step, after a top-down level that claimed by distance: the vertices
+// it claimed are those at distance level + 1.  Waves read the distances
+// (coalesced), set the next frontier bitmap and the visited bits from them
+// and count them for the next level.
+__device__ __forceinline__ void bfs_pack(const LevelArgs& a, uint32_t level) {
   BfsState* st = a.st;
-  if (st->done_level <= level || !st->ctl[(level + 1) & 3].no_queue) return;
   const uint32_t words = a.words, next_level = level + 1;
   unsigned long long* bm_next = a.bm + static_cast<uint64_t>((level + 1) % 3) * words;
   const uint32_t W = gridDim.x * kWavesPerBlock;
@@ -635,15 +675,12 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_pack(LevelArgs a, uint32_t le
   }
 }
 
-// Bottom-up phase B of a level: the parents of pending rows beyond the first
-// kHead are scanned edge-balanced and coalesced; a parent in the frontier
-// claims the row's vertex through the visited bitmap.  Exits at once when
-// the level ran top-down or left nothing pending.
-__global__ __launch_bounds__(kBfsBlock) void k_bfs_bottom_up_tail(LevelArgs a, uint32_t level) {
+// Tail step of a bottom-up level that left rows pending (phase B): the
+// parents of pending rows beyond the first kHead are scanned edge-balanced
+// and coalesced; a parent in the frontier claims the row's vertex through
+// the visited bitmap.
+__device__ __forceinline__ void bfs_tail(const LevelArgs& a, uint32_t level) {
   BfsState* st = a.st;
-  if (st->done_level <= level) return;
-  if (found_total(st->rec[level & 3]) == 0) return;
-  if (st->ctl[(level + 1) & 3].prev_mode != 1) return;
   const unsigned long long pq = st->pend[level & 3].q;
   if (blockIdx.x == 0 && threadIdx.x == 0 && level < kLogLevels) st->log_pend[level] = pq;
   if (pq == 0) return;
@@ -690,6 +727,26 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_bottom_up_tail(LevelArgs a, u
     for (uint32_t i = 0; i < kWavesPerBlock; ++i) f += s_found[i];
     if (f) atomicAdd(&rn->found[blockIdx.x % kFoundStripes].v, f);
   }
+}
+
+// One step of a traversal: a level, the tail of a bottom-up level or the
+// pack after a top-down level that claimed by distance.  Which one is
+// decided on the device from what the previous step recorded (next_step),
+// so every launch does work until the traversal is over and the same launch
+// program serves every root; steps launched past the end exit at once.
+__global__ __launch_bounds__(kBfsBlock) void k_bfs_step(LevelArgs a, uint32_t step) {
+  BfsState* st = a.st;
+  const StepRec prev = st->srec[step & 3];
+  const StepPlan p = next_step(prev, st->pend[prev.level & 3].q);
+  if (p.phase == kPhLevel) {
+    bfs_level(a, p.level, step);
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) write_step(st, step, p.phase, p.level, prev.mode, prev.claims, prev.inline_tail);
+  if (p.phase == kPhTail)
+    bfs_tail(a, p.level);
+  else if (p.phase == kPhPack)
+    bfs_pack(a, p.level);
 }
 
 // -------------------------------------------------------- incremental kernels
@@ -1125,7 +1182,8 @@ static LevelArgs level_args(vgx_bfs* b) {
 // the level-0 frontier empty, which ends the traversal at once.
 __global__ void k_bfs_init(LevelArgs a, const unsigned long long* __restrict__ zin,
                            const unsigned long long* __restrict__ zcount, uint32_t root,
-                           unsigned long long total_edges, float alpha, float beta) {
+                           unsigned long long total_edges, float alpha, float beta, float tail_ratio,
+                           float claim_ratio) {
   const uint64_t n = a.n, words = a.words;
   const bool has_root = root < n;
   const uint64_t rw = has_root ? root >> 6 : ~0ull;
@@ -1150,6 +1208,11 @@ __global__ void k_bfs_init(LevelArgs a, const unsigned long long* __restrict__ z
     st->ctl[threadIdx.x].no_queue = 0;
     st->ctl[threadIdx.x].unvisited = 0;
     st->pend[threadIdx.x].q = 0;
+    st->srec[threadIdx.x].phase = kPhStart;
+    st->srec[threadIdx.x].level = 0;
+    st->srec[threadIdx.x].mode = 0;
+    st->srec[threadIdx.x].claims = kClaimQueue;
+    st->srec[threadIdx.x].inline_tail = 1;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -1158,6 +1221,8 @@ __global__ void k_bfs_init(LevelArgs a, const unsigned long long* __restrict__ z
   st->bu_levels = 0;
   st->alpha = alpha;
   st->beta = beta;
+  st->tail_ratio = tail_ratio;
+  st->claim_ratio = claim_ratio;
   st->done_level = ~0u;
   st->reached = 0;
   st->log_pend[0] = 0;
@@ -1176,11 +1241,24 @@ __global__ void k_bfs_init(LevelArgs a, const unsigned long long* __restrict__ z
   }
 }
 
-static void drop_bfs_graph(vgx_bfs* b) {
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  if (b->ggraph) (void)hipGraphDestroy(b->ggraph);
-  b->gexec = nullptr;
-  b->ggraph = nullptr;
+// Whether a traversal whose last launched step left `h` is over, and at
+// which level (the first one with an empty frontier).
+static bool traversal_over(const BfsState& h, uint32_t launched, uint32_t* done_level) {
+  const StepRec& last = h.srec[launched & 3];
+  if (last.phase == kPhDone) {
+    *done_level = h.done_level;
+    return true;
+  }
+  const StepPlan p = next_step(last, h.pend[last.level & 3].q);
+  if (p.phase == kPhLevel) {
+    unsigned long long f = 0;
+    for (uint32_t j = 0; j < kFoundStripes; ++j) f += h.rec[p.level & 3].found[j].v;
+    if (f == 0) {
+      *done_level = p.level;
+      return true;
+    }
+  }
+  return false;
 }
 
 static void bfs_full(vgx_bfs* b) {
@@ -1216,138 +1294,40 @@ static void bfs_full(vgx_bfs* b) {
     uint64_t waves = std::max<uint64_t>(words, g->adj[0].stored / 512);
     blocks = static_cast<unsigned>(std::min<uint64_t>(1536, std::max<uint64_t>(1, (waves + 3) / 4)));
   }
-  // With automatic sizing, a level that ran top-down from a small queue in
-  // the previous traversal of this object gets a grid sized to its edges,
-  // and no tail kernel where that level was not bottom-up (the level kernel
-  // then finishes bottom-up rows inline): dispatching ~1500 idle workgroups
-  // costs microseconds per launch.  Every kernel is grid-stride, so a wrong
-  // guess costs time, never results.
-  const bool adaptive = b->tune.blocks == 0 && b->prof_levels > 0;
-  // tb == 0: no tail kernel, the level kernel finishes bottom-up rows inline.
-  auto grids = [&](uint32_t L, unsigned& lb, unsigned& tb) {
-    lb = blocks;
-    tb = blocks;
-    if (!adaptive) return;
-    if (L >= b->prof_levels) {
-      tb = 0;
-      return;
-    }
-    if (L >= kLogLevels) return;
-    // no tail where the level was not bottom-up or left few rows for phase B
-    if (b->prof_mode[L] != 1 || ((b->prof_pend[L] >> kEdgeBits) <= kInlineRows &&
-                                  (b->prof_pend[L] & kEdgeMask) <= kInlineEdges))
-      tb = 0;
-    if (b->prof_mode[L] == 0) {
-      const uint64_t windows = (b->prof_mf[L] + kWave - 1) / kWave;  // 64-edge windows
-      // a light level is one chain of round trips per window: one window per wave
-      const uint64_t per_wave = b->prof_mf[L] <= kLightEdges ? 1 : kTdWindowsPerWave;
-      const uint64_t per_block = per_wave * kWavesPerBlock;
-      const uint64_t want = (windows + per_block - 1) / per_block;
-      lb = static_cast<unsigned>(std::min<uint64_t>(blocks, std::max<uint64_t>(16, want)));
-    }
-  };
-  // A level builds its queue unless the previous traversal of this object
-  // ran the next level bottom-up on a frontier large enough that the
-  // bottom-up rule alone picks that direction again (so the choice is stable
-  // from one traversal to the next).
-  // Such a level claims by distance where it ran top-down itself, with a
-  // pack launch after it; where it was bottom-up (no claims to pack unless
-  // the direction changes) it keeps the atomic claim and needs no pack.
-  auto claims = [&](uint32_t L) -> uint32_t {
-    if (!adaptive || L + 1 >= b->prof_levels || L + 1 >= kLogLevels) return kClaimQueue;
-    if (!(b->prof_mode[L + 1] == 1 &&
-          static_cast<double>(b->prof_nf[L + 1]) * b->tune.beta >= static_cast<double>(n)))
-      return kClaimQueue;
-    return b->prof_mode[L] == 1 ? kClaimBitmap : kClaimDistance;
-  };
-  // The first chunk covers the depth of the previous traversal; the host
-  // sees completion in the counters of the level after the chunk, so no
-  // launch is spent on detecting it.  Later chunks shrink.
-  uint32_t chunk = b->levels ? b->levels : std::max<uint32_t>(1, b->tune.levels_per_sync);
+  // Steps go out in chunks between host checks.  The first chunk is the
+  // number of steps the traversals of this graph version have needed so far
+  // (any root; the graph keeps the count), so a traversal usually ends
+  // within it: steps launched past the end exit at once, and the host sees
+  // completion in the record of the last step without a launch spent on it.
+  uint32_t chunk = b->tune.levels_per_sync;
+  if (chunk == 0) chunk = g->bfs_steps ? g->bfs_steps : 8u;
   const unsigned long long total_edges = g->adj[0].stored;
   const float alpha = static_cast<float>(b->tune.alpha), beta = static_cast<float>(b->tune.beta);
+  const float tail_ratio = static_cast<float>(b->tune.tail_ratio);
+  const float claim_ratio = static_cast<float>(b->tune.claim_ratio);
   const unsigned ginit = blocks_for(std::max<uint64_t>(n, 1), kBfsBlock, 2048);
-  const unsigned gpack = blocks_for(std::max<uint64_t>(words, 1), kPackWords * kWavesPerBlock, 1024);
-  auto launch_levels = [&](uint32_t first, uint32_t count, bool events) {
-    if (events && b->events.size() < 2 * static_cast<size_t>(count)) {
-      size_t old = b->events.size();
-      b->events.resize(2 * static_cast<size_t>(count));
-      for (size_t i = old; i < b->events.size(); ++i) VGX_HIP(hipEventCreate(&b->events[i]));
-    }
-    for (uint32_t i = 0; i < count; ++i) {
-      if (events) VGX_HIP(hipEventRecord(b->events[2 * i], s));
-      unsigned lb, tb;
-      grids(first + i, lb, tb);
-      const uint32_t cl = claims(first + i);
-      k_bfs_level<<<lb, kBfsBlock, 0, s>>>(a, first + i, tb == 0 ? 1 : 0, cl);
-      VGX_LAUNCH_CHECK();
-      if (tb) {
-        k_bfs_bottom_up_tail<<<tb, kBfsBlock, 0, s>>>(a, first + i);
-        VGX_LAUNCH_CHECK();
-      }
-      if (cl == kClaimDistance) {
-        k_bfs_pack<<<gpack, kBfsBlock, 0, s>>>(a, first + i);
-        VGX_LAUNCH_CHECK();
-      }
-      if (events) VGX_HIP(hipEventRecord(b->events[2 * i + 1], s));
-    }
-  };
-  auto launch_start = [&]() {
-    k_bfs_init<<<ginit, kBfsBlock, 0, s>>>(a, b->w.zin.p, b->w.zcount.p, b->source, total_edges, alpha, beta);
-    VGX_LAUNCH_CHECK();
-  };
-  // A traversal that repeats the previous one's launch program exactly
-  // (same buffers, root, graph size and level shape) replays it from a
-  // hipGraph captured on the second occurrence; anything else runs eagerly.
-  const bool per_level = b->profiling == 1;
+  const bool per_step = b->profiling == 1;
   const bool whole = b->profiling == 2;
   if (whole && !b->tev[0]) {
     VGX_HIP(hipEventCreate(&b->tev[0]));
     VGX_HIP(hipEventCreate(&b->tev[1]));
   }
   if (whole) VGX_HIP(hipEventRecord(b->tev[0], s));
-  bool replayed = false;
-  if (b->use_graph && !per_level) {
-    std::vector<uint64_t> key;
-    key.reserve(sizeof(LevelArgs) / 8 + 8 + 2 * chunk);
-    const uint64_t* ab = reinterpret_cast<const uint64_t*>(&a);
-    static_assert(sizeof(LevelArgs) % 8 == 0, "LevelArgs is hashed as 8-byte words");
-    key.assign(ab, ab + sizeof(LevelArgs) / 8);
-    key.push_back(n);
-    key.push_back(total_edges);
-    key.push_back(b->source);
-    key.push_back(chunk);
-    key.push_back((static_cast<uint64_t>(__builtin_bit_cast(uint32_t, alpha)) << 32) |
-                  __builtin_bit_cast(uint32_t, beta));
-    key.push_back(reinterpret_cast<uint64_t>(b->w.dist.p));
-    key.push_back(reinterpret_cast<uint64_t>(b->w.zin.p));
-    for (uint32_t i = 0; i < chunk; ++i) {
-      unsigned lb, tb;
-      grids(i, lb, tb);
-      key.push_back((static_cast<uint64_t>(lb) << 32) | (static_cast<uint64_t>(claims(i)) << 30) | tb);
-    }
-    if (key == b->gkey) {
-      if (!b->gexec) {
-        VGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        launch_start();
-        launch_levels(0, chunk, false);
-        VGX_HIP(hipStreamEndCapture(s, &b->ggraph));
-        VGX_HIP(hipGraphInstantiate(&b->gexec, b->ggraph, nullptr, nullptr, 0));
-      }
-      VGX_HIP(hipGraphLaunch(b->gexec, s));
-      b->graph_replays += 1;
-      replayed = true;
-    } else {
-      drop_bfs_graph(b);
-      b->gkey = std::move(key);
-    }
-  }
-  uint32_t launched = 0, done_level = ~0u;
-  bool first = true;
+  k_bfs_init<<<ginit, kBfsBlock, 0, s>>>(a, b->w.zin.p, b->w.zcount.p, b->source, total_edges, alpha, beta,
+                                         tail_ratio, claim_ratio);
+  VGX_LAUNCH_CHECK();
+  uint32_t launched = 0, done_level = 0;
   for (;;) {
-    if (!(first && replayed)) {
-      if (first) launch_start();
-      launch_levels(launched, chunk, per_level);
+    if (per_step && b->events.size() < 2 * static_cast<size_t>(chunk)) {
+      const size_t old = b->events.size();
+      b->events.resize(2 * static_cast<size_t>(chunk));
+      for (size_t i = old; i < b->events.size(); ++i) VGX_HIP(hipEventCreate(&b->events[i]));
+    }
+    for (uint32_t i = 0; i < chunk; ++i) {
+      if (per_step) VGX_HIP(hipEventRecord(b->events[2 * i], s));
+      k_bfs_step<<<blocks, kBfsBlock, 0, s>>>(a, launched + i);
+      VGX_LAUNCH_CHECK();
+      if (per_step) VGX_HIP(hipEventRecord(b->events[2 * i + 1], s));
     }
     launched += chunk;
     if (whole) VGX_HIP(hipEventRecord(b->tev[1], s));
@@ -1359,38 +1339,41 @@ static void bfs_full(vgx_bfs* b) {
       b->traversal_ms += ms;  // over all chunks of this traversal
       VGX_HIP(hipEventRecord(b->tev[0], s));
     }
-    if (per_level && !(first && replayed)) {
+    if (per_step) {
       for (uint32_t i = 0; i < chunk; ++i) {
         float ms = 0.f;
         VGX_HIP(hipEventElapsedTime(&ms, b->events[2 * i], b->events[2 * i + 1]));
+        const uint32_t step = launched - chunk + i;
+        const uint32_t rec = step < kLogSteps ? b->h_st->log_step[step] : 0u;
+        // steps past the end (phase done) count as launches of the last level
+        const uint32_t lv = rec & 0xFFFFFFu;
+        if (step < kLogSteps && lv < kLogLevels) b->last_level_ms[lv] += ms;
         b->level_ms += ms;
-        if (launched - chunk + i < kLogLevels) b->last_level_ms[launched - chunk + i] = ms;
       }
       b->level_launches += chunk;
     }
-    first = false;
-    done_level = b->h_st->done_level;
-    if (done_level == ~0u) {
-      // the last launched level found nothing: the traversal is complete
-      unsigned long long f = 0;
-      for (uint32_t j = 0; j < kFoundStripes; ++j) f += b->h_st->rec[launched & 3].found[j].v;
-      if (f == 0) done_level = launched;
-    }
-    if (done_level != ~0u) break;
-    if (launched > n + 2) throw Error(VGX_DEVICE_ERROR, "BFS did not terminate");
+    if (traversal_over(*b->h_st, launched, &done_level)) break;
+    if (launched > 3 * n + 8) throw Error(VGX_DEVICE_ERROR, "BFS did not terminate");
     chunk = std::max<uint32_t>(2, chunk / 2);
   }
   if (whole) b->traversals += 1;
+  // steps this traversal needed: the first one that found the traversal over
+  uint32_t needed = launched;
+  for (uint32_t i = 0; i < launched && i < kLogSteps; ++i)
+    if ((b->h_st->log_step[i] >> 24) == kPhDone) {
+      needed = i;
+      break;
+    }
+  b->steps = needed;
+  if (g->bfs_steps_version != g->version) {
+    g->bfs_steps_version = g->version;
+    g->bfs_steps = needed;
+  } else {
+    g->bfs_steps = std::max(g->bfs_steps, needed);
+  }
   b->reachable = b->h_st->reached;
   b->levels = done_level;
   b->bu_levels = b->h_st->bu_levels;
-  b->prof_levels = b->levels;
-  for (uint32_t L = 0; L < b->levels && L < kLogLevels; ++L) {
-    b->prof_mode[L] = b->h_st->log_mode[L];
-    b->prof_mf[L] = b->h_st->log_mf[L];
-    b->prof_pend[L] = b->h_st->log_pend[L];
-    b->prof_nf[L] = b->h_st->log_nf[L];
-  }
 }
 
 static IncArgs inc_args(vgx_bfs* b) {
@@ -1690,7 +1673,6 @@ vgx_status vgx_bfs_destroy(vgx_bfs* b) {
   if (b->h_st) (void)hipHostFree(b->h_st);
   if (b->h_inc) (void)hipHostFree(b->h_inc);
   for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
-  drop_bfs_graph(b);
   for (hipEvent_t e : b->tev)
     if (e) (void)hipEventDestroy(e);
   delete b;
@@ -1701,6 +1683,15 @@ vgx_status vgx_bfs_recompute(vgx_bfs* b) {
   VGX_API_BEGIN
   if (!b) throw Error(VGX_INVALID_ARGUMENT, "null bfs");
   VGX_HIP(hipSetDevice(b->g->device));
+  bfs_full(b);
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_set_source(vgx_bfs* b, uint32_t source) {
+  VGX_API_BEGIN
+  if (!b) throw Error(VGX_INVALID_ARGUMENT, "null bfs");
+  VGX_HIP(hipSetDevice(b->g->device));
+  b->source = source;
   bfs_full(b);
   VGX_API_END
 }
@@ -1762,7 +1753,7 @@ vgx_status vgx_bfs_stats_get(vgx_bfs* b, vgx_bfs_stats* out) {
   out->last_relaxed_vertices = b->last_relaxed;
   out->level_launches = b->level_launches;
   out->level_kernel_ms = b->level_ms;
-  out->graph_replays = b->graph_replays;
+  out->steps = b->steps;
   out->traversal_ms = b->traversal_ms;
   out->traversals = b->traversals;
   VGX_API_END
@@ -1785,6 +1776,8 @@ vgx_status vgx_bfs_set_tuning(vgx_bfs* b, const vgx_bfs_tuning* t) {
   if (!b || !t) throw Error(VGX_INVALID_ARGUMENT, "null argument");
   if (!(t->alpha > 0) || !(t->beta > 0)) throw Error(VGX_INVALID_ARGUMENT, "alpha and beta must be positive");
   if (t->blocks > 65535) throw Error(VGX_INVALID_ARGUMENT, "too many workgroups");
+  if (!(t->tail_ratio >= 0) || !(t->claim_ratio >= 0))
+    throw Error(VGX_INVALID_ARGUMENT, "tail_ratio and claim_ratio must be non-negative");
   b->tune = *t;
   VGX_API_END
 }
@@ -1805,7 +1798,7 @@ vgx_status vgx_bfs_autotune(vgx_bfs* b, uint32_t reps, vgx_bfs_tuning* best, dou
   b->profiling = 0;
   auto time_of = [&](const vgx_bfs_tuning& t) {
     b->tune = t;
-    bfs_full(b);  // settles the chunk length for this setting
+    bfs_full(b);  // settles the graph's step count for this setting
     const auto t0 = SelClock::now();
     for (uint32_t r = 0; r < reps; ++r) bfs_full(b);
     return us_since(t0) / 1e3 / reps;
@@ -1825,12 +1818,14 @@ vgx_status vgx_bfs_autotune(vgx_bfs* b, uint32_t reps, vgx_bfs_tuning* best, dou
   };
   const double alphas[] = {5.0, 10.0, 15.0, 25.0, 40.0};
   const double betas[] = {12.0, 18.0, 24.0, 48.0};
-  const uint32_t blocks[] = {0u, 512u, 1024u, 2048u, 4096u};
-  const uint32_t lps[] = {4u, 8u, 16u};
+  const uint32_t blocks[] = {0u, 512u, 1024u, 2048u};
+  const double tails[] = {8.0, 16.0, 32.0, 64.0, 128.0};
+  const double claims[] = {2.0, 3.0, 5.0, 8.0, 12.0};
   sweep([](vgx_bfs_tuning& t, double v) { t.alpha = v; }, alphas);
   sweep([](vgx_bfs_tuning& t, double v) { t.beta = v; }, betas);
   sweep([](vgx_bfs_tuning& t, uint32_t v) { t.blocks = v; }, blocks);
-  sweep([](vgx_bfs_tuning& t, uint32_t v) { t.levels_per_sync = v; }, lps);
+  sweep([](vgx_bfs_tuning& t, double v) { t.tail_ratio = v; }, tails);
+  sweep([](vgx_bfs_tuning& t, double v) { t.claim_ratio = v; }, claims);
   b->tune = cur;
   b->profiling = prof;
   if (best) *best = cur;

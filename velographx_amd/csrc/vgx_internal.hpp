@@ -209,6 +209,10 @@ struct vgx_graph {
   vgx::CsrSnapshot snap[2];
   vgx::PprWork ppr;
   vgx::PackScratch pk;
+  // launches (steps) the single-source traversals of this version have
+  // needed at most, any root: the first chunk a new traversal enqueues
+  uint32_t bfs_steps = 0;
+  uint64_t bfs_steps_version = ~0ull;
   int ndir() const { return directed ? 2 : 1; }
   vgx::Adjacency& in() { return directed ? adj[1] : adj[0]; }
 };

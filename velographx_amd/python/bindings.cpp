@@ -626,7 +626,7 @@ PYBIND11_MODULE(_velographx, m) {
              d["last_relaxed_vertices"] = s.last_relaxed_vertices;
              d["level_launches"] = s.level_launches;
              d["level_kernel_ms"] = s.level_kernel_ms;
-             d["graph_replays"] = s.graph_replays;
+             d["steps"] = s.steps;
              d["traversal_ms"] = s.traversal_ms;
              d["traversals"] = s.traversals;
              return d;
@@ -635,11 +635,20 @@ PYBIND11_MODULE(_velographx, m) {
            [](IncrementalBFS& b, int mode) { velographx::vgx_check(vgx_bfs_set_profiling(b.handle(), mode)); },
            py::arg("mode") = 1)
       .def("set_tuning",
-           [](IncrementalBFS& b, double alpha, double beta, std::uint32_t blocks, std::uint32_t levels_per_sync) {
-             vgx_bfs_tuning t{alpha, beta, blocks, levels_per_sync};
+           [](IncrementalBFS& b, double alpha, double beta, std::uint32_t blocks, std::uint32_t levels_per_sync,
+              double tail_ratio, double claim_ratio) {
+             vgx_bfs_tuning t{alpha, beta, blocks, levels_per_sync, tail_ratio, claim_ratio};
              velographx::vgx_check(vgx_bfs_set_tuning(b.handle(), &t));
            },
-           py::arg("alpha") = 15.0, py::arg("beta") = 18.0, py::arg("blocks") = 0, py::arg("levels_per_sync") = 8)
+           py::arg("alpha") = 15.0, py::arg("beta") = 18.0, py::arg("blocks") = 0, py::arg("levels_per_sync") = 0,
+           py::arg("tail_ratio") = 32.0, py::arg("claim_ratio") = 5.0)
+      .def(
+          "set_source",
+          [](IncrementalBFS& b, std::uint32_t source) {
+            py::gil_scoped_release nogil;
+            velographx::vgx_check(vgx_bfs_set_source(b.handle(), source));
+          },
+          py::arg("source"))
       .def("get_tuning",
            [](const IncrementalBFS& b) {
              vgx_bfs_tuning t{};
@@ -649,6 +658,8 @@ PYBIND11_MODULE(_velographx, m) {
              d["beta"] = t.beta;
              d["blocks"] = t.blocks;
              d["levels_per_sync"] = t.levels_per_sync;
+             d["tail_ratio"] = t.tail_ratio;
+             d["claim_ratio"] = t.claim_ratio;
              return d;
            })
       .def("level_log",
@@ -684,6 +695,8 @@ PYBIND11_MODULE(_velographx, m) {
             d["beta"] = t.beta;
             d["blocks"] = t.blocks;
             d["levels_per_sync"] = t.levels_per_sync;
+            d["tail_ratio"] = t.tail_ratio;
+            d["claim_ratio"] = t.claim_ratio;
             d["ms"] = ms;
             return d;
           },
