@@ -33,7 +33,8 @@ def build(force: bool = False) -> str:
     """Compile vgx_oracle.c into oracle/liboracle.so (gcc, host only)."""
     src = os.path.join(HERE, "vgx_oracle.c")
     if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
-        subprocess.check_call(["gcc", "-O2", "-std=c11", "-shared", "-fPIC", src, "-o", LIB_PATH, "-lm"])
+        subprocess.check_call(["gcc", "-O2", "-std=c11", "-fopenmp", "-shared", "-fPIC", src, "-o", LIB_PATH,
+                               "-lm"])
     return LIB_PATH
 
 
@@ -85,6 +86,11 @@ def _setup(L):
     L.orc_pagerank_static.restype = ctypes.c_int
     L.orc_pagerank_static.argtypes = [ctypes.c_uint32, _P, _P, ctypes.c_double, ctypes.c_uint64,
                                       ctypes.c_double, _P]
+    L.orc_csr_build.restype = ctypes.c_uint64
+    L.orc_csr_build.argtypes = [ctypes.c_uint32, ctypes.c_uint64, _P, _P, ctypes.c_int, _P, _P]
+    L.orc_csr_apply.restype = ctypes.c_uint64
+    L.orc_csr_apply.argtypes = [ctypes.c_uint32, _P, _P, ctypes.c_int, ctypes.c_uint64, _P, _P, _P,
+                                ctypes.c_uint32, _P, _P]
 
 
 def _ptr(a):
@@ -176,6 +182,81 @@ class ModelGraph:
         return np.bincount(u, minlength=self.n).astype(np.uint32)
 
 
+class CsrModel:
+    """The same logical semantics as ModelGraph, held as a canonical CSR and
+    updated by the C oracle (orc_csr_build, orc_csr_apply), for the checks at
+    2^20-2^22 vertices where sorting the numpy key array per batch is too
+    slow.  Pinned against ModelGraph in tests/test_oracle.py."""
+
+    def __init__(self, vertices: int = 0, directed: bool = False):
+        self.n = int(vertices)
+        self.directed = bool(directed)
+        self.off = np.zeros(self.n + 1, dtype=np.uint64)
+        self.adj = np.zeros(0, dtype=np.uint32)
+        self.version = 0
+        self._rev = None
+
+    # dynamic_graph.hpp:502-529
+    def bulk_load_edges(self, src, dst):
+        src = _c(src, np.uint32)
+        dst = _c(dst, np.uint32)
+        keep = src != dst
+        if keep.any():
+            self.n = max(self.n, int(max(src[keep].max(), dst[keep].max())) + 1)
+        off = np.empty(self.n + 1, dtype=np.uint64)
+        adj = np.empty(src.size * (1 if self.directed else 2), dtype=np.uint32)
+        m = lib().orc_csr_build(self.n, src.size, _ptr(src), _ptr(dst), int(self.directed), _ptr(off), _ptr(adj))
+        self.off, self.adj = off, adj[:m].copy()
+        self.version += 1
+        self._rev = None
+
+    # dynamic_graph.hpp:543-549, 717-733
+    def apply(self, src, dst, add):
+        src = _c(src, np.uint32)
+        dst = _c(dst, np.uint32)
+        add = _c(add, np.uint8)
+        if src.size == 0:
+            return
+        keep = src != dst
+        n_new = self.n
+        if keep.any():
+            n_new = max(self.n, int(max(src[keep].max(), dst[keep].max())) + 1)
+        off = np.empty(n_new + 1, dtype=np.uint64)
+        adj = np.empty(self.adj.size + 2 * src.size, dtype=np.uint32)
+        m = lib().orc_csr_apply(self.n, _ptr(self.off), _ptr(self.adj), int(self.directed), src.size, _ptr(src),
+                                _ptr(dst), _ptr(add), n_new, _ptr(off), _ptr(adj))
+        self.n = n_new
+        self.off, self.adj = off, adj[:m].copy()
+        self.version += 1
+        self._rev = None
+
+    def edge_count_directed(self):
+        return int(self.adj.size)
+
+    def csr(self, reverse: bool = False):
+        """(off uint64[n+1], adj uint32[m]) with ascending rows."""
+        if not reverse:
+            return self.off, self.adj
+        if not self.directed:
+            return self.off, self.adj
+        if self._rev is None:
+            rows = np.repeat(np.arange(self.n, dtype=np.uint32), np.diff(self.off).astype(np.int64))
+            off = np.empty(self.n + 1, dtype=np.uint64)
+            adj = np.empty(self.adj.size, dtype=np.uint32)
+            lib().orc_csr_build(self.n, self.adj.size, _ptr(self.adj), _ptr(rows), 1, _ptr(off), _ptr(adj))
+            self._rev = (off, adj)
+        return self._rev
+
+    def out_degree(self):
+        return np.diff(self.off).astype(np.uint32)
+
+    def undirected_edges(self):
+        """Each undirected edge once as (u, v) with u < v (undirected graphs)."""
+        rows = np.repeat(np.arange(self.n, dtype=np.uint32), np.diff(self.off).astype(np.int64))
+        keep = rows < self.adj
+        return rows[keep], self.adj[keep]
+
+
 # --------------------------------------------------------------------------
 # algorithm wrappers
 
@@ -189,7 +270,7 @@ def bfs(n, off, adj, source):
     return dist, int(reached)
 
 
-def model_bfs(g: ModelGraph, source):
+def model_bfs(g, source):
     off, adj = g.csr()
     return bfs(g.n, off, adj, source)
 
@@ -218,7 +299,7 @@ def incbfs_apply(pre: ModelGraph, post: ModelGraph, source, fallback, src, dst, 
                                 "used_full_recompute": bool(d.used_full_recompute)}
 
 
-def pagerank_full(g: ModelGraph, damping=0.85, max_iterations=200, tol=1e-12):
+def pagerank_full(g, damping=0.85, max_iterations=200, tol=1e-12):
     io, ia = g.csr(reverse=True)
     od = g.out_degree()
     vals = np.empty(g.n, dtype=np.float64)

@@ -461,6 +461,9 @@ int orc_pagerank_full(uint32_t n, const uint64_t* in_off, const uint32_t* in_adj
     for (uint32_t u = 0; u < n; ++u)
       if (outdeg[u] == 0) dangling_mass += values[u];
     const double dangling_share = damping * dangling_mass / (double)n;
+    /* each next[v] is one sequential sum over v's in-row, so splitting the
+     * vertices over threads gives the same bits as one thread */
+#pragma omp parallel for schedule(dynamic, 4096)
     for (uint32_t v = 0; v < n; ++v) {
       double incoming = 0.0;
       for (uint64_t e = in_off[v]; e < in_off[v + 1]; ++e) {
@@ -688,4 +691,140 @@ int orc_pagerank_static(uint32_t n, const uint64_t* out_off, const uint32_t* out
   }
   free(next);
   return 0;
+}
+
+/* ------------------------------------------------------- canonical CSR */
+
+static int cmp_u32(const void* a, const void* b) {
+  const uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+  return x < y ? -1 : x > y;
+}
+
+static uint64_t sort_unique_row(uint32_t* r, uint64_t len) {
+  if (len < 2) return len;
+  if (len <= 32) {
+    for (uint64_t i = 1; i < len; ++i) {
+      uint32_t x = r[i];
+      uint64_t j = i;
+      while (j > 0 && r[j - 1] > x) {
+        r[j] = r[j - 1];
+        --j;
+      }
+      r[j] = x;
+    }
+  } else {
+    qsort(r, len, sizeof(uint32_t), cmp_u32);
+  }
+  uint64_t w = 1;
+  for (uint64_t i = 1; i < len; ++i)
+    if (r[i] != r[w - 1]) r[w++] = r[i];
+  return w;
+}
+
+uint64_t orc_csr_build(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst, int directed,
+                       uint64_t* off, uint32_t* adj) {
+  /* bulk_load_edges, dynamic_graph.hpp:502-529: loops dropped (:508), both
+   * arcs when undirected (:512); SegmentedCsr::build :79-111: sorted rows,
+   * duplicates removed (:85-86). */
+  memset(off, 0, ((size_t)n + 1) * sizeof(uint64_t));
+  for (uint64_t i = 0; i < m; ++i) {
+    const uint32_t u = src[i], v = dst[i];
+    if (u == v) continue;
+    off[u + 1]++;
+    if (!directed) off[v + 1]++;
+  }
+  for (uint32_t u = 0; u < n; ++u) off[u + 1] += off[u];
+  uint64_t* cur = (uint64_t*)malloc(((size_t)n + 1) * sizeof(uint64_t));
+  memcpy(cur, off, ((size_t)n + 1) * sizeof(uint64_t));
+  for (uint64_t i = 0; i < m; ++i) {
+    const uint32_t u = src[i], v = dst[i];
+    if (u == v) continue;
+    adj[cur[u]++] = v;
+    if (!directed) adj[cur[v]++] = u;
+  }
+  /* cur[u] becomes the de-duplicated row length */
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (uint32_t u = 0; u < n; ++u) cur[u] = sort_unique_row(adj + off[u], off[u + 1] - off[u]);
+  uint64_t w = 0;
+  for (uint32_t u = 0; u < n; ++u) {
+    const uint64_t b = off[u], len = cur[u];
+    off[u] = w;
+    if (w != b) memmove(adj + w, adj + b, len * sizeof(uint32_t));
+    w += len;
+  }
+  off[n] = w;
+  free(cur);
+  return w;
+}
+
+typedef struct {
+  uint64_t key; /* u << 32 | v */
+  uint64_t order;
+  uint8_t add;
+} arc_op;
+
+static int cmp_arc_op(const void* a, const void* b) {
+  const arc_op* x = (const arc_op*)a;
+  const arc_op* y = (const arc_op*)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->order < y->order ? -1 : (x->order > y->order);
+}
+
+uint64_t orc_csr_apply(uint32_t n_old, const uint64_t* off, const uint32_t* adj, int directed, uint64_t k,
+                       const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint32_t n_new,
+                       uint64_t* off_new, uint32_t* adj_new) {
+  /* DynamicGraph::apply, dynamic_graph.hpp:543-549: apply_unversioned per op
+   * in order (:717-724: loops ignored, both arcs when undirected), so for
+   * every arc only its last op decides its final presence. */
+  arc_op* ops = (arc_op*)malloc((2 * k + 1) * sizeof(arc_op));
+  uint64_t no = 0;
+  for (uint64_t i = 0; i < k; ++i) {
+    const uint32_t u = src[i], v = dst[i];
+    if (u == v) continue;
+    ops[no].key = ((uint64_t)u << 32) | v;
+    ops[no].order = i;
+    ops[no].add = add[i] ? 1 : 0;
+    ++no;
+    if (!directed) {
+      ops[no].key = ((uint64_t)v << 32) | u;
+      ops[no].order = i;
+      ops[no].add = add[i] ? 1 : 0;
+      ++no;
+    }
+  }
+  qsort(ops, no, sizeof(arc_op), cmp_arc_op);
+  /* keep the last op of every arc, and only where it changes the arc */
+  uint64_t nf = 0;
+  for (uint64_t i = 0; i < no; ++i) {
+    if (i + 1 < no && ops[i + 1].key == ops[i].key) continue;
+    const uint32_t u = (uint32_t)(ops[i].key >> 32), v = (uint32_t)ops[i].key;
+    const int present = row_contains(off, adj, n_old, u, v);
+    if ((ops[i].add != 0) != (present != 0)) ops[nf++] = ops[i];
+  }
+  uint64_t w = 0, j = 0;
+  for (uint32_t u = 0; u < n_new; ++u) {
+    off_new[u] = w;
+    const uint64_t b = u < n_old ? off[u] : 0, e = u < n_old ? off[u + 1] : 0;
+    if (j >= nf || (uint32_t)(ops[j].key >> 32) != u) {
+      memcpy(adj_new + w, adj + b, (e - b) * sizeof(uint32_t));
+      w += e - b;
+      continue;
+    }
+    /* merge the row with its changes (both sorted by id) */
+    uint64_t x = b;
+    while (j < nf && (uint32_t)(ops[j].key >> 32) == u) {
+      const uint32_t v = (uint32_t)ops[j].key;
+      while (x < e && adj[x] < v) adj_new[w++] = adj[x++];
+      if (ops[j].add) {
+        adj_new[w++] = v;
+      } else {
+        ++x; /* adj[x] == v: the removed arc */
+      }
+      ++j;
+    }
+    while (x < e) adj_new[w++] = adj[x++];
+  }
+  off_new[n_new] = w;
+  free(ops);
+  return w;
 }

@@ -21,6 +21,12 @@
  *                           dangling term; sum-to-one; oracle/_ref.
  *   orc_pagerank_apply      tests/test_pagerank_localized.cpp cases and oracle/_ref.
  *   orc_pagerank_static     algorithms.cpp:56-81 (push form), dense solve.
+ *   orc_csr_build           the ModelGraph of oracle.py (bulk load) on random edge
+ *                           lists with loops, duplicates and both directions.
+ *   orc_csr_apply           ModelGraph.apply on random batch streams (loops, repeated
+ *                           arcs with mixed ops, vertex growth by removals).
+ * The last two, with orc_bfs and orc_pagerank_full, carry the at-size checks
+ * (graphs of 2^20-2^22 vertices), where the numpy model is too slow.
  */
 #ifndef VGX_ORACLE_H
 #define VGX_ORACLE_H
@@ -98,6 +104,25 @@ int orc_pagerank_apply(uint32_t n_prev, uint64_t dangling_before,
 int orc_pagerank_static(uint32_t n, const uint64_t* out_off, const uint32_t* out_adj,
                         double damping, uint64_t max_iterations, double tolerance,
                         double* rank);
+
+/* Canonical CSR of an edge list, the state after DynamicGraph::bulk_load_edges
+ * (dynamic_graph.hpp:502-529) with SegmentedCsr::build's row order (:79-111):
+ * self-loops dropped, both arcs of an edge when undirected, rows sorted and
+ * de-duplicated.  n is the vertex count (the caller makes it at least
+ * max id + 1 over the non-loop edges); off gets n + 1 entries, adj room for
+ * m (directed) or 2m (undirected) arcs.  Returns the arc count. */
+uint64_t orc_csr_build(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst, int directed,
+                       uint64_t* off, uint32_t* adj);
+
+/* DynamicGraph::apply of one batch to a canonical CSR (dynamic_graph.hpp:543-549,
+ * 717-733): ops applied in order (so the last op on an arc wins), self-loops
+ * ignored, undirected ops on both arcs.  n_new is the post-batch vertex count
+ * (at least max id + 1 over the non-loop ops, removals included); off_new gets
+ * n_new + 1 entries, adj_new room for the old arcs plus 2k.  Returns the arc
+ * count. */
+uint64_t orc_csr_apply(uint32_t n_old, const uint64_t* off, const uint32_t* adj, int directed, uint64_t k,
+                       const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint32_t n_new,
+                       uint64_t* off_new, uint32_t* adj_new);
 
 #ifdef __cplusplus
 }

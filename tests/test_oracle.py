@@ -302,3 +302,84 @@ def test_oracle_matches_compiled_reference_pagerank(orc, seed):
         assert diag["last_repaired_vertices"] == pr.diag.last_repaired_vertices
         assert diag["last_repair_iterations"] == pr.diag.last_repair_iterations
         assert diag["last_full_recompute_converged"] == bool(pr.diag.last_full_recompute_converged)
+
+
+# ------------------------------------------------- at-size graph model
+
+
+def _random_ops(rng, n, k, loops=True):
+    s = rng.integers(0, n, k).astype(np.uint32)
+    d = rng.integers(0, n, k).astype(np.uint32)
+    if loops:
+        d[::17] = s[::17]  # self-loops are ignored
+    return s, d
+
+
+@pytest.mark.parametrize("directed", [False, True])
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_csr_model_equals_model_graph(orc, directed, seed):
+    """CsrModel (orc_csr_build + orc_csr_apply) against the numpy ModelGraph:
+    bulk loads with loops and duplicates, then batch streams where the same
+    arc gets several ops in one batch (the last one wins), removals of absent
+    arcs, inserts of present ones, and vertex growth by an insert and by a
+    removal beyond the current vertex count."""
+    rng = np.random.default_rng(seed)
+    n = 300
+    s, d = _random_ops(rng, n, 4000)
+    s = np.concatenate([s, s[:500]])  # duplicates
+    d = np.concatenate([d, d[:500]])
+    mg = orc.ModelGraph(n, directed)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    cm = orc.CsrModel(n, directed)
+    cm.bulk_load_edges(s, d)
+    for rev in (False, True):
+        assert all(np.array_equal(x, y) for x, y in zip(cm.csr(rev), mg.csr(rev)))
+    for b in range(12):
+        k = int(rng.integers(1, 900))
+        bs, bd = _random_ops(rng, n, k)
+        ba = (rng.random(k) < 0.6).astype(np.uint8)
+        # repeat some arcs inside the batch with the opposite op, also mirrored
+        r = rng.integers(0, k, k // 4)
+        bs = np.concatenate([bs, bs[r], bd[r]])
+        bd = np.concatenate([bd, bd[r], bs[r]])
+        ba = np.concatenate([ba, 1 - ba[r], ba[r]]).astype(np.uint8)
+        if b == 4:  # growth by an insert
+            bs = np.append(bs, np.uint32(n + 5))
+            bd = np.append(bd, np.uint32(3))
+            ba = np.append(ba, np.uint8(1))
+        if b == 7:  # growth by a removal of an absent arc
+            bs = np.append(bs, np.uint32(2))
+            bd = np.append(bd, np.uint32(n + 40))
+            ba = np.append(ba, np.uint8(0))
+        mg.apply(bs, bd, ba)
+        cm.apply(bs, bd, ba)
+        assert cm.n == mg.n and cm.edge_count_directed() == mg.edge_count_directed()
+        for rev in (False, True):
+            assert all(np.array_equal(x, y) for x, y in zip(cm.csr(rev), mg.csr(rev))), (b, rev)
+        if not directed:
+            u, v = cm.undirected_edges()
+            assert u.size * 2 == cm.edge_count_directed() and np.all(u < v)
+    dist, _ = orc.model_bfs(cm, 0)
+    ref, _ = orc.model_bfs(mg, 0)
+    assert np.array_equal(dist, ref)
+
+
+def test_pagerank_full_threads_give_the_same_bits(orc):
+    """orc_pagerank_full splits its per-vertex loop over OpenMP threads; every
+    rank is one sequential sum, so one thread and many give identical bits."""
+    import ctypes
+
+    gomp = ctypes.CDLL("libgomp.so.1")
+    rng = np.random.default_rng(5)
+    n = 5000
+    s, d = _random_ops(rng, n, 40000)
+    g = orc.CsrModel(n, True)
+    g.bulk_load_edges(s, d)
+    gomp.omp_set_num_threads(1)
+    try:
+        one, i1 = orc.pagerank_full(g, 0.85, 60, 0.0)
+    finally:
+        gomp.omp_set_num_threads(8)
+    many, i8 = orc.pagerank_full(g, 0.85, 60, 0.0)
+    assert i1["iterations"] == i8["iterations"] == 60
+    assert np.array_equal(one, many)
