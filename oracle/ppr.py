@@ -19,11 +19,13 @@ and, for a loss with dLoss/dX = C,
     dw_uv = (G_uv - sum_v' P_uv' G_uv') / W_u
 
 Nothing in the reference pins these, so every function here is pinned in
-tests/test_oracle.py by what the mathematics fixes: d = 0 gives X = S; a
+tests/test_oracle_ppr.py by what the mathematics fixes: d = 0 gives X = S; a
 directed cycle has a closed form that a transposed P fails; the solve and
 the Jacobi limit agree; the adjoint gradients equal FP64 central finite
-differences of the solve.  Arcs are given as (src, dst, w) arrays; their
-order is the caller's.
+differences of the solve; the C sweeps and the rank-one finite differences
+used at 2^20 vertices equal the numpy sweeps and the dense-solve finite
+differences.  Arcs are given as (src, dst, w) arrays; their order is the
+caller's.
 """
 from __future__ import annotations
 
@@ -109,3 +111,113 @@ def finite_difference_w(n: int, src, dst, w, S, C, d: float, arcs, eps: float = 
         wm[a] -= h
         out[i] = (loss(n, src, dst, wp, S, C, d, dense) - loss(n, src, dst, wm, S, C, d, dense)) / (2 * h)
     return out
+
+
+# ------------------------------------------------------- at 2^20 vertices
+# The numpy sweeps and the LU solves above do not finish in test time at
+# 2^20 vertices and 16M arcs; the C sweeps (orc_ppr_sweeps, OpenMP over rows)
+# do, and the rank-one identity below turns three of their solves into exact
+# perturbed losses.
+
+
+def csr_rows(n: int, keys, idx, p):
+    """Rows of arcs grouped by `keys` (stable within a key): (off, idx, p)."""
+    keys = np.asarray(keys, np.int64)
+    order = np.argsort(keys, kind="stable")
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(np.bincount(keys, minlength=n), out=off[1:])
+    return off, np.ascontiguousarray(np.asarray(idx)[order], np.uint32), np.ascontiguousarray(
+        np.asarray(p, np.float64)[order])
+
+
+def sweeps_c(n: int, rows, B, d: float, iterations: int):
+    """Y_0 = B, then Y = B + d * (row-sum of p * Y[idx]) `iterations` times
+    (orc_ppr_sweeps): the forward over the in-rows with B = (1 - d) S, the
+    adjoint over the out-rows with B = C."""
+    import ctypes
+
+    from . import oracle as orc
+
+    off, idx, p = rows
+    B = np.ascontiguousarray(np.asarray(B, np.float64).reshape(n, -1))
+    K = B.shape[1]
+    Y = np.empty_like(B)
+    L = orc.lib()
+    L.orc_ppr_sweeps.restype = ctypes.c_int
+    L.orc_ppr_sweeps.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_uint32, ctypes.c_void_p, ctypes.c_double, ctypes.c_uint64,
+                                 ctypes.c_void_p]
+    rc = L.orc_ppr_sweeps(n, off.ctypes.data, idx.ctypes.data if idx.size else None,
+                          p.ctypes.data if p.size else None, K, B.ctypes.data, float(d), int(iterations),
+                          Y.ctypes.data)
+    if rc != 0:
+        raise ValueError("at most 64 columns")
+    return Y
+
+
+def forward_c(n: int, src, dst, w, S, d: float, iterations: int):
+    """X after `iterations` sweeps from X_0 = (1 - d) S (same as forward())."""
+    src = np.asarray(src, np.int64)
+    w = np.asarray(w, np.float64)
+    W = np.bincount(src, weights=w, minlength=n)
+    p = w / np.where(W[src] > 0, W[src], 1.0)
+    return sweeps_c(n, csr_rows(n, dst, src, p), (1.0 - d) * np.asarray(S, np.float64).reshape(n, -1), d,
+                    iterations)
+
+
+def finite_difference_w_rank1(n: int, src, dst, w, S, C, d: float, arcs, eps: float = 1e-5,
+                              iterations: int = 150):
+    """FP64 central differences of Loss = sum(C * X) w.r.t. the weights of
+    `arcs`, each perturbed loss evaluated exactly from unperturbed solves.
+
+    A change of w_a, a = (u, v), changes row u of P only: P' = P + e_u r^T,
+    so I - d P'^T = A - d r e_u^T with A = I - d P^T, a rank-one change.  By
+    the Sherman-Morrison identity
+
+        Loss' - Loss = d * sum_j r_j <X_u, L_j> / (1 - d * sum_j r_j y_j)
+
+    where X = (1 - d) A^-1 S (forward), L = A^-T C (adjoint: L = C + d P L),
+    y = A^-T e_u (adjoint with a one-hot right-hand side) and j runs over
+    u's out-neighbours.  Each of Loss(w +- h e_a) - Loss(w) is therefore
+    exact up to the solves' truncation (d^(iterations+1)); the central
+    difference of the two is free of the cancellation of subtracting two
+    whole losses.  Costs three solves (the y of all sampled sources in one
+    multi-column solve) instead of two per arc."""
+    src = np.asarray(src, np.int64)
+    dst = np.asarray(dst, np.int64)
+    w = np.asarray(w, np.float64)
+    S = np.asarray(S, np.float64).reshape(n, -1)
+    C = np.asarray(C, np.float64).reshape(n, -1)
+    W = np.bincount(src, weights=w, minlength=n)
+    p = w / np.where(W[src] > 0, W[src], 1.0)
+    out_rows = csr_rows(n, src, dst, p)
+    X = sweeps_c(n, csr_rows(n, dst, src, p), (1.0 - d) * S, d, iterations)
+    L = sweeps_c(n, out_rows, C, d, iterations)
+    arcs = np.asarray(arcs, np.int64)
+    us = np.unique(src[arcs])
+    if us.size > 64:
+        raise ValueError("at most 64 distinct sources")
+    E = np.zeros((n, us.size))
+    E[us, np.arange(us.size)] = 1.0
+    Y = sweeps_c(n, out_rows, E, d, iterations)
+    col = {int(u): i for i, u in enumerate(us)}
+    # arcs of every row in arc order (a row's arcs are contiguous in out-CSR order)
+    order = np.argsort(src, kind="stable")
+    starts = np.searchsorted(src[order], np.arange(n + 1))
+    out = np.empty(arcs.size)
+    for i, a in enumerate(arcs):
+        u = int(src[a])
+        row = order[starts[u]:starts[u + 1]]
+        wr = w[row]
+        Wu = wr.sum()
+        xl = L[dst[row]] @ X[u]          # <X_u, L_j> for every j in the row
+        yj = Y[dst[row], col[u]]
+        h = eps * max(1.0, abs(w[a]))
+        deltas = []
+        for sgn in (1.0, -1.0):
+            wp = wr.copy()
+            wp[row == a] += sgn * h
+            r = wp / (Wu + sgn * h) - wr / Wu
+            deltas.append(d * float(r @ xl) / (1.0 - d * float(r @ yj)))
+        out[i] = (deltas[0] - deltas[1]) / (2.0 * h)
+    return out, X, L

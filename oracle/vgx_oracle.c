@@ -828,3 +828,34 @@ uint64_t orc_csr_apply(uint32_t n_old, const uint64_t* off, const uint32_t* adj,
   free(ops);
   return w;
 }
+
+/* ----------------------------------------------- personalized PageRank */
+
+int orc_ppr_sweeps(uint32_t n, const uint64_t* off, const uint32_t* idx, const double* p, uint32_t K,
+                   const double* B, double d, uint64_t iterations, double* Y) {
+  if (K == 0 || K > 64) return -1;
+  const size_t cells = (size_t)n * K;
+  double* T = (double*)malloc(cells * sizeof(double) + 1);
+  double* cur = Y;
+  double* nxt = T;
+  memcpy(cur, B, cells * sizeof(double));
+  for (uint64_t it = 0; it < iterations; ++it) {
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (uint32_t v = 0; v < n; ++v) {
+      double acc[64];
+      for (uint32_t k = 0; k < K; ++k) acc[k] = 0.0;
+      for (uint64_t e = off[v]; e < off[v + 1]; ++e) {
+        const double* y = cur + (size_t)idx[e] * K;
+        const double pe = p[e];
+        for (uint32_t k = 0; k < K; ++k) acc[k] += pe * y[k];
+      }
+      for (uint32_t k = 0; k < K; ++k) nxt[(size_t)v * K + k] = B[(size_t)v * K + k] + d * acc[k];
+    }
+    double* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  if (cur != Y) memcpy(Y, cur, cells * sizeof(double));
+  free(T);
+  return 0;
+}

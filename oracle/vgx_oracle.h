@@ -25,6 +25,7 @@
  *                           lists with loops, duplicates and both directions.
  *   orc_csr_apply           ModelGraph.apply on random batch streams (loops, repeated
  *                           arcs with mixed ops, vertex growth by removals).
+ *   orc_ppr_sweeps          oracle/ppr.py's Jacobi sweeps (forward and adjoint).
  * The last two, with orc_bfs and orc_pagerank_full, carry the at-size checks
  * (graphs of 2^20-2^22 vertices), where the numpy model is too slow.
  */
@@ -123,6 +124,15 @@ uint64_t orc_csr_build(uint32_t n, uint64_t m, const uint32_t* src, const uint32
 uint64_t orc_csr_apply(uint32_t n_old, const uint64_t* off, const uint32_t* adj, int directed, uint64_t k,
                        const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint32_t n_new,
                        uint64_t* off_new, uint32_t* adj_new);
+
+/* Weighted personalized PageRank sweeps (no reference counterpart; the
+ * mathematics of oracle/ppr.py): Y_0 = B, then `iterations` times
+ * Y'[v] = B[v] + d * sum_{e in row v} p[e] * Y[idx[e]], K <= 64 columns,
+ * row-major n x K.  Over the in-rows with B = (1 - d) S this is the forward
+ * X = (1 - d) S + d P^T X; over the out-rows with B = C the adjoint
+ * L = C + d P L.  Pinned against oracle/ppr.py's numpy sweeps. */
+int orc_ppr_sweeps(uint32_t n, const uint64_t* off, const uint32_t* idx, const double* p, uint32_t K,
+                   const double* B, double d, uint64_t iterations, double* Y);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,12 @@ CPU oracle on the host (nothing here compares the GPU with itself):
   launch program) after each of 4 uneven update batches;
 * directed R-MAT scale 22: the FP32 PageRank solve to an L1 residual below
   1e-7, and its warm-started re-solve after 3 batches of 10,000 ops, within
-  1e-6 (L1 and max-abs) of the oracle's FP64 solve.
+  1e-6 (L1 and max-abs) of the oracle's FP64 solve;
+* directed weighted R-MAT scale 20, 32 one-hot teleport columns: the
+  forward and the teleport gradient against the oracle's FP64 sweeps, and
+  dLoss/dw on 200 sampled arcs against FP64 central differences (each
+  perturbed loss evaluated exactly by the rank-one identity of
+  oracle/ppr.finite_difference_w_rank1).
 
 The oracle graph is oracle.CsrModel (pinned against ModelGraph in
 tests/test_oracle.py); BFS is orc_bfs, PageRank orc_pagerank_full.
@@ -205,3 +210,47 @@ def test_config4_s22_directed_fp32_pagerank(vx, wl, orc, rmat22):
         model.apply(bs, bd, ba)
         pr.solve_l1(200, 1e-7, True)
         check(f"batch {b}")
+
+
+def test_config5_s20_ppr_fd_200_arcs(vx):
+    import torch
+
+    from oracle import ppr
+    from velographx_amd import autograd as ag
+
+    n = 1 << 20
+    K = 32
+    D = 0.85
+    T = 150  # sweeps: truncation d^151 < 3e-11 of each quantity
+    s, d = vx.rmat_edges(20, 16, 1)
+    g = vx.Graph(n, True)
+    g.bulk_load_arrays(s, d)
+    src, dst = (a.astype(np.int64) for a in ag.arc_list(g))
+    rng = np.random.default_rng(1)
+    w = (1.0 - rng.random(src.size)).astype(np.float32)  # uniform on (0, 1]
+    S = np.zeros((n, K), np.float32)
+    S[rng.integers(0, n, K), np.arange(K)] = 1.0
+    C = rng.standard_normal((n, K)).astype(np.float32)
+    wt = torch.tensor(w, device="cuda", requires_grad=True)
+    st = torch.tensor(S, device="cuda", requires_grad=True)
+    x = ag.personalized_pagerank(g, wt, st, D, T)
+    (x * torch.tensor(C, device="cuda")).sum().backward()
+    torch.cuda.synchronize()
+    x, gw, gs = x.detach().cpu().numpy(), wt.grad.cpu().numpy(), st.grad.cpu().numpy()
+    # 200 arcs: 25 from each of 8 sampled sources with at least 25 out-arcs
+    outdeg = np.bincount(src, minlength=n)
+    cand = np.nonzero(outdeg >= 25)[0]
+    sources = np.sort(rng.choice(cand, 8, replace=False))
+    starts = np.searchsorted(src, sources)
+    arcs = np.concatenate([st0 + np.sort(rng.choice(outdeg[u], 25, replace=False))
+                           for u, st0 in zip(sources, starts)])
+    assert arcs.size == 200 and np.all(src[arcs] == np.repeat(sources, 25))
+    fd, X, L = ppr.finite_difference_w_rank1(n, src, dst, w.astype(np.float64), S, C, D, arcs, eps=1e-5,
+                                             iterations=T)
+    ex = float(np.max(np.abs(x - X)))
+    es = float(np.max(np.abs(gs - (1 - D) * L)) / np.max(np.abs((1 - D) * L)))
+    ew = float(np.max(np.abs(gw[arcs] - fd)) / np.max(np.abs(fd)))
+    log(f"PPR s20: X max-abs {ex:.2e}, dS rel {es:.2e}, dw vs FD rel {ew:.2e} on {arcs.size} arcs")
+    assert ex <= 1e-6
+    assert es <= 1e-5
+    assert ew <= 1e-5

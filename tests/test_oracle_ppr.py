@@ -111,3 +111,58 @@ def test_gradient_planted_mistake_is_caught():
     slipped = 0.85 * np.einsum("ek,ek->e", X[src], L[dst]) / W[src]
     fd = ppr.finite_difference_w(n, src, dst, w, S, C, 0.85, np.arange(src.size))
     assert not np.allclose(slipped, fd, rtol=1e-3, atol=1e-6)
+
+
+D = 0.85
+
+
+def _small_problem(seed, n=200, m=1500, K=4):
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    keep = src != dst
+    key = np.unique(src[keep] * n + dst[keep])
+    src, dst = key // n, key % n  # out-CSR order: by source, then destination
+    w = 1.0 - rng.random(src.size)
+    S = np.zeros((n, K))
+    S[rng.integers(0, n, K), np.arange(K)] = 1.0
+    C = rng.standard_normal((n, K))
+    return n, src, dst, w, S, C, rng
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_c_sweeps_equal_numpy_sweeps(seed):
+    """orc_ppr_sweeps over the in-rows and the out-rows gives the forward and
+    adjoint Jacobi sweeps of the numpy oracle."""
+    n, src, dst, w, S, C, _ = _small_problem(seed)
+    for it in (0, 1, 7, 60):
+        assert np.allclose(ppr.forward_c(n, src, dst, w, S, D, it), ppr.forward(n, src, dst, w, S, D, it),
+                           rtol=0, atol=1e-15)
+        W = np.bincount(src, weights=w, minlength=n)
+        p = w / W[src]
+        L = ppr.sweeps_c(n, ppr.csr_rows(n, src, dst, p), C, D, it)
+        assert np.allclose(L, ppr.adjoint(n, src, dst, w, C, D, it), rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_rank_one_finite_differences_equal_dense_ones(seed):
+    """The rank-one evaluation of the perturbed losses gives the central
+    differences of the exact dense solve (to the latter's cancellation
+    error), and both approach the exact adjoint gradient; the same
+    differences on the transposed graph do not."""
+    n, src, dst, w, S, C, rng = _small_problem(seed)
+    arcs = rng.choice(src.size, 40, replace=False)
+    dense = ppr.finite_difference_w(n, src, dst, w, S, C, D, arcs, eps=1e-6)
+    r1, X, L = ppr.finite_difference_w_rank1(n, src, dst, w, S, C, D, arcs, eps=1e-5, iterations=300)
+    scale = np.max(np.abs(dense))
+    assert np.max(np.abs(r1 - dense)) <= 1e-6 * scale
+    _, gw, gs = ppr.gradients(n, src, dst, w, S, C, D, dense=True)
+    assert np.max(np.abs(gw[arcs] - r1)) <= 1e-8 * np.max(np.abs(gw[arcs]))
+    assert np.max(np.abs(X - ppr.forward(n, src, dst, w, S, D, None, dense=True))) <= 1e-12
+    assert np.max(np.abs((1 - D) * L - gs)) <= 1e-12
+    # planted mistake: the transposed graph's differences are different numbers
+    key = np.argsort(dst * n + src)
+    t_src, t_dst, t_w = dst[key], src[key], w[key]
+    t_arcs = np.argsort(key)[arcs]
+    wrong, _, _ = ppr.finite_difference_w_rank1(n, t_src, t_dst, t_w, S, C, D, t_arcs, eps=1e-5, iterations=300)
+    assert np.max(np.abs(wrong - dense)) > 1e-3 * scale
