@@ -1261,33 +1261,41 @@ static bool traversal_over(const BfsState& h, uint32_t launched, uint32_t* done_
   return false;
 }
 
-static void bfs_full(vgx_bfs* b) {
-  vgx_graph* g = b->g;
-  ensure_work(b);
-  std::fill(std::begin(b->last_level_ms), std::end(b->last_level_ms), 0.f);
+// The dense head array and the zero-in-degree bitmap of the graph's
+// current in-rows (zcount is cleared by k_build_head, then summed by
+// k_build_zin on the same stream).
+static void build_head_zin(vgx_graph* g, uint4* head, unsigned long long* zin, unsigned long long* zcount,
+                           hipStream_t s) {
+  const uint64_t n = g->n, words = (n + 63) / 64;
+  if (!n) return;
+  // one vertex per thread: each is a chain of two dependent loads
+  k_build_head<<<blocks_for(n, kBfsBlock, 65535), kBfsBlock, 0, s>>>(g->in().row.p, g->in().arena.p,
+                                                                     static_cast<uint32_t>(n), head, zcount);
+  VGX_LAUNCH_CHECK();
+  k_build_zin<<<blocks_for(words, kWavesPerBlock, 1024), kBfsBlock, 0, s>>>(
+      g->in().deg.p, static_cast<uint32_t>(n), static_cast<uint32_t>(words), zin, zcount);
+  VGX_LAUNCH_CHECK();
+}
+
+struct TraversalResult {
+  uint32_t done_level = 0;
+  uint32_t steps = 0;
+  uint32_t bu_levels = 0;
+  uint64_t reached = 0;
+};
+
+// One traversal from `source` over the buffers of `a` (distances, bitmaps,
+// queues, state), with head/zin/zcount already built for the graph's
+// current version: the init kernel and the step program, in chunks between
+// host checks of the state (copied to h_st).  `prof`, when not null, is the
+// traversal object whose profiling mode asks for HIP events.
+static TraversalResult run_traversal(vgx_graph* g, const LevelArgs& a, const unsigned long long* zin,
+                                     const unsigned long long* zcount, uint32_t source,
+                                     const vgx_bfs_tuning& tune, BfsState* h_st, vgx_bfs* prof) {
   hipStream_t s = g->stream;
   const uint64_t n = g->n;
-  b->dist_len = n;
   const uint64_t words = (n + 63) / 64;
-  // the dense head array and the zero-in-degree bitmap follow the graph's version
-  if (b->w.head_version != g->version || b->w.head_n != n) {
-    b->w.head.ensure(g->ncap + 1);
-    b->w.zin.ensure(words + 1);
-    b->w.zcount.ensure(kZinStripes * kStripeWords);
-    if (n) {
-      // one vertex per thread: each is a chain of two dependent loads
-      k_build_head<<<blocks_for(n, kBfsBlock, 65535), kBfsBlock, 0, s>>>(
-          g->in().row.p, g->in().arena.p, static_cast<uint32_t>(n), b->w.head.p, b->w.zcount.p);
-      VGX_LAUNCH_CHECK();
-      k_build_zin<<<blocks_for(words, kWavesPerBlock, 1024), kBfsBlock, 0, s>>>(
-          g->in().deg.p, static_cast<uint32_t>(n), static_cast<uint32_t>(words), b->w.zin.p, b->w.zcount.p);
-      VGX_LAUNCH_CHECK();
-    }
-    b->w.head_version = g->version;
-    b->w.head_n = n;
-  }
-  LevelArgs a = level_args(b);
-  unsigned blocks = b->tune.blocks;
+  unsigned blocks = tune.blocks;
   if (blocks == 0) {
     // about one resident round of workgroups: more waves only add flushes
     // on the level counters (measured at R-MAT scale 22)
@@ -1299,81 +1307,106 @@ static void bfs_full(vgx_bfs* b) {
   // (any root; the graph keeps the count), so a traversal usually ends
   // within it: steps launched past the end exit at once, and the host sees
   // completion in the record of the last step without a launch spent on it.
-  uint32_t chunk = b->tune.levels_per_sync;
+  uint32_t chunk = tune.levels_per_sync;
   if (chunk == 0) chunk = g->bfs_steps ? g->bfs_steps : 8u;
   const unsigned long long total_edges = g->adj[0].stored;
-  const float alpha = static_cast<float>(b->tune.alpha), beta = static_cast<float>(b->tune.beta);
-  const float tail_ratio = static_cast<float>(b->tune.tail_ratio);
-  const float claim_ratio = static_cast<float>(b->tune.claim_ratio);
+  const float alpha = static_cast<float>(tune.alpha), beta = static_cast<float>(tune.beta);
+  const float tail_ratio = static_cast<float>(tune.tail_ratio);
+  const float claim_ratio = static_cast<float>(tune.claim_ratio);
   const unsigned ginit = blocks_for(std::max<uint64_t>(n, 1), kBfsBlock, 2048);
-  const bool per_step = b->profiling == 1;
-  const bool whole = b->profiling == 2;
-  if (whole && !b->tev[0]) {
-    VGX_HIP(hipEventCreate(&b->tev[0]));
-    VGX_HIP(hipEventCreate(&b->tev[1]));
+  const bool per_step = prof && prof->profiling == 1;
+  const bool whole = prof && prof->profiling == 2;
+  if (whole && !prof->tev[0]) {
+    VGX_HIP(hipEventCreate(&prof->tev[0]));
+    VGX_HIP(hipEventCreate(&prof->tev[1]));
   }
-  if (whole) VGX_HIP(hipEventRecord(b->tev[0], s));
-  k_bfs_init<<<ginit, kBfsBlock, 0, s>>>(a, b->w.zin.p, b->w.zcount.p, b->source, total_edges, alpha, beta,
-                                         tail_ratio, claim_ratio);
+  if (whole) VGX_HIP(hipEventRecord(prof->tev[0], s));
+  k_bfs_init<<<ginit, kBfsBlock, 0, s>>>(a, zin, zcount, source, total_edges, alpha, beta, tail_ratio, claim_ratio);
   VGX_LAUNCH_CHECK();
   uint32_t launched = 0, done_level = 0;
   for (;;) {
-    if (per_step && b->events.size() < 2 * static_cast<size_t>(chunk)) {
-      const size_t old = b->events.size();
-      b->events.resize(2 * static_cast<size_t>(chunk));
-      for (size_t i = old; i < b->events.size(); ++i) VGX_HIP(hipEventCreate(&b->events[i]));
+    if (per_step && prof->events.size() < 2 * static_cast<size_t>(chunk)) {
+      const size_t old = prof->events.size();
+      prof->events.resize(2 * static_cast<size_t>(chunk));
+      for (size_t i = old; i < prof->events.size(); ++i) VGX_HIP(hipEventCreate(&prof->events[i]));
     }
     for (uint32_t i = 0; i < chunk; ++i) {
-      if (per_step) VGX_HIP(hipEventRecord(b->events[2 * i], s));
+      if (per_step) VGX_HIP(hipEventRecord(prof->events[2 * i], s));
       k_bfs_step<<<blocks, kBfsBlock, 0, s>>>(a, launched + i);
       VGX_LAUNCH_CHECK();
-      if (per_step) VGX_HIP(hipEventRecord(b->events[2 * i + 1], s));
+      if (per_step) VGX_HIP(hipEventRecord(prof->events[2 * i + 1], s));
     }
     launched += chunk;
-    if (whole) VGX_HIP(hipEventRecord(b->tev[1], s));
-    VGX_HIP(hipMemcpyAsync(b->h_st, b->d_st, sizeof(BfsState), hipMemcpyDeviceToHost, s));
+    if (whole) VGX_HIP(hipEventRecord(prof->tev[1], s));
+    VGX_HIP(hipMemcpyAsync(h_st, a.st, sizeof(BfsState), hipMemcpyDeviceToHost, s));
     VGX_HIP(hipStreamSynchronize(s));
     if (whole) {
       float ms = 0.f;
-      VGX_HIP(hipEventElapsedTime(&ms, b->tev[0], b->tev[1]));
-      b->traversal_ms += ms;  // over all chunks of this traversal
-      VGX_HIP(hipEventRecord(b->tev[0], s));
+      VGX_HIP(hipEventElapsedTime(&ms, prof->tev[0], prof->tev[1]));
+      prof->traversal_ms += ms;  // over all chunks of this traversal
+      VGX_HIP(hipEventRecord(prof->tev[0], s));
     }
     if (per_step) {
       for (uint32_t i = 0; i < chunk; ++i) {
         float ms = 0.f;
-        VGX_HIP(hipEventElapsedTime(&ms, b->events[2 * i], b->events[2 * i + 1]));
+        VGX_HIP(hipEventElapsedTime(&ms, prof->events[2 * i], prof->events[2 * i + 1]));
         const uint32_t step = launched - chunk + i;
-        const uint32_t rec = step < kLogSteps ? b->h_st->log_step[step] : 0u;
+        const uint32_t rec = step < kLogSteps ? h_st->log_step[step] : 0u;
         // steps past the end (phase done) count as launches of the last level
         const uint32_t lv = rec & 0xFFFFFFu;
-        if (step < kLogSteps && lv < kLogLevels) b->last_level_ms[lv] += ms;
-        b->level_ms += ms;
+        if (step < kLogSteps && lv < kLogLevels) prof->last_level_ms[lv] += ms;
+        prof->level_ms += ms;
       }
-      b->level_launches += chunk;
+      prof->level_launches += chunk;
     }
-    if (traversal_over(*b->h_st, launched, &done_level)) break;
+    if (traversal_over(*h_st, launched, &done_level)) break;
     if (launched > 3 * n + 8) throw Error(VGX_DEVICE_ERROR, "BFS did not terminate");
     chunk = std::max<uint32_t>(2, chunk / 2);
   }
-  if (whole) b->traversals += 1;
+  if (whole) prof->traversals += 1;
   // steps this traversal needed: the first one that found the traversal over
   uint32_t needed = launched;
   for (uint32_t i = 0; i < launched && i < kLogSteps; ++i)
-    if ((b->h_st->log_step[i] >> 24) == kPhDone) {
+    if ((h_st->log_step[i] >> 24) == kPhDone) {
       needed = i;
       break;
     }
-  b->steps = needed;
   if (g->bfs_steps_version != g->version) {
     g->bfs_steps_version = g->version;
     g->bfs_steps = needed;
   } else {
     g->bfs_steps = std::max(g->bfs_steps, needed);
   }
-  b->reachable = b->h_st->reached;
-  b->levels = done_level;
-  b->bu_levels = b->h_st->bu_levels;
+  TraversalResult r;
+  r.done_level = done_level;
+  r.steps = needed;
+  r.bu_levels = h_st->bu_levels;
+  r.reached = h_st->reached;
+  return r;
+}
+
+static void bfs_full(vgx_bfs* b) {
+  vgx_graph* g = b->g;
+  ensure_work(b);
+  std::fill(std::begin(b->last_level_ms), std::end(b->last_level_ms), 0.f);
+  const uint64_t n = g->n;
+  b->dist_len = n;
+  const uint64_t words = (n + 63) / 64;
+  // the dense head array and the zero-in-degree bitmap follow the graph's version
+  if (b->w.head_version != g->version || b->w.head_n != n) {
+    b->w.head.ensure(g->ncap + 1);
+    b->w.zin.ensure(words + 1);
+    b->w.zcount.ensure(kZinStripes * kStripeWords);
+    build_head_zin(g, b->w.head.p, b->w.zin.p, b->w.zcount.p, g->stream);
+    b->w.head_version = g->version;
+    b->w.head_n = n;
+  }
+  const TraversalResult r =
+      run_traversal(g, level_args(b), b->w.zin.p, b->w.zcount.p, b->source, b->tune, b->h_st, b);
+  b->steps = r.steps;
+  b->reachable = r.reached;
+  b->levels = r.done_level;
+  b->bu_levels = r.bu_levels;
 }
 
 static IncArgs inc_args(vgx_bfs* b) {
