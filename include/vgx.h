@@ -153,6 +153,22 @@ vgx_status vgx_bfs_recompute(vgx_bfs* b);
  * source outside the graph gives all-unreachable distances. */
 vgx_status vgx_bfs_set_source(vgx_bfs* b, uint32_t source);
 
+/* Allocation-free BFS.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under stream capture, serving loops
+ * that forbid allocation).  The distances of a full direction-optimizing
+ * BFS from `source` (UINT32_MAX = unreachable; a source outside the graph
+ * leaves every entry unreachable) are written to dist_dev[0, vertex_count),
+ * device memory on the graph's device, with the same kernels and results as
+ * vgx_bfs_create / vgx_bfs_recompute; *reachable (if not null) gets the
+ * reached count.  The traversal's scratch is one array per GPU of about
+ * 53 MB, held until the process ends, so the graph may have at most
+ * 1,048,576 vertices (more is VGX_INVALID_ARGUMENT, as is cap below the
+ * vertex count).  Calls that overlap on one GPU (from several host threads)
+ * race on that array and must be serialised by the caller.  Runs on the
+ * graph's stream and returns answer-ready. */
+vgx_status vgx_bfs_distances_noalloc(vgx_graph* g, uint32_t source, uint32_t* dist_dev, uint64_t cap,
+                                     uint64_t* reachable);
+
 /* BasicIncrementalBFS::apply -- bfs.hpp:29-98: applies the batch to the
  * graph and repairs the distances exactly (localized invalidation and
  * re-relaxation, or a full recompute past the fallback fraction). */

@@ -42,6 +42,8 @@
 //      realisable upper bounds, hence equals the BFS distances.
 #include <algorithm>
 #include <chrono>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "policy.hpp"
@@ -70,6 +72,9 @@ enum Claims : uint32_t { kClaimQueue = 0, kClaimDistance = 2 };
 // 128-byte line each): additions to one address serialise.
 constexpr uint32_t kZinStripes = 16;
 constexpr uint32_t kStripeWords = 16;
+// Default direction switch (Beamer's alpha 15, beta 18), automatic grid and
+// chunking, tail and claim ratios of the device rules.
+constexpr vgx_bfs_tuning kDefaultTuning{15.0, 18.0, 0, 0, 32.0, 5.0};
 // Words of the frontier bitmap one wave of a pack step derives per loop.
 constexpr uint32_t kPackWords = 16;
 
@@ -239,7 +244,7 @@ struct vgx_bfs {
   uint64_t last_candidates = 0, last_affected = 0, last_relaxed = 0;
   bool last_full = false;
   uint32_t levels = 0, bu_levels = 0;
-  vgx_bfs_tuning tune{15.0, 18.0, 0, 0, 32.0, 5.0};
+  vgx_bfs_tuning tune = vgx::kDefaultTuning;
   vgx::BfsWork w;
   vgx::BfsState* d_st = nullptr;
   vgx::BfsState* h_st = nullptr;
@@ -1409,6 +1414,112 @@ static void bfs_full(vgx_bfs* b) {
   b->bu_levels = r.bu_levels;
 }
 
+// ------------------------------------------------ allocation-free traversal
+// For callers that cannot allocate device memory: the traversal's scratch
+// (state, visited and frontier bitmaps, queues, pending rows, head array,
+// zero-in-degree bitmap) is one file-scope device array sized for graphs of
+// at most kNoAllocVertices vertices, and the distances go to the caller's
+// buffer.  Every call on a GPU uses that GPU's copy of the array and leaves
+// its contents behind; each call rebuilds or re-initialises every part it
+// reads (head and zin from the graph, the rest in k_bfs_init and the steps)
+// before reading it, so nothing has to start at zero.
+
+constexpr uint64_t kNoAllocVertices = 1ull << 20;
+
+struct NoAllocLayout {
+  size_t st, visited, bm, queue0, queue1, pre0, pre1, pend, ppre, head, zin, zcount, total;
+};
+constexpr size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+constexpr NoAllocLayout no_alloc_layout(uint64_t n) {
+  NoAllocLayout l{};
+  const size_t words = (n + 63) / 64;
+  size_t o = 0;
+  l.st = o;
+  o = align256(o + sizeof(BfsState));
+  l.visited = o;
+  o = align256(o + words * 8);
+  l.bm = o;
+  o = align256(o + 3 * words * 8);
+  l.queue0 = o;
+  o = align256(o + n * 4);
+  l.queue1 = o;
+  o = align256(o + n * 4);
+  l.pre0 = o;
+  o = align256(o + n * 8);
+  l.pre1 = o;
+  o = align256(o + n * 8);
+  l.pend = o;
+  o = align256(o + n * 4);
+  l.ppre = o;
+  o = align256(o + n * 8);
+  l.head = o;
+  o = align256(o + (n + 1) * 16);
+  l.zin = o;
+  o = align256(o + (words + 1) * 8);
+  l.zcount = o;
+  o = align256(o + kZinStripes * kStripeWords * 8);
+  l.total = o;
+  return l;
+}
+constexpr NoAllocLayout kNoAllocLayout = no_alloc_layout(kNoAllocVertices);
+
+}  // namespace vgx
+
+// The scratch array (about 53 MB).  In host code this name is a host-side
+// stand-in, not a device address: only the address from
+// hipGetSymbolAddress (kept per device below) may reach a kernel or a copy.
+static __device__ __align__(256) unsigned char scratch[vgx::kNoAllocLayout.total];
+
+namespace vgx {
+
+constexpr int kMaxNoAllocDevices = 64;
+static std::mutex g_noalloc_mu;
+static unsigned char* g_noalloc_ptr[kMaxNoAllocDevices] = {};
+// host copies of the traversal state, one per device (pageable memory)
+static BfsState g_noalloc_state[kMaxNoAllocDevices];
+
+// Device address of `scratch` on `device` (current), looked up on the first
+// call on that device.
+static unsigned char* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxNoAllocDevices) throw Error(VGX_INVALID_ARGUMENT, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    VGX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(VGX_DEVICE_ERROR, "no device address for the scratch array");
+    g_noalloc_ptr[device] = static_cast<unsigned char*>(p);
+  }
+  return g_noalloc_ptr[device];
+}
+
+static uint64_t bfs_noalloc(vgx_graph* g, uint32_t source, uint32_t* dist_dev) {
+  unsigned char* base = noalloc_scratch(g->device);
+  const NoAllocLayout& l = kNoAllocLayout;
+  LevelArgs a;
+  a.row = g->adj[0].row.p;
+  a.arena = g->adj[0].arena.p;
+  a.in_row = g->in().row.p;
+  a.in_arena = g->in().arena.p;
+  a.dist = dist_dev;
+  a.visited = reinterpret_cast<unsigned long long*>(base + l.visited);
+  a.bm = reinterpret_cast<unsigned long long*>(base + l.bm);
+  a.queue0 = reinterpret_cast<uint32_t*>(base + l.queue0);
+  a.queue1 = reinterpret_cast<uint32_t*>(base + l.queue1);
+  a.pre0 = reinterpret_cast<unsigned long long*>(base + l.pre0);
+  a.pre1 = reinterpret_cast<unsigned long long*>(base + l.pre1);
+  a.pend = reinterpret_cast<uint32_t*>(base + l.pend);
+  a.ppre = reinterpret_cast<unsigned long long*>(base + l.ppre);
+  a.st = reinterpret_cast<BfsState*>(base + l.st);
+  a.n = static_cast<uint32_t>(g->n);
+  a.words = static_cast<uint32_t>((g->n + 63) / 64);
+  auto* head = reinterpret_cast<uint4*>(base + l.head);
+  auto* zin = reinterpret_cast<unsigned long long*>(base + l.zin);
+  auto* zcount = reinterpret_cast<unsigned long long*>(base + l.zcount);
+  a.head = head;
+  build_head_zin(g, head, zin, zcount, g->stream);
+  return run_traversal(g, a, zin, zcount, source, kDefaultTuning, &g_noalloc_state[g->device], nullptr).reached;
+}
+
 static IncArgs inc_args(vgx_bfs* b) {
   vgx_graph* g = b->g;
   IncArgs a;
@@ -1726,6 +1837,20 @@ vgx_status vgx_bfs_set_source(vgx_bfs* b, uint32_t source) {
   VGX_HIP(hipSetDevice(b->g->device));
   b->source = source;
   bfs_full(b);
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_distances_noalloc(vgx_graph* g, uint32_t source, uint32_t* dist_dev, uint64_t cap,
+                                     uint64_t* reachable) {
+  VGX_API_BEGIN
+  if (!g || !dist_dev) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  if (g->n > kNoAllocVertices)
+    throw Error(VGX_INVALID_ARGUMENT, "the allocation-free BFS takes graphs of at most 1048576 vertices; this one has " +
+                                          std::to_string(g->n));
+  if (cap < g->n) throw Error(VGX_INVALID_ARGUMENT, "dist_dev holds fewer entries than the graph has vertices");
+  VGX_HIP(hipSetDevice(g->device));
+  const uint64_t r = g->n ? bfs_noalloc(g, source, dist_dev) : 0;
+  if (reachable) *reachable = r;
   VGX_API_END
 }
 
