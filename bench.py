@@ -50,9 +50,12 @@ Secondary legs, each in its own object of the same JSON line:
                10 .. 100,000, each applied and answered by a replay; sampled
                roots checked bit-exact against the single-root path;
 * pagerank  -- directed R-MAT scale 22: FP32 pull PageRank to an L1
-               residual below 1e-7, then 50 batches of 10,000 ops with a
-               warm-started FP32 re-solve; L1 / max-abs error against the
-               FP64 solve on a second copy of the graph, per batch;
+               residual below 1e-7, then 50 batches of 10,000 ops answered
+               by the residual-push repair and, on a second copy, by a
+               warm-started FP32 re-solve; L1 / max-abs error of both
+               against the FP64 solve on a third copy, per batch (that
+               solve checked against the CPU oracle's on the first and last
+               batch); then 10 batches each of 100 and 1,000 ops;
 * ppr       -- directed weighted R-MAT scale 20, 32 one-hot teleport
                columns: weighted personalized PageRank forward + adjoint
                through torch.autograd, 100 sweeps each.
@@ -117,6 +120,9 @@ def parse():
     p.add_argument("--no-pagerank", action="store_true")
     p.add_argument("--no-ppr", action="store_true")
     p.add_argument("--only-bfs", action="store_true", help="plain BFS leg only (for profiling)")
+    p.add_argument("--legs", default="",
+                   help="comma list of secondary legs to run beside the plain BFS (batched, consolidation, static, "
+                        "repairs, contrast, batching, kcore, pagerank, ppr); default all")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                    help="only run 5 device-to-device copies of BYTES (HBM counter calibration)")
     p.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
@@ -787,6 +793,12 @@ def leg_contrast(vx, wl, steps):
 
 
 def leg_pagerank(vx, wl, args, no_cpu):
+    """Directed R-MAT: the FP32 solve, then the update stream answered two
+    ways on two copies of the graph -- the residual-push repair (apply_push:
+    localized pushes, warm full sweeps past its push budget) and the warm
+    re-solve (full sweeps from the previous ranks) -- each checked per batch
+    against the FP64 solve on a third copy, which is itself checked against
+    the CPU oracle's FP64 solve on the first and last batch."""
     import torch
 
     scale = args.scale
@@ -795,11 +807,14 @@ def leg_pagerank(vx, wl, args, no_cpu):
     s, d = vx.rmat_edges(scale, args.edgefactor, 1)
     g32 = vx.Graph(n, True)
     g32.bulk_load_arrays(s, d)
+    gpush = vx.Graph(n, True)
+    gpush.bulk_load_arrays(s, d)
     g64 = vx.Graph(n, True)
     g64.bulk_load_arrays(s, d)
     arcs = int(g32.edge_count_directed)
     log(f"PageRank graph: directed R-MAT s{scale}, {arcs} arcs, built in {time.time() - t0:.1f}s")
     pr32 = vx.IncrementalPageRank(g32, 0.85, 32)
+    prp = vx.IncrementalPageRank(gpush, 0.85, 32)
     pr64 = vx.IncrementalPageRank(g64, 0.85, 64)
     pr32.solve_l1(200, 1e-7, False)  # warm-up (class lists, allocations)
     torch.cuda.synchronize()
@@ -808,15 +823,19 @@ def leg_pagerank(vx, wl, args, no_cpu):
     torch.cuda.synchronize()
     t_solve = time.perf_counter() - t0
     iters = int(pr32.last_repair_iterations)
+    prp.solve_l1(200, 1e-7, False)
+    # the push arm keeps the exact residual of its ranks: one sweep builds it
+    # (a self-loop batch changes no arc)
+    prp.apply_push_arrays(np.array([0], np.uint32), np.array([0], np.uint32), np.array([1], np.uint8))
     pr64.solve_l1(1000, 1e-11, False)
-    err_l1 = []
-    err_max = []
+    errs = {"warm": ([], []), "push": ([], [])}
 
     def errors():
-        a = pr32.values_numpy()
         b = pr64.values_numpy()
-        err_l1.append(float(np.abs(a - b).sum()))
-        err_max.append(float(np.abs(a - b).max()))
+        for key, pr in (("warm", pr32), ("push", prp)):
+            a = pr.values_numpy()
+            errs[key][0].append(float(np.abs(a - b).sum()))
+            errs[key][1].append(float(np.abs(a - b).max()))
 
     errors()
     bytes_it = 8.0 * arcs + 20.0 * n
@@ -826,33 +845,100 @@ def leg_pagerank(vx, wl, args, no_cpu):
            "gbps_per_iteration_wall": bytes_it * iters / t_solve / 1e9,
            "roofline_frac_wall": bytes_it * iters / t_solve / 1e9 / HBM_PEAK_GBPS,
            "residual_l1": float(pr32.last_residual_l1)}
+    oracle_check = []
+
+    pending = []
+
+    def cpu_check(model):
+        from oracle import oracle as orc
+
+        if pending:  # the batches since the last check, in order (the last op on an arc wins)
+            model.apply(*(np.concatenate(x) for x in zip(*pending)))
+            pending.clear()
+        ref, _ = orc.pagerank_full(model, 0.85, 150, 1e-15)
+        b = pr64.values_numpy()
+        oracle_check.append({"l1": float(np.abs(b - ref).sum()), "max_abs": float(np.abs(b - ref).max())})
+
+    model = None
+    if not no_cpu:
+        from oracle import oracle as orc
+
+        orc.build()
+        model = orc.CsrModel(n, True)
+        model.bulk_load_edges(s, d)
     if args.pr_batches > 0:
         eu, ev = wl.canonical_edges(s, d, True)
         stream = wl.BatchStream(eu, ev, n, True, seed=2)
-        bt = []
-        its = []
-        for _ in range(args.pr_batches):
+        t_warm, t_push, its, push_info = [], [], [], []
+        for i in range(args.pr_batches):
             bs, bd, ba = stream.next(args.batch_size, 0.8)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             g32.apply_arrays(bs, bd, ba)
             pr32.solve_l1(200, 1e-7, True)
             torch.cuda.synchronize()
-            bt.append(time.perf_counter() - t0)
+            t1 = time.perf_counter()
+            r = prp.apply_push_arrays(bs, bd, ba, 1e-7)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            t_warm.append(t1 - t0)
+            t_push.append(t2 - t1)
             its.append(int(pr32.last_repair_iterations))
+            push_info.append(r)
             g64.apply_arrays(bs, bd, ba)
             pr64.solve_l1(1000, 1e-11, True)
             errors()
+            if model is not None:
+                pending.append((bs, bd, ba))
+                if i == 0 or i == args.pr_batches - 1:
+                    cpu_check(model)
         out.update({"batches": args.pr_batches, "batch_size": args.batch_size,
-                    "mean_batch_ms": 1e3 * float(np.mean(bt)), "batches_per_s": len(bt) / sum(bt),
-                    "mean_batch_iterations": float(np.mean(its))})
-    out.update({"fp64_reference": "GPU FP64 solve to L1 residual 1e-11 on a second copy of the graph",
-                "max_l1_error": max(err_l1), "max_abs_error": max(err_max),
-                "within_1e-6": bool(max(err_l1) <= 1e-6 and max(err_max) <= 1e-6)})
+                    "warm_resolve": {"mean_batch_ms": 1e3 * float(np.mean(t_warm)),
+                                     "batches_per_s": len(t_warm) / sum(t_warm),
+                                     "mean_sweeps": float(np.mean(its)),
+                                     "max_l1_error": max(errs["warm"][0]), "max_abs_error": max(errs["warm"][1])},
+                    "push_repair": {"mean_batch_ms": 1e3 * float(np.mean(t_push)),
+                                    "batches_per_s": len(t_push) / sum(t_push),
+                                    "fallback_batches": int(sum(x["used_fallback"] for x in push_info)),
+                                    "mean_rounds": float(np.mean([x["rounds"] for x in push_info])),
+                                    "mean_pushed_arcs": float(np.mean([x["pushed_arcs"] for x in push_info])),
+                                    "mean_fallback_sweeps": float(np.mean([x["fallback_sweeps"] for x in push_info])),
+                                    "max_l1_error": max(errs["push"][0]), "max_abs_error": max(errs["push"][1])}})
+        out["batches_per_s"] = out["push_repair"]["batches_per_s"]
+        # smaller batches on the same stream: where the localized pushes pay
+        sweep = {}
+        for k in (10, 100, 1000):
+            tw, tp, fb = [], [], 0
+            for _ in range(10):
+                bs, bd, ba = stream.next(k, 0.8)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                g32.apply_arrays(bs, bd, ba)
+                pr32.solve_l1(200, 1e-7, True)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                r = prp.apply_push_arrays(bs, bd, ba, 1e-7)
+                torch.cuda.synchronize()
+                t2 = time.perf_counter()
+                tw.append(t1 - t0)
+                tp.append(t2 - t1)
+                fb += int(r["used_fallback"])
+                g64.apply_arrays(bs, bd, ba)
+                pr64.solve_l1(1000, 1e-11, True)
+                errors()
+            sweep[str(k)] = {"warm_resolve_ms": 1e3 * float(np.mean(tw)), "push_repair_ms": 1e3 * float(np.mean(tp)),
+                             "push_fallback_batches": fb, "batches": 10}
+        out["batch_size_sweep"] = sweep
+    out.update({"fp64_reference": "GPU FP64 solve to L1 residual 1e-11 on a third copy of the graph",
+                "max_l1_error": max(errs["warm"][0] + errs["push"][0]),
+                "max_abs_error": max(errs["warm"][1] + errs["push"][1])})
+    out["within_1e-6"] = bool(out["max_l1_error"] <= 1e-6 and out["max_abs_error"] <= 1e-6)
+    if oracle_check:
+        out["fp64_reference_vs_cpu_oracle"] = oracle_check
     if not no_cpu:
         out["cpu_reference"] = cpu_pagerank(s, d, n, 3)
     log(f"pagerank: {out}")
-    del pr32, pr64, g32, g64
+    del pr32, prp, pr64, g32, gpush, g64
     return out
 
 
@@ -932,6 +1018,11 @@ def main():
     if args.pmc_child:
         pmc_child(args)
         return
+    if args.legs:
+        want = {x.strip() for x in args.legs.split(",") if x.strip()}
+        for leg in ("batched", "consolidation", "static", "repairs", "contrast", "batching", "kcore", "pagerank", "ppr"):
+            if leg not in want:
+                setattr(args, f"no_{leg}", True)
     if args.only_bfs:
         args.no_cpu = args.no_repairs = args.no_batched = args.no_contrast = args.no_consolidation = args.no_static = True
         args.no_kcore = True

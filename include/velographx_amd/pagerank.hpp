@@ -53,6 +53,21 @@ class IncrementalPageRank {
     vgx_check(vgx_pagerank_solve_l1(p_, max_iterations, l1_tol, warm_start ? 1 : 0));
   }
 
+  // addition: localized residual-push repair with a full-sweep fallback
+  vgx_pagerank_push_stats apply_push(const std::uint32_t* src, const std::uint32_t* dst, const std::uint8_t* add,
+                                     std::size_t k, double l1_tol = 1e-7, double threshold = 0.0,
+                                     double max_push_fraction = 0.05, std::size_t max_sweeps = 200) {
+    vgx_pagerank_push_stats st{};
+    vgx_check(vgx_pagerank_apply_push(p_, src, dst, add, k, l1_tol, threshold, max_push_fraction, max_sweeps, &st));
+    return st;
+  }
+  vgx_pagerank_push_stats apply_push(const UpdateBatch& batch, double l1_tol = 1e-7, double threshold = 0.0,
+                                     double max_push_fraction = 0.05, std::size_t max_sweeps = 200) {
+    auto c = batch.columns();
+    return apply_push(c.src.data(), c.dst.data(), c.add.data(), c.src.size(), l1_tol, threshold, max_push_fraction,
+                      max_sweeps);
+  }
+
   [[nodiscard]] PageRankValidation validate_against_full(std::size_t reference_max_iterations = 500,
                                                          double reference_tol = 1e-12, double l1_tolerance = 1e-6,
                                                          double linf_tolerance = 1e-7) const {

@@ -277,6 +277,39 @@ vgx_status vgx_pagerank_static(vgx_graph* g, double damping, uint64_t max_iterat
 /* apply(batch, local_iterations, tol, full_fallback_fraction) -- pagerank.hpp:47-191. */
 vgx_status vgx_pagerank_apply(vgx_pagerank* p, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
                               uint64_t k, uint64_t local_iterations, double tol, double full_fallback_fraction);
+/* Localized residual-push repair (no reference counterpart: the
+ * reference's apply, pagerank.hpp:47-191, runs Jacobi sweeps over an active
+ * set and falls back whenever a dangling vertex exists).  The object keeps
+ * the exact residual r = b + d M x - x of its ranks x.  The batch is applied
+ * to the graph; the changed rows' old contributions leave r and their new
+ * ones enter it (a dangling row's share goes to a uniform scalar); then
+ * rounds of pushes move every entry above `threshold` (0 = l1_tol / 1024)
+ * into its rank and d r_u / deg(u) to each out-neighbour, lowering the
+ * threshold 16-fold whenever no entry is above it, until sum |r| <= l1_tol;
+ * the uniform part is folded into the ranks exactly (A^-1 1 = n/(1-d) x*).
+ * Past max_push_fraction x stored arcs pushed in total, it falls back: the
+ * ranks from before the pushes are restored and warm full sweeps (at most
+ * max_sweeps) run until the L1 change is below l1_tol.  After a batch
+ * whose first push round alone was over budget, batches of at least half
+ * its size go to the sweeps directly (every 8th of them tries the pushes
+ * again; a push phase that finishes clears this).  Either way the L1 norm of the residual of the reported ranks is
+ * below l1_tol (the L1 error is at most that over 1 - d).  A change of the
+ * vertex count recomputes from scratch. */
+typedef struct {
+  uint64_t touched_rows;    /* rows whose out-arcs the batch's ops address */
+  uint64_t rounds;          /* push rounds */
+  uint64_t pushed_vertices; /* pushes (a vertex pushed in two rounds counts twice) */
+  uint64_t pushed_arcs;     /* out-arcs those pushes scanned */
+  uint64_t fallback_sweeps; /* full sweeps run (fallback, or the residual's first sync) */
+  int used_fallback;
+  int skipped_push;         /* no push phase: an earlier batch of at least half this size had
+                               its first push round over budget (every 8th such batch tries) */
+  double residual_l1;       /* sum |r| of the reported ranks */
+  double threshold;         /* the last push threshold */
+} vgx_pagerank_push_stats;
+vgx_status vgx_pagerank_apply_push(vgx_pagerank* p, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
+                                   uint64_t k, double l1_tol, double threshold, double max_push_fraction,
+                                   uint64_t max_sweeps, vgx_pagerank_push_stats* out);
 /* values() as float64 -- pagerank.hpp:40. */
 vgx_status vgx_pagerank_values(vgx_pagerank* p, double* out, uint64_t cap, uint64_t* count);
 

@@ -10,7 +10,8 @@ CPU oracle on the host (nothing here compares the GPU with itself):
 * undirected R-MAT scale 20: all 64 roots of a multi-source pass (captured
   launch program) after each of 4 uneven update batches;
 * directed R-MAT scale 22: the FP32 PageRank solve to an L1 residual below
-  1e-7, and its warm-started re-solve after 3 batches of 10,000 ops, within
+  1e-7, its warm-started re-solve after 3 batches of 10,000 ops and the
+  residual-push repair after 3 more (10,000, 10,000 and 100 ops), within
   1e-6 (L1 and max-abs) of the oracle's FP64 solve;
 * directed weighted R-MAT scale 20, 32 one-hot teleport columns: the
   forward and the teleport gradient against the oracle's FP64 sweeps, and
@@ -210,6 +211,13 @@ def test_config4_s22_directed_fp32_pagerank(vx, wl, orc, rmat22):
         model.apply(bs, bd, ba)
         pr.solve_l1(200, 1e-7, True)
         check(f"batch {b}")
+    # the residual-push repair on the same stream (10,000 ops, then 100)
+    for b, k in enumerate((10000, 10000, 100)):
+        bs, bd, ba = stream.next(k, 0.8)
+        r = pr.apply_push_arrays(bs, bd, ba, l1_tol=1e-7)
+        model.apply(bs, bd, ba)
+        assert r["residual_l1"] <= 1e-7, r
+        check(f"push batch {b} ({k} ops, fallback {r['used_fallback']}, {r['rounds']} rounds)")
 
 
 def test_config5_s20_ppr_fd_200_arcs(vx):

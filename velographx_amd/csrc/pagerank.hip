@@ -39,6 +39,8 @@ constexpr uint32_t kChunk = 4096;
 // Workgroups per in-degree class and launch; each class loops grid-stride,
 // so the last workgroup reduces at most 3 x 1024 partials.
 constexpr uint64_t kMaxClassBlocks = 1024;
+// Every kPushProbe-th batch marked to skip the residual pushes tries them.
+constexpr uint32_t kPushProbe = 8;
 
 struct PrState {
   double dangling_share;  // d * dangling / n for the next iteration
@@ -56,6 +58,20 @@ struct PrState {
 
 struct PrPartial {
   double l1, linf, dangling;
+};
+
+// Counters of the residual push.  Round k's claim reads the active list of
+// acnt[k % 3] and appends the rows it pushes to q[k & 1] (packed count << 38
+// | arcs); its distribute appends the vertices whose residual crosses the
+// threshold to acnt[(k + 1) % 3].  Claim k clears acnt[(k + 1) % 3] and
+// q[(k + 1) & 1], slots no kernel of round k reads, so rounds go out back to
+// back with no host step between them.
+struct PushState {
+  unsigned long long acnt[3];
+  unsigned long long q[2];
+  unsigned long long pushed_vertices;
+  double c;   // uniform residual: every entry of the residual is r_s[v] + c
+  double l1;  // sum |r_s| (residual reduction)
 };
 
 }  // namespace vgx
@@ -83,6 +99,22 @@ struct vgx_pagerank {
   vgx::DevBuf<uint32_t> act_flag, ever_flag, alist[2], counters;
   vgx::DevBuf<uint8_t> upd;
   vgx::DevBuf<uint8_t> scan_tmp;
+  // residual push: the residual of the current ranks (exact while
+  // res_valid), its entries' bound, and the push lists
+  vgx::DevBuf<uint8_t> res, push_amt, push_save;
+  vgx::DevBuf<uint32_t> push_stamp, push_list[3], push_q;
+  vgx::DevBuf<unsigned long long> push_pre;
+  vgx::PushState* d_push = nullptr;
+  vgx::PushState* h_push = nullptr;
+  bool res_valid = false;
+  uint64_t res_version = ~0ull;
+  double res_bound = INFINITY;  // every |residual entry| is at most this
+  uint32_t push_epoch = 0;
+  // batches of at least half this many ops skip the pushes (the last push
+  // phase of such a batch exceeded its budget in the first round), except
+  // every kPushProbe-th one, which tries again
+  uint64_t push_skip_k = 0;
+  uint32_t push_skipped = 0;
 };
 
 namespace vgx {
@@ -572,6 +604,239 @@ __global__ void k_diff(const T* a, const double* ref, uint64_t n, double* out2) 
   }
 }
 
+// ------------------------------------------------------ residual push
+//
+// The residual of ranks x is r = b + d M x - x (b = (1-d)/n, M the pull
+// operator with the dangling mass spread uniformly); the exact ranks are
+// x* = x + A^-1 r with A = I - d M.  A batch changes the rows of its
+// sources: their old contributions d x_u / deg_old(u) leave the residual of
+// their old out-neighbours and the new ones enter at the new out-neighbours
+// (a dangling row contributes d x_u / n to every vertex: that part goes to
+// the uniform scalar c).  A push moves r_u into x_u and d r_u / deg(u) to
+// each out-neighbour's residual, or d r_u / n into c when u is dangling; it
+// keeps x* = x + A^-1 (r_s + c 1) exact.  Since A x* = b with b uniform,
+// A^-1 1 = n / (1 - d) x*, so once r_s is small the uniform part is folded
+// exactly: x* = (x + A^-1 r_s) / (1 - k), k = c n / (1 - d), and the residual
+// of x / (1 - k) is r_s / (1 - k).
+
+struct PushArgs {
+  const uint2* out_row;
+  const uint32_t* out_arena;
+  const uint32_t* outdeg;
+  void* x;
+  void* r;
+  void* amt;                  // per pushed row: d r_u / deg(u)
+  uint32_t* stamp;            // epoch-stamped list membership
+  uint32_t* list[3];          // active lists, ring by round
+  uint32_t* q;                // rows pushed this round
+  unsigned long long* pre;    // their arc prefixes
+  PushState* st;
+  double damping;
+  double theta;               // push entries with |r| > theta
+  uint32_t n;
+  uint32_t epoch;
+};
+
+// Append w to the active list `li` once per epoch (wave-aggregated).  Must
+// be called by every lane of the wave.
+__device__ __forceinline__ void push_activate(const PushArgs& a, int li, uint32_t epoch, uint32_t w, bool want) {
+  const bool add = want && atomicExch(&a.stamp[w], epoch) != epoch;
+  const unsigned long long bb = __ballot(add);
+  if (!bb) return;
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(&a.st->acnt[li], static_cast<unsigned long long>(__popcll(bb)));
+  base = __shfl(base, 0, kWave);
+  if (add) a.list[li][base + lanes_below(bb)] = w;
+}
+
+template <class T>
+__device__ __forceinline__ void add_residual(const PushArgs& a, int li, uint32_t epoch, uint32_t w, T amount,
+                                             bool want) {
+  bool cross = false;
+  if (want) {
+    T* r = static_cast<T*>(a.r);
+    const T old = atomicAdd(&r[w], amount);
+    const T nw = old + amount;
+    cross = fabs(static_cast<double>(old)) <= a.theta && fabs(static_cast<double>(nw)) > a.theta;
+  }
+  push_activate(a, li, epoch, w, cross);
+}
+
+// Rows whose out-arcs the batch changes: the sources of its ops (both
+// endpoints when undirected), each once (stamp epoch), into list 2.
+__global__ void k_push_mark_rows(PushArgs a, const uint32_t* src, const uint32_t* dst, uint64_t k, int undirected) {
+  for (uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); i0 < k;
+       i0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t i = i0 + threadIdx.x;
+    const bool in = i < k;
+    const uint32_t u = in ? src[i] : 0u, v = in ? dst[i] : 0u;
+    const bool op = in && u != v && u < a.n && v < a.n;
+    push_activate(a, 2, a.epoch, u, op);
+    push_activate(a, 2, a.epoch, v, op && undirected);
+  }
+}
+
+// Wave-aggregated reservation of queue slot and arc prefix for (u, len,
+// part) in the packed counter *qc.  Called by every lane of the wave.
+template <class T>
+__device__ __forceinline__ void push_queue(const PushArgs& a, unsigned long long* qc, bool row, uint32_t u,
+                                           uint32_t len, T part) {
+  const unsigned long long ball = __ballot(row);
+  if (!ball) return;
+  const uint32_t e = row ? len : 0u;
+  const unsigned long long einc = wave_inclusive_scan<unsigned long long>(e);
+  const unsigned long long etot = __shfl(einc, kWave - 1, kWave);
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(qc, (static_cast<unsigned long long>(__popcll(ball)) << kEdgeBits) + etot);
+  base = __shfl(base, 0, kWave);
+  if (row) {
+    const uint64_t slot = (base >> kEdgeBits) + lanes_below(ball);
+    a.q[slot] = u;
+    a.pre[slot] = (base & kEdgeMask) + einc - e;
+    static_cast<T*>(a.amt)[slot] = part;
+  }
+}
+
+// Contributions of the rows of list 2, with `sign` -1 (before the batch:
+// the old rows) or +1 (after it: the new rows): rows with live out-arcs go
+// to the pushed queue q[0] with sign d x_u / deg(u) for k_push_spread, rows
+// without add sign d x_u / n to the uniform part.
+template <class T>
+__global__ void k_push_queue_rows(PushArgs a, double sign) {
+  const T* x = static_cast<const T*>(a.x);
+  const uint64_t nrows = a.st->acnt[2];
+  double cl = 0.0;
+  for (uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); i0 < nrows;
+       i0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t i = i0 + threadIdx.x;
+    bool row = false;
+    uint32_t u = 0, len = 0;
+    T part = T(0);
+    if (i < nrows) {
+      u = a.list[2][i];
+      const uint32_t deg = a.outdeg[u];
+      const double xu = static_cast<double>(x[u]);
+      if (deg == 0) {
+        cl += sign * a.damping * xu / static_cast<double>(a.n);
+      } else {
+        row = true;
+        len = a.out_row[u].y;
+        part = static_cast<T>(sign * a.damping * xu / static_cast<double>(deg));
+      }
+    }
+    push_queue<T>(a, &a.st->q[0], row, u, len, part);
+  }
+  cl = wave_sum(cl);
+  if (lane_id() == 0 && cl != 0.0) atomicAdd(&a.st->c, cl);
+}
+
+// Every vertex with |r| > theta into active list `li` (after the threshold
+// was lowered, or when no bound on the entries is known).
+template <class T>
+__global__ void k_push_seed(PushArgs a, int li) {
+  const T* r = static_cast<const T*>(a.r);
+  for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < a.n; v0 += gridDim.x * blockDim.x) {
+    const uint32_t v = v0 + threadIdx.x;
+    const bool want = v < a.n && fabs(static_cast<double>(r[v])) > a.theta;
+    push_activate(a, li, a.epoch, v, want);
+  }
+}
+
+// Claim of round k: every listed vertex whose residual is above the
+// threshold moves it into its rank; rows with live out-arcs go to the
+// pushed list q[k & 1] with d r_u / deg(u), dangling ones add d r_u / n to
+// the uniform part.  Workgroup 0 clears the counters round k + 1 appends to.
+template <class T>
+__global__ void k_push_claim(PushArgs a, uint32_t k) {
+  PushState* st = a.st;
+  const int cur = static_cast<int>(k % 3);
+  const uint64_t na = st->acnt[cur];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->acnt[(k + 1) % 3] = 0;
+    st->q[(k + 1) & 1] = 0;
+  }
+  T* x = static_cast<T*>(a.x);
+  T* r = static_cast<T*>(a.r);
+  T* amt = static_cast<T*>(a.amt);
+  unsigned long long* qc = &st->q[k & 1];
+  double cl = 0.0;
+  unsigned long long pushed = 0;
+  for (uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); i0 < na;
+       i0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t i = i0 + threadIdx.x;
+    bool row = false;
+    uint32_t u = 0, len = 0;
+    T part = T(0);
+    if (i < na) {
+      u = a.list[cur][i];
+      const T ru = r[u];
+      if (fabs(static_cast<double>(ru)) > a.theta) {
+        r[u] = T(0);
+        x[u] += ru;
+        ++pushed;
+        const uint32_t deg = a.outdeg[u];
+        if (deg == 0) {
+          cl += a.damping * static_cast<double>(ru) / static_cast<double>(a.n);
+        } else {
+          row = true;
+          len = a.out_row[u].y;
+          part = static_cast<T>(a.damping * static_cast<double>(ru) / static_cast<double>(deg));
+        }
+      }
+    }
+    push_queue<T>(a, qc, row, u, len, part);
+  }
+  cl = wave_sum(cl);
+  pushed = wave_sum(pushed);
+  if (lane_id() == 0) {
+    if (cl != 0.0) atomicAdd(&st->c, cl);
+    if (pushed) atomicAdd(&st->pushed_vertices, pushed);
+  }
+}
+
+// The queued rows' amounts (queue slot qs) to their live out-neighbours'
+// residuals, edge-balanced; entries that cross the threshold join list li
+// (epoch `epoch`).  Round k's distribute is (k & 1, (k + 1) % 3, its epoch).
+template <class T>
+__global__ __launch_bounds__(kPrBlock) void k_push_spread(PushArgs a, int qs, int li, uint32_t epoch) {
+  const unsigned long long qc = a.st->q[qs];
+  const uint32_t F = static_cast<uint32_t>(qc >> kEdgeBits);
+  const unsigned long long E = qc & kEdgeMask;
+  const T* amt = static_cast<const T*>(a.amt);
+  const uint32_t W = gridDim.x * (blockDim.x / kWave);
+  const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  lbs_edges(a.q, a.pre, F, E, a.out_row, gw, W, [&](uint32_t item, uint32_t, uint2, uint32_t slot, bool valid) {
+    const uint32_t w = valid ? a.out_arena[slot] : kTomb;
+    const T amount = valid ? amt[item] : T(0);
+    add_residual<T>(a, li, epoch, w & kIdMask, amount, !(w & kTomb));
+  });
+}
+
+template <class T>
+__global__ void k_push_l1(const T* r, uint32_t n, double* out) {
+  double s = 0.0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    s += fabs(static_cast<double>(r[v]));
+  s = wave_sum(s);
+  if (lane_id() == 0 && s != 0.0) atomicAdd(out, s);
+}
+
+// x <- x * scale, r <- r * scale with scale = 1 / (1 - k), k = c n / (1 - d):
+// the uniform part of the residual folded into the ranks (see above).
+template <class T>
+__global__ void k_push_fold(T* x, T* r, uint32_t n, double scale) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    x[v] = static_cast<T>(static_cast<double>(x[v]) * scale);
+    r[v] = static_cast<T>(static_cast<double>(r[v]) * scale);
+  }
+}
+
+// r = next - x: the residual of x from one sweep's output next.
+template <class T>
+__global__ void k_push_resync(const T* next, const T* x, T* r, uint32_t n) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) r[v] = next[v] - x[v];
+}
+
 // ---------------------------------------------------------------- host side
 
 static unsigned gblocks(uint64_t items, unsigned per = kPrBlock, unsigned cap = 4096) {
@@ -732,6 +997,7 @@ static void count_dangling(vgx_pagerank* p) {
 
 // recompute(), pagerank.hpp:193-202
 static void recompute(vgx_pagerank* p, uint64_t max_iterations, double tol, bool stop_l1 = false, bool warm = false) {
+  p->res_valid = false;
   int r = full_solve(p, p->rank, p->precision, max_iterations, tol, stop_l1, warm);
   p->cur = r;
   p->n = p->g->n;
@@ -835,6 +1101,7 @@ static void local_repair(vgx_pagerank* p, const uint32_t* d_src, const uint32_t*
 static void apply(vgx_pagerank* p, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
                   uint64_t local_iterations, double tol, double frac) {
   vgx_graph* g = p->g;
+  p->res_valid = false;
   if (k == 0) {
     reset_local_stats(p);
     return;
@@ -879,6 +1146,279 @@ static void apply(vgx_pagerank* p, const uint32_t* src, const uint32_t* dst, con
   uint64_t limit = static_cast<uint64_t>(fl > 1.0 ? fl : 1.0);
   if (p->precision == 32) local_repair<float>(p, d_src, d_dst, k, local_iterations, tol, limit);
   else local_repair<double>(p, d_src, d_dst, k, local_iterations, tol, limit);
+}
+
+// ------------------------------------------------------ residual push (host)
+
+static void push_ensure(vgx_pagerank* p) {
+  const uint64_t n = p->g->n;
+  const size_t es = elt(p);
+  p->res.ensure(n * es + 64);
+  p->push_amt.ensure((n + 1) * es + 64);
+  if (p->push_stamp.n < n + 1) {
+    p->push_stamp.ensure(n + 1);
+    VGX_HIP(hipMemsetAsync(p->push_stamp.p, 0, p->push_stamp.n * 4, p->g->stream));
+    p->push_epoch = 0;
+  }
+  for (auto& l : p->push_list) l.ensure(n + 1);
+  p->push_q.ensure(n + 1);
+  p->push_pre.ensure(n + 1);
+  if (!p->d_push) {
+    p->d_push = static_cast<PushState*>(dev_alloc(sizeof(PushState)));
+    VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_push), sizeof(PushState), hipHostMallocDefault));
+  }
+}
+
+static PushArgs push_args(vgx_pagerank* p, double theta, uint32_t epoch) {
+  vgx_graph* g = p->g;
+  PushArgs a{};
+  a.out_row = g->adj[0].row.p;
+  a.out_arena = g->adj[0].arena.p;
+  a.outdeg = g->adj[0].deg.p;
+  a.x = p->rank[p->cur].p;
+  a.r = p->res.p;
+  a.amt = p->push_amt.p;
+  a.stamp = p->push_stamp.p;
+  for (int i = 0; i < 3; ++i) a.list[i] = p->push_list[i].p;
+  a.q = p->push_q.p;
+  a.pre = p->push_pre.p;
+  a.st = p->d_push;
+  a.damping = p->damping;
+  a.theta = theta;
+  a.n = static_cast<uint32_t>(g->n);
+  a.epoch = epoch;
+  return a;
+}
+
+// Sweeps from the current ranks (warm) until the L1 change is below tol,
+// then the residual of the ranks kept: the last sweep's output minus its
+// input, with the input kept as the ranks (its residual is that L1 change).
+template <class T>
+static uint64_t push_sweeps(vgx_pagerank* p, uint64_t max_sweeps, double tol) {
+  vgx_graph* g = p->g;
+  const uint64_t n = g->n;
+  push_ensure(p);  // sized for the current vertex count (it may have grown)
+  if (p->cur != 0) {
+    VGX_HIP(hipMemcpyAsync(p->rank[0].p, p->rank[p->cur].p, n * sizeof(T), hipMemcpyDeviceToDevice, g->stream));
+    p->cur = 0;
+  }
+  const int r = full_solve(p, p->rank, p->precision, std::max<uint64_t>(max_sweeps, 1), tol, true, true);
+  const uint64_t sweeps = p->h_st->iterations;
+  const int prev = r ^ 1;
+  k_push_resync<T><<<gblocks(n, kPrBlock, 2048), kPrBlock, 0, g->stream>>>(
+      reinterpret_cast<const T*>(p->rank[r].p), reinterpret_cast<const T*>(p->rank[prev].p),
+      reinterpret_cast<T*>(p->res.p), static_cast<uint32_t>(n));
+  VGX_LAUNCH_CHECK();
+  p->cur = prev;
+  p->n = n;
+  p->have_values = true;
+  p->res_valid = true;
+  p->res_version = g->version;
+  p->res_bound = INFINITY;
+  p->stats.last_residual_l1 = p->h_st->l1;
+  p->stats.last_residual_linf = p->h_st->linf;
+  return sweeps;
+}
+
+template <class T>
+static double push_residual_l1(vgx_pagerank* p) {
+  vgx_graph* g = p->g;
+  VGX_HIP(hipMemsetAsync(&p->d_push->l1, 0, 8, g->stream));
+  k_push_l1<T><<<gblocks(g->n, kPrBlock, 1024), kPrBlock, 0, g->stream>>>(reinterpret_cast<const T*>(p->res.p),
+                                                                          static_cast<uint32_t>(g->n), &p->d_push->l1);
+  VGX_LAUNCH_CHECK();
+  double l1 = 0.0;
+  VGX_HIP(hipMemcpyAsync(&l1, &p->d_push->l1, 8, hipMemcpyDeviceToHost, g->stream));
+  VGX_HIP(hipStreamSynchronize(g->stream));
+  return l1;
+}
+
+template <class T>
+static void apply_push(vgx_pagerank* p, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
+                       double tol, double theta, double max_push_fraction, uint64_t max_sweeps,
+                       vgx_pagerank_push_stats* out) {
+  vgx_graph* g = p->g;
+  hipStream_t s = g->stream;
+  vgx_pagerank_push_stats st{};
+  if (theta <= 0.0) theta = tol / 1024.0;
+  st.threshold = theta;
+  if (k == 0) {
+    reset_local_stats(p);
+    if (out) *out = st;
+    return;
+  }
+  push_ensure(p);
+  const uint64_t n0 = g->n;
+  // the residual of the current ranks, exact before the batch is injected
+  if (!p->have_values || !p->res_valid || p->res_version != g->version || p->n != n0) {
+    if (!p->have_values || p->n != n0) recompute(p, 200, 1e-12);
+    st.fallback_sweeps += push_sweeps<T>(p, 1, 0.0);
+  }
+  graph_stage(g, src, dst, add, k);
+  const uint32_t* d_src = g->bs[0].src.p;
+  const uint32_t* d_dst = g->bs[0].src.p + k;
+  const uint64_t n = g->n;
+  if (n != n0) {
+    // vertex growth changes the teleport of every vertex: full solve
+    graph_apply_deletions(g);
+    graph_apply_additions(g);
+    graph_finish_batch(g);
+    recompute(p, 200, 1e-12);
+    st.fallback_sweeps += p->h_st->iterations;
+    st.fallback_sweeps += push_sweeps<T>(p, 1, 0.0);
+    st.used_fallback = 1;
+    st.residual_l1 = p->h_st->l1;
+    count_dangling(p);
+    p->stats.last_used_full_recompute = 1;
+    if (out) *out = st;
+    return;
+  }
+  if (p->push_skip_k && 2 * k >= p->push_skip_k && ++p->push_skipped % kPushProbe != 0) {
+    // the pushes of a batch this size did not fit their budget: warm sweeps
+    // straight away (they keep the residual exact: the last sweep's change)
+    graph_apply_deletions(g);
+    graph_apply_additions(g);
+    graph_finish_batch(g);
+    st.used_fallback = 1;
+    st.skipped_push = 1;
+    st.fallback_sweeps += push_sweeps<T>(p, max_sweeps, tol);
+    st.residual_l1 = p->h_st->l1;
+    count_dangling(p);
+    reset_local_stats(p);
+    p->stats.last_used_full_recompute = 1;
+    p->stats.last_repair_iterations = st.fallback_sweeps;
+    p->stats.last_repaired_vertices = n;
+    p->stats.last_residual_l1 = st.residual_l1;
+    if (out) *out = st;
+    return;
+  }
+  if (p->push_epoch > 0xFFFFFFFFu - 4096) {
+    VGX_HIP(hipMemsetAsync(p->push_stamp.p, 0, p->push_stamp.n * 4, s));
+    p->push_epoch = 0;
+  }
+  const uint32_t base = p->push_epoch + 1;
+  VGX_HIP(hipMemsetAsync(p->d_push, 0, sizeof(PushState), s));
+  const unsigned grid = 1024;
+  // rows the batch changes; their old contributions leave the residual
+  PushArgs a = push_args(p, theta, base);
+  k_push_mark_rows<<<gblocks(k, kPrBlock, 1024), kPrBlock, 0, s>>>(a, d_src, d_dst, k, g->directed ? 0 : 1);
+  VGX_LAUNCH_CHECK();
+  auto inject = [&](double sign) {
+    VGX_HIP(hipMemsetAsync(&p->d_push->q[0], 0, sizeof(unsigned long long), s));
+    k_push_queue_rows<T><<<grid, kPrBlock, 0, s>>>(a, sign);
+    VGX_LAUNCH_CHECK();
+    k_push_spread<T><<<2 * grid, kPrBlock, 0, s>>>(a, 0, 0, base + 1);
+    VGX_LAUNCH_CHECK();
+  };
+  inject(-1.0);
+  graph_apply_deletions(g);
+  graph_apply_additions(g);
+  graph_finish_batch(g);
+  // the new rows' contributions enter (the arena may have moved)
+  a = push_args(p, theta, base);
+  inject(1.0);
+  VGX_HIP(hipMemsetAsync(&p->d_push->q[0], 0, sizeof(unsigned long long), s));
+  const uint64_t budget = static_cast<uint64_t>(max_push_fraction * static_cast<double>(g->adj[0].stored));
+  uint32_t round = 0;
+  bool fallback = false;
+  double l1 = 0.0;
+  auto seed = [&](int li, uint32_t epoch) {
+    PushArgs sa = push_args(p, theta, epoch);
+    k_push_seed<T><<<gblocks(n, kPrBlock, 2048), kPrBlock, 0, s>>>(sa, li);
+    VGX_LAUNCH_CHECK();
+  };
+  // entries above the threshold from before the batch
+  if (p->res_bound > theta) seed(0, base + 1);
+  // the ranks before any push: a fallback re-solves from them (sweeps from
+  // partly pushed ranks converge more slowly: the pushes leave the error in
+  // the slowly decaying directions of the iteration)
+  p->push_save.ensure(n * sizeof(T) + 64);
+  VGX_HIP(hipMemcpyAsync(p->push_save.p, p->rank[p->cur].p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+  // One round: the claim, a host check of its pushed arcs against the
+  // budget, then the distribute.  A claim that finds nothing above the
+  // threshold ends the level: the residual's L1 norm is checked and, when
+  // still above l1_tol, the threshold drops 16-fold and the entries above
+  // it seed the next round.
+  uint64_t pushed_arcs = 0, last_pv = 0;
+  bool first = true;
+  for (;;) {
+    PushArgs ra = push_args(p, theta, base);
+    k_push_claim<T><<<grid, kPrBlock, 0, s>>>(ra, round);
+    VGX_LAUNCH_CHECK();
+    VGX_HIP(hipMemcpyAsync(p->h_push, p->d_push, sizeof(PushState), hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    if (first) st.touched_rows = p->h_push->acnt[2];
+    first = false;
+    const uint64_t E = p->h_push->q[round & 1] & kEdgeMask;
+    const uint64_t pv = p->h_push->pushed_vertices;
+    if (E == 0 && pv == last_pv) {
+      l1 = push_residual_l1<T>(p);
+      if (l1 <= tol) break;
+      theta /= 16.0;
+      if (theta < 1e-300) {
+        fallback = true;
+        break;
+      }
+      ++round;
+      seed(static_cast<int>(round % 3), base + 1 + round);
+      continue;
+    }
+    last_pv = pv;
+    if (pushed_arcs + E > budget) {
+      fallback = true;
+      break;
+    }
+    pushed_arcs += E;
+    k_push_spread<T><<<2 * grid, kPrBlock, 0, s>>>(ra, static_cast<int>(round & 1), static_cast<int>((round + 1) % 3),
+                                                  base + 2 + round);
+    VGX_LAUNCH_CHECK();
+    ++round;
+  }
+  st.rounds = round;
+  st.pushed_vertices = p->h_push->pushed_vertices;
+  st.pushed_arcs = pushed_arcs;
+  st.threshold = theta;
+  p->push_epoch = base + 2 + round;
+  // fold the uniform part: x /= 1 - k, r /= 1 - k, k = c n / (1 - d)
+  const double kc = p->h_push->c * static_cast<double>(n) / (1.0 - p->damping);
+  const double scale = 1.0 / (1.0 - kc);
+  if (!fallback) {
+    k_push_fold<T><<<gblocks(n, kPrBlock, 2048), kPrBlock, 0, s>>>(reinterpret_cast<T*>(p->rank[p->cur].p),
+                                                                  reinterpret_cast<T*>(p->res.p),
+                                                                  static_cast<uint32_t>(n), scale);
+    VGX_LAUNCH_CHECK();
+  }
+  p->res_valid = true;
+  p->res_version = g->version;
+  count_dangling(p);
+  reset_local_stats(p);
+  // the selector: a push phase over budget in its first round marks this
+  // batch size as one to skip; a push phase that finishes clears the mark
+  if (fallback && round == 0) {
+    p->push_skip_k = p->push_skip_k ? std::min(p->push_skip_k, k) : k;
+  } else if (!fallback) {
+    p->push_skip_k = 0;
+    p->push_skipped = 0;
+  }
+  if (fallback) {
+    st.used_fallback = 1;
+    VGX_HIP(hipMemcpyAsync(p->rank[0].p, p->push_save.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    p->cur = 0;
+    st.fallback_sweeps += push_sweeps<T>(p, max_sweeps, tol);
+    st.residual_l1 = p->h_st->l1;
+    p->stats.last_used_full_recompute = 1;
+    p->stats.last_repair_iterations = st.fallback_sweeps;
+    p->stats.last_repaired_vertices = n;
+    p->stats.last_full_recompute_converged = p->h_st->done == 1;
+  } else {
+    p->res_bound = theta * fabs(scale);
+    st.residual_l1 = l1 * fabs(scale);
+    p->stats.last_used_full_recompute = 0;
+    p->stats.last_repair_iterations = round;
+    p->stats.last_repaired_vertices = st.pushed_vertices;
+    p->stats.last_residual_l1 = st.residual_l1;
+  }
+  if (out) *out = st;
 }
 
 static vgx_pagerank_stats validate(vgx_pagerank* p, uint64_t ref_it, double ref_tol, double l1_tol, double linf_tol,
@@ -969,6 +1509,8 @@ vgx_status vgx_pagerank_destroy(vgx_pagerank* p) {
   if (p->g) (void)hipStreamSynchronize(p->g->stream);
   if (p->d_st) dev_free(p->d_st);
   if (p->h_st) (void)hipHostFree(p->h_st);
+  if (p->d_push) dev_free(p->d_push);
+  if (p->h_push) (void)hipHostFree(p->h_push);
   delete p;
   VGX_API_END
 }
@@ -1036,6 +1578,22 @@ vgx_status vgx_pagerank_apply(vgx_pagerank* p, const uint32_t* src, const uint32
   if (!p || (k && (!src || !dst || !add))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
   VGX_HIP(hipSetDevice(p->g->device));
   apply(p, src, dst, add, k, local_iterations, tol, full_fallback_fraction);
+  VGX_API_END
+}
+
+vgx_status vgx_pagerank_apply_push(vgx_pagerank* p, const uint32_t* src, const uint32_t* dst, const uint8_t* add,
+                                   uint64_t k, double l1_tol, double threshold, double max_push_fraction,
+                                   uint64_t max_sweeps, vgx_pagerank_push_stats* out) {
+  VGX_API_BEGIN
+  if (!p || (k && (!src || !dst || !add))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  if (!(l1_tol > 0.0)) throw Error(VGX_INVALID_ARGUMENT, "l1_tol must be positive");
+  if (!(threshold >= 0.0) || !(max_push_fraction >= 0.0))
+    throw Error(VGX_INVALID_ARGUMENT, "threshold and max_push_fraction must be non-negative");
+  VGX_HIP(hipSetDevice(p->g->device));
+  if (p->precision == 32)
+    apply_push<float>(p, src, dst, add, k, l1_tol, threshold, max_push_fraction, max_sweeps, out);
+  else
+    apply_push<double>(p, src, dst, add, k, l1_tol, threshold, max_push_fraction, max_sweeps, out);
   VGX_API_END
 }
 
@@ -1109,6 +1667,7 @@ vgx_status vgx_pagerank_install_reference(vgx_pagerank* p, uint64_t reference_ma
   p->cur = 0;
   p->n = n;
   p->have_values = true;
+  p->res_valid = false;
   count_dangling(p);
   p->stats.last_repaired_vertices = n;
   p->stats.last_repair_iterations = p->h_st->iterations;

@@ -234,6 +234,20 @@ int device_of(vgx_graph* g) {
 
 }  // namespace
 
+py::dict push_stats_dict(const vgx_pagerank_push_stats& st) {
+  py::dict d;
+  d["touched_rows"] = st.touched_rows;
+  d["rounds"] = st.rounds;
+  d["pushed_vertices"] = st.pushed_vertices;
+  d["pushed_arcs"] = st.pushed_arcs;
+  d["fallback_sweeps"] = st.fallback_sweeps;
+  d["used_fallback"] = st.used_fallback != 0;
+  d["skipped_push"] = st.skipped_push != 0;
+  d["residual_l1"] = st.residual_l1;
+  d["threshold"] = st.threshold;
+  return d;
+}
+
 PYBIND11_MODULE(_velographx, m) {
   m.doc() = "VeloGraphX traversal path on AMD Instinct MI355X (gfx950)";
 
@@ -743,7 +757,40 @@ PYBIND11_MODULE(_velographx, m) {
       .def("solve_l1", &IncrementalPageRank::solve_l1, py::arg("max_iterations") = 200, py::arg("l1_tol") = 1e-7,
            py::arg("warm_start") = false, py::call_guard<py::gil_scoped_release>())
       .def_property_readonly("last_used_full_recompute", &IncrementalPageRank::last_used_full_recompute)
-      .def_property_readonly("precision", &IncrementalPageRank::precision);
+      .def_property_readonly("precision", &IncrementalPageRank::precision)
+      .def(
+          "apply_push",
+          [](IncrementalPageRank& x, const UpdateBatch& batch, double l1_tol, double threshold,
+             double max_push_fraction, std::size_t max_sweeps) {
+            vgx_pagerank_push_stats st{};
+            {
+              py::gil_scoped_release nogil;
+              st = x.apply_push(batch, l1_tol, threshold, max_push_fraction, max_sweeps);
+            }
+            return push_stats_dict(st);
+          },
+          py::arg("batch"), py::arg("l1_tol") = 1e-7, py::arg("threshold") = 0.0, py::arg("max_push_fraction") = 0.05,
+          py::arg("max_sweeps") = 200)
+      .def(
+          "apply_push_arrays",
+          [](IncrementalPageRank& x, const U32Array& src, const U32Array& dst,
+             const py::array_t<std::uint8_t, py::array::c_style | py::array::forcecast>& add, double l1_tol,
+             double threshold, double max_push_fraction, std::size_t max_sweeps) {
+            if (src.size() != dst.size() || src.size() != add.size())
+              throw std::invalid_argument("src, dst and add must have equal length");
+            const auto* s = src.data();
+            const auto* d = dst.data();
+            const auto* a = add.data();
+            const auto k = static_cast<std::size_t>(src.size());
+            vgx_pagerank_push_stats st{};
+            {
+              py::gil_scoped_release nogil;
+              st = x.apply_push(s, d, a, k, l1_tol, threshold, max_push_fraction, max_sweeps);
+            }
+            return push_stats_dict(st);
+          },
+          py::arg("src"), py::arg("dst"), py::arg("add"), py::arg("l1_tol") = 1e-7, py::arg("threshold") = 0.0,
+          py::arg("max_push_fraction") = 0.05, py::arg("max_sweeps") = 200);
 
   // addition: adaptive repair-vs-recompute selection
   py::class_<velographx::AdaptivePolicy>(m, "AdaptivePolicy")
