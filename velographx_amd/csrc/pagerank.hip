@@ -84,11 +84,11 @@ struct vgx_pagerank {
   bool have_values = false;
   // per graph version
   uint64_t lists_version = ~0ull;
-  vgx::DevBuf<uint32_t> slist, mlist, hlist;
+  vgx::DevBuf<uint32_t> zlist, slist, mlist, hlist;
   vgx::DevBuf<uint32_t> hchunk_v, hchunk_off;  // chunk -> H index, chunk -> arc start
   vgx::DevBuf<uint32_t> hchunk_first;          // H index -> first chunk (size nh+1)
   vgx::DevBuf<uint32_t> class_counts, class_bases, class_tail;  // list building scratch
-  uint64_t ns = 0, nm = 0, nh = 0, nchunks = 0;
+  uint64_t nz = 0, ns = 0, nm = 0, nh = 0, nchunks = 0;
   vgx::DevBuf<uint8_t> rank[2], contrib[2], hpart, ref_rank[2];
   vgx::DevBuf<vgx::PrPartial> partials;
   vgx::PrState* d_st = nullptr;
@@ -146,14 +146,15 @@ struct PrArgs {
   const uint2* row;
   const uint32_t* arena;
   const uint32_t* outdeg;
+  const uint32_t* zlist;
   const uint32_t* slist;
   const uint32_t* mlist;
   const uint32_t* hlist;
   const uint32_t* hchunk_v;
   const uint32_t* hchunk_off;
   const uint32_t* hchunk_first;
-  uint32_t ns, nm, nh, nchunks;
-  uint32_t bs, bm;  // workgroups for S and M (H uses the rest)
+  uint32_t nz, ns, nm, nh, nchunks;
+  uint32_t bz, bs, bm;  // workgroups for Z, S and M (H uses the rest)
   uint32_t n;
   void* rank0;
   void* rank1;
@@ -199,10 +200,25 @@ __device__ void block_reduce3(double& a, double& b, double& c, double* sm) {
   }
 }
 
+// Lane's sum of contrib over the live entries e, e + 64, ... below end,
+// two entries per step with both loads issued before either is used; the
+// additions run in entry order, as a one-entry loop would add them.
+template <class T>
+__device__ __forceinline__ void sum_range(const uint32_t* __restrict__ arena, const T* __restrict__ contrib,
+                                          uint32_t e, uint32_t end, Acc<T>& acc) {
+  for (; e < end; e += 2 * kWave) {
+    const uint32_t x0 = arena[e];
+    const uint32_t x1 = e + kWave < end ? arena[e + kWave] : kTomb;
+    const T c0 = (x0 & kTomb) ? T(0) : contrib[x0];
+    const T c1 = (x1 & kTomb) ? T(0) : contrib[x1];
+    if (!(x0 & kTomb)) acc.add(c0);
+    if (!(x1 & kTomb)) acc.add(c1);
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(kPrBlock) void k_pr_iter(PrArgs a) {
   __shared__ double sm[3 * (kPrBlock / kWave)];
-  __shared__ int s_last;
   PrState* st = a.st;
   if (st->done) return;
   const int par = static_cast<int>(st->iterations & 1ull);
@@ -213,9 +229,16 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_iter(PrArgs a) {
   T* hpart = static_cast<T*>(a.hpart);
   const T base = static_cast<T>(st->teleport + st->dangling_share);
   double l1 = 0.0, linf = 0.0, dang = 0.0;
-  const uint32_t b = blockIdx.x;
+  const uint32_t b0 = blockIdx.x;
   // Each class is covered grid-stride by its own range of workgroups.
-  if (b < a.bs) {
+  if (b0 < a.bz) {
+    // no in-arcs: the next rank is the base, one vertex per thread
+    for (uint32_t i = b0 * blockDim.x + threadIdx.x; i < a.nz; i += a.bz * blockDim.x)
+      finish_vertex<T>(a, a.zlist[i], T(0), base, rank, next, cnext, l1, linf, dang);
+  }
+  const uint32_t b = b0 - a.bz;  // wraps for the Z workgroups, which match nothing below
+  if (b0 < a.bz) {
+  } else if (b < a.bs) {
     // Short rows by groups of kSmallGroup lanes: a group's loads of one row
     // are consecutive (coalesced), and a wave's lanes stay busy however the
     // row lengths vary, where a lane per row would send 64 scattered
@@ -245,15 +268,12 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_iter(PrArgs a) {
       const uint32_t v = a.mlist[i];
       Acc<T> acc;
       const uint2 rb = a.row[v];
-      for (uint32_t e = rb.x + lane_id(); e < rb.x + rb.y; e += kWave) {
-        const uint32_t x = a.arena[e];
-        if (!(x & kTomb)) acc.add(contrib[x]);
-      }
+      sum_range<T>(a.arena, contrib, rb.x + lane_id(), rb.x + rb.y, acc);
       T s = wave_sum(acc.get());
       if (lane_id() == 0) finish_vertex<T>(a, v, s, base, rank, next, cnext, l1, linf, dang);
     }
   } else {
-    const uint32_t bh = gridDim.x - a.bs - a.bm;
+    const uint32_t bh = gridDim.x - a.bz - a.bs - a.bm;
     for (uint32_t c = (b - a.bs - a.bm) * (kPrBlock / kWave) + threadIdx.x / kWave; c < a.nchunks;
          c += bh * (kPrBlock / kWave)) {
       const uint32_t hv = a.hlist[a.hchunk_v[c]];
@@ -261,35 +281,53 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_iter(PrArgs a) {
       const uint32_t e0 = a.hchunk_off[c];
       const uint32_t e1 = min(e0 + kChunk, rb.x + rb.y);
       Acc<T> acc;
-      for (uint32_t e = e0 + lane_id(); e < e1; e += kWave) {
-        const uint32_t x = a.arena[e];
-        if (!(x & kTomb)) acc.add(contrib[x]);
-      }
-      T s = wave_sum(acc.get());
+      sum_range<T>(a.arena, contrib, e0 + lane_id(), e1, acc);
+      const T s = wave_sum(acc.get());
       if (lane_id() == 0) hpart[c] = s;
     }
   }
+  // this workgroup's partials; k_pr_finish reduces them after this launch
+  block_reduce3<T>(l1, linf, dang, sm);
+  if (threadIdx.x == 0) a.partials[b0] = PrPartial{l1, linf, dang};
+}
+
+// After k_pr_iter (the kernel boundary makes its writes visible, so no
+// fence is needed): one thread per large row sums its chunk partials in
+// chunk order and finishes it; the last workgroup reduces the sweep's
+// partials (k_pr_iter's nprev, then this launch's) in a fixed order, sets
+// the next dangling share and the convergence flag.
+template <class T>
+__global__ __launch_bounds__(kPrBlock) void k_pr_finish(PrArgs a, uint32_t nprev) {
+  __shared__ double sm[3 * (kPrBlock / kWave)];
+  __shared__ int s_last;
+  PrState* st = a.st;
+  if (st->done) return;
+  const int par = static_cast<int>(st->iterations & 1ull);
+  const T* rank = static_cast<const T*>(par ? a.rank1 : a.rank0);
+  T* next = static_cast<T*>(par ? a.rank0 : a.rank1);
+  T* cnext = static_cast<T*>(par ? a.contrib0 : a.contrib1);
+  const T* hpart = static_cast<const T*>(a.hpart);
+  const T base = static_cast<T>(st->teleport + st->dangling_share);
+  double l1 = 0.0, linf = 0.0, dang = 0.0;
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < a.nh; h += gridDim.x * blockDim.x) {
+    Acc<T> acc;
+    for (uint32_t c = a.hchunk_first[h]; c < a.hchunk_first[h + 1]; ++c) acc.add(hpart[c]);
+    finish_vertex<T>(a, a.hlist[h], acc.get(), base, rank, next, cnext, l1, linf, dang);
+  }
   block_reduce3<T>(l1, linf, dang, sm);
   if (threadIdx.x == 0) {
-    a.partials[b] = PrPartial{l1, linf, dang};
+    a.partials[nprev + blockIdx.x] = PrPartial{l1, linf, dang};
     __threadfence();
     s_last = atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1;
   }
   __syncthreads();
   if (!s_last) return;
   __threadfence();
-  // last workgroup: finish the large rows, then reduce in a fixed order
+  // every partial, thread t taking t, t + 256, ... in order, then the
+  // fixed-order block reduction (the result does not depend on timing)
   double hl1 = 0.0, hlinf = 0.0, hdang = 0.0;
-  for (uint32_t h = threadIdx.x; h < a.nh; h += blockDim.x) {
-    const uint32_t v = a.hlist[h];
-    Acc<T> acc;
-    for (uint32_t c = a.hchunk_first[h]; c < a.hchunk_first[h + 1]; ++c)
-      acc.add(__hip_atomic_load(&hpart[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    finish_vertex<T>(a, v, acc.get(), base, rank, next, cnext, hl1, hlinf, hdang);
-  }
-  // per-workgroup partials: thread t takes t, t + 256, ... in order, then
-  // the fixed-order block reduction (the result does not depend on timing)
-  for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+  const uint32_t total = nprev + gridDim.x;
+  for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
     hl1 += __hip_atomic_load(&a.partials[i].l1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const double pl = __hip_atomic_load(&a.partials[i].linf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     hlinf = hlinf > pl ? hlinf : pl;
@@ -347,66 +385,67 @@ __global__ __launch_bounds__(kPrBlock) void k_pr_init(PrArgs a, int warm) {
   }
 }
 
-// Split vertices by in-degree into three lists kept in vertex order.  Wave
-// w owns 64-vertex groups [w*gpw, (w+1)*gpw): k_pr_class_count counts its
-// members per class, an exclusive scan over [S counts | M counts | H counts]
-// gives every wave its write positions, k_pr_class_write fills the lists.
-// No atomics, and the lists come out sorted.
+// Split vertices by in-degree into four lists kept in vertex order: Z (no
+// in-arcs: their next rank is the base alone), S (at most kSmallRow stored
+// entries), M (at most kChunk), H (longer, chunked).  Wave w owns 64-vertex
+// groups [w*gpw, (w+1)*gpw): k_pr_class_count counts its members per
+// class, an exclusive scan over [Z counts | S | M | H] gives every wave its
+// write positions, k_pr_class_write fills the lists.  No atomics, and the
+// lists come out sorted.
 constexpr uint32_t kClassWaves = 4096;
+constexpr uint32_t kClasses = 4;
 
-__device__ __forceinline__ void class_ballots(const uint2* row, uint32_t n, uint32_t g, unsigned long long& bs,
-                                              unsigned long long& bm, unsigned long long& bh) {
+__device__ __forceinline__ void class_ballots(const uint2* row, uint32_t n, uint32_t g, unsigned long long* b) {
   const uint32_t v = g * kWave + lane_id();
   const uint32_t d = v < n ? row[v].y : 0u;  // stored entries, tombstones included
-  bs = __ballot(v < n && d <= kSmallRow);
-  bm = __ballot(v < n && d > kSmallRow && d <= kChunk);
-  bh = __ballot(v < n && d > kChunk);
+  b[0] = __ballot(v < n && d == 0);
+  b[1] = __ballot(v < n && d > 0 && d <= kSmallRow);
+  b[2] = __ballot(v < n && d > kSmallRow && d <= kChunk);
+  b[3] = __ballot(v < n && d > kChunk);
 }
 
 __global__ void k_pr_class_count(const uint2* __restrict__ row, uint32_t n, uint32_t gpw,
                                  uint32_t* __restrict__ counts) {
   const uint32_t w = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   const uint32_t groups = (n + kWave - 1) / kWave;
-  uint32_t cs = 0, cm = 0, ch = 0;
+  uint32_t c[kClasses] = {0, 0, 0, 0};
   for (uint32_t g = w * gpw; g < min((w + 1) * gpw, groups); ++g) {
-    unsigned long long bs, bm, bh;
-    class_ballots(row, n, g, bs, bm, bh);
-    cs += __popcll(bs);
-    cm += __popcll(bm);
-    ch += __popcll(bh);
+    unsigned long long b[kClasses];
+    class_ballots(row, n, g, b);
+#pragma unroll
+    for (uint32_t k = 0; k < kClasses; ++k) c[k] += __popcll(b[k]);
   }
-  if (lane_id() == 0) {
-    counts[w] = cs;
-    counts[kClassWaves + w] = cm;
-    counts[2 * kClassWaves + w] = ch;
-  }
+  if (lane_id() == 0)
+#pragma unroll
+    for (uint32_t k = 0; k < kClasses; ++k) counts[k * kClassWaves + w] = c[k];
 }
 
 __global__ void k_pr_class_write(const uint2* __restrict__ row, uint32_t n, uint32_t gpw,
-                                 const uint32_t* __restrict__ bases, uint32_t* __restrict__ slist,
-                                 uint32_t* __restrict__ mlist, uint32_t* __restrict__ hlist) {
+                                 const uint32_t* __restrict__ bases, uint32_t* __restrict__ zlist,
+                                 uint32_t* __restrict__ slist, uint32_t* __restrict__ mlist,
+                                 uint32_t* __restrict__ hlist) {
   const uint32_t w = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   const uint32_t groups = (n + kWave - 1) / kWave;
-  uint32_t os = bases[w];
-  uint32_t om = bases[kClassWaves + w] - bases[kClassWaves];
-  uint32_t oh = bases[2 * kClassWaves + w] - bases[2 * kClassWaves];
+  uint32_t* lists[kClasses] = {zlist, slist, mlist, hlist};
+  uint32_t o[kClasses];
+#pragma unroll
+  for (uint32_t k = 0; k < kClasses; ++k) o[k] = bases[k * kClassWaves + w] - bases[k * kClassWaves];
   for (uint32_t g = w * gpw; g < min((w + 1) * gpw, groups); ++g) {
-    unsigned long long bs, bm, bh;
-    class_ballots(row, n, g, bs, bm, bh);
+    unsigned long long b[kClasses];
+    class_ballots(row, n, g, b);
     const uint32_t v = g * kWave + lane_id();
-    if ((bs >> lane_id()) & 1ull) slist[os + lanes_below(bs)] = v;
-    if ((bm >> lane_id()) & 1ull) mlist[om + lanes_below(bm)] = v;
-    if ((bh >> lane_id()) & 1ull) hlist[oh + lanes_below(bh)] = v;
-    os += __popcll(bs);
-    om += __popcll(bm);
-    oh += __popcll(bh);
+#pragma unroll
+    for (uint32_t k = 0; k < kClasses; ++k) {
+      if ((b[k] >> lane_id()) & 1ull) lists[k][o[k] + lanes_below(b[k])] = v;
+      o[k] += __popcll(b[k]);
+    }
   }
 }
 
 // Chunk table of the large rows (one workgroup of kHubThreads, each a
 // contiguous run of H rows): hub h owns chunks [first[h], first[h+1]), chunk
-// c starts at arena entry off[c] of row hlist[cv[c]].  tail[0..3] receive
-// the class list sizes (S, M, H) and the chunk count for the host.
+// c starts at arena entry off[c] of row hlist[cv[c]].  tail[0..4] receive
+// the class list sizes (Z, S, M, H) and the chunk count for the host.
 constexpr uint32_t kHubThreads = 1024;
 __global__ __launch_bounds__(kHubThreads) void k_pr_hub_chunks(const uint2* __restrict__ row,
                                                                const uint32_t* __restrict__ hlist,
@@ -417,8 +456,8 @@ __global__ __launch_bounds__(kHubThreads) void k_pr_hub_chunks(const uint2* __re
   constexpr uint32_t kW = kHubThreads / kWave;
   __shared__ unsigned long long s_tot[kW];
   const uint32_t W = kClassWaves;
-  const uint32_t ns = bases[W], nm = bases[2 * W] - bases[W];
-  const uint32_t nh = bases[3 * W - 1] + counts[3 * W - 1] - bases[2 * W];
+  const uint32_t nz = bases[W], ns = bases[2 * W] - bases[W], nm = bases[3 * W] - bases[2 * W];
+  const uint32_t nh = bases[4 * W - 1] + counts[4 * W - 1] - bases[3 * W];
   const uint32_t per = (nh + kHubThreads - 1) / kHubThreads;
   const uint32_t h0 = min(nh, threadIdx.x * per), h1 = min(nh, h0 + per);
   unsigned long long mine = 0;
@@ -444,10 +483,11 @@ __global__ __launch_bounds__(kHubThreads) void k_pr_hub_chunks(const uint2* __re
   }
   if (threadIdx.x == 0) {
     first[nh] = static_cast<uint32_t>(total);
-    tail[0] = ns;
-    tail[1] = nm;
-    tail[2] = nh;
-    tail[3] = static_cast<uint32_t>(total);
+    tail[0] = nz;
+    tail[1] = ns;
+    tail[2] = nm;
+    tail[3] = nh;
+    tail[4] = static_cast<uint32_t>(total);
   }
 }
 
@@ -862,25 +902,26 @@ static void build_lists(vgx_pagerank* p) {
   p->hchunk_v.ensure(max_chunks);
   p->hchunk_off.ensure(max_chunks);
   p->hchunk_first.ensure(std::min<uint64_t>(n, in.arena.n / kChunk + 1) + 1);
-  p->class_tail.ensure(4);
-  uint32_t tail[4] = {0, 0, 0, 0};
+  p->zlist.ensure(n + 1);
+  p->class_tail.ensure(kClasses + 1);
+  uint32_t tail[kClasses + 1] = {0, 0, 0, 0, 0};
   if (n) {
     const uint32_t W = kClassWaves;
     const uint32_t groups = static_cast<uint32_t>((n + kWave - 1) / kWave);
     const uint32_t gpw = (groups + W - 1) / W;
-    p->class_counts.ensure(3 * W);
-    p->class_bases.ensure(3 * W);
+    p->class_counts.ensure(kClasses * W);
+    p->class_bases.ensure(kClasses * W);
     const unsigned blocks = W / (kPrBlock / kWave);
     k_pr_class_count<<<blocks, kPrBlock, 0, s>>>(in.row.p, static_cast<uint32_t>(n), gpw, p->class_counts.p);
     VGX_LAUNCH_CHECK();
     size_t bytes = 0;
     VGX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, p->class_counts.p, p->class_bases.p,
-                                             static_cast<int>(3 * W), s));
+                                             static_cast<int>(kClasses * W), s));
     p->scan_tmp.ensure(bytes + 16);
     VGX_HIP(hipcub::DeviceScan::ExclusiveSum(p->scan_tmp.p, bytes, p->class_counts.p, p->class_bases.p,
-                                             static_cast<int>(3 * W), s));
+                                             static_cast<int>(kClasses * W), s));
     k_pr_class_write<<<blocks, kPrBlock, 0, s>>>(in.row.p, static_cast<uint32_t>(n), gpw, p->class_bases.p,
-                                                 p->slist.p, p->mlist.p, p->hlist.p);
+                                                 p->zlist.p, p->slist.p, p->mlist.p, p->hlist.p);
     VGX_LAUNCH_CHECK();
     k_pr_hub_chunks<<<1, kHubThreads, 0, s>>>(in.row.p, p->hlist.p, p->class_counts.p, p->class_bases.p,
                                               p->hchunk_v.p, p->hchunk_off.p, p->hchunk_first.p,
@@ -889,10 +930,11 @@ static void build_lists(vgx_pagerank* p) {
     VGX_HIP(hipMemcpyAsync(tail, p->class_tail.p, sizeof(tail), hipMemcpyDeviceToHost, s));
     VGX_HIP(hipStreamSynchronize(s));
   }
-  p->ns = tail[0];
-  p->nm = tail[1];
-  p->nh = tail[2];
-  p->nchunks = tail[3];
+  p->nz = tail[0];
+  p->ns = tail[1];
+  p->nm = tail[2];
+  p->nh = tail[3];
+  p->nchunks = tail[4];
   p->hpart.ensure((p->nchunks + 1) * 8);
   p->lists_version = g->version;
 }
@@ -903,16 +945,19 @@ static PrArgs pr_args(vgx_pagerank* p, DevBuf<uint8_t>* rk) {
   a.row = g->in().row.p;
   a.arena = g->in().arena.p;
   a.outdeg = g->adj[0].deg.p;
+  a.zlist = p->zlist.p;
   a.slist = p->slist.p;
   a.mlist = p->mlist.p;
   a.hlist = p->hlist.p;
   a.hchunk_v = p->hchunk_v.p;
   a.hchunk_off = p->hchunk_off.p;
   a.hchunk_first = p->hchunk_first.p;
+  a.nz = static_cast<uint32_t>(p->nz);
   a.ns = static_cast<uint32_t>(p->ns);
   a.nm = static_cast<uint32_t>(p->nm);
   a.nh = static_cast<uint32_t>(p->nh);
   a.nchunks = static_cast<uint32_t>(p->nchunks);
+  a.bz = static_cast<uint32_t>(std::min<uint64_t>((p->nz + kPrBlock - 1) / kPrBlock, kMaxClassBlocks));
   a.bs = static_cast<uint32_t>(
       std::min<uint64_t>((p->ns + kPrBlock / kSmallGroup - 1) / (kPrBlock / kSmallGroup), kMaxClassBlocks));
   a.bm = static_cast<uint32_t>(std::min<uint64_t>((p->nm + 3) / 4, kMaxClassBlocks));
@@ -948,10 +993,11 @@ static int full_solve(vgx_pagerank* p, DevBuf<uint8_t>* rk, int precision, uint6
   h.done = (n == 0 || max_iterations == 0) ? 1u : 0u;
   VGX_HIP(hipMemcpyAsync(p->d_st, &h, sizeof(h), hipMemcpyHostToDevice, s));
   PrArgs a = pr_args(p, rk);
-  const unsigned grid =
-      a.bs + a.bm + static_cast<uint32_t>(std::min<uint64_t>((p->nchunks + 3) / 4, kMaxClassBlocks));
+  const unsigned grid = std::max<unsigned>(
+      1, a.bz + a.bs + a.bm + static_cast<uint32_t>(std::min<uint64_t>((p->nchunks + 3) / 4, kMaxClassBlocks)));
   const unsigned ginit = gblocks(n, kPrBlock, 2048);
-  p->partials.ensure(std::max<unsigned>(grid, ginit) + 1);
+  const unsigned gfin = gblocks(std::max<uint64_t>(p->nh, 1), kPrBlock, 64);
+  p->partials.ensure(std::max<unsigned>(grid + gfin, ginit) + 1);
   a.partials = p->partials.p;
   if (n == 0) {
     VGX_HIP(hipStreamSynchronize(s));
@@ -967,8 +1013,13 @@ static int full_solve(vgx_pagerank* p, DevBuf<uint8_t>* rk, int precision, uint6
   while (launched < max_iterations) {
     uint64_t c = std::min<uint64_t>(chunk, max_iterations - launched);
     for (uint64_t i = 0; i < c; ++i) {
-      if (precision == 32) k_pr_iter<float><<<grid, kPrBlock, 0, s>>>(a);
-      else k_pr_iter<double><<<grid, kPrBlock, 0, s>>>(a);
+      if (precision == 32) {
+        k_pr_iter<float><<<grid, kPrBlock, 0, s>>>(a);
+        k_pr_finish<float><<<gfin, kPrBlock, 0, s>>>(a, grid);
+      } else {
+        k_pr_iter<double><<<grid, kPrBlock, 0, s>>>(a);
+        k_pr_finish<double><<<gfin, kPrBlock, 0, s>>>(a, grid);
+      }
       VGX_LAUNCH_CHECK();
     }
     launched += c;
