@@ -1,0 +1,4 @@
+// include/velographx/algorithms.hpp of the reference, served by the gfx950 facade.
+#pragma once
+
+#include "velographx_amd/algorithms.hpp"

@@ -1,26 +1,32 @@
 // Incremental connected components -- the interface of
-// include/velographx/incremental/connected_components.hpp
-// (BasicIncrementalComponents<DynamicGraph>) over the device engine
-// (vgx_components_*).  component(v) is the smallest vertex id of v's
-// component: the reference's partition, with an order-free representative
-// instead of its union-find root.
+// include/velographx/incremental/connected_components.hpp:16-157
+// (BasicIncrementalComponents<Graph>; IncrementalComponents is the
+// DynamicGraph instance, :157) over the device engine (vgx_components_*),
+// on the device graph GraphBinding<Graph> supplies.  component(v) is the
+// smallest vertex id of v's component: the reference's partition, with an
+// order-free representative instead of its union-find root.
 #pragma once
 
+#include <concepts>
 #include <cstdint>
 #include <vector>
 
-#include "velographx_amd/velographx.hpp"
+#include "velographx_amd/dynamic_graph.hpp"
+#include "velographx_amd/graph_binding.hpp"
 
 namespace velographx {
 
-class IncrementalComponents {
+template <class Graph>
+class BasicIncrementalComponents {
  public:
-  explicit IncrementalComponents(DynamicGraph& g) { vgx_check(vgx_components_create(g.handle(), &c_)); }
-  ~IncrementalComponents() {
+  explicit BasicIncrementalComponents(Graph& g) : bind_(g) {
+    vgx_check(vgx_components_create(bind_.device().handle(), &c_));
+  }
+  ~BasicIncrementalComponents() {
     if (c_) vgx_components_destroy(c_);
   }
-  IncrementalComponents(const IncrementalComponents&) = delete;
-  IncrementalComponents& operator=(const IncrementalComponents&) = delete;
+  BasicIncrementalComponents(const BasicIncrementalComponents&) = delete;
+  BasicIncrementalComponents& operator=(const BasicIncrementalComponents&) = delete;
 
   // connected_components.hpp:27-30
   [[nodiscard]] std::uint32_t component(VertexId v) const {
@@ -41,19 +47,29 @@ class IncrementalComponents {
     return r;
   }
 
+  // connected_components.hpp:35-119
   void apply(const UpdateBatch& batch) {
-    auto c = batch.columns();
+    auto c = bind_.forward(batch).columns();
     vgx_check(vgx_components_apply(c_, c.src.data(), c.dst.data(), c.add.data(), c.src.size()));
   }
-  void apply_arrays(const std::uint32_t* src, const std::uint32_t* dst, const std::uint8_t* add, std::size_t k) {
+  void apply_arrays(const std::uint32_t* src, const std::uint32_t* dst, const std::uint8_t* add, std::size_t k)
+    requires std::same_as<Graph, DynamicGraph>
+  {
     vgx_check(vgx_components_apply(c_, src, dst, add, k));
   }
-  void recompute() { vgx_check(vgx_components_recompute(c_)); }
+  void recompute() {
+    bind_.refresh();
+    vgx_check(vgx_components_recompute(c_));
+  }
 
   [[nodiscard]] vgx_components* handle() const noexcept { return c_; }
 
  private:
+  detail::GraphBinding<Graph> bind_;
   vgx_components* c_ = nullptr;
 };
+
+// connected_components.hpp:157
+using IncrementalComponents = BasicIncrementalComponents<DynamicGraph>;
 
 }  // namespace velographx

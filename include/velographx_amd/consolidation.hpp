@@ -13,7 +13,7 @@
 #include <cstdint>
 #include <utility>
 
-#include "velographx_amd/velographx.hpp"
+#include "velographx_amd/dynamic_graph.hpp"
 
 namespace velographx {
 

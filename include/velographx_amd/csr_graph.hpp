@@ -1,13 +1,12 @@
-// Static CSR graph and its one-shot algorithms on the device -- the
-// interface of include/velographx/csr_graph.hpp and the bfs_distances /
-// pagerank entries of include/velographx/algorithms.hpp:8-10
-// (src/csr_graph.cpp, src/algorithms.cpp:11-27, 56-81).
+// Static CSR graph on the device -- the interface of
+// include/velographx/csr_graph.hpp (src/csr_graph.cpp).
 //
 // A CsrGraph is a device graph built once by the bulk load (self-loops
 // dropped, duplicates removed, both directions kept, vertex count = largest
-// id + 1) and never updated; bfs_distances runs the direction-optimizing
-// BFS kernels and pagerank the PageRank sweep kernels of the dynamic graph
-// over its rows.  Row accessors return copies (the rows live in HBM).
+// id + 1) and never updated; the algorithms run on its rows with the
+// kernels of the dynamic graph (algorithms.hpp, and the BasicIncremental*
+// templates bound to a CsrGraph).  Row accessors return copies (the rows
+// live in HBM).
 #pragma once
 
 #include <cstddef>
@@ -16,7 +15,7 @@
 #include <utility>
 #include <vector>
 
-#include "velographx_amd/velographx.hpp"
+#include "velographx_amd/dynamic_graph.hpp"
 
 namespace velographx {
 
@@ -63,27 +62,5 @@ class CsrGraph {
   }
   mutable DynamicGraph g_{0, false};
 };
-
-// algorithms.cpp:11-27: hop distances from source (kUnreachable where not
-// reached); a source outside the graph throws std::out_of_range.
-inline std::vector<std::uint32_t> bfs_distances(const CsrGraph& graph, VertexId source) {
-  if (source >= graph.vertex_count()) throw std::out_of_range("source outside graph");
-  IncrementalBFS bfs(graph.device_graph(), source);
-  return bfs.distances();
-}
-
-// algorithms.cpp:56-81: power iteration from 1/n with uniform teleport and
-// the dangling mass spread evenly, stopping once the L1 change is below
-// `tolerance`; damping outside [0, 1] throws std::invalid_argument.
-inline std::vector<double> pagerank(const CsrGraph& graph, double damping = 0.85, std::size_t max_iterations = 100,
-                                    double tolerance = 1e-10) {
-  if (!(damping >= 0.0 && damping <= 1.0)) throw std::invalid_argument("PageRank damping must be in [0, 1]");
-  std::vector<double> ranks(graph.vertex_count());
-  std::uint64_t count = 0;
-  vgx_check(vgx_pagerank_static(graph.device_graph().handle(), damping, max_iterations, tolerance, ranks.data(),
-                                ranks.size(), &count, nullptr));
-  ranks.resize(count);
-  return ranks;
-}
 
 }  // namespace velographx

@@ -1,23 +1,28 @@
-// Incremental k-core -- the interface of include/velographx/incremental/kcore.hpp
-// (BasicIncrementalKCore<DynamicGraph>) over the device engine's k-core
-// (vgx_kcore_*): core numbers by level-synchronous peeling on the live rows.
+// Incremental k-core -- the interface of
+// include/velographx/incremental/kcore.hpp:17-127 (BasicIncrementalKCore<Graph>;
+// IncrementalKCore is the DynamicGraph instance, :127) over the device
+// engine's k-core (vgx_kcore_*): core numbers by level-synchronous peeling
+// on the live rows of the device graph GraphBinding<Graph> supplies.
 #pragma once
 
+#include <concepts>
 #include <cstdint>
 #include <vector>
 
-#include "velographx_amd/velographx.hpp"
+#include "velographx_amd/dynamic_graph.hpp"
+#include "velographx_amd/graph_binding.hpp"
 
 namespace velographx {
 
-class IncrementalKCore {
+template <class Graph>
+class BasicIncrementalKCore {
  public:
-  explicit IncrementalKCore(DynamicGraph& g) { vgx_check(vgx_kcore_create(g.handle(), &k_)); }
-  ~IncrementalKCore() {
+  explicit BasicIncrementalKCore(Graph& g) : bind_(g) { vgx_check(vgx_kcore_create(bind_.device().handle(), &k_)); }
+  ~BasicIncrementalKCore() {
     if (k_) vgx_kcore_destroy(k_);
   }
-  IncrementalKCore(const IncrementalKCore&) = delete;
-  IncrementalKCore& operator=(const IncrementalKCore&) = delete;
+  BasicIncrementalKCore(const BasicIncrementalKCore&) = delete;
+  BasicIncrementalKCore& operator=(const BasicIncrementalKCore&) = delete;
 
   // kcore.hpp:28 (a copy: the values live in HBM)
   [[nodiscard]] std::vector<std::uint32_t> core() const {
@@ -33,19 +38,29 @@ class IncrementalKCore {
     return r;
   }
 
+  // kcore.hpp:31-64
   void apply(const UpdateBatch& batch) {
-    auto c = batch.columns();
+    auto c = bind_.forward(batch).columns();
     vgx_check(vgx_kcore_apply(k_, c.src.data(), c.dst.data(), c.add.data(), c.src.size()));
   }
-  void apply_arrays(const std::uint32_t* src, const std::uint32_t* dst, const std::uint8_t* add, std::size_t k) {
+  void apply_arrays(const std::uint32_t* src, const std::uint32_t* dst, const std::uint8_t* add, std::size_t k)
+    requires std::same_as<Graph, DynamicGraph>
+  {
     vgx_check(vgx_kcore_apply(k_, src, dst, add, k));
   }
-  void recompute() { vgx_check(vgx_kcore_recompute(k_)); }
+  void recompute() {
+    bind_.refresh();
+    vgx_check(vgx_kcore_recompute(k_));
+  }
 
   [[nodiscard]] vgx_kcore* handle() const noexcept { return k_; }
 
  private:
+  detail::GraphBinding<Graph> bind_;
   vgx_kcore* k_ = nullptr;
 };
+
+// kcore.hpp:127
+using IncrementalKCore = BasicIncrementalKCore<DynamicGraph>;
 
 }  // namespace velographx

@@ -1,26 +1,51 @@
-// IncrementalPageRank facade (incremental/pagerank.hpp:29-336) over the
+// BasicIncrementalPageRank<Graph> -- the interface of
+// include/velographx/incremental/pagerank.hpp:29-336 (IncrementalPageRank
+// is BasicIncrementalPageRank<DynamicGraph>, pagerank.hpp:334) over the
 // gfx950 C ABI.  precision 64 keeps the reference's FP64 arithmetic and
-// stopping rules; precision 32 runs the same iteration in float.
+// stopping rules; precision 32 runs the same iteration in float.  The
+// sweeps run on the device graph GraphBinding<Graph> supplies (bfs.hpp
+// says which).
 #pragma once
 
+#include <concepts>
 #include <cstdint>
 #include <vector>
 
-#include "velographx_amd/velographx.hpp"
+#include "velographx_amd/dynamic_graph.hpp"
+#include "velographx_amd/graph_binding.hpp"
 #include "vgx.h"
 
 namespace velographx {
 
-class IncrementalPageRank {
+// pagerank.hpp:16-27
+struct PageRankValidation {
+  double l1_error{0.0};
+  double linf_error{0.0};
+  double local_residual_l1{0.0};
+  double local_residual_linf{0.0};
+  double reference_residual_l1{0.0};
+  double reference_residual_linf{0.0};
+  std::size_t reference_iterations{0};
+  bool reference_converged{false};
+  bool within_tolerance{false};
+  bool fallback_applied{false};
+};
+
+template <class Graph>
+class BasicIncrementalPageRank {
  public:
-  explicit IncrementalPageRank(DynamicGraph& g, double damping = 0.85, int precision = 64) : precision_(precision) {
-    vgx_check(vgx_pagerank_create(g.handle(), damping, precision, &p_));
+  // pagerank.hpp:31-37
+  explicit BasicIncrementalPageRank(Graph& g, double damping = 0.85, int precision = 64)
+      : bind_(g), precision_(precision) {
+    vgx_check(vgx_pagerank_create(bind_.device().handle(), damping, precision, &p_));
   }
-  ~IncrementalPageRank() {
+  ~BasicIncrementalPageRank() {
     if (p_) vgx_pagerank_destroy(p_);
   }
-  IncrementalPageRank(const IncrementalPageRank&) = delete;
-  IncrementalPageRank& operator=(const IncrementalPageRank&) = delete;
+  BasicIncrementalPageRank(const BasicIncrementalPageRank&) = delete;
+  BasicIncrementalPageRank& operator=(const BasicIncrementalPageRank&) = delete;
+
+  // pagerank.hpp:39 (a copy: the values live in HBM)
 
   [[nodiscard]] std::vector<double> values() const {
     std::uint64_t n = 0;
@@ -37,14 +62,16 @@ class IncrementalPageRank {
   [[nodiscard]] bool last_used_full_recompute() const { return stats().last_used_full_recompute != 0; }
   [[nodiscard]] int precision() const noexcept { return precision_; }
 
+  // pagerank.hpp:47-191
   void apply(const UpdateBatch& batch, std::size_t local_iterations = 24, double tol = 1e-9,
              double full_fallback_fraction = 0.60) {
-    auto c = batch.columns();
+    auto c = bind_.forward(batch).columns();
     vgx_check(vgx_pagerank_apply(p_, c.src.data(), c.dst.data(), c.add.data(), c.src.size(), local_iterations, tol,
                                  full_fallback_fraction));
   }
 
   void recompute(std::size_t max_iterations = 200, double tol = 1e-12) {
+    bind_.refresh();
     vgx_check(vgx_pagerank_recompute(p_, max_iterations, tol));
   }
 
@@ -56,16 +83,20 @@ class IncrementalPageRank {
   // addition: localized residual-push repair with a full-sweep fallback
   vgx_pagerank_push_stats apply_push(const std::uint32_t* src, const std::uint32_t* dst, const std::uint8_t* add,
                                      std::size_t k, double l1_tol = 1e-7, double threshold = 0.0,
-                                     double max_push_fraction = 0.05, std::size_t max_sweeps = 200) {
+                                     double max_push_fraction = 0.05, std::size_t max_sweeps = 200)
+    requires std::same_as<Graph, DynamicGraph>
+  {
     vgx_pagerank_push_stats st{};
     vgx_check(vgx_pagerank_apply_push(p_, src, dst, add, k, l1_tol, threshold, max_push_fraction, max_sweeps, &st));
     return st;
   }
   vgx_pagerank_push_stats apply_push(const UpdateBatch& batch, double l1_tol = 1e-7, double threshold = 0.0,
                                      double max_push_fraction = 0.05, std::size_t max_sweeps = 200) {
-    auto c = batch.columns();
-    return apply_push(c.src.data(), c.dst.data(), c.add.data(), c.src.size(), l1_tol, threshold, max_push_fraction,
-                      max_sweeps);
+    auto c = bind_.forward(batch).columns();
+    vgx_pagerank_push_stats st{};
+    vgx_check(vgx_pagerank_apply_push(p_, c.src.data(), c.dst.data(), c.add.data(), c.src.size(), l1_tol, threshold,
+                                      max_push_fraction, max_sweeps, &st));
+    return st;
   }
 
   [[nodiscard]] PageRankValidation validate_against_full(std::size_t reference_max_iterations = 500,
@@ -111,8 +142,12 @@ class IncrementalPageRank {
   [[nodiscard]] vgx_pagerank* handle() const noexcept { return p_; }
 
  private:
+  detail::GraphBinding<Graph> bind_;
   vgx_pagerank* p_ = nullptr;
   int precision_ = 64;
 };
+
+// pagerank.hpp:334
+using IncrementalPageRank = BasicIncrementalPageRank<DynamicGraph>;
 
 }  // namespace velographx

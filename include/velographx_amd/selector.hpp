@@ -6,7 +6,7 @@
 #include <cstdint>
 #include <string>
 
-#include "velographx.hpp"
+#include "velographx_amd/bfs.hpp"
 
 namespace velographx {
 

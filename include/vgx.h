@@ -115,6 +115,33 @@ typedef struct {
 } vgx_graph_storage;
 vgx_status vgx_graph_storage_info(vgx_graph* g, vgx_graph_storage* out);
 
+/* DynamicGraph::ensure_vertex -- dynamic_graph.hpp:493-500: grows the
+ * vertex count to v + 1 when it is smaller (no arcs, version unchanged). */
+vgx_status vgx_graph_ensure_vertex(vgx_graph* g, uint32_t v);
+
+/* Delta introspection -- base_edge_count_directed, delta_edge_count,
+ * delta_ratio, dirty_out/in_segment_count, is_compact
+ * (dynamic_graph.hpp:569-618).  Rows are updated in place here, so a
+ * row's delta is read off its state against its last bulk load or
+ * compaction: entries inserted since plus tombstones (an arc inserted and
+ * deleted again since counts twice, where the reference's overlay drops
+ * it; a base arc deleted and re-added counts nothing, as there). */
+typedef struct {
+  uint64_t base_arcs;      /* out-row entries at each row's last bulk load or compaction */
+  uint64_t delta_arcs;     /* out-row delta: inserted entries + tombstones */
+  uint64_t delta_in_arcs;  /* the same over the in-rows (= delta_arcs when undirected) */
+  uint64_t dirty_out_rows; /* out-rows with a nonzero delta */
+  uint64_t dirty_in_rows;  /* in-rows with a nonzero delta (= dirty_out_rows when undirected) */
+} vgx_graph_delta;
+vgx_status vgx_graph_delta_info(vgx_graph* g, vgx_graph_delta* out);
+
+/* DynamicGraph::maybe_compact -- dynamic_graph.hpp:620-631, 764-789: every
+ * row (each direction) whose delta is at least threshold * max(8, base
+ * length) drops its tombstones in place and becomes its own base; other
+ * rows keep their delta.  *compacted (may be null) = 1 when any row was
+ * compacted.  The logical graph and version are unchanged. */
+vgx_status vgx_graph_maybe_compact(vgx_graph* g, double threshold, int* compacted);
+
 /* consolidate_to_csr_snapshot -- storage/consolidation.hpp:132-163: a new
  * graph (*out, owned by the caller) holding the current logical graph in
  * compacted rows (no tombstones, no relocated rows), version 1, on the
@@ -445,16 +472,17 @@ vgx_status vgx_kcore_create(vgx_graph* g, vgx_kcore** out);
 vgx_status vgx_kcore_destroy(vgx_kcore* kc);
 /* recompute() -- kcore.hpp:58-63. */
 vgx_status vgx_kcore_recompute(vgx_kcore* kc);
-/* apply(batch) -- kcore.hpp:31-56: applies the batch to the graph and leaves
- * the core numbers of the new graph (the reference repairs the touched
- * components; the values are the same).  k = 0 changes nothing and sets
- * last_repaired_vertices to 0, as the reference's empty batch does. */
+/* apply(batch) -- kcore.hpp:31-56: applies the batch to the graph and
+ * re-peels the reference's repair region, the components of the batch's
+ * endpoints before and after the update (the other vertices keep their core
+ * numbers).  k = 0 changes nothing and sets last_repaired_vertices to 0, as
+ * the reference's empty batch does. */
 vgx_status vgx_kcore_apply(vgx_kcore* kc, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k);
 /* core() -- kcore.hpp:28: copies min(cap, n) core numbers, *count = n. */
 vgx_status vgx_kcore_core(vgx_kcore* kc, uint32_t* out, uint64_t cap, uint64_t* count);
-/* last_repaired_vertices() -- kcore.hpp:29 (here: the vertex count after a
- * non-empty batch or recompute, 0 after an empty batch), and the peeling's
- * levels and rounds of the last computation. */
+/* last_repaired_vertices() -- kcore.hpp:29: the vertices re-peeled by the
+ * last apply (its region) or recompute (all); and the level loop's levels
+ * (levels whose scan claimed vertices) and peeling rounds. */
 vgx_status vgx_kcore_stats(vgx_kcore* kc, uint64_t* last_repaired_vertices, uint32_t* levels, uint32_t* rounds);
 
 /* ------------------------------------------------------------ components */

@@ -206,3 +206,35 @@ def test_controller_driven_cutover(vx, orc):
             cutovers += 1
         check_model(g, mg)
     assert cutovers >= 1
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_row_compaction_keeps_graph_and_objects(vx, orc, directed):
+    """maybe_compact (dynamic_graph.hpp:620-631) at falling thresholds on a
+    churned R-MAT graph: the logical graph and version stay, the delta and
+    tombstones only shrink, threshold 0 leaves the graph compact, and BFS and
+    PageRank objects made before it recompute to the oracle's answer."""
+    g, mg = churned(vx, orc, directed)
+    bfs = vx.IncrementalBFS(g, 0)
+    pr = vx.IncrementalPageRank(g)
+    version = g.version
+    delta, dirty = g.delta_edge_count, g.dirty_out_segment_count()
+    tomb = g.storage()["tombstones"]
+    assert delta > 0 and dirty > 0 and not g.is_compact()
+    assert g.delta_ratio == pytest.approx(delta / max(1, g.base_edge_count_directed))
+    for threshold in (0.5, 0.1, 0.0):
+        g.maybe_compact(threshold)
+        check_model(g, mg)
+        assert g.version == version
+        assert g.delta_edge_count <= delta and g.dirty_out_segment_count() <= dirty
+        assert g.storage()["tombstones"] <= tomb
+        delta, dirty, tomb = g.delta_edge_count, g.dirty_out_segment_count(), g.storage()["tombstones"]
+    assert g.is_compact() and tomb == 0 and dirty == 0 and g.dirty_in_segment_count() == 0
+    assert g.base_edge_count_directed == g.edge_count_directed
+    assert not g.maybe_compact(0.0)
+    bfs.recompute()
+    full, _ = orc.model_bfs(mg, 0)
+    assert np.array_equal(bfs.distances_numpy(), full)
+    pr.recompute()
+    ref, _ = orc.pagerank_full(mg)
+    assert np.abs(np.asarray(pr.values) - ref).max() < 1e-10

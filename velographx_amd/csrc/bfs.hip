@@ -1397,13 +1397,13 @@ static void bfs_full(vgx_bfs* b) {
   const uint64_t n = g->n;
   b->dist_len = n;
   const uint64_t words = (n + 63) / 64;
-  // the dense head array and the zero-in-degree bitmap follow the graph's version
-  if (b->w.head_version != g->version || b->w.head_n != n) {
+  // the dense head array and the zero-in-degree bitmap follow the rows
+  if (b->w.head_version != g->rows_epoch || b->w.head_n != n) {
     b->w.head.ensure(g->ncap + 1);
     b->w.zin.ensure(words + 1);
     b->w.zcount.ensure(kZinStripes * kStripeWords);
     build_head_zin(g, b->w.head.p, b->w.zin.p, b->w.zcount.p, g->stream);
-    b->w.head_version = g->version;
+    b->w.head_version = g->rows_epoch;
     b->w.head_n = n;
   }
   const TraversalResult r =

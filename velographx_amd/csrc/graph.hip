@@ -718,13 +718,22 @@ void graph_grow_vertices(vgx_graph* g, uint64_t n_new) {
       a.row.grow_copy(c, g->ncap, g->stream);
       a.cap.grow_copy(c, g->ncap, g->stream);
       a.deg.grow_copy(c, g->ncap, g->stream);
+      a.base.grow_copy(c, g->ncap, g->stream);
       VGX_HIP(hipMemsetAsync(a.row.p + g->ncap, 0, (c - g->ncap) * sizeof(uint2), g->stream));
       VGX_HIP(hipMemsetAsync(a.cap.p + g->ncap, 0, (c - g->ncap) * sizeof(uint32_t), g->stream));
       VGX_HIP(hipMemsetAsync(a.deg.p + g->ncap, 0, (c - g->ncap) * sizeof(uint32_t), g->stream));
+      VGX_HIP(hipMemsetAsync(a.base.p + g->ncap, 0, (c - g->ncap) * sizeof(uint32_t), g->stream));
     }
     g->ncap = c;
   }
   g->n = n_new;
+}
+
+// A freshly packed direction: every row's base length is its live degree.
+static void rebase_rows(vgx_graph* g, int d) {
+  Adjacency& a = g->adj[d];
+  if (g->n)
+    VGX_HIP(hipMemcpyAsync(a.base.p, a.deg.p, g->n * sizeof(uint32_t), hipMemcpyDeviceToDevice, g->stream));
 }
 
 static void ensure_arena(vgx_graph* g, int d, uint64_t extra) {
@@ -840,8 +849,10 @@ static void bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t* dst, ui
       VGX_LAUNCH_CHECK();
     }
     build_direction(g, d, keys, skeys, M);
+    rebase_rows(g, d);
   }
   g->version += 1;
+  g->rows_epoch += 1;
 }
 
 // ------------------------------------------------------------ batch apply
@@ -1054,6 +1065,7 @@ void graph_finish_batch(vgx_graph* g) {
     if (tomb > a.live / 2 + 4096 || a.used > 2 * (a.live + g->n) + (1u << 20)) need_compact = true;
   }
   g->version += 1;
+  g->rows_epoch += 1;
   if (need_compact)
     for (int d = 0; d < g->ndir(); ++d) compact_direction(g, d);
 }
@@ -1154,6 +1166,8 @@ static void compact_direction(vgx_graph* g, int d) {
   a.arena.swap(pk.spare_arena);
   a.acap = a.arena.n;
   a.row.swap(pk.spare_row);
+  rebase_rows(g, d);
+  g->rows_epoch += 1;
   a.stored = a.live;
   a.relocated = 0;
   uint32_t total = 0;
@@ -1161,6 +1175,121 @@ static void compact_direction(vgx_graph* g, int d) {
   VGX_HIP(hipStreamSynchronize(s));
   if (total > a.arena.n) throw Error(VGX_DEVICE_ERROR, "compaction exceeded its arena bound");
   a.used = total;
+}
+
+// ---------------------------------------------------- per-row delta state
+// A row's changes since its last bulk load or compaction, read off its
+// stored length `len`, live degree `deg` and base length `base` (its stored
+// length right after that bulk load or compaction): the entries inserted
+// since (len - base: an insert adds a stored entry, a deletion only marks
+// one, and only compaction removes entries) plus its tombstones
+// (len - deg).  An arc inserted and deleted again since counts twice; a
+// base arc deleted and re-added (its tombstone revived) counts nothing.
+__device__ __forceinline__ unsigned long long row_delta(uint32_t len, uint32_t deg, uint32_t base) {
+  return static_cast<unsigned long long>(len - deg) + static_cast<unsigned long long>(len - base);
+}
+
+// Sums over the rows of one direction: out[0] base entries, out[1] row
+// deltas, out[2] rows whose delta is not zero.
+__global__ __launch_bounds__(kBlock) void k_row_delta_sums(const uint2* __restrict__ row,
+                                                          const uint32_t* __restrict__ deg,
+                                                          const uint32_t* __restrict__ base, uint64_t n,
+                                                          unsigned long long* __restrict__ out) {
+  unsigned long long b = 0, dl = 0, dirty = 0;
+  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; u < n;
+       u += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t bs = base[u];
+    const unsigned long long x = row_delta(row[u].y, deg[u], bs);
+    b += bs;
+    dl += x;
+    dirty += x != 0 ? 1ull : 0ull;
+  }
+  b = wave_sum<unsigned long long>(b);
+  dl = wave_sum<unsigned long long>(dl);
+  dirty = wave_sum<unsigned long long>(dirty);
+  if (lane_id() == 0) {
+    if (b) atomicAdd(&out[0], b);
+    if (dl) atomicAdd(&out[1], dl);
+    if (dirty) atomicAdd(&out[2], dirty);
+  }
+}
+
+// Row compaction: every row whose delta is at least `threshold` times
+// max(8, base) drops its tombstones in place.  Each lane tests one of 64
+// consecutive rows; the wave then squeezes the chosen rows one after
+// another, 64 entries a step: a live entry moves to its rank among the
+// row's live entries, which never lies past the entry itself, so no write
+// reaches a slot that a later step still reads.  The row's length and
+// base become its live count.  acc[0] += rows compacted, acc[1] +=
+// entries dropped.
+__global__ __launch_bounds__(kBlock) void k_squeeze_rows(uint2* __restrict__ row, const uint32_t* __restrict__ deg,
+                                                        uint32_t* __restrict__ base, uint32_t* __restrict__ arena,
+                                                        uint64_t n, double threshold,
+                                                        unsigned long long* __restrict__ acc) {
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+  const uint64_t wid = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave;
+  const uint32_t lane = lane_id();
+  unsigned long long rows = 0, dropped = 0;
+  for (uint64_t u0 = wid * kWave; u0 < n; u0 += waves * kWave) {
+    const uint64_t u = u0 + lane;
+    uint2 rb = make_uint2(0u, 0u);
+    bool pick = false;
+    if (u < n) {
+      rb = row[u];
+      const uint32_t bs = base[u];
+      const unsigned long long x = row_delta(rb.y, deg[u], bs);
+      const double density = static_cast<double>(x) / static_cast<double>(max(8u, bs));
+      pick = x != 0 && !(density < threshold);
+    }
+    unsigned long long m = __ballot(pick);
+    while (m) {
+      const int l = __ffsll(static_cast<long long>(m)) - 1;
+      m &= m - 1;
+      const uint32_t beg = __shfl(rb.x, l, kWave), len = __shfl(rb.y, l, kWave);
+      uint32_t w = 0;
+      for (uint32_t i0 = 0; i0 < len; i0 += kWave) {
+        const uint32_t i = i0 + lane;
+        const uint32_t x = i < len ? arena[beg + i] : kTomb;
+        const bool keep = !(x & kTomb);
+        const unsigned long long km = __ballot(keep);
+        if (keep) arena[beg + w + __popcll(km & ((1ull << lane) - 1ull))] = x;
+        w += __popcll(km);
+      }
+      if (static_cast<int>(lane) == l) {
+        row[u].y = w;
+        base[u] = w;
+      }
+      ++rows;
+      dropped += len - w;
+    }
+  }
+  if (lane == 0 && rows) {
+    atomicAdd(&acc[0], rows);
+    atomicAdd(&acc[1], dropped);
+  }
+}
+
+struct RowDeltaSums {
+  uint64_t base = 0, delta = 0, dirty = 0;
+};
+
+static RowDeltaSums row_delta_sums(vgx_graph* g, int d) {
+  RowDeltaSums r;
+  if (!g->n) return r;
+  hipStream_t s = g->stream;
+  const Adjacency& a = g->adj[d];
+  DevBuf<unsigned long long>& out = g->pk.rowsum;
+  out.ensure(4);
+  VGX_HIP(hipMemsetAsync(out.p, 0, 4 * sizeof(unsigned long long), s));
+  k_row_delta_sums<<<grid_for(g->n, kBlock, 2048), kBlock, 0, s>>>(a.row.p, a.deg.p, a.base.p, g->n, out.p);
+  VGX_LAUNCH_CHECK();
+  unsigned long long h[3] = {0, 0, 0};
+  VGX_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipStreamSynchronize(s));
+  r.base = h[0];
+  r.delta = h[1];
+  r.dirty = h[2];
+  return r;
 }
 
 // Consolidated snapshot: the live rows of `src` packed, without tombstones
@@ -1176,6 +1305,7 @@ static void snapshot_into(vgx_graph* src, vgx_graph* dst) {
     k_compact_caps<<<grid_for(n), kBlock, 0, s>>>(sa.deg.p, n, da.cap.p);
     VGX_LAUNCH_CHECK();
     VGX_HIP(hipMemcpyAsync(da.deg.p, sa.deg.p, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    VGX_HIP(hipMemcpyAsync(da.base.p, sa.deg.p, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     pack_direction(src, d, dst->pk.off, da.arena, da.row.p, da.cap.p);
     k_set_bump<<<1, 64, 0, s>>>(dst->pk.off.p, n, dst->d_ctr, d);
     VGX_LAUNCH_CHECK();
@@ -1451,6 +1581,60 @@ vgx_status vgx_graph_storage_info(vgx_graph* g, vgx_graph_storage* out) {
     out->relocated_rows += a.relocated;
     out->storage_bytes += a.acap * 4 + g->ncap * (sizeof(uint2) + 8);
   }
+  VGX_API_END
+}
+
+vgx_status vgx_graph_ensure_vertex(vgx_graph* g, uint32_t v) {
+  VGX_API_BEGIN
+  if (!g) throw Error(VGX_INVALID_ARGUMENT, "null graph");
+  if (v >= (1u << 31) - 1) throw Error(VGX_INVALID_ARGUMENT, "vertex ids must be below 2^31 - 1");
+  DeviceGuard dg(g->device);
+  if (static_cast<uint64_t>(v) + 1 > g->n) {
+    graph_grow_vertices(g, static_cast<uint64_t>(v) + 1);
+    g->rows_epoch += 1;
+    VGX_HIP(hipStreamSynchronize(g->stream));
+  }
+  VGX_API_END
+}
+
+vgx_status vgx_graph_delta_info(vgx_graph* g, vgx_graph_delta* out) {
+  VGX_API_BEGIN
+  if (!g || !out) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  DeviceGuard dg(g->device);
+  const RowDeltaSums o = row_delta_sums(g, 0);
+  const RowDeltaSums i = g->directed ? row_delta_sums(g, 1) : o;
+  out->base_arcs = o.base;
+  out->delta_arcs = o.delta;
+  out->delta_in_arcs = i.delta;
+  out->dirty_out_rows = o.dirty;
+  out->dirty_in_rows = i.dirty;
+  VGX_API_END
+}
+
+vgx_status vgx_graph_maybe_compact(vgx_graph* g, double threshold, int* compacted) {
+  VGX_API_BEGIN
+  if (!g) throw Error(VGX_INVALID_ARGUMENT, "null graph");
+  DeviceGuard dg(g->device);
+  hipStream_t s = g->stream;
+  bool any = false;
+  for (int d = 0; d < g->ndir() && g->n; ++d) {
+    Adjacency& a = g->adj[d];
+    DevBuf<unsigned long long>& acc = g->pk.rowsum;
+    acc.ensure(4);
+    VGX_HIP(hipMemsetAsync(acc.p, 0, 4 * sizeof(unsigned long long), s));
+    const uint64_t groups = (g->n + kWave - 1) / kWave;  // one 64-row group per wave step
+    const unsigned blocks = static_cast<unsigned>(std::min<uint64_t>(1024, (groups + 3) / 4));
+    k_squeeze_rows<<<blocks, kBlock, 0, s>>>(a.row.p, a.deg.p, a.base.p, a.arena.p, g->n, threshold, acc.p);
+    VGX_LAUNCH_CHECK();
+    unsigned long long h[2] = {0, 0};
+    VGX_HIP(hipMemcpyAsync(h, acc.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    if (h[1] > a.stored - a.live) throw Error(VGX_DEVICE_ERROR, "row compaction dropped live entries");
+    a.stored -= h[1];
+    any = any || h[0] != 0;
+  }
+  if (any) g->rows_epoch += 1;
+  if (compacted) *compacted = any ? 1 : 0;
   VGX_API_END
 }
 

@@ -511,8 +511,8 @@ static void ms_run(vgx_msbfs* m) {
   hipStream_t s = g->stream;
   const Adjacency& out = g->adj[0];
   const Adjacency& in = g->in();
-  const MsPlan& po = ms_plan(m, 0, out, g->version, g->n, s);
-  const MsPlan& pi = g->directed ? ms_plan(m, 1, in, g->version, g->n, s) : po;  // undirected: one table, one plan
+  const MsPlan& po = ms_plan(m, 0, out, g->rows_epoch, g->n, s);
+  const MsPlan& pi = g->directed ? ms_plan(m, 1, in, g->rows_epoch, g->n, s) : po;  // undirected: one table, one plan
   const uint64_t n = g->n;
   if (n != m->n) {  // the per-level arrays are laid out with stride n
     m->levels_cap = 0;

@@ -890,7 +890,7 @@ static size_t elt(const vgx_pagerank* p) { return p->precision == 32 ? 4 : 8; }
 
 static void build_lists(vgx_pagerank* p) {
   vgx_graph* g = p->g;
-  if (p->lists_version == g->version && p->n == g->n) return;
+  if (p->lists_version == g->rows_epoch && p->n == g->n) return;
   hipStream_t s = g->stream;
   const uint64_t n = g->n;
   const Adjacency& in = g->in();
@@ -936,7 +936,7 @@ static void build_lists(vgx_pagerank* p) {
   p->nh = tail[3];
   p->nchunks = tail[4];
   p->hpart.ensure((p->nchunks + 1) * 8);
-  p->lists_version = g->version;
+  p->lists_version = g->rows_epoch;
 }
 
 static PrArgs pr_args(vgx_pagerank* p, DevBuf<uint8_t>* rk) {

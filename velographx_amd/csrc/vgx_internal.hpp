@@ -99,6 +99,7 @@ struct Adjacency {
   DevBuf<uint2> row;      // [ncap]
   DevBuf<uint32_t> cap;   // [ncap]
   DevBuf<uint32_t> deg;   // [ncap] live degree
+  DevBuf<uint32_t> base;  // [ncap] len at the row's last bulk load or compaction
   DevBuf<uint32_t> arena; // [acap]
   uint64_t acap = 0;       // arena capacity (entries)
   uint64_t used = 0;       // host mirror of the bump pointer
@@ -191,6 +192,7 @@ struct PackScratch {
   DevBuf<uint32_t> row_chunks, chunk_first;  // chunks per row and their exclusive scan
   DevBuf<uint32_t> chunk_u, chunk_j, chunk_live, chunk_scan;
   DevBuf<unsigned long long> counts;  // [0] chunks
+  DevBuf<unsigned long long> rowsum;  // per-row state sums (delta queries, row compaction)
 };
 }  // namespace vgx
 
@@ -201,6 +203,10 @@ struct vgx_graph {
   uint64_t n = 0;
   uint64_t ncap = 0;
   uint64_t version = 0;
+  // Advances whenever the arc set changes (with `version`) and whenever
+  // compaction moves or shortens rows without changing it: the key of
+  // every cache of row positions or stored lengths.
+  uint64_t rows_epoch = 0;
   vgx::Adjacency adj[2];  // [0] out, [1] in (unused when undirected)
   vgx::GraphCounters* d_ctr = nullptr;
   vgx::GraphCounters* h_ctr = nullptr;  // pinned

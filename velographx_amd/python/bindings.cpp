@@ -359,6 +359,14 @@ PYBIND11_MODULE(_velographx, m) {
           py::arg("src_ptr"), py::arg("dst_ptr"), py::arg("add_ptr"), py::arg("k"), py::arg("stream") = 0)
       .def_property_readonly("device", [](const DynamicGraph& g) { return device_of(g.handle()); })
       .def("is_compact", &DynamicGraph::is_compact)
+      .def("ensure_vertex", &DynamicGraph::ensure_vertex, py::arg("v"))
+      .def_property_readonly("base_edge_count_directed", &DynamicGraph::base_edge_count_directed)
+      .def_property_readonly("delta_edge_count", &DynamicGraph::delta_edge_count)
+      .def_property_readonly("delta_ratio", &DynamicGraph::delta_ratio)
+      .def("dirty_out_segment_count", &DynamicGraph::dirty_out_segment_count)
+      .def("dirty_in_segment_count", &DynamicGraph::dirty_in_segment_count)
+      .def("maybe_compact", &DynamicGraph::maybe_compact, py::arg("threshold") = 0.25,
+           py::call_guard<py::gil_scoped_release>())
       .def(
           "probe_neighbors",
           [](const DynamicGraph& g, std::size_t count, std::uint32_t repeats) {
