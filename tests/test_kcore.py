@@ -114,9 +114,14 @@ def test_engine_matches_oracle(orc, scale):
     assert kc.last_repaired_vertices == n
     for bs, bd, ba in random_batches(n, 40 + scale, 4, grow=20) + [
             (np.arange(0, 200, dtype=np.uint32), np.arange(1, 201, dtype=np.uint32), np.zeros(200, np.uint8))]:
+        off0, adj0 = mg.csr()
+        before = (mg.n, np.asarray(off0), np.asarray(adj0))
         kc.apply_arrays(bs, bd, ba)
         mg.apply(bs, bd, ba)
         assert np.array_equal(kc.core_numpy(), cores(mg))
+        off1, adj1 = mg.csr()
+        after = (mg.n, np.asarray(off1), np.asarray(adj1))
+        assert kc.last_repaired_vertices == okc.affected_count(before, after, bs, bd)
     kc.apply_arrays(np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.uint8))
     assert kc.last_repaired_vertices == 0 and np.array_equal(kc.core_numpy(), cores(mg))
     kc.recompute()
@@ -145,3 +150,42 @@ def test_engine_closed_forms_and_errors(orc):
     assert vx.IncrementalKCore(vx.Graph(0, False)).core == []
     iso = vx.Graph(5, False)
     assert vx.IncrementalKCore(iso).core == [0] * 5
+
+
+@pytest.mark.gpu
+def test_region_repair_on_separate_components(orc):
+    """Batches that touch one of several components re-peel that component
+    only (kcore.hpp:31-56), growth included, and match full recomputes."""
+    import velographx_amd as vx
+
+    assert vx.device_count() > 0
+    rng = np.random.default_rng(9)
+    parts = []
+    for c in range(8):  # eight components of 500 vertices, ids interleaved
+        ids = np.arange(c, 4000, 8)
+        e = rng.integers(0, ids.size, size=(3000, 2))
+        parts.append(np.stack([ids[e[:, 0]], ids[e[:, 1]]], 1))
+    e = np.concatenate(parts).astype(np.uint32)
+    g = vx.Graph(4000, False)
+    g.bulk_load_edges(e)
+    mg = orc.ModelGraph(4000, False)
+    mg.bulk_load_edges(e)
+    kc = vx.IncrementalKCore(g)
+    for step in range(12):
+        c = step % 8
+        ids = np.arange(c, 4000, 8)
+        k = 40
+        bs = ids[rng.integers(0, ids.size, k)].astype(np.uint32)
+        bd = ids[rng.integers(0, ids.size, k)].astype(np.uint32)
+        if step == 5:
+            bd[0] = 4100 + step  # a new vertex joins component c
+        ba = (rng.random(k) < 0.5).astype(np.uint8)
+        off0, adj0 = mg.csr()
+        before = (mg.n, np.asarray(off0), np.asarray(adj0))
+        kc.apply_arrays(bs, bd, ba)
+        mg.apply(bs, bd, ba)
+        off1, adj1 = mg.csr()
+        after = (mg.n, np.asarray(off1), np.asarray(adj1))
+        assert np.array_equal(kc.core_numpy(), cores(mg))
+        rep = okc.affected_count(before, after, bs, bd)
+        assert kc.last_repaired_vertices == rep and rep < mg.n
