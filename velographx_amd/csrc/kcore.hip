@@ -12,13 +12,24 @@
 // depend only on the graph.
 //
 // Level k: a scan claims every unpeeled region vertex whose remaining
-// degree is at most k (core number k) into a queue, and copies the others to
-// the next scan's list with the smallest of their degrees; then rounds of
-// peeling, one wave per queued vertex, decrement the remaining degree of
-// each unpeeled neighbour and claim (compare-and-swap on its core slot) the
-// ones that drop to k or below into the next queue.  A level whose scan
-// claims nothing jumps straight to the smallest remaining degree.  Every
-// arc is decremented at most once, so degrees never wrap.
+// degree is at most k (core number k) into a queue; then rounds of peeling
+// decrement the remaining degree of each unpeeled neighbour of the queued
+// vertices, and the decrement that lands on k claims that neighbour into
+// the next queue (after the scan every unpeeled vertex has degree above k,
+// and concurrent decrements return distinct values, so exactly one does).
+// A level whose scan claims nothing jumps straight to the smallest
+// remaining degree.  Every arc is decremented at most once, so degrees
+// never wrap.
+//
+// Scans read two lists: a near list of the unpeeled vertices of degree at
+// most `bound`, read by every scan, and a far list of the others, read only
+// when a level passes the bound -- then the far list is split again with
+// bound = k + kKcWindow.  A far vertex whose degree drops to the bound
+// during peeling is appended to the near list there (a degree falls by one
+// at a time, so it crosses the bound once), and the far scans skip vertices
+// whose degree is at or below the old bound (they are on the near list).
+// Most vertices of a skewed graph stay unpeeled for many levels at degrees
+// far above the level; they are read once per window instead of per level.
 //
 // The level loop is a device-steered step program: launch s reads launch
 // s - 1's plan and results from a ring of four records and decides its own
@@ -38,6 +49,7 @@ constexpr uint32_t kKcUnset = 0xFFFFFFFFu;
 constexpr int kKcBlock = 256;
 constexpr uint32_t kKcUnroll = 4;
 constexpr uint32_t kKcPhInit = 0, kKcPhScan = 1, kKcPhPeel = 2, kKcPhDone = 3;
+constexpr int32_t kKcWindow = 8;  // near-list degree window above the level
 // Flood-fill rounds run several per host check, on a ring of three counters
 // like the peeling rounds of the step program's queues.
 constexpr uint32_t kKcFloodPerCheck = 4;
@@ -47,13 +59,19 @@ constexpr uint32_t kKcFloodPerCheck = 4;
 // every workgroup of the step.
 struct KcRec {
   uint32_t phase, k, round;
-  uint32_t list;                   // scan: the list it writes (it reads the other one);
-                                   // peeling round: the list the next scan writes
-  unsigned long long in_count;     // entries of this step's input (queue or list)
-  unsigned long long level_rem;    // vertices the level's scan left on its list
+  uint32_t nl;                     // near list a scan writes (it reads nl ^ 1); kept by peeling rounds
+  uint32_t fl;                     // far list in use (a splitting scan reads it and writes fl ^ 1)
+  uint32_t split;                  // scan: the level passed the bound, the far list is read
+  int32_t bound;                   // degree bound of the near list (-1: no near list yet)
+  uint32_t pad;
+  unsigned long long in_count;     // scan: near entries; peeling round: queued vertices
+  unsigned long long level_rem;    // peeling round: entries on the near list being written
+  unsigned long long far_count;    // entries of the far list in use
   unsigned long long queued;       // claims appended to this step's output queue
-  unsigned long long remaining;    // scan: vertices copied to the next list
-  unsigned long long min_deg;      // scan: the smallest remaining degree among them
+  unsigned long long remaining;    // scan: vertices written to the near list
+  unsigned long long far_rem;      // splitting scan: vertices written to the new far list
+  unsigned long long min_deg;      // scan: the smallest degree among the vertices it kept
+  unsigned long long crossed;      // peeling round: far vertices appended to the near list
 };
 
 struct KcState {
@@ -67,7 +85,9 @@ struct KcState {
 
 struct vgx_kcore {
   vgx_graph* g = nullptr;
-  vgx::DevBuf<uint32_t> deg, core, mark, q[2], list[2], ends;
+  vgx::DevBuf<uint32_t> deg, core, mark, near[2], far[2], ends;
+  vgx::DevBuf<unsigned long long> items[2];  // peeling items of the level loop
+  vgx::DevBuf<uint32_t> fq[2];               // flood-fill queues
   vgx::KcState* d_st = nullptr;
   vgx::KcState* h_st = nullptr;  // pinned
   uint64_t n = 0;
@@ -96,6 +116,119 @@ __device__ __forceinline__ void kc_push(bool flag, uint32_t v, uint32_t* __restr
   if (flag) q[base + lanes_below(m)] = v;
 }
 
+// Per-wave staging of queue pushes in LDS: a push lands in the wave's
+// buffer, and one atomic per flush (or, at the end of a kernel, one per
+// workgroup) reserves the queue slots -- instead of one atomic on a hot
+// counter per wave step, which serialises thousands of waves at the L2.
+constexpr uint32_t kKcStage = 512;
+struct KcStage {
+  uint32_t v[kKcStage];
+};
+
+__device__ __forceinline__ void kc_flush(KcStage& sb, uint32_t& nbuf, uint32_t* __restrict__ q,
+                                         unsigned long long* count) {
+  if (nbuf == 0) return;
+  __builtin_amdgcn_wave_barrier();
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(count, static_cast<unsigned long long>(nbuf));
+  base = __shfl(base, 0, kWave);
+  for (uint32_t c = lane_id(); c < nbuf; c += kWave) q[base + c] = sb.v[c];
+  __builtin_amdgcn_wave_barrier();
+  nbuf = 0;
+}
+
+// Stages v for the flagged lanes; called by every lane of the wave.
+__device__ __forceinline__ void kc_stage(KcStage& sb, uint32_t& nbuf, bool flag, uint32_t v,
+                                         uint32_t* __restrict__ q, unsigned long long* count) {
+  const unsigned long long m = __ballot(flag);
+  if (!m) return;
+  if (flag) sb.v[nbuf + lanes_below(m)] = v;
+  nbuf += __popcll(m);
+  if (nbuf > kKcStage - kWave) kc_flush(sb, nbuf, q, count);
+}
+
+// The last flush of every wave of a workgroup, with one atomic for the
+// workgroup.  Every thread of the workgroup calls it; `sm` holds
+// waves + 1 words.
+__device__ __forceinline__ void kc_flush_block(KcStage& sb, uint32_t nbuf, uint32_t* __restrict__ q,
+                                               unsigned long long* count, unsigned long long* sm) {
+  const uint32_t wv = threadIdx.x / kWave, waves = blockDim.x / kWave;
+  if (lane_id() == 0) sm[wv] = nbuf;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (uint32_t i = 0; i < waves; ++i) t += sm[i];
+    sm[waves] = t ? atomicAdd(count, t) : 0ull;
+  }
+  __syncthreads();
+  unsigned long long base = sm[waves];
+  for (uint32_t i = 0; i < wv; ++i) base += sm[i];
+  for (uint32_t c = lane_id(); c < nbuf; c += kWave) q[base + c] = sb.v[c];
+  __syncthreads();  // sm is reused by the next flush
+}
+
+// Peeling work items: (chunk << 32) | v names entries [chunk * kKcChunk,
+// (chunk + 1) * kKcChunk) of v's row, so a hub row is spread over many
+// waves instead of being walked by one.  A claimed vertex with an empty row
+// has nothing to peel and queues no item.
+constexpr uint32_t kKcChunk = kKcUnroll * kWave;
+struct KcItems {
+  unsigned long long v[kKcStage];
+};
+
+__device__ __forceinline__ void kc_flush_items(KcItems& sb, uint32_t& nbuf, unsigned long long* __restrict__ q,
+                                               unsigned long long* count) {
+  if (nbuf == 0) return;
+  __builtin_amdgcn_wave_barrier();
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(count, static_cast<unsigned long long>(nbuf));
+  base = __shfl(base, 0, kWave);
+  for (uint32_t c = lane_id(); c < nbuf; c += kWave) q[base + c] = sb.v[c];
+  __builtin_amdgcn_wave_barrier();
+  nbuf = 0;
+}
+
+// Queues the nch items of v for every lane (nch = 0: none); called by every
+// lane of the wave.  A wave step with more items than the buffer holds
+// (long rows) writes them to the queue directly.
+__device__ __forceinline__ void kc_stage_items(KcItems& sb, uint32_t& nbuf, uint32_t nch, uint32_t v,
+                                               unsigned long long* __restrict__ q, unsigned long long* count) {
+  const unsigned long long incl = wave_inclusive_scan<unsigned long long>(nch);
+  const uint32_t total = static_cast<uint32_t>(__shfl(incl, kWave - 1, kWave));
+  if (total == 0) return;
+  const uint32_t first = static_cast<uint32_t>(incl) - nch;
+  if (total > kKcStage - kWave) {
+    unsigned long long base = 0;
+    if (lane_id() == 0) base = atomicAdd(count, static_cast<unsigned long long>(total));
+    base = __shfl(base, 0, kWave);
+    for (uint32_t c = 0; c < nch; ++c) q[base + first + c] = (static_cast<unsigned long long>(c) << 32) | v;
+    return;
+  }
+  if (nbuf + total > kKcStage) kc_flush_items(sb, nbuf, q, count);
+  for (uint32_t c = 0; c < nch; ++c) sb.v[nbuf + first + c] = (static_cast<unsigned long long>(c) << 32) | v;
+  nbuf += total;
+  if (nbuf > kKcStage - kWave) kc_flush_items(sb, nbuf, q, count);
+}
+
+__device__ __forceinline__ void kc_flush_items_block(KcItems& sb, uint32_t nbuf, unsigned long long* __restrict__ q,
+                                                     unsigned long long* count, unsigned long long* sm) {
+  const uint32_t wv = threadIdx.x / kWave, waves = blockDim.x / kWave;
+  if (lane_id() == 0) sm[wv] = nbuf;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (uint32_t i = 0; i < waves; ++i) t += sm[i];
+    sm[waves] = t ? atomicAdd(count, t) : 0ull;
+  }
+  __syncthreads();
+  unsigned long long base = sm[waves];
+  for (uint32_t i = 0; i < wv; ++i) base += sm[i];
+  for (uint32_t c = lane_id(); c < nbuf; c += kWave) q[base + c] = sb.v[c];
+  __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t kc_chunks(uint32_t len) { return (len + kKcChunk - 1) / kKcChunk; }
+
 // ---------------------------------------------------------------- region
 
 // Seeds the flood fill with the batch's endpoints below n: a vertex joins
@@ -119,6 +252,10 @@ __global__ __launch_bounds__(kKcBlock) void k_kc_flood(uint32_t round, uint32_t 
                                                        const uint32_t* __restrict__ arena,
                                                        const uint32_t* __restrict__ q, uint32_t* __restrict__ mark,
                                                        uint32_t* __restrict__ qnext, KcState* __restrict__ st) {
+  __shared__ KcStage s_stage[kKcBlock / kWave];
+  __shared__ unsigned long long s_flush[kKcBlock / kWave + 1];
+  KcStage& sb = s_stage[threadIdx.x / kWave];
+  uint32_t nbuf = 0;
   const uint64_t count = st->flood[round % 3];
   if (blockIdx.x == 0 && threadIdx.x == 0) st->flood[(round + 2) % 3] = 0;
   unsigned long long* next = &st->flood[(round + 1) % 3];
@@ -132,9 +269,10 @@ __global__ __launch_bounds__(kKcBlock) void k_kc_flood(uint32_t round, uint32_t 
       const uint32_t x = j < rb.y ? arena[rb.x + j] : kTomb;
       const uint32_t v = x & ~kTomb;
       const bool join = !(x & kTomb) && mark[v] < epoch && atomicMax(&mark[v], epoch) < epoch;
-      kc_push(join, v, qnext, next);
+      kc_stage(sb, nbuf, join, v, qnext, next);
     }
   }
+  kc_flush_block(sb, nbuf, qnext, next, s_flush);
 }
 
 // The level loop's start: every vertex of the region (mark >= lo, or all
@@ -145,6 +283,10 @@ __global__ __launch_bounds__(kKcBlock) void k_kc_flood(uint32_t round, uint32_t 
 __global__ void k_kc_init(const uint32_t* __restrict__ live_deg, uint64_t n, uint64_t n_old, uint32_t lo,
                           const uint32_t* __restrict__ mark, uint32_t* __restrict__ deg, uint32_t* __restrict__ core,
                           uint32_t* __restrict__ list, KcState* __restrict__ st) {
+  __shared__ KcStage s_stage[kKcBlock / kWave];
+  __shared__ unsigned long long s_flush[kKcBlock / kWave + 1];
+  KcStage& sb = s_stage[threadIdx.x / kWave];
+  uint32_t nbuf = 0;
   for (uint64_t v0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); v0 < n;
        v0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint64_t v = v0 + threadIdx.x;
@@ -158,36 +300,50 @@ __global__ void k_kc_init(const uint32_t* __restrict__ live_deg, uint64_t n, uin
         core[v] = 0;
       }
     }
-    kc_push(in, static_cast<uint32_t>(v), list, &st->rec[3].remaining);
-    const unsigned long long m = __ballot(in);
-    if (lane_id() == 0 && m) atomicAdd(&st->marked, static_cast<unsigned long long>(__popcll(m)));
+    kc_stage(sb, nbuf, in, static_cast<uint32_t>(v), list, &st->rec[3].remaining);
   }
+  kc_flush_block(sb, nbuf, list, &st->rec[3].remaining, s_flush);
 }
 
 // ------------------------------------------------------------ step program
 
 struct KcPlan {
-  uint32_t phase, k, round, list;
-  unsigned long long in_count, level_rem;
+  uint32_t phase, k, round, nl, fl, split;
+  int32_t bound;
+  unsigned long long in_count, level_rem, far_count;
 };
 
 // The step after `p` (its plan and results); shared by every workgroup.
 __device__ __forceinline__ KcPlan kc_next(const KcRec& p) {
-  KcPlan x{kKcPhDone, p.k, 0u, p.list, 0ull, 0ull};
+  KcPlan x{kKcPhDone, p.k, 0u, p.nl, p.fl, 0u, p.bound, 0ull, p.level_rem, p.far_count};
   if (p.phase == kKcPhInit) {
-    x = {kKcPhScan, 0u, 0u, 0u, p.remaining, 0ull};
-    if (p.remaining == 0) x.phase = kKcPhDone;
+    // the init put the region on far list 0; the first scan splits it
+    if (p.remaining != 0) x = {kKcPhScan, 0u, 0u, 0u, 0u, 1u, -1, 0ull, 0ull, p.remaining};
   } else if (p.phase == kKcPhScan) {
+    const unsigned long long far = p.split ? p.far_rem : p.far_count;
+    const int32_t bound = p.split ? static_cast<int32_t>(p.k) + kKcWindow : p.bound;
+    const uint32_t fl = p.split ? (p.fl ^ 1u) : p.fl;
     if (p.queued == 0) {
-      // nothing at this level: the next level with vertices
-      if (p.remaining != 0)
-        x = {kKcPhScan, static_cast<uint32_t>(p.min_deg), 0u, p.list ^ 1u, p.remaining, 0ull};
+      // nothing at this level: the next level with vertices (past the bound
+      // when the near list is empty and the far list was not read)
+      if (p.remaining + far != 0) {
+        const uint32_t k = (p.split || p.remaining != 0) ? static_cast<uint32_t>(p.min_deg)
+                                                          : static_cast<uint32_t>(bound + 1);
+        x = {kKcPhScan, k, 0u, p.nl ^ 1u, fl, static_cast<int32_t>(k) > bound ? 1u : 0u, bound, p.remaining, 0ull,
+             far};
+      }
     } else {
-      x = {kKcPhPeel, p.k, 0u, p.list ^ 1u, p.queued, p.remaining};
+      x = {kKcPhPeel, p.k, 0u, p.nl, fl, 0u, bound, p.queued, p.remaining, far};
     }
   } else if (p.phase == kKcPhPeel) {
-    if (p.queued != 0) x = {kKcPhPeel, p.k, p.round + 1u, p.list, p.queued, p.level_rem};
-    else if (p.level_rem != 0) x = {kKcPhScan, p.k + 1u, 0u, p.list, p.level_rem, 0ull};
+    const unsigned long long near = p.level_rem + p.crossed;
+    if (p.queued != 0) {
+      x = {kKcPhPeel, p.k, p.round + 1u, p.nl, p.fl, 0u, p.bound, p.queued, near, p.far_count};
+    } else if (near + p.far_count != 0) {
+      const uint32_t k = p.k + 1u;
+      x = {kKcPhScan, k, 0u, p.nl ^ 1u, p.fl, static_cast<int32_t>(k) > p.bound ? 1u : 0u, p.bound, near, 0ull,
+           p.far_count};
+    }
   }
   return x;
 }
@@ -197,8 +353,9 @@ struct KcArgs {
   const uint32_t* arena;
   uint32_t* deg;
   uint32_t* core;
-  uint32_t* q[2];
-  uint32_t* list[2];
+  unsigned long long* q[2];  // peeling items
+  uint32_t* near[2];
+  uint32_t* far[2];
   KcState* st;
 };
 
@@ -210,74 +367,128 @@ __global__ __launch_bounds__(kKcBlock) void k_kc_step(uint32_t step, KcArgs a) {
     cur.phase = p.phase;
     cur.k = p.k;
     cur.round = p.round;
-    cur.list = p.list;
+    cur.nl = p.nl;
+    cur.fl = p.fl;
+    cur.split = p.split;
+    cur.bound = p.bound;
     cur.in_count = p.in_count;
     cur.level_rem = p.level_rem;
+    cur.far_count = p.far_count;
     KcRec& nxt = a.st->rec[(step + 1) & 3];  // last read by step - 2
     nxt.queued = 0;
     nxt.remaining = 0;
+    nxt.far_rem = 0;
     nxt.min_deg = ~0ull;
+    nxt.crossed = 0;
+    if (prev.phase == kKcPhInit) a.st->marked = prev.remaining;  // the region, counted by the init
     if (prev.phase == kKcPhScan && prev.queued != 0) a.st->levels += 1;
     if (p.phase == kKcPhScan) a.st->scans += 1;
     if (p.phase == kKcPhPeel) a.st->rounds += 1;
   }
+  // workgroups past the step's input have nothing to do (a uniform exit:
+  // no barrier below is left waiting)
+  if (p.phase == kKcPhDone) return;
+  if (p.phase == kKcPhScan &&
+      blockIdx.x * static_cast<uint64_t>(blockDim.x) >= p.in_count + (p.split ? p.far_count : 0ull))
+    return;
+  if (p.phase == kKcPhPeel && blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) >= p.in_count) return;
+  __shared__ KcItems s_items[kKcBlock / kWave];
+  __shared__ KcStage s_stage[2][kKcBlock / kWave];
+  __shared__ unsigned long long s_flush[kKcBlock / kWave + 1];
+  KcItems& sb0 = s_items[threadIdx.x / kWave];
+  KcStage& sb1 = s_stage[0][threadIdx.x / kWave];
+  KcStage& sb2 = s_stage[1][threadIdx.x / kWave];
+  uint32_t nb0 = 0, nb1 = 0, nb2 = 0;
   if (p.phase == kKcPhScan) {
-    // the scan reads the list the previous scan (or the init) wrote
-    const uint32_t* in = a.list[p.list ^ 1u];
-    uint32_t* out = a.list[p.list];
+    // the near list the previous scan (and peeling) wrote, then, when the
+    // level passed the bound, the far list
+    const uint32_t* near_in = a.near[p.nl ^ 1u];
+    uint32_t* near_out = a.near[p.nl];
+    const uint32_t* far_in = a.far[p.fl];
+    uint32_t* far_out = a.far[p.fl ^ 1u];
+    const long long nb = static_cast<long long>(p.k) + kKcWindow;
+    const uint64_t total = p.in_count + (p.split ? p.far_count : 0ull);
     unsigned long long mn = ~0ull;
-    for (uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); i0 < p.in_count;
+    for (uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); i0 < total;
          i0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
       const uint64_t i = i0 + threadIdx.x;
-      bool claim = false, keep = false;
-      uint32_t v = 0;
-      if (i < p.in_count) {
-        v = in[i];
+      bool keep_near = false, keep_far = false;
+      uint32_t v = 0, nch = 0;
+      if (i < total) {
+        const bool from_far = i >= p.in_count;
+        v = from_far ? far_in[i - p.in_count] : near_in[i];
         if (a.core[v] == kKcUnset) {
           const uint32_t d = a.deg[v];
-          if (d <= p.k) {
+          if (from_far && static_cast<long long>(d) <= p.bound) {
+            // crossed the bound while peeling: it is on a near list
+          } else if (d <= p.k) {
             a.core[v] = p.k;
-            claim = true;
+            nch = kc_chunks(a.row[v].y);
           } else {
-            keep = true;
+            keep_near = !p.split || static_cast<long long>(d) <= nb;
+            keep_far = !keep_near;
             mn = min(mn, static_cast<unsigned long long>(d));
           }
         }
       }
-      kc_push(claim, v, a.q[0], &cur.queued);
-      kc_push(keep, v, out, &cur.remaining);
+      kc_stage_items(sb0, nb0, nch, v, a.q[0], &cur.queued);
+      kc_stage(sb1, nb1, keep_near, v, near_out, &cur.remaining);
+      kc_stage(sb2, nb2, keep_far, v, far_out, &cur.far_rem);
     }
+    kc_flush_items_block(sb0, nb0, a.q[0], &cur.queued, s_flush);
+    kc_flush_block(sb1, nb1, near_out, &cur.remaining, s_flush);
+    kc_flush_block(sb2, nb2, far_out, &cur.far_rem, s_flush);
     mn = ~wave_max(~mn);  // the wave's minimum
-    if (lane_id() == 0 && mn != ~0ull) atomicMin(&cur.min_deg, mn);
+    __shared__ unsigned long long s_mn[kKcBlock / kWave];
+    if (lane_id() == 0) s_mn[threadIdx.x / kWave] = mn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (uint32_t w = 1; w < kKcBlock / kWave; ++w) mn = min(mn, s_mn[w]);
+      if (mn != ~0ull) atomicMin(&cur.min_deg, mn);
+    }
   } else if (p.phase == kKcPhPeel) {
-    const uint32_t* q = a.q[p.round & 1u];
-    uint32_t* qnext = a.q[(p.round + 1u) & 1u];
+    const unsigned long long* q = a.q[p.round & 1u];
+    unsigned long long* qnext = a.q[(p.round + 1u) & 1u];
+    // far vertices that drop to the bound join the near list after its
+    // current entries
+    uint32_t* cross = a.near[p.nl] + p.level_rem;
     const uint64_t W = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
     const unsigned lane = lane_id();
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave; i < p.in_count;
          i += W) {
-      const uint2 rb = a.row[q[i]];
-      // uniform across the wave; four arena loads in flight per lane (hub
-      // rows are bound by load latency)
-      for (uint32_t j0 = 0; j0 < rb.y; j0 += kKcUnroll * kWave) {
-        uint32_t x[kKcUnroll];
+      // one item: kKcChunk entries of a row, four arena loads in flight per
+      // lane
+      const unsigned long long item = q[i];
+      const uint2 rb = a.row[static_cast<uint32_t>(item)];
+      const uint32_t j0 = static_cast<uint32_t>(item >> 32) * kKcChunk;
+      uint32_t x[kKcUnroll];
 #pragma unroll
-        for (uint32_t t = 0; t < kKcUnroll; ++t) {
-          const uint32_t j = j0 + t * kWave + lane;
-          x[t] = j < rb.y ? a.arena[rb.x + j] : kTomb;  // past the row: skipped like a tombstone
-        }
+      for (uint32_t t = 0; t < kKcUnroll; ++t) {
+        const uint32_t j = j0 + t * kWave + lane;
+        x[t] = j < rb.y ? a.arena[rb.x + j] : kTomb;  // past the row: skipped like a tombstone
+      }
 #pragma unroll
-        for (uint32_t t = 0; t < kKcUnroll; ++t) {
-          bool claim = false;
-          const uint32_t v = x[t] & ~kTomb;
-          if (!(x[t] & kTomb) && a.core[v] == kKcUnset) {
-            const uint32_t before = atomicSub(&a.deg[v], 1u);
-            if (before - 1u <= p.k) claim = atomicCAS(&a.core[v], kKcUnset, p.k) == kKcUnset;
+      for (uint32_t t = 0; t < kKcUnroll; ++t) {
+        uint32_t nch = 0;
+        bool crossed = false;
+        const uint32_t v = x[t] & ~kTomb;
+        if (!(x[t] & kTomb) && a.core[v] == kKcUnset) {
+          // unpeeled vertices hold degree > k during level k, so exactly
+          // one decrement -- the one that lands on k -- claims v
+          const uint32_t after = atomicSub(&a.deg[v], 1u) - 1u;
+          if (after == p.k) {
+            a.core[v] = p.k;
+            nch = kc_chunks(a.row[v].y);
+          } else {
+            crossed = static_cast<long long>(after) == p.bound;
           }
-          kc_push(claim, v, qnext, &cur.queued);
         }
+        kc_stage_items(sb0, nb0, nch, v, qnext, &cur.queued);
+        kc_stage(sb1, nb1, crossed, v, cross, &cur.crossed);
       }
     }
+    kc_flush_items_block(sb0, nb0, qnext, &cur.queued, s_flush);
+    kc_flush_block(sb1, nb1, cross, &cur.crossed, s_flush);
   }
 }
 
@@ -292,7 +503,11 @@ static void kc_ensure(vgx_kcore* kc, uint64_t n) {
     VGX_HIP(hipMemsetAsync(kc->mark.p + kc->mark_valid, 0, (kc->mark.n - kc->mark_valid) * sizeof(uint32_t), s));
     kc->mark_valid = kc->mark.n;
   }
-  for (auto* b : {&kc->q[0], &kc->q[1], &kc->list[0], &kc->list[1]}) b->ensure(n + 1);
+  for (auto* b : {&kc->fq[0], &kc->fq[1], &kc->near[0], &kc->near[1], &kc->far[0], &kc->far[1]}) b->ensure(n + 1);
+  // a claimed vertex queues one item per kKcChunk stored row entries
+  const uint64_t items = n + kc->g->adj[0].stored / kKcChunk + 64;
+  kc->items[0].ensure(items);
+  kc->items[1].ensure(items);
 }
 
 // The level loop over the region: all vertices (lo = 0) or those marked at
@@ -307,11 +522,17 @@ static void kc_run(vgx_kcore* kc, uint32_t lo, uint64_t n_old) {
   std::memcpy(kc->h_st, &init, sizeof(init));
   VGX_HIP(hipMemcpyAsync(kc->d_st, kc->h_st, sizeof(KcState), hipMemcpyHostToDevice, s));
   const Adjacency& adj = g->adj[0];
-  // the init writes list[1]: the first scan has list 0 and reads list 0 ^ 1
+  // the init writes far list 0, which the first scan splits
   k_kc_init<<<kc_blocks(n, kKcBlock, 4096), kKcBlock, 0, s>>>(adj.deg.p, n, n_old, lo, kc->mark.p, kc->deg.p,
-                                                              kc->core.p, kc->list[1].p, kc->d_st);
+                                                              kc->core.p, kc->far[0].p, kc->d_st);
   VGX_LAUNCH_CHECK();
-  KcArgs a{adj.row.p, adj.arena.p, kc->deg.p, kc->core.p, {kc->q[0].p, kc->q[1].p}, {kc->list[0].p, kc->list[1].p},
+  KcArgs a{adj.row.p,
+           adj.arena.p,
+           kc->deg.p,
+           kc->core.p,
+           {kc->items[0].p, kc->items[1].p},
+           {kc->near[0].p, kc->near[1].p},
+           {kc->far[0].p, kc->far[1].p},
            kc->d_st};
   // one grid for every step: a wave per queued vertex, grid-stride
   const unsigned blocks = kc_blocks(n, kKcBlock / kWave, 1024);
@@ -358,15 +579,15 @@ static void kc_flood(vgx_kcore* kc, uint64_t nends, uint32_t epoch) {
   if (n == 0 || nends == 0) return;
   VGX_HIP(hipMemsetAsync(kc->d_st->flood, 0, sizeof(kc->d_st->flood), s));
   k_kc_flood_seed<<<kc_blocks(nends, kKcBlock, 1024), kKcBlock, 0, s>>>(kc->ends.p, nends, n, epoch, kc->mark.p,
-                                                                        kc->q[0].p, kc->d_st);
+                                                                        kc->fq[0].p, kc->d_st);
   VGX_LAUNCH_CHECK();
   const Adjacency& adj = g->adj[0];
   const unsigned blocks = kc_blocks(n, kKcBlock / kWave, 1024);
   // each vertex joins once per flood, so at most n rounds have a queue
   for (uint32_t r = 0;;) {
     for (uint32_t i = 0; i < kKcFloodPerCheck; ++i, ++r) {
-      k_kc_flood<<<blocks, kKcBlock, 0, s>>>(r, epoch, adj.row.p, adj.arena.p, kc->q[r & 1].p, kc->mark.p,
-                                             kc->q[(r + 1) & 1].p, kc->d_st);
+      k_kc_flood<<<blocks, kKcBlock, 0, s>>>(r, epoch, adj.row.p, adj.arena.p, kc->fq[r & 1].p, kc->mark.p,
+                                             kc->fq[(r + 1) & 1].p, kc->d_st);
       VGX_LAUNCH_CHECK();
     }
     unsigned long long ring[3] = {0, 0, 0};
