@@ -21,8 +21,11 @@ loop/duplicate removal).
   BFS from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over a child run, each
   scaled by its reading for a 1 GiB copy;
 * cpu_baseline: the reference engine (oracle/_ref/vgx_ref, compiled from the
-  reference sources) or, where it is absent, the oracle's queue BFS, timed
-  on this host on a bounded sample of the same roots.
+  reference sources) timed on this host on the same roots: the value is its
+  root-parallel BFS over all host threads (at most 16, one GPU's share of a
+  box), as its thread-scaling benchmark runs it, with the one-thread figure
+  and the CPU model beside it; where the reference build is absent, the
+  oracle's queue BFS on one thread.
 
 Secondary legs, each in its own object of the same JSON line:
 * batched   -- 64 roots per multi-source pass on the same graph (aggregate
@@ -67,6 +70,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import platform
 import subprocess
 import sys
 import tempfile
@@ -126,6 +130,7 @@ def parse():
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                    help="only run 5 device-to-device copies of BYTES (HBM counter calibration)")
     p.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
+    p.add_argument("--pmc-workload", choices=PMC_WORKLOADS, default="rmat", help=argparse.SUPPRESS)
     p.add_argument("--pmc-child", action="store_true",
                    help="(internal) profiled child: a 1 GiB copy, then the plain BFS roots")
     return p.parse_args()
@@ -140,8 +145,46 @@ def _edges_file(td, s, d):
     return path
 
 
+def cpu_model():
+    """The host CPU's model name (the "model name" line lscpu shows)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_threads():
+    """Host threads for the all-core baseline: the CPU share this process may
+    run on, at most 16 (one GPU's share of a box)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def cpu_baseline_all_cores(s, d, roots, m_c, threads):
+    """The reference's root-parallel BFS (benchmarks/thread_scaling_bfs.cpp:
+    78-108): every thread takes the next root off a shared counter and runs
+    bfs_distances on one CsrGraph; the 16 roots once each."""
+    with tempfile.TemporaryDirectory() as td:
+        cmd = [REF_BIN, "bench-bfs-mt", _edges_file(td, s, d), str(threads), "1"] + [str(int(r)) for r in roots]
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=900, check=True).stdout
+    w = [line.split() for line in out.splitlines() if line.startswith("mt ")][0]
+    queries, sec = int(w[2]), float(w[3])
+    return {"value": m_c * queries / sec / 1e9, "unit": "GTEPS", "cores": threads, "kind": "reference",
+            "sample": f"bfs_distances(CsrGraph, root) of the reference engine, the {queries} roots shared by "
+                      f"{threads} threads (its thread-scaling benchmark), wall time",
+            "ms_per_bfs_wall": 1e3 * sec / queries}
+
+
 def cpu_baseline(s, d, n, roots, m_c, budget):
-    """Time the reference engine's BFS (one thread) on a bounded sample."""
+    """Time the reference engine's BFS on a bounded sample: all host threads
+    root-parallel (the reported value) and one thread."""
     times = []
     if os.path.exists(REF_BIN):
         with tempfile.TemporaryDirectory() as td:
@@ -173,8 +216,15 @@ def cpu_baseline(s, d, n, roots, m_c, budget):
         sample = f"oracle queue BFS (C restatement) on the same graph, first {len(times)} of the same roots"
     if not times:
         return None
-    return {"value": m_c * len(times) / sum(times) / 1e9, "unit": "GTEPS", "cores": 1, "kind": kind,
-            "sample": sample, "ms_per_bfs": 1e3 * sum(times) / len(times)}
+    one = {"value": m_c * len(times) / sum(times) / 1e9, "unit": "GTEPS", "cores": 1, "kind": kind,
+           "sample": sample, "ms_per_bfs": 1e3 * sum(times) / len(times)}
+    if kind != "reference":
+        one["cpu_model"] = cpu_model()
+        return one
+    out = cpu_baseline_all_cores(s, d, roots, m_c, cpu_threads())
+    out["cpu_model"] = cpu_model()
+    out["single_thread"] = one
+    return out
 
 
 def cpu_repairs(s, d, n, root, batches, budget):
@@ -240,12 +290,33 @@ def _per_bfs(rows):
     return groups
 
 
-def measure_traffic(args, steps):
-    """HBM bytes per plain BFS from two rocprofv3 --pmc passes (FETCH_SIZE,
-    WRITE_SIZE) over a child run of this script.  Each counter is scaled by
-    its own reading for a 1 GiB device copy made first in the same child,
-    i.e. bytes = counter * (2^30 / counter_of_copy); the median over the
-    child's last `steps` traversals is reported."""
+def _per_sweep(rows, with_finish):
+    """Counter values (KiB) of each FP32 PageRank sweep: k_pr_iter<float>,
+    plus the k_pr_finish<float> launch that follows it when with_finish."""
+    groups = []
+    for r in rows:
+        name = r["Kernel_Name"]
+        if "k_pr_iter<float>" in name:
+            groups.append(float(r["Counter_Value"]))
+        elif with_finish and groups and "k_pr_finish<float>" in name:
+            groups[-1] += float(r["Counter_Value"])
+    return groups
+
+
+# Profiled workloads: what the child runs and how its dispatches group into
+# units (one traversal, one sweep).
+PMC_WORKLOADS = ("rmat", "er", "pagerank")
+
+
+def measure_traffic(args, steps, workload="rmat"):
+    """HBM bytes per unit of a workload from two rocprofv3 --pmc passes
+    (FETCH_SIZE, WRITE_SIZE) over a child run of this script: per traversal
+    of the headline BFS ("rmat") or of the Erdos-Renyi BFS ("er"), per FP32
+    PageRank sweep ("pagerank", k_pr_iter<float> alone and with its
+    k_pr_finish<float>).  Each counter is scaled by its own reading for a
+    1 GiB device copy made first in the same child, i.e. bytes = counter *
+    (2^30 / counter_of_copy); the median over the child's last `steps` units
+    is reported."""
     import shutil
     import signal
 
@@ -256,8 +327,8 @@ def measure_traffic(args, steps):
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         with tempfile.TemporaryDirectory(dir="/tmp") as td:
             cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", td, "-o", "p", "--", sys.executable,
-                   os.path.abspath(__file__), "--pmc-child", "--scale", str(args.scale), "--edgefactor",
-                   str(args.edgefactor), "--roots", str(args.roots), "--steps", str(steps)]
+                   os.path.abspath(__file__), "--pmc-child", "--pmc-workload", workload, "--scale", str(args.scale),
+                   "--edgefactor", str(args.edgefactor), "--roots", str(args.roots), "--steps", str(steps)]
             env = dict(os.environ, TMPDIR="/tmp")
             proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
                                     start_new_session=True)
@@ -266,25 +337,34 @@ def measure_traffic(args, steps):
             except subprocess.TimeoutExpired:
                 os.killpg(proc.pid, signal.SIGKILL)
                 proc.communicate()
-                log(f"traffic pass {counter} timed out")
+                log(f"traffic pass {workload} {counter} timed out")
                 return None
             rows = _counter_rows(td)
         copies = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
-        groups = _per_bfs(rows)[-steps:]
-        if not copies or not groups:
-            log(f"traffic pass {counter}: no data (rc={proc.returncode})")
+        units = {"unit": _per_sweep(rows, True), "k_pr_iter": _per_sweep(rows, False)} if workload == "pagerank" \
+            else {"unit": _per_bfs(rows)}
+        if not copies or not units["unit"]:
+            log(f"traffic pass {workload} {counter}: no data (rc={proc.returncode})")
             return None
         # the calibration copy is by far the largest copy dispatch of the child
         scale = (COPY_BYTES / 1024.0) / max(copies)
-        out[counter] = {"kib_per_bfs": float(np.median(groups)), "copy_scale": scale,
-                        "bytes_per_bfs": float(np.median(groups)) * 1024.0 * scale}
+        key = "bytes_per_sweep" if workload == "pagerank" else "bytes_per_bfs"
+        rec = {"copy_scale": scale}
+        for name, groups in units.items():
+            med = float(np.median(groups[-steps:]))
+            label = key if name == "unit" else "bytes_per_k_pr_iter"
+            rec[label] = med * 1024.0 * scale
+            if name == "unit":
+                rec["kib_per_unit"] = med
+        out[counter] = rec
     return out
 
 
 def pmc_child(args):
     """Profiled child: a 1 GiB device copy (counter calibration), then the
-    plain BFS of the headline workload as the timed region runs it (warm-up
-    roots, then one traversal per root by a re-targeted object)."""
+    workload as its timed region runs it -- the headline BFS (warm-up roots,
+    then one traversal per root by a re-targeted object), the same on the
+    Erdos-Renyi graph, or FP32 PageRank solves on directed R-MAT."""
     import torch
 
     import velographx_amd as vx
@@ -296,7 +376,20 @@ def pmc_child(args):
     torch.cuda.synchronize()
     del a, b
     n = 1 << args.scale
-    s, d = vx.rmat_edges(args.scale, args.edgefactor, 1)
+    if args.pmc_workload == "pagerank":
+        s, d = vx.rmat_edges(args.scale, args.edgefactor, 1)
+        g = vx.Graph(n, True)
+        g.bulk_load_arrays(s, d)
+        pr = vx.IncrementalPageRank(g, 0.85, 32)
+        pr.solve_l1(200, 1e-7, False)
+        pr.solve_l1(200, 1e-7, False)
+        torch.cuda.synchronize()
+        return
+    if args.pmc_workload == "er":
+        n = 1 << 22
+        s, d = wl.erdos_renyi(n, 8 * n, seed=1)
+    else:
+        s, d = vx.rmat_edges(args.scale, args.edgefactor, 1)
     g = vx.Graph(n, False)
     g.bulk_load_arrays(s, d)
     lcc, _ = component(vx, wl, g)
@@ -840,7 +933,7 @@ def leg_pagerank(vx, wl, args, no_cpu):
     errors()
     bytes_it = 8.0 * arcs + 20.0 * n
     out = {"workload": f"directed R-MAT s{scale} ef{args.edgefactor} seed 1, d=0.85, FP32, L1 residual < 1e-7",
-           "arcs": arcs, "solve_ms": 1e3 * t_solve, "iterations": iters,
+           "arcs": arcs, "solve_ms": 1e3 * t_solve, "iterations": iters, "bytes_per_iteration": bytes_it,
            "ms_per_iteration": 1e3 * t_solve / max(iters, 1),
            "gbps_per_iteration_wall": bytes_it * iters / t_solve / 1e9,
            "roofline_frac_wall": bytes_it * iters / t_solve / 1e9 / HBM_PEAK_GBPS,
@@ -1165,6 +1258,27 @@ def main():
             rf["traffic"] = traffic
             rf["traffic_detail"] = tr
             rf["traffic_frac"] = traffic / (kernel_ms_per_bfs * 1e-3) / 1e9 / HBM_PEAK_GBPS
+            rf["traffic_over_algorithmic"] = traffic / bytes_bfs
+        if "contrast" in result:
+            t0 = time.time()
+            tr = measure_traffic(args, 8, "er")
+            log(f"ER traffic passes in {time.time() - t0:.1f}s: {tr}")
+            if tr:
+                c = result["contrast"]
+                c["traffic"] = tr["FETCH_SIZE"]["bytes_per_bfs"] + tr["WRITE_SIZE"]["bytes_per_bfs"]
+                c["traffic_detail"] = tr
+                c["traffic_over_algorithmic"] = c["traffic"] / c["bytes_per_bfs"]
+        if "pagerank" in result:
+            t0 = time.time()
+            tr = measure_traffic(args, 16, "pagerank")
+            log(f"PageRank traffic passes in {time.time() - t0:.1f}s: {tr}")
+            if tr:
+                pg = result["pagerank"]
+                sweep = tr["FETCH_SIZE"]["bytes_per_sweep"] + tr["WRITE_SIZE"]["bytes_per_sweep"]
+                it = tr["FETCH_SIZE"]["bytes_per_k_pr_iter"] + tr["WRITE_SIZE"]["bytes_per_k_pr_iter"]
+                pg["traffic"] = {"bytes_per_sweep": sweep, "bytes_per_k_pr_iter": it, "detail": tr,
+                                 "algorithmic_bytes_per_sweep": pg["bytes_per_iteration"],
+                                 "sweep_over_algorithmic": sweep / pg["bytes_per_iteration"]}
     print(json.dumps(result), flush=True)
 
 

@@ -29,9 +29,10 @@
 //   cons_ctl <min_epochs> <samples> <growth>   new ConsolidationController
 //   cons_observe <sr> <lr> <se> <le> <sc> <epoch>   controller.observe
 //   cons_mark <epoch>               controller.mark_consolidated
-// Bench modes: bench-bfs, bench-pr, bench-repair, bench-snapshot,
-// bench-static, bench-kcore, bench-cc (see their comments).
+// Bench modes: bench-bfs, bench-bfs-mt, bench-pr, bench-repair,
+// bench-snapshot, bench-static, bench-kcore, bench-cc (see their comments).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -41,6 +42,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "velographx/algorithms.hpp"
@@ -288,6 +290,45 @@ int bench_bfs(int argc, char** argv) {
   return 0;
 }
 
+// bench-bfs-mt <edges.bin> <threads> <passes> <root...>
+// The root-parallel BFS of the reference's thread-scaling benchmark
+// (benchmarks/thread_scaling_bfs.cpp:78-108): an undirected CsrGraph, then
+// `threads` workers that take the next query off a shared counter and run
+// bfs_distances from its root; the queries are the roots repeated `passes`
+// times.  Prints "mt <threads> <queries> <seconds> <sum of reached>".
+int bench_bfs_mt(int argc, char** argv) {
+  const auto edges = read_edges_bin(argv[2]);
+  const unsigned threads = static_cast<unsigned>(std::stoul(argv[3]));
+  const std::size_t passes = std::stoull(argv[4]);
+  std::vector<VertexId> roots;
+  for (int i = 5; i < argc; ++i) roots.push_back(static_cast<VertexId>(std::stoul(argv[i])));
+  const velographx::CsrGraph graph(edges, false);
+  std::vector<VertexId> queries;
+  for (std::size_t p = 0; p < passes; ++p) queries.insert(queries.end(), roots.begin(), roots.end());
+  std::vector<std::uint64_t> reached(queries.size(), 0);
+  std::atomic<std::size_t> next{0};
+  const auto t0 = Clock::now();
+  std::vector<std::thread> workers;
+  for (unsigned t = 0; t < threads; ++t) {
+    workers.emplace_back([&] {
+      for (;;) {
+        const auto i = next.fetch_add(1, std::memory_order_relaxed);
+        if (i >= queries.size()) break;
+        const auto d = velographx::bfs_distances(graph, queries[i]);
+        std::uint64_t c = 0;
+        for (auto x : d) c += x != velographx::kUnreachable ? 1 : 0;
+        reached[i] = c;
+      }
+    });
+  }
+  for (auto& w : workers) w.join();
+  const double sec = std::chrono::duration<double>(Clock::now() - t0).count();
+  std::uint64_t total = 0;
+  for (auto c : reached) total += c;
+  std::printf("mt %u %zu %.9f %llu\n", threads, queries.size(), sec, static_cast<unsigned long long>(total));
+  return 0;
+}
+
 // bench-snapshot <edges.bin> <n> <directed> <repeats>
 // Builds the graph, then times consolidate_to_csr_snapshot (one thread)
 // <repeats> times, and the neighbour probe of the steady-state storage
@@ -440,6 +481,7 @@ int bench_repair(char** argv) {
 
 int main(int argc, char** argv) {
   if (argc >= 2 && std::strcmp(argv[1], "bench-bfs") == 0 && argc >= 7) return bench_bfs(argc, argv);
+  if (argc >= 6 && std::strcmp(argv[1], "bench-bfs-mt") == 0) return bench_bfs_mt(argc, argv);
   if (argc == 6 && std::strcmp(argv[1], "bench-pr") == 0) return bench_pr(argv);
   if (argc == 6 && std::strcmp(argv[1], "bench-snapshot") == 0) return bench_snapshot(argv);
   if (argc == 6 && std::strcmp(argv[1], "bench-static") == 0) return bench_static(argv);
