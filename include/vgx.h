@@ -215,6 +215,16 @@ vgx_status vgx_bfs_apply_device(vgx_bfs* b, const uint32_t* src_dev, const uint3
 vgx_status vgx_bfs_distances(vgx_bfs* b, uint32_t* out, uint64_t cap, uint64_t* count);
 /* Device pointer to the distance array (valid until the next call). */
 vgx_status vgx_bfs_distances_device(vgx_bfs* b, const uint32_t** dist_dev, uint64_t* count);
+/* Zero-copy views of that array (no reference counterpart; the DLPack
+ * views of the Python module use it): delta = +1 when a view is made, with
+ * the stream its reader queues work on (null: the default stream), -1 when
+ * it is released.  While views are held, every call that writes the
+ * distances first makes the graph's stream wait (on the device) for the
+ * work queued so far on those streams, and an array that has to grow keeps
+ * its old allocation until the last view is released (then, once those
+ * streams have finished, it is freed).  The streams must outlive their
+ * views. */
+vgx_status vgx_bfs_hold_view(vgx_bfs* b, int delta, void* stream);
 /* The graph a BFS object borrows (no reference counterpart; the device of
  * its arrays is vgx_graph_device of it). */
 vgx_status vgx_bfs_graph(const vgx_bfs* b, vgx_graph** g);
@@ -350,6 +360,8 @@ vgx_status vgx_pagerank_validate(vgx_pagerank* p, uint64_t reference_max_iterati
 vgx_status vgx_pagerank_install_reference(vgx_pagerank* p, uint64_t reference_max_iterations, double reference_tol);
 /* Device pointer to the current ranks (float32 or float64 per precision). */
 vgx_status vgx_pagerank_values_device(vgx_pagerank* p, const void** dev, uint64_t* count, int* precision);
+/* vgx_bfs_hold_view for views of the ranks. */
+vgx_status vgx_pagerank_hold_view(vgx_pagerank* p, int delta, void* stream);
 /* The graph a PageRank object borrows (no reference counterpart). */
 vgx_status vgx_pagerank_graph(const vgx_pagerank* p, vgx_graph** g);
 

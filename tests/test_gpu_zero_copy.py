@@ -147,3 +147,128 @@ def test_device_batch_errors(vx):
     v = g.version
     g.apply_tensors(e, e, torch.zeros(0, dtype=torch.uint8, device="cuda"))
     assert g.version == v
+
+
+def _delay(torch, stream):
+    """Queues a few tens of milliseconds of device work on `stream`: a bounded
+    spin where torch has one, else a chain of matrix products."""
+    with torch.cuda.stream(stream):
+        if hasattr(torch.cuda, "_sleep"):
+            torch.cuda._sleep(100_000_000)
+        else:
+            a = torch.randn(2048, 2048, device="cuda")
+            for _ in range(40):
+                a = a @ a
+                a = a / a.norm()
+
+
+def test_view_reader_on_a_side_stream_is_waited_for(vx, orc):
+    """A tensor view read on a side stream behind slow work, then an apply
+    that changes the distances: the apply waits on the device for the
+    reader, so the reader copies the distances the view had."""
+    import torch
+
+    s, d = vx.rmat_edges(12, 8, 3)
+    n = 1 << 12
+    g = vx.Graph(n, False)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(n, False)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    bfs = vx.IncrementalBFS(g, 0)
+    before = bfs.distances_numpy()
+    # a batch that moves many distances: every vertex of the component gets an arc to the root
+    reached = np.flatnonzero(before != 0xFFFFFFFF).astype(np.uint32)
+    bs = np.zeros(reached.size, np.uint32)
+    bd = reached
+    ba = np.ones(reached.size, np.uint8)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        view = bfs.distances_tensor()  # read on `side`
+    _delay(torch, side)
+    with torch.cuda.stream(side):
+        copy = view.clone()
+    bfs.apply_arrays(bs, bd, ba)  # host order: after the reader was queued
+    side.synchronize()
+    assert np.array_equal(copy.cpu().numpy(), before)
+    mg.apply(bs, bd, ba)
+    off, adj = mg.csr()
+    want, _ = orc.bfs(mg.n, off, adj, 0)
+    assert not np.array_equal(want, before)
+    assert np.array_equal(bfs.distances_numpy(), want)
+    assert np.array_equal(view.cpu().numpy(), want)  # same memory, current values
+    del view, copy
+
+
+def test_views_survive_array_growth(vx, orc):
+    """A batch that grows the vertex count past the arrays' capacity while
+    views are held: the views keep the old allocations (old contents), the
+    objects move to new ones, and releasing the views frees the old ones."""
+    import torch
+
+    s, d = vx.rmat_edges(10, 8, 5)
+    n = 1 << 10
+    g = vx.Graph(n, True)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(n, True)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    bfs = vx.IncrementalBFS(g, 1)
+    pr = vx.IncrementalPageRank(g)
+    dv, rv = bfs.distances_tensor(), pr.values_tensor()
+    d0, r0 = bfs.distances_numpy(), np.asarray(pr.values)
+    big = 40 * n  # well past the vertex capacity
+    bs = np.array([1, big - 1, 5], np.uint32)
+    bd = np.array([big - 1, 2, 6], np.uint32)
+    ba = np.ones(3, np.uint8)
+    bfs.apply_arrays(bs, bd, ba)
+    pr.apply(vx.UpdateBatch.from_arrays(bs, bd, ba))  # the vertex count changed: a full solve
+    mg.apply(bs, bd, ba)
+    torch.cuda.synchronize()
+    assert np.array_equal(dv.cpu().numpy(), d0)
+    assert np.array_equal(rv.cpu().numpy(), r0)
+    off, adj = mg.csr()
+    want, _ = orc.bfs(mg.n, off, adj, 1)
+    assert np.array_equal(bfs.distances_numpy(), want)
+    ref, _ = orc.pagerank_full(mg)
+    assert np.abs(np.asarray(pr.values) - ref).max() < 1e-10
+    del dv, rv  # the old allocations go now
+    bfs.recompute()
+    assert np.array_equal(bfs.distances_numpy(), want)
+    v2 = bfs.distances_tensor()
+    assert np.array_equal(v2.cpu().numpy(), want)
+
+
+def test_batch_from_a_slow_producer_stream(vx, orc):
+    """A device batch written by device work on a side stream queued behind
+    slow work: applying it with that stream current reads the finished ids
+    (the engine waits for the producer), not the zeros the buffers held."""
+    import torch
+
+    s, d = vx.rmat_edges(11, 16, 9)
+    n = 1 << 11
+    g = vx.Graph(n, False)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(n, False)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    bfs = vx.IncrementalBFS(g, 0)
+    rng = np.random.default_rng(4)
+    bs = rng.integers(1, n, 3000).astype(np.uint32)
+    bd = rng.integers(1, n, 3000).astype(np.uint32)
+    ba = np.ones(3000, np.uint8)
+    ba[:1000] = 0
+    host = [torch.from_numpy(x.view(np.int32) if x.dtype == np.uint32 else x).cuda() for x in (bs, bd, ba)]
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    src = torch.zeros(3000, dtype=torch.int32, device="cuda")
+    dst = torch.zeros(3000, dtype=torch.int32, device="cuda")
+    add = torch.zeros(3000, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    _delay(torch, side)
+    with torch.cuda.stream(side):
+        src.copy_(host[0])
+        dst.copy_(host[1])
+        add.copy_(host[2])
+        bfs.apply_tensors(src, dst, add)
+    mg.apply(bs, bd, ba)
+    off, adj = mg.csr()
+    want, _ = orc.bfs(mg.n, off, adj, 0)
+    assert np.array_equal(bfs.distances_numpy(), want)

@@ -103,14 +103,19 @@ def _bfs_apply_tensors(self, src, dst, add):
 
 def _bfs_distances_tensor(self):
     """The current distances as a uint32 device tensor sharing the engine's
-    array (no copy); valid until the next apply or recompute."""
-    return torch.from_dlpack(self.distances_dlpack())
+    array (no copy), for reading on the current stream: the object's next
+    apply or recompute waits on the device for the work queued there before
+    it writes the distances (the tensor then shows the new ones, or, where
+    the array had to grow, keeps the old ones), and the memory stays valid
+    while the tensor lives."""
+    return torch.from_dlpack(self.distances_dlpack(torch.cuda.current_stream().cuda_stream))
 
 
 def _pr_values_tensor(self):
     """The current ranks as a float64 (float32 at precision 32) device tensor
-    sharing the engine's array; valid until the next apply or recompute."""
-    return torch.from_dlpack(self.values_dlpack())
+    sharing the engine's array, for reading on the current stream; as
+    IncrementalBFS.distances_tensor."""
+    return torch.from_dlpack(self.values_dlpack(torch.cuda.current_stream().cuda_stream))
 
 
 Graph.apply_tensors = _graph_apply_tensors

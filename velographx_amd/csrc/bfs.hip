@@ -261,6 +261,7 @@ struct vgx_bfs {
   hipEvent_t tev[2] = {nullptr, nullptr};
   double traversal_ms = 0.0;
   uint64_t traversals = 0;
+  vgx::ViewGuard views;  // zero-copy views of w.dist
 };
 
 namespace vgx {
@@ -1114,7 +1115,7 @@ static void ensure_work(vgx_bfs* b) {
   BfsWork& w = b->w;
   if (w.cap >= g->ncap && w.cap != 0) return;
   uint64_t c = g->ncap;
-  w.dist.grow_copy(c, b->dist_len, g->stream);
+  w.dist.grow_copy(c, b->dist_len, g->stream, b->views.keep());
   uint64_t words = (c + 63) / 64;
   w.visited.ensure(words);
   w.pend.ensure(c);
@@ -1391,6 +1392,7 @@ static TraversalResult run_traversal(vgx_graph* g, const LevelArgs& a, const uns
 }
 
 static void bfs_full(vgx_bfs* b) {
+  b->views.wait(b->g->stream);
   vgx_graph* g = b->g;
   ensure_work(b);
   std::fill(std::begin(b->last_level_ms), std::end(b->last_level_ms), 0.f);
@@ -1563,6 +1565,7 @@ constexpr uint32_t kPropagationRounds = 4;
 
 static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
                       bool device = false, hipStream_t producer = nullptr) {
+  b->views.wait(b->g->stream);
   b->last_candidates = 0;
   b->last_affected = 0;
   b->last_relaxed = 0;
@@ -1819,6 +1822,7 @@ vgx_status vgx_bfs_destroy(vgx_bfs* b) {
   for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
   for (hipEvent_t e : b->tev)
     if (e) (void)hipEventDestroy(e);
+  b->views.destroy();
   delete b;
   VGX_API_END
 }
@@ -1888,6 +1892,15 @@ vgx_status vgx_bfs_graph(const vgx_bfs* b, vgx_graph** g) {
   VGX_API_BEGIN
   if (!b || !g) throw Error(VGX_INVALID_ARGUMENT, "null argument");
   *g = b->g;
+  VGX_API_END
+}
+
+vgx_status vgx_bfs_hold_view(vgx_bfs* b, int delta, void* stream) {
+  VGX_API_BEGIN
+  if (!b || (delta != 1 && delta != -1)) throw Error(VGX_INVALID_ARGUMENT, "delta must be +1 or -1");
+  VGX_HIP(hipSetDevice(b->g->device));
+  if (delta > 0) b->views.hold(static_cast<hipStream_t>(stream));
+  else b->views.release();
   VGX_API_END
 }
 

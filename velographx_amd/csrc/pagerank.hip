@@ -9,16 +9,18 @@
 //   validate_against_full / apply_validated :204-258
 // plus the static push pagerank of src/algorithms.cpp:56-81 (L1 stop).
 //
-// One iteration is one launch over the graph's live in-rows (deleted
-// entries tombstoned and skipped).  Vertices are split by stored row length:
-// rows of <= 32 entries are summed by groups of 8 lanes, rows up to 4096 by one wave,
-// larger rows in 4096-entry chunks by several waves whose
-// partial sums the last workgroup combines.  The last workgroup also reduces
-// the per-workgroup residual and dangling partials in a fixed order, sets
-// the next iteration's dangling share and the convergence flag, so the host
-// only polls every few iterations.  FP64 reproduces the reference's
-// arithmetic; FP32 keeps ranks and sums in float with Kahan-compensated
-// lane sums.
+// One iteration is two launches over the graph's live in-rows (deleted
+// entries tombstoned and skipped).  k_pr_iter splits the vertices by stored
+// row length: rows without entries take the base rank, rows of <= 32
+// entries are summed by groups of 8 lanes, rows up to 4096 by one wave, and
+// larger rows in 4096-entry chunks by several waves that leave partial
+// sums; every workgroup leaves its residual and dangling partials.
+// k_pr_finish then sums each large row's chunk partials in chunk order and
+// its last workgroup reduces all partials in a fixed order, sets the next
+// iteration's dangling share and the convergence flag, so the host only
+// polls every few iterations and the sweep kernel needs no fence.  FP64
+// reproduces the reference's arithmetic; FP32 keeps ranks and sums in float
+// with Kahan-compensated lane sums.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -78,6 +80,7 @@ struct PushState {
 
 struct vgx_pagerank {
   vgx_graph* g = nullptr;
+  vgx::ViewGuard views;  // zero-copy views of rank[cur]
   double damping = 0.85;
   int precision = 64;
   uint64_t n = 0;
@@ -980,8 +983,9 @@ static int full_solve(vgx_pagerank* p, DevBuf<uint8_t>* rk, int precision, uint6
   const uint64_t n = g->n;
   const size_t es = precision == 32 ? 4 : 8;
   build_lists(p);
-  rk[0].ensure(n * es + 64);
-  rk[1].ensure(n * es + 64);
+  std::vector<void*>* keep = rk == p->rank ? p->views.keep() : nullptr;
+  rk[0].ensure(n * es + 64, keep);
+  rk[1].ensure(n * es + 64, keep);
   p->contrib[0].ensure(n * es + 64);
   p->contrib[1].ensure(n * es + 64);
   PrState h{};
@@ -1562,6 +1566,7 @@ vgx_status vgx_pagerank_destroy(vgx_pagerank* p) {
   if (p->h_st) (void)hipHostFree(p->h_st);
   if (p->d_push) dev_free(p->d_push);
   if (p->h_push) (void)hipHostFree(p->h_push);
+  p->views.destroy();
   delete p;
   VGX_API_END
 }
@@ -1570,6 +1575,7 @@ vgx_status vgx_pagerank_recompute(vgx_pagerank* p, uint64_t max_iterations, doub
   VGX_API_BEGIN
   if (!p) throw Error(VGX_INVALID_ARGUMENT, "null pagerank");
   VGX_HIP(hipSetDevice(p->g->device));
+  p->views.wait(p->g->stream);
   recompute(p, max_iterations, tol);
   VGX_API_END
 }
@@ -1578,6 +1584,7 @@ vgx_status vgx_pagerank_solve_l1(vgx_pagerank* p, uint64_t max_iterations, doubl
   VGX_API_BEGIN
   if (!p) throw Error(VGX_INVALID_ARGUMENT, "null pagerank");
   VGX_HIP(hipSetDevice(p->g->device));
+  p->views.wait(p->g->stream);
   const bool warm = warm_start && p->have_values && p->n == p->g->n;
   if (warm && p->cur != 0) {
     // warm start reads rank[0]
@@ -1628,6 +1635,7 @@ vgx_status vgx_pagerank_apply(vgx_pagerank* p, const uint32_t* src, const uint32
   VGX_API_BEGIN
   if (!p || (k && (!src || !dst || !add))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
   VGX_HIP(hipSetDevice(p->g->device));
+  p->views.wait(p->g->stream);
   apply(p, src, dst, add, k, local_iterations, tol, full_fallback_fraction);
   VGX_API_END
 }
@@ -1641,6 +1649,7 @@ vgx_status vgx_pagerank_apply_push(vgx_pagerank* p, const uint32_t* src, const u
   if (!(threshold >= 0.0) || !(max_push_fraction >= 0.0))
     throw Error(VGX_INVALID_ARGUMENT, "threshold and max_push_fraction must be non-negative");
   VGX_HIP(hipSetDevice(p->g->device));
+  p->views.wait(p->g->stream);
   if (p->precision == 32)
     apply_push<float>(p, src, dst, add, k, l1_tol, threshold, max_push_fraction, max_sweeps, out);
   else
@@ -1702,10 +1711,11 @@ vgx_status vgx_pagerank_install_reference(vgx_pagerank* p, uint64_t reference_ma
   VGX_API_BEGIN
   if (!p) throw Error(VGX_INVALID_ARGUMENT, "null pagerank");
   VGX_HIP(hipSetDevice(p->g->device));
+  p->views.wait(p->g->stream);
   int r = full_solve(p, p->ref_rank, 64, reference_max_iterations, reference_tol, false, false);
   vgx_graph* g = p->g;
   const uint64_t n = g->n;
-  p->rank[0].ensure(n * elt(p) + 64);
+  p->rank[0].ensure(n * elt(p) + 64, p->views.keep());
   if (n) {
     if (p->precision == 64) {
       VGX_HIP(hipMemcpyAsync(p->rank[0].p, p->ref_rank[r].p, n * 8, hipMemcpyDeviceToDevice, g->stream));
@@ -1737,6 +1747,15 @@ vgx_status vgx_pagerank_graph(const vgx_pagerank* p, vgx_graph** g) {
 }
 
 // Device pointer to the current ranks (float32 or float64 per precision).
+vgx_status vgx_pagerank_hold_view(vgx_pagerank* p, int delta, void* stream) {
+  VGX_API_BEGIN
+  if (!p || (delta != 1 && delta != -1)) throw Error(VGX_INVALID_ARGUMENT, "delta must be +1 or -1");
+  VGX_HIP(hipSetDevice(p->g->device));
+  if (delta > 0) p->views.hold(static_cast<hipStream_t>(stream));
+  else p->views.release();
+  VGX_API_END
+}
+
 vgx_status vgx_pagerank_values_device(vgx_pagerank* p, const void** dev, uint64_t* count, int* precision) {
   VGX_API_BEGIN
   if (!p || !dev) throw Error(VGX_INVALID_ARGUMENT, "null argument");

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "vgx.h"
 
@@ -68,19 +69,79 @@ struct DevBuf {
     p = static_cast<T*>(dev_alloc(c * sizeof(T)));
     n = c;
   }
-  void grow_copy(size_t m, size_t used, hipStream_t s) {
+  // The same, keeping the old allocation in *keep (not freed) when given.
+  void ensure(size_t m, std::vector<void*>* keep) {
+    if (m <= n) return;
+    if (keep && p) {
+      keep->push_back(p);
+      p = nullptr;
+      n = 0;
+    }
+    ensure(m);
+  }
+  // Capacity for m elements keeping the first `used`; the old allocation
+  // goes to *keep instead of being freed when keep is given.
+  void grow_copy(size_t m, size_t used, hipStream_t s, std::vector<void*>* keep = nullptr) {
     if (m <= n) return;
     size_t c = m < 1024 ? 1024 : m;
     T* q = static_cast<T*>(dev_alloc(c * sizeof(T)));
     if (used) VGX_HIP(hipMemcpyAsync(q, p, used * sizeof(T), hipMemcpyDeviceToDevice, s));
     VGX_HIP(hipStreamSynchronize(s));
-    release();
+    if (keep && p) keep->push_back(p);
+    else release();
     p = q;
     n = c;
   }
   void swap(DevBuf& o) {
     std::swap(p, o.p);
     std::swap(n, o.n);
+  }
+};
+
+// An array that zero-copy views alias (a traversal's distances, a PageRank
+// object's ranks).  While views are held, the engine's stream waits (on the
+// device) for the work queued so far on the streams the views were taken
+// for before each write to the array, and an allocation the array outgrows
+// is kept, not freed, until the last view is released (then, once those
+// streams have finished, it is freed).
+struct ViewGuard {
+  int views = 0;
+  std::vector<hipStream_t> streams;
+  std::vector<void*> retired;
+  hipEvent_t ev = nullptr;
+
+  std::vector<void*>* keep() { return views > 0 ? &retired : nullptr; }
+  void hold(hipStream_t s) {
+    ++views;
+    for (hipStream_t x : streams)
+      if (x == s) return;
+    streams.push_back(s);
+  }
+  void wait(hipStream_t engine) {
+    if (views <= 0) return;
+    if (!ev) VGX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    for (hipStream_t x : streams) {
+      VGX_HIP(hipEventRecord(ev, x));
+      VGX_HIP(hipStreamWaitEvent(engine, ev, 0));
+    }
+  }
+  void drop_retired() {
+    for (hipStream_t x : streams) VGX_HIP(hipStreamSynchronize(x));
+    for (void* q : retired) dev_free(q);
+    retired.clear();
+  }
+  void release() {
+    if (views <= 0) throw Error(VGX_INVALID_ARGUMENT, "no view is held");
+    if (--views > 0) return;
+    if (!retired.empty()) drop_retired();
+    streams.clear();
+  }
+  // At the owner's destruction (no view can be out: a view keeps its owner).
+  void destroy() {
+    for (void* q : retired) dev_free(q);
+    retired.clear();
+    if (ev) (void)hipEventDestroy(ev);
+    ev = nullptr;
   }
 };
 

@@ -164,7 +164,8 @@ py::array_t<T> to_numpy(std::vector<T>&& v) {
 // DLPack (the published v0 ABI of dlpack.h: DLDevice, DLDataType, DLTensor,
 // DLManagedTensor) for zero-copy views of device arrays, consumed by
 // torch.from_dlpack.  The view keeps the owning Python object alive until
-// the consumer releases it.
+// the consumer releases it, and holds the engine's view guard on the array
+// (vgx_bfs_hold_view / vgx_pagerank_hold_view) until then.
 struct DLDevice {
   std::int32_t device_type;  // 10 = kDLROCM
   std::int32_t device_id;
@@ -192,20 +193,25 @@ struct DLManagedTensor {
 struct DlpackCtx {
   std::int64_t shape[1];
   py::object owner;
+  void (*release)(void*);  // drops the view's hold on the engine's array
+  void* handle;
 };
 
 void dlpack_delete(DLManagedTensor* t) {
   auto* ctx = static_cast<DlpackCtx*>(t->manager_ctx);
   {
     py::gil_scoped_acquire gil;
+    if (ctx->release) ctx->release(ctx->handle);
     delete ctx;
   }
   delete t;
 }
 
-// A 1-D view of `count` elements at device pointer `data` on `device`.
-py::capsule dlpack_view(void* data, std::uint64_t count, int device, DLDataType dtype, py::object owner) {
-  auto* ctx = new DlpackCtx{{static_cast<std::int64_t>(count)}, std::move(owner)};
+// A 1-D view of `count` elements at device pointer `data` on `device`; the
+// caller has taken the hold that `release(handle)` gives back.
+py::capsule dlpack_view(void* data, std::uint64_t count, int device, DLDataType dtype, py::object owner,
+                        void (*release)(void*), void* handle) {
+  auto* ctx = new DlpackCtx{{static_cast<std::int64_t>(count)}, std::move(owner), release, handle};
   auto* t = new DLManagedTensor{};
   t->dl_tensor.data = data;
   t->dl_tensor.device = DLDevice{10, device};
@@ -600,17 +606,23 @@ PYBIND11_MODULE(_velographx, m) {
       .def("distances_numpy", [](const IncrementalBFS& x) { return to_numpy(x.distances()); })
       .def(
           "distances_dlpack",
-          [](py::object self) {
+          [](py::object self, std::uintptr_t stream) {
             const auto& b = self.cast<const IncrementalBFS&>();
             const std::uint32_t* dev = nullptr;
             std::uint64_t n = 0;
             velographx::vgx_check(vgx_bfs_distances_device(b.handle(), &dev, &n));
             vgx_graph* g = nullptr;
             velographx::vgx_check(vgx_bfs_graph(b.handle(), &g));
-            return dlpack_view(const_cast<std::uint32_t*>(dev), n, device_of(g), DLDataType{1, 32, 1}, self);
+            velographx::vgx_check(vgx_bfs_hold_view(b.handle(), 1, reinterpret_cast<void*>(stream)));
+            return dlpack_view(
+                const_cast<std::uint32_t*>(dev), n, device_of(g), DLDataType{1, 32, 1}, self,
+                [](void* h) { (void)vgx_bfs_hold_view(static_cast<vgx_bfs*>(h), -1, nullptr); }, b.handle());
           },
-          "Zero-copy DLPack view (uint32, on the device) of the current distances; valid until the "
-          "object's next apply or recompute.")
+          py::arg("stream") = 0,
+          "Zero-copy DLPack view (uint32, on the device) of the current distances, read on `stream` "
+          "(a hipStream_t; 0: the default stream).  Its contents are those of the current distances "
+          "until the object's next apply or recompute, which first waits on the device for the "
+          "work queued on `stream` so far; the memory stays valid while the view lives.")
       .def(
           "_apply_device",
           [](IncrementalBFS& b, std::uintptr_t src, std::uintptr_t dst, std::uintptr_t add, std::uint64_t k,
@@ -731,7 +743,7 @@ PYBIND11_MODULE(_velographx, m) {
       .def_property_readonly("values", [](const IncrementalPageRank& x) { return x.values(); })
       .def(
           "values_dlpack",
-          [](py::object self) {
+          [](py::object self, std::uintptr_t stream) {
             const auto& pr = self.cast<const IncrementalPageRank&>();
             const void* dev = nullptr;
             std::uint64_t n = 0;
@@ -740,10 +752,15 @@ PYBIND11_MODULE(_velographx, m) {
             vgx_graph* g = nullptr;
             velographx::vgx_check(vgx_pagerank_graph(pr.handle(), &g));
             const DLDataType dt{2, static_cast<std::uint8_t>(precision), 1};
-            return dlpack_view(const_cast<void*>(dev), n, device_of(g), dt, self);
+            velographx::vgx_check(vgx_pagerank_hold_view(pr.handle(), 1, reinterpret_cast<void*>(stream)));
+            return dlpack_view(
+                const_cast<void*>(dev), n, device_of(g), dt, self,
+                [](void* h) { (void)vgx_pagerank_hold_view(static_cast<vgx_pagerank*>(h), -1, nullptr); },
+                pr.handle());
           },
+          py::arg("stream") = 0,
           "Zero-copy DLPack view (float64, or float32 at precision 32, on the device) of the current "
-          "ranks; valid until the object's next apply or recompute.")
+          "ranks, read on `stream` (0: the default stream); as distances_dlpack.")
       .def("apply", &IncrementalPageRank::apply, py::arg("batch"), py::arg("local_iterations") = 24,
            py::arg("tol") = 1e-9, py::arg("full_fallback_fraction") = 0.60, py::call_guard<py::gil_scoped_release>())
       .def("recompute", &IncrementalPageRank::recompute, py::arg("max_iterations") = 200, py::arg("tol") = 1e-12,
