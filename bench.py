@@ -752,6 +752,8 @@ def leg_kcore(vx, args, with_cpu):
                                 "sample": "IncrementalKCore(graph) construction of the reference engine on the "
                                           "same edges"}
         out["max_core_equal"] = int(w[2]) == out["max_core"]
+        # every core number, by the digest of the whole vector
+        out["core_numbers_equal"] = int(w[3]) == vertex_digest(core)
         with tempfile.TemporaryDirectory() as td:
             cmd = [REF_BIN, "bench-cc", _edges_file(td, s, d), str(n)]
             res = subprocess.run(cmd, capture_output=True, text=True, timeout=900, check=True).stdout
@@ -760,9 +762,24 @@ def leg_kcore(vx, args, with_cpu):
                                               "kind": "reference",
                                               "sample": "IncrementalComponents(graph) construction of the "
                                                         "reference engine on the same edges"}
+        # the partition, by the digest of each vertex's smallest component member
+        out["components"]["partition_equal"] = int(w[2]) == ncomp and int(w[3]) == vertex_digest(labels)
     del g
     log(f"kcore: {out}")
     return out
+
+
+def vertex_digest(values):
+    """The reference driver's order-free digest of a per-vertex array: the
+    sum mod 2^64 of splitmix64's output function of (v << 32 | value[v])."""
+    v = np.arange(values.size, dtype=np.uint64)
+    x = (v << np.uint64(32)) | values.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+        return int(np.sum(x, dtype=np.uint64))
 
 
 def cpu_static(s, d, root):

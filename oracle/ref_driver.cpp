@@ -386,9 +386,26 @@ int bench_static(char** argv) {
   return d.empty() || pr.empty() ? 1 : 0;
 }
 
+// Order-free digest of a per-vertex array: the sum mod 2^64 of splitmix64's
+// output function (Steele, Lea and Flood, OOPSLA 2014) applied to
+// (v << 32 | value[v]), equal for two arrays exactly when (with
+// overwhelming probability) every entry is.
+std::uint64_t vertex_digest(const std::vector<std::uint32_t>& value) {
+  std::uint64_t h = 0;
+  for (std::size_t v = 0; v < value.size(); ++v) {
+    std::uint64_t x = (static_cast<std::uint64_t>(v) << 32) | value[v];
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    h += x ^ (x >> 31);
+  }
+  return h;
+}
+
 // bench-kcore <edges.bin> <n>
 // Times IncrementalKCore(graph) on the undirected graph (its constructor
-// peels every vertex, one thread).  Prints "kcore <seconds> <max_core>".
+// peels every vertex, one thread).  Prints "kcore <seconds> <max_core>
+// <digest>" with the digest of the core numbers.
 int bench_kcore(char** argv) {
   const auto edges = read_edges_bin(argv[2]);
   const std::size_t n = std::stoull(argv[3]);
@@ -399,14 +416,16 @@ int bench_kcore(char** argv) {
   const auto t1 = Clock::now();
   std::uint32_t mx = 0;
   for (auto c : kc.core()) mx = std::max(mx, c);
-  std::printf("kcore %.9f %u\n", std::chrono::duration<double>(t1 - t0).count(), mx);
+  std::printf("kcore %.9f %u %llu\n", std::chrono::duration<double>(t1 - t0).count(), mx,
+              static_cast<unsigned long long>(vertex_digest(kc.core())));
   return 0;
 }
 
 // bench-cc <edges.bin> <n>
 // Times IncrementalComponents(graph) on the undirected graph (its
-// constructor unions every arc, one thread).  Prints "cc <seconds> <count>"
-// with the number of components.
+// constructor unions every arc, one thread).  Prints "cc <seconds> <count>
+// <digest>" with the number of components and the digest of the partition
+// labelled by each component's smallest vertex.
 int bench_cc(char** argv) {
   const auto edges = read_edges_bin(argv[2]);
   const std::size_t n = std::stoull(argv[3]);
@@ -416,9 +435,17 @@ int bench_cc(char** argv) {
   velographx::IncrementalComponents cc(g);
   const auto t1 = Clock::now();
   std::size_t roots = 0;
-  for (std::size_t v = 0; v < g.vertex_count(); ++v)
-    if (cc.component(static_cast<VertexId>(v)) == v) ++roots;
-  std::printf("cc %.9f %zu\n", std::chrono::duration<double>(t1 - t0).count(), roots);
+  const std::size_t nv = g.vertex_count();
+  // the partition, labelled by each component's smallest vertex
+  std::vector<std::uint32_t> root(nv), smallest(nv, ~0u), canon(nv);
+  for (std::size_t v = 0; v < nv; ++v) {
+    root[v] = cc.component(static_cast<VertexId>(v));
+    if (root[v] == v) ++roots;
+    smallest[root[v]] = std::min<std::uint32_t>(smallest[root[v]], static_cast<std::uint32_t>(v));
+  }
+  for (std::size_t v = 0; v < nv; ++v) canon[v] = smallest[root[v]];
+  std::printf("cc %.9f %zu %llu\n", std::chrono::duration<double>(t1 - t0).count(), roots,
+              static_cast<unsigned long long>(vertex_digest(canon)));
   return 0;
 }
 

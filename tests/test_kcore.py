@@ -189,3 +189,39 @@ def test_region_repair_on_separate_components(orc):
         assert np.array_equal(kc.core_numpy(), cores(mg))
         rep = okc.affected_count(before, after, bs, bd)
         assert kc.last_repaired_vertices == rep and rep < mg.n
+
+
+@pytest.mark.skipif(not refdriver.available(), reason="oracle/_ref (compiled reference) not built")
+def test_bench_digests_equal_the_reference_drivers(orc):
+    """The digests bench.py compares at s20 (every core number; the
+    partition labelled by smallest members) are the reference driver's: on
+    a small graph the driver's digests of the reference's IncrementalKCore
+    and IncrementalComponents equal bench.vertex_digest of the oracle's core
+    numbers and smallest-member labels, and a single changed core number
+    changes the digest."""
+    import os
+    import subprocess
+    import tempfile
+
+    import bench
+    from oracle import components as occ
+
+    rng = np.random.default_rng(21)
+    n = 3000
+    e = rng.integers(0, n, size=(12000, 2)).astype(np.uint32)
+    g = model(orc, n, e)
+    off, adj = g.csr()
+    core = cores(g)
+    labels = np.asarray(occ.min_labels(n, off, adj), dtype=np.uint32)
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "e.bin")
+        e.tofile(p)
+        k = subprocess.run([refdriver.REF_BIN, "bench-kcore", p, str(n)], capture_output=True, text=True,
+                           check=True).stdout.split()
+        c = subprocess.run([refdriver.REF_BIN, "bench-cc", p, str(n)], capture_output=True, text=True,
+                           check=True).stdout.split()
+    assert int(k[3]) == bench.vertex_digest(core)
+    assert int(c[3]) == bench.vertex_digest(labels)
+    changed = core.copy()
+    changed[17] += 1
+    assert bench.vertex_digest(changed) != int(k[3])

@@ -3,6 +3,8 @@ reference policy (oracle/selector.py) is pinned by hand cases that walk each
 branch of benchmarks/publication_policy_bfs.cpp:162-233, then the library's
 native policy (vgx_policy_*, host code, no device work) must make the same
 decisions with the same predictions on random replayed traces."""
+import os
+
 import numpy as np
 import pytest
 
@@ -110,3 +112,46 @@ def test_native_policy_matches_oracle(vertices, seed):
         ref.observe(r, fallback, cost, affected)
         t.update({"execution_us": cost, "internal_fallback": fallback, "affected_vertices": affected})
         nat.observe(t)
+
+
+POLICY_BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                          "policy_bfs")
+
+
+@pytest.mark.skipif(not os.path.exists(POLICY_BIN),
+                    reason="oracle/_ref/policy_bfs (the reference's publication_policy_bfs.cpp) not built")
+@pytest.mark.parametrize("scale,edgefactor,batch", [(11, 8, 300), (12, 16, 400), (18, 2, 8000), (18, 2, 3000)])
+def test_policy_replay_of_the_compiled_reference(scale, edgefactor, batch, tmp_path):
+    """The reference's own harness (benchmarks/publication_policy_bfs.cpp,
+    compiled as it is) runs its adaptive policy on a sliding-window stream
+    over an R-MAT edge list; each batch's recorded features and measured
+    execution time, replayed through oracle/selector.py, give the same
+    reason, choice and predictions (the harness prints six significant
+    digits, so predictions agree to that).  Scale 18 exercises the
+    large-graph rules, scale 11-12 the small-graph ones."""
+    import json
+    import subprocess
+
+    vx = pytest.importorskip("velographx_amd")
+    s, d = vx.rmat_edges(scale, edgefactor, 3)
+    path = tmp_path / "edges.txt"
+    path.write_text("\n".join(f"{u} {v}" for u, v in zip(s.tolist(), d.tolist())))
+    out = subprocess.run([POLICY_BIN, str(path), "0", "0.9", str(batch), "0.01"], capture_output=True, text=True,
+                         timeout=300, check=True).stdout
+    run = json.loads(out)
+    ad = next(p for p in run["policies"] if p["name"] == "adaptive")
+    traces, exec_us, fallback = ad["trace"], ad["execution_us"], ad["internal_full_fallback"]
+    n = run["vertices"]
+    pol = osel.Policy(n, traces[0]["predicted_full_us"], traces[0]["reachable_fraction"])
+    reasons = set()
+    for i, t in enumerate(traces):
+        r = pol.decide(t["update_fraction"], t["shallow_parent_deletion_fraction"], i == 0)
+        assert r["reason"] == t["reason"], (i, r, t)
+        assert r["chose_full"] == t["explicit_full"], i
+        for key in ("predicted_incremental_us", "predicted_full_us"):
+            assert r[key] == pytest.approx(t[key], rel=2e-5, abs=1e-9), (i, key)
+        reasons.add(t["reason"])
+        # the next batch's trace records the affected fraction this one left
+        nxt = traces[i + 1]["previous_affected_fraction"] if i + 1 < len(traces) else 0.0
+        pol.observe(r, fallback[i], exec_us[i], nxt * n)
+    assert len(traces) >= 5 and len(reasons) >= 2
