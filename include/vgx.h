@@ -68,6 +68,29 @@ vgx_status vgx_graph_destroy(vgx_graph* g);
  * edges, mirrored when undirected; grows to max id + 1. */
 vgx_status vgx_graph_bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t* dst, uint64_t m);
 
+/* vgx_graph_bulk_load over an edge list already in device memory -- the
+ * device form of from_numpy_edges / from_arrow_columns (python/bindings.cpp
+ * :21-25, :34), whose host forms copy the list through an UpdateBatch.
+ * Edge i is (src_dev[i * stride], dst_dev[i * stride]); ids are id_bytes
+ * wide (4: uint32 or int32, a negative int32 being out of range; 8: int64)
+ * and must lie in [0, 2^31 - 1) (VGX_INVALID_ARGUMENT otherwise, checked on
+ * the device before the graph changes).  `stream` is the stream the list was
+ * written on (null: the default stream); the call waits for it.  Replaces
+ * the adjacency, grows to max id + 1 over non-loop edges, version += 1. */
+vgx_status vgx_graph_bulk_load_device(vgx_graph* g, const void* src_dev, const void* dst_dev, uint64_t m,
+                                      uint64_t stride, uint32_t id_bytes, void* stream);
+
+/* from_scipy_csr (python/bindings.cpp:26-32) for a CSR already in device
+ * memory: indptr_dev[vertices + 1] and indices_dev[nnz], both index_bytes
+ * wide (4: int32, 8: int64).  Same checks and messages as the host form
+ * ("invalid CSR offsets", "CSR contains out-of-range vertex id"), made on the
+ * device before the graph changes; an undirected graph keeps the entries with
+ * v >= u and mirrors them; loops are dropped.  The graph grows to `vertices`;
+ * when at least one entry is kept its adjacency is replaced and version
+ * += 1 (the host form applies one UpdateBatch of those entries). */
+vgx_status vgx_graph_bulk_load_csr_device(vgx_graph* g, const void* indptr_dev, const void* indices_dev,
+                                          uint64_t vertices, uint64_t nnz, uint32_t index_bytes, void* stream);
+
 /* DynamicGraph::apply(UpdateBatch) -- dynamic_graph.hpp:543-549, 717-733.
  * add[i] != 0 inserts (src[i], dst[i]), 0 removes it; sequential semantics
  * (last op on an edge wins), self-loops ignored, vertex growth to max id+1,

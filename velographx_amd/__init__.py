@@ -122,3 +122,133 @@ Graph.apply_tensors = _graph_apply_tensors
 IncrementalBFS.apply_tensors = _bfs_apply_tensors
 IncrementalBFS.distances_tensor = _bfs_distances_tensor
 IncrementalPageRank.values_tensor = _pr_values_tensor
+
+
+# ------------------------------------------------ edge lists in device memory
+# from_numpy_edges / from_scipy_csr / from_arrow_columns / from_arrow_table
+# (python/bindings.cpp:21-36) read host arrays.  Given arrays that live in
+# device memory (torch tensors on the GPU, or any DLPack producer on a ROCm
+# device), the same functions build the graph from them where they lie: the
+# ids are checked and sorted into rows on the device and only two 8-byte
+# results cross PCIe.  Host inputs keep the host path unchanged.
+
+_host_from_numpy_edges = from_numpy_edges
+_host_from_scipy_csr = from_scipy_csr
+_host_from_arrow_columns = from_arrow_columns
+_KDL_ROCM = 10
+
+
+def _device_tensor(x):
+    """x as a torch device tensor when it lives in device memory, else None."""
+    if isinstance(x, torch.Tensor):
+        return x if x.is_cuda else None
+    dev = getattr(x, "__dlpack_device__", None)
+    if dev is not None and not isinstance(x, type):
+        kind, _ = dev()
+        if kind == _KDL_ROCM:
+            return torch.from_dlpack(x)
+    return None
+
+
+def _id_tensor(t, what):
+    """t with a dtype the device build reads (int32, uint32, int64); narrower
+    integer types are widened, anything else is refused."""
+    if t.dtype in (torch.int32, torch.uint32, torch.int64):
+        return t
+    if t.dtype in (torch.int8, torch.int16, torch.uint8, torch.uint16):
+        return t.to(torch.int32)
+    raise TypeError(f"{what} must hold integer ids, not {t.dtype}")
+
+
+def _edges_on_device(src, dst, directed):
+    """Graph(0, directed) + one UpdateBatch of additions (src[i], dst[i]), the
+    lists in device memory: bulk-loaded on the device (same adjacency, vertex
+    count and version as the batch; version stays 0 for an empty list)."""
+    if src.device != dst.device:
+        raise ValueError("source and destination ids must be on one device")
+    if src.dtype != dst.dtype:
+        src, dst = src.to(torch.int64), dst.to(torch.int64)
+    with torch.cuda.device(src.device):
+        g = Graph(0, directed)
+        m = src.shape[0]
+        if m == 0:
+            return g
+        stride = src.stride(0)
+        if stride <= 0 or dst.stride(0) != stride:
+            src, dst = src.contiguous(), dst.contiguous()
+            stride = 1
+        stream = torch.cuda.current_stream(src.device).cuda_stream
+        g._bulk_load_device(src.data_ptr(), dst.data_ptr(), m, stride, src.element_size(), stream)
+    return g
+
+
+def from_numpy_edges(edges, directed=False):
+    """python/bindings.cpp:21-25: a Graph from an (N, 2) array of edges.
+    Device arrays (torch GPU tensors, DLPack producers on a ROCm device) are
+    read in place on the device."""
+    t = _device_tensor(edges)
+    if t is None:
+        return _host_from_numpy_edges(edges, directed)
+    if t.dim() != 2 or t.shape[1] != 2:
+        raise ValueError("edge array must have shape (N, 2)")
+    t = _id_tensor(t, "edge array")
+    return _edges_on_device(t[:, 0], t[:, 1], directed)
+
+
+def from_arrow_columns(src, dst, directed=False):
+    """python/bindings.cpp:34: a Graph from source and destination columns;
+    device columns are read in place."""
+    s, d = _device_tensor(src), _device_tensor(dst)
+    if s is None and d is None:
+        return _host_from_arrow_columns(src, dst, directed)
+    if s is None or d is None:
+        raise ValueError("source and destination columns must both be in device memory or both on the host")
+    if s.dim() != 1 or d.dim() != 1:
+        raise ValueError("columns must be one-dimensional")
+    if s.shape[0] != d.shape[0]:
+        raise ValueError("Arrow source and destination columns must have equal length")
+    return _edges_on_device(_id_tensor(s, "source column"), _id_tensor(d, "destination column"), directed)
+
+
+def from_arrow_table(table, src="src", dst="dst", directed=False):
+    """python/bindings.cpp:35: from_arrow_columns over two named columns."""
+    if not hasattr(table, "column"):
+        raise ValueError("expected an Arrow-like table exposing column(name)")
+    return from_arrow_columns(table.column(src), table.column(dst), directed)
+
+
+def from_scipy_csr(csr, directed=False):
+    """python/bindings.cpp:26-32: a Graph from a square CSR matrix (an
+    undirected graph takes the entries with v >= u).  A torch sparse CSR
+    tensor on the GPU, or a CSR-like object whose indptr and indices are
+    device arrays, is read in place."""
+    if isinstance(csr, torch.Tensor) and csr.layout == torch.sparse_csr and csr.is_cuda:
+        indptr, indices, shape = csr.crow_indices(), csr.col_indices(), tuple(csr.shape)
+    else:
+        if not (hasattr(csr, "indptr") and hasattr(csr, "indices") and hasattr(csr, "shape")):
+            raise ValueError("expected a SciPy CSR-like object with indptr, indices, and shape")
+        indptr, indices = _device_tensor(csr.indptr), _device_tensor(csr.indices)
+        if indptr is None and indices is None:
+            return _host_from_scipy_csr(csr, directed)
+        if indptr is None or indices is None:
+            raise ValueError("indptr and indices must both be in device memory or both on the host")
+        shape = tuple(csr.shape)
+    if len(shape) != 2 or shape[0] != shape[1]:
+        raise ValueError("CSR graph matrix must be square")
+    n = int(shape[0])
+    if indptr.dim() != 1 or indptr.shape[0] != n + 1:
+        raise ValueError("invalid CSR indptr length")
+    if indptr.device != indices.device:
+        raise ValueError("indptr and indices must be on one device")
+    if indptr.dtype == torch.int32 and indices.dtype == torch.int32:
+        wide = torch.int32
+    else:
+        wide = torch.int64
+    indptr = _id_tensor(indptr, "indptr").to(wide).contiguous()
+    indices = _id_tensor(indices, "indices").to(wide).contiguous()
+    with torch.cuda.device(indptr.device):
+        g = Graph(n, directed)
+        stream = torch.cuda.current_stream(indptr.device).cuda_stream
+        g._bulk_load_csr_device(indptr.data_ptr(), indices.data_ptr() if indices.numel() else indptr.data_ptr(),
+                                n, indices.numel(), indptr.element_size(), stream)
+    return g

@@ -18,6 +18,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -855,6 +856,219 @@ static void bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t* dst, ui
   g->rows_epoch += 1;
 }
 
+// ------------------------------------------------- edge lists in device memory
+//
+// The same builds from edge lists that already sit in HBM (a torch tensor, a
+// DLPack column): nothing crosses PCIe but two 8-byte results.  Ids come as
+// 32-bit (signed or unsigned: both read as uint32, so a negative int32 is a
+// large id and is refused) or int64 words; a list is checked on the device
+// before any state changes: `bad` counts ids outside [0, 2^31 - 1), and the
+// extent is the largest endpoint + 1 over the non-loop pairs.
+
+template <class T>
+__device__ __forceinline__ bool edge_id(const T* p, uint64_t i, uint32_t& out) {
+  const T x = p[i];
+  if (sizeof(T) == 8) {
+    const long long y = static_cast<long long>(x);
+    if (y < 0 || y >= static_cast<long long>(kIdMask)) return false;
+    out = static_cast<uint32_t>(y);
+    return true;
+  }
+  out = static_cast<uint32_t>(x);
+  return out < kIdMask;
+}
+
+// Pair list: src[i * stride], dst[i * stride].
+template <class T>
+__global__ void k_edges_check(const T* __restrict__ src, const T* __restrict__ dst, uint64_t m, uint64_t stride,
+                              unsigned long long* __restrict__ res) {
+  unsigned long long ext = 0, bad = 0;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < m;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t u, v;
+    const bool ok = edge_id(src, i * stride, u) & edge_id(dst, i * stride, v);
+    if (!ok)
+      ++bad;
+    else if (u != v)
+      ext = max(ext, static_cast<unsigned long long>(max(u, v)) + 1ull);
+  }
+  ext = wave_max(ext);
+  bad = wave_sum(bad);
+  if (lane_id() == 0) {
+    if (ext) atomicMax(&res[0], ext);
+    if (bad) atomicAdd(&res[1], bad);
+  }
+}
+
+// Arc keys of a checked pair list (as k_expand_keys, for a bulk load: no
+// batch order to keep).
+template <class T>
+__global__ void k_edges_keys(const T* __restrict__ src, const T* __restrict__ dst, uint64_t m, uint64_t stride,
+                             int undirected, int swap, unsigned long long* __restrict__ keys) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < m;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t u, v;
+    edge_id(src, i * stride, u);
+    edge_id(dst, i * stride, v);
+    if (swap) {
+      const uint32_t t = u;
+      u = v;
+      v = t;
+    }
+    const bool loop = u == v;
+    const uint64_t o = undirected ? 2 * i : i;
+    keys[o] = loop ? kInvalidKey : ((static_cast<unsigned long long>(u) << 32) | v);
+    if (undirected) keys[o + 1] = loop ? kInvalidKey : ((static_cast<unsigned long long>(v) << 32) | u);
+  }
+}
+
+// CSR of a square matrix of n rows, offsets and column ids read as signed
+// integers (int32 or int64): res[1] counts rows whose offsets start below 0,
+// fall, or end past nnz; res[0] counts column ids of the rows' entries
+// outside [0, n); res[2] counts the entries a build keeps (all of a row's
+// entries when directed, those with v >= u when not, loops included: the
+// ops of the batch the host path applies).
+template <class T>
+__global__ void k_csr_check(const T* __restrict__ indptr, const T* __restrict__ indices, uint64_t n, uint64_t nnz,
+                            int undirected, unsigned long long* __restrict__ res) {
+  unsigned long long bad_off = 0, bad_id = 0, kept = 0;
+  const long long first = static_cast<long long>(indptr[0]), last = static_cast<long long>(indptr[n]);
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < nnz + n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (i < n) {
+      const long long b = static_cast<long long>(indptr[i]), e = static_cast<long long>(indptr[i + 1]);
+      if (b < 0 || e < b || static_cast<uint64_t>(e) > nnz) ++bad_off;
+    } else {
+      const uint64_t p = i - n;
+      if (static_cast<long long>(p) < first || static_cast<long long>(p) >= last) continue;  // in no row
+      const long long v = static_cast<long long>(indices[p]);
+      if (v < 0 || static_cast<uint64_t>(v) >= n) {
+        ++bad_id;
+      } else if (undirected) {
+        // the row of entry p: the last u with indptr[u] <= p
+        uint64_t lo = 0, hi = n;
+        while (hi - lo > 1) {
+          const uint64_t mid = (lo + hi) / 2;
+          if (static_cast<long long>(indptr[mid]) <= static_cast<long long>(p))
+            lo = mid;
+          else
+            hi = mid;
+        }
+        if (static_cast<uint64_t>(v) >= lo) ++kept;
+      } else {
+        ++kept;
+      }
+    }
+  }
+  bad_off = wave_sum(bad_off);
+  bad_id = wave_sum(bad_id);
+  kept = wave_sum(kept);
+  if (lane_id() == 0) {
+    if (bad_id) atomicAdd(&res[0], bad_id);
+    if (bad_off) atomicAdd(&res[1], bad_off);
+    if (kept) atomicAdd(&res[2], kept);
+  }
+}
+
+// Arc keys of a checked CSR, one wave per row: entry (u, v) of row u gives
+// arc u->v; an undirected graph keeps only v >= u (each pair once, as the
+// host path does) and mirrors it; loops and skipped entries are invalid
+// keys, which sort last and are cut off.
+template <class T>
+__global__ void k_csr_keys(const T* __restrict__ indptr, const T* __restrict__ indices, uint64_t n, int undirected,
+                           int swap, unsigned long long* __restrict__ keys) {
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave; u < n;
+       u += waves) {
+    const uint64_t b = static_cast<uint64_t>(indptr[u]), e = static_cast<uint64_t>(indptr[u + 1]);
+    for (uint64_t p = b + lane_id(); p < e; p += kWave) {
+      const uint32_t v = static_cast<uint32_t>(indices[p]);
+      const uint32_t uu = static_cast<uint32_t>(u);
+      const bool skip = uu == v || (undirected && v < uu);
+      uint32_t a = uu, c = v;
+      if (swap) {
+        a = v;
+        c = uu;
+      }
+      const uint64_t o = undirected ? 2 * p : p;
+      keys[o] = skip ? kInvalidKey : ((static_cast<unsigned long long>(a) << 32) | c);
+      if (undirected) keys[o + 1] = skip ? kInvalidKey : ((static_cast<unsigned long long>(c) << 32) | a);
+    }
+  }
+}
+
+// Builds every direction from `arcs` keys written by fill(keys, d), replacing
+// the adjacency (bulk_load_edges semantics: vertex count grows, version + 1).
+template <class Fill>
+static void bulk_load_keys(vgx_graph* g, uint64_t arcs, Fill&& fill) {
+  for (int d = 0; d < g->ndir(); ++d) {
+    DevBuf<unsigned long long> keys, skeys;
+    keys.ensure(arcs + 1);
+    skeys.ensure(arcs + 1);
+    if (arcs) fill(keys.p, d);
+    build_direction(g, d, keys, skeys, arcs);
+    rebase_rows(g, d);
+  }
+  g->version += 1;
+  g->rows_epoch += 1;
+}
+
+// Waits for the producer stream, runs `check` into res[0..2] on the graph's
+// stream and returns them.
+template <class Check>
+static std::array<uint64_t, 3> device_check(vgx_graph* g, hipStream_t producer, Check&& check) {
+  hipStream_t s = g->stream;
+  if (producer != s) VGX_HIP(hipStreamSynchronize(producer));  // null: the default stream
+  BatchScratch& b0 = g->bs[0];
+  b0.extent.ensure(3);
+  VGX_HIP(hipMemsetAsync(b0.extent.p, 0, 3 * sizeof(unsigned long long), s));
+  check(b0.extent.p);
+  VGX_LAUNCH_CHECK();
+  unsigned long long res[3] = {0, 0, 0};
+  VGX_HIP(hipMemcpyAsync(res, b0.extent.p, sizeof(res), hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipStreamSynchronize(s));
+  return {res[0], res[1], res[2]};
+}
+
+template <class T>
+static void bulk_load_device_t(vgx_graph* g, const T* src, const T* dst, uint64_t m, uint64_t stride,
+                               hipStream_t producer) {
+  hipStream_t s = g->stream;
+  const auto [ext, bad, unused] = device_check(g, producer, [&](unsigned long long* res) {
+    if (m) k_edges_check<T><<<grid_for(m, kBlock, 4096), kBlock, 0, s>>>(src, dst, m, stride, res);
+  });
+  (void)unused;
+  if (bad) throw Error(VGX_INVALID_ARGUMENT, "vertex ids must lie in [0, 2^31 - 1)");
+  if (ext) graph_grow_vertices(g, ext);
+  const int undirected = g->directed ? 0 : 1;
+  bulk_load_keys(g, undirected ? 2 * m : m, [&](unsigned long long* keys, int d) {
+    k_edges_keys<T><<<grid_for(m), kBlock, 0, s>>>(src, dst, m, stride, undirected, d == 1 ? 1 : 0, keys);
+    VGX_LAUNCH_CHECK();
+  });
+}
+
+template <class T>
+static void bulk_load_csr_device_t(vgx_graph* g, const T* indptr, const T* indices, uint64_t n, uint64_t nnz,
+                                   hipStream_t producer) {
+  hipStream_t s = g->stream;
+  const auto [bad_id, bad, kept] = device_check(g, producer, [&](unsigned long long* res) {
+    if (n + nnz) k_csr_check<T><<<grid_for(n + nnz, kBlock, 4096), kBlock, 0, s>>>(indptr, indices, n, nnz,
+                                                                            g->directed ? 0 : 1, res);
+  });
+  if (bad) throw Error(VGX_INVALID_ARGUMENT, "invalid CSR offsets");
+  if (bad_id) throw Error(VGX_INVALID_ARGUMENT, "CSR contains out-of-range vertex id");
+  graph_grow_vertices(g, n);
+  if (kept == 0) return;  // an empty batch: adjacency and version unchanged
+  const int undirected = g->directed ? 0 : 1;
+  bulk_load_keys(g, undirected ? 2 * nnz : nnz, [&](unsigned long long* keys, int d) {
+    // entries past indptr[n] (a CSR whose nnz exceeds its last offset) stay invalid
+    VGX_HIP(hipMemsetAsync(keys, 0xFF, (undirected ? 2 * nnz : nnz) * sizeof(unsigned long long), s));
+    if (n) k_csr_keys<T><<<grid_for(n * kWave, kBlock, 8192), kBlock, 0, s>>>(indptr, indices, n, undirected,
+                                                                            d == 1 ? 1 : 0, keys);
+    VGX_LAUNCH_CHECK();
+  });
+}
+
 // ------------------------------------------------------------ batch apply
 
 static void stage_sorted(vgx_graph* g, uint64_t k);
@@ -1467,6 +1681,40 @@ vgx_status vgx_graph_bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t
   if (!g || (m && (!src || !dst))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
   DeviceGuard dg(g->device);
   bulk_load(g, src, dst, m);
+  VGX_API_END
+}
+
+vgx_status vgx_graph_bulk_load_device(vgx_graph* g, const void* src_dev, const void* dst_dev, uint64_t m,
+                                      uint64_t stride, uint32_t id_bytes, void* stream) {
+  VGX_API_BEGIN
+  if (!g || (m && (!src_dev || !dst_dev))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  if (id_bytes != 4 && id_bytes != 8) throw Error(VGX_INVALID_ARGUMENT, "ids must be 4 or 8 bytes wide");
+  if (stride == 0) throw Error(VGX_INVALID_ARGUMENT, "stride must be positive");
+  if ((g && !g->directed ? 2 * m : m) >= (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "edge list too large");
+  DeviceGuard dg(g->device);
+  hipStream_t ps = static_cast<hipStream_t>(stream);
+  if (id_bytes == 4)
+    bulk_load_device_t(g, static_cast<const uint32_t*>(src_dev), static_cast<const uint32_t*>(dst_dev), m, stride, ps);
+  else
+    bulk_load_device_t(g, static_cast<const int64_t*>(src_dev), static_cast<const int64_t*>(dst_dev), m, stride, ps);
+  VGX_API_END
+}
+
+vgx_status vgx_graph_bulk_load_csr_device(vgx_graph* g, const void* indptr_dev, const void* indices_dev,
+                                          uint64_t vertices, uint64_t nnz, uint32_t index_bytes, void* stream) {
+  VGX_API_BEGIN
+  if (!g || !indptr_dev || (nnz && !indices_dev)) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  if (index_bytes != 4 && index_bytes != 8) throw Error(VGX_INVALID_ARGUMENT, "indices must be 4 or 8 bytes wide");
+  if (vertices > (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "vertex ids must be below 2^31");
+  if ((!g->directed ? 2 * nnz : nnz) >= (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "CSR too large");
+  DeviceGuard dg(g->device);
+  hipStream_t ps = static_cast<hipStream_t>(stream);
+  if (index_bytes == 4)
+    bulk_load_csr_device_t(g, static_cast<const int32_t*>(indptr_dev), static_cast<const int32_t*>(indices_dev),
+                           vertices, nnz, ps);
+  else
+    bulk_load_csr_device_t(g, static_cast<const int64_t*>(indptr_dev), static_cast<const int64_t*>(indices_dev),
+                           vertices, nnz, ps);
   VGX_API_END
 }
 

@@ -363,6 +363,28 @@ PYBIND11_MODULE(_velographx, m) {
                                                          reinterpret_cast<void*>(stream)));
           },
           py::arg("src_ptr"), py::arg("dst_ptr"), py::arg("add_ptr"), py::arg("k"), py::arg("stream") = 0)
+      .def(
+          "_bulk_load_device",
+          [](DynamicGraph& g, std::uintptr_t src, std::uintptr_t dst, std::uint64_t m, std::uint64_t stride,
+             std::uint32_t id_bytes, std::uintptr_t stream) {
+            py::gil_scoped_release nogil;
+            velographx::vgx_check(vgx_graph_bulk_load_device(g.handle(), reinterpret_cast<const void*>(src),
+                                                             reinterpret_cast<const void*>(dst), m, stride, id_bytes,
+                                                             reinterpret_cast<void*>(stream)));
+          },
+          py::arg("src_ptr"), py::arg("dst_ptr"), py::arg("m"), py::arg("stride"), py::arg("id_bytes"),
+          py::arg("stream") = 0)
+      .def(
+          "_bulk_load_csr_device",
+          [](DynamicGraph& g, std::uintptr_t indptr, std::uintptr_t indices, std::uint64_t vertices,
+             std::uint64_t nnz, std::uint32_t index_bytes, std::uintptr_t stream) {
+            py::gil_scoped_release nogil;
+            velographx::vgx_check(vgx_graph_bulk_load_csr_device(
+                g.handle(), reinterpret_cast<const void*>(indptr), reinterpret_cast<const void*>(indices), vertices,
+                nnz, index_bytes, reinterpret_cast<void*>(stream)));
+          },
+          py::arg("indptr_ptr"), py::arg("indices_ptr"), py::arg("vertices"), py::arg("nnz"),
+          py::arg("index_bytes"), py::arg("stream") = 0)
       .def_property_readonly("device", [](const DynamicGraph& g) { return device_of(g.handle()); })
       .def("is_compact", &DynamicGraph::is_compact)
       .def("ensure_vertex", &DynamicGraph::ensure_vertex, py::arg("v"))
