@@ -89,7 +89,8 @@ using IncrementalBFS = BasicIncrementalBFS<DynamicGraph>;
 // benchmarks/thread_scaling_bfs.cpp:78-108).
 class MultiSourceBFS {
  public:
-  MultiSourceBFS(DynamicGraph& g, const std::vector<VertexId>& roots) : n_(g.vertex_count()), k_(roots.size()) {
+  MultiSourceBFS(DynamicGraph& g, const std::vector<VertexId>& roots)
+      : g_(&g), n_(g.vertex_count()), k_(roots.size()) {
     vgx_check(vgx_msbfs_create(g.handle(), roots.data(), static_cast<std::uint32_t>(roots.size()), &m_));
   }
   ~MultiSourceBFS() {
@@ -101,6 +102,17 @@ class MultiSourceBFS {
   void run(std::size_t vertex_count) {
     n_ = vertex_count;
     vgx_check(vgx_msbfs_run(m_));
+  }
+  // IncrementalBFS::apply (bfs.hpp:29-98) for every root: the batch goes
+  // into the graph and every root's distances are repaired on the device
+  // (or recomputed by a full pass, see vgx_msbfs_apply).
+  void apply(const UpdateBatch& batch) {
+    const auto c = batch.columns();
+    apply_arrays(c.src.data(), c.dst.data(), c.add.data(), c.src.size());
+  }
+  void apply_arrays(const std::uint32_t* src, const std::uint32_t* dst, const std::uint8_t* add, std::size_t k) {
+    vgx_check(vgx_msbfs_apply(m_, src, dst, add, k));
+    n_ = g_->vertex_count();
   }
   [[nodiscard]] std::vector<std::uint32_t> distances(std::uint32_t r) const {
     std::vector<std::uint32_t> d(n_);
@@ -126,6 +138,7 @@ class MultiSourceBFS {
   [[nodiscard]] vgx_msbfs* handle() const noexcept { return m_; }
 
  private:
+  DynamicGraph* g_ = nullptr;
   vgx_msbfs* m_ = nullptr;
   std::size_t n_ = 0, k_ = 0;
 };

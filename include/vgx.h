@@ -430,6 +430,37 @@ vgx_status vgx_msbfs_stats(vgx_msbfs* m, uint32_t* push_levels, uint64_t* graph_
 vgx_status vgx_msbfs_autotune(vgx_msbfs* m, uint32_t reps, uint32_t* best_blocks, double* best_alpha,
                               double* best_ms);
 
+/* IncrementalBFS::apply (incremental/bfs.hpp:29-98) for every root at once:
+ * applies the batch to the graph (DynamicGraph::apply semantics) and makes
+ * every root's distances those of a BFS of the new graph.  The repair keeps
+ * the reference's steps per root -- pairs whose shortest parents were all
+ * deleted or affected are affected, reset, seeded from unaffected
+ * in-neighbours, the added arcs relaxed and decreases propagated -- run for
+ * all roots together (no reference counterpart for several roots); a full
+ * pass runs instead when the repair's affected pairs pass the fallback
+ * fraction of vertices x roots, when the cost model prefers it (mode 0),
+ * when the graph changed since the distances were computed, or when the
+ * batch brings a root's id into the graph.  Before the first run() the
+ * batch is applied and a full pass runs. */
+vgx_status vgx_msbfs_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k);
+/* vgx_msbfs_apply for a batch in device memory (see vgx_bfs_apply_device). */
+vgx_status vgx_msbfs_apply_device(vgx_msbfs* m, const uint32_t* src_dev, const uint32_t* dst_dev,
+                                  const uint8_t* add_dev, uint64_t k, void* stream);
+/* Repair policy (no reference counterpart): mode 0 chooses per batch from
+ * running means of the repair time per op and the full-pass time, 1 always
+ * repairs, 2 always runs a full pass; a repair whose affected (vertex, root)
+ * pairs exceed fallback_fraction * vertices * roots stops and runs a full
+ * pass (default 0.1, as the reference's deletion_fallback_fraction,
+ * bfs.hpp:18, bounds one root's affected vertices). */
+vgx_status vgx_msbfs_set_repair(vgx_msbfs* m, int mode, double fallback_fraction);
+typedef struct {
+  uint64_t batches, repaired, full_passes, fallbacks;
+  int last_repaired, last_fallback;
+  uint64_t last_candidates, last_affected_pairs, last_lowered;
+  double pass_ms, repair_us_per_op;
+} vgx_msbfs_repair_info;
+vgx_status vgx_msbfs_repair_stats(vgx_msbfs* m, vgx_msbfs_repair_info* out);
+
 /* -------------------------------------- repair-vs-recompute selection */
 
 /* Adaptive selector of the reference's publication policy

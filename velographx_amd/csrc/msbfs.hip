@@ -100,6 +100,79 @@ struct MsArgs {
   MsCtl* ctl;
 };
 
+// Incremental repair of all roots after an update batch.  Distances are kept
+// vertex-major, d[v * 64 + r] (kInf = unreachable), so a wave handles one
+// vertex with one lane per root and every neighbour costs one coalesced
+// 256-byte load.  Per-vertex 64-bit root masks carry the state of the
+// invalidation (cand, aff, tm, sup, nm) and of the propagation (pend, pm).
+// Lists come in two parities: round i fills parity i & 1 and its first
+// kernel clears the counters of parity (i + 1) & 1, which the previous round
+// has finished with.  Plain lists hold vertices; packed lists (vertex + edge
+// prefix, see wave_push) drive the edge-balanced loops.
+struct MsrState {
+  unsigned long long tq[2];  // plain: vertices given candidate roots (invalidation input)
+  unsigned long long vq[2];  // plain: of those, the ones with roots left to evaluate
+  unsigned long long sq[2];  // packed: of those, the ones with in-arcs (support scan)
+  unsigned long long eq[2];  // packed: newly affected vertices with out-arcs (emit)
+  unsigned long long aall;   // plain: affected vertices (any root)
+  unsigned long long apairs; // affected (vertex, root) pairs
+  unsigned long long seedq;  // packed: affected vertices with in-arcs
+  unsigned long long fq[2];  // plain: vertices with lowered roots (propagation input)
+  unsigned long long pq[2];  // packed: of those, the ones with out-arcs
+  unsigned long long candidates;  // (vertex, root) candidate pairs
+  unsigned long long lowered;     // pairs lowered by seeds, additions and propagation
+  uint32_t fallback;
+  uint32_t pad;
+};
+
+struct MsrArgs {
+  const uint2* out_row;
+  const uint32_t* out_arena;
+  const uint2* in_row;
+  const uint32_t* in_arena;
+  uint32_t* d;  // [n][64]
+  unsigned long long* cand;
+  unsigned long long* aff;
+  unsigned long long* tm;
+  unsigned long long* sup;
+  unsigned long long* nm;
+  unsigned long long* pend;
+  unsigned long long* pm;
+  uint32_t* tl[2];
+  uint32_t* vl[2];
+  uint32_t* sl[2];
+  unsigned long long* spre[2];
+  uint32_t* el[2];
+  unsigned long long* epre[2];
+  uint32_t* al;
+  uint32_t* seed;
+  unsigned long long* seedpre;
+  uint32_t* fl[2];
+  uint32_t* pl[2];
+  unsigned long long* ppre[2];
+  // the staged batch (out-direction arcs, mirrored when undirected)
+  const uint32_t* fin;
+  const unsigned long long* skeys;
+  const uint8_t* act;
+  unsigned long long nfin;
+  const uint32_t* roots;
+  uint32_t k;
+  uint32_t pad;
+  unsigned long long limit;  // affected pairs past which the repair gives up
+  MsrState* st;
+};
+
+struct MsRepair {
+  DevBuf<uint32_t> d;
+  DevBuf<unsigned long long> cand, aff, tm, sup, nm, pend, pm;
+  DevBuf<uint32_t> tl[2], vl[2], sl[2], el[2], al, seed, fl[2], pl[2];
+  DevBuf<unsigned long long> spre[2], epre[2], seedpre, ppre[2];
+  uint64_t n = 0;          // vertices the arrays cover
+  MsrArgs* d_args = nullptr;
+  MsrState* d_st = nullptr;
+  MsrState* h_st = nullptr;  // pinned
+};
+
 // Chunks of the rows longer than kMsShortRow in one direction's row table:
 // row v owns chunks [row_chunk[v], row_chunk[v+1]).
 struct MsPlan {
@@ -139,6 +212,17 @@ struct vgx_msbfs {
   hipGraph_t graph = nullptr;
   uint64_t graph_key[5] = {0, 0, 0, 0, 0};
   uint64_t replays = 0;
+  // incremental repair
+  vgx::MsRepair rep;
+  bool in_d32 = false;       // the current distances are rep.d (not newbits)
+  uint64_t dist_version = ~0ull;  // graph version the distances belong to
+  int repair_mode = 0;       // 0 adaptive, 1 always repair, 2 always a full pass
+  double fallback = 0.1;     // affected pairs past fallback * n * k give up
+  double pass_ms = 0.0;      // running means of the two arms (answer-ready)
+  double repair_us_per_op = 0.0;
+  uint64_t batches = 0, repaired = 0, full_passes = 0, fallbacks = 0;
+  bool last_repaired = false, last_fallback = false;
+  uint64_t last_affected_pairs = 0, last_lowered = 0, last_candidates = 0;
 };
 
 namespace vgx {
@@ -531,8 +615,10 @@ static void ms_run(vgx_msbfs* m) {
       m->levels ? std::min<uint32_t>(kMsProgram, std::max<uint32_t>(4, m->levels + 2)) : kMsProgram;
   m->levels = 0;
   m->push_levels = 0;
+  m->in_d32 = false;
   if (n == 0) {
     m->ran = true;
+    m->dist_version = g->version;
     return;
   }
   MsGrids gr;
@@ -599,7 +685,754 @@ static void ms_run(vgx_msbfs* m) {
     base += prog;
   }
   m->ran = true;
+  m->dist_version = g->version;
   m->reached.assign(m->k, 0);
+}
+
+// ------------------------------------------------------------------ repair
+//
+// Exact repair of every root after a batch, in the reference's order for one
+// root (incremental/bfs.hpp:29-98, 206-354): on the graph without the
+// deleted arcs, a (vertex, root) pair is affected once no live in-neighbour
+// that is itself unaffected sits one level above it; affected pairs are
+// reset, seeded from their unaffected in-neighbours in the post-batch graph,
+// the added arcs are relaxed, and Bellman-Ford rounds propagate every
+// lowered pair.  Lanes are roots: the masks say which lanes do work, and a
+// 64-bit ballot turns a lane test back into a mask.
+
+constexpr uint32_t kMsrWindow = 16;  // items a wave takes at a time from a plain list
+constexpr unsigned kMsrBlocks = 1024;
+
+__device__ __forceinline__ bool lane_bit(unsigned long long m) { return (m >> lane_id()) & 1ull; }
+
+// A wave-uniform copy of x (taken from lane 0), so loads it indexes go
+// through the scalar unit.
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Wave-aggregated append of v to a plain list (the lanes that want it).
+__device__ __forceinline__ void push_plain(unsigned long long* ctr, uint32_t* list, bool want, uint32_t v) {
+  const unsigned long long b = __ballot(want);
+  if (!b) return;
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(ctr, static_cast<unsigned long long>(__popcll(b)));
+  base = __shfl(base, 0, kWave);
+  if (want) list[base + lanes_below(b)] = v;
+}
+
+// Grid-stride over windows of kMsrWindow items of a plain list: fn(first)
+// with lane j < kMsrWindow holding item first + j (fn reads it itself).
+template <class Fn>
+__device__ __forceinline__ void msr_windows(uint64_t count, Fn&& fn) {
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+  for (uint64_t w = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave;
+       w * kMsrWindow < count; w += waves)
+    fn(w * kMsrWindow);
+}
+
+// Heads of deleted arcs u->v that were shortest-parent arcs for some root:
+// d_r(u) + 1 == d_r(v) (bfs.hpp:52-63).  They become candidates of round 0.
+__global__ __launch_bounds__(kMsBlock) void k_msr_candidates(const MsrArgs* __restrict__ pa) {
+  const MsrArgs& a = *pa;
+  const unsigned lane = lane_id();
+  const uint32_t myroot = lane < a.k ? a.roots[lane] : kInf;
+  unsigned long long ncand = 0;
+  msr_windows(a.nfin, [&](uint64_t first) {
+    const uint64_t j = first + lane;
+    uint32_t u = 0, v = 0;
+    bool del = false;
+    if (lane < kMsrWindow && j < a.nfin) {
+      const uint32_t p = a.fin[j];
+      if (a.act[p] == kActDel) {
+        del = true;
+        u = static_cast<uint32_t>(a.skeys[p] >> 32);
+        v = static_cast<uint32_t>(a.skeys[p]);
+      }
+    }
+    bool want = false;
+    uint32_t pv = 0;
+    for (unsigned long long todo = __ballot(del); todo; todo &= todo - 1) {
+      const int t = __ffsll(static_cast<long long>(todo)) - 1;
+      const uint32_t ut = uni(__shfl(u, t, kWave)), vt = uni(__shfl(v, t, kWave));
+      const uint32_t du = a.d[ut * 64ull + lane], dv = a.d[vt * 64ull + lane];
+      const unsigned long long bits = __ballot(du != kInf && du + 1 == dv && myroot != vt);
+      if (bits) {
+        ncand += __popcll(bits);
+        unsigned long long old = 0;
+        if (lane == 0) old = atomicOr(&a.cand[vt], bits);
+        old = __shfl(old, 0, kWave);
+        if (!old && lane == static_cast<unsigned>(t)) {
+          want = true;
+          pv = vt;
+        }
+      }
+    }
+    push_plain(&a.st->tq[0], a.tl[0], want, pv);
+  });
+  if (lane == 0 && ncand) atomicAdd(&a.st->candidates, ncand);
+}
+
+// Round `par`, step 1 (lane per vertex): take each touched vertex's
+// candidate roots, minus those already affected and the roots it is; list
+// the vertices with roots left (and, for the support scan, those with
+// in-arcs).  Clears the other parity's counters.
+__global__ __launch_bounds__(kMsBlock) void k_msr_take(const MsrArgs* __restrict__ pa, int par) {
+  const MsrArgs& a = *pa;
+  if (a.st->fallback) return;
+  __shared__ uint32_t s_roots[64];
+  if (threadIdx.x < 64) s_roots[threadIdx.x] = threadIdx.x < a.k ? a.roots[threadIdx.x] : kInf;
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.st->tq[par ^ 1] = 0;
+    a.st->vq[par ^ 1] = 0;
+    a.st->sq[par ^ 1] = 0;
+    a.st->eq[par ^ 1] = 0;
+  }
+  const uint64_t nt = a.st->tq[par];
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t j0 = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x - lane_id(); j0 < nt;
+       j0 += stride) {
+    const uint64_t j = j0 + lane_id();
+    uint32_t v = 0, len = 0;
+    bool keep = false;
+    if (j < nt) {
+      v = a.tl[par][j];
+      unsigned long long m = atomicExch(&a.cand[v], 0ull) & ~a.aff[v];
+      for (uint32_t r = 0; r < 64; ++r)
+        if (s_roots[r] == v) m &= ~(1ull << r);
+      if (m) {
+        keep = true;
+        a.tm[v] = m;
+        a.sup[v] = 0;
+        len = a.in_row[v].y;
+      }
+    }
+    push_plain(&a.st->vq[par], a.vl[par], keep, v);
+    wave_push(&a.st->sq[par], a.sl[par], a.spre[par], keep && len > 0, v, len);
+  }
+}
+
+// Step 2 (edge-balanced over the in-arcs w->v of the listed vertices): root
+// r of v is supported when some live w has d_r(w) + 1 == d_r(v) and is not
+// affected for r.
+__global__ __launch_bounds__(kMsBlock) void k_msr_support(const MsrArgs* __restrict__ pa, int par) {
+  const MsrArgs& a = *pa;
+  if (a.st->fallback) return;
+  const unsigned long long qc = a.st->sq[par];
+  const uint32_t W = gridDim.x * kMsWaves;
+  const uint32_t gw = blockIdx.x * kMsWaves + threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  lbs_edges(a.sl[par], a.spre[par], static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.in_row, gw, W,
+            [&](uint32_t, uint32_t v, uint2, uint32_t slot, bool valid) {
+              const uint32_t w = valid ? a.in_arena[slot] : kTomb;
+              for (unsigned long long todo = __ballot(!(w & kTomb)); todo; todo &= todo - 1) {
+                const int t = __ffsll(static_cast<long long>(todo)) - 1;
+                const uint32_t vt = uni(__shfl(v, t, kWave)), wt = uni(__shfl(w, t, kWave));
+                const unsigned long long need = a.tm[vt] & ~a.sup[vt];
+                if (!need) continue;
+                const uint32_t dw = a.d[wt * 64ull + lane], dv = a.d[vt * 64ull + lane];
+                const unsigned long long bits =
+                    __ballot(lane_bit(need) && dw != kInf && dw + 1 == dv && !lane_bit(a.aff[wt]));
+                if (bits && lane == 0) atomicOr(&a.sup[vt], bits);
+              }
+            });
+}
+
+// Step 3 (lane per vertex): roots left without support become affected;
+// the vertices with newly affected roots and out-arcs are listed for the
+// emit step.
+__global__ __launch_bounds__(kMsBlock) void k_msr_evaluate(const MsrArgs* __restrict__ pa, int par) {
+  const MsrArgs& a = *pa;
+  if (a.st->fallback) return;
+  const uint64_t nv = a.st->vq[par];
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  unsigned long long pairs = 0;
+  for (uint64_t j0 = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x - lane_id(); j0 < nv;
+       j0 += stride) {
+    const uint64_t j = j0 + lane_id();
+    uint32_t v = 0, len = 0;
+    bool newly = false, first = false;
+    if (j < nv) {
+      v = a.vl[par][j];
+      const unsigned long long nm = a.tm[v] & ~a.sup[v];
+      if (nm) {
+        const unsigned long long old = a.aff[v];
+        a.aff[v] = old | nm;  // this round's only writer of aff[v]
+        a.nm[v] = nm;
+        first = old == 0;
+        newly = true;
+        pairs += __popcll(nm);
+        len = a.out_row[v].y;
+      }
+    }
+    push_plain(&a.st->aall, a.al, first, v);
+    wave_push(&a.st->eq[par], a.el[par], a.epre[par], newly && len > 0, v, len);
+  }
+  pairs = wave_sum(pairs);
+  if (lane_id() == 0 && pairs) atomicAdd(&a.st->apairs, pairs);
+}
+
+// Step 4 (edge-balanced over the out-arcs u->x of the newly affected): x
+// loses a parent for root r when d_r(x) == d_r(u) + 1; it becomes a
+// candidate of the next round.  Past the pair limit the repair stops.
+__global__ __launch_bounds__(kMsBlock) void k_msr_emit(const MsrArgs* __restrict__ pa, int par) {
+  const MsrArgs& a = *pa;
+  if (a.st->fallback) return;
+  if (a.st->apairs > a.limit) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.st->fallback = 1;
+    return;
+  }
+  const unsigned long long qc = a.st->eq[par];
+  const uint32_t W = gridDim.x * kMsWaves;
+  const uint32_t gw = blockIdx.x * kMsWaves + threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  lbs_edges(a.el[par], a.epre[par], static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.out_row, gw, W,
+            [&](uint32_t, uint32_t u, uint2, uint32_t slot, bool valid) {
+              const uint32_t x = valid ? a.out_arena[slot] : kTomb;
+              bool want = false;
+              uint32_t px = 0;
+              for (unsigned long long todo = __ballot(!(x & kTomb)); todo; todo &= todo - 1) {
+                const int t = __ffsll(static_cast<long long>(todo)) - 1;
+                const uint32_t ut = uni(__shfl(u, t, kWave)), xt = uni(__shfl(x, t, kWave));
+                const unsigned long long nm = a.nm[ut];
+                const uint32_t du = a.d[ut * 64ull + lane], dx = a.d[xt * 64ull + lane];
+                const unsigned long long bits = __ballot(lane_bit(nm) && du != kInf && dx == du + 1);
+                if (!bits) continue;
+                unsigned long long old = 0;
+                if (lane == 0) old = atomicOr(&a.cand[xt], bits);
+                old = __shfl(old, 0, kWave);
+                if (!old && lane == static_cast<unsigned>(t)) {
+                  want = true;
+                  px = xt;
+                }
+              }
+              push_plain(&a.st->tq[par ^ 1], a.tl[par ^ 1], want, px);
+            });
+}
+
+// Affected pairs become unreachable; affected vertices with in-arcs are
+// listed for seeding.
+__global__ __launch_bounds__(kMsBlock) void k_msr_reset(const MsrArgs* __restrict__ pa) {
+  const MsrArgs& a = *pa;
+  const unsigned lane = lane_id();
+  msr_windows(a.st->aall, [&](uint64_t first) {
+    const uint64_t na = a.st->aall;
+    const uint64_t j = first + lane;
+    uint32_t v = 0, len = 0;
+    const bool in = lane < kMsrWindow && j < na;
+    if (in) {
+      v = a.al[j];
+      len = a.in_row[v].y;
+    }
+    for (unsigned long long todo = __ballot(in); todo; todo &= todo - 1) {
+      const int t = __ffsll(static_cast<long long>(todo)) - 1;
+      const uint32_t vt = uni(__shfl(v, t, kWave));
+      if (lane_bit(a.aff[vt])) a.d[vt * 64ull + lane] = kInf;
+    }
+    wave_push(&a.st->seedq, a.seed, a.seedpre, in && len > 0, v, len);
+  });
+}
+
+// best_boundary_distance (bfs.hpp:261-269) for every affected pair, over
+// the in-arcs w->v of the post-batch graph from pairs that are not affected.
+__global__ __launch_bounds__(kMsBlock) void k_msr_seed(const MsrArgs* __restrict__ pa) {
+  const MsrArgs& a = *pa;
+  const unsigned long long qc = a.st->seedq;
+  const uint32_t W = gridDim.x * kMsWaves;
+  const uint32_t gw = blockIdx.x * kMsWaves + threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  lbs_edges(a.seed, a.seedpre, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.in_row, gw, W,
+            [&](uint32_t, uint32_t v, uint2, uint32_t slot, bool valid) {
+              const uint32_t w = valid ? a.in_arena[slot] : kTomb;
+              for (unsigned long long todo = __ballot(!(w & kTomb)); todo; todo &= todo - 1) {
+                const int t = __ffsll(static_cast<long long>(todo)) - 1;
+                const uint32_t vt = uni(__shfl(v, t, kWave)), wt = uni(__shfl(w, t, kWave));
+                const unsigned long long m = a.aff[vt] & ~a.aff[wt];
+                if (!m) continue;
+                const uint32_t dw = a.d[wt * 64ull + lane];
+                if (lane_bit(m) && dw != kInf && dw + 1 < a.d[vt * 64ull + lane])
+                  atomicMin(&a.d[vt * 64ull + lane], dw + 1);
+              }
+            });
+}
+
+// Affected pairs that found a distance start the propagation.
+__global__ __launch_bounds__(kMsBlock) void k_msr_collect(const MsrArgs* __restrict__ pa) {
+  const MsrArgs& a = *pa;
+  const unsigned lane = lane_id();
+  unsigned long long low = 0;
+  msr_windows(a.st->aall, [&](uint64_t first) {
+    const uint64_t na = a.st->aall;
+    const uint64_t j = first + lane;
+    uint32_t v = 0;
+    const bool in = lane < kMsrWindow && j < na;
+    if (in) v = a.al[j];
+    bool want = false;
+    uint32_t pv = 0;
+    for (unsigned long long todo = __ballot(in); todo; todo &= todo - 1) {
+      const int t = __ffsll(static_cast<long long>(todo)) - 1;
+      const uint32_t vt = uni(__shfl(v, t, kWave));
+      const unsigned long long bits = __ballot(lane_bit(a.aff[vt]) && a.d[vt * 64ull + lane] != kInf);
+      if (!bits) continue;
+      low += __popcll(bits);
+      unsigned long long old = 0;
+      if (lane == 0) old = atomicOr(&a.pend[vt], bits);
+      old = __shfl(old, 0, kWave);
+      if (!old && lane == static_cast<unsigned>(t)) {
+        want = true;
+        pv = vt;
+      }
+    }
+    push_plain(&a.st->fq[0], a.fl[0], want, pv);
+  });
+  if (lane == 0 && low) atomicAdd(&a.st->lowered, low);
+}
+
+// relax_edge over the arcs the batch added (bfs.hpp:91-96, 332-339).
+__global__ __launch_bounds__(kMsBlock) void k_msr_relax_additions(const MsrArgs* __restrict__ pa) {
+  const MsrArgs& a = *pa;
+  const unsigned lane = lane_id();
+  unsigned long long low = 0;
+  msr_windows(a.nfin, [&](uint64_t first) {
+    const uint64_t j = first + lane;
+    uint32_t u = 0, v = 0;
+    bool ins = false;
+    if (lane < kMsrWindow && j < a.nfin) {
+      const uint32_t p = a.fin[j];
+      if (a.act[p] == kActIns || a.act[p] == kActUndel) {
+        ins = true;
+        u = static_cast<uint32_t>(a.skeys[p] >> 32);
+        v = static_cast<uint32_t>(a.skeys[p]);
+      }
+    }
+    bool want = false;
+    uint32_t pv = 0;
+    for (unsigned long long todo = __ballot(ins); todo; todo &= todo - 1) {
+      const int t = __ffsll(static_cast<long long>(todo)) - 1;
+      const uint32_t ut = uni(__shfl(u, t, kWave)), vt = uni(__shfl(v, t, kWave));
+      const uint32_t du = a.d[ut * 64ull + lane];
+      bool lowered = false;
+      if (du != kInf && du + 1 < a.d[vt * 64ull + lane]) lowered = atomicMin(&a.d[vt * 64ull + lane], du + 1) > du + 1;
+      const unsigned long long bits = __ballot(lowered);
+      if (!bits) continue;
+      low += __popcll(bits);
+      unsigned long long old = 0;
+      if (lane == 0) old = atomicOr(&a.pend[vt], bits);
+      old = __shfl(old, 0, kWave);
+      if (!old && lane == static_cast<unsigned>(t)) {
+        want = true;
+        pv = vt;
+      }
+    }
+    push_plain(&a.st->fq[0], a.fl[0], want, pv);
+  });
+  if (lane == 0 && low) atomicAdd(&a.st->lowered, low);
+}
+
+// Propagation round `par`, step 1 (lane per vertex): take each listed
+// vertex's lowered roots; list those with out-arcs.  Clears the other
+// parity's counters.
+__global__ __launch_bounds__(kMsBlock) void k_msr_ptake(const MsrArgs* __restrict__ pa, int par) {
+  const MsrArgs& a = *pa;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.st->fq[par ^ 1] = 0;
+    a.st->pq[par ^ 1] = 0;
+  }
+  const uint64_t nf = a.st->fq[par];
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t j0 = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x - lane_id(); j0 < nf;
+       j0 += stride) {
+    const uint64_t j = j0 + lane_id();
+    uint32_t v = 0, len = 0;
+    bool keep = false;
+    if (j < nf) {
+      v = a.fl[par][j];
+      const unsigned long long m = atomicExch(&a.pend[v], 0ull);
+      a.pm[v] = m;
+      len = a.out_row[v].y;
+      keep = m != 0;
+    }
+    wave_push(&a.st->pq[par], a.pl[par], a.ppre[par], keep && len > 0, v, len);
+  }
+}
+
+// Step 2 (edge-balanced over the out-arcs u->x): d_r(x) = min(d_r(x),
+// d_r(u) + 1) for the taken roots of u (propagate_decreases, bfs.hpp
+// :341-354); lowered pairs go to the next round.
+__global__ __launch_bounds__(kMsBlock) void k_msr_propagate(const MsrArgs* __restrict__ pa, int par) {
+  const MsrArgs& a = *pa;
+  const unsigned long long qc = a.st->pq[par];
+  const uint32_t W = gridDim.x * kMsWaves;
+  const uint32_t gw = blockIdx.x * kMsWaves + threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  unsigned long long low = 0;
+  lbs_edges(a.pl[par], a.ppre[par], static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.out_row, gw, W,
+            [&](uint32_t, uint32_t u, uint2, uint32_t slot, bool valid) {
+              const uint32_t x = valid ? a.out_arena[slot] : kTomb;
+              bool want = false;
+              uint32_t px = 0;
+              for (unsigned long long todo = __ballot(!(x & kTomb)); todo; todo &= todo - 1) {
+                const int t = __ffsll(static_cast<long long>(todo)) - 1;
+                const uint32_t ut = uni(__shfl(u, t, kWave)), xt = uni(__shfl(x, t, kWave));
+                const unsigned long long pm = a.pm[ut];
+                bool lowered = false;
+                if (lane_bit(pm)) {
+                  const uint32_t du = a.d[ut * 64ull + lane];
+                  if (du != kInf && du + 1 < a.d[xt * 64ull + lane])
+                    lowered = atomicMin(&a.d[xt * 64ull + lane], du + 1) > du + 1;
+                }
+                const unsigned long long bits = __ballot(lowered);
+                if (!bits) continue;
+                low += __popcll(bits);
+                unsigned long long old = 0;
+                if (lane == 0) old = atomicOr(&a.pend[xt], bits);
+                old = __shfl(old, 0, kWave);
+                if (!old && lane == static_cast<unsigned>(t)) {
+                  want = true;
+                  px = xt;
+                }
+              }
+              push_plain(&a.st->fq[par ^ 1], a.fl[par ^ 1], want, px);
+            });
+  if (lane == 0 && low) atomicAdd(&a.st->lowered, low);
+}
+
+// Affected masks back to zero for the next batch.
+__global__ void k_msr_clear(const MsrArgs* __restrict__ pa) {
+  const MsrArgs& a = *pa;
+  const uint64_t na = a.st->aall;
+  for (uint64_t j = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; j < na;
+       j += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    a.aff[a.al[j]] = 0;
+}
+
+__global__ void k_msr_zero_state(MsrState* st) {
+  if (threadIdx.x < sizeof(MsrState) / 8) reinterpret_cast<unsigned long long*>(st)[threadIdx.x] = 0;
+}
+
+// d[v][r] from the level masks of a full pass (first level whose mask has
+// bit r), one wave per vertex, lane per root.
+__global__ __launch_bounds__(kMsBlock) void k_ms_to_d32(const unsigned long long* __restrict__ newbits,
+                                                        uint32_t levels, uint64_t n, uint32_t* __restrict__ d) {
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * kMsWaves;
+  const unsigned lane = lane_id();
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kMsWaves) + threadIdx.x / kWave; v < n; v += waves) {
+    uint32_t dist = kInf;
+    for (uint32_t L = 0; L < levels; ++L) {
+      const unsigned long long m = newbits[static_cast<uint64_t>(L) * n + v];
+      if (dist == kInf && lane_bit(m)) dist = L;
+    }
+    d[v * 64 + lane] = dist;
+  }
+}
+
+// Columns [from, to) of d set to unreachable (vertices a batch added).
+__global__ void k_msr_fill_inf(uint32_t* d, uint64_t from, uint64_t to) {
+  for (uint64_t i = from * 64 + blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < to * 64;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    d[i] = kInf;
+}
+
+// Root r's distances (uint32, kInf unreachable) from d.
+__global__ void k_msr_root(const uint32_t* __restrict__ d, uint64_t n, uint32_t r, uint32_t* __restrict__ out) {
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; v < n;
+       v += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    out[v] = d[v * 64 + r];
+}
+
+// Per-root uint8 levels, root-major (255 unreachable), from d; also the
+// largest finite distance (for the level count) and reached counts.
+__global__ __launch_bounds__(kMsBlock) void k_msr_u8(const uint32_t* __restrict__ d, uint64_t n, uint32_t k,
+                                                     uint8_t* __restrict__ out) {
+  // a workgroup stages 64 vertices x 64 roots through LDS so that both the
+  // 256-byte row reads and the root-major byte writes are coalesced
+  __shared__ uint8_t tile[64][65];
+  for (uint64_t v0 = blockIdx.x * 64ull; v0 < n; v0 += gridDim.x * 64ull) {
+    for (uint32_t i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
+      const uint32_t vv = i / 64, r = i % 64;
+      const uint32_t x = v0 + vv < n ? d[(v0 + vv) * 64 + r] : kInf;
+      tile[r][vv] = x > 254u ? 255u : static_cast<uint8_t>(x);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
+      const uint32_t r = i / 64, vv = i % 64;
+      if (r < k && v0 + vv < n) out[static_cast<uint64_t>(r) * n + v0 + vv] = tile[r][vv];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_msr_stats(const uint32_t* __restrict__ d, uint64_t n, uint32_t k,
+                            unsigned long long* __restrict__ reached, unsigned int* __restrict__ maxd) {
+  const unsigned lane = lane_id();
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+  unsigned long long c = 0;
+  uint32_t mx = 0;
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave; v < n;
+       v += waves) {
+    const uint32_t x = d[v * 64 + lane];
+    if (lane < k && x != kInf) {
+      ++c;
+      mx = max(mx, x);
+    }
+  }
+  if (lane < k && c) atomicAdd(&reached[lane], c);
+  mx = wave_max(mx);
+  if (lane == 0) atomicMax(maxd, mx);
+}
+
+static unsigned msr_grid(uint64_t items, unsigned per = kMsBlock) { return ms_blocks(items, per, 4096); }
+
+// Repair arrays sized for the graph's vertex count (new vertices start
+// unreachable with empty masks; the list buffers hold at most one entry per
+// vertex and round).
+static void msr_ensure(vgx_msbfs* m, uint64_t n) {
+  MsRepair& R = m->rep;
+  hipStream_t s = m->g->stream;
+  if (n > R.n) {
+    const uint64_t keep = R.n;
+    const uint64_t cap = std::max<uint64_t>(n, R.n + R.n / 4);
+    R.d.grow_copy(cap * 64 + 64, keep * 64, s);
+    for (DevBuf<unsigned long long>* b : {&R.cand, &R.aff, &R.pend}) {
+      b->grow_copy(cap + 1, keep, s);
+      VGX_HIP(hipMemsetAsync(b->p + keep, 0, (cap + 1 - keep) * 8, s));
+    }
+    for (DevBuf<unsigned long long>* b : {&R.tm, &R.sup, &R.nm, &R.pm}) b->ensure(cap + 1);
+    for (int p = 0; p < 2; ++p) {
+      R.tl[p].ensure(cap + 1);
+      R.vl[p].ensure(cap + 1);
+      R.sl[p].ensure(cap + 1);
+      R.spre[p].ensure(cap + 1);
+      R.el[p].ensure(cap + 1);
+      R.epre[p].ensure(cap + 1);
+      R.fl[p].ensure(cap + 1);
+      R.pl[p].ensure(cap + 1);
+      R.ppre[p].ensure(cap + 1);
+    }
+    R.al.ensure(cap + 1);
+    R.seed.ensure(cap + 1);
+    R.seedpre.ensure(cap + 1);
+    R.n = cap;
+  }
+}
+
+static MsrArgs msr_args(vgx_msbfs* m) {
+  vgx_graph* g = m->g;
+  MsRepair& R = m->rep;
+  const BatchScratch& bs = g->bs[0];
+  MsrArgs a{};
+  a.out_row = g->adj[0].row.p;
+  a.out_arena = g->adj[0].arena.p;
+  a.in_row = g->in().row.p;
+  a.in_arena = g->in().arena.p;
+  a.d = R.d.p;
+  a.cand = R.cand.p;
+  a.aff = R.aff.p;
+  a.tm = R.tm.p;
+  a.sup = R.sup.p;
+  a.nm = R.nm.p;
+  a.pend = R.pend.p;
+  a.pm = R.pm.p;
+  for (int p = 0; p < 2; ++p) {
+    a.tl[p] = R.tl[p].p;
+    a.vl[p] = R.vl[p].p;
+    a.sl[p] = R.sl[p].p;
+    a.spre[p] = R.spre[p].p;
+    a.el[p] = R.el[p].p;
+    a.epre[p] = R.epre[p].p;
+    a.fl[p] = R.fl[p].p;
+    a.pl[p] = R.pl[p].p;
+    a.ppre[p] = R.ppre[p].p;
+  }
+  a.al = R.al.p;
+  a.seed = R.seed.p;
+  a.seedpre = R.seedpre.p;
+  a.fin = bs.fin.p;
+  a.skeys = bs.skeys.p;
+  a.act = bs.act.p;
+  a.nfin = bs.nfinal;
+  a.roots = m->droots.p;
+  a.k = m->k;
+  a.limit = static_cast<unsigned long long>(m->fallback * static_cast<double>(g->n) * m->k);
+  if (a.limit < 1) a.limit = 1;
+  a.st = R.d_st;
+  return a;
+}
+
+static void msr_upload(vgx_msbfs* m) {
+  const MsrArgs a = msr_args(m);
+  VGX_HIP(hipMemcpyAsync(m->rep.d_args, &a, sizeof(a), hipMemcpyHostToDevice, m->g->stream));
+}
+
+static void msr_sync(vgx_msbfs* m) {
+  VGX_HIP(hipMemcpyAsync(m->rep.h_st, m->rep.d_st, sizeof(MsrState), hipMemcpyDeviceToHost, m->g->stream));
+  VGX_HIP(hipStreamSynchronize(m->g->stream));
+}
+
+// The current distances in rep.d (from the last full pass when they are
+// still held as level masks).
+static void msr_materialize(vgx_msbfs* m) {
+  msr_ensure(m, m->n);
+  if (m->in_d32) return;
+  hipStream_t s = m->g->stream;
+  if (m->n) {
+    k_ms_to_d32<<<msr_grid(m->n * kWave), kMsBlock, 0, s>>>(m->newbits.p, m->levels, m->n, m->rep.d.p);
+    VGX_LAUNCH_CHECK();
+  }
+  m->in_d32 = true;
+}
+
+constexpr uint32_t kMsrInvRounds = 2;   // invalidation rounds per host check
+constexpr uint32_t kMsrPropRounds = 4;  // propagation rounds per host check
+
+// One batch: graph update and repair of every root (or a full pass when
+// the policy or the repair's own limit says so).
+static void msr_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
+                      bool device, hipStream_t producer) {
+  vgx_graph* g = m->g;
+  hipStream_t s = g->stream;
+  m->last_repaired = false;
+  m->last_fallback = false;
+  m->last_affected_pairs = m->last_lowered = m->last_candidates = 0;
+  if (k == 0) return;
+  const auto t0 = std::chrono::steady_clock::now();
+  // the arm: a repair needs distances to repair; a root id the batch brings
+  // into the graph needs a pass (its own distance becomes 0)
+  bool repair = m->ran && m->repair_mode != 2 && m->dist_version == g->version;
+  if (repair && m->repair_mode == 0 && m->pass_ms > 0.0) {
+    const double arcs = static_cast<double>(g->adj[0].live) + 1.0;
+    const double per_op = m->repair_us_per_op > 0.0 ? m->repair_us_per_op : 1e3 * m->pass_ms / (arcs / 256.0);
+    repair = per_op * static_cast<double>(k) < 1e3 * m->pass_ms;
+  }
+  if (repair) msr_materialize(m);
+  if (device) graph_stage_device(g, src, dst, add, k, producer);
+  else graph_stage(g, src, dst, add, k);
+  // distances cover the vertices of the last pass or repair (m->n): a root
+  // whose id the graph now reaches needs a pass (its own distance becomes 0)
+  const uint64_t n_cov = m->n;
+  for (uint32_t r = 0; r < m->k; ++r)
+    if (m->roots[r] >= n_cov && m->roots[r] < g->n) repair = false;
+  m->batches += 1;
+  if (!repair) {
+    graph_apply_deletions(g);
+    graph_apply_additions(g);
+    graph_finish_batch(g);
+    const auto p0 = std::chrono::steady_clock::now();
+    ms_run(m);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p0).count();
+    m->pass_ms = m->pass_ms > 0.0 ? 0.75 * m->pass_ms + 0.25 * ms : ms;
+    m->full_passes += 1;
+    return;
+  }
+  MsRepair& R = m->rep;
+  msr_ensure(m, g->n);
+  if (g->n > n_cov) {
+    k_msr_fill_inf<<<msr_grid((g->n - n_cov) * 64), kMsBlock, 0, s>>>(R.d.p, n_cov, g->n);
+    VGX_LAUNCH_CHECK();
+  }
+  m->n = g->n;
+  graph_apply_deletions(g);
+  msr_upload(m);
+  const MsrArgs* pa = R.d_args;
+  k_msr_zero_state<<<1, 64, 0, s>>>(R.d_st);
+  VGX_LAUNCH_CHECK();
+  if (g->bs[0].nfinal) {
+    k_msr_candidates<<<msr_grid(g->bs[0].nfinal * kWave / kMsrWindow), kMsBlock, 0, s>>>(pa);
+    VGX_LAUNCH_CHECK();
+  }
+  bool fallback = false;
+  for (uint32_t r0 = 0;; r0 += kMsrInvRounds) {
+    for (uint32_t i = 0; i < kMsrInvRounds; ++i) {
+      const int par = static_cast<int>(i & 1);  // kMsrInvRounds is even
+      k_msr_take<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
+      VGX_LAUNCH_CHECK();
+      k_msr_support<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
+      VGX_LAUNCH_CHECK();
+      k_msr_evaluate<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
+      VGX_LAUNCH_CHECK();
+      k_msr_emit<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
+      VGX_LAUNCH_CHECK();
+    }
+    msr_sync(m);
+    if (R.h_st->fallback || R.h_st->apairs > static_cast<unsigned long long>(m->fallback * g->n * m->k + 1)) {
+      fallback = true;
+      break;
+    }
+    if (R.h_st->tq[0] == 0) break;  // nothing touched for the next round
+    if (r0 > 2 * g->n + 4) throw Error(VGX_DEVICE_ERROR, "multi-source invalidation did not terminate");
+  }
+  m->last_candidates = R.h_st->candidates;
+  m->last_affected_pairs = R.h_st->apairs;
+  graph_apply_additions(g);
+  graph_finish_batch(g);  // may compact: row pointers change
+  msr_upload(m);
+  if (fallback) {
+    VGX_HIP(hipMemsetAsync(R.cand.p, 0, (g->n + 1) * 8, s));
+    VGX_HIP(hipMemsetAsync(R.aff.p, 0, (g->n + 1) * 8, s));
+    ms_run(m);
+    m->fallbacks += 1;
+    m->full_passes += 1;
+    m->last_fallback = true;
+    return;
+  }
+  const uint64_t n_aff = R.h_st->aall;
+  if (n_aff) {
+    k_msr_reset<<<msr_grid(n_aff * kWave / kMsrWindow), kMsBlock, 0, s>>>(pa);
+    VGX_LAUNCH_CHECK();
+    k_msr_seed<<<kMsrBlocks, kMsBlock, 0, s>>>(pa);
+    VGX_LAUNCH_CHECK();
+    k_msr_collect<<<msr_grid(n_aff * kWave / kMsrWindow), kMsBlock, 0, s>>>(pa);
+    VGX_LAUNCH_CHECK();
+  }
+  if (g->bs[0].nfinal) {
+    k_msr_relax_additions<<<msr_grid(g->bs[0].nfinal * kWave / kMsrWindow), kMsBlock, 0, s>>>(pa);
+    VGX_LAUNCH_CHECK();
+  }
+  for (uint32_t r0 = 0;; r0 += kMsrPropRounds) {
+    for (uint32_t i = 0; i < kMsrPropRounds; ++i) {
+      const int par = static_cast<int>(i & 1);  // kMsrPropRounds is even
+      k_msr_ptake<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
+      VGX_LAUNCH_CHECK();
+      k_msr_propagate<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
+      VGX_LAUNCH_CHECK();
+    }
+    msr_sync(m);
+    if (R.h_st->fq[0] == 0) break;
+    if (r0 > 2 * g->n + 4) throw Error(VGX_DEVICE_ERROR, "multi-source propagation did not terminate");
+  }
+  if (n_aff) {
+    k_msr_clear<<<msr_grid(n_aff), kMsBlock, 0, s>>>(pa);
+    VGX_LAUNCH_CHECK();
+  }
+  m->last_lowered = R.h_st->lowered;
+  m->in_d32 = true;
+  m->dist_version = g->version;
+  m->levels = 0;  // recomputed on demand from d
+  m->last_repaired = true;
+  m->repaired += 1;
+  const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+  const double per_op = us / static_cast<double>(k);
+  m->repair_us_per_op = m->repair_us_per_op > 0.0 ? 0.75 * m->repair_us_per_op + 0.25 * per_op : per_op;
+}
+
+// Reached count per root and the level count (deepest distance + 1) of
+// the distances held in rep.d.
+static uint32_t msr_stats(vgx_msbfs* m, std::vector<uint64_t>* reached) {
+  hipStream_t s = m->g->stream;
+  DevBuf<unsigned long long> acc;
+  acc.ensure(65);
+  VGX_HIP(hipMemsetAsync(acc.p, 0, 65 * 8, s));
+  if (m->n) {
+    k_msr_stats<<<msr_grid(m->n * kWave), kMsBlock, 0, s>>>(m->rep.d.p, m->n, m->k, acc.p,
+                                                           reinterpret_cast<unsigned int*>(acc.p + 64));
+    VGX_LAUNCH_CHECK();
+  }
+  unsigned long long h[65];
+  VGX_HIP(hipMemcpyAsync(h, acc.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipStreamSynchronize(s));
+  if (reached) reached->assign(h, h + m->k);
+  bool any = false;
+  for (uint32_t r = 0; r < m->k; ++r) any = any || h[r] > 0;
+  return any ? static_cast<uint32_t>(h[64] & 0xFFFFFFFFu) + 1 : 0;
 }
 
 }  // namespace vgx
@@ -636,6 +1469,9 @@ vgx_status vgx_msbfs_create(vgx_graph* g, const uint32_t* roots, uint32_t k, vgx
     m->d_args = static_cast<MsArgs*>(dev_alloc(sizeof(MsArgs)));
     m->d_ctl = static_cast<MsCtl*>(dev_alloc(sizeof(MsCtl)));
     VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&m->h_ctl), sizeof(MsCtl), hipHostMallocDefault));
+    m->rep.d_args = static_cast<MsrArgs*>(dev_alloc(sizeof(MsrArgs)));
+    m->rep.d_st = static_cast<MsrState*>(dev_alloc(sizeof(MsrState)));
+    VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&m->rep.h_st), sizeof(MsrState), hipHostMallocDefault));
   } catch (...) {
     vgx_msbfs_destroy(m);
     throw;
@@ -652,6 +1488,9 @@ vgx_status vgx_msbfs_destroy(vgx_msbfs* m) {
   if (m->d_args) dev_free(m->d_args);
   if (m->d_ctl) dev_free(m->d_ctl);
   if (m->h_ctl) (void)hipHostFree(m->h_ctl);
+  if (m->rep.d_args) dev_free(m->rep.d_args);
+  if (m->rep.d_st) dev_free(m->rep.d_st);
+  if (m->rep.h_st) (void)hipHostFree(m->rep.h_st);
   delete m;
   VGX_API_END
 }
@@ -691,7 +1530,10 @@ vgx_status vgx_msbfs_distances(vgx_msbfs* m, uint32_t r, uint32_t* out, uint64_t
   DevBuf<uint32_t> tmp;
   tmp.ensure(m->n + 1);
   if (m->n) {
-    k_ms_distances<<<ms_blocks(m->n), kMsBlock, 0, s>>>(m->newbits.p, m->levels, m->n, r, tmp.p);
+    if (m->in_d32)
+      k_msr_root<<<ms_blocks(m->n), kMsBlock, 0, s>>>(m->rep.d.p, m->n, r, tmp.p);
+    else
+      k_ms_distances<<<ms_blocks(m->n), kMsBlock, 0, s>>>(m->newbits.p, m->levels, m->n, r, tmp.p);
     VGX_LAUNCH_CHECK();
     VGX_HIP(hipMemcpyAsync(out, tmp.p, m->n * 4, hipMemcpyDeviceToHost, s));
   }
@@ -704,13 +1546,17 @@ vgx_status vgx_msbfs_levels_u8(vgx_msbfs* m, uint8_t* out, uint64_t cap) {
   if (!m || !out) throw Error(VGX_INVALID_ARGUMENT, "null argument");
   if (!m->ran) throw Error(VGX_INVALID_ARGUMENT, "run() first");
   if (cap < static_cast<uint64_t>(m->k) * m->n) throw Error(VGX_INVALID_ARGUMENT, "output buffer too small");
-  if (m->levels > 255) throw Error(VGX_UNSUPPORTED, "depth exceeds 254: use vgx_msbfs_distances");
   VGX_HIP(hipSetDevice(m->g->device));
+  const uint32_t levels = m->in_d32 ? msr_stats(m, nullptr) : m->levels;
+  if (levels > 255) throw Error(VGX_UNSUPPORTED, "depth exceeds 254: use vgx_msbfs_distances");
   hipStream_t s = m->g->stream;
   DevBuf<uint8_t> tmp;
   tmp.ensure(static_cast<uint64_t>(m->k) * m->n + 1);
   if (m->n) {
-    k_ms_levels_u8<<<ms_blocks(m->n), kMsBlock, 0, s>>>(m->newbits.p, m->levels, m->n, m->k, tmp.p);
+    if (m->in_d32)
+      k_msr_u8<<<ms_blocks((m->n + 63) / 64, 1, 4096), kMsBlock, 0, s>>>(m->rep.d.p, m->n, m->k, tmp.p);
+    else
+      k_ms_levels_u8<<<ms_blocks(m->n), kMsBlock, 0, s>>>(m->newbits.p, m->levels, m->n, m->k, tmp.p);
     VGX_LAUNCH_CHECK();
     VGX_HIP(hipMemcpyAsync(out, tmp.p, static_cast<uint64_t>(m->k) * m->n, hipMemcpyDeviceToHost, s));
   }
@@ -723,6 +1569,13 @@ vgx_status vgx_msbfs_reached(vgx_msbfs* m, uint64_t* reached_per_root, uint32_t 
   if (!m || !reached_per_root) throw Error(VGX_INVALID_ARGUMENT, "null argument");
   if (!m->ran) throw Error(VGX_INVALID_ARGUMENT, "run() first");
   VGX_HIP(hipSetDevice(m->g->device));
+  if (m->in_d32) {
+    std::vector<uint64_t> rc;
+    msr_stats(m, &rc);
+    for (uint32_t r = 0; r < m->k && r < cap; ++r) reached_per_root[r] = rc[r];
+    vgx::t_last_error.clear();
+    return VGX_OK;
+  }
   std::vector<unsigned long long> seen(m->n);
   if (m->n) {
     VGX_HIP(hipMemcpyAsync(seen.data(), m->seen.p, m->n * 8, hipMemcpyDeviceToHost, m->g->stream));
@@ -790,7 +1643,56 @@ vgx_status vgx_msbfs_autotune(vgx_msbfs* m, uint32_t reps, uint32_t* best_blocks
 vgx_status vgx_msbfs_level_count(vgx_msbfs* m, uint32_t* levels) {
   VGX_API_BEGIN
   if (!m || !levels) throw Error(VGX_INVALID_ARGUMENT, "null argument");
-  *levels = m->levels;
+  if (m->in_d32) {
+    VGX_HIP(hipSetDevice(m->g->device));
+    *levels = msr_stats(m, nullptr);
+  } else {
+    *levels = m->levels;
+  }
+  VGX_API_END
+}
+
+vgx_status vgx_msbfs_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k) {
+  VGX_API_BEGIN
+  if (!m || (k && (!src || !dst || !add))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  VGX_HIP(hipSetDevice(m->g->device));
+  msr_apply(m, src, dst, add, k, false, nullptr);
+  VGX_API_END
+}
+
+vgx_status vgx_msbfs_apply_device(vgx_msbfs* m, const uint32_t* src_dev, const uint32_t* dst_dev,
+                                  const uint8_t* add_dev, uint64_t k, void* stream) {
+  VGX_API_BEGIN
+  if (!m || (k && (!src_dev || !dst_dev || !add_dev))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  VGX_HIP(hipSetDevice(m->g->device));
+  msr_apply(m, src_dev, dst_dev, add_dev, k, true, static_cast<hipStream_t>(stream));
+  VGX_API_END
+}
+
+vgx_status vgx_msbfs_set_repair(vgx_msbfs* m, int mode, double fallback_fraction) {
+  VGX_API_BEGIN
+  if (!m) throw Error(VGX_INVALID_ARGUMENT, "null msbfs");
+  if (mode < 0 || mode > 2) throw Error(VGX_INVALID_ARGUMENT, "mode must be 0 (adaptive), 1 (repair) or 2 (full)");
+  if (!(fallback_fraction > 0.0)) throw Error(VGX_INVALID_ARGUMENT, "fallback fraction must be positive");
+  m->repair_mode = mode;
+  m->fallback = fallback_fraction;
+  VGX_API_END
+}
+
+vgx_status vgx_msbfs_repair_stats(vgx_msbfs* m, vgx_msbfs_repair_info* out) {
+  VGX_API_BEGIN
+  if (!m || !out) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  out->batches = m->batches;
+  out->repaired = m->repaired;
+  out->full_passes = m->full_passes;
+  out->fallbacks = m->fallbacks;
+  out->last_repaired = m->last_repaired ? 1 : 0;
+  out->last_fallback = m->last_fallback ? 1 : 0;
+  out->last_candidates = m->last_candidates;
+  out->last_affected_pairs = m->last_affected_pairs;
+  out->last_lowered = m->last_lowered;
+  out->pass_ms = m->pass_ms;
+  out->repair_us_per_op = m->repair_us_per_op;
   VGX_API_END
 }
 

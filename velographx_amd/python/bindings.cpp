@@ -949,7 +949,49 @@ PYBIND11_MODULE(_velographx, m) {
             d["ms"] = ms;
             return d;
           },
-          py::arg("reps") = 3);
+          py::arg("reps") = 3)
+      .def("apply",
+           [](velographx::MultiSourceBFS& m, const UpdateBatch& batch) {
+             py::gil_scoped_release nogil;
+             m.apply(batch);
+           },
+           py::arg("batch"))
+      .def("apply_arrays",
+           [](velographx::MultiSourceBFS& m, const U32Array& src, const U32Array& dst,
+              const py::array_t<std::uint8_t, py::array::c_style | py::array::forcecast>& add) {
+             if (src.size() != dst.size() || src.size() != add.size())
+               throw std::invalid_argument("src, dst and add must have equal length");
+             const auto* s = src.data();
+             const auto* d = dst.data();
+             const auto* a = add.data();
+             const auto k = static_cast<std::size_t>(src.size());
+             py::gil_scoped_release nogil;
+             m.apply_arrays(s, d, a, k);
+           },
+           py::arg("src"), py::arg("dst"), py::arg("add"))
+      .def(
+          "set_repair",
+          [](velographx::MultiSourceBFS& m, int mode, double fallback_fraction) {
+            velographx::vgx_check(vgx_msbfs_set_repair(m.handle(), mode, fallback_fraction));
+          },
+          py::arg("mode") = 0, py::arg("fallback_fraction") = 0.1)
+      .def("repair_stats", [](const velographx::MultiSourceBFS& m) {
+        vgx_msbfs_repair_info r{};
+        velographx::vgx_check(vgx_msbfs_repair_stats(m.handle(), &r));
+        py::dict d;
+        d["batches"] = r.batches;
+        d["repaired"] = r.repaired;
+        d["full_passes"] = r.full_passes;
+        d["fallbacks"] = r.fallbacks;
+        d["last_repaired"] = r.last_repaired != 0;
+        d["last_fallback"] = r.last_fallback != 0;
+        d["last_candidates"] = r.last_candidates;
+        d["last_affected_pairs"] = r.last_affected_pairs;
+        d["last_lowered"] = r.last_lowered;
+        d["pass_ms"] = r.pass_ms;
+        d["repair_us_per_op"] = r.repair_us_per_op;
+        return d;
+      });
 
   // addition: weighted personalized PageRank kernels on device pointers
   m.def(
