@@ -50,8 +50,11 @@ Secondary legs, each in its own object of the same JSON line:
 * contrast  -- Erdos-Renyi, 4,194,304 vertices, mean degree 16;
 * batching  -- R-MAT scale 20: 64 roots per hipGraph-captured multi-source
                pass (autotuned), then 256 update batches of log-uniform size
-               10 .. 100,000, each applied and answered by a replay; sampled
-               roots checked bit-exact against the single-root path;
+               10 .. 100,000, answered by two arms on two copies: apply and
+               replay the pass, and the 64-root repair (captured repair
+               programs, a pass where its cost model prefers one); the arms
+               compared on all 64 roots and sampled roots checked bit-exact
+               against the single-root path;
 * pagerank  -- directed R-MAT scale 22: FP32 pull PageRank to an L1
                residual below 1e-7, then 50 batches of 10,000 ops answered
                by the residual-push repair and, on a second copy, by a
@@ -502,8 +505,13 @@ def leg_batched(vx, wl, g, lcc, m_c, k, plain_dist, plain_gteps):
 
 def leg_batching(vx, wl, args):
     """R-MAT scale 20: 64 roots per captured multi-source pass, then 256
-    update batches of log-uniform size (10 .. 100,000 ops, 80/20), each
-    applied to the graph and answered by replaying the captured pass."""
+    update batches of log-uniform size (10 .. 100,000 ops, 80/20).  Two arms
+    over the same batches, each on its own copy of the graph: the
+    recompute arm applies the batch and replays the captured pass; the
+    repair arm (MultiSourceBFS.apply) repairs every root in captured repair
+    programs or, when its cost model prefers it, replays the pass.  Both
+    arms' 64 roots are compared after every 32nd batch, and 8 roots against
+    the single-root path."""
     import torch
 
     scale = args.batch_scale
@@ -511,6 +519,8 @@ def leg_batching(vx, wl, args):
     s, d = vx.rmat_edges(scale, args.edgefactor, 1)
     g = vx.Graph(n, False)
     g.bulk_load_arrays(s, d)
+    g2 = vx.Graph(n, False)
+    g2.bulk_load_arrays(s, d)
     lcc, arcs_c = component(vx, wl, g)
     roots = wl.roots_in_component(lcc, 64, seed=1).tolist()
     ms = vx.MultiSourceBFS(g, roots)
@@ -533,12 +543,19 @@ def leg_batching(vx, wl, args):
     torch.cuda.synchronize()
     t_eager = (time.perf_counter() - t0) / reps
     ms.set_graph(True)
+    ms2 = vx.MultiSourceBFS(g2, roots)
+    ms2.set_tuning(tuned["blocks"], tuned["alpha"])
+    ms2.set_graph(True)
+    ms2.set_repair(0, 0.1)
+    ms2.run(g2)
     sizes = wl.log_uniform_sizes(args.uneven_batches, 1.0, 5.0, seed=1)
     eu, ev = wl.canonical_edges(s, d, False)
     stream = wl.BatchStream(eu, ev, n, False, seed=3)
     batches = [stream.next(int(k), 0.8) for k in sizes]
-    t_apply, t_pass = [], []
+    t_apply, t_pass, t_rep = [], [], []
+    repaired = []
     exact = True
+    arms_equal = True
     checked = 0
     singles = [vx.IncrementalBFS(g, int(r)) for r in roots[:8]]
     for i, (bs, bd, ba) in enumerate(batches):
@@ -550,24 +567,50 @@ def leg_batching(vx, wl, args):
         ms.run(g)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
+        ms2.apply_arrays(bs, bd, ba)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
         t_apply.append(t1 - t0)
         t_pass.append(t2 - t1)
-        if i % 32 == 31 or i == len(batches) - 1:  # verification outside the timer
+        t_rep.append(t3 - t2)
+        repaired.append(bool(ms2.repair_stats()["last_repaired"]))
+        if i % 32 == 31 or i == len(batches) - 1:  # verification outside the timers
             for j, o in enumerate(singles):
                 o.recompute()
                 exact &= bool(np.array_equal(ms.distances(j), o.distances_numpy()))
+            arms_equal &= bool(np.array_equal(ms.levels_u8(), ms2.levels_u8()))
             checked += 1
     ops = int(sum(int(k) for k in sizes))
+    t_rec = np.array(t_apply) + np.array(t_pass)
+    t_rep = np.array(t_rep)
+    rep_mask = np.array(repaired)
+    by_size = {}
+    for lo, hi in ((10, 100), (100, 1000), (1000, 10000), (10000, 100001)):
+        sel = (sizes >= lo) & (sizes < hi)
+        if sel.any():
+            by_size[f"{lo}-{hi - 1 if hi > 100000 else hi}"] = {
+                "batches": int(sel.sum()), "repaired": int((rep_mask & sel).sum()),
+                "recompute_ms": 1e3 * float(np.mean(t_rec[sel])), "repair_arm_ms": 1e3 * float(np.mean(t_rep[sel]))}
+    st = ms2.repair_stats()
     out = {"workload": f"undirected R-MAT s{scale} ef{args.edgefactor} seed 1, 64 LCC roots per captured pass; "
                        f"{len(batches)} batches of log-uniform size 10..100000 (80/20)",
            "arcs": int(g.edge_count_directed), "pass_ms_graph": 1e3 * t_graph, "pass_ms_eager": 1e3 * t_eager,
            "aggregate_gteps_graph": arcs_c / 2.0 * 64 / t_graph / 1e9,
            "tuning": {"blocks": tuned["blocks"], "alpha": tuned["alpha"]},
            "batches": len(batches), "update_ops": ops,
-           "mean_apply_ms": 1e3 * float(np.mean(t_apply)), "mean_pass_ms": 1e3 * float(np.mean(t_pass)),
-           "batches_per_s": len(batches) / (sum(t_apply) + sum(t_pass)),
+           "recompute_arm": {"mean_apply_ms": 1e3 * float(np.mean(t_apply)),
+                             "mean_pass_ms": 1e3 * float(np.mean(t_pass)),
+                             "batches_per_s": len(batches) / float(t_rec.sum())},
+           "repair_arm": {"batches_per_s": len(batches) / float(t_rep.sum()),
+                          "mean_batch_ms": 1e3 * float(t_rep.mean()),
+                          "repaired_batches": int(rep_mask.sum()), "full_pass_batches": int(st["full_passes"]),
+                          "fallbacks": int(st["fallbacks"]), "policy": "per-size-class running means (mode 0)"},
+           "by_batch_size": by_size,
+           "batches_per_s": len(batches) / float(t_rep.sum()),
+           "repair_vs_recompute": float(t_rec.sum() / t_rep.sum()),
            "graph_replays": int(ms.stats()["graph_replays"]),
-           "exact_vs_single_root": exact, "verified_batches": checked, "verified_roots_per_check": len(singles)}
+           "exact_vs_single_root": exact, "arms_equal_all_64_roots": arms_equal, "verified_batches": checked,
+           "verified_roots_per_check": len(singles)}
     log(f"batching: {out}")
     return out
 

@@ -447,11 +447,16 @@ vgx_status vgx_msbfs_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* ds
 vgx_status vgx_msbfs_apply_device(vgx_msbfs* m, const uint32_t* src_dev, const uint32_t* dst_dev,
                                   const uint8_t* add_dev, uint64_t k, void* stream);
 /* Repair policy (no reference counterpart): mode 0 chooses per batch from
- * running means of the repair time per op and the full-pass time, 1 always
- * repairs, 2 always runs a full pass; a repair whose affected (vertex, root)
- * pairs exceed fallback_fraction * vertices * roots stops and runs a full
- * pass (default 0.1, as the reference's deletion_fallback_fraction,
- * bfs.hpp:18, bounds one root's affected vertices). */
+ * running means of the answer-ready time of each arm -- the full pass, and
+ * the repair per batch-size class floor(log2 ops), a class without its own
+ * mean borrowing the nearest smaller class's scaled by the size ratio; a
+ * repair that allocated its arrays or captured its programs is not counted,
+ * and a class predicted within twice the pass time that chose the pass 16
+ * times tries the repair again
+ * --, 1 always repairs, 2 always runs a full pass.  A repair whose affected
+ * (vertex, root) pairs exceed fallback_fraction * vertices * roots stops and
+ * runs a full pass (default 0.1; the reference's deletion_fallback_fraction,
+ * 0.35 at bfs.hpp:18, plays this part for one root). */
 vgx_status vgx_msbfs_set_repair(vgx_msbfs* m, int mode, double fallback_fraction);
 typedef struct {
   uint64_t batches, repaired, full_passes, fallbacks;

@@ -8,7 +8,9 @@ CPU oracle on the host (nothing here compares the GPU with itself):
   against the oracle's BFS of the oracle's own graph after the same ops;
 * Erdos-Renyi, 4,194,304 vertices, mean degree 16: 16 roots;
 * undirected R-MAT scale 20: all 64 roots of a multi-source pass (captured
-  launch program) after each of 4 uneven update batches;
+  launch program) after each of 4 uneven update batches, and all 64 roots
+  of the multi-source incremental repair after each of 6 batches of 10 to
+  60,000 ops (4 forced repairs, 2 under the adaptive policy);
 * directed R-MAT scale 22: the FP32 PageRank solve to an L1 residual below
   1e-7, its warm-started re-solve after 3 batches of 10,000 ops and the
   residual-push repair after 3 more (10,000, 10,000 and 100 ops), within
@@ -180,6 +182,45 @@ def test_config3_s20_64_roots_after_uneven_batches(vx, wl, orc):
         for i, want in enumerate(expect):
             assert np.array_equal(ms.distances(i), want), (b, k, i, roots[i])
     log(f"s20 multi-source: 64 roots equal the oracle after batches of {sizes} ops")
+
+
+def test_config3_s20_64_root_repairs(vx, wl, orc):
+    """The 64-root incremental repair at full size: every batch repaired
+    (never handed to a pass), then the adaptive policy; all 64 roots against
+    the oracle after every batch."""
+    n = 1 << 20
+    s, d = vx.rmat_edges(20, 16, 1)
+    model = orc.CsrModel(n, False)
+    model.bulk_load_edges(s, d)
+    lcc = component_of_hub(orc, model)
+    roots = [int(r) for r in wl.roots_in_component(lcc, 64, seed=1)]
+    g = vx.Graph(n, False)
+    g.bulk_load_arrays(s, d)
+    ms = vx.MultiSourceBFS(g, roots)
+    ms.set_graph(True)
+    ms.set_repair(1, 1.0)
+    ms.run(g)
+    eu, ev = model.undirected_edges()
+    stream = wl.BatchStream(eu, ev, n, False, seed=5)
+    plan = [(10, 1), (300, 1), (3000, 1), (30000, 1), (25, 0), (60000, 0)]
+    for b, (k, mode) in enumerate(plan):
+        ms.set_repair(mode, 1.0 if mode == 1 else 0.1)
+        bs, bd, ba = stream.next(k, 0.8)
+        ms.apply_arrays(bs, bd, ba)
+        model.apply(bs, bd, ba)
+        st = ms.repair_stats()
+        if mode == 1:
+            assert st["last_repaired"], (b, k, st)
+        expect = oracle_bfs_many(orc, model, roots)
+        for i, want in enumerate(expect):
+            assert np.array_equal(ms.distances(i), want), (b, k, i, roots[i])
+        lv = ms.levels_u8()
+        for i, want in enumerate(expect[:8]):
+            w8 = want.astype(np.int64)
+            w8[want == INF] = 255
+            assert np.array_equal(lv[i], w8.astype(np.uint8)), (b, i)
+        log(f"s20 multi-source repair: batch of {k} ops ({'repaired' if st['last_repaired'] else 'full pass'}, "
+            f"{st['last_affected_pairs']} affected pairs, {st['last_lowered']} lowered), 64 roots equal the oracle")
 
 
 def test_config4_s22_directed_fp32_pagerank(vx, wl, orc, rmat22):

@@ -112,7 +112,8 @@ struct MsArgs {
 struct MsrState {
   unsigned long long tq[2];  // plain: vertices given candidate roots (invalidation input)
   unsigned long long vq[2];  // plain: of those, the ones with roots left to evaluate
-  unsigned long long sq[2];  // packed: of those, the ones with in-arcs (support scan)
+  unsigned long long sq[2];  // packed: of those, the ones with in-arcs (support scan of the row heads)
+  unsigned long long rq;     // packed: vertices still unsupported after the head scan (rest of the rows)
   unsigned long long eq[2];  // packed: newly affected vertices with out-arcs (emit)
   unsigned long long aall;   // plain: affected vertices (any root)
   unsigned long long apairs; // affected (vertex, root) pairs
@@ -142,6 +143,8 @@ struct MsrArgs {
   uint32_t* vl[2];
   uint32_t* sl[2];
   unsigned long long* spre[2];
+  uint32_t* rl;
+  unsigned long long* rpre;
   uint32_t* el[2];
   unsigned long long* epre[2];
   uint32_t* al;
@@ -167,10 +170,16 @@ struct MsRepair {
   DevBuf<unsigned long long> cand, aff, tm, sup, nm, pend, pm;
   DevBuf<uint32_t> tl[2], vl[2], sl[2], el[2], al, seed, fl[2], pl[2];
   DevBuf<unsigned long long> spre[2], epre[2], seedpre, ppre[2];
+  DevBuf<uint32_t> rl;
+  DevBuf<unsigned long long> rpre;
   uint64_t n = 0;          // vertices the arrays cover
+  bool setup = false;      // the current batch allocated arrays or captured a program
   MsrArgs* d_args = nullptr;
+  MsrArgs* h_args = nullptr;  // pinned staging of the argument block
   MsrState* d_st = nullptr;
   MsrState* h_st = nullptr;  // pinned
+  hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 // Chunks of the rows longer than kMsShortRow in one direction's row table:
@@ -218,7 +227,11 @@ struct vgx_msbfs {
   uint64_t dist_version = ~0ull;  // graph version the distances belong to
   int repair_mode = 0;       // 0 adaptive, 1 always repair, 2 always a full pass
   double fallback = 0.1;     // affected pairs past fallback * n * k give up
-  double pass_ms = 0.0;      // running means of the two arms (answer-ready)
+  // running means of the two arms, answer-ready, graph update included: the
+  // full pass, and the repair per batch-size class (floor(log2 ops))
+  double pass_ms = 0.0;
+  double repair_ms[48] = {};
+  uint32_t passes_in_class[48] = {};  // full passes chosen in a row per class (a repair is retried every 16th)
   double repair_us_per_op = 0.0;
   uint64_t batches = 0, repaired = 0, full_passes = 0, fallbacks = 0;
   bool last_repaired = false, last_fallback = false;
@@ -701,6 +714,7 @@ static void ms_run(vgx_msbfs* m) {
 // 64-bit ballot turns a lane test back into a mask.
 
 constexpr uint32_t kMsrWindow = 16;  // items a wave takes at a time from a plain list
+constexpr uint32_t kMsrHead = 64;    // in-arcs of a row the first support scan reads
 constexpr unsigned kMsrBlocks = 1024;
 
 __device__ __forceinline__ bool lane_bit(unsigned long long m) { return (m >> lane_id()) & 1ull; }
@@ -729,6 +743,42 @@ __device__ __forceinline__ void msr_windows(uint64_t count, Fn&& fn) {
     fn(w * kMsrWindow);
 }
 
+// Walks the pairs (a, b) a wave holds one per lane (b = kTomb: none) in
+// groups of eight: the eight pairs' rows d[a][lane], d[b][lane] and masks
+// ma[a], mb[b] (when given) are loaded together before body(t, a, b, da,
+// db, ma, mb) runs for each pair in lane order, so a wave keeps sixteen row
+// loads in flight instead of two.
+template <class Body>
+__device__ __forceinline__ void msr_pairs8(const uint32_t* __restrict__ d, const unsigned long long* ma,
+                                           const unsigned long long* mb, uint32_t av, uint32_t bv, Body&& body) {
+  const unsigned lane = lane_id();
+  unsigned long long todo = __ballot(!(bv & kTomb));
+  while (todo) {
+    int tt[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      tt[i] = todo ? __ffsll(static_cast<long long>(todo)) - 1 : -1;
+      todo = todo ? todo & (todo - 1) : 0ull;
+    }
+    uint32_t A[8], B[8], DA[8], DB[8];
+    unsigned long long MA[8], MB[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int t = tt[i] < 0 ? 0 : tt[i];
+      A[i] = uni(__shfl(av, t, kWave));
+      B[i] = uni(__shfl(bv, t, kWave));
+      const bool on = tt[i] >= 0;
+      DA[i] = on ? d[A[i] * 64ull + lane] : kInf;
+      DB[i] = on ? d[B[i] * 64ull + lane] : kInf;
+      MA[i] = on && ma ? ma[A[i]] : 0ull;
+      MB[i] = on && mb ? mb[B[i]] : 0ull;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (tt[i] >= 0) body(tt[i], A[i], B[i], DA[i], DB[i], MA[i], MB[i]);
+  }
+}
+
 // Heads of deleted arcs u->v that were shortest-parent arcs for some root:
 // d_r(u) + 1 == d_r(v) (bfs.hpp:52-63).  They become candidates of round 0.
 __global__ __launch_bounds__(kMsBlock) void k_msr_candidates(const MsrArgs* __restrict__ pa) {
@@ -738,34 +788,29 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_candidates(const MsrArgs* __re
   unsigned long long ncand = 0;
   msr_windows(a.nfin, [&](uint64_t first) {
     const uint64_t j = first + lane;
-    uint32_t u = 0, v = 0;
-    bool del = false;
+    uint32_t u = 0, v = kTomb;
     if (lane < kMsrWindow && j < a.nfin) {
       const uint32_t p = a.fin[j];
       if (a.act[p] == kActDel) {
-        del = true;
         u = static_cast<uint32_t>(a.skeys[p] >> 32);
         v = static_cast<uint32_t>(a.skeys[p]);
       }
     }
     bool want = false;
     uint32_t pv = 0;
-    for (unsigned long long todo = __ballot(del); todo; todo &= todo - 1) {
-      const int t = __ffsll(static_cast<long long>(todo)) - 1;
-      const uint32_t ut = uni(__shfl(u, t, kWave)), vt = uni(__shfl(v, t, kWave));
-      const uint32_t du = a.d[ut * 64ull + lane], dv = a.d[vt * 64ull + lane];
-      const unsigned long long bits = __ballot(du != kInf && du + 1 == dv && myroot != vt);
-      if (bits) {
-        ncand += __popcll(bits);
-        unsigned long long old = 0;
-        if (lane == 0) old = atomicOr(&a.cand[vt], bits);
-        old = __shfl(old, 0, kWave);
-        if (!old && lane == static_cast<unsigned>(t)) {
-          want = true;
-          pv = vt;
-        }
-      }
-    }
+    msr_pairs8(a.d, nullptr, nullptr, u, v,
+               [&](int t, uint32_t, uint32_t vt, uint32_t du, uint32_t dv, unsigned long long, unsigned long long) {
+                 const unsigned long long bits = __ballot(du != kInf && du + 1 == dv && myroot != vt);
+                 if (!bits) return;
+                 ncand += __popcll(bits);
+                 unsigned long long old = 0;
+                 if (lane == 0) old = atomicOr(&a.cand[vt], bits);
+                 old = __shfl(old, 0, kWave);
+                 if (!old && lane == static_cast<unsigned>(t)) {
+                   want = true;
+                   pv = vt;
+                 }
+               });
     push_plain(&a.st->tq[0], a.tl[0], want, pv);
   });
   if (lane == 0 && ncand) atomicAdd(&a.st->candidates, ncand);
@@ -786,6 +831,7 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_take(const MsrArgs* __restrict
     a.st->vq[par ^ 1] = 0;
     a.st->sq[par ^ 1] = 0;
     a.st->eq[par ^ 1] = 0;
+    a.st->rq = 0;
   }
   const uint64_t nt = a.st->tq[par];
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -807,33 +853,58 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_take(const MsrArgs* __restrict
       }
     }
     push_plain(&a.st->vq[par], a.vl[par], keep, v);
-    wave_push(&a.st->sq[par], a.sl[par], a.spre[par], keep && len > 0, v, len);
+    wave_push(&a.st->sq[par], a.sl[par], a.spre[par], keep && len > 0, v, min(len, kMsrHead));
+  }
+}
+
+// Step 1b (lane per vertex): the vertices the head scan left with roots
+// unsupported scan the rest of their in-rows.
+__global__ __launch_bounds__(kMsBlock) void k_msr_take_rest(const MsrArgs* __restrict__ pa, int par) {
+  const MsrArgs& a = *pa;
+  if (a.st->fallback) return;
+  const uint64_t nv = a.st->vq[par];
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t j0 = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x - lane_id(); j0 < nv;
+       j0 += stride) {
+    const uint64_t j = j0 + lane_id();
+    uint32_t v = 0, len = 0;
+    if (j < nv) {
+      v = a.vl[par][j];
+      if (a.tm[v] & ~a.sup[v]) len = a.in_row[v].y;
+    }
+    wave_push(&a.st->rq, a.rl, a.rpre, len > kMsrHead, v, len > kMsrHead ? len - kMsrHead : 0u);
   }
 }
 
 // Step 2 (edge-balanced over the in-arcs w->v of the listed vertices): root
 // r of v is supported when some live w has d_r(w) + 1 == d_r(v) and is not
-// affected for r.
-__global__ __launch_bounds__(kMsBlock) void k_msr_support(const MsrArgs* __restrict__ pa, int par) {
+// affected for r.  Run over the first kMsrHead in-arcs of every row, then
+// over the rest of the rows that still need it: shortest parents are found
+// early in most rows, and a hub's row need not be read whole.
+__global__ __launch_bounds__(kMsBlock) void k_msr_support(const MsrArgs* __restrict__ pa, int par, int rest) {
   const MsrArgs& a = *pa;
   if (a.st->fallback) return;
-  const unsigned long long qc = a.st->sq[par];
+  const unsigned long long qc = rest ? a.st->rq : a.st->sq[par];
+  const uint32_t* list = rest ? a.rl : a.sl[par];
+  const unsigned long long* pre = rest ? a.rpre : a.spre[par];
+  const uint32_t skip = rest ? kMsrHead : 0u;  // the rest of a row starts after its head
   const uint32_t W = gridDim.x * kMsWaves;
   const uint32_t gw = blockIdx.x * kMsWaves + threadIdx.x / kWave;
   const unsigned lane = lane_id();
-  lbs_edges(a.sl[par], a.spre[par], static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.in_row, gw, W,
+  lbs_edges(list, pre, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.in_row, gw, W,
             [&](uint32_t, uint32_t v, uint2, uint32_t slot, bool valid) {
-              const uint32_t w = valid ? a.in_arena[slot] : kTomb;
-              for (unsigned long long todo = __ballot(!(w & kTomb)); todo; todo &= todo - 1) {
-                const int t = __ffsll(static_cast<long long>(todo)) - 1;
-                const uint32_t vt = uni(__shfl(v, t, kWave)), wt = uni(__shfl(w, t, kWave));
-                const unsigned long long need = a.tm[vt] & ~a.sup[vt];
-                if (!need) continue;
-                const uint32_t dw = a.d[wt * 64ull + lane], dv = a.d[vt * 64ull + lane];
-                const unsigned long long bits =
-                    __ballot(lane_bit(need) && dw != kInf && dw + 1 == dv && !lane_bit(a.aff[wt]));
-                if (bits && lane == 0) atomicOr(&a.sup[vt], bits);
-              }
+              // a vertex whose roots are all supported by now sends no loads
+              const bool open = valid && (a.tm[v] & ~a.sup[v]) != 0ull;
+              const uint32_t w = open ? a.in_arena[slot + skip] : kTomb;
+              msr_pairs8(a.d, a.tm, a.aff, v, w,
+                         [&](int, uint32_t vt, uint32_t, uint32_t dv, uint32_t dw, unsigned long long tm,
+                             unsigned long long affw) {
+                           const unsigned long long need = tm & ~a.sup[vt];
+                           if (!need) return;
+                           const unsigned long long bits =
+                               __ballot(lane_bit(need) && dw != kInf && dw + 1 == dv && !lane_bit(affw));
+                           if (bits && lane == 0) atomicOr(&a.sup[vt], bits);
+                         });
             });
 }
 
@@ -890,21 +961,19 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_emit(const MsrArgs* __restrict
               const uint32_t x = valid ? a.out_arena[slot] : kTomb;
               bool want = false;
               uint32_t px = 0;
-              for (unsigned long long todo = __ballot(!(x & kTomb)); todo; todo &= todo - 1) {
-                const int t = __ffsll(static_cast<long long>(todo)) - 1;
-                const uint32_t ut = uni(__shfl(u, t, kWave)), xt = uni(__shfl(x, t, kWave));
-                const unsigned long long nm = a.nm[ut];
-                const uint32_t du = a.d[ut * 64ull + lane], dx = a.d[xt * 64ull + lane];
-                const unsigned long long bits = __ballot(lane_bit(nm) && du != kInf && dx == du + 1);
-                if (!bits) continue;
-                unsigned long long old = 0;
-                if (lane == 0) old = atomicOr(&a.cand[xt], bits);
-                old = __shfl(old, 0, kWave);
-                if (!old && lane == static_cast<unsigned>(t)) {
-                  want = true;
-                  px = xt;
-                }
-              }
+              msr_pairs8(a.d, a.nm, nullptr, u, x,
+                         [&](int t, uint32_t, uint32_t xt, uint32_t du, uint32_t dx, unsigned long long nm,
+                             unsigned long long) {
+                           const unsigned long long bits = __ballot(lane_bit(nm) && du != kInf && dx == du + 1);
+                           if (!bits) return;
+                           unsigned long long old = 0;
+                           if (lane == 0) old = atomicOr(&a.cand[xt], bits);
+                           old = __shfl(old, 0, kWave);
+                           if (!old && lane == static_cast<unsigned>(t)) {
+                             want = true;
+                             px = xt;
+                           }
+                         });
               push_plain(&a.st->tq[par ^ 1], a.tl[par ^ 1], want, px);
             });
 }
@@ -943,15 +1012,12 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_seed(const MsrArgs* __restrict
   lbs_edges(a.seed, a.seedpre, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.in_row, gw, W,
             [&](uint32_t, uint32_t v, uint2, uint32_t slot, bool valid) {
               const uint32_t w = valid ? a.in_arena[slot] : kTomb;
-              for (unsigned long long todo = __ballot(!(w & kTomb)); todo; todo &= todo - 1) {
-                const int t = __ffsll(static_cast<long long>(todo)) - 1;
-                const uint32_t vt = uni(__shfl(v, t, kWave)), wt = uni(__shfl(w, t, kWave));
-                const unsigned long long m = a.aff[vt] & ~a.aff[wt];
-                if (!m) continue;
-                const uint32_t dw = a.d[wt * 64ull + lane];
-                if (lane_bit(m) && dw != kInf && dw + 1 < a.d[vt * 64ull + lane])
-                  atomicMin(&a.d[vt * 64ull + lane], dw + 1);
-              }
+              msr_pairs8(a.d, a.aff, a.aff, v, w,
+                         [&](int, uint32_t vt, uint32_t, uint32_t dv, uint32_t dw, unsigned long long affv,
+                             unsigned long long affw) {
+                           if (lane_bit(affv & ~affw) && dw != kInf && dw + 1 < dv)
+                             atomicMin(&a.d[vt * 64ull + lane], dw + 1);
+                         });
             });
 }
 
@@ -994,35 +1060,31 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_relax_additions(const MsrArgs*
   unsigned long long low = 0;
   msr_windows(a.nfin, [&](uint64_t first) {
     const uint64_t j = first + lane;
-    uint32_t u = 0, v = 0;
-    bool ins = false;
+    uint32_t u = 0, v = kTomb;
     if (lane < kMsrWindow && j < a.nfin) {
       const uint32_t p = a.fin[j];
       if (a.act[p] == kActIns || a.act[p] == kActUndel) {
-        ins = true;
         u = static_cast<uint32_t>(a.skeys[p] >> 32);
         v = static_cast<uint32_t>(a.skeys[p]);
       }
     }
     bool want = false;
     uint32_t pv = 0;
-    for (unsigned long long todo = __ballot(ins); todo; todo &= todo - 1) {
-      const int t = __ffsll(static_cast<long long>(todo)) - 1;
-      const uint32_t ut = uni(__shfl(u, t, kWave)), vt = uni(__shfl(v, t, kWave));
-      const uint32_t du = a.d[ut * 64ull + lane];
-      bool lowered = false;
-      if (du != kInf && du + 1 < a.d[vt * 64ull + lane]) lowered = atomicMin(&a.d[vt * 64ull + lane], du + 1) > du + 1;
-      const unsigned long long bits = __ballot(lowered);
-      if (!bits) continue;
-      low += __popcll(bits);
-      unsigned long long old = 0;
-      if (lane == 0) old = atomicOr(&a.pend[vt], bits);
-      old = __shfl(old, 0, kWave);
-      if (!old && lane == static_cast<unsigned>(t)) {
-        want = true;
-        pv = vt;
-      }
-    }
+    msr_pairs8(a.d, nullptr, nullptr, u, v,
+               [&](int t, uint32_t, uint32_t vt, uint32_t du, uint32_t dv, unsigned long long, unsigned long long) {
+                 bool lowered = false;
+                 if (du != kInf && du + 1 < dv) lowered = atomicMin(&a.d[vt * 64ull + lane], du + 1) > du + 1;
+                 const unsigned long long bits = __ballot(lowered);
+                 if (!bits) return;
+                 low += __popcll(bits);
+                 unsigned long long old = 0;
+                 if (lane == 0) old = atomicOr(&a.pend[vt], bits);
+                 old = __shfl(old, 0, kWave);
+                 if (!old && lane == static_cast<unsigned>(t)) {
+                   want = true;
+                   pv = vt;
+                 }
+               });
     push_plain(&a.st->fq[0], a.fl[0], want, pv);
   });
   if (lane == 0 && low) atomicAdd(&a.st->lowered, low);
@@ -1070,27 +1132,23 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_propagate(const MsrArgs* __res
               const uint32_t x = valid ? a.out_arena[slot] : kTomb;
               bool want = false;
               uint32_t px = 0;
-              for (unsigned long long todo = __ballot(!(x & kTomb)); todo; todo &= todo - 1) {
-                const int t = __ffsll(static_cast<long long>(todo)) - 1;
-                const uint32_t ut = uni(__shfl(u, t, kWave)), xt = uni(__shfl(x, t, kWave));
-                const unsigned long long pm = a.pm[ut];
-                bool lowered = false;
-                if (lane_bit(pm)) {
-                  const uint32_t du = a.d[ut * 64ull + lane];
-                  if (du != kInf && du + 1 < a.d[xt * 64ull + lane])
-                    lowered = atomicMin(&a.d[xt * 64ull + lane], du + 1) > du + 1;
-                }
-                const unsigned long long bits = __ballot(lowered);
-                if (!bits) continue;
-                low += __popcll(bits);
-                unsigned long long old = 0;
-                if (lane == 0) old = atomicOr(&a.pend[xt], bits);
-                old = __shfl(old, 0, kWave);
-                if (!old && lane == static_cast<unsigned>(t)) {
-                  want = true;
-                  px = xt;
-                }
-              }
+              msr_pairs8(a.d, a.pm, nullptr, u, x,
+                         [&](int t, uint32_t, uint32_t xt, uint32_t du, uint32_t dx, unsigned long long pm,
+                             unsigned long long) {
+                           bool lowered = false;
+                           if (lane_bit(pm) && du != kInf && du + 1 < dx)
+                             lowered = atomicMin(&a.d[xt * 64ull + lane], du + 1) > du + 1;
+                           const unsigned long long bits = __ballot(lowered);
+                           if (!bits) return;
+                           low += __popcll(bits);
+                           unsigned long long old = 0;
+                           if (lane == 0) old = atomicOr(&a.pend[xt], bits);
+                           old = __shfl(old, 0, kWave);
+                           if (!old && lane == static_cast<unsigned>(t)) {
+                             want = true;
+                             px = xt;
+                           }
+                         });
               push_plain(&a.st->fq[par ^ 1], a.fl[par ^ 1], want, px);
             });
   if (lane == 0 && low) atomicAdd(&a.st->lowered, low);
@@ -1109,19 +1167,27 @@ __global__ void k_msr_zero_state(MsrState* st) {
   if (threadIdx.x < sizeof(MsrState) / 8) reinterpret_cast<unsigned long long*>(st)[threadIdx.x] = 0;
 }
 
-// d[v][r] from the level masks of a full pass (first level whose mask has
-// bit r), one wave per vertex, lane per root.
+// d[v][r] from the level masks of a full pass: bit r of newbits[L][v] is
+// set for exactly one level L (v's distance from root r), so each (v, r) of
+// a tile of 64 vertices is written once in LDS, then the tile goes out as
+// 64 contiguous 256-byte rows.
 __global__ __launch_bounds__(kMsBlock) void k_ms_to_d32(const unsigned long long* __restrict__ newbits,
                                                         uint32_t levels, uint64_t n, uint32_t* __restrict__ d) {
-  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * kMsWaves;
-  const unsigned lane = lane_id();
-  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kMsWaves) + threadIdx.x / kWave; v < n; v += waves) {
-    uint32_t dist = kInf;
-    for (uint32_t L = 0; L < levels; ++L) {
-      const unsigned long long m = newbits[static_cast<uint64_t>(L) * n + v];
-      if (dist == kInf && lane_bit(m)) dist = L;
+  __shared__ uint32_t tile[64][65];
+  const uint32_t vv = threadIdx.x / 4, r0 = (threadIdx.x % 4) * 16;
+  for (uint64_t v0 = blockIdx.x * 64ull; v0 < n; v0 += gridDim.x * 64ull) {
+    for (uint32_t i = threadIdx.x; i < 64 * 64; i += blockDim.x) tile[i / 64][i % 64] = kInf;
+    __syncthreads();
+    if (v0 + vv < n) {
+      for (uint32_t L = 0; L < levels; ++L) {
+        const uint32_t m = static_cast<uint32_t>(newbits[static_cast<uint64_t>(L) * n + v0 + vv] >> r0) & 0xFFFFu;
+        for (uint32_t b = m; b; b &= b - 1) tile[vv][r0 + __ffs(b) - 1] = L;
+      }
     }
-    d[v * 64 + lane] = dist;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 64 * 64; i += blockDim.x)
+      if (v0 + i / 64 < n) d[(v0 + i / 64) * 64 + i % 64] = tile[i / 64][i % 64];
+    __syncthreads();
   }
 }
 
@@ -1189,6 +1255,7 @@ static void msr_ensure(vgx_msbfs* m, uint64_t n) {
   MsRepair& R = m->rep;
   hipStream_t s = m->g->stream;
   if (n > R.n) {
+    R.setup = true;
     const uint64_t keep = R.n;
     const uint64_t cap = std::max<uint64_t>(n, R.n + R.n / 4);
     R.d.grow_copy(cap * 64 + 64, keep * 64, s);
@@ -1208,6 +1275,8 @@ static void msr_ensure(vgx_msbfs* m, uint64_t n) {
       R.pl[p].ensure(cap + 1);
       R.ppre[p].ensure(cap + 1);
     }
+    R.rl.ensure(cap + 1);
+    R.rpre.ensure(cap + 1);
     R.al.ensure(cap + 1);
     R.seed.ensure(cap + 1);
     R.seedpre.ensure(cap + 1);
@@ -1243,6 +1312,8 @@ static MsrArgs msr_args(vgx_msbfs* m) {
     a.pl[p] = R.pl[p].p;
     a.ppre[p] = R.ppre[p].p;
   }
+  a.rl = R.rl.p;
+  a.rpre = R.rpre.p;
   a.al = R.al.p;
   a.seed = R.seed.p;
   a.seedpre = R.seedpre.p;
@@ -1259,8 +1330,10 @@ static MsrArgs msr_args(vgx_msbfs* m) {
 }
 
 static void msr_upload(vgx_msbfs* m) {
-  const MsrArgs a = msr_args(m);
-  VGX_HIP(hipMemcpyAsync(m->rep.d_args, &a, sizeof(a), hipMemcpyHostToDevice, m->g->stream));
+  // every upload is followed by a stream synchronisation before the next
+  // one, so no copy from the pinned staging block is pending here
+  *m->rep.h_args = msr_args(m);
+  VGX_HIP(hipMemcpyAsync(m->rep.d_args, m->rep.h_args, sizeof(MsrArgs), hipMemcpyHostToDevice, m->g->stream));
 }
 
 static void msr_sync(vgx_msbfs* m) {
@@ -1275,7 +1348,8 @@ static void msr_materialize(vgx_msbfs* m) {
   if (m->in_d32) return;
   hipStream_t s = m->g->stream;
   if (m->n) {
-    k_ms_to_d32<<<msr_grid(m->n * kWave), kMsBlock, 0, s>>>(m->newbits.p, m->levels, m->n, m->rep.d.p);
+    k_ms_to_d32<<<ms_blocks((m->n + 63) / 64, 1, 4096), kMsBlock, 0, s>>>(m->newbits.p, m->levels, m->n,
+                                                                           m->rep.d.p);
     VGX_LAUNCH_CHECK();
   }
   m->in_d32 = true;
@@ -1284,8 +1358,94 @@ static void msr_materialize(vgx_msbfs* m) {
 constexpr uint32_t kMsrInvRounds = 2;   // invalidation rounds per host check
 constexpr uint32_t kMsrPropRounds = 4;  // propagation rounds per host check
 
-// One batch: graph update and repair of every root (or a full pass when
-// the policy or the repair's own limit says so).
+// The repair's four launch programs: 0 = state cleared, candidates and the
+// first invalidation rounds; 1 = more invalidation rounds; 2 = reset, seeds,
+// added arcs and the first propagation rounds; 3 = more propagation rounds.
+// Fixed grids, every size read on the device, arguments in device memory:
+// each is captured once into a hipGraph and replayed for every batch.
+static void msr_program(vgx_msbfs* m, int which, hipStream_t s) {
+  const MsrArgs* pa = m->rep.d_args;
+  if (which == 0) {
+    k_msr_zero_state<<<1, 64, 0, s>>>(m->rep.d_st);
+    VGX_LAUNCH_CHECK();
+    k_msr_candidates<<<kMsrBlocks, kMsBlock, 0, s>>>(pa);
+    VGX_LAUNCH_CHECK();
+  }
+  if (which == 0 || which == 1) {
+    for (uint32_t i = 0; i < kMsrInvRounds; ++i) {
+      const int par = static_cast<int>(i & 1);  // kMsrInvRounds is even
+      k_msr_take<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
+      VGX_LAUNCH_CHECK();
+      k_msr_support<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par, 0);
+      VGX_LAUNCH_CHECK();
+      k_msr_take_rest<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
+      VGX_LAUNCH_CHECK();
+      k_msr_support<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par, 1);
+      VGX_LAUNCH_CHECK();
+      k_msr_evaluate<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
+      VGX_LAUNCH_CHECK();
+      k_msr_emit<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
+      VGX_LAUNCH_CHECK();
+    }
+    return;
+  }
+  if (which == 2) {
+    k_msr_reset<<<kMsrBlocks, kMsBlock, 0, s>>>(pa);
+    VGX_LAUNCH_CHECK();
+    k_msr_seed<<<kMsrBlocks, kMsBlock, 0, s>>>(pa);
+    VGX_LAUNCH_CHECK();
+    k_msr_collect<<<kMsrBlocks, kMsBlock, 0, s>>>(pa);
+    VGX_LAUNCH_CHECK();
+    k_msr_relax_additions<<<kMsrBlocks, kMsBlock, 0, s>>>(pa);
+    VGX_LAUNCH_CHECK();
+  }
+  for (uint32_t i = 0; i < kMsrPropRounds; ++i) {
+    const int par = static_cast<int>(i & 1);  // kMsrPropRounds is even
+    k_msr_ptake<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
+    VGX_LAUNCH_CHECK();
+    k_msr_propagate<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
+    VGX_LAUNCH_CHECK();
+  }
+}
+
+static void msr_run_program(vgx_msbfs* m, int which) {
+  hipStream_t s = m->g->stream;
+  MsRepair& R = m->rep;
+  if (!m->use_graph) {
+    msr_program(m, which, s);
+    return;
+  }
+  if (!R.exec[which]) {
+    R.setup = true;
+    VGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    msr_program(m, which, s);
+    VGX_HIP(hipStreamEndCapture(s, &R.graph[which]));
+    VGX_HIP(hipGraphInstantiate(&R.exec[which], R.graph[which], nullptr, nullptr, 0));
+  }
+  VGX_HIP(hipGraphLaunch(R.exec[which], s));
+}
+
+static void msr_drop_graphs(MsRepair& R) {
+  for (int i = 0; i < 4; ++i) {
+    if (R.exec[i]) (void)hipGraphExecDestroy(R.exec[i]);
+    if (R.graph[i]) (void)hipGraphDestroy(R.graph[i]);
+    R.exec[i] = nullptr;
+    R.graph[i] = nullptr;
+  }
+}
+
+// Predicted repair time of a batch of size class cls: its own running mean,
+// else the nearest smaller class's scaled by the size ratio (pessimistic),
+// else the nearest larger class's (optimistic), else 0 (try it).
+static double msr_predict(const vgx_msbfs* m, int cls) {
+  if (m->repair_ms[cls] > 0.0) return m->repair_ms[cls];
+  for (int c = cls - 1; c >= 0; --c)
+    if (m->repair_ms[c] > 0.0) return m->repair_ms[c] * static_cast<double>(1ull << (cls - c));
+  for (int c = cls + 1; c < 48; ++c)
+    if (m->repair_ms[c] > 0.0) return m->repair_ms[c];
+  return 0.0;
+}
+
 static void msr_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
                       bool device, hipStream_t producer) {
   vgx_graph* g = m->g;
@@ -1298,11 +1458,15 @@ static void msr_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, co
   // the arm: a repair needs distances to repair; a root id the batch brings
   // into the graph needs a pass (its own distance becomes 0)
   bool repair = m->ran && m->repair_mode != 2 && m->dist_version == g->version;
+  const int cls = std::min(47, 63 - __builtin_clzll(k));
   if (repair && m->repair_mode == 0 && m->pass_ms > 0.0) {
-    const double arcs = static_cast<double>(g->adj[0].live) + 1.0;
-    const double per_op = m->repair_us_per_op > 0.0 ? m->repair_us_per_op : 1e3 * m->pass_ms / (arcs / 256.0);
-    repair = per_op * static_cast<double>(k) < 1e3 * m->pass_ms;
+    const double pred = msr_predict(m, cls);
+    repair = pred < m->pass_ms;
+    // the means drift: a class within twice the pass looks again now and then
+    if (!repair && pred < 2.0 * m->pass_ms && ++m->passes_in_class[cls] % 16 == 0) repair = true;
+    if (repair) m->passes_in_class[cls] = 0;
   }
+  m->rep.setup = false;
   if (repair) msr_materialize(m);
   if (device) graph_stage_device(g, src, dst, add, k, producer);
   else graph_stage(g, src, dst, add, k);
@@ -1312,14 +1476,16 @@ static void msr_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, co
   for (uint32_t r = 0; r < m->k; ++r)
     if (m->roots[r] >= n_cov && m->roots[r] < g->n) repair = false;
   m->batches += 1;
+  auto elapsed_ms = [&]() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  };
+  auto ema = [](double& x, double v) { x = x > 0.0 ? 0.75 * x + 0.25 * v : v; };
   if (!repair) {
     graph_apply_deletions(g);
     graph_apply_additions(g);
     graph_finish_batch(g);
-    const auto p0 = std::chrono::steady_clock::now();
     ms_run(m);
-    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p0).count();
-    m->pass_ms = m->pass_ms > 0.0 ? 0.75 * m->pass_ms + 0.25 * ms : ms;
+    ema(m->pass_ms, elapsed_ms());
     m->full_passes += 1;
     return;
   }
@@ -1332,26 +1498,9 @@ static void msr_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, co
   m->n = g->n;
   graph_apply_deletions(g);
   msr_upload(m);
-  const MsrArgs* pa = R.d_args;
-  k_msr_zero_state<<<1, 64, 0, s>>>(R.d_st);
-  VGX_LAUNCH_CHECK();
-  if (g->bs[0].nfinal) {
-    k_msr_candidates<<<msr_grid(g->bs[0].nfinal * kWave / kMsrWindow), kMsBlock, 0, s>>>(pa);
-    VGX_LAUNCH_CHECK();
-  }
   bool fallback = false;
   for (uint32_t r0 = 0;; r0 += kMsrInvRounds) {
-    for (uint32_t i = 0; i < kMsrInvRounds; ++i) {
-      const int par = static_cast<int>(i & 1);  // kMsrInvRounds is even
-      k_msr_take<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
-      VGX_LAUNCH_CHECK();
-      k_msr_support<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
-      VGX_LAUNCH_CHECK();
-      k_msr_evaluate<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
-      VGX_LAUNCH_CHECK();
-      k_msr_emit<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
-      VGX_LAUNCH_CHECK();
-    }
+    msr_run_program(m, r0 == 0 ? 0 : 1);
     msr_sync(m);
     if (R.h_st->fallback || R.h_st->apairs > static_cast<unsigned long long>(m->fallback * g->n * m->k + 1)) {
       fallback = true;
@@ -1369,38 +1518,21 @@ static void msr_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, co
     VGX_HIP(hipMemsetAsync(R.cand.p, 0, (g->n + 1) * 8, s));
     VGX_HIP(hipMemsetAsync(R.aff.p, 0, (g->n + 1) * 8, s));
     ms_run(m);
+    if (!m->rep.setup) ema(m->repair_ms[cls], elapsed_ms());  // what trying the repair cost at this size
     m->fallbacks += 1;
     m->full_passes += 1;
     m->last_fallback = true;
     return;
   }
   const uint64_t n_aff = R.h_st->aall;
-  if (n_aff) {
-    k_msr_reset<<<msr_grid(n_aff * kWave / kMsrWindow), kMsBlock, 0, s>>>(pa);
-    VGX_LAUNCH_CHECK();
-    k_msr_seed<<<kMsrBlocks, kMsBlock, 0, s>>>(pa);
-    VGX_LAUNCH_CHECK();
-    k_msr_collect<<<msr_grid(n_aff * kWave / kMsrWindow), kMsBlock, 0, s>>>(pa);
-    VGX_LAUNCH_CHECK();
-  }
-  if (g->bs[0].nfinal) {
-    k_msr_relax_additions<<<msr_grid(g->bs[0].nfinal * kWave / kMsrWindow), kMsBlock, 0, s>>>(pa);
-    VGX_LAUNCH_CHECK();
-  }
   for (uint32_t r0 = 0;; r0 += kMsrPropRounds) {
-    for (uint32_t i = 0; i < kMsrPropRounds; ++i) {
-      const int par = static_cast<int>(i & 1);  // kMsrPropRounds is even
-      k_msr_ptake<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
-      VGX_LAUNCH_CHECK();
-      k_msr_propagate<<<kMsrBlocks, kMsBlock, 0, s>>>(pa, par);
-      VGX_LAUNCH_CHECK();
-    }
+    msr_run_program(m, r0 == 0 ? 2 : 3);
     msr_sync(m);
     if (R.h_st->fq[0] == 0) break;
     if (r0 > 2 * g->n + 4) throw Error(VGX_DEVICE_ERROR, "multi-source propagation did not terminate");
   }
   if (n_aff) {
-    k_msr_clear<<<msr_grid(n_aff), kMsBlock, 0, s>>>(pa);
+    k_msr_clear<<<msr_grid(n_aff), kMsBlock, 0, s>>>(R.d_args);
     VGX_LAUNCH_CHECK();
   }
   m->last_lowered = R.h_st->lowered;
@@ -1409,9 +1541,11 @@ static void msr_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, co
   m->levels = 0;  // recomputed on demand from d
   m->last_repaired = true;
   m->repaired += 1;
-  const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
-  const double per_op = us / static_cast<double>(k);
-  m->repair_us_per_op = m->repair_us_per_op > 0.0 ? 0.75 * m->repair_us_per_op + 0.25 * per_op : per_op;
+  if (!m->rep.setup) {  // one-time allocations and captures are not the arm's cost
+    const double ms = elapsed_ms();
+    ema(m->repair_ms[cls], ms);
+    ema(m->repair_us_per_op, 1e3 * ms / static_cast<double>(k));
+  }
 }
 
 // Reached count per root and the level count (deepest distance + 1) of
@@ -1471,6 +1605,7 @@ vgx_status vgx_msbfs_create(vgx_graph* g, const uint32_t* roots, uint32_t k, vgx
     VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&m->h_ctl), sizeof(MsCtl), hipHostMallocDefault));
     m->rep.d_args = static_cast<MsrArgs*>(dev_alloc(sizeof(MsrArgs)));
     m->rep.d_st = static_cast<MsrState*>(dev_alloc(sizeof(MsrState)));
+    VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&m->rep.h_args), sizeof(MsrArgs), hipHostMallocDefault));
     VGX_HIP(hipHostMalloc(reinterpret_cast<void**>(&m->rep.h_st), sizeof(MsrState), hipHostMallocDefault));
   } catch (...) {
     vgx_msbfs_destroy(m);
@@ -1488,7 +1623,9 @@ vgx_status vgx_msbfs_destroy(vgx_msbfs* m) {
   if (m->d_args) dev_free(m->d_args);
   if (m->d_ctl) dev_free(m->d_ctl);
   if (m->h_ctl) (void)hipHostFree(m->h_ctl);
+  msr_drop_graphs(m->rep);
   if (m->rep.d_args) dev_free(m->rep.d_args);
+  if (m->rep.h_args) (void)hipHostFree(m->rep.h_args);
   if (m->rep.d_st) dev_free(m->rep.d_st);
   if (m->rep.h_st) (void)hipHostFree(m->rep.h_st);
   delete m;
@@ -1499,7 +1636,10 @@ vgx_status vgx_msbfs_run(vgx_msbfs* m) {
   VGX_API_BEGIN
   if (!m) throw Error(VGX_INVALID_ARGUMENT, "null msbfs");
   VGX_HIP(hipSetDevice(m->g->device));
+  const auto t0 = std::chrono::steady_clock::now();
   ms_run(m);
+  if (m->pass_ms <= 0.0)  // the repair policy's first estimate of the full-pass arm
+    m->pass_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   VGX_API_END
 }
 
@@ -1507,7 +1647,10 @@ vgx_status vgx_msbfs_set_graph(vgx_msbfs* m, int on) {
   VGX_API_BEGIN
   if (!m) throw Error(VGX_INVALID_ARGUMENT, "null msbfs");
   m->use_graph = on != 0;
-  if (!m->use_graph) ms_drop_graph(m);
+  if (!m->use_graph) {
+    ms_drop_graph(m);
+    msr_drop_graphs(m->rep);
+  }
   VGX_API_END
 }
 
