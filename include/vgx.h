@@ -414,9 +414,10 @@ vgx_status vgx_msbfs_reached(vgx_msbfs* m, uint64_t* reached_per_root, uint32_t 
 vgx_status vgx_msbfs_level_count(vgx_msbfs* m, uint32_t* levels);
 /* Tuning (no reference counterpart): workgroups of the short-row kernels
  * (0 = auto) and the direction switch (push while the frontier's out-arcs
- * are fewer than arcs / alpha).  The autotuner times `reps` runs per
- * candidate, alpha over {2, 4, 8, 16, 32} then workgroups over {auto, 512,
- * 1024, 2048, 4096, 8192}, and keeps the fastest. */
+ * are fewer than arcs / alpha); the other parameters keep their values (see
+ * vgx_msbfs_set_params).  The autotuner times `reps` runs per candidate of
+ * every parameter of vgx_msbfs_tuning (see there) and reports workgroups and
+ * alpha; vgx_msbfs_get_params reads all of them. */
 vgx_status vgx_msbfs_set_tuning(vgx_msbfs* m, uint32_t blocks, double alpha);
 /* Run the pass as a captured hipGraph (no reference counterpart): the
  * launch program of the first 16 levels is captured once and replayed by
@@ -429,6 +430,23 @@ vgx_status vgx_msbfs_set_graph(vgx_msbfs* m, int on);
 vgx_status vgx_msbfs_stats(vgx_msbfs* m, uint32_t* push_levels, uint64_t* graph_replays);
 vgx_status vgx_msbfs_autotune(vgx_msbfs* m, uint32_t reps, uint32_t* best_blocks, double* best_alpha,
                               double* best_ms);
+/* All launch parameters of a pass (no reference counterpart): the
+ * short-row kernels' workgroups (0 = auto); alpha (push while the frontier's
+ * out-arcs are fewer than arcs / alpha); beta (a pulling pass turns back to
+ * push only once its new (vertex, root) pairs * beta < vertices * roots; 0 =
+ * off); the degree buckets -- rows of up to short_row arcs (1..1024) take one
+ * lane, longer rows chunks of `chunk` arcs (64..2^20), one wave each.  None
+ * changes a result.  vgx_msbfs_autotune sweeps, one at a time, alpha over
+ * {2, 4, 8, 16, 32}, beta over {0, 4, 16, 64}, short_row over {8, 16, 32,
+ * 64, 128}, chunk over {256, ..., 4096} and the workgroups, and keeps the
+ * fastest of each. */
+typedef struct {
+  uint32_t blocks;
+  double alpha, beta;
+  uint32_t short_row, chunk;
+} vgx_msbfs_tuning;
+vgx_status vgx_msbfs_set_params(vgx_msbfs* m, const vgx_msbfs_tuning* t);
+vgx_status vgx_msbfs_get_params(vgx_msbfs* m, vgx_msbfs_tuning* t);
 
 /* IncrementalBFS::apply (incremental/bfs.hpp:29-98) for every root at once:
  * applies the batch to the graph (DynamicGraph::apply semantics) and makes

@@ -146,7 +146,37 @@ def test_autotune_keeps_results(vx, orc):
     ms = vx.MultiSourceBFS(g, roots)
     best = ms.autotune(1)
     assert best["alpha"] > 0 and best["ms"] > 0
+    assert best["short_row"] in (8, 16, 32, 64, 128) and best["chunk"] in (256, 512, 1024, 2048, 4096)
+    assert best["beta"] in (0.0, 4.0, 16.0, 64.0)
+    assert ms.params()["short_row"] == best["short_row"]
     check_all_roots(ms, g, mg, orc, roots)
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_degree_buckets_and_beta_keep_results(vx, orc, directed):
+    """Every row in one lane (short_row 1024), no row in a lane (short_row
+    1), the smallest and a large chunk, the beta rule on and off, eager and
+    captured: the oracle's distances each time."""
+    s, d = vx.rmat_edges(13, 16, 8)
+    g = vx.Graph(1 << 13, directed)
+    g.bulk_load_arrays(s, d)
+    mg = orc.ModelGraph(1 << 13, directed)
+    mg.bulk_load_edges(np.stack([s, d], 1))
+    roots = np.random.default_rng(5).integers(0, 1 << 13, size=64).tolist()
+    ms = vx.MultiSourceBFS(g, roots)
+    for graph_mode in (False, True):
+        ms.set_graph(graph_mode)
+        for short_row, chunk, alpha, beta in ((1, 64, 4.0, 0.0), (1024, 4096, 4.0, 0.0), (8, 256, 2.0, 64.0),
+                                              (128, 1 << 20, 32.0, 4.0), (32, 1024, 1e9, 16.0)):
+            ms.set_params(0, alpha, beta, short_row, chunk)
+            assert ms.params() == {"blocks": 0, "alpha": alpha, "beta": beta, "short_row": short_row, "chunk": chunk}
+            check_all_roots(ms, g, mg, orc, roots)
+    with pytest.raises(ValueError):
+        ms.set_params(0, 4.0, 0.0, 0, 1024)
+    with pytest.raises(ValueError):
+        ms.set_params(0, 4.0, 0.0, 32, 32)
+    with pytest.raises(ValueError):
+        ms.set_params(0, 4.0, -1.0, 32, 1024)
 
 
 def test_graph_replay_across_updates(vx, orc):

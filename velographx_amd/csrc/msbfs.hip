@@ -47,6 +47,8 @@ extern thread_local std::string t_last_error;
 
 constexpr int kMsBlock = 256;
 constexpr uint32_t kMsWaves = kMsBlock / kWave;
+// Defaults of the tunable degree buckets: rows of up to kMsShortRow arcs are
+// handled by one lane, longer rows in chunks of kMsChunk arcs, one wave each.
 constexpr uint32_t kMsShortRow = 32;
 constexpr uint32_t kMsChunk = 1024;
 constexpr uint32_t kMsStripes = 32;
@@ -97,6 +99,9 @@ struct MsArgs {
   unsigned long long all;
   double arcs;
   double alpha;
+  double beta;          // a pulling pass turns back to push only when new pairs * beta < n * k (0: off)
+  uint32_t short_row;   // rows of at most this many arcs: one lane
+  uint32_t chunk;       // longer rows: chunks of this many arcs, one wave each
   MsCtl* ctl;
 };
 
@@ -182,8 +187,8 @@ struct MsRepair {
   hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
-// Chunks of the rows longer than kMsShortRow in one direction's row table:
-// row v owns chunks [row_chunk[v], row_chunk[v+1]).
+// Chunks of the rows longer than the short-row bound in one direction's row
+// table: row v owns chunks [row_chunk[v], row_chunk[v+1]).
 struct MsPlan {
   DevBuf<uint32_t> row_chunk;
   DevBuf<uint32_t> chunk_row;
@@ -191,6 +196,7 @@ struct MsPlan {
   uint64_t nchunks = 0;
   uint64_t version = ~0ull;
   uint64_t n = 0;
+  uint32_t short_row = 0, chunk = 0;
 };
 
 }  // namespace vgx
@@ -212,6 +218,9 @@ struct vgx_msbfs {
   std::vector<uint64_t> reached;  // per root
   uint32_t blocks = 0;            // short-row kernel workgroups, 0 = auto
   double alpha = 4.0;             // push while frontier arcs < arcs / alpha
+  double beta = 0.0;              // pull -> push also needs new pairs * beta < n * k (0: off)
+  uint32_t short_row = vgx::kMsShortRow;
+  uint32_t chunk = vgx::kMsChunk;
   vgx::MsPlan plan[2];            // [0] out-rows (push), [1] in-rows (pull)
   vgx::DevBuf<uint8_t> scan;
   uint32_t push_levels = 0;
@@ -278,10 +287,11 @@ __global__ void k_ms_start(const MsArgs* __restrict__ pa, int fresh, uint32_t fi
   }
 }
 
-__global__ void k_ms_chunk_count(const uint2* __restrict__ row, uint32_t n, uint32_t* __restrict__ cnt) {
+__global__ void k_ms_chunk_count(const uint2* __restrict__ row, uint32_t n, uint32_t short_row, uint32_t chunk,
+                                 uint32_t* __restrict__ cnt) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     const uint32_t len = row[v].y;
-    cnt[v] = len > kMsShortRow ? (len + kMsChunk - 1) / kMsChunk : 0u;
+    cnt[v] = len > short_row ? (len + chunk - 1) / chunk : 0u;
   }
 }
 
@@ -306,7 +316,7 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_push_short(const MsArgs* __rest
     const unsigned long long f = frontier[u];
     if (!f) continue;
     const uint2 rb = a.out_row[u];
-    if (rb.y > kMsShortRow) continue;
+    if (rb.y > a.short_row) continue;
     for (uint32_t i = rb.x; i < rb.x + rb.y; ++i) {
       const uint32_t v = a.out_arena[i];
       if (v & kTomb) continue;
@@ -328,8 +338,8 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_push_long(const MsArgs* __restr
     const unsigned long long f = frontier[u];
     if (!f) continue;
     const uint2 rb = a.out_row[u];
-    const uint32_t b = rb.x + (c - a.out_rc[u]) * kMsChunk;
-    const uint32_t e = min(b + kMsChunk, rb.x + rb.y);
+    const uint32_t b = rb.x + (c - a.out_rc[u]) * a.chunk;
+    const uint32_t e = min(b + a.chunk, rb.x + rb.y);
     for (uint32_t i = b + lane_id(); i < e; i += kWave) {
       const uint32_t v = a.out_arena[i];
       if (v & kTomb) continue;
@@ -358,7 +368,7 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_pull_short(const MsArgs* __rest
     const unsigned long long s = a.seen[v];
     if (s == all) continue;
     const uint2 rb = a.in_row[v];
-    if (rb.y > kMsShortRow) continue;
+    if (rb.y > a.short_row) continue;
     const uint32_t* adj = a.in_arena;
     const uint32_t b = rb.x, e = rb.x + rb.y;
     unsigned long long acc = 0;
@@ -393,8 +403,8 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_pull_long(const MsArgs* __restr
     const unsigned long long s = a.seen[v];
     if (s == all) continue;
     const uint2 rb = a.in_row[v];
-    const uint32_t b = rb.x + (c - a.in_rc[v]) * kMsChunk;
-    const uint32_t e = min(b + kMsChunk, rb.x + rb.y);
+    const uint32_t b = rb.x + (c - a.in_rc[v]) * a.chunk;
+    const uint32_t e = min(b + a.chunk, rb.x + rb.y);
     unsigned long long acc = 0;
     for (uint32_t i0 = b; i0 < e; i0 += 4 * kWave) {
 #pragma unroll
@@ -463,7 +473,11 @@ __global__ void k_ms_decide(const MsArgs* __restrict__ pa, uint32_t l) {
       c->levels = a.base + l + 1;
     } else {
       c->levels = a.base + l + 2;
-      c->dir[l + 1] = static_cast<double>(arcs) * a.alpha < a.arcs ? 0u : 1u;
+      bool push = static_cast<double>(arcs) * a.alpha < a.arcs;
+      // leaving pull also needs the frontier small next to all (vertex, root) pairs
+      if (push && c->dir[l] == 1 && a.beta > 0.0)
+        push = static_cast<double>(pairs) * a.beta < static_cast<double>(a.n) * a.k;
+      c->dir[l + 1] = push ? 0u : 1u;
     }
   }
 }
@@ -543,7 +557,9 @@ static void ms_ensure_levels(vgx_msbfs* m, uint64_t need) {
 static const MsPlan& ms_plan(vgx_msbfs* m, int dir, const Adjacency& adj, uint64_t version, uint64_t nv,
                              hipStream_t s) {
   MsPlan& pl = m->plan[dir];
-  if (pl.version == version && pl.n == nv && pl.row_chunk.p) return pl;
+  if (pl.version == version && pl.n == nv && pl.row_chunk.p && pl.short_row == m->short_row &&
+      pl.chunk == m->chunk)
+    return pl;
   const uint32_t n = static_cast<uint32_t>(nv);
   pl.row_chunk.ensure(nv + 2);
   VGX_HIP(hipMemsetAsync(pl.row_chunk.p, 0, sizeof(uint32_t), s));
@@ -551,7 +567,7 @@ static const MsPlan& ms_plan(vgx_msbfs* m, int dir, const Adjacency& adj, uint64
   if (n) {
     DevBuf<uint32_t>& cnt = pl.cnt;
     cnt.ensure(nv + 1);
-    k_ms_chunk_count<<<ms_blocks(n), kMsBlock, 0, s>>>(adj.row.p, n, cnt.p);
+    k_ms_chunk_count<<<ms_blocks(n), kMsBlock, 0, s>>>(adj.row.p, n, m->short_row, m->chunk, cnt.p);
     VGX_LAUNCH_CHECK();
     size_t bytes = 0;
     VGX_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, cnt.p, pl.row_chunk.p + 1, static_cast<int>(n), s));
@@ -566,6 +582,8 @@ static const MsPlan& ms_plan(vgx_msbfs* m, int dir, const Adjacency& adj, uint64
   pl.nchunks = total;
   pl.version = version;
   pl.n = nv;
+  pl.short_row = m->short_row;
+  pl.chunk = m->chunk;
   return pl;
 }
 
@@ -667,6 +685,9 @@ static void ms_run(vgx_msbfs* m) {
     a.all = m->k == 64 ? ~0ull : ((1ull << m->k) - 1);
     a.arcs = static_cast<double>(out.live);
     a.alpha = m->alpha;
+    a.beta = m->beta;
+    a.short_row = m->short_row;
+    a.chunk = m->chunk;
     a.ctl = m->d_ctl;
     VGX_HIP(hipMemcpyAsync(m->d_args, &a, sizeof(a), hipMemcpyHostToDevice, s));
     const bool fresh = base == 0;
@@ -1742,6 +1763,33 @@ vgx_status vgx_msbfs_set_tuning(vgx_msbfs* m, uint32_t blocks, double alpha) {
   VGX_API_END
 }
 
+vgx_status vgx_msbfs_set_params(vgx_msbfs* m, const vgx_msbfs_tuning* t) {
+  VGX_API_BEGIN
+  if (!m || !t) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  if (t->blocks > 65535) throw Error(VGX_INVALID_ARGUMENT, "too many workgroups");
+  if (!(t->alpha > 0.0)) throw Error(VGX_INVALID_ARGUMENT, "alpha must be positive");
+  if (!(t->beta >= 0.0)) throw Error(VGX_INVALID_ARGUMENT, "beta must be zero or positive");
+  if (t->short_row < 1 || t->short_row > 1024) throw Error(VGX_INVALID_ARGUMENT, "short_row must be 1..1024");
+  if (t->chunk < 64 || t->chunk > (1u << 20)) throw Error(VGX_INVALID_ARGUMENT, "chunk must be 64..2^20");
+  m->blocks = t->blocks;
+  m->alpha = t->alpha;
+  m->beta = t->beta;
+  m->short_row = t->short_row;
+  m->chunk = t->chunk;
+  VGX_API_END
+}
+
+vgx_status vgx_msbfs_get_params(vgx_msbfs* m, vgx_msbfs_tuning* t) {
+  VGX_API_BEGIN
+  if (!m || !t) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  t->blocks = m->blocks;
+  t->alpha = m->alpha;
+  t->beta = m->beta;
+  t->short_row = m->short_row;
+  t->chunk = m->chunk;
+  VGX_API_END
+}
+
 vgx_status vgx_msbfs_autotune(vgx_msbfs* m, uint32_t reps, uint32_t* best_blocks, double* best_alpha,
                               double* best_ms) {
   VGX_API_BEGIN
@@ -1755,29 +1803,27 @@ vgx_status vgx_msbfs_autotune(vgx_msbfs* m, uint32_t reps, uint32_t* best_blocks
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
   };
   double best_t = time_run();
-  uint32_t bb = m->blocks;
-  double ba = m->alpha;
-  for (double a : {2.0, 4.0, 8.0, 16.0, 32.0}) {
-    m->alpha = a;
-    const double t = time_run();
-    if (t < best_t) {
-      best_t = t;
-      ba = a;
+  // one parameter at a time, each over its candidates, keeping the fastest
+  auto sweep = [&](auto& field, std::initializer_list<std::decay_t<decltype(field)>> cands) {
+    auto best = field;
+    for (auto c : cands) {
+      field = c;
+      const double t = time_run();
+      if (t < best_t) {
+        best_t = t;
+        best = c;
+      }
     }
-  }
-  m->alpha = ba;
-  for (uint32_t b : {0u, 512u, 1024u, 2048u, 4096u, 8192u}) {
-    m->blocks = b;
-    const double t = time_run();
-    if (t < best_t) {
-      best_t = t;
-      bb = b;
-    }
-  }
-  m->blocks = bb;
+    field = best;
+  };
+  sweep(m->alpha, {2.0, 4.0, 8.0, 16.0, 32.0});
+  sweep(m->beta, {0.0, 4.0, 16.0, 64.0});
+  sweep(m->short_row, {8u, 16u, 32u, 64u, 128u});
+  sweep(m->chunk, {256u, 512u, 1024u, 2048u, 4096u});
+  sweep(m->blocks, {0u, 512u, 1024u, 2048u, 4096u, 8192u});
   ms_run(m);
-  if (best_blocks) *best_blocks = bb;
-  if (best_alpha) *best_alpha = ba;
+  if (best_blocks) *best_blocks = m->blocks;
+  if (best_alpha) *best_alpha = m->alpha;
   if (best_ms) *best_ms = best_t;
   VGX_API_END
 }

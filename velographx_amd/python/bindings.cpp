@@ -943,13 +943,39 @@ PYBIND11_MODULE(_velographx, m) {
               py::gil_scoped_release nogil;
               velographx::vgx_check(vgx_msbfs_autotune(m.handle(), reps, &blocks, &alpha, &ms));
             }
+            vgx_msbfs_tuning t{};
+            velographx::vgx_check(vgx_msbfs_get_params(m.handle(), &t));
             py::dict d;
             d["blocks"] = blocks;
             d["alpha"] = alpha;
+            d["beta"] = t.beta;
+            d["short_row"] = t.short_row;
+            d["chunk"] = t.chunk;
             d["ms"] = ms;
             return d;
           },
           py::arg("reps") = 3)
+      .def(
+          "set_params",
+          [](velographx::MultiSourceBFS& m, std::uint32_t blocks, double alpha, double beta, std::uint32_t short_row,
+             std::uint32_t chunk) {
+            const vgx_msbfs_tuning t{blocks, alpha, beta, short_row, chunk};
+            velographx::vgx_check(vgx_msbfs_set_params(m.handle(), &t));
+          },
+          py::arg("blocks") = 0, py::arg("alpha") = 4.0, py::arg("beta") = 0.0, py::arg("short_row") = 32,
+          py::arg("chunk") = 1024)
+      .def("params",
+           [](const velographx::MultiSourceBFS& m) {
+             vgx_msbfs_tuning t{};
+             velographx::vgx_check(vgx_msbfs_get_params(m.handle(), &t));
+             py::dict d;
+             d["blocks"] = t.blocks;
+             d["alpha"] = t.alpha;
+             d["beta"] = t.beta;
+             d["short_row"] = t.short_row;
+             d["chunk"] = t.chunk;
+             return d;
+           })
       .def("apply",
            [](velographx::MultiSourceBFS& m, const UpdateBatch& batch) {
              py::gil_scoped_release nogil;
