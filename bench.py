@@ -47,7 +47,9 @@ Secondary legs, each in its own object of the same JSON line:
                repairs/s, next to the always-repair and always-recompute arms
                and the reference engine's IncrementalBFS::apply timed on a
                bounded sample of the same batches;
-* contrast  -- Erdos-Renyi, 4,194,304 vertices, mean degree 16;
+* contrast  -- Erdos-Renyi, 4,194,304 vertices, mean degree 16: the same
+               16-root BFS and the same 100-batch repair stream (adaptive
+               selector, consolidation every 20; always-recompute beside);
 * batching  -- R-MAT scale 20: 64 roots per hipGraph-captured multi-source
                pass (autotuned), then 256 update batches of log-uniform size
                10 .. 100,000, answered by two arms on two copies: apply and
@@ -921,7 +923,12 @@ def leg_repairs(vx, wl, g, s, d, n, root, args):
     return out, batches
 
 
-def leg_contrast(vx, wl, steps):
+def leg_contrast(vx, wl, steps, args):
+    """Erdos-Renyi contrast at the plain workload's size: the BFS from 16
+    roots of its largest component, then the same update stream as the
+    repairs leg (100 batches of 10,000 ops, 80/20, consolidation every 20)
+    answered through the adaptive selector, with the always-recompute arm
+    on a copy for comparison."""
     n = 1 << 22
     t0 = time.time()
     u, v = wl.erdos_renyi(n, 8 * n, seed=1)
@@ -940,8 +947,21 @@ def leg_contrast(vx, wl, steps):
            "gpu_ms_per_bfs": kernel_ms, "bytes_per_bfs": bytes_bfs,
            "roofline_frac": bytes_bfs / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
            "levels": st["levels"], "bottom_up_levels": st["bottom_up_levels"], "steps": st["steps"]}
+    del t_bfs
+    if args.batches:
+        eu, ev = wl.canonical_edges(u, v, False)
+        stream = wl.BatchStream(eu, ev, n, False, seed=1)
+        batches = [stream.next(args.batch_size, 0.8) for _ in range(args.batches)]
+        rep = _run_repairs(vx, g, int(roots[0]), batches, args, "adaptive")
+        rep["policy"] = "adaptive selector (publication-preflight-v1), consolidation every %d" % args.consolidate_every
+        g2 = vx.Graph(n, False)
+        g2.bulk_load_arrays(u, v)
+        r = _run_repairs(vx, g2, int(roots[0]), batches, args, "full")
+        rep["always_full"] = {k: r[k] for k in ("repairs_per_s", "mean_batch_ms", "full_recompute_batches", "exact")}
+        del g2
+        out["repairs"] = rep
     log(f"contrast: {out}")
-    del t_bfs, g
+    del g
     return out
 
 
@@ -1299,7 +1319,7 @@ def main():
                 result["repairs"]["cpu_reference"] = cr
                 log(f"cpu repairs: {cr}")
     if not args.no_contrast:
-        result["contrast"] = leg_contrast(vx, wl, args.steps)
+        result["contrast"] = leg_contrast(vx, wl, args.steps, args)
     if not args.no_batching:
         result["batching"] = leg_batching(vx, wl, args)
     if not args.no_kcore:

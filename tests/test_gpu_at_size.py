@@ -6,7 +6,9 @@ CPU oracle on the host (nothing here compares the GPU with itself):
   inserts of non-edges, 20% deletions) answered through the adaptive
   selector with consolidation every 20 batches, checked every 10 batches
   against the oracle's BFS of the oracle's own graph after the same ops;
-* Erdos-Renyi, 4,194,304 vertices, mean degree 16: 16 roots;
+* Erdos-Renyi, 4,194,304 vertices, mean degree 16: 16 roots, then 20
+  update batches of 10,000 ops through the adaptive selector, checked every
+  10 batches;
 * undirected R-MAT scale 20: all 64 roots of a multi-source pass (captured
   launch program) after each of 4 uneven update batches, and all 64 roots
   of the multi-source incremental repair after each of 6 batches of 10 to
@@ -153,6 +155,24 @@ def test_config2_erdos_renyi_4m_bfs(vx, wl, orc):
         walker.set_source(int(r))
         assert np.array_equal(walker.distances_numpy(), want), int(r)
     log(f"ER: 16 roots equal the oracle (component {lcc.size})")
+    # the repairs leg's update stream on the same graph, through the selector
+    eu, ev = model.undirected_edges()
+    stream = wl.BatchStream(eu, ev, n, False, seed=1)
+    root = int(roots[0])
+    inc = vx.IncrementalBFS(g, root, 2.0)
+    sel = vx.AdaptiveSelector(inc)
+    pending = []
+    for i in range(20):
+        bs, bd, ba = stream.next(10000, 0.8)
+        sel.apply_arrays(bs, bd, ba)
+        pending.append((bs, bd, ba))
+        if (i + 1) % 10 == 0:
+            model.apply(*(np.concatenate(x) for x in zip(*pending)))
+            pending = []
+            assert g.edge_count_directed == model.edge_count_directed(), i
+            want = oracle_bfs_many(orc, model, [root])[0]
+            assert np.array_equal(inc.distances_numpy(), want), i
+    log("ER: 20 update batches of 10,000 ops through the selector equal the oracle")
 
 
 def test_config3_s20_64_roots_after_uneven_batches(vx, wl, orc):
