@@ -176,65 +176,23 @@ __device__ __forceinline__ uint32_t row_lower_bound(const uint32_t* arena, uint2
   return lo;
 }
 
-// Both compactions of a staged batch in one workgroup (batches of up to
-// kSelectSmall arc ops; larger ones use the library's device-wide select):
-// positions i with f1[i] != 0 go to out1, those with f2[i] != 0 to out2, in
-// order.  Each thread takes a contiguous run of whole 16-byte words of both
-// flag arrays (16-byte aligned, padded past M), counts, and after a block
-// scan writes its positions.
-constexpr uint32_t kSelectThreads = 1024;
+// Batches of up to kSelectSmall arc ops are compacted by k_resolve's
+// per-workgroup counts and k_select_write; larger ones use the library's
+// device-wide select.
 constexpr uint64_t kSelectSmall = 1u << 17;
-__global__ __launch_bounds__(kSelectThreads) void k_select_two(const uint8_t* __restrict__ f1,
-                                                               const uint8_t* __restrict__ f2, uint32_t M,
-                                                               uint32_t* __restrict__ out1, unsigned long long* n1,
-                                                               uint32_t* __restrict__ out2, unsigned long long* n2) {
-  constexpr uint32_t kW = kSelectThreads / kWave;
-  __shared__ unsigned long long s_tot[kW];
-  const uint32_t per = ((M + kSelectThreads - 1) / kSelectThreads + 15) & ~15u;
-  const uint32_t i0 = min(M, threadIdx.x * per), i1 = min(M, i0 + per);
-  auto bytes = [](const uint4& q, uint32_t b) {
-    const uint32_t w = b < 4 ? q.x : b < 8 ? q.y : b < 12 ? q.z : q.w;
-    return (w >> ((b & 3) * 8)) & 0xFFu;
-  };
-  unsigned long long c = 0;
-  for (uint32_t i = i0; i < i1; i += 16) {
-    const uint4 a1 = *reinterpret_cast<const uint4*>(f1 + i);
-    const uint4 a2 = *reinterpret_cast<const uint4*>(f2 + i);
-#pragma unroll
-    for (uint32_t b = 0; b < 16; ++b)
-      if (i + b < i1) c += (static_cast<unsigned long long>(bytes(a1, b) != 0) << 32) + (bytes(a2, b) != 0);
-  }
-  const unsigned long long inc = wave_inclusive_scan<unsigned long long>(c);
-  const uint32_t wv = threadIdx.x / kWave;
-  if (lane_id() == kWave - 1) s_tot[wv] = inc;
-  __syncthreads();
-  unsigned long long base = inc - c, total = 0;
-  for (uint32_t w = 0; w < kW; ++w) {
-    if (w < wv) base += s_tot[w];
-    total += s_tot[w];
-  }
-  uint32_t o1 = static_cast<uint32_t>(base >> 32), o2 = static_cast<uint32_t>(base);
-  for (uint32_t i = i0; i < i1; i += 16) {
-    const uint4 a1 = *reinterpret_cast<const uint4*>(f1 + i);
-    const uint4 a2 = *reinterpret_cast<const uint4*>(f2 + i);
-#pragma unroll
-    for (uint32_t b = 0; b < 16; ++b) {
-      if (i + b >= i1) break;
-      if (bytes(a1, b)) out1[o1++] = i + b;
-      if (bytes(a2, b)) out2[o2++] = i + b;
-    }
-  }
-  if (threadIdx.x == 0) {
-    *n1 = total >> 32;
-    *n2 = total & 0xFFFFFFFFull;
-  }
-}
 
 // Decide the action of every final arc op (the last op on its arc wins).
-__global__ void k_resolve(const unsigned long long* __restrict__ skeys, const uint32_t* __restrict__ svals,
-                          uint64_t M, const uint8_t* __restrict__ add, const uint2* __restrict__ row,
-                          const uint32_t* __restrict__ arena, uint64_t n, uint8_t* __restrict__ act,
-                          uint32_t* __restrict__ pos, uint8_t* __restrict__ fflag, uint8_t* __restrict__ iflag) {
+// With bcount set the grid covers M exactly (one arc per thread) and each
+// workgroup also leaves its counts of final arcs and inserts in
+// bcount[blockIdx.x] (final << 32 | inserts) for k_select_write.
+__global__ __launch_bounds__(kBlock) void k_resolve(const unsigned long long* __restrict__ skeys,
+                                                    const uint32_t* __restrict__ svals, uint64_t M,
+                                                    const uint8_t* __restrict__ add, const uint2* __restrict__ row,
+                                                    const uint32_t* __restrict__ arena, uint64_t n,
+                                                    uint8_t* __restrict__ act, uint32_t* __restrict__ pos,
+                                                    uint8_t* __restrict__ fflag, uint8_t* __restrict__ iflag,
+                                                    unsigned long long* __restrict__ bcount) {
+  unsigned long long cnt = 0;
   for (uint64_t p = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p < M;
        p += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     unsigned long long key = skeys[p];
@@ -258,6 +216,55 @@ __global__ void k_resolve(const unsigned long long* __restrict__ skeys, const ui
     }
     fflag[p] = f;
     iflag[p] = ins;
+    cnt += (static_cast<unsigned long long>(f) << 32) + ins;
+  }
+  if (bcount) {
+    __shared__ unsigned long long s_cnt[kBlock / kWave];
+    cnt = wave_sum(cnt);
+    if (lane_id() == 0) s_cnt[threadIdx.x / kWave] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t = 0;
+      for (uint32_t w = 0; w < kBlock / kWave; ++w) t += s_cnt[w];
+      bcount[blockIdx.x] = t;
+    }
+  }
+}
+
+// Order-keeping compaction of both flag arrays over the tiling of k_resolve
+// with bcount: workgroup b sums the counts of the workgroups before it, scans
+// its own flags and writes the positions of final arcs to out1 and of
+// inserts to out2; the last workgroup writes the totals.  Many workgroups
+// share the scattered stores that one workgroup would issue alone.
+__global__ __launch_bounds__(kBlock) void k_select_write(const uint8_t* __restrict__ f1, const uint8_t* __restrict__ f2,
+                                                         uint64_t M, const unsigned long long* __restrict__ bcount,
+                                                         uint32_t* __restrict__ out1, unsigned long long* n1,
+                                                         uint32_t* __restrict__ out2, unsigned long long* n2) {
+  __shared__ unsigned long long s_w[kBlock / kWave];
+  __shared__ unsigned long long s_base;
+  const uint32_t wv = threadIdx.x / kWave;
+  if (wv == 0) {
+    unsigned long long b = 0;
+    for (uint32_t i = lane_id(); i < blockIdx.x; i += kWave) b += bcount[i];
+    b = wave_sum(b);
+    if (lane_id() == 0) s_base = b;
+  }
+  const uint64_t p = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x;
+  const bool in = p < M;
+  const unsigned long long c = in ? (static_cast<unsigned long long>(f1[p] != 0) << 32) + (f2[p] != 0) : 0ull;
+  const unsigned long long inc = wave_inclusive_scan<unsigned long long>(c);
+  if (lane_id() == kWave - 1) s_w[wv] = inc;
+  __syncthreads();
+  unsigned long long base = s_base + inc - c;
+  for (uint32_t w = 0; w < wv; ++w) base += s_w[w];
+  if (in) {
+    if (c >> 32) out1[base >> 32] = static_cast<uint32_t>(p);
+    if (c & 0xFFFFFFFFull) out2[base & 0xFFFFFFFFull] = static_cast<uint32_t>(p);
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) {
+    const unsigned long long total = base + c;
+    *n1 = total >> 32;
+    *n2 = total & 0xFFFFFFFFull;
   }
 }
 
@@ -1177,14 +1184,21 @@ static void stage_sorted(vgx_graph* g, uint64_t k) {
     });
     uint8_t* fflag = b.flags.p;
     uint8_t* iflag = b.flags.p + fstride;
-    k_resolve<<<grid_for(M), kBlock, 0, s>>>(b.skeys.p, b.svals.p, M, b0.add.p, a.row.p, a.arena.p, g->n,
-                                             b.act.p, b.pos.p, fflag, iflag);
-    VGX_LAUNCH_CHECK();
     if (M <= kSelectSmall) {
-      k_select_two<<<1, kSelectThreads, 0, s>>>(fflag, iflag, static_cast<uint32_t>(M), b.fin.p,
-                                                &g->d_ctr->nfinal[d], b.ins.p, &g->d_ctr->nins[d]);
+      // one arc per thread: the workgroups leave their counts for the
+      // order-keeping write (a zero-arc batch still writes zero totals)
+      const unsigned nb = static_cast<unsigned>(std::max<uint64_t>(1, (M + kBlock - 1) / kBlock));
+      b.bcount.ensure(nb + 1);
+      k_resolve<<<nb, kBlock, 0, s>>>(b.skeys.p, b.svals.p, M, b0.add.p, a.row.p, a.arena.p, g->n, b.act.p,
+                                       b.pos.p, fflag, iflag, b.bcount.p);
+      VGX_LAUNCH_CHECK();
+      k_select_write<<<nb, kBlock, 0, s>>>(fflag, iflag, M, b.bcount.p, b.fin.p, &g->d_ctr->nfinal[d], b.ins.p,
+                                           &g->d_ctr->nins[d]);
       VGX_LAUNCH_CHECK();
     } else {
+      k_resolve<<<grid_for(M), kBlock, 0, s>>>(b.skeys.p, b.svals.p, M, b0.add.p, a.row.p, a.arena.p, g->n,
+                                               b.act.p, b.pos.p, fflag, iflag, nullptr);
+      VGX_LAUNCH_CHECK();
       hipcub::CountingInputIterator<uint32_t> it(0);
       with_temp(g, b.tmp, [&](void* t, size_t& bytes) {
         return hipcub::DeviceSelect::Flagged(t, bytes, it, fflag, b.fin.p, &g->d_ctr->nfinal[d],

@@ -201,6 +201,7 @@ struct BatchScratch {
   DevBuf<uint8_t> flags;
   DevBuf<uint8_t> tmp;      // library temp storage
   DevBuf<unsigned long long> extent;  // device batches: largest non-loop endpoint + 1
+  DevBuf<unsigned long long> bcount;  // per-workgroup counts of final arcs and inserts (compaction)
   uint64_t k = 0, m = 0;    // ops, arcs
   uint64_t nfinal = 0, nins = 0, nbig = 0, nmid = 0, need = 0;
 };
