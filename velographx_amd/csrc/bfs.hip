@@ -811,9 +811,10 @@ __device__ __forceinline__ void touch(const IncArgs& a, int par, uint32_t v, uin
 
 // Heads of deleted arcs that were shortest-parent arcs (bfs.hpp:52-63, 242-245),
 // touched for round 0.
-__global__ void k_inc_candidates(IncArgs a, const uint32_t* fin, uint64_t nfin, const unsigned long long* skeys,
-                                 const uint8_t* act, uint32_t* tlist, uint32_t* tall, uint32_t* tflist,
-                                 unsigned long long* tfpre) {
+__global__ void k_inc_candidates(IncArgs a, const uint32_t* fin, const unsigned long long* nfin_p,
+                                 const unsigned long long* skeys, const uint8_t* act, uint32_t* tlist,
+                                 uint32_t* tall, uint32_t* tflist, unsigned long long* tfpre) {
+  const uint64_t nfin = *nfin_p;  // the staged batch's count, read on the device
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   for (uint64_t j0 = base - lane_id(); j0 < nfin; j0 += stride) {
@@ -1553,7 +1554,11 @@ static uint32_t take_epochs(vgx_bfs* b, uint32_t count) {
   return first;
 }
 
-static void inc_sync(vgx_bfs* b) {
+// One synchronisation reads the repair state and, with `graph`, the
+// graph's counters too (staging or a finished batch left them for the host).
+static void inc_sync(vgx_bfs* b, bool graph = false) {
+  if (graph)
+    VGX_HIP(hipMemcpyAsync(b->g->h_ctr, b->g->d_ctr, sizeof(GraphCounters), hipMemcpyDeviceToHost, b->g->stream));
   VGX_HIP(hipMemcpyAsync(b->h_inc, b->d_inc, sizeof(IncState), hipMemcpyDeviceToHost, b->g->stream));
   VGX_HIP(hipStreamSynchronize(b->g->stream));
 }
@@ -1577,8 +1582,11 @@ static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, cons
   uint64_t limit = static_cast<uint64_t>(static_cast<double>(n_pre) * b->fallback);
   if (limit < 1) limit = 1;
 
-  if (device) graph_stage_device(g, src, dst, add, k, producer);  // may grow vertices
-  else graph_stage(g, src, dst, add, k);
+  // Staging leaves its counts on the device: the deletions and the first
+  // invalidation rounds read them there, and the host collects them at the
+  // invalidation's first synchronisation.
+  if (device) graph_stage_device(g, src, dst, add, k, producer, true);  // may grow vertices
+  else graph_stage(g, src, dst, add, k, true);
   extend_dist(b);
   BfsWork& w = b->w;
   VGX_HIP(hipMemsetAsync(b->d_inc, 0, sizeof(IncState), s));
@@ -1588,12 +1596,13 @@ static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, cons
   // ---- invalidation cascade on the graph without the deleted arcs
   IncArgs a = inc_args(b);
   a.epoch = take_epochs(b, 1);
-  if (bs.nfinal) {
-    k_inc_candidates<<<blocks_for(bs.nfinal), kBfsBlock, 0, s>>>(a, bs.fin.p, bs.nfinal, bs.skeys.p, bs.act.p,
-                                                                 w.tlist.p, w.tall.p, w.tflist.p, w.tfpre.p);
+  if (bs.m) {
+    k_inc_candidates<<<blocks_for(bs.m), kBfsBlock, 0, s>>>(a, bs.fin.p, &g->d_ctr->nfinal[0], bs.skeys.p, bs.act.p,
+                                                            w.tlist.p, w.tall.p, w.tflist.p, w.tfpre.p);
     VGX_LAUNCH_CHECK();
   }
   bool fallback = false;
+  bool collected = false;
   for (uint32_t r0 = 0;; r0 += kInvalidationRounds) {
     const uint32_t e0 = take_epochs(b, kInvalidationRounds);
     for (uint32_t i = 0; i < kInvalidationRounds; ++i) {
@@ -1608,7 +1617,11 @@ static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, cons
                                                      w.tflist.p, w.tfpre.p);
       VGX_LAUNCH_CHECK();
     }
-    inc_sync(b);
+    inc_sync(b, !collected);
+    if (!collected) {
+      graph_stage_collect(g);  // may grow the arena (rows keep their places)
+      collected = true;
+    }
     b->last_candidates = b->h_inc->candidates;
     if (b->h_inc->fallback || b->h_inc->aall > limit) {
       fallback = true;
@@ -1622,16 +1635,20 @@ static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, cons
 
   // ---- apply the additions; the graph is now the post-batch graph
   graph_apply_additions(g);
-  graph_finish_batch(g);  // may compact: row pointers change, refresh args
-  extend_dist(b);
-  a = inc_args(b);
   if (fallback) {
+    graph_finish_batch(g);  // may compact: row pointers change, refresh args
+    extend_dist(b);
+    a = inc_args(b);
     k_inc_cleanup<<<kRepairBlocks, kBfsBlock, 0, s>>>(a, w.tall.p);
     VGX_LAUNCH_CHECK();
     bfs_full(b);
     b->last_full = true;
     return;
   }
+  // The batch's bookkeeping (which may compact and move rows) waits for the
+  // propagation's last synchronisation: until then the rows stay where the
+  // additions left them, which is what the propagation reads.
+  a = inc_args(b);
 
   // ---- re-relaxation: seeds from unaffected in-neighbours and the added
   // arcs, then Bellman-Ford rounds, several per synchronisation
@@ -1660,13 +1677,14 @@ static void bfs_apply(vgx_bfs* b, const uint32_t* src, const uint32_t* dst, cons
                                                           w.queue[(r + 1) & 1].p, w.pre[(r + 1) & 1].p);
       VGX_LAUNCH_CHECK();
     }
-    inc_sync(b);
+    inc_sync(b, true);  // the finished batch's counters come along
     if (b->h_inc->fq[(r0 + kPropagationRounds) % 3] == 0) break;
     if (r0 > n_pre + 2) throw Error(VGX_DEVICE_ERROR, "propagation did not terminate");
   }
   // scratch flags only: the distances are final at the synchronisation above
   k_inc_cleanup<<<kRepairBlocks, kBfsBlock, 0, s>>>(a, w.tall.p);
   VGX_LAUNCH_CHECK();
+  graph_finish_collect(g);  // may compact: rows move, the distances do not
   b->reachable = b->reachable - n_aff + b->h_inc->became_reachable;
   b->last_relaxed = b->h_inc->relaxed;
 }

@@ -1078,9 +1078,9 @@ static void bulk_load_csr_device_t(vgx_graph* g, const T* indptr, const T* indic
 
 // ------------------------------------------------------------ batch apply
 
-static void stage_sorted(vgx_graph* g, uint64_t k);
+static void stage_sorted(vgx_graph* g, uint64_t k, bool defer);
 
-void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k) {
+void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k, bool defer) {
   if (k >= (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "update batch too large");
   uint64_t maxid = 0;
   bool any = false;
@@ -1104,7 +1104,7 @@ void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const u
   b0.add.ensure(k + 64);
   VGX_HIP(hipMemcpyAsync(b0.src.p, hs, 8 * k, hipMemcpyHostToDevice, s));
   VGX_HIP(hipMemcpyAsync(b0.add.p, hs + 2 * k, k, hipMemcpyHostToDevice, s));
-  stage_sorted(g, k);
+  stage_sorted(g, k, defer);
 }
 
 // Copies a device batch into the staging layout (src | dst, add) and
@@ -1127,7 +1127,7 @@ __global__ void k_batch_ingest(const uint32_t* __restrict__ src, const uint32_t*
 }
 
 void graph_stage_device(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
-                        hipStream_t producer) {
+                        hipStream_t producer, bool defer) {
   if (k >= (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "update batch too large");
   hipStream_t s = g->stream;
   if (producer != s) VGX_HIP(hipStreamSynchronize(producer));  // null: the default stream
@@ -1143,12 +1143,13 @@ void graph_stage_device(vgx_graph* g, const uint32_t* src, const uint32_t* dst, 
   VGX_HIP(hipStreamSynchronize(s));
   if (ext > 0 && ext - 1 >= (1ull << 31) - 1) throw Error(VGX_INVALID_ARGUMENT, "vertex ids must be below 2^31 - 1");
   if (ext > 0) graph_grow_vertices(g, ext);
-  stage_sorted(g, k);
+  stage_sorted(g, k, defer);
 }
 
 // Sort, resolve and plan a batch whose src | dst and add already sit in
-// bs[0] (from either staging path).
-static void stage_sorted(vgx_graph* g, uint64_t k) {
+// bs[0] (from either staging path).  With `defer` the host counts are left
+// for graph_stage_collect after the caller's next synchronisation.
+static void stage_sorted(vgx_graph* g, uint64_t k, bool defer) {
   hipStream_t s = g->stream;
   BatchScratch& b0 = g->bs[0];
   const uint32_t* d_src = b0.src.p;
@@ -1213,7 +1214,13 @@ static void stage_sorted(vgx_graph* g, uint64_t k) {
                                                   b.segk.p, b.big.p, b.mid.p, g->d_ctr, d);
     VGX_LAUNCH_CHECK();
   }
+  for (int d = 0; d < g->ndir(); ++d) g->bs[d].counted = false;
+  if (defer) return;
   graph_sync_counters(g);
+  graph_stage_collect(g);
+}
+
+void graph_stage_collect(vgx_graph* g) {
   for (int d = 0; d < g->ndir(); ++d) {
     BatchScratch& b = g->bs[d];
     b.nfinal = g->h_ctr->nfinal[d];
@@ -1221,6 +1228,7 @@ static void stage_sorted(vgx_graph* g, uint64_t k) {
     b.nbig = g->h_ctr->nbig[d];
     b.nmid = g->h_ctr->nmid[d];
     b.need = g->h_ctr->need[d];
+    b.counted = true;
     ensure_arena(g, d, g->h_ctr->need[d]);
   }
 }
@@ -1229,9 +1237,12 @@ void graph_apply_deletions(vgx_graph* g) {
   for (int d = 0; d < g->ndir(); ++d) {
     BatchScratch& b = g->bs[d];
     Adjacency& a = g->adj[d];
-    if (!b.nfinal) continue;
-    k_apply_flip<<<grid_for(b.nfinal), kBlock, 0, g->stream>>>(b.fin.p, &g->d_ctr->nfinal[d], b.skeys.p, b.act.p,
-                                                               b.pos.p, a.arena.p, a.deg.p, g->d_ctr, d, 0);
+    // before the counts reach the host, the grid covers every staged arc
+    // (the kernel reads the count of final arcs on the device)
+    const uint64_t items = b.counted ? b.nfinal : b.m;
+    if (!items) continue;
+    k_apply_flip<<<grid_for(items), kBlock, 0, g->stream>>>(b.fin.p, &g->d_ctr->nfinal[d], b.skeys.p, b.act.p,
+                                                            b.pos.p, a.arena.p, a.deg.p, g->d_ctr, d, 0);
     VGX_LAUNCH_CHECK();
   }
 }
@@ -1279,6 +1290,10 @@ static void compact_direction(vgx_graph* g, int d);
 
 void graph_finish_batch(vgx_graph* g) {
   graph_sync_counters(g);
+  graph_finish_collect(g);
+}
+
+void graph_finish_collect(vgx_graph* g) {
   bool need_compact = false;
   for (int d = 0; d < g->ndir(); ++d) {
     Adjacency& a = g->adj[d];

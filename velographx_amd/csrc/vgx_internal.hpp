@@ -203,6 +203,7 @@ struct BatchScratch {
   DevBuf<unsigned long long> extent;  // device batches: largest non-loop endpoint + 1
   DevBuf<unsigned long long> bcount;  // per-workgroup counts of final arcs and inserts (compaction)
   uint64_t k = 0, m = 0;    // ops, arcs
+  bool counted = true;      // nfinal .. need below hold the device counts
   uint64_t nfinal = 0, nins = 0, nbig = 0, nmid = 0, need = 0;
 };
 
@@ -290,16 +291,26 @@ namespace vgx {
 // Graph internals used by the algorithm files.
 void graph_grow_vertices(vgx_graph* g, uint64_t n_new);
 // Stages a batch: uploads, sorts, resolves actions; grows vertices; sizes
-// the arena for the inserts.  Does not change adjacency.
-void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k);
+// the arena for the inserts.  Does not change adjacency.  With `defer` it
+// returns without synchronising: the host counts of the staged batch (and
+// the arena sizing) wait for graph_stage_collect, which the caller runs
+// after its own next synchronisation of the counters (graph_sync_counters or
+// a copy of g->d_ctr to g->h_ctr); graph_apply_deletions may run before.
+void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
+                 bool defer = false);
 // The same for a batch already in device memory (src, dst, add device
 // pointers), written on `producer` (waited for first when it is not the
 // graph's stream; null = the default stream).
 void graph_stage_device(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k,
-                        hipStream_t producer);
+                        hipStream_t producer, bool defer = false);
+void graph_stage_collect(vgx_graph* g);
 void graph_apply_deletions(vgx_graph* g);
 void graph_apply_additions(vgx_graph* g);
+// Host bookkeeping of a finished batch (live and stored counts, version,
+// storage maintenance, which may compact and move rows): graph_finish_batch
+// synchronises the counters first, graph_finish_collect expects them synced.
 void graph_finish_batch(vgx_graph* g);
+void graph_finish_collect(vgx_graph* g);
 const CsrSnapshot& graph_snapshot(vgx_graph* g, int dir);
 void graph_sync_counters(vgx_graph* g);
 
