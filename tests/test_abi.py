@@ -23,7 +23,10 @@ def test_header_declares_the_entry_points():
                  "vgx_pagerank_create", "vgx_pagerank_apply", "vgx_msbfs_run", "vgx_ppr_forward",
                  "vgx_ppr_backward", "vgx_last_error", "vgx_graph_snapshot", "vgx_graph_probe_neighbors",
                  "vgx_pagerank_static", "vgx_graph_apply_device", "vgx_bfs_apply_device", "vgx_kcore_create",
-                 "vgx_kcore_apply", "vgx_components_create", "vgx_components_apply", "vgx_graph_device"):
+                 "vgx_kcore_apply", "vgx_components_create", "vgx_components_apply", "vgx_graph_device",
+                 "vgx_graph_bulk_load_device", "vgx_graph_bulk_load_csr_device", "vgx_msbfs_apply",
+                 "vgx_msbfs_apply_device", "vgx_msbfs_set_repair", "vgx_msbfs_repair_stats", "vgx_msbfs_set_params",
+                 "vgx_msbfs_get_params"):
         assert must in names
 
 
@@ -54,6 +57,16 @@ def test_null_handles_are_rejected_without_a_device():
     assert lib.vgx_graph_snapshot(None, ctypes.byref(out)) != 0
     assert lib.vgx_kcore_create(None, ctypes.byref(out)) != 0
     assert lib.vgx_components_create(None, ctypes.byref(out)) != 0
+    # device edge lists and the multi-source repair entry points
+    u64 = ctypes.c_uint64
+    assert lib.vgx_graph_bulk_load_device(None, None, None, u64(0), u64(1), ctypes.c_uint32(4), None) != 0
+    assert lib.vgx_graph_bulk_load_csr_device(None, None, None, u64(0), u64(0), ctypes.c_uint32(4), None) != 0
+    assert lib.vgx_msbfs_apply(None, None, None, None, u64(0)) != 0 and lib.vgx_last_error()
+    assert lib.vgx_msbfs_apply_device(None, None, None, None, u64(0), None) != 0
+    assert lib.vgx_msbfs_set_repair(None, 0, ctypes.c_double(0.1)) != 0
+    buf = ctypes.create_string_buffer(256)
+    assert lib.vgx_msbfs_repair_stats(None, buf) != 0
+    assert lib.vgx_msbfs_set_params(None, buf) != 0 and lib.vgx_msbfs_get_params(None, buf) != 0
     ns, dig = ctypes.c_double(), ctypes.c_uint64()
     assert lib.vgx_graph_probe_neighbors(None, ctypes.c_uint64(1), ctypes.c_uint32(1), ctypes.byref(ns),
                                          ctypes.byref(dig)) != 0
