@@ -176,6 +176,38 @@ __device__ __forceinline__ uint32_t row_lower_bound(const uint32_t* arena, uint2
   return lo;
 }
 
+// A batch of up to kSortSmall arcs is sorted by one workgroup in LDS (one
+// launch instead of the device-wide sort's histogram and digit passes):
+// keys and op indices in blocked order, padded with invalid keys, through a
+// stable block radix sort over the key's low end_bit bits, so the ops on an
+// arc stay in batch order.
+constexpr uint32_t kSortSmallThreads = 1024, kSortSmallItems = 4;
+constexpr uint64_t kSortSmall = kSortSmallThreads * kSortSmallItems;
+__global__ __launch_bounds__(kSortSmallThreads) void k_sort_small(const unsigned long long* __restrict__ keys,
+                                                                  const uint32_t* __restrict__ vals, uint32_t M,
+                                                                  int end_bit, unsigned long long* __restrict__ skeys,
+                                                                  uint32_t* __restrict__ svals) {
+  using Sort = hipcub::BlockRadixSort<unsigned long long, kSortSmallThreads, kSortSmallItems, uint32_t>;
+  __shared__ typename Sort::TempStorage tmp;
+  unsigned long long k[kSortSmallItems];
+  uint32_t v[kSortSmallItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kSortSmallItems; ++j) {
+    const uint32_t i = threadIdx.x * kSortSmallItems + j;
+    k[j] = i < M ? keys[i] : kInvalidKey;
+    v[j] = i < M ? vals[i] : 0u;
+  }
+  Sort(tmp).Sort(k, v, 0, end_bit);
+#pragma unroll
+  for (uint32_t j = 0; j < kSortSmallItems; ++j) {
+    const uint32_t i = threadIdx.x * kSortSmallItems + j;
+    if (i < M) {
+      skeys[i] = k[j];
+      svals[i] = v[j];
+    }
+  }
+}
+
 // Batches of up to kSelectSmall arc ops are compacted by k_resolve's
 // per-workgroup counts and k_select_write; larger ones use the library's
 // device-wide select.
@@ -1179,10 +1211,16 @@ static void stage_sorted(vgx_graph* g, uint64_t k, bool defer) {
     k_expand_keys<<<grid_for(k), kBlock, 0, s>>>(d_src, d_dst, k, g->directed ? 0 : 1, d == 1 ? 1 : 0, b.keys.p,
                                                   b.vals.p);
     VGX_LAUNCH_CHECK();
-    with_temp(g, b.tmp, [&](void* t, size_t& bytes) {
-      return hipcub::DeviceRadixSort::SortPairs(t, bytes, b.keys.p, b.skeys.p, b.vals.p, b.svals.p,
-                                                static_cast<int>(M), 0, end_bit, s);
-    });
+    if (M <= kSortSmall) {
+      k_sort_small<<<1, kSortSmallThreads, 0, s>>>(b.keys.p, b.vals.p, static_cast<uint32_t>(M), end_bit, b.skeys.p,
+                                                   b.svals.p);
+      VGX_LAUNCH_CHECK();
+    } else {
+      with_temp(g, b.tmp, [&](void* t, size_t& bytes) {
+        return hipcub::DeviceRadixSort::SortPairs(t, bytes, b.keys.p, b.skeys.p, b.vals.p, b.svals.p,
+                                                  static_cast<int>(M), 0, end_bit, s);
+      });
+    }
     uint8_t* fflag = b.flags.p;
     uint8_t* iflag = b.flags.p + fstride;
     if (M <= kSelectSmall) {

@@ -162,7 +162,7 @@ struct MsrArgs {
   const uint32_t* fin;
   const unsigned long long* skeys;
   const uint8_t* act;
-  unsigned long long nfin;
+  const unsigned long long* nfin;  // count of the staged batch's final arcs (device)
   const uint32_t* roots;
   uint32_t k;
   uint32_t pad;
@@ -807,10 +807,11 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_candidates(const MsrArgs* __re
   const unsigned lane = lane_id();
   const uint32_t myroot = lane < a.k ? a.roots[lane] : kInf;
   unsigned long long ncand = 0;
-  msr_windows(a.nfin, [&](uint64_t first) {
+  const uint64_t nfin = *a.nfin;
+  msr_windows(nfin, [&](uint64_t first) {
     const uint64_t j = first + lane;
     uint32_t u = 0, v = kTomb;
-    if (lane < kMsrWindow && j < a.nfin) {
+    if (lane < kMsrWindow && j < nfin) {
       const uint32_t p = a.fin[j];
       if (a.act[p] == kActDel) {
         u = static_cast<uint32_t>(a.skeys[p] >> 32);
@@ -1079,10 +1080,11 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_relax_additions(const MsrArgs*
   const MsrArgs& a = *pa;
   const unsigned lane = lane_id();
   unsigned long long low = 0;
-  msr_windows(a.nfin, [&](uint64_t first) {
+  const uint64_t nfin = *a.nfin;
+  msr_windows(nfin, [&](uint64_t first) {
     const uint64_t j = first + lane;
     uint32_t u = 0, v = kTomb;
-    if (lane < kMsrWindow && j < a.nfin) {
+    if (lane < kMsrWindow && j < nfin) {
       const uint32_t p = a.fin[j];
       if (a.act[p] == kActIns || a.act[p] == kActUndel) {
         u = static_cast<uint32_t>(a.skeys[p] >> 32);
@@ -1341,7 +1343,7 @@ static MsrArgs msr_args(vgx_msbfs* m) {
   a.fin = bs.fin.p;
   a.skeys = bs.skeys.p;
   a.act = bs.act.p;
-  a.nfin = bs.nfinal;
+  a.nfin = &g->d_ctr->nfinal[0];
   a.roots = m->droots.p;
   a.k = m->k;
   a.limit = static_cast<unsigned long long>(m->fallback * static_cast<double>(g->n) * m->k);
@@ -1357,7 +1359,11 @@ static void msr_upload(vgx_msbfs* m) {
   VGX_HIP(hipMemcpyAsync(m->rep.d_args, m->rep.h_args, sizeof(MsrArgs), hipMemcpyHostToDevice, m->g->stream));
 }
 
-static void msr_sync(vgx_msbfs* m) {
+// One synchronisation reads the repair state and, with `graph`, the graph's
+// counters (the staged batch's counts, or a finished batch's) too.
+static void msr_sync(vgx_msbfs* m, bool graph = false) {
+  if (graph)
+    VGX_HIP(hipMemcpyAsync(m->g->h_ctr, m->g->d_ctr, sizeof(GraphCounters), hipMemcpyDeviceToHost, m->g->stream));
   VGX_HIP(hipMemcpyAsync(m->rep.h_st, m->rep.d_st, sizeof(MsrState), hipMemcpyDeviceToHost, m->g->stream));
   VGX_HIP(hipStreamSynchronize(m->g->stream));
 }
@@ -1489,14 +1495,20 @@ static void msr_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, co
   }
   m->rep.setup = false;
   if (repair) msr_materialize(m);
-  if (device) graph_stage_device(g, src, dst, add, k, producer);
-  else graph_stage(g, src, dst, add, k);
+  // A repair leaves the staged batch's counts on the device until the
+  // invalidation's first synchronisation (its kernels read them there).
+  if (device) graph_stage_device(g, src, dst, add, k, producer, repair);
+  else graph_stage(g, src, dst, add, k, repair);
   // distances cover the vertices of the last pass or repair (m->n): a root
   // whose id the graph now reaches needs a pass (its own distance becomes 0)
   const uint64_t n_cov = m->n;
   for (uint32_t r = 0; r < m->k; ++r)
     if (m->roots[r] >= n_cov && m->roots[r] < g->n) repair = false;
   m->batches += 1;
+  if (!repair && !g->bs[0].counted) {  // staged for a repair, answered by a pass
+    graph_sync_counters(g);
+    graph_stage_collect(g);
+  }
   auto elapsed_ms = [&]() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   };
@@ -1522,7 +1534,8 @@ static void msr_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, co
   bool fallback = false;
   for (uint32_t r0 = 0;; r0 += kMsrInvRounds) {
     msr_run_program(m, r0 == 0 ? 0 : 1);
-    msr_sync(m);
+    msr_sync(m, r0 == 0);
+    if (r0 == 0) graph_stage_collect(g);  // may grow the arena (rows keep their places)
     if (R.h_st->fallback || R.h_st->apairs > static_cast<unsigned long long>(m->fallback * g->n * m->k + 1)) {
       fallback = true;
       break;
@@ -1533,9 +1546,8 @@ static void msr_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, co
   m->last_candidates = R.h_st->candidates;
   m->last_affected_pairs = R.h_st->apairs;
   graph_apply_additions(g);
-  graph_finish_batch(g);  // may compact: row pointers change
-  msr_upload(m);
   if (fallback) {
+    graph_finish_batch(g);
     VGX_HIP(hipMemsetAsync(R.cand.p, 0, (g->n + 1) * 8, s));
     VGX_HIP(hipMemsetAsync(R.aff.p, 0, (g->n + 1) * 8, s));
     ms_run(m);
@@ -1545,10 +1557,14 @@ static void msr_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, co
     m->last_fallback = true;
     return;
   }
+  // The batch's bookkeeping (which may compact and move rows) waits for the
+  // propagation's last synchronisation: the rows stay where the additions
+  // left them, which is what the propagation reads.
+  msr_upload(m);
   const uint64_t n_aff = R.h_st->aall;
   for (uint32_t r0 = 0;; r0 += kMsrPropRounds) {
     msr_run_program(m, r0 == 0 ? 2 : 3);
-    msr_sync(m);
+    msr_sync(m, true);  // the finished batch's counters come along
     if (R.h_st->fq[0] == 0) break;
     if (r0 > 2 * g->n + 4) throw Error(VGX_DEVICE_ERROR, "multi-source propagation did not terminate");
   }
@@ -1556,6 +1572,7 @@ static void msr_apply(vgx_msbfs* m, const uint32_t* src, const uint32_t* dst, co
     k_msr_clear<<<msr_grid(n_aff), kMsBlock, 0, s>>>(R.d_args);
     VGX_LAUNCH_CHECK();
   }
+  graph_finish_collect(g);  // may compact: rows move, the distances do not
   m->last_lowered = R.h_st->lowered;
   m->in_d32 = true;
   m->dist_version = g->version;
