@@ -23,11 +23,11 @@
 // order of the atomics.
 //
 // The pass is a fixed program of launches that the device steers by
-// itself: every level runs push/pull kernels that return at once unless the
-// level's direction (decided on the device at the end of the previous level)
-// is theirs, a finalize kernel that folds the level into seen and counts new
-// pairs and next-frontier arcs, and a one-wave decide kernel that picks the
-// next direction or marks the pass done.  Kernel arguments are one device
+// itself: every level runs one level kernel that takes the direction the
+// previous level decided on the device (its workgroups split between short
+// rows and long-row chunks), a finalize kernel that folds the level into
+// seen and counts new pairs and next-frontier arcs, and a one-wave decide
+// kernel that picks the next direction or marks the pass done.  Kernel arguments are one device
 // block rewritten before each pass, so the same launch program can be
 // captured once in a hipGraph and replayed for every pass while the graph
 // changes underneath (only its size class forces a new capture).
@@ -300,19 +300,11 @@ __global__ void k_ms_chunk_fill(const uint32_t* __restrict__ row_chunk, uint32_t
     for (uint32_t c = row_chunk[v]; c < row_chunk[v + 1]; ++c) chunk_row[c] = v;
 }
 
-// Whether program level l runs in direction `dir`.
-__device__ __forceinline__ bool level_runs(const MsArgs& a, uint32_t l, uint32_t dir) {
-  const MsCtl* c = a.ctl;
-  return !c->done && c->dir[l] == dir;
-}
-
 // push, short rows: lane per frontier vertex u.
-__global__ __launch_bounds__(kMsBlock) void k_ms_push_short(const MsArgs* __restrict__ pa, uint32_t l) {
-  const MsArgs a = *pa;
-  if (!level_runs(a, l, 0)) return;
+__device__ __forceinline__ void ms_push_short(const MsArgs& a, uint32_t l, uint32_t blk, uint32_t nblk) {
   const unsigned long long* frontier = level_ptr(a, l);
   unsigned long long* next = level_ptr(a, l + 1);
-  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < a.n; u += gridDim.x * blockDim.x) {
+  for (uint32_t u = blk * blockDim.x + threadIdx.x; u < a.n; u += nblk * blockDim.x) {
     const unsigned long long f = frontier[u];
     if (!f) continue;
     const uint2 rb = a.out_row[u];
@@ -328,12 +320,10 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_push_short(const MsArgs* __rest
 }
 
 // push, long rows: wave per chunk, lane per arc.
-__global__ __launch_bounds__(kMsBlock) void k_ms_push_long(const MsArgs* __restrict__ pa, uint32_t l) {
-  const MsArgs a = *pa;
-  if (!level_runs(a, l, 0)) return;
+__device__ __forceinline__ void ms_push_long(const MsArgs& a, uint32_t l, uint32_t blk, uint32_t nblk) {
   const unsigned long long* frontier = level_ptr(a, l);
   unsigned long long* next = level_ptr(a, l + 1);
-  for (uint32_t c = blockIdx.x * kMsWaves + threadIdx.x / kWave; c < a.out_nch; c += gridDim.x * kMsWaves) {
+  for (uint32_t c = blk * kMsWaves + threadIdx.x / kWave; c < a.out_nch; c += nblk * kMsWaves) {
     const uint32_t u = a.out_cr[c];
     const unsigned long long f = frontier[u];
     if (!f) continue;
@@ -358,13 +348,11 @@ __device__ __forceinline__ unsigned long long in_mask(const unsigned long long* 
 
 // pull, short rows: lane per unfinished vertex v, early exit once every
 // root is accounted for.
-__global__ __launch_bounds__(kMsBlock) void k_ms_pull_short(const MsArgs* __restrict__ pa, uint32_t l) {
-  const MsArgs a = *pa;
-  if (!level_runs(a, l, 1)) return;
+__device__ __forceinline__ void ms_pull_short(const MsArgs& a, uint32_t l, uint32_t blk, uint32_t nblk) {
   const unsigned long long* frontier = level_ptr(a, l);
   unsigned long long* next = level_ptr(a, l + 1);
   const unsigned long long all = a.all;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
+  for (uint32_t v = blk * blockDim.x + threadIdx.x; v < a.n; v += nblk * blockDim.x) {
     const unsigned long long s = a.seen[v];
     if (s == all) continue;
     const uint2 rb = a.in_row[v];
@@ -392,13 +380,11 @@ __device__ __forceinline__ unsigned long long wave_or(unsigned long long x) {
 }
 
 // pull, long rows: wave per chunk; the early-exit test runs every 4 steps.
-__global__ __launch_bounds__(kMsBlock) void k_ms_pull_long(const MsArgs* __restrict__ pa, uint32_t l) {
-  const MsArgs a = *pa;
-  if (!level_runs(a, l, 1)) return;
+__device__ __forceinline__ void ms_pull_long(const MsArgs& a, uint32_t l, uint32_t blk, uint32_t nblk) {
   const unsigned long long* frontier = level_ptr(a, l);
   unsigned long long* next = level_ptr(a, l + 1);
   const unsigned long long all = a.all;
-  for (uint32_t c = blockIdx.x * kMsWaves + threadIdx.x / kWave; c < a.in_nch; c += gridDim.x * kMsWaves) {
+  for (uint32_t c = blk * kMsWaves + threadIdx.x / kWave; c < a.in_nch; c += nblk * kMsWaves) {
     const uint32_t v = a.in_cr[c];
     const unsigned long long s = a.seen[v];
     if (s == all) continue;
@@ -417,6 +403,32 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_pull_long(const MsArgs* __restr
     }
     const unsigned long long m = acc & ~s;
     if (lane_id() == 0 && m) atomicOr(&next[v], m);
+  }
+}
+
+// One level in the direction the previous level's decide kernel set: the
+// first `shrt` workgroups take the rows of at most short_row arcs, the
+// others the long-row chunks of that direction (one launch per level in
+// place of a push and a pull pair, of which one would exit at once).
+__global__ __launch_bounds__(kMsBlock) void k_ms_level(const MsArgs* __restrict__ pa, uint32_t l, uint32_t shrt,
+                                                       uint32_t out_long, uint32_t in_long) {
+  const MsArgs a = *pa;
+  const MsCtl* c = a.ctl;
+  if (c->done) return;
+  const bool pull = c->dir[l] != 0;
+  const uint32_t b = blockIdx.x;
+  if (b < shrt) {
+    if (pull)
+      ms_pull_short(a, l, b, shrt);
+    else
+      ms_push_short(a, l, b, shrt);
+    return;
+  }
+  const uint32_t lb = b - shrt;
+  if (pull) {
+    if (lb < in_long) ms_pull_long(a, l, lb, in_long);
+  } else if (lb < out_long) {
+    ms_push_long(a, l, lb, out_long);
   }
 }
 
@@ -599,13 +611,8 @@ static void ms_program(const vgx_msbfs* m, const MsGrids& gr, uint32_t prog, int
   k_ms_start<<<gr.start, kMsBlock, 0, s>>>(pa, fresh, first_dir);
   VGX_LAUNCH_CHECK();
   for (uint32_t l = 0; l < prog; ++l) {
-    k_ms_push_short<<<gr.shrt, kMsBlock, 0, s>>>(pa, l);
-    VGX_LAUNCH_CHECK();
-    k_ms_push_long<<<gr.out_long, kMsBlock, 0, s>>>(pa, l);
-    VGX_LAUNCH_CHECK();
-    k_ms_pull_short<<<gr.shrt, kMsBlock, 0, s>>>(pa, l);
-    VGX_LAUNCH_CHECK();
-    k_ms_pull_long<<<gr.in_long, kMsBlock, 0, s>>>(pa, l);
+    k_ms_level<<<gr.shrt + std::max(gr.out_long, gr.in_long), kMsBlock, 0, s>>>(pa, l, gr.shrt, gr.out_long,
+                                                                                 gr.in_long);
     VGX_LAUNCH_CHECK();
     k_ms_finalize<<<gr.fin, kMsBlock, 0, s>>>(pa, l);
     VGX_LAUNCH_CHECK();
