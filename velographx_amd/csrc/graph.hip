@@ -307,32 +307,56 @@ __global__ void k_plan_inserts(const uint32_t* __restrict__ ins, const unsigned 
                                const uint32_t* __restrict__ cap, uint32_t* __restrict__ segk,
                                uint32_t* __restrict__ big, uint32_t* __restrict__ mid, GraphCounters* ctr, int dir) {
   const uint64_t nins = *nins_p;
-  for (uint64_t j = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; j < nins;
-       j += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    uint32_t u = static_cast<uint32_t>(skeys[ins[j]] >> 32);
-    if (j > 0 && static_cast<uint32_t>(skeys[ins[j - 1]] >> 32) == u) {
-      segk[j] = 0;
-      continue;
+  const unsigned lane = lane_id();
+  // Every counter below takes one atomic per wave (a hot counter takes an
+  // atomic per row otherwise: thousands in a row on one address).
+  unsigned long long mid_added = 0;  // inserts merged in place by k_insert_mid
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t j0 = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x - lane; j0 < nins; j0 += stride) {
+    const uint64_t j = j0 + lane;
+    bool is_big = false, is_mid = false;
+    unsigned long long nc = 0;
+    if (j < nins) {
+      const uint32_t u = static_cast<uint32_t>(skeys[ins[j]] >> 32);
+      if (j > 0 && static_cast<uint32_t>(skeys[ins[j - 1]] >> 32) == u) {
+        segk[j] = 0;
+      } else {
+        uint64_t e = j + 1;
+        while (e < nins && static_cast<uint32_t>(skeys[ins[e]] >> 32) == u) ++e;
+        const uint32_t k = static_cast<uint32_t>(e - j);
+        segk[j] = k;
+        const uint32_t len = row[u].y, c = cap[u];
+        const uint32_t need = len + k;
+        // a row that outgrows its slots, or is too long to shift in place
+        // by one workgroup, is rewritten at the bump pointer by many
+        // workgroups; a row of more than kBigRow entries that fits is merged
+        // in place
+        is_big = need > c || len > kInPlaceMax || (need > kBigRow && k > kMidInsMax);
+        if (is_big) {
+          nc = relocated_cap(need);
+        } else if (need > kBigRow) {
+          is_mid = true;
+          mid_added += k;
+        }
+      }
     }
-    uint64_t e = j + 1;
-    while (e < nins && static_cast<uint32_t>(skeys[ins[e]] >> 32) == u) ++e;
-    uint32_t k = static_cast<uint32_t>(e - j);
-    segk[j] = k;
-    uint32_t len = row[u].y, c = cap[u];
-    uint32_t need = len + k;
-    // a row that outgrows its slots, or is too long to shift in place by one
-    // workgroup, is rewritten at the bump pointer by many workgroups; a row
-    // of more than kBigRow entries that fits is merged in place
-    const bool reloc = need > c || len > kInPlaceMax || (need > kBigRow && k > kMidInsMax);
-    if (reloc) {
-      uint32_t nc = relocated_cap(need);
-      atomicAdd(&ctr->need[dir], static_cast<unsigned long long>(nc));
-      unsigned long long slot = atomicAdd(&ctr->nbig[dir], 1ull);
-      big[slot] = static_cast<uint32_t>(j);
-    } else if (need > kBigRow) {
-      unsigned long long slot = atomicAdd(&ctr->nmid[dir], 1ull);
-      mid[slot] = static_cast<uint32_t>(j);
+    const unsigned long long bb = __ballot(is_big), bm = __ballot(is_mid);
+    unsigned long long base_big = 0, base_mid = 0;
+    nc = wave_sum(nc);
+    if (lane == 0) {
+      if (bb) base_big = atomicAdd(&ctr->nbig[dir], static_cast<unsigned long long>(__popcll(bb)));
+      if (bm) base_mid = atomicAdd(&ctr->nmid[dir], static_cast<unsigned long long>(__popcll(bm)));
+      if (nc) atomicAdd(&ctr->need[dir], nc);
     }
+    base_big = __shfl(base_big, 0, kWave);
+    base_mid = __shfl(base_mid, 0, kWave);
+    if (is_big) big[base_big + lanes_below(bb)] = static_cast<uint32_t>(j);
+    if (is_mid) mid[base_mid + lanes_below(bm)] = static_cast<uint32_t>(j);
+  }
+  mid_added = wave_sum(mid_added);
+  if (lane == 0 && mid_added) {
+    atomicAdd(&ctr->live_delta[dir], mid_added);
+    atomicAdd(&ctr->stored_delta[dir], mid_added);
   }
 }
 
@@ -422,7 +446,7 @@ __global__ __launch_bounds__(256) void k_insert_mid(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ segk,
                                                     const unsigned long long* __restrict__ skeys,
                                                     uint2* __restrict__ row, uint32_t* __restrict__ arena,
-                                                    uint32_t* __restrict__ deg, GraphCounters* ctr, int dir) {
+                                                    uint32_t* __restrict__ deg) {
   __shared__ uint32_t s_row[kInPlaceMax];
   __shared__ uint32_t s_ins[kMidMaxIns];
   const uint32_t j = mid[blockIdx.x];
@@ -458,11 +482,9 @@ __global__ __launch_bounds__(256) void k_insert_mid(const uint32_t* __restrict__
     }
     a[t + lo] = x;
   }
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0) {  // the live and stored counts were added by k_plan_inserts
     row[u].y = len + k;
     atomicAdd(&deg[u], k);
-    atomicAdd(&ctr->live_delta[dir], static_cast<unsigned long long>(k));
-    atomicAdd(&ctr->stored_delta[dir], static_cast<unsigned long long>(k));
   }
 }
 
@@ -1303,7 +1325,7 @@ void graph_apply_additions(vgx_graph* g) {
     }
     if (b.nmid) {
       k_insert_mid<<<static_cast<unsigned>(b.nmid), 256, 0, g->stream>>>(b.mid.p, b.ins.p, b.segk.p, b.skeys.p,
-                                                                        a.row.p, a.arena.p, a.deg.p, g->d_ctr, d);
+                                                                        a.row.p, a.arena.p, a.deg.p);
       VGX_LAUNCH_CHECK();
     }
     if (b.nbig) {
