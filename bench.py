@@ -498,7 +498,7 @@ def leg_batched(vx, wl, g, lcc, m_c, k, plain_dist, plain_gteps):
     reached = ms.reached()
     agg = m_c * k / t / 1e9
     out = {"roots": k, "ms_per_pass": 1e3 * t, "aggregate_gteps": agg, "batched_vs_plain": agg / plain_gteps,
-           "tuning": {"blocks": tuned["blocks"], "alpha": tuned["alpha"]},
+           "tuning": {k: tuned[k] for k in ("blocks", "alpha", "beta", "short_row", "chunk")},
            "levels": int(ms.level_count), "exact_vs_plain": bool(exact),
            "all_roots_reach_component": bool(np.all(reached == lcc.size))}
     log(f"batched: {out}")
@@ -546,7 +546,7 @@ def leg_batching(vx, wl, args):
     t_eager = (time.perf_counter() - t0) / reps
     ms.set_graph(True)
     ms2 = vx.MultiSourceBFS(g2, roots)
-    ms2.set_tuning(tuned["blocks"], tuned["alpha"])
+    ms2.set_params(tuned["blocks"], tuned["alpha"], tuned["beta"], tuned["short_row"], tuned["chunk"])
     ms2.set_graph(True)
     ms2.set_repair(0, 0.1)
     ms2.run(g2)
@@ -598,7 +598,7 @@ def leg_batching(vx, wl, args):
                        f"{len(batches)} batches of log-uniform size 10..100000 (80/20)",
            "arcs": int(g.edge_count_directed), "pass_ms_graph": 1e3 * t_graph, "pass_ms_eager": 1e3 * t_eager,
            "aggregate_gteps_graph": arcs_c / 2.0 * 64 / t_graph / 1e9,
-           "tuning": {"blocks": tuned["blocks"], "alpha": tuned["alpha"]},
+           "tuning": {k: tuned[k] for k in ("blocks", "alpha", "beta", "short_row", "chunk")},
            "batches": len(batches), "update_ops": ops,
            "recompute_arm": {"mean_apply_ms": 1e3 * float(np.mean(t_apply)),
                              "mean_pass_ms": 1e3 * float(np.mean(t_pass)),
