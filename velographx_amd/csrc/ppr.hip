@@ -23,6 +23,11 @@
 // 128-byte row of X as float4s and a wave gathers eight arcs per load;
 // otherwise the two half-waves take one arc each per step and their 32
 // lanes the K <= 32 columns.
+// A sweep gives rows of at most 8 arcs (empty ones included: most rows of
+// a skewed graph) to eight-lane groups, a row per group and eight
+// consecutive rows per wave, in place of a wave each: a wave's pass over such
+// a row is a chain of dependent loads with most lanes idle.  The two kinds of
+// work share one launch, in separate workgroups.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -36,6 +41,7 @@ extern thread_local std::string t_last_error;
 constexpr int kPprBlock = 256;
 constexpr uint32_t kPprChunk = 512;
 constexpr uint32_t kPprWaves = kPprBlock / kWave;
+constexpr uint32_t kPprShort = 8;  // rows of at most this many arcs: one eight-lane group each
 
 // ------------------------------------------------------------ chunk plans
 
@@ -155,7 +161,8 @@ __device__ __forceinline__ float chunk_gather(const uint32_t* __restrict__ nbr, 
 
 // The same for K = 32 with 16-byte loads: lane group g = lane / 8 takes arcs
 // b+g, b+g+8, ...; lane s = lane % 8 of the group holds columns 4s..4s+3, so
-// one load instruction fetches eight 128-byte rows.  Two steps are in flight.
+// one load instruction fetches eight 128-byte rows.  Four steps are in flight
+// (two in the tail).
 // Returns the chunk's sums on lanes 0..7 (columns 4*lane..4*lane+3).
 __device__ __forceinline__ float4 chunk_gather32(const uint32_t* __restrict__ nbr, const float* __restrict__ p,
                                                  const float* __restrict__ x, uint32_t b, uint32_t e,
@@ -164,6 +171,30 @@ __device__ __forceinline__ float4 chunk_gather32(const uint32_t* __restrict__ nb
   const float4* x4 = reinterpret_cast<const float4*>(x);
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t i = b + grp;
+  for (; i + 24 < e; i += 32) {
+    const uint32_t u0 = nbr[i], u1 = nbr[i + 8], u2 = nbr[i + 16], u3 = nbr[i + 24];
+    const float p0 = p[i], p1 = p[i + 8], p2 = p[i + 16], p3 = p[i + 24];
+    const float4 a0 = x4[static_cast<uint64_t>(u0) * 8 + sub];
+    const float4 a1 = x4[static_cast<uint64_t>(u1) * 8 + sub];
+    const float4 a2 = x4[static_cast<uint64_t>(u2) * 8 + sub];
+    const float4 a3 = x4[static_cast<uint64_t>(u3) * 8 + sub];
+    acc.x += p0 * a0.x;
+    acc.y += p0 * a0.y;
+    acc.z += p0 * a0.z;
+    acc.w += p0 * a0.w;
+    acc.x += p1 * a1.x;
+    acc.y += p1 * a1.y;
+    acc.z += p1 * a1.z;
+    acc.w += p1 * a1.w;
+    acc.x += p2 * a2.x;
+    acc.y += p2 * a2.y;
+    acc.z += p2 * a2.z;
+    acc.w += p2 * a2.w;
+    acc.x += p3 * a3.x;
+    acc.y += p3 * a3.y;
+    acc.z += p3 * a3.z;
+    acc.w += p3 * a3.w;
+  }
   for (; i + 8 < e; i += 16) {
     const uint32_t u0 = nbr[i], u1 = nbr[i + 8];
     const float p0 = p[i], p1 = p[i + 8];
@@ -197,20 +228,23 @@ __device__ __forceinline__ float4 chunk_gather32(const uint32_t* __restrict__ nb
   return acc;
 }
 
-// y[v,:] = a * b[v,:] + d * sum_e p[e] * x[nbr_e,:]; rows of several chunks
-// leave per-chunk partial rows for k_ppr_finish.
+// y[v,:] = a * b[v,:] + d * sum_e p[e] * x[nbr_e,:] over the chunks of rows
+// longer than kPprShort; rows of several chunks leave per-chunk partial rows
+// for k_ppr_finish.
 template <bool kVec32>
-__global__ __launch_bounds__(kPprBlock) void k_ppr_sweep(const uint32_t* __restrict__ off,
+__device__ __forceinline__ void ppr_long_chunks(uint32_t blk, uint32_t nblk, const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ nbr,
                                                          const float* __restrict__ p,
                                                          const uint32_t* __restrict__ row_chunk,
-                                                         const uint32_t* __restrict__ chunk_row, uint32_t nchunks,
+                                                         const uint32_t* __restrict__ chunk_row,
+                                                         const uint32_t* __restrict__ lchunk, uint32_t nlchunks,
                                                          uint32_t K, float a, const float* __restrict__ b, float d,
                                                          const float* __restrict__ x, float* __restrict__ y,
                                                          float* __restrict__ part) {
   const unsigned lane = lane_id();
   const unsigned half = lane >> 5, col = lane & 31;
-  for (uint32_t c = blockIdx.x * kPprWaves + threadIdx.x / kWave; c < nchunks; c += gridDim.x * kPprWaves) {
+  for (uint32_t j = blk * kPprWaves + threadIdx.x / kWave; j < nlchunks; j += nblk * kPprWaves) {
+    const uint32_t c = lchunk[j];
     const uint32_t v = chunk_row[c];
     const bool single = row_chunk[v + 1] - row_chunk[v] == 1;
     uint32_t s, e;
@@ -238,6 +272,142 @@ __global__ __launch_bounds__(kPprBlock) void k_ppr_sweep(const uint32_t* __restr
         }
       }
     }
+  }
+}
+
+// Rows of at most kPprShort arcs, eight lanes each, rows in vertex order:
+// y[v,:] = a * b[v,:] + d * sum_e p[e] * x[nbr_e,:], the arcs added in row
+// order.  Lane s of a group holds columns 4s..4s+3 (K = 32, float4) or
+// s, s+8, s+16, s+24 (below K).  Longer rows are ppr_long_chunks'.
+template <bool kVec32>
+__device__ __forceinline__ void ppr_short_rows(uint32_t blk, uint32_t nblk, const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ nbr,
+                                                         const float* __restrict__ p, uint32_t n, uint32_t K,
+                                                         float a, const float* __restrict__ b, float d,
+                                                         const float* __restrict__ x, float* __restrict__ y) {
+  const unsigned sub = lane_id() & 7;
+  const uint32_t stride = nblk * (kPprBlock / 8);
+  for (uint32_t v = (blk * kPprBlock + threadIdx.x) >> 3; v < n; v += stride) {
+    const uint32_t s = off[v], e = off[v + 1];
+    if (e - s > kPprShort) continue;
+    if (kVec32) {
+      const float4* x4 = reinterpret_cast<const float4*>(x);
+      const uint64_t o = static_cast<uint64_t>(v) * 8 + sub;
+      const float4 bb = reinterpret_cast<const float4*>(b)[o];
+      float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+      uint32_t i = s;
+      for (; i + 3 < e; i += 4) {
+        const uint32_t u0 = nbr[i], u1 = nbr[i + 1], u2 = nbr[i + 2], u3 = nbr[i + 3];
+        const float p0 = p[i], p1 = p[i + 1], p2 = p[i + 2], p3 = p[i + 3];
+        const float4 a0 = x4[static_cast<uint64_t>(u0) * 8 + sub];
+        const float4 a1 = x4[static_cast<uint64_t>(u1) * 8 + sub];
+        const float4 a2 = x4[static_cast<uint64_t>(u2) * 8 + sub];
+        const float4 a3 = x4[static_cast<uint64_t>(u3) * 8 + sub];
+        acc.x += p0 * a0.x;
+        acc.y += p0 * a0.y;
+        acc.z += p0 * a0.z;
+        acc.w += p0 * a0.w;
+        acc.x += p1 * a1.x;
+        acc.y += p1 * a1.y;
+        acc.z += p1 * a1.z;
+        acc.w += p1 * a1.w;
+        acc.x += p2 * a2.x;
+        acc.y += p2 * a2.y;
+        acc.z += p2 * a2.z;
+        acc.w += p2 * a2.w;
+        acc.x += p3 * a3.x;
+        acc.y += p3 * a3.y;
+        acc.z += p3 * a3.z;
+        acc.w += p3 * a3.w;
+      }
+      for (; i < e; ++i) {
+        const float p0 = p[i];
+        const float4 a0 = x4[static_cast<uint64_t>(nbr[i]) * 8 + sub];
+        acc.x += p0 * a0.x;
+        acc.y += p0 * a0.y;
+        acc.z += p0 * a0.z;
+        acc.w += p0 * a0.w;
+      }
+      reinterpret_cast<float4*>(y)[o] =
+          make_float4(a * bb.x + d * acc.x, a * bb.y + d * acc.y, a * bb.z + d * acc.z, a * bb.w + d * acc.w);
+    } else {
+      float acc[4] = {0.f, 0.f, 0.f, 0.f};
+      uint32_t i = s;
+      for (; i + 1 < e; i += 2) {
+        const uint32_t u0 = nbr[i], u1 = nbr[i + 1];
+        const float p0 = p[i], p1 = p[i + 1];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t col = sub + 8 * j;
+          if (col < K) {
+            const float x0 = x[static_cast<uint64_t>(u0) * K + col];
+            const float x1 = x[static_cast<uint64_t>(u1) * K + col];
+            acc[j] += p0 * x0;
+            acc[j] += p1 * x1;
+          }
+        }
+      }
+      if (i < e) {
+        const uint32_t u0 = nbr[i];
+        const float p0 = p[i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t col = sub + 8 * j;
+          if (col < K) acc[j] += p0 * x[static_cast<uint64_t>(u0) * K + col];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t col = sub + 8 * j;
+        if (col < K) {
+          const uint64_t o = static_cast<uint64_t>(v) * K + col;
+          y[o] = a * b[o] + d * acc[j];
+        }
+      }
+    }
+  }
+}
+
+// One sweep: the first `long_blocks` workgroups take the chunks of long rows
+// (the hub chunks start first), the rest the short rows.
+template <bool kVec32>
+__global__ __launch_bounds__(kPprBlock) void k_ppr_sweep(const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ nbr,
+                                                         const float* __restrict__ p,
+                                                         const uint32_t* __restrict__ row_chunk,
+                                                         const uint32_t* __restrict__ chunk_row,
+                                                         const uint32_t* __restrict__ lchunk, uint32_t nlchunks,
+                                                         uint32_t long_blocks, uint32_t n, uint32_t K, float a,
+                                                         const float* __restrict__ b, float d,
+                                                         const float* __restrict__ x, float* __restrict__ y,
+                                                         float* __restrict__ part) {
+  if (blockIdx.x < long_blocks)
+    ppr_long_chunks<kVec32>(blockIdx.x, long_blocks, off, nbr, p, row_chunk, chunk_row, lchunk, nlchunks, K, a, b,
+                            d, x, y, part);
+  else
+    ppr_short_rows<kVec32>(blockIdx.x - long_blocks, gridDim.x - long_blocks, off, nbr, p, n, K, a, b, d, x, y);
+}
+
+// Chunks of rows longer than kPprShort, one append per wave.
+__global__ void k_ppr_long_collect(const uint32_t* __restrict__ off, const uint32_t* __restrict__ chunk_row,
+                                   uint32_t nchunks, uint32_t* __restrict__ lchunk,
+                                   unsigned long long* __restrict__ count) {
+  const unsigned lane = lane_id();
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t c0 = blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1)); c0 < nchunks; c0 += stride) {
+    const uint32_t c = c0 + lane;
+    bool take = false;
+    if (c < nchunks) {
+      const uint32_t v = chunk_row[c];
+      take = off[v + 1] - off[v] > kPprShort;
+    }
+    const unsigned long long m = __ballot(take);
+    if (!m) continue;
+    const unsigned first = __ffsll(static_cast<long long>(m)) - 1;
+    unsigned long long base = 0;
+    if (lane == first) base = atomicAdd(count, static_cast<unsigned long long>(__popcll(m)));
+    base = __shfl(base, first, kWave);
+    if (take) lchunk[base + lanes_below(m)] = c;
   }
 }
 
@@ -380,17 +550,23 @@ static const PprPlan& ppr_plan(vgx_graph* g, int dir, const CsrSnapshot& sn, hip
     // row's sum runs over its own chunks in a fixed order)
     const uint64_t most = std::min<uint64_t>(n, total / 2 + 1);
     pl.multi.ensure(most + 1);
+    pl.lchunk.ensure(static_cast<uint64_t>(total) + 1);
     DevBuf<unsigned long long> cnt64;
-    cnt64.ensure(1);
-    VGX_HIP(hipMemsetAsync(cnt64.p, 0, 8, s));
+    cnt64.ensure(2);
+    VGX_HIP(hipMemsetAsync(cnt64.p, 0, 16, s));
     k_ppr_multi_collect<<<ppr_blocks(n, kPprBlock), kPprBlock, 0, s>>>(pl.row_chunk.p, n, pl.multi.p, cnt64.p);
     VGX_LAUNCH_CHECK();
-    unsigned long long nm = 0;
-    VGX_HIP(hipMemcpyAsync(&nm, cnt64.p, 8, hipMemcpyDeviceToHost, s));
+    k_ppr_long_collect<<<ppr_blocks(total, kPprBlock), kPprBlock, 0, s>>>(sn.off.p, pl.chunk_row.p, total,
+                                                                         pl.lchunk.p, cnt64.p + 1);
+    VGX_LAUNCH_CHECK();
+    unsigned long long nm[2] = {0, 0};
+    VGX_HIP(hipMemcpyAsync(nm, cnt64.p, 16, hipMemcpyDeviceToHost, s));
     VGX_HIP(hipStreamSynchronize(s));
-    pl.nmulti = nm;
+    pl.nmulti = nm[0];
+    pl.nlchunks = nm[1];
   } else {
     pl.nmulti = 0;
+    pl.nlchunks = 0;
   }
   pl.nchunks = total;
   pl.version = sn.version;
@@ -442,7 +618,8 @@ static void ppr_sweeps(vgx_graph* g, const CsrSnapshot& sn, const PprPlan& pl, c
   const uint64_t nk = static_cast<uint64_t>(n) * K;
   pw.tmp.ensure(nk + 1);
   pw.part.ensure(pl.nchunks * K + 1);
-  const unsigned gs = ppr_blocks(pl.nchunks, kPprWaves);
+  const unsigned gs = ppr_blocks(pl.nlchunks, kPprWaves);
+  const unsigned gr = ppr_blocks(n, kPprBlock / 8);
   const unsigned gf = ppr_blocks(pl.nmulti, kPprWaves);
   // 16-byte rows need K = 32 and 16-byte aligned X, S and output buffers
   const bool vec = K == 32 && ((reinterpret_cast<uintptr_t>(io) | reinterpret_cast<uintptr_t>(b) |
@@ -451,12 +628,15 @@ static void ppr_sweeps(vgx_graph* g, const CsrSnapshot& sn, const PprPlan& pl, c
   float* cur = io;
   float* nxt = pw.tmp.p;
   for (uint32_t it = 0; it < iterations; ++it) {
+    const uint32_t lb = pl.nlchunks ? gs : 0;
     if (vec)
-      k_ppr_sweep<true><<<gs, kPprBlock, 0, s>>>(sn.off.p, sn.adj.p, p, pl.row_chunk.p, pl.chunk_row.p,
-                                                 static_cast<uint32_t>(pl.nchunks), K, a, b, d, cur, nxt, pw.part.p);
+      k_ppr_sweep<true><<<lb + gr, kPprBlock, 0, s>>>(sn.off.p, sn.adj.p, p, pl.row_chunk.p, pl.chunk_row.p,
+                                                      pl.lchunk.p, static_cast<uint32_t>(pl.nlchunks), lb, n, K, a,
+                                                      b, d, cur, nxt, pw.part.p);
     else
-      k_ppr_sweep<false><<<gs, kPprBlock, 0, s>>>(sn.off.p, sn.adj.p, p, pl.row_chunk.p, pl.chunk_row.p,
-                                                  static_cast<uint32_t>(pl.nchunks), K, a, b, d, cur, nxt, pw.part.p);
+      k_ppr_sweep<false><<<lb + gr, kPprBlock, 0, s>>>(sn.off.p, sn.adj.p, p, pl.row_chunk.p, pl.chunk_row.p,
+                                                       pl.lchunk.p, static_cast<uint32_t>(pl.nlchunks), lb, n, K, a,
+                                                       b, d, cur, nxt, pw.part.p);
     VGX_LAUNCH_CHECK();
     if (pl.nmulti) {
       k_ppr_finish<<<gf, kPprBlock, 0, s>>>(pl.row_chunk.p, pl.multi.p, static_cast<uint32_t>(pl.nmulti), K, a, b,

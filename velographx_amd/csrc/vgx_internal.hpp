@@ -223,8 +223,10 @@ struct PprPlan {
   DevBuf<uint32_t> row_chunk;  // [n+1]
   DevBuf<uint32_t> chunk_row;  // [nchunks]
   DevBuf<uint32_t> multi;      // rows of more than one chunk
+  DevBuf<uint32_t> lchunk;     // chunks of rows longer than the short-row bound
   uint64_t nchunks = 0;
   uint64_t nmulti = 0;
+  uint64_t nlchunks = 0;
   uint64_t version = ~0ull;
   uint64_t n = 0;
 };
