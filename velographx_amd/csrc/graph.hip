@@ -602,9 +602,14 @@ constexpr uint32_t kPackGroup = 8;
 constexpr uint32_t kPackChunk = 1024;
 
 // Copy the live entries of every row of up to kPackShort entries to
-// out + off[u] (order preserved): groups of eight lanes, eight rows side by
-// side, eight entries per step.  Optionally writes the new row table (every
-// row) for compaction.
+// out + off[u] (order preserved), a wave per 64 consecutive rows.  Rows
+// without deleted entries (all of them after a bulk load) are copied
+// flattened: the wave scans their lengths, and lane l takes positions
+// l, l + 64, ... of their concatenation, finding its row by a binary search
+// over the 64 starts held in the lanes, four positions in flight.  Rows with
+// deleted entries are compacted by groups of eight lanes, eight rows side by
+// side, each row's entries loaded at once (eight per lane) and placed eight
+// per step.  Optionally writes the new row table (every row) for compaction.
 __global__ void k_pack_rows(const uint2* __restrict__ row, const uint32_t* __restrict__ arena, uint64_t n,
                             const uint32_t* __restrict__ off, uint32_t* __restrict__ out, uint64_t out_cap,
                             uint2* __restrict__ new_row, const uint32_t* __restrict__ deg) {
@@ -615,27 +620,71 @@ __global__ void k_pack_rows(const uint2* __restrict__ row, const uint32_t* __res
   for (uint64_t u0 = gw * kWave; u0 < n; u0 += W * kWave) {
     const uint64_t u = u0 + lane;
     uint2 rb = make_uint2(0u, 0u);
-    uint32_t o = 0;
+    uint32_t o = 0, dg = 0;
     if (u < n) {
       rb = row[u];
       o = off[u];
-      if (new_row != nullptr) new_row[u] = make_uint2(o, deg[u]);
+      dg = deg[u];
+      if (new_row != nullptr) new_row[u] = make_uint2(o, dg);
     }
     const bool shrt = rb.y <= kPackShort;
+    const bool dirty = shrt && dg != rb.y;
+    // clean short rows, flattened
+    const uint32_t flen = shrt && !dirty ? rb.y : 0u;
+    const uint32_t fend = wave_inclusive_scan(flen);
+    const uint32_t fstart = fend - flen;
+    const uint32_t total = __shfl(fend, kWave - 1, kWave);
+    constexpr uint32_t kFlat = 4;
+    for (uint32_t k = 0; k < total; k += kFlat * kWave) {
+      uint32_t x[kFlat];
+      uint64_t at[kFlat];
+#pragma unroll
+      for (uint32_t q = 0; q < kFlat; ++q) {
+        const uint32_t pos = k + q * kWave + lane;
+        const bool in = pos < total;
+        const uint32_t pq = in ? pos : 0u;
+        // the last row whose start is <= pq has pq inside it (zero-length
+        // rows share their start with the next row)
+        uint32_t r = 0;
+#pragma unroll
+        for (uint32_t step = kWave / 2; step; step >>= 1)
+          if (__shfl(fstart, static_cast<int>(r + step), kWave) <= pq) r += step;
+        const uint32_t rs = __shfl(fstart, static_cast<int>(r), kWave);
+        const uint32_t rx = __shfl(rb.x, static_cast<int>(r), kWave);
+        const uint32_t ro = __shfl(o, static_cast<int>(r), kWave);
+        at[q] = in ? static_cast<uint64_t>(ro) + (pq - rs) : out_cap;
+        x[q] = in ? arena[rx + (pq - rs)] : 0u;
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < kFlat; ++q)
+        if (at[q] < out_cap) out[at[q]] = x[q];  // the host checks the total
+    }
+    // short rows with deleted entries, by groups
+    if (!__ballot(dirty)) continue;
     for (unsigned r = 0; r < kWave / kPackGroup; ++r) {
       // row handled by group grp in this round: lane grp * 8 + r
       const int src = static_cast<int>(grp * kPackGroup + r);
       const uint32_t bx = __shfl(rb.x, src, kWave);
-      const uint32_t by = __shfl(shrt ? rb.y : 0u, src, kWave);
+      const uint32_t by = __shfl(dirty ? rb.y : 0u, src, kWave);
       uint64_t og = __shfl(o, src, kWave);
       const uint32_t most = wave_max(by);
-      for (uint32_t j = 0; j < most; j += kPackGroup) {
-        const bool in = j + gl < by;
-        const uint32_t x = in ? arena[bx + j + gl] : kTomb;
-        const bool keep = !(x & kTomb);
+      if (most == 0) continue;
+      // the row's entries (at most kPackShort, kPer per lane) are all
+      // loaded before the first is placed
+      constexpr uint32_t kPer = kPackShort / kPackGroup;
+      uint32_t x[kPer];
+#pragma unroll
+      for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t j = q * kPackGroup + gl;
+        x[q] = j < by ? arena[bx + j] : kTomb;
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < kPer; ++q) {
+        if (q * kPackGroup >= most) break;
+        const bool keep = !(x[q] & kTomb);
         const unsigned long long m = (__ballot(keep) >> (grp * kPackGroup)) & 0xFFull;
         const uint64_t at = og + __popcll(m & ((1ull << gl) - 1ull));
-        if (keep && at < out_cap) out[at] = x;  // the host checks the total
+        if (keep && at < out_cap) out[at] = x[q];  // the host checks the total
         og += __popcll(m);
       }
     }
@@ -670,29 +719,37 @@ __global__ void k_pack_chunk_fill(const uint32_t* __restrict__ first, uint64_t n
   }
 }
 
-// Live entries of every chunk slot up to `slots` (0 past the last chunk):
-// counted for rows with deleted entries, the chunk's length otherwise.
+// Live entries of every chunk, a wave per chunk: counted for rows with
+// deleted entries, the chunk's length otherwise.  Slots past the last chunk
+// are left as they are: the scan's results there are never read.
 __global__ void k_pack_chunk_live(const uint2* __restrict__ row, const uint32_t* __restrict__ arena,
                                   const uint32_t* __restrict__ deg, const uint32_t* __restrict__ chunk_u,
                                   const uint32_t* __restrict__ chunk_j, const unsigned long long* counts,
-                                  uint32_t slots, uint32_t* __restrict__ live) {
+                                  uint32_t* __restrict__ live) {
   const uint32_t nch = static_cast<uint32_t>(counts[0]);
+  const unsigned lane = lane_id();
   const uint32_t W = gridDim.x * (blockDim.x / kWave);
-  for (uint32_t c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < slots; c += W) {
-    uint32_t k = 0;
-    if (c < nch) {
-      const uint32_t u = chunk_u[c];
-      const uint2 rb = row[u];
-      const uint32_t b = chunk_j[c] * kPackChunk;
-      const uint32_t e = min(b + kPackChunk, rb.y);
-      if (deg[u] == rb.y) {
-        k = e - b;
-      } else {
-        for (uint32_t i = b + lane_id(); i < e; i += kWave) k += (arena[rb.x + i] & kTomb) ? 0u : 1u;
-        k = wave_sum(k);
+  for (uint32_t c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nch; c += W) {
+    const uint32_t u = chunk_u[c];
+    const uint2 rb = row[u];
+    const uint32_t b = chunk_j[c] * kPackChunk;
+    const uint32_t len = min(b + kPackChunk, rb.y) - b;
+    uint32_t k = len;
+    if (deg[u] != rb.y) {
+      constexpr uint32_t kPer = kPackChunk / kWave;
+      const uint32_t* src = arena + rb.x + b;
+      uint32_t x[kPer];
+#pragma unroll
+      for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t i = q * kWave + lane;
+        x[q] = i < len ? src[i] : kTomb;
       }
+      k = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < kPer; ++q) k += (x[q] & kTomb) ? 0u : 1u;
+      k = wave_sum(k);
     }
-    if (lane_id() == 0) live[c] = k;
+    if (lane == 0) live[c] = k;
   }
 }
 
@@ -712,20 +769,33 @@ __global__ void k_pack_chunk_write(const uint2* __restrict__ row, const uint32_t
     uint64_t o = static_cast<uint64_t>(off[u]) + (lscan[c] - lscan[c - j]);  // the row's chunks are consecutive
     const uint32_t b = j * kPackChunk;
     const uint32_t e = min(b + kPackChunk, rb.y);
+    // the chunk's entries (at most kPackChunk = 16 per lane) are all loaded
+    // before any is stored
+    constexpr uint32_t kPer = kPackChunk / kWave;
+    const uint32_t len = e - b;
+    const uint32_t* src = arena + rb.x + b;
+    uint32_t x[kPer];
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+      const uint32_t i = q * kWave + lane;
+      x[q] = i < len ? src[i] : kTomb;
+    }
     if (deg[u] == rb.y) {
-      if (o + (e - b) <= out_cap) {  // the host checks the total
-#pragma unroll 4
-        for (uint32_t i = b + lane; i < e; i += kWave) out[o + (i - b)] = arena[rb.x + i];
+      if (o + len <= out_cap) {  // the host checks the total
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; ++q) {
+          const uint32_t i = q * kWave + lane;
+          if (i < len) out[o + i] = x[q];
+        }
       }
       continue;
     }
-    for (uint32_t i = b; i < e; i += kWave) {
-      const bool in = i + lane < e;
-      const uint32_t x = in ? arena[rb.x + i + lane] : kTomb;
-      const bool keep = !(x & kTomb);
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+      const bool keep = !(x[q] & kTomb);
       const unsigned long long m = __ballot(keep);
       const uint64_t at = o + lanes_below(m);
-      if (keep && at < out_cap) out[at] = x;  // the host checks the total
+      if (keep && at < out_cap) out[at] = x[q];  // the host checks the total
       o += __popcll(m);
     }
   }
@@ -1388,8 +1458,8 @@ static void graph_apply(vgx_graph* g, const uint32_t* src, const uint32_t* dst, 
 // Chunked copy of the rows longer than kPackShort; the chunk table is built
 // on the device, so packing needs no host round trip.  `stored` bounds the
 // entries of all rows (the arena's size does).
-static void pack_chunked_rows(vgx_graph* g, Adjacency& a, const uint32_t* off, uint32_t* out, uint64_t out_cap,
-                              uint64_t stored) {
+// Chunk table and live counts of the rows longer than kPackShort.
+static void pack_chunk_prep(vgx_graph* g, Adjacency& a, uint64_t stored) {
   hipStream_t s = g->stream;
   const uint64_t n = g->n;
   PackScratch& pk = g->pk;
@@ -1413,14 +1483,20 @@ static void pack_chunked_rows(vgx_graph* g, Adjacency& a, const uint32_t* off, u
   VGX_LAUNCH_CHECK();
   const unsigned grid = grid_for(max_chunks, 4, 4096);
   k_pack_chunk_live<<<grid, kBlock, 0, s>>>(a.row.p, a.arena.p, a.deg.p, pk.chunk_u.p, pk.chunk_j.p, pk.counts.p,
-                                            static_cast<uint32_t>(max_chunks + 1), pk.chunk_live.p);
+                                            pk.chunk_live.p);
   VGX_LAUNCH_CHECK();
   with_temp(g, g->bs[0].tmp, [&](void* t, size_t& b) {
     return hipcub::DeviceScan::ExclusiveSum(t, b, pk.chunk_live.p, pk.chunk_scan.p,
                                             static_cast<int>(max_chunks + 1), s);
   });
-  k_pack_chunk_write<<<grid, kBlock, 0, s>>>(a.row.p, a.arena.p, a.deg.p, off, pk.chunk_u.p, pk.chunk_j.p,
-                                             pk.chunk_scan.p, pk.counts.p, out, out_cap);
+}
+
+static void pack_chunk_write(vgx_graph* g, Adjacency& a, const uint32_t* off, uint32_t* out, uint64_t out_cap,
+                             uint64_t stored) {
+  PackScratch& pk = g->pk;
+  const uint64_t max_chunks = stored / kPackChunk + stored / kPackShort + 2;
+  k_pack_chunk_write<<<grid_for(max_chunks, 4, 4096), kBlock, 0, g->stream>>>(
+      a.row.p, a.arena.p, a.deg.p, off, pk.chunk_u.p, pk.chunk_j.p, pk.chunk_scan.p, pk.counts.p, out, out_cap);
   VGX_LAUNCH_CHECK();
 }
 
@@ -1445,7 +1521,8 @@ static void pack_direction(vgx_graph* g, int d, DevBuf<uint32_t>& off, DevBuf<ui
     k_pack_rows<<<grid_for(n, 4 * kWave, 4096), kBlock, 0, s>>>(a.row.p, a.arena.p, n, off.p, out.p, out.n,
                                                                 new_row, a.deg.p);
     VGX_LAUNCH_CHECK();
-    pack_chunked_rows(g, a, off.p, out.p, out.n, a.arena.n);
+    pack_chunk_prep(g, a, a.arena.n);
+    pack_chunk_write(g, a, off.p, out.p, out.n, a.arena.n);
   }
 }
 
