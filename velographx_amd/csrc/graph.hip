@@ -177,32 +177,38 @@ __device__ __forceinline__ uint32_t row_lower_bound(const uint32_t* arena, uint2
 }
 
 // A batch of up to kSortSmall arcs is sorted by one workgroup in LDS (one
-// launch instead of the device-wide sort's histogram and digit passes):
+// launch instead of the device-wide sort's histogram and digit passes),
+// the workgroup sized to the batch (64, 256 or 1024 threads of four keys):
 // keys and op indices in blocked order, padded with invalid keys, through a
-// stable block radix sort over the key's low end_bit bits, so the ops on an
-// arc stay in batch order.
-constexpr uint32_t kSortSmallThreads = 1024, kSortSmallItems = 4;
-constexpr uint64_t kSortSmall = kSortSmallThreads * kSortSmallItems;
-__global__ __launch_bounds__(kSortSmallThreads) void k_sort_small(const unsigned long long* __restrict__ keys,
-                                                                  const uint32_t* __restrict__ vals, uint32_t M,
-                                                                  int end_bit, unsigned long long* __restrict__ skeys,
-                                                                  uint32_t* __restrict__ svals) {
-  using Sort = hipcub::BlockRadixSort<unsigned long long, kSortSmallThreads, kSortSmallItems, uint32_t>;
+// stable block radix sort, so the ops on an arc stay in batch order.  The
+// sort runs over a compact key (u << b) | v with b = bits of the vertex
+// count, and an invalid key as 1 << 2b, so 2b + 1 bits instead of 32 + b.
+constexpr uint32_t kSortSmallItems = 4;
+constexpr uint64_t kSortSmall = 1024 * kSortSmallItems;
+template <uint32_t kThreads>
+__global__ __launch_bounds__(kThreads) void k_sort_small(const unsigned long long* __restrict__ keys,
+                                                         const uint32_t* __restrict__ vals, uint32_t M, int b,
+                                                         unsigned long long* __restrict__ skeys,
+                                                         uint32_t* __restrict__ svals) {
+  using Sort = hipcub::BlockRadixSort<unsigned long long, kThreads, kSortSmallItems, uint32_t>;
   __shared__ typename Sort::TempStorage tmp;
+  const unsigned long long inv = 1ull << (2 * b);
+  const unsigned long long low = (1ull << b) - 1ull;
   unsigned long long k[kSortSmallItems];
   uint32_t v[kSortSmallItems];
 #pragma unroll
   for (uint32_t j = 0; j < kSortSmallItems; ++j) {
     const uint32_t i = threadIdx.x * kSortSmallItems + j;
-    k[j] = i < M ? keys[i] : kInvalidKey;
+    const unsigned long long x = i < M ? keys[i] : kInvalidKey;
+    k[j] = x == kInvalidKey ? inv : (((x >> 32) << b) | (x & 0xFFFFFFFFull));
     v[j] = i < M ? vals[i] : 0u;
   }
-  Sort(tmp).Sort(k, v, 0, end_bit);
+  Sort(tmp).Sort(k, v, 0, 2 * b + 1);
 #pragma unroll
   for (uint32_t j = 0; j < kSortSmallItems; ++j) {
     const uint32_t i = threadIdx.x * kSortSmallItems + j;
     if (i < M) {
-      skeys[i] = k[j];
+      skeys[i] = k[j] == inv ? kInvalidKey : (((k[j] >> b) << 32) | (k[j] & low));
       svals[i] = v[j];
     }
   }
@@ -1304,8 +1310,14 @@ static void stage_sorted(vgx_graph* g, uint64_t k, bool defer) {
                                                   b.vals.p);
     VGX_LAUNCH_CHECK();
     if (M <= kSortSmall) {
-      k_sort_small<<<1, kSortSmallThreads, 0, s>>>(b.keys.p, b.vals.p, static_cast<uint32_t>(M), end_bit, b.skeys.p,
-                                                   b.svals.p);
+      const uint32_t m32 = static_cast<uint32_t>(M);
+      const int vb = bits_for(g->n);
+      if (M <= 64 * kSortSmallItems)
+        k_sort_small<64><<<1, 64, 0, s>>>(b.keys.p, b.vals.p, m32, vb, b.skeys.p, b.svals.p);
+      else if (M <= 256 * kSortSmallItems)
+        k_sort_small<256><<<1, 256, 0, s>>>(b.keys.p, b.vals.p, m32, vb, b.skeys.p, b.svals.p);
+      else
+        k_sort_small<1024><<<1, 1024, 0, s>>>(b.keys.p, b.vals.p, m32, vb, b.skeys.p, b.svals.p);
       VGX_LAUNCH_CHECK();
     } else {
       with_temp(g, b.tmp, [&](void* t, size_t& bytes) {
