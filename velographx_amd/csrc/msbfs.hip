@@ -742,6 +742,11 @@ static void ms_run(vgx_msbfs* m) {
 // 64-bit ballot turns a lane test back into a mask.
 
 constexpr uint32_t kMsrWindow = 16;  // items a wave takes at a time from a plain list
+// Edges a wave takes at least in the edge-balanced repair steps: each edge
+// is a whole wave's work (one lane per root), so a small frontier is spread
+// over eight-edge ranges, one group of eight pairs per wave, rather than
+// 64-edge windows that leave most waves idle.
+constexpr uint32_t kMsrQuantum = 8;
 constexpr uint32_t kMsrHead = 64;    // in-arcs of a row the first support scan reads
 constexpr unsigned kMsrBlocks = 1024;
 
@@ -771,40 +776,102 @@ __device__ __forceinline__ void msr_windows(uint64_t count, Fn&& fn) {
     fn(w * kMsrWindow);
 }
 
+// Eight pairs of a wave, as msr_groups8 loads them: the pair's lane t
+// (-1: no pair), its ends a and b, the rows d[a][lane] and d[b][lane] and
+// the masks ma[a] and mb[b] (0 when not asked for).
+struct MsrGroup {
+  int t[8];
+  uint32_t a[8], b[8], da[8], db[8];
+  unsigned long long ma[8], mb[8];
+};
+
 // Walks the pairs (a, b) a wave holds one per lane (b = kTomb: none) in
-// groups of eight: the eight pairs' rows d[a][lane], d[b][lane] and masks
-// ma[a], mb[b] (when given) are loaded together before body(t, a, b, da,
-// db, ma, mb) runs for each pair in lane order, so a wave keeps sixteen row
-// loads in flight instead of two.
-template <class Body>
-__device__ __forceinline__ void msr_pairs8(const uint32_t* __restrict__ d, const unsigned long long* ma,
-                                           const unsigned long long* mb, uint32_t av, uint32_t bv, Body&& body) {
+// groups of eight: the eight pairs' rows and masks are loaded together
+// before group(g) runs, so a wave keeps sixteen row loads in flight
+// instead of two.
+template <class Group>
+__device__ __forceinline__ void msr_groups8(const uint32_t* __restrict__ d, const unsigned long long* ma,
+                                            const unsigned long long* mb, uint32_t av, uint32_t bv,
+                                            Group&& group) {
   const unsigned lane = lane_id();
   unsigned long long todo = __ballot(!(bv & kTomb));
   while (todo) {
-    int tt[8];
+    MsrGroup g;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      tt[i] = todo ? __ffsll(static_cast<long long>(todo)) - 1 : -1;
+      g.t[i] = todo ? __ffsll(static_cast<long long>(todo)) - 1 : -1;
       todo = todo ? todo & (todo - 1) : 0ull;
     }
-    uint32_t A[8], B[8], DA[8], DB[8];
-    unsigned long long MA[8], MB[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      const int t = tt[i] < 0 ? 0 : tt[i];
-      A[i] = uni(__shfl(av, t, kWave));
-      B[i] = uni(__shfl(bv, t, kWave));
-      const bool on = tt[i] >= 0;
-      DA[i] = on ? d[A[i] * 64ull + lane] : kInf;
-      DB[i] = on ? d[B[i] * 64ull + lane] : kInf;
-      MA[i] = on && ma ? ma[A[i]] : 0ull;
-      MB[i] = on && mb ? mb[B[i]] : 0ull;
+      const int t = g.t[i] < 0 ? 0 : g.t[i];
+      g.a[i] = uni(__shfl(av, t, kWave));
+      g.b[i] = uni(__shfl(bv, t, kWave));
+      const bool on = g.t[i] >= 0;
+      g.da[i] = on ? d[g.a[i] * 64ull + lane] : kInf;
+      g.db[i] = on ? d[g.b[i] * 64ull + lane] : kInf;
+      g.ma[i] = on && ma ? ma[g.a[i]] : 0ull;
+      g.mb[i] = on && mb ? mb[g.b[i]] : 0ull;
     }
+    group(g);
+  }
+}
+
+// The same with body(t, a, b, da, db, ma, mb) run for each pair in lane
+// order.
+template <class Body>
+__device__ __forceinline__ void msr_pairs8(const uint32_t* __restrict__ d, const unsigned long long* ma,
+                                           const unsigned long long* mb, uint32_t av, uint32_t bv, Body&& body) {
+  msr_groups8(d, ma, mb, av, bv, [&](const MsrGroup& g) {
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-      if (tt[i] >= 0) body(tt[i], A[i], B[i], DA[i], DB[i], MA[i], MB[i]);
+      if (g.t[i] >= 0) body(g.t[i], g.a[i], g.b[i], g.da[i], g.db[i], g.ma[i], g.mb[i]);
+  });
+}
+
+// Claims for the eight pairs of a group: lane 0 ORs bits[i] (wave-uniform)
+// into claim[b[i]], the eight atomics issued before any result is read;
+// lane t[i] learns whether its pair set the first bits of b[i] (then it
+// lists b[i]: want, pv), and `count` adds every claimed bit.
+__device__ __forceinline__ void msr_claim8(unsigned long long* claim, const MsrGroup& g,
+                                           const unsigned long long (&bits)[8], bool& want, uint32_t& pv,
+                                           unsigned long long& count) {
+  const unsigned lane = lane_id();
+  unsigned long long old[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    old[i] = 0ull;
+    if (bits[i] && lane == 0) old[i] = atomicOr(&claim[g.b[i]], bits[i]);
   }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    if (!bits[i]) continue;
+    count += __popcll(bits[i]);
+    const unsigned long long o = __shfl(old[i], 0, kWave);
+    if (!o && lane == static_cast<unsigned>(g.t[i])) {
+      want = true;
+      pv = g.b[i];
+    }
+  }
+}
+
+// Relaxations d_r(b) = min(d_r(b), d_r(a) + 1) of the pairs of a group on
+// the roots in `roots` (per pair, lane mask) with d_r(a) + 1 < d_r(b): the
+// eight atomics issued before any result is read; bits[i] = the roots
+// whose distance the pair lowered.
+template <class Roots>
+__device__ __forceinline__ void msr_relax8(uint32_t* __restrict__ d, const MsrGroup& g, Roots&& roots,
+                                           unsigned long long (&bits)[8]) {
+  const unsigned lane = lane_id();
+  bool c[8];
+  uint32_t ret[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    c[i] = g.t[i] >= 0 && roots(i) && g.da[i] != kInf && g.da[i] + 1 < g.db[i];
+    ret[i] = c[i] ? atomicMin(&d[g.b[i] * 64ull + lane], g.da[i] + 1) : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) bits[i] = g.t[i] >= 0 ? __ballot(c[i] && ret[i] > g.da[i] + 1) : 0ull;
 }
 
 // Heads of deleted arcs u->v that were shortest-parent arcs for some root:
@@ -827,19 +894,13 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_candidates(const MsrArgs* __re
     }
     bool want = false;
     uint32_t pv = 0;
-    msr_pairs8(a.d, nullptr, nullptr, u, v,
-               [&](int t, uint32_t, uint32_t vt, uint32_t du, uint32_t dv, unsigned long long, unsigned long long) {
-                 const unsigned long long bits = __ballot(du != kInf && du + 1 == dv && myroot != vt);
-                 if (!bits) return;
-                 ncand += __popcll(bits);
-                 unsigned long long old = 0;
-                 if (lane == 0) old = atomicOr(&a.cand[vt], bits);
-                 old = __shfl(old, 0, kWave);
-                 if (!old && lane == static_cast<unsigned>(t)) {
-                   want = true;
-                   pv = vt;
-                 }
-               });
+    msr_groups8(a.d, nullptr, nullptr, u, v, [&](const MsrGroup& g) {
+      unsigned long long bits[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        bits[i] = g.t[i] >= 0 ? __ballot(g.da[i] != kInf && g.da[i] + 1 == g.db[i] && myroot != g.b[i]) : 0ull;
+      msr_claim8(a.cand, g, bits, want, pv, ncand);
+    });
     push_plain(&a.st->tq[0], a.tl[0], want, pv);
   });
   if (lane == 0 && ncand) atomicAdd(&a.st->candidates, ncand);
@@ -920,16 +981,18 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_support(const MsrArgs* __restr
   const uint32_t W = gridDim.x * kMsWaves;
   const uint32_t gw = blockIdx.x * kMsWaves + threadIdx.x / kWave;
   const unsigned lane = lane_id();
-  lbs_edges(list, pre, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.in_row, gw, W,
+  lbs_edges<false, kMsrQuantum>(list, pre, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.in_row, gw, W,
             [&](uint32_t, uint32_t v, uint2, uint32_t slot, bool valid) {
-              // a vertex whose roots are all supported by now sends no loads
-              const bool open = valid && (a.tm[v] & ~a.sup[v]) != 0ull;
+              // the roots of v still without support when its lane looked (a
+              // vertex whose roots are all supported by then sends no loads);
+              // a pair reads them from its lane instead of loading sup again
+              // behind the previous pair's atomic
+              const unsigned long long open = valid ? (a.tm[v] & ~a.sup[v]) : 0ull;
               const uint32_t w = open ? a.in_arena[slot + skip] : kTomb;
-              msr_pairs8(a.d, a.tm, a.aff, v, w,
-                         [&](int, uint32_t vt, uint32_t, uint32_t dv, uint32_t dw, unsigned long long tm,
+              msr_pairs8(a.d, nullptr, a.aff, v, w,
+                         [&](int t, uint32_t vt, uint32_t, uint32_t dv, uint32_t dw, unsigned long long,
                              unsigned long long affw) {
-                           const unsigned long long need = tm & ~a.sup[vt];
-                           if (!need) return;
+                           const unsigned long long need = __shfl(open, t, kWave);
                            const unsigned long long bits =
                                __ballot(lane_bit(need) && dw != kInf && dw + 1 == dv && !lane_bit(affw));
                            if (bits && lane == 0) atomicOr(&a.sup[vt], bits);
@@ -985,24 +1048,21 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_emit(const MsrArgs* __restrict
   const uint32_t W = gridDim.x * kMsWaves;
   const uint32_t gw = blockIdx.x * kMsWaves + threadIdx.x / kWave;
   const unsigned lane = lane_id();
-  lbs_edges(a.el[par], a.epre[par], static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.out_row, gw, W,
+  lbs_edges<false, kMsrQuantum>(a.el[par], a.epre[par], static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.out_row, gw, W,
             [&](uint32_t, uint32_t u, uint2, uint32_t slot, bool valid) {
               const uint32_t x = valid ? a.out_arena[slot] : kTomb;
               bool want = false;
               uint32_t px = 0;
-              msr_pairs8(a.d, a.nm, nullptr, u, x,
-                         [&](int t, uint32_t, uint32_t xt, uint32_t du, uint32_t dx, unsigned long long nm,
-                             unsigned long long) {
-                           const unsigned long long bits = __ballot(lane_bit(nm) && du != kInf && dx == du + 1);
-                           if (!bits) return;
-                           unsigned long long old = 0;
-                           if (lane == 0) old = atomicOr(&a.cand[xt], bits);
-                           old = __shfl(old, 0, kWave);
-                           if (!old && lane == static_cast<unsigned>(t)) {
-                             want = true;
-                             px = xt;
-                           }
-                         });
+              unsigned long long claimed = 0;
+              msr_groups8(a.d, a.nm, nullptr, u, x, [&](const MsrGroup& g) {
+                unsigned long long bits[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                  bits[i] = g.t[i] >= 0
+                                ? __ballot(lane_bit(g.ma[i]) && g.da[i] != kInf && g.db[i] == g.da[i] + 1)
+                                : 0ull;
+                msr_claim8(a.cand, g, bits, want, px, claimed);
+              });
               push_plain(&a.st->tq[par ^ 1], a.tl[par ^ 1], want, px);
             });
 }
@@ -1038,7 +1098,7 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_seed(const MsrArgs* __restrict
   const uint32_t W = gridDim.x * kMsWaves;
   const uint32_t gw = blockIdx.x * kMsWaves + threadIdx.x / kWave;
   const unsigned lane = lane_id();
-  lbs_edges(a.seed, a.seedpre, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.in_row, gw, W,
+  lbs_edges<false, kMsrQuantum>(a.seed, a.seedpre, static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.in_row, gw, W,
             [&](uint32_t, uint32_t v, uint2, uint32_t slot, bool valid) {
               const uint32_t w = valid ? a.in_arena[slot] : kTomb;
               msr_pairs8(a.d, a.aff, a.aff, v, w,
@@ -1100,21 +1160,11 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_relax_additions(const MsrArgs*
     }
     bool want = false;
     uint32_t pv = 0;
-    msr_pairs8(a.d, nullptr, nullptr, u, v,
-               [&](int t, uint32_t, uint32_t vt, uint32_t du, uint32_t dv, unsigned long long, unsigned long long) {
-                 bool lowered = false;
-                 if (du != kInf && du + 1 < dv) lowered = atomicMin(&a.d[vt * 64ull + lane], du + 1) > du + 1;
-                 const unsigned long long bits = __ballot(lowered);
-                 if (!bits) return;
-                 low += __popcll(bits);
-                 unsigned long long old = 0;
-                 if (lane == 0) old = atomicOr(&a.pend[vt], bits);
-                 old = __shfl(old, 0, kWave);
-                 if (!old && lane == static_cast<unsigned>(t)) {
-                   want = true;
-                   pv = vt;
-                 }
-               });
+    msr_groups8(a.d, nullptr, nullptr, u, v, [&](const MsrGroup& g) {
+      unsigned long long bits[8];
+      msr_relax8(a.d, g, [](int) { return true; }, bits);
+      msr_claim8(a.pend, g, bits, want, pv, low);
+    });
     push_plain(&a.st->fq[0], a.fl[0], want, pv);
   });
   if (lane == 0 && low) atomicAdd(&a.st->lowered, low);
@@ -1157,28 +1207,16 @@ __global__ __launch_bounds__(kMsBlock) void k_msr_propagate(const MsrArgs* __res
   const uint32_t gw = blockIdx.x * kMsWaves + threadIdx.x / kWave;
   const unsigned lane = lane_id();
   unsigned long long low = 0;
-  lbs_edges(a.pl[par], a.ppre[par], static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.out_row, gw, W,
+  lbs_edges<false, kMsrQuantum>(a.pl[par], a.ppre[par], static_cast<uint32_t>(qc >> kEdgeBits), qc & kEdgeMask, a.out_row, gw, W,
             [&](uint32_t, uint32_t u, uint2, uint32_t slot, bool valid) {
               const uint32_t x = valid ? a.out_arena[slot] : kTomb;
               bool want = false;
               uint32_t px = 0;
-              msr_pairs8(a.d, a.pm, nullptr, u, x,
-                         [&](int t, uint32_t, uint32_t xt, uint32_t du, uint32_t dx, unsigned long long pm,
-                             unsigned long long) {
-                           bool lowered = false;
-                           if (lane_bit(pm) && du != kInf && du + 1 < dx)
-                             lowered = atomicMin(&a.d[xt * 64ull + lane], du + 1) > du + 1;
-                           const unsigned long long bits = __ballot(lowered);
-                           if (!bits) return;
-                           low += __popcll(bits);
-                           unsigned long long old = 0;
-                           if (lane == 0) old = atomicOr(&a.pend[xt], bits);
-                           old = __shfl(old, 0, kWave);
-                           if (!old && lane == static_cast<unsigned>(t)) {
-                             want = true;
-                             px = xt;
-                           }
-                         });
+              msr_groups8(a.d, a.pm, nullptr, u, x, [&](const MsrGroup& g) {
+                unsigned long long bits[8];
+                msr_relax8(a.d, g, [&](int i) { return lane_bit(g.ma[i]); }, bits);
+                msr_claim8(a.pend, g, bits, want, px, low);
+              });
               push_plain(&a.st->fq[par ^ 1], a.fl[par ^ 1], want, px);
             });
   if (lane == 0 && low) atomicAdd(&a.st->lowered, low);

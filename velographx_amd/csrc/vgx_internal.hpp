@@ -390,13 +390,17 @@ __device__ __forceinline__ void wave_push(unsigned long long* ctr, uint32_t* que
 // With Begins set the queue holds each item's row begin instead of its
 // vertex (row is not read; fn gets the begin as both vertex and row.x), so
 // no dependent row load stands between an item and its arena entries.
-template <bool Begins = false, class Fn>
+// A wave's range is a multiple of Quantum edges (a full window by default;
+// less spreads a small frontier over more waves when fn does a wave's work
+// per edge).
+template <bool Begins = false, uint32_t Quantum = kWave, class Fn>
 __device__ __forceinline__ void lbs_edges(const uint32_t* queue, const unsigned long long* pre, uint32_t F,
                                           unsigned long long E, const uint2* row, uint32_t gw, uint32_t W,
                                           Fn&& fn) {
+  static_assert(Quantum > 0 && Quantum <= kWave && (Quantum & (Quantum - 1)) == 0, "quantum: a power of two <= 64");
   if (E == 0) return;
   unsigned long long S = (E + W - 1) / W;
-  S = (S + kWave - 1) & ~static_cast<unsigned long long>(kWave - 1);
+  S = (S + Quantum - 1) & ~static_cast<unsigned long long>(Quantum - 1);
   unsigned long long a = static_cast<unsigned long long>(gw) * S;
   if (a >= E) return;
   unsigned long long b = a + S < E ? a + S : E;
