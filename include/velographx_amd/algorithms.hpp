@@ -21,8 +21,9 @@ namespace velographx {
 // reached); a source outside the graph throws std::out_of_range.
 inline std::vector<std::uint32_t> bfs_distances(const CsrGraph& graph, VertexId source) {
   if (source >= graph.vertex_count()) throw std::out_of_range("source outside graph");
-  IncrementalBFS bfs(graph.device_graph(), source);
-  return bfs.distances();
+  std::vector<std::uint32_t> d(graph.vertex_count());
+  graph.bfs_distances_into(source, d.data());
+  return d;
 }
 
 // algorithms.cpp:28-54: component ids 0, 1, ... numbered in the order of

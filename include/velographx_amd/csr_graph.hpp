@@ -11,6 +11,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <utility>
 #include <vector>
@@ -56,11 +58,48 @@ class CsrGraph {
   // they touch).
   [[nodiscard]] DynamicGraph& device_graph() const noexcept { return g_; }
 
+  // Hop distances from `source` (< vertex_count()) into out[0, vertex_count()),
+  // for algorithms.hpp's bfs_distances.  The graph keeps one traversal
+  // object, re-targeted per call, so a call costs a traversal and the
+  // copy-out rather than a new workspace and a first, uncaptured run; a
+  // concurrent call that finds it busy uses an object of its own.
+  void bfs_distances_into(VertexId source, std::uint32_t* out) const {
+    const std::uint64_t n = vertex_count();
+    std::uint64_t count = 0;
+    std::unique_lock<std::mutex> lock(bfs_->mu, std::try_to_lock);
+    if (!lock.owns_lock()) {
+      BfsHandle own;
+      vgx_check(vgx_bfs_create(g_.handle(), source, 0.35, &own.b));
+      vgx_check(vgx_bfs_distances(own.b, out, n, &count));
+      return;
+    }
+    if (!bfs_->h.b)
+      vgx_check(vgx_bfs_create(g_.handle(), source, 0.35, &bfs_->h.b));
+    else
+      vgx_check(vgx_bfs_set_source(bfs_->h.b, source));
+    vgx_check(vgx_bfs_distances(bfs_->h.b, out, n, &count));
+  }
+
  private:
   void check(VertexId v) const {
     if (v >= vertex_count()) throw std::out_of_range("vertex id outside graph");
   }
+  struct BfsHandle {
+    vgx_bfs* b = nullptr;
+    BfsHandle() = default;
+    BfsHandle(const BfsHandle&) = delete;
+    BfsHandle& operator=(const BfsHandle&) = delete;
+    ~BfsHandle() {
+      if (b) vgx_bfs_destroy(b);
+    }
+  };
+  struct BfsCache {
+    std::mutex mu;
+    BfsHandle h;
+  };
   mutable DynamicGraph g_{0, false};
+  // declared after g_, so the traversal object goes before the graph it reads
+  std::unique_ptr<BfsCache> bfs_ = std::make_unique<BfsCache>();
 };
 
 }  // namespace velographx

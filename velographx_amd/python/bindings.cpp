@@ -523,12 +523,16 @@ PYBIND11_MODULE(_velographx, m) {
   m.def(
       "bfs_distances",
       [](const velographx::CsrGraph& g, VertexId source) {
-        std::vector<std::uint32_t> d;
+        // algorithms.cpp:11-27; copied from the device straight into the
+        // returned array
+        if (source >= g.vertex_count()) throw std::out_of_range("source outside graph");
+        py::array_t<std::uint32_t> out(static_cast<py::ssize_t>(g.vertex_count()));
+        std::uint32_t* p = out.mutable_data();
         {
           py::gil_scoped_release nogil;
-          d = velographx::bfs_distances(g, source);
+          g.bfs_distances_into(source, p);
         }
-        return to_numpy(std::move(d));
+        return out;
       },
       py::arg("graph"), py::arg("source"));
   m.def(
