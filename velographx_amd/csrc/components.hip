@@ -7,13 +7,21 @@
 // vertex id, so the partition is the reference's and the label is a fixed,
 // order-free representative.
 //
-// Labels start at the smallest of each vertex and its neighbours (no
-// atomics), jumped to their roots.  Every round: each live arc (u, v) lowers the larger of the two current
-// labels' parent slot to the smaller label (atomicMin, from the row of the
-// endpoint holding the larger label), then every vertex jumps to its root.  Parent values only ever decrease and start at the
-// vertex itself, so parent[v] <= v and the pointers form a forest; a round
-// with no hook ends the loop, and then every vertex of a component points at
-// the component's smallest id (its own parent is itself).
+// Labels start at the vertex itself.  Every round: each live arc (u, v)
+// whose other end has the smaller label lowers u's root's parent slot to it
+// (atomicMin), then every vertex jumps to its root.  Parent values only ever
+// decrease and start at the vertex itself, so parent[v] <= v and the
+// pointers form a forest; a round in which no arc sees a smaller label ends
+// the loop (every arc is stored in both rows, so its two labels are then
+// equal), and then every vertex of a component points at the component's
+// smallest id (its own parent is itself).
+//
+// The arcs are walked edge-balanced, a lane per stored entry, over the list
+// of vertices with stored entries (built once per labelling by two scans);
+// the lanes of one row in a window take their minimum first, so a row sends
+// one atomic per window, and no lane waits for an atomic's result.
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -25,7 +33,6 @@ vgx_status set_error(vgx_status s, const std::string& msg);
 extern thread_local std::string t_last_error;
 
 constexpr int kCcBlock = 256;
-constexpr uint32_t kCcUnroll = 4;
 
 }  // namespace vgx
 
@@ -34,6 +41,11 @@ struct vgx_components {
   vgx::DevBuf<uint32_t> parent;
   vgx::DevBuf<uint32_t> aff;  // pre-batch labels whose component lost an arc
   vgx::DevBuf<unsigned long long> hooked;
+  // the edge-balanced loop's items: vertices with stored entries (queue)
+  // and their edge prefix (pre), from scans of flag and len
+  vgx::DevBuf<uint32_t> flag, fpos, queue;
+  vgx::DevBuf<unsigned long long> len, lpre, pre;
+  vgx::DevBuf<uint8_t> scan_tmp;
   uint64_t n = 0;
   uint64_t last_repaired = 0;
   uint32_t rounds = 0;
@@ -46,62 +58,68 @@ static unsigned cc_blocks(uint64_t items, unsigned per_block, unsigned cap) {
   return static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(b, cap)));
 }
 
-// Start labels without atomics: each vertex points at the smallest of itself
-// and its live neighbours (a vertex of its own component, never above it),
-// one wave per row.
-__global__ __launch_bounds__(256) void k_cc_init(const uint2* __restrict__ row, const uint32_t* __restrict__ arena,
-                                                 uint64_t n, uint32_t* __restrict__ parent) {
-  const uint64_t W = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
-  const unsigned lane = lane_id();
-  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += W) {
-    const uint2 rb = row[u];
-    uint32_t m = static_cast<uint32_t>(u);
-    // four loads in flight per lane: hub rows are bound by load latency
-    for (uint32_t j0 = 0; j0 < rb.y; j0 += kCcUnroll * kWave) {
-      uint32_t x[kCcUnroll];
-#pragma unroll
-      for (uint32_t q = 0; q < kCcUnroll; ++q) {
-        const uint32_t j = j0 + q * kWave + lane;
-        x[q] = j < rb.y ? arena[rb.x + j] : kTomb;  // past the row: skipped like a tombstone
-      }
-#pragma unroll
-      for (uint32_t q = 0; q < kCcUnroll; ++q)
-        if (!(x[q] & kTomb)) m = min(m, x[q]);
-    }
-    m = ~wave_max(~m);  // the wave's minimum
-    if (lane == 0) parent[u] = m;
+// flag[v] = 1 and len[v] = stored entries of v's row for v < n (0 past it,
+// which closes the scans), parent[v] = v.
+__global__ void k_cc_items(const uint2* __restrict__ row, uint64_t n, uint32_t* __restrict__ flag,
+                           unsigned long long* __restrict__ len, uint32_t* __restrict__ parent) {
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; v <= n;
+       v += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t l = v < n ? row[v].y : 0u;
+    flag[v] = l ? 1u : 0u;
+    len[v] = l;
+    if (v < n) parent[v] = static_cast<uint32_t>(v);
   }
 }
 
-// One wave per vertex u: a live arc (u, v) whose label is below u's lowers
-// u's label slot to it.  Each undirected arc is stored in both rows, so the
-// row of the endpoint with the larger label does the hook; a plain read
-// first skips the atomic where the slot is already low enough.
-__global__ __launch_bounds__(kCcBlock) void k_cc_hook(const uint2* __restrict__ row, const uint32_t* __restrict__ arena,
-                                                      uint64_t n, uint32_t* __restrict__ parent,
-                                                      unsigned long long* __restrict__ hooked) {
-  const uint64_t W = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
-  const unsigned lane = lane_id();
-  unsigned long long any = 0;
-  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += W) {
-    const uint2 rb = row[u];
-    const uint32_t pu = parent[u];
-    for (uint32_t j0 = 0; j0 < rb.y; j0 += kCcUnroll * kWave) {
-      uint32_t pv[kCcUnroll];
-#pragma unroll
-      for (uint32_t q = 0; q < kCcUnroll; ++q) {
-        const uint32_t j = j0 + q * kWave + lane;
-        const uint32_t x = j < rb.y ? arena[rb.x + j] : kTomb;
-        pv[q] = (x & kTomb) ? pu : parent[x];  // past the row or deleted: no hook
-      }
-#pragma unroll
-      for (uint32_t q = 0; q < kCcUnroll; ++q) {
-        if (pv[q] >= pu || parent[pu] <= pv[q]) continue;
-        if (atomicMin(&parent[pu], pv[q]) > pv[q]) any = 1;
-      }
-    }
+__global__ void k_cc_list(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ fpos,
+                          const unsigned long long* __restrict__ lpre, uint64_t n, uint32_t* __restrict__ queue,
+                          unsigned long long* __restrict__ pre) {
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; v < n;
+       v += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (!flag[v]) continue;
+    queue[fpos[v]] = static_cast<uint32_t>(v);
+    pre[fpos[v]] = lpre[v];
   }
-  if (__ballot(any != 0) && lane == 0) atomicAdd(hooked, 1ull);
+}
+
+// One round of hooks, a lane per stored entry (u, x): when x's label is
+// below u's, u's root slot is lowered to it.  The lanes of a row in one
+// window (a run of equal u) take their minimum and the run's last lane
+// sends the one atomic.  hooked counts the waves that saw a smaller label.
+__global__ __launch_bounds__(kCcBlock) void k_cc_hook(const uint32_t* __restrict__ queue,
+                                                      const unsigned long long* __restrict__ pre, uint32_t F,
+                                                      unsigned long long E, const uint2* __restrict__ row,
+                                                      const uint32_t* __restrict__ arena, uint32_t* __restrict__ parent,
+                                                      unsigned long long* __restrict__ hooked) {
+  const uint32_t W = gridDim.x * (kCcBlock / kWave);
+  const uint32_t gw = blockIdx.x * (kCcBlock / kWave) + threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  bool any = false;
+  lbs_edges(queue, pre, F, E, row, gw, W, [&](uint32_t, uint32_t u, uint2, uint32_t slot, bool valid) {
+    const uint32_t x = valid ? arena[slot] : kTomb;
+    const bool live = valid && !(x & kTomb);
+    const uint32_t pu = valid ? parent[u] : 0u;
+    const uint32_t px = live ? parent[x] : kInf;
+    uint32_t m = px < pu ? px : kInf;
+    // runs: consecutive valid lanes of one row (the valid lanes are a
+    // prefix of the wave; each invalid lane is a run of its own)
+    const uint32_t uprev = __shfl_up(u, 1, kWave);
+    const bool vprev = __shfl_up(static_cast<int>(valid), 1, kWave) != 0;
+    const unsigned long long starts = __ballot(lane == 0 || !valid || !vprev || uprev != u);
+    const unsigned long long upto = starts & (((1ull << lane) - 1ull) | (1ull << lane));
+    const unsigned first = 63u - static_cast<unsigned>(__clzll(static_cast<long long>(upto)));
+#pragma unroll
+    for (unsigned d = 1; d < kWave; d <<= 1) {
+      const uint32_t y = __shfl_up(m, d, kWave);
+      if (lane >= first + d) m = min(m, y);
+    }
+    const bool last = lane == kWave - 1 || ((starts >> (lane + 1)) & 1ull);
+    if (valid && last && m != kInf) {
+      atomicMin(&parent[pu], m);
+      any = true;
+    }
+  });
+  if (__ballot(any) && lane == 0) atomicAdd(hooked, 1ull);
 }
 
 // Pointer jumping to the root (parent[v] <= v, so every chain ends).
@@ -142,6 +160,14 @@ __global__ void k_cc_count_marked(uint64_t n_old, const uint32_t* __restrict__ p
   if (lane_id() == 0 && c) atomicAdd(count, c);
 }
 
+template <class F>
+static void cc_scan(vgx_components* cc, F&& f) {
+  size_t bytes = 0;
+  VGX_HIP(f(static_cast<void*>(nullptr), bytes));
+  cc->scan_tmp.ensure(bytes + 16);
+  VGX_HIP(f(static_cast<void*>(cc->scan_tmp.p), bytes));
+}
+
 static void cc_recompute(vgx_components* cc) {
   vgx_graph* g = cc->g;
   hipStream_t s = g->stream;
@@ -153,16 +179,38 @@ static void cc_recompute(vgx_components* cc) {
   cc->hooked.ensure(1);
   if (n == 0) return;
   const Adjacency& a = g->adj[0];
-  k_cc_init<<<cc_blocks(n, kCcBlock / kWave, 2048), kCcBlock, 0, s>>>(a.row.p, a.arena.p, n, cc->parent.p);
+  cc->flag.ensure(n + 1);
+  cc->fpos.ensure(n + 1);
+  cc->len.ensure(n + 1);
+  cc->lpre.ensure(n + 1);
+  cc->queue.ensure(n + 1);
+  cc->pre.ensure(n + 1);
+  k_cc_items<<<cc_blocks(n + 1, kCcBlock, 2048), kCcBlock, 0, s>>>(a.row.p, n, cc->flag.p, cc->len.p, cc->parent.p);
   VGX_LAUNCH_CHECK();
-  k_cc_compress<<<cc_blocks(n, kCcBlock, 2048), kCcBlock, 0, s>>>(n, cc->parent.p);
+  const int items = static_cast<int>(n + 1);
+  cc_scan(cc, [&](void* t, size_t& b) {
+    return hipcub::DeviceScan::ExclusiveSum(t, b, cc->flag.p, cc->fpos.p, items, s);
+  });
+  cc_scan(cc, [&](void* t, size_t& b) {
+    return hipcub::DeviceScan::ExclusiveSum(t, b, cc->len.p, cc->lpre.p, items, s);
+  });
+  k_cc_list<<<cc_blocks(n, kCcBlock, 2048), kCcBlock, 0, s>>>(cc->flag.p, cc->fpos.p, cc->lpre.p, n, cc->queue.p,
+                                                              cc->pre.p);
   VGX_LAUNCH_CHECK();
+  uint32_t F = 0;
+  unsigned long long E = 0;
+  VGX_HIP(hipMemcpyAsync(&F, cc->fpos.p + n, sizeof(F), hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipMemcpyAsync(&E, cc->lpre.p + n, sizeof(E), hipMemcpyDeviceToHost, s));
+  VGX_HIP(hipStreamSynchronize(s));
+  if (E == 0) return;
+  // about one wave per 64 entries, up to 2048 workgroups
+  const unsigned grid = cc_blocks(E, kCcBlock, 2048);
   // every round either hooks (a label strictly decreases) or ends the loop;
   // labels are bounded below, so the loop ends
   for (;;) {
     VGX_HIP(hipMemsetAsync(cc->hooked.p, 0, sizeof(unsigned long long), s));
-    k_cc_hook<<<cc_blocks(n, kCcBlock / kWave, 2048), kCcBlock, 0, s>>>(a.row.p, a.arena.p, n, cc->parent.p,
-                                                                       cc->hooked.p);
+    k_cc_hook<<<grid, kCcBlock, 0, s>>>(cc->queue.p, cc->pre.p, F, E, a.row.p, a.arena.p, cc->parent.p,
+                                        cc->hooked.p);
     VGX_LAUNCH_CHECK();
     k_cc_compress<<<cc_blocks(n, kCcBlock, 2048), kCcBlock, 0, s>>>(n, cc->parent.p);
     VGX_LAUNCH_CHECK();
