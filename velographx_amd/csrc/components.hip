@@ -7,7 +7,8 @@
 // vertex id, so the partition is the reference's and the label is a fixed,
 // order-free representative.
 //
-// Labels start at the vertex itself.  Every round: each live arc (u, v)
+// Labels start at the smallest of each vertex and its live neighbours,
+// jumped to their roots.  Every round: each live arc (u, v)
 // whose other end has the smaller label lowers u's root's parent slot to it
 // (atomicMin), then every vertex jumps to its root.  Parent values only ever
 // decrease and start at the vertex itself, so parent[v] <= v and the
@@ -86,6 +87,10 @@ __global__ void k_cc_list(const uint32_t* __restrict__ flag, const uint32_t* __r
 // below u's, u's root slot is lowered to it.  The lanes of a row in one
 // window (a run of equal u) take their minimum and the run's last lane
 // sends the one atomic.  hooked counts the waves that saw a smaller label.
+// With kStart (the first round, on identity labels) each row's run takes
+// the smallest of its live neighbours instead, and a run that holds the
+// whole row stores min(u, that) without an atomic.
+template <bool kStart>
 __global__ __launch_bounds__(kCcBlock) void k_cc_hook(const uint32_t* __restrict__ queue,
                                                       const unsigned long long* __restrict__ pre, uint32_t F,
                                                       unsigned long long E, const uint2* __restrict__ row,
@@ -95,12 +100,18 @@ __global__ __launch_bounds__(kCcBlock) void k_cc_hook(const uint32_t* __restrict
   const uint32_t gw = blockIdx.x * (kCcBlock / kWave) + threadIdx.x / kWave;
   const unsigned lane = lane_id();
   bool any = false;
-  lbs_edges(queue, pre, F, E, row, gw, W, [&](uint32_t, uint32_t u, uint2, uint32_t slot, bool valid) {
+  lbs_edges(queue, pre, F, E, row, gw, W, [&](uint32_t, uint32_t u, uint2 rb, uint32_t slot, bool valid) {
     const uint32_t x = valid ? arena[slot] : kTomb;
     const bool live = valid && !(x & kTomb);
-    const uint32_t pu = valid ? parent[u] : 0u;
-    const uint32_t px = live ? parent[x] : kInf;
-    uint32_t m = px < pu ? px : kInf;
+    uint32_t pu, m;
+    if constexpr (kStart) {
+      pu = u;
+      m = live ? x : kInf;
+    } else {
+      pu = valid ? parent[u] : 0u;
+      const uint32_t px = live ? parent[x] : kInf;
+      m = px < pu ? px : kInf;
+    }
     // runs: consecutive valid lanes of one row (the valid lanes are a
     // prefix of the wave; each invalid lane is a run of its own)
     const uint32_t uprev = __shfl_up(u, 1, kWave);
@@ -114,7 +125,18 @@ __global__ __launch_bounds__(kCcBlock) void k_cc_hook(const uint32_t* __restrict
       if (lane >= first + d) m = min(m, y);
     }
     const bool last = lane == kWave - 1 || ((starts >> (lane + 1)) & 1ull);
-    if (valid && last && m != kInf) {
+    if constexpr (kStart) {
+      // the run holds the whole row when its first entry is the row's first
+      // and its last the row's last
+      const uint32_t at = slot - rb.x;
+      const uint32_t at_first = __shfl(at, static_cast<int>(first), kWave);
+      if (valid && last) {
+        if (at_first == 0 && at + 1 == rb.y)
+          parent[u] = min(u, m);
+        else if (m < u)
+          atomicMin(&parent[u], m);
+      }
+    } else if (valid && last && m != kInf) {
       atomicMin(&parent[pu], m);
       any = true;
     }
@@ -205,12 +227,18 @@ static void cc_recompute(vgx_components* cc) {
   if (E == 0) return;
   // about one wave per 64 entries, up to 2048 workgroups
   const unsigned grid = cc_blocks(E, kCcBlock, 2048);
+  // start: every vertex at the smallest of itself and its live neighbours
+  k_cc_hook<true><<<grid, kCcBlock, 0, s>>>(cc->queue.p, cc->pre.p, F, E, a.row.p, a.arena.p, cc->parent.p,
+                                            cc->hooked.p);
+  VGX_LAUNCH_CHECK();
+  k_cc_compress<<<cc_blocks(n, kCcBlock, 2048), kCcBlock, 0, s>>>(n, cc->parent.p);
+  VGX_LAUNCH_CHECK();
   // every round either hooks (a label strictly decreases) or ends the loop;
   // labels are bounded below, so the loop ends
   for (;;) {
     VGX_HIP(hipMemsetAsync(cc->hooked.p, 0, sizeof(unsigned long long), s));
-    k_cc_hook<<<grid, kCcBlock, 0, s>>>(cc->queue.p, cc->pre.p, F, E, a.row.p, a.arena.p, cc->parent.p,
-                                        cc->hooked.p);
+    k_cc_hook<false><<<grid, kCcBlock, 0, s>>>(cc->queue.p, cc->pre.p, F, E, a.row.p, a.arena.p, cc->parent.p,
+                                               cc->hooked.p);
     VGX_LAUNCH_CHECK();
     k_cc_compress<<<cc_blocks(n, kCcBlock, 2048), kCcBlock, 0, s>>>(n, cc->parent.p);
     VGX_LAUNCH_CHECK();
