@@ -12,8 +12,8 @@
 // One iteration is two launches over the graph's live in-rows (deleted
 // entries tombstoned and skipped).  k_pr_iter splits the vertices by stored
 // row length: rows without entries take the base rank, rows of <= 32
-// entries are summed by groups of 8 lanes, rows up to 4096 by one wave, and
-// larger rows in 4096-entry chunks by several waves that leave partial
+// entries are summed by groups of 8 lanes, rows up to 1024 by one wave, and
+// larger rows in 1024-entry chunks by several waves that leave partial
 // sums; every workgroup leaves its residual and dangling partials.
 // k_pr_finish then sums each large row's chunk partials in chunk order and
 // its last workgroup reduces all partials in a fixed order, sets the next
@@ -37,7 +37,7 @@ constexpr int kPrBlock = 256;
 constexpr uint32_t kSmallRow = 32;
 // Lanes per short row in a sweep.
 constexpr uint32_t kSmallGroup = 8;
-constexpr uint32_t kChunk = 4096;
+constexpr uint32_t kChunk = 1024;
 // Workgroups per in-degree class and launch; each class loops grid-stride,
 // so the last workgroup reduces at most 3 x 1024 partials.
 constexpr uint64_t kMaxClassBlocks = 1024;
