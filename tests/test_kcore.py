@@ -192,10 +192,10 @@ def test_region_repair_on_separate_components(orc):
 
 
 @pytest.mark.skipif(not refdriver.available(), reason="oracle/_ref (compiled reference) not built")
-def test_bench_digests_equal_the_reference_drivers(orc):
+def test_bench_digests_equal_the_reference_programs(orc):
     """The digests bench.py compares at s20 (every core number; the
-    partition labelled by smallest members) are the reference driver's: on
-    a small graph the driver's digests of the reference's IncrementalKCore
+    partition labelled by smallest members) are those oracle/ref_driver.cpp
+    prints: on a small graph its digests of the reference's IncrementalKCore
     and IncrementalComponents equal bench.vertex_digest of the oracle's core
     numbers and smallest-member labels, and a single changed core number
     changes the digest."""
