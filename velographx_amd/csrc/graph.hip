@@ -21,6 +21,7 @@
 #include <array>
 #include <mutex>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
 #include "vgx_internal.hpp"
@@ -41,19 +42,141 @@ void throw_hip(hipError_t e, const char* what, const char* file, int line) {
   throw Error(VGX_DEVICE_ERROR, m);
 }
 
+// Freed device buffers of at least kCacheMinBytes are kept, per device, for
+// the next allocation of about their size: a snapshot or cutover builds a
+// graph of the size of the one it replaces, and each of its row tables,
+// degree arrays and arena would otherwise be a fresh hipMalloc (hundreds of
+// MB, and a wait for the queued work) right after the old graph's hipFree.
+// A buffer is cached only once the device is idle, the wait a hipFree makes,
+// so no queued kernel can still use it; an allocation takes the smallest
+// cached buffer of at least its size and at most half as large again.  At
+// most kCacheBuffers buffers and kCacheBytes bytes per device are kept (the
+// oldest go first); a failed allocation releases them all and retries.
+namespace {
+constexpr size_t kCacheMinBytes = size_t(1) << 20;
+constexpr size_t kCacheBytes = size_t(4) << 30;
+constexpr size_t kCacheBuffers = 32;
+struct CacheEntry {
+  int device;
+  void* p;
+  size_t bytes;
+};
+std::mutex g_cache_mu;
+// every cacheable live allocation's size and device (the free path needs
+// them), and the cache
+struct LiveBuf {
+  size_t bytes;
+  int device;
+};
+std::unordered_map<void*, LiveBuf>& live_sizes() {
+  static auto* m = new std::unordered_map<void*, LiveBuf>();  // lives to process exit
+  return *m;
+}
+std::vector<CacheEntry>& cache() {
+  static auto* v = new std::vector<CacheEntry>();
+  return *v;
+}
+void cache_release_all() {
+  std::vector<CacheEntry> drop;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    drop.swap(cache());
+  }
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (const CacheEntry& e : drop) {
+    (void)hipSetDevice(e.device);
+    (void)hipFree(e.p);
+  }
+  (void)hipSetDevice(cur);
+}
+}  // namespace
+
 void* dev_alloc(size_t bytes) {
-  void* p = nullptr;
   if (bytes == 0) bytes = 256;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (bytes >= kCacheMinBytes) {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto& v = cache();
+    size_t best = v.size();
+    for (size_t i = 0; i < v.size(); ++i) {
+      const CacheEntry& e = v[i];
+      if (e.device != dev || e.bytes < bytes || e.bytes > bytes + bytes / 2) continue;
+      if (best == v.size() || e.bytes < v[best].bytes) best = i;
+    }
+    if (best != v.size()) {
+      const CacheEntry e = v[best];
+      v.erase(v.begin() + static_cast<std::ptrdiff_t>(best));
+      live_sizes()[e.p] = LiveBuf{e.bytes, e.device};
+      return e.p;
+    }
+  }
+  void* p = nullptr;
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
+    cache_release_all();
+    e = hipMalloc(&p, bytes);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
     throw Error(VGX_OUT_OF_MEMORY, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+  }
+  if (bytes >= kCacheMinBytes) {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    live_sizes()[p] = LiveBuf{bytes, dev};
   }
   return p;
 }
 
 void dev_free(void* p) {
-  if (p) (void)hipFree(p);
+  if (!p) return;
+  LiveBuf lb{0, 0};
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto& m = live_sizes();
+    auto it = m.find(p);
+    if (it != m.end()) {
+      lb = it->second;
+      m.erase(it);
+    }
+  }
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  if (lb.bytes && lb.device != cur) (void)hipSetDevice(lb.device);
+  // small buffers, and any free that cannot first see the device idle, go
+  // back to the runtime
+  if (lb.bytes == 0 || lb.bytes > kCacheBytes || hipDeviceSynchronize() != hipSuccess) {
+    (void)hipFree(p);
+    if (lb.bytes && lb.device != cur) (void)hipSetDevice(cur);
+    return;
+  }
+  const int dev = lb.device;
+  const size_t bytes = lb.bytes;
+  std::vector<void*> drop;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto& v = cache();
+    v.push_back({dev, p, bytes});
+    for (;;) {
+      size_t count = 0, total = 0;
+      for (const CacheEntry& e : v)
+        if (e.device == dev) {
+          ++count;
+          total += e.bytes;
+        }
+      if (count <= kCacheBuffers && total <= kCacheBytes) break;
+      for (size_t i = 0; i < v.size(); ++i)
+        if (v[i].device == dev) {
+          drop.push_back(v[i].p);
+          v.erase(v.begin() + static_cast<std::ptrdiff_t>(i));
+          break;
+        }
+    }
+  }
+  for (void* q : drop) (void)hipFree(q);
+  if (dev != cur) (void)hipSetDevice(cur);
 }
 
 HostBuf::~HostBuf() {
