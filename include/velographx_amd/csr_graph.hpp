@@ -33,6 +33,11 @@ class CsrGraph {
       : g_(0, directed) {
     g_.bulk_load_edges(src, dst, m);
   }
+  // From one array of m (src, dst) pairs (pairs[2i], pairs[2i + 1]).
+  struct Pairs {};
+  CsrGraph(Pairs, const std::uint32_t* pairs, std::size_t m, bool directed = false) : g_(0, directed) {
+    g_.bulk_load_pairs(pairs, m);
+  }
 
   [[nodiscard]] std::size_t vertex_count() const { return g_.vertex_count(); }
   [[nodiscard]] std::size_t edge_entry_count() const { return g_.edge_count_directed(); }

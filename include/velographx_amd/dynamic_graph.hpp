@@ -12,6 +12,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <type_traits>
 #include <vector>
 
 #include "velographx_amd/types.hpp"
@@ -97,15 +98,17 @@ class DynamicGraph {
 
   // dynamic_graph.hpp:502-529
   void bulk_load_edges(const std::vector<std::pair<VertexId, VertexId>>& edges) {
-    std::vector<std::uint32_t> s(edges.size()), d(edges.size());
-    for (std::size_t i = 0; i < edges.size(); ++i) {
-      s[i] = edges[i].first;
-      d[i] = edges[i].second;
-    }
-    vgx_check(vgx_graph_bulk_load(g_, s.data(), d.data(), s.size()));
+    // a pair of two 32-bit ids is the (src, dst) layout the pair entry point reads
+    static_assert(sizeof(std::pair<VertexId, VertexId>) == 2 * sizeof(std::uint32_t) &&
+                  std::is_standard_layout_v<std::pair<VertexId, VertexId>>);
+    vgx_check(vgx_graph_bulk_load_pairs(g_, reinterpret_cast<const std::uint32_t*>(edges.data()), edges.size()));
   }
   void bulk_load_edges(const std::uint32_t* src, const std::uint32_t* dst, std::size_t m) {
     vgx_check(vgx_graph_bulk_load(g_, src, dst, m));
+  }
+  // The same from one array of m (src, dst) pairs (pairs[2i], pairs[2i + 1]).
+  void bulk_load_pairs(const std::uint32_t* pairs, std::size_t m) {
+    vgx_check(vgx_graph_bulk_load_pairs(g_, pairs, m));
   }
 
   // dynamic_graph.hpp:531-549

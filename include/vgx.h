@@ -67,6 +67,12 @@ vgx_status vgx_graph_destroy(vgx_graph* g);
  * graph with the canonical (sorted, de-duplicated, loop-free) CSR of the m
  * edges, mirrored when undirected; grows to max id + 1. */
 vgx_status vgx_graph_bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t* dst, uint64_t m);
+/* No counterpart in the reference: vgx_graph_bulk_load for callers that hold
+ * the m edges as one host array of (src, dst) pairs, pairs[2i] and
+ * pairs[2i + 1] (an (m, 2) NumPy edge array, a vector of uint32 pairs).  One
+ * upload; the pairs are split and checked on the device.  Same result and
+ * id rule as vgx_graph_bulk_load (grows to max id + 1, below 2^31). */
+vgx_status vgx_graph_bulk_load_pairs(vgx_graph* g, const uint32_t* pairs, uint64_t m);
 
 /* vgx_graph_bulk_load over an edge list already in device memory -- the
  * device form of from_numpy_edges / from_arrow_columns (python/bindings.cpp

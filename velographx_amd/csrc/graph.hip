@@ -1985,6 +1985,44 @@ vgx_status vgx_graph_bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t
   VGX_API_END
 }
 
+vgx_status vgx_graph_bulk_load_pairs(vgx_graph* g, const uint32_t* pairs, uint64_t m) {
+  VGX_API_BEGIN
+  if (!g || (m && !pairs)) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  if ((g->directed ? m : 2 * m) >= (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "edge list too large");
+  DeviceGuard dg(g->device);
+  hipStream_t s = g->stream;
+  DevBuf<uint32_t> buf;
+  buf.ensure(2 * m + 2);
+  if (m) VGX_HIP(hipMemcpyAsync(buf.p, pairs, m * 2 * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  const uint32_t* src = buf.p;
+  const uint32_t* dst = buf.p + 1;
+  const auto [ext_dev, bad, unused] = device_check(g, s, [&](unsigned long long* res) {
+    if (m) k_edges_check<uint32_t><<<grid_for(m, kBlock, 4096), kBlock, 0, s>>>(src, dst, m, 2, res);
+  });
+  (void)unused;
+  uint64_t ext = ext_dev;
+  if (bad) {
+    // ids at or past 2^31 - 1, which the device check does not size: the
+    // host form's rule (max id + 1 over non-loop edges, below 2^31) decides
+    uint64_t maxid = 0;
+    bool any = false;
+    for (uint64_t i = 0; i < m; ++i) {
+      const uint32_t u = pairs[2 * i], v = pairs[2 * i + 1];
+      if (u == v) continue;
+      any = true;
+      maxid = std::max<uint64_t>(maxid, std::max(u, v));
+    }
+    ext = any ? maxid + 1 : 0;
+  }
+  if (ext) graph_grow_vertices(g, ext);
+  const int undirected = g->directed ? 0 : 1;
+  bulk_load_keys(g, undirected ? 2 * m : m, [&](unsigned long long* keys, int d) {
+    k_edges_keys<uint32_t><<<grid_for(m), kBlock, 0, s>>>(src, dst, m, 2, undirected, d == 1 ? 1 : 0, keys);
+    VGX_LAUNCH_CHECK();
+  });
+  VGX_API_END
+}
+
 vgx_status vgx_graph_bulk_load_device(vgx_graph* g, const void* src_dev, const void* dst_dev, uint64_t m,
                                       uint64_t stride, uint32_t id_bytes, void* stream) {
   VGX_API_BEGIN

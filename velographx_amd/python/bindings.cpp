@@ -300,13 +300,8 @@ PYBIND11_MODULE(_velographx, m) {
              if (info.ndim != 2 || info.shape[1] != 2) throw std::invalid_argument("edge array must have shape (N, 2)");
              const auto m2 = static_cast<std::size_t>(info.shape[0]);
              const auto* data = static_cast<const std::uint32_t*>(info.ptr);
-             std::vector<std::uint32_t> s(m2), d(m2);
-             for (std::size_t i = 0; i < m2; ++i) {
-               s[i] = data[2 * i];
-               d[i] = data[2 * i + 1];
-             }
              py::gil_scoped_release nogil;
-             g.bulk_load_edges(s.data(), d.data(), m2);
+             g.bulk_load_pairs(data, m2);
            },
            py::arg("edges"))
       .def("bulk_load_arrays",
@@ -504,13 +499,8 @@ PYBIND11_MODULE(_velographx, m) {
              if (info.ndim != 2 || info.shape[1] != 2) throw std::invalid_argument("edge array must have shape (N, 2)");
              const auto m = static_cast<std::size_t>(info.shape[0]);
              const auto* data = static_cast<const std::uint32_t*>(info.ptr);
-             std::vector<std::uint32_t> s(m), d(m);
-             for (std::size_t i = 0; i < m; ++i) {
-               s[i] = data[2 * i];
-               d[i] = data[2 * i + 1];
-             }
              py::gil_scoped_release nogil;
-             return std::make_unique<velographx::CsrGraph>(s.data(), d.data(), m, directed);
+             return std::make_unique<velographx::CsrGraph>(velographx::CsrGraph::Pairs{}, data, m, directed);
            }),
            py::arg("edges"), py::arg("directed") = false)
       .def("vertex_count", &velographx::CsrGraph::vertex_count)
