@@ -65,7 +65,9 @@ vgx_status vgx_graph_destroy(vgx_graph* g);
 
 /* DynamicGraph::bulk_load_edges -- dynamic_graph.hpp:502-529: replaces the
  * graph with the canonical (sorted, de-duplicated, loop-free) CSR of the m
- * edges, mirrored when undirected; grows to max id + 1. */
+ * edges, mirrored when undirected; grows to max id + 1.  The edges are
+ * uploaded and checked on the device; a list of 2^31 arcs or more (m, twice m
+ * when undirected) is refused with VGX_INVALID_ARGUMENT. */
 vgx_status vgx_graph_bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t* dst, uint64_t m);
 /* No counterpart in the reference: vgx_graph_bulk_load for callers that hold
  * the m edges as one host array of (src, dst) pairs, pairs[2i] and

@@ -217,3 +217,36 @@ def test_csr_errors_and_empty(vx):
     assert g.version == 1 and g.neighbors(1) == [0] and g.neighbors(2) == [1]
     g = vx.from_scipy_csr(DevCsr([0, 0, 0], [], 2))
     assert g.vertex_count == 2 and g.version == 0
+
+
+@pytest.mark.parametrize("form", ["arrays", "pairs"])
+def test_host_bulk_load_id_rule(vx, form):
+    """Host edge lists are uploaded and checked on the device; ids at or past
+    2^31 - 1, which that check does not size, take the host rule of
+    dynamic_graph.hpp:502-529: self-loops are dropped whatever their ids,
+    the graph grows to max id + 1 over the other edges, and an id that would
+    take the vertex count past 2^31 is refused with the graph unchanged."""
+    big = (1 << 32) - 1
+
+    def load(g, e):
+        e = np.asarray(e, np.uint64).astype(np.uint32)
+        if form == "arrays":
+            g.bulk_load_arrays(np.ascontiguousarray(e[:, 0]), np.ascontiguousarray(e[:, 1]))
+        else:
+            g.bulk_load_edges(e)
+
+    g = vx.Graph(5, False)
+    load(g, [[0, 1], [1, 2]])
+    assert (g.vertex_count, g.version, g.edge_count_directed) == (5, 1, 4)
+    # loops only, at ids the device check refuses: nothing kept, no growth
+    load(g, [[big, big], [1 << 31, 1 << 31], [3, 3]])
+    assert (g.vertex_count, g.version, g.edge_count_directed) == (5, 2, 0)
+    # a loop at 2^31 - 1 beside ordinary edges: the edges decide the extent
+    load(g, [[0, 6], [(1 << 31) - 1, (1 << 31) - 1], [6, 2]])
+    assert (g.vertex_count, g.version, g.edge_count_directed) == (7, 3, 4)
+    assert g.neighbors(6) == [0, 2] and g.neighbors(0) == [6]
+    # a non-loop id of 2^31 is refused; the graph is left as it was
+    with pytest.raises(ValueError, match="vertex ids"):
+        load(g, [[0, 1], [2, 1 << 31]])
+    assert (g.vertex_count, g.version, g.edge_count_directed) == (7, 3, 4)
+    assert g.neighbors(6) == [0, 2]

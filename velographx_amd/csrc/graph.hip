@@ -1082,40 +1082,6 @@ static void build_direction(vgx_graph* g, int d, DevBuf<unsigned long long>& key
   VGX_HIP(hipStreamSynchronize(s));
 }
 
-static void bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t* dst, uint64_t m) {
-  uint64_t maxid = 0;
-  bool any = false;
-  for (uint64_t i = 0; i < m; ++i) {
-    if (src[i] == dst[i]) continue;
-    any = true;
-    maxid = std::max<uint64_t>(maxid, std::max(src[i], dst[i]));
-  }
-  if (any) graph_grow_vertices(g, maxid + 1);
-  hipStream_t s = g->stream;
-  DevBuf<uint32_t> ds, dd;
-  ds.ensure(m + 1);
-  dd.ensure(m + 1);
-  if (m) {
-    VGX_HIP(hipMemcpyAsync(ds.p, src, m * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    VGX_HIP(hipMemcpyAsync(dd.p, dst, m * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-  }
-  for (int d = 0; d < g->ndir(); ++d) {
-    const uint64_t M = g->directed ? m : 2 * m;
-    DevBuf<unsigned long long> keys, skeys;
-    keys.ensure(M + 1);
-    skeys.ensure(M + 1);
-    if (m) {
-      k_expand_keys<<<grid_for(m), kBlock, 0, s>>>(ds.p, dd.p, m, g->directed ? 0 : 1, d == 1 ? 1 : 0, keys.p,
-                                                    nullptr);
-      VGX_LAUNCH_CHECK();
-    }
-    build_direction(g, d, keys, skeys, M);
-    rebase_rows(g, d);
-  }
-  g->version += 1;
-  g->rows_epoch += 1;
-}
-
 // ------------------------------------------------- edge lists in device memory
 //
 // The same builds from edge lists that already sit in HBM (a torch tensor, a
@@ -1307,6 +1273,38 @@ static void bulk_load_device_t(vgx_graph* g, const T* src, const T* dst, uint64_
   });
 }
 
+// A host edge list (src[i * stride], dst[i * stride]) already uploaded to
+// dsrc/ddst with the same stride, checked on the device.  When the check
+// finds ids at or past 2^31 - 1, which it does not size, the host form's rule
+// (grow to max id + 1 over the non-loop edges; an id of 2^31 or more is
+// refused by the growth) is applied by a scan of the host list.
+static void bulk_load_host(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint32_t* dsrc,
+                           const uint32_t* ddst, uint64_t m, uint64_t stride) {
+  hipStream_t s = g->stream;
+  const auto [ext_dev, bad, unused] = device_check(g, s, [&](unsigned long long* res) {
+    if (m) k_edges_check<uint32_t><<<grid_for(m, kBlock, 4096), kBlock, 0, s>>>(dsrc, ddst, m, stride, res);
+  });
+  (void)unused;
+  uint64_t ext = ext_dev;
+  if (bad) {
+    uint64_t maxid = 0;
+    bool any = false;
+    for (uint64_t i = 0; i < m; ++i) {
+      const uint32_t u = src[i * stride], v = dst[i * stride];
+      if (u == v) continue;
+      any = true;
+      maxid = std::max<uint64_t>(maxid, std::max(u, v));
+    }
+    ext = any ? maxid + 1 : 0;
+  }
+  if (ext) graph_grow_vertices(g, ext);
+  const int undirected = g->directed ? 0 : 1;
+  bulk_load_keys(g, undirected ? 2 * m : m, [&](unsigned long long* keys, int d) {
+    k_edges_keys<uint32_t><<<grid_for(m), kBlock, 0, s>>>(dsrc, ddst, m, stride, undirected, d == 1 ? 1 : 0, keys);
+    VGX_LAUNCH_CHECK();
+  });
+}
+
 template <class T>
 static void bulk_load_csr_device_t(vgx_graph* g, const T* indptr, const T* indices, uint64_t n, uint64_t nnz,
                                    hipStream_t producer) {
@@ -1335,22 +1333,28 @@ static void stage_sorted(vgx_graph* g, uint64_t k, bool defer);
 
 void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const uint8_t* add, uint64_t k, bool defer) {
   if (k >= (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "update batch too large");
-  uint64_t maxid = 0;
-  bool any = false;
-  for (uint64_t i = 0; i < k; ++i) {
-    if (src[i] == dst[i]) continue;
-    any = true;
-    uint32_t mx = std::max(src[i], dst[i]);
-    if (mx > maxid) maxid = mx;
-  }
-  if (any && maxid >= (1ull << 31) - 1) throw Error(VGX_INVALID_ARGUMENT, "vertex ids must be below 2^31 - 1");
-  if (any) graph_grow_vertices(g, maxid + 1);
   hipStream_t s = g->stream;
-  // one pinned staging copy: src | dst | add
+  // one pinned staging copy (src | dst | add), the ids copied in the same
+  // branch-free pass that finds the largest endpoint of the non-loop ops (a
+  // non-loop op's larger end is at least 1, so 0 means only loops)
   g->stage.ensure(k * 9 + 64);
   auto* hs = static_cast<uint32_t*>(g->stage.p);
-  std::memcpy(hs, src, k * 4);
-  std::memcpy(hs + k, dst, k * 4);
+  {
+    uint32_t* __restrict__ hs_src = hs;
+    uint32_t* __restrict__ hs_dst = hs + k;
+    const uint32_t* __restrict__ ps = src;
+    const uint32_t* __restrict__ pd = dst;
+    uint32_t mx = 0;
+    for (uint64_t i = 0; i < k; ++i) {
+      const uint32_t u = ps[i], v = pd[i];
+      hs_src[i] = u;
+      hs_dst[i] = v;
+      const uint32_t m = u == v ? 0u : (u > v ? u : v);
+      mx = m > mx ? m : mx;
+    }
+    if (mx >= (1u << 31) - 1) throw Error(VGX_INVALID_ARGUMENT, "vertex ids must be below 2^31 - 1");
+    if (mx) graph_grow_vertices(g, static_cast<uint64_t>(mx) + 1);
+  }
   std::memcpy(reinterpret_cast<uint8_t*>(hs + 2 * k), add, k);
   BatchScratch& b0 = g->bs[0];
   b0.src.ensure(2 * k + 64);
@@ -1980,8 +1984,16 @@ vgx_status vgx_graph_destroy(vgx_graph* g) {
 vgx_status vgx_graph_bulk_load(vgx_graph* g, const uint32_t* src, const uint32_t* dst, uint64_t m) {
   VGX_API_BEGIN
   if (!g || (m && (!src || !dst))) throw Error(VGX_INVALID_ARGUMENT, "null argument");
+  if ((g->directed ? m : 2 * m) >= (1ull << 31)) throw Error(VGX_INVALID_ARGUMENT, "edge list too large");
   DeviceGuard dg(g->device);
-  bulk_load(g, src, dst, m);
+  hipStream_t s = g->stream;
+  DevBuf<uint32_t> buf;
+  buf.ensure(2 * m + 2);
+  if (m) {
+    VGX_HIP(hipMemcpyAsync(buf.p, src, m * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    VGX_HIP(hipMemcpyAsync(buf.p + m, dst, m * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  }
+  bulk_load_host(g, src, dst, buf.p, buf.p + m, m, 1);
   VGX_API_END
 }
 
@@ -1994,32 +2006,7 @@ vgx_status vgx_graph_bulk_load_pairs(vgx_graph* g, const uint32_t* pairs, uint64
   DevBuf<uint32_t> buf;
   buf.ensure(2 * m + 2);
   if (m) VGX_HIP(hipMemcpyAsync(buf.p, pairs, m * 2 * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-  const uint32_t* src = buf.p;
-  const uint32_t* dst = buf.p + 1;
-  const auto [ext_dev, bad, unused] = device_check(g, s, [&](unsigned long long* res) {
-    if (m) k_edges_check<uint32_t><<<grid_for(m, kBlock, 4096), kBlock, 0, s>>>(src, dst, m, 2, res);
-  });
-  (void)unused;
-  uint64_t ext = ext_dev;
-  if (bad) {
-    // ids at or past 2^31 - 1, which the device check does not size: the
-    // host form's rule (max id + 1 over non-loop edges, below 2^31) decides
-    uint64_t maxid = 0;
-    bool any = false;
-    for (uint64_t i = 0; i < m; ++i) {
-      const uint32_t u = pairs[2 * i], v = pairs[2 * i + 1];
-      if (u == v) continue;
-      any = true;
-      maxid = std::max<uint64_t>(maxid, std::max(u, v));
-    }
-    ext = any ? maxid + 1 : 0;
-  }
-  if (ext) graph_grow_vertices(g, ext);
-  const int undirected = g->directed ? 0 : 1;
-  bulk_load_keys(g, undirected ? 2 * m : m, [&](unsigned long long* keys, int d) {
-    k_edges_keys<uint32_t><<<grid_for(m), kBlock, 0, s>>>(src, dst, m, 2, undirected, d == 1 ? 1 : 0, keys);
-    VGX_LAUNCH_CHECK();
-  });
+  bulk_load_host(g, pairs, pairs + 1, buf.p, buf.p + 1, m, 2);
   VGX_API_END
 }
 
