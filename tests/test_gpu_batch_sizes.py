@@ -78,3 +78,30 @@ def test_sort_tier_boundaries(vx, orc, directed):
         m.apply(s, d, a)
         check_graph(g, m)
         assert g.version == m.version
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_host_batch_over_several_staging_pieces(vx, orc, directed):
+    """A host batch is staged and uploaded in pieces of 2^18 ops: a batch of
+    two pieces and a part, with one arc's alternating ops spread over all of
+    them (the last op wins) and the only ids past the vertex count in its
+    last ops (the growth is decided over every piece), equals the oracle's
+    model graph."""
+    rng = np.random.default_rng(31 + int(directed))
+    n = 20000
+    g = vx.Graph(n, directed)
+    m = orc.ModelGraph(n, directed)
+    e = rng.integers(0, n, size=(100000, 2)).astype(np.uint32)
+    g.bulk_load_edges(e)
+    m.bulk_load_edges(e.tolist())
+    k = 2 * (1 << 18) + 1000
+    s, d, a = make_ops(rng, m, n, k - 16)
+    gs = rng.integers(0, n + 5000, 16).astype(np.uint32)
+    gd = (n + 4000 + np.arange(16)).astype(np.uint32)
+    s, d = np.concatenate([s, gs]), np.concatenate([d, gd])
+    a = np.concatenate([a, np.ones(16, np.uint8)])
+    g.apply_arrays(s, d, a)
+    m.apply(s, d, a)
+    assert g.vertex_count == n + 4016
+    check_graph(g, m)
+    assert g.version == m.version

@@ -1336,31 +1336,39 @@ void graph_stage(vgx_graph* g, const uint32_t* src, const uint32_t* dst, const u
   hipStream_t s = g->stream;
   // one pinned staging copy (src | dst | add), the ids copied in the same
   // branch-free pass that finds the largest endpoint of the non-loop ops (a
-  // non-loop op's larger end is at least 1, so 0 means only loops)
+  // non-loop op's larger end is at least 1, so 0 means only loops); uploaded
+  // in pieces, so that each piece's upload overlaps the pass over the next
   g->stage.ensure(k * 9 + 64);
+  BatchScratch& b0 = g->bs[0];
+  b0.src.ensure(2 * k + 64);
+  b0.add.ensure(k + 64);
   auto* hs = static_cast<uint32_t*>(g->stage.p);
-  {
+  auto* hadd = reinterpret_cast<uint8_t*>(hs + 2 * k);
+  constexpr uint64_t kPiece = 1ull << 18;
+  uint32_t mx = 0;
+  for (uint64_t a = 0; a < k; a += kPiece) {
+    const uint64_t e = std::min(k, a + kPiece);
     uint32_t* __restrict__ hs_src = hs;
     uint32_t* __restrict__ hs_dst = hs + k;
     const uint32_t* __restrict__ ps = src;
     const uint32_t* __restrict__ pd = dst;
-    uint32_t mx = 0;
-    for (uint64_t i = 0; i < k; ++i) {
+    uint32_t m_piece = 0;
+    for (uint64_t i = a; i < e; ++i) {
       const uint32_t u = ps[i], v = pd[i];
       hs_src[i] = u;
       hs_dst[i] = v;
       const uint32_t m = u == v ? 0u : (u > v ? u : v);
-      mx = m > mx ? m : mx;
+      m_piece = m > m_piece ? m : m_piece;
     }
-    if (mx >= (1u << 31) - 1) throw Error(VGX_INVALID_ARGUMENT, "vertex ids must be below 2^31 - 1");
-    if (mx) graph_grow_vertices(g, static_cast<uint64_t>(mx) + 1);
+    mx = std::max(mx, m_piece);
+    std::memcpy(hadd + a, add + a, e - a);
+    VGX_HIP(hipMemcpyAsync(b0.src.p + a, hs + a, 4 * (e - a), hipMemcpyHostToDevice, s));
+    VGX_HIP(hipMemcpyAsync(b0.src.p + k + a, hs + k + a, 4 * (e - a), hipMemcpyHostToDevice, s));
+    VGX_HIP(hipMemcpyAsync(b0.add.p + a, hadd + a, e - a, hipMemcpyHostToDevice, s));
   }
-  std::memcpy(reinterpret_cast<uint8_t*>(hs + 2 * k), add, k);
-  BatchScratch& b0 = g->bs[0];
-  b0.src.ensure(2 * k + 64);
-  b0.add.ensure(k + 64);
-  VGX_HIP(hipMemcpyAsync(b0.src.p, hs, 8 * k, hipMemcpyHostToDevice, s));
-  VGX_HIP(hipMemcpyAsync(b0.add.p, hs + 2 * k, k, hipMemcpyHostToDevice, s));
+  // an id check that fails leaves only the staging scratch written
+  if (mx >= (1u << 31) - 1) throw Error(VGX_INVALID_ARGUMENT, "vertex ids must be below 2^31 - 1");
+  if (mx) graph_grow_vertices(g, static_cast<uint64_t>(mx) + 1);
   stage_sorted(g, k, defer);
 }
 
