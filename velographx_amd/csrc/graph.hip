@@ -562,56 +562,69 @@ __global__ void k_insert_small(const uint32_t* __restrict__ ins, const unsigned 
   }
 }
 
-// One workgroup per large row: merge-path merge of the old row and its
-// sorted inserts into a fresh arena range.
-// Rows of kBigRow < len + k <= cap, len <= kInPlaceMax, k <= kMidMaxIns:
-// one workgroup reads the whole row and its inserts into LDS, places every
-// old entry at i + (inserts below it) and every insert at t + (old entries
-// below it) by binary searches in LDS, and writes the merged row back over
-// the old one (all sources are in LDS, so the writes cannot overtake reads).
+// One wave per mid row, four rows per workgroup: rows of kBigRow < len + k
+// <= cap, len <= kInPlaceMax, k <= kMidMaxIns, merged in place.  An insert's
+// id is in neither the live nor the deleted entries of its row (a deleted
+// entry of the same id is flipped back instead), so the final positions of
+// the old entries, i + (inserts below it), and of the inserts, t + (old
+// entries below it), are disjoint.  The wave finds each insert's position by
+// a binary search of the old row and keeps it in LDS with the insert; then it
+// shifts the entries from the first insert's position on, 64 at a time from
+// the row's tail (an entry only moves right, so a chunk's stores land on
+// entries already read), and stores the inserts last.
 constexpr uint32_t kMidMaxIns = kMidInsMax;
-__global__ __launch_bounds__(256) void k_insert_mid(const uint32_t* __restrict__ mid,
-                                                    const uint32_t* __restrict__ ins,
-                                                    const uint32_t* __restrict__ segk,
-                                                    const unsigned long long* __restrict__ skeys,
-                                                    uint2* __restrict__ row, uint32_t* __restrict__ arena,
-                                                    uint32_t* __restrict__ deg) {
-  __shared__ uint32_t s_row[kInPlaceMax];
-  __shared__ uint32_t s_ins[kMidMaxIns];
-  const uint32_t j = mid[blockIdx.x];
-  const uint32_t k = segk[j];
+constexpr uint32_t kMidWaves = 4;
+__global__ __launch_bounds__(kMidWaves * kWave) void k_insert_mid(const uint32_t* __restrict__ mid, uint32_t nmid,
+                                                                  const uint32_t* __restrict__ ins,
+                                                                  const uint32_t* __restrict__ segk,
+                                                                  const unsigned long long* __restrict__ skeys,
+                                                                  uint2* __restrict__ row, uint32_t* arena,
+                                                                  uint32_t* __restrict__ deg) {
+  __shared__ uint32_t s_ins[kMidWaves][kMidMaxIns];
+  __shared__ uint32_t s_pos[kMidWaves][kMidMaxIns];
+  const uint32_t w = threadIdx.x / kWave;
+  const uint32_t r = blockIdx.x * kMidWaves + w;
+  if (r >= nmid) return;  // no workgroup barrier below
+  const uint32_t lane = lane_id();
+  const uint32_t j = mid[r];
+  const uint32_t k = segk[j];  // at least 1
   const uint32_t u = static_cast<uint32_t>(skeys[ins[j]] >> 32);
   const uint2 rb = row[u];
   const uint32_t len = rb.y;
   uint32_t* a = arena + rb.x;
-  for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) s_ins[t] = static_cast<uint32_t>(skeys[ins[j + t]]);
-  for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) s_row[i] = a[i];
-  __syncthreads();
-  // old entry i moves to i + (inserts below it); ids keep the row sorted,
-  // deleted entries included
-  for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
-    const uint32_t v = s_row[i];
-    const uint32_t x = v & kIdMask;
-    uint32_t lo = 0, hi = k;
-    while (lo < hi) {
-      const uint32_t m = (lo + hi) >> 1;
-      if (s_ins[m] < x) lo = m + 1;
-      else hi = m;
-    }
-    a[i + lo] = v;
-  }
-  // insert t lands at t + (old entries below it)
-  for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) {
-    const uint32_t x = s_ins[t];
+  uint32_t* si = s_ins[w];
+  uint32_t* sp = s_pos[w];
+  for (uint32_t t = lane; t < k; t += kWave) {
+    const uint32_t x = static_cast<uint32_t>(skeys[ins[j + t]]);
     uint32_t lo = 0, hi = len;
     while (lo < hi) {
       const uint32_t m = (lo + hi) >> 1;
-      if ((s_row[m] & kIdMask) < x) lo = m + 1;
+      if ((a[m] & kIdMask) < x) lo = m + 1;
       else hi = m;
     }
-    a[t + lo] = x;
+    si[t] = x;
+    sp[t] = t + lo;
   }
-  if (threadIdx.x == 0) {  // the live and stored counts were added by k_plan_inserts
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t p0 = sp[0];  // entries below the first insert stay
+  for (uint32_t end = len; end > p0;) {
+    const uint32_t beg = end - p0 > kWave ? end - kWave : p0;
+    const uint32_t i = beg + lane;
+    if (i < end) {
+      const uint32_t v = a[i];
+      const uint32_t x = v & kIdMask;
+      uint32_t lo = 0, hi = k;
+      while (lo < hi) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (si[m] < x) lo = m + 1;
+        else hi = m;
+      }
+      a[i + lo] = v;
+    }
+    end = beg;
+  }
+  for (uint32_t t = lane; t < k; t += kWave) a[sp[t]] = si[t];
+  if (lane == 0) {  // the live and stored counts were added by k_plan_inserts
     row[u].y = len + k;
     atomicAdd(&deg[u], k);
   }
@@ -1541,8 +1554,9 @@ void graph_apply_additions(vgx_graph* g) {
       VGX_LAUNCH_CHECK();
     }
     if (b.nmid) {
-      k_insert_mid<<<static_cast<unsigned>(b.nmid), 256, 0, g->stream>>>(b.mid.p, b.ins.p, b.segk.p, b.skeys.p,
-                                                                        a.row.p, a.arena.p, a.deg.p);
+      k_insert_mid<<<static_cast<unsigned>((b.nmid + kMidWaves - 1) / kMidWaves), kMidWaves * kWave, 0,
+                     g->stream>>>(b.mid.p, static_cast<uint32_t>(b.nmid), b.ins.p, b.segk.p, b.skeys.p, a.row.p,
+                                  a.arena.p, a.deg.p);
       VGX_LAUNCH_CHECK();
     }
     if (b.nbig) {
